@@ -1,0 +1,72 @@
+"""Python handle on libbem3qd2.so, the MI355X (gfx950) implementation of the
+bem3_emf_qd2 boundary-integral path.
+
+The product is the C-ABI library (include/bem3_emf_qd2.h, include/bq_amd.h).  This module
+only loads it with ctypes and wraps the batch entry points with numpy conversions; it
+computes nothing itself and has no CPU fallback: every call fails loudly (RuntimeError)
+when the library is not built or no gfx950 device is usable.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "lib" / "libbem3qd2.so"
+
+_D = ctypes.POINTER(ctypes.c_double)
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """Load the built library (once).  Raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+        _lib = ctypes.CDLL(str(LIB_PATH))
+        _lib.bq_last_error.restype = ctypes.c_char_p
+        _declare(_lib)
+    return _lib
+
+
+def _declare(L: ctypes.CDLL) -> None:
+    L.bq_device_ok.restype = ctypes.c_int
+    L.bq_faddeeva_w_batch.argtypes = [ctypes.c_int, _D, _D]
+    L.bq_qpgf_v1_batch.argtypes = [_D, ctypes.c_int, _D, _D]
+
+
+def _check(rc: int, what: str) -> None:
+    if rc != 0:
+        raise RuntimeError(f"{what} failed ({rc}): {lib().bq_last_error().decode()}")
+
+
+def device_ok() -> bool:
+    return lib().bq_device_ok() == 0
+
+
+def _as(a, dtype=np.float64):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+def faddeeva_w(z) -> np.ndarray:
+    """w(z) on the GPU for an array of complex z."""
+    z = np.asarray(z, complex).ravel()
+    zz = _as(np.stack([z.real, z.imag], 1))
+    w = np.empty_like(zz)
+    _check(lib().bq_faddeeva_w_batch(len(z), zz.ctypes.data_as(_D), w.ctypes.data_as(_D)), "faddeeva_w")
+    return w[:, 0] + 1j * w[:, 1]
+
+
+def qpgf_v1(qd8, r) -> np.ndarray:
+    """Quasi-periodic Green's function and gradient, shape (n, 4) complex."""
+    q = _as(qd8)
+    assert q.shape == (8,)
+    rr = _as(np.asarray(r).reshape(-1, 3))
+    out = np.empty((len(rr), 8))
+    _check(lib().bq_qpgf_v1_batch(q.ctypes.data_as(_D), len(rr), rr.ctypes.data_as(_D), out.ctypes.data_as(_D)),
+           "qpgf_v1")
+    return out[:, 0::2] + 1j * out[:, 1::2]

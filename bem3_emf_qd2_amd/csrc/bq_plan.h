@@ -1,0 +1,40 @@
+/*
+ * bq_plan.h -- lattice/Bloch constants for the quasi-periodic Green's function, computed
+ * once per (lattice, incidence) on the host and passed to kernels by value.
+ *
+ * Shared by the host C code and the HIP kernels, so it holds plain doubles and ints only.
+ */
+#ifndef BQ_PLAN_H
+#define BQ_PLAN_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Exponent cut for truncated lattice sums: terms below exp(-BQ_EXP_CUT) (4e-18) of the
+ * leading term are dropped. */
+#define BQ_EXP_CUT 40.0
+#define BQ_EW_EXPM 3.0 /* Ewald amplification limit, reference qgf2_src/d3_qpgf_d2.h:26 */
+#define BQ_S_LIMIT 15  /* reference qgf2_src/d3_qpgf_d2.h:19 */
+
+typedef struct bq_qplan {
+  double k, k2;            /* real wave number of the open region */
+  double vk0, vk1;         /* in-plane Bloch wave vector */
+  double d1x, d1y, d2x, d2y;
+  double idd, aidd;        /* 1/det(d1,d2), |1/det| */
+  double g1x, g1y, g2x, g2y; /* reciprocal vectors, g_i . d_j = 2 pi delta_ij */
+  double b1x, b1y, b2x, b2y; /* dual vectors, b_i . d_j = delta_ij */
+  double veps0;            /* lattice default Ewald parameter */
+  double fd_bN, fd_bNp, fd_bNm; /* Fourier-domain selection data (bNp/bNm = 0 -> Ewald only) */
+  int fd_allowed;
+  double eps;              /* relative accuracy used in the Fourier/Ewald choice */
+} bq_qplan;
+
+/* Fill a plan from the reference's QPD2 fields (k, vk[3], vd1[2], vd2[2]).
+ * Returns 0, or -1 if the lattice is degenerate. */
+int bq_qplan_init(bq_qplan *p, double k, const double *vk, const double *vd1, const double *vd2, double eps);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,110 @@
+// bq_faddeeva.hpp -- Faddeeva function w(z) = exp(-z^2) erfc(-iz) in FP64 for device code.
+//
+// Methods (published):
+//   * M. R. Zaghloul, A. N. Ali, "Algorithm 916", ACM TOMS 38(2):15, 2011 -- w as erfcx(y),
+//     cos/sin(2xy) terms plus five series in n with weights exp(-a^2 n^2)/(a^2 n^2 + y^2),
+//     a = pi / sqrt(-ln(relerr/2)).  Here relerr = 1e-16, and exp(-a^2 n^2) is advanced by
+//     the exact recurrence t_{n+1} = t_n * exp(-a^2 (2n+1)), so no table and no per-term exp.
+//   * the Laplace continued fraction for large |z| (Gautschi 1970), with the number of
+//     levels of Poppe & Wijers (1990).
+// The region split is the one used by the reference's Faddeeva package
+// (qgf2_src/Faddeeva.cc:712-717): the series lose accuracy in Re w for x ~ 6, small y.
+// The number of series terms is fixed per call from x (the series decays like
+// exp(-(a n - x)^2)), so lanes of a wavefront run the same trip count when their x agree.
+#pragma once
+#include "bq_complex.hpp"
+
+namespace bq {
+
+constexpr double kISQPI = 0.56418958354775628694807945156;   // 1/sqrt(pi)
+constexpr double kFA = 0.5127840463909251172929805;          // a for relerr = 1e-16
+constexpr double kFA2 = 0.2629474782330504428098607;         // a^2
+constexpr double kFC = 0.3264484628871180195202143;          // 2a/pi
+
+__device__ __forceinline__ double rcp_nr(double d)
+{
+  // hardware reciprocal estimate refined by two Newton steps (full double precision)
+  double r = __builtin_amdgcn_rcp(d);
+  double e = fma(-d, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-d, r, 1.0);
+  return fma(r, e, r);
+}
+
+__device__ __forceinline__ double sinc_of(double x, double sx)
+{
+  return (fabs(x) < 1e-4) ? 1.0 - x * x * (1.0 / 6.0) : sx / x;
+}
+
+// continued fraction for Im z >= 0
+__device__ __forceinline__ cplx w_cf_upper(double x, double y)
+{
+  double rho2 = (x * (1.0 / 6.3)) * (x * (1.0 / 6.3)) + (y * (1.0 / 4.4)) * (y * (1.0 / 4.4));
+  int nu = (int)(3.0 + 1442.0 / (26.0 * sqrt(rho2) + 77.0)) + 2;
+  cplx z = mk(x, y), t = z;
+  for (int k = nu; k >= 1; --k) t = z - cdiv(mk(0.5 * k, 0.0), t);
+  cplx r = cdiv(mk(0.0, kISQPI), t);
+  return r;
+}
+
+__device__ cplx faddeeva_w(double xs, double y)
+{
+  const double x = fabs(xs), ya = fabs(y);
+  if (xs == 0.0) return mk(erfcx(y), 0.0);
+  if (ya > 7.0 || (x > 6.0 && (ya > 0.1 || (x > 8.0 && ya > 1e-10) || x > 28.0))) {
+    double xr = (y < 0.0) ? -xs : xs;
+    cplx w = w_cf_upper(xr, ya);
+    if (y < 0.0) {
+      cplx e = cexp(mk((ya - xr) * (xr + ya), 2.0 * xr * y));
+      return 2.0 * e - w;
+    }
+    return w;
+  }
+  double s1 = 0.0, s2 = 0.0, s3 = 0.0, s4 = 0.0, s5 = 0.0;
+  if (x < 10.0) {
+    const double ex2 = exp(-x * x);
+    const double e2ax = exp(2.0 * kFA * x), em2ax = 1.0 / e2ax;
+    const int nterm = min(60, (int)((x + 6.3) * (1.0 / kFA)) + 2);
+    double t = exp(-kFA2), q = exp(-3.0 * kFA2);       // exp(-a^2 n^2), exp(-a^2 (2n+1))
+    const double q2 = exp(-2.0 * kFA2);
+    double p2 = 1.0, pm2 = 1.0, y2 = y * y;
+    for (int n = 1; n <= nterm; ++n) {
+      double an = kFA * n;
+      double coef = t * ex2 * rcp_nr(fma(an, an, y2));
+      p2 *= e2ax;
+      pm2 *= em2ax;
+      double cm = coef * pm2, cp = coef * p2;
+      s1 += coef;
+      s2 += cm;
+      s4 = fma(cm, an, s4);
+      s3 += cp;
+      s5 = fma(cp, an, s5);
+      t *= q;
+      q *= q2;
+    }
+    const double ex2erfcx = (y > -6.0) ? ex2 * erfcx(y) : 2.0 * exp(y2 - x * x);
+    double sxy, cxy, s2xy, c2xy;
+    sincos(xs * y, &sxy, &cxy);
+    s2xy = 2.0 * sxy * cxy;
+    c2xy = fma(-2.0 * sxy, sxy, 1.0);
+    const double coef1 = ex2erfcx - kFC * y * s1;
+    const double coef2 = kFC * xs * ex2;
+    double re = coef1 * c2xy + coef2 * sxy * sinc_of(xs * y, sxy) + 0.5 * kFC * y * (s2 + s3);
+    double im = coef2 * sinc_of(2.0 * xs * y, s2xy) - coef1 * s2xy + copysign(0.5 * kFC * (s5 - s4), xs);
+    return mk(re, im);
+  }
+  // x >= 10, |y| tiny: series centred on n0 = x/a
+  double n0 = floor(x * (1.0 / kFA) + 0.5), dx = kFA * n0 - x;
+  s3 = exp(-dx * dx) / fma(kFA2 * n0, n0, y * y);
+  s5 = kFA * n0 * s3;
+  for (int dn = 1; dn < 40; ++dn) {
+    double np = n0 + dn, nm = n0 - dn;
+    double tp = exp(-(kFA * dn + dx) * (kFA * dn + dx)) / fma(kFA2 * np, np, y * y);
+    double tm = (nm > 0.0) ? exp(-(kFA * dn - dx) * (kFA * dn - dx)) / fma(kFA2 * nm, nm, y * y) : 0.0;
+    s3 += tp + tm;
+    s5 += kFA * (np * tp + nm * tm);
+  }
+  return mk(exp(-x * x) + 0.5 * kFC * y * s3, 0.5 * kFC * copysign(s5, xs));
+}
+
+}  // namespace bq

@@ -1,0 +1,148 @@
+// bq_runtime.hip -- library runtime (errors, device, stream) and the batch entry points for
+// the special functions: Faddeeva w(z) and the quasi-periodic Green's function.
+#include <cstdarg>
+#include <cstring>
+#include <mutex>
+#include "bq_runtime.hpp"
+#include "bq_qpgf.hpp"
+#include "../../../include/bq_amd.h"
+
+namespace bq {
+
+static thread_local char g_err[1024];
+static std::mutex g_mu;
+static hipStream_t g_stream[64];
+static int g_ready[64];
+
+void set_error(const char* fmt, ...)
+{
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof g_err, fmt, ap);
+  va_end(ap);
+  if (getenv("BQ_VERBOSE")) fprintf(stderr, "bem3qd2: %s\n", g_err);
+}
+
+int ensure_device()
+{
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) {
+    set_error("no HIP device available (%s)", hipGetErrorString(e));
+    return -1;
+  }
+  if (dev < 0 || dev >= 64) {
+    set_error("device index %d out of range", dev);
+    return -1;
+  }
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!g_ready[dev]) {
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, dev) != hipSuccess) {
+      set_error("hipGetDeviceProperties failed");
+      return -1;
+    }
+    if (strncmp(p.gcnArchName, "gfx950", 6) != 0) {
+      set_error("device %d is %s; this library is built for gfx950 only", dev, p.gcnArchName);
+      return -1;
+    }
+    if (hipStreamCreateWithFlags(&g_stream[dev], hipStreamNonBlocking) != hipSuccess) {
+      set_error("stream creation failed");
+      return -1;
+    }
+    g_ready[dev] = 1;
+  }
+  return 0;
+}
+
+hipStream_t stream()
+{
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  return g_stream[dev];
+}
+
+}  // namespace bq
+
+// ------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_faddeeva(int n, const double* __restrict__ z, double* __restrict__ w)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  cplx v = bq::faddeeva_w(z[2 * i], z[2 * i + 1]);
+  w[2 * i] = v.re;
+  w[2 * i + 1] = v.im;
+}
+
+__global__ void __launch_bounds__(256) k_qpgf_v1(bq_qplan P, int n, const double* __restrict__ r, double* __restrict__ out)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  bq::QG q;
+  bq::qpgf_v1(P, r[3 * i], r[3 * i + 1], r[3 * i + 2], q);
+  double* o = out + 8 * i;
+  o[0] = q.g.re; o[1] = q.g.im;
+  for (int c = 0; c < 3; ++c) { o[2 + 2 * c] = q.d[c].re; o[3 + 2 * c] = q.d[c].im; }
+}
+
+extern "C" {
+
+const char* bq_last_error(void) { return bq::g_err; }
+
+int bq_device_ok(void) { return bq::ensure_device(); }
+
+int bq_faddeeva_w_batch(int n, const double* z, double* w)
+{
+  if (n < 0 || (n > 0 && (!z || !w))) { bq::set_error("bq_faddeeva_w_batch: bad arguments"); return -2; }
+  if (n == 0) return 0;
+  if (bq::ensure_device()) return -1;
+  hipStream_t s = bq::stream();
+  double *dz = nullptr, *dw = nullptr;
+  size_t bytes = sizeof(double) * 2 * (size_t)n;
+  BQ_HIP(hipMallocAsync((void**)&dz, bytes, s));
+  BQ_HIP(hipMallocAsync((void**)&dw, bytes, s));
+  BQ_HIP(hipMemcpyAsync(dz, z, bytes, hipMemcpyHostToDevice, s));
+  k_faddeeva<<<(n + 255) / 256, 256, 0, s>>>(n, dz, dw);
+  BQ_HIP(hipGetLastError());
+  BQ_HIP(hipMemcpyAsync(w, dw, bytes, hipMemcpyDeviceToHost, s));
+  BQ_HIP(hipFreeAsync(dz, s));
+  BQ_HIP(hipFreeAsync(dw, s));
+  BQ_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+int bq_qpgf_plan(const double* qd8, double eps, bq_qplan* plan)
+{
+  if (!qd8 || !plan) return -2;
+  return bq_qplan_init(plan, qd8[0], qd8 + 1, qd8 + 4, qd8 + 6, eps);
+}
+
+int bq_qpgf_v1_dev(const bq_qplan* plan, int n, const double* d_r, double* d_out, hipStream_t s)
+{
+  if (n <= 0) return 0;
+  k_qpgf_v1<<<(n + 255) / 256, 256, 0, s>>>(*plan, n, d_r, d_out);
+  BQ_HIP(hipGetLastError());
+  return 0;
+}
+
+int bq_qpgf_v1_batch(const double* qd8, int n, const double* r, double* out)
+{
+  if (n < 0 || !qd8 || (n > 0 && (!r || !out))) { bq::set_error("bq_qpgf_v1_batch: bad arguments"); return -2; }
+  bq_qplan P;
+  if (bq_qpgf_plan(qd8, 1e-15, &P)) { bq::set_error("bq_qpgf_v1_batch: degenerate lattice"); return -3; }
+  if (n == 0) return 0;
+  if (bq::ensure_device()) return -1;
+  hipStream_t s = bq::stream();
+  double *dr = nullptr, *dout = nullptr;
+  BQ_HIP(hipMallocAsync((void**)&dr, sizeof(double) * 3 * (size_t)n, s));
+  BQ_HIP(hipMallocAsync((void**)&dout, sizeof(double) * 8 * (size_t)n, s));
+  BQ_HIP(hipMemcpyAsync(dr, r, sizeof(double) * 3 * (size_t)n, hipMemcpyHostToDevice, s));
+  if (bq_qpgf_v1_dev(&P, n, dr, dout, s)) return -1;
+  BQ_HIP(hipMemcpyAsync(out, dout, sizeof(double) * 8 * (size_t)n, hipMemcpyDeviceToHost, s));
+  BQ_HIP(hipFreeAsync(dr, s));
+  BQ_HIP(hipFreeAsync(dout, s));
+  BQ_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+}  // extern "C"

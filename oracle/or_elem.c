@@ -1,0 +1,578 @@
+/*
+ * or_elem.c -- element-pair boundary-integral coefficients for bilinear quads.
+ *
+ * TEST INFRASTRUCTURE ONLY (see oracle.h).
+ *
+ * Restates, for PREC_DEF_BE = 2 (d3qd2_src/bem3_emf_qd2.h:13) and SW_* = 0
+ * (d3qd2_src/d3qd2_const.h:80-90):
+ *   periodic regular pair, GL/GH         q0_bil_dcoef_v2_cc_GL/_GH  d3qd2_src/q0_bil_dcoef_v2_cc.c:129-290
+ *   periodic singular pair (polar + FD)  q0_bil_dcoef_bd_vt2_cc_DV  :292-301, q0_bil_coef_bd
+ *                                        d3qd2_src/q0_bil_coef.c:246-338, q0_bil_coef_bd_F
+ *                                        q0_bil_dcoef_v2_cc.c:336-454
+ *   free-space regular pair, GL/GH       bil_dcoef_v2_GL/_GH  d3qd2_src/bil_dcoef_v2.c:141-325
+ *   free-space singular pair (PV)        bil_coef_bd  d3qd2_src/bil_coef.c:244-342,
+ *                                        bil_dcoef_bd_tz0_PV/_te0_PV  d3qd2_src/bil_dcoef.c:369-739,
+ *                                        integrands :916-1630
+ *   dispatch + GL->GH fallback           d3qd2_src/d3qd2_elem.c:515-572, 830-886
+ *   field-point coefficients             q0_bil_coef_{4p,9p,GL,GH} d3qd2_src/q0_bil_coef.c:10-198,
+ *                                        bil_coef_{4p,9p,GL,GH} d3qd2_src/bil_coef.c:25-202
+ * Linear triangles (lit_*, q0_lit_*) are not restated yet; or_pair aborts on them.
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <math.h>
+#include <complex.h>
+#include "oracle.h"
+
+#define I_FP 7.957747154594766788444e-2 /* 1/(4 pi) */
+#define P44_N 0.5773502691896257645091
+#define MEPS 1.0e-15
+#define GEPS 1.0e-15
+#define DEPS 0.1
+#define CDFH 1.0e-3
+#define CBD_SDF 0.5
+#define GLN 12
+#define GHN 64
+
+typedef struct { double cr[3][4], cw[3][3]; } elc;
+
+static void get_el(const or_model *m, int s, elc *e)
+{
+  for (int d = 0; d < 3; d++) {
+    for (int c = 0; c < 4; c++) e->cr[d][c] = m->cr[12 * s + 4 * d + c];
+    for (int c = 0; c < 3; c++) e->cw[d][c] = m->cw[9 * s + 3 * d + c];
+  }
+}
+
+static double Nn(int n, double z, double e)
+{
+  double ia = 1.0 / P44_N;
+  switch (n) {
+  case 0: return 0.25 * (1.0 - z * ia) * (1.0 - e * ia);
+  case 1: return 0.25 * (1.0 + z * ia) * (1.0 - e * ia);
+  case 2: return 0.25 * (1.0 + z * ia) * (1.0 + e * ia);
+  default: return 0.25 * (1.0 - z * ia) * (1.0 + e * ia);
+  }
+}
+
+static void rw_at(const elc *e, double z, double t, double *r, double *w)
+{
+  for (int d = 0; d < 3; d++) {
+    r[d] = e->cr[d][0] + e->cr[d][1] * z + e->cr[d][2] * t + e->cr[d][3] * z * t;
+    w[d] = e->cw[d][0] + e->cw[d][1] * z + e->cw[d][2] * t;
+  }
+}
+
+static double dot3(const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+static double nrm3(const double *a) { return sqrt(dot3(a, a)); }
+
+static int is_plane(const elc *e)
+{
+  double b[3] = {e->cr[0][1], e->cr[1][1], e->cr[2][1]}, c[3] = {e->cw[0][2], e->cw[1][2], e->cw[2][2]};
+  return fabs(dot3(b, c) / (nrm3(b) * nrm3(c))) < GEPS;
+}
+
+static double bdC_of(const elc *e) { return e->cr[0][1] * e->cw[0][2] + e->cr[1][1] * e->cw[1][2] + e->cr[2][1] * e->cw[2][2]; }
+
+/* unit tangents at element node nid (d3qd2_elem.c:122-143) */
+static void node_tangents(const or_model *m, int s, int nid, double *vtz, double *vte)
+{
+  elc e;
+  get_el(m, s, &e);
+  for (int d = 0; d < 3; d++) {
+    vtz[d] = e.cr[d][1] + e.cr[d][3] * m->et44[nid];
+    vte[d] = e.cr[d][2] + e.cr[d][3] * m->zt44[nid];
+  }
+  double a = nrm3(vtz), b = nrm3(vte);
+  for (int d = 0; d < 3; d++) { vtz[d] /= a; vte[d] /= b; }
+}
+
+static void flip_sign(double complex *C) { for (int i = 4; i < 9; i++) C[i] = -C[i]; }
+
+static void qpgf_or_die(const or_model *m, int mode, const double *R, double complex *o)
+{
+  int err = or_qpgf(mode, R, MEPS, &m->qd, o);
+  if (err < 0) {
+    fprintf(stderr, "oracle: qpgf error %d at R=(%g,%g,%g)\n", err, R[0], R[1], R[2]);
+    exit(1);
+  }
+}
+
+/* ---- periodic regular pair: tensor GL rule of order n ------------------------------ */
+static void q0_regular(const or_model *m, const double *rt, const double *v0, const double *v1,
+                       int s, int n, const double *x, const double *wq, double complex *CC,
+                       double complex *dC0, double complex *dC1)
+{
+  elc e;
+  get_el(m, s, &e);
+  for (int l = 0; l < 9; l++) CC[l] = dC0[l] = dC1[l] = 0.0;
+  double F = 0, dF0 = 0, dF1 = 0;
+  for (int i = 0; i < n; i++) {
+    double complex tg[4] = {0}, th[4] = {0};
+    double tf = 0, td0 = 0, td1 = 0;
+    for (int j = 0; j < n; j++) {
+      double z = x[i], t = x[j], R[3], W[3];
+      rw_at(&e, z, t, R, W);
+      for (int d = 0; d < 3; d++) R[d] -= rt[d];
+      double aR = nrm3(R), iR = 1.0 / aR, aW = nrm3(W);
+      double rdW = dot3(R, W) * iR, rdv0 = dot3(R, v0) * iR, rdv1 = dot3(R, v1) * iR;
+      double complex o[10];
+      qpgf_or_die(m, 0, R, o);
+      double complex hn = o[1] * W[0] + o[2] * W[1] + o[3] * W[2];
+      for (int ep = 0; ep < 4; ep++) {
+        double Np = Nn(ep, z, t);
+        tg[ep] += wq[j] * o[0] * Np * aW;
+        th[ep] += wq[j] * hn * Np;
+      }
+      tf += wq[j] * rdW * iR * iR;
+      td0 += wq[j] * iR * iR * iR * (3.0 * rdv0 * rdW - dot3(W, v0));
+      td1 += wq[j] * iR * iR * iR * (3.0 * rdv1 * rdW - dot3(W, v1));
+    }
+    for (int ep = 0; ep < 4; ep++) {
+      CC[ep] += wq[i] * tg[ep];
+      CC[ep + 4] += wq[i] * th[ep];
+    }
+    F += wq[i] * tf;
+    dF0 += wq[i] * td0;
+    dF1 += wq[i] * td1;
+  }
+  CC[8] = F * I_FP;
+  dC0[8] = dF0 * I_FP;
+  dC1[8] = dF1 * I_FP;
+}
+
+/* ---- polar rule around (zt, et): visits every (r, theta) node of the 4 sub-triangles -- */
+typedef void (*polar_fn)(void *ctx, double r, double cth, double sth, double wgt);
+
+static void node_r_th(double zt, double et, double *r, double *th)
+{
+  double ztp = 1.0 - zt, ztm = -1.0 - zt, etp = 1.0 - et, etm = -1.0 - et;
+  r[0] = sqrt(ztm * ztm + etm * etm); r[1] = sqrt(ztp * ztp + etm * etm);
+  r[2] = sqrt(ztp * ztp + etp * etp); r[3] = sqrt(ztm * ztm + etp * etp);
+  r[4] = r[0];
+  th[0] = atan2(etm, ztm); th[1] = atan2(etm, ztp); th[2] = atan2(etp, ztp); th[3] = atan2(etp, ztm);
+  th[4] = th[0] + 2.0 * M_PI;
+}
+
+/* theta: two GHN halves per sub-triangle; r: rule (nr, xr, wr) on [r_lo, rm(theta)] */
+static void polar_visit(const or_model *m, double zt, double et, double rlo, int nr,
+                        const double *xr, const double *wr, polar_fn f, void *ctx)
+{
+  double r[5], th[5];
+  node_r_th(zt, et, r, th);
+  for (int j = 0; j < 4; j++) {
+    double r0 = r[j], r1 = r[j + 1], t0 = th[j], t1 = th[j + 1];
+    for (int h = 0; h < 2; h++) {
+      for (int mm = 0; mm < GHN; mm++) {
+        double theta = 0.25 * (t1 - t0) * m->xhi[mm] + (h == 0 ? 0.25 * (t1 + 3.0 * t0) : 0.25 * (3.0 * t1 + t0));
+        double wt = 0.25 * (t1 - t0) * m->whi[mm];
+        double rm = r0 * r1 * sin(t1 - t0) / (r0 * sin(theta - t0) - r1 * sin(theta - t1));
+        double c = cos(theta), s = sin(theta);
+        for (int i = 0; i < nr; i++) {
+          double rr = 0.5 * (rm - rlo) * xr[i] + 0.5 * (rm + rlo);
+          f(ctx, rr, c, s, wt * 0.5 * (rm - rlo) * wr[i]);
+        }
+      }
+    }
+  }
+}
+
+/* K = (R(zt + r c, et + r s) - R(zt, et)) / r and W at that point */
+static void polar_KW(const elc *e, double zt, double et, double r, double c, double s, double *K, double *W,
+                     double *z, double *t)
+{
+  *z = r * c + zt;
+  *t = r * s + et;
+  for (int d = 0; d < 3; d++) {
+    K[d] = e->cr[d][1] * c + e->cr[d][2] * s + e->cr[d][3] * (r * c * s + zt * s + et * c);
+    W[d] = e->cw[d][0] + e->cw[d][1] * (*z) + e->cw[d][2] * (*t);
+  }
+}
+
+typedef struct {
+  const or_model *m;
+  elc e;
+  double zt, et;
+  double complex k;
+  double complex G[4], H[4];
+  double F;
+} pctx;
+
+static void q0_sing_fn(void *vp, double r, double c, double s, double w)
+{
+  pctx *p = vp;
+  double K[3], W[3], z, t, R[3];
+  polar_KW(&p->e, p->zt, p->et, r, c, s, K, W, &z, &t);
+  for (int d = 0; d < 3; d++) R[d] = r * K[d];
+  double complex o[10];
+  qpgf_or_die(p->m, 0, R, o);
+  double aW = nrm3(W);
+  double complex hn = o[1] * W[0] + o[2] * W[1] + o[3] * W[2];
+  for (int n = 0; n < 4; n++) {
+    double Np = Nn(n, z, t);
+    p->G[n] += w * o[0] * Np * aW * r;
+    p->H[n] += w * hn * Np * r;
+  }
+}
+
+static void F_fn(void *vp, double r, double c, double s, double w)
+{
+  pctx *p = vp;
+  double K[3];
+  for (int d = 0; d < 3; d++) K[d] = p->e.cr[d][1] * c + p->e.cr[d][2] * s + p->e.cr[d][3] * (r * c * s + p->zt * s + p->et * c);
+  double aK = nrm3(K);
+  p->F += w * c * s / (aK * aK * aK);
+}
+
+/* F of the element at its own point (zt, et) by the polar rule (free-space solid angle) */
+static double polar_F(const or_model *m, int s, double zt, double et)
+{
+  pctx p = {0};
+  p.m = m;
+  get_el(m, s, &p.e);
+  if (is_plane(&p.e)) return 0.0;
+  p.zt = zt; p.et = et;
+  polar_visit(m, zt, et, 0.0, GLN, m->xli, m->wli, F_fn, &p);
+  return p.F * I_FP * bdC_of(&p.e);
+}
+
+static void q0_singular(const or_model *m, int s, int nid, double complex *CC, double complex *dCz,
+                        double complex *dCe)
+{
+  pctx p = {0};
+  p.m = m;
+  get_el(m, s, &p.e);
+  p.zt = m->zt44[nid];
+  p.et = m->et44[nid];
+  polar_visit(m, p.zt, p.et, 0.0, GLN, m->xli, m->wli, q0_sing_fn, &p);
+  for (int n = 0; n < 4; n++) { CC[n] = p.G[n]; CC[n + 4] = p.H[n]; }
+  CC[8] = polar_F(m, s, p.zt, p.et);
+  for (int i = 0; i < 9; i++) dCz[i] = dCe[i] = 0.0;
+  double Tz[3], Te[3];
+  for (int d = 0; d < 3; d++) {
+    Tz[d] = p.e.cr[d][1] + p.e.cr[d][3] * p.et;
+    Te[d] = p.e.cr[d][2] + p.e.cr[d][3] * p.zt;
+  }
+  double Fp = polar_F(m, s, p.zt + CDFH, p.et), Fm = polar_F(m, s, p.zt - CDFH, p.et);
+  dCz[8] = (Fp - Fm) / (2.0 * CDFH) / nrm3(Tz);
+  Fp = polar_F(m, s, p.zt, p.et + CDFH);
+  Fm = polar_F(m, s, p.zt, p.et - CDFH);
+  dCe[8] = (Fp - Fm) / (2.0 * CDFH) / nrm3(Te);
+}
+
+/* ---- free-space regular pair ------------------------------------------------------- */
+static void fs_regular(const or_model *m, const double *rt, const double *v0, const double *v1,
+                       int ss, double complex k, int n, const double *x, const double *wq,
+                       double complex *CC, double complex *dC0, double complex *dC1)
+{
+  int s = abs(ss);
+  elc e;
+  get_el(m, s, &e);
+  for (int l = 0; l < 9; l++) CC[l] = dC0[l] = dC1[l] = 0.0;
+  double F = 0, dF0 = 0, dF1 = 0;
+  for (int i = 0; i < n; i++) {
+    for (int j = 0; j < n; j++) {
+      double z = x[i], t = x[j], R[3], W[3];
+      rw_at(&e, z, t, R, W);
+      for (int d = 0; d < 3; d++) R[d] -= rt[d];
+      double aR = nrm3(R), iR = 1.0 / aR, aW = nrm3(W);
+      double complex ca = I * k * aR, cb = ca - 1.0, ce = cexp(ca);
+      double rdW = dot3(R, W) * iR, tD = rdW * iR * iR;
+      double rdv0 = dot3(R, v0) * iR, rdv1 = dot3(R, v1) * iR, Wdv0 = dot3(W, v0), Wdv1 = dot3(W, v1);
+      double wij = wq[i] * wq[j];
+      for (int ep = 0; ep < 4; ep++) {
+        double Np = Nn(ep, z, t);
+        CC[ep] += wij * ce * iR * aW * Np;
+        CC[ep + 4] += wij * cb * ce * tD * Np;
+        dC0[ep] += wij * cb * ce * iR * iR * rdv0 * aW * Np;
+        dC1[ep] += wij * cb * ce * iR * iR * rdv1 * aW * Np;
+        dC0[ep + 4] += wij * ce * iR * iR * iR * ((k * k * aR * aR + 3.0 * cb) * rdv0 * rdW - cb * Wdv0) * Np;
+        dC1[ep + 4] += wij * ce * iR * iR * iR * ((k * k * aR * aR + 3.0 * cb) * rdv1 * rdW - cb * Wdv1) * Np;
+      }
+      F += wij * tD;
+      dF0 += wij * iR * iR * iR * (3.0 * rdv0 * rdW - Wdv0);
+      dF1 += wij * iR * iR * iR * (3.0 * rdv1 * rdW - Wdv1);
+    }
+  }
+  for (int ep = 0; ep < 8; ep++) {
+    CC[ep] *= I_FP;
+    dC0[ep] *= (ep < 4 ? -I_FP : I_FP);
+    dC1[ep] *= (ep < 4 ? -I_FP : I_FP);
+  }
+  CC[8] = F * I_FP;
+  dC0[8] = dF0 * I_FP;
+  dC1[8] = dF1 * I_FP;
+  if (ss < 0) { flip_sign(CC); flip_sign(dC0); flip_sign(dC1); }
+}
+
+/* ---- free-space singular pair: G,H,F by polar rule (bil_coef_bd) ------------------- */
+static void fs_sing_GHF_fn(void *vp, double r, double c, double s, double w)
+{
+  pctx *p = vp;
+  double K[3], W[3], z, t;
+  polar_KW(&p->e, p->zt, p->et, r, c, s, K, W, &z, &t);
+  double aK = nrm3(K), aW = nrm3(W);
+  double complex kR = p->k * r * aK, ce = cexp(I * kR);
+  for (int n = 0; n < 4; n++) {
+    double Np = Nn(n, z, t);
+    p->G[n] += w * ce / aK * aW * Np;
+    p->H[n] += w * (I * kR - 1.0) * ce / (aK * aK * aK) * c * s * Np;
+  }
+  p->F += w * c * s / (aK * aK * aK);
+}
+
+/* PV tangential derivative (direction 0 = zeta, 1 = eta), bil_dcoef.c:369-739 */
+typedef struct {
+  const or_model *m;
+  elc e;
+  double zt, et, T[3];
+  int dir;
+  double complex k;
+  double complex dG[4], dH[4];
+  double dF;
+} pvctx;
+
+static double KdT_of(const pvctx *p, const double *K) { return dot3(K, p->T); }
+
+/* D-bar region: r in [DEPS, rm] with GHN points */
+static void pv_Db_fn(void *vp, double r, double c, double s, double w)
+{
+  pvctx *p = vp;
+  double K[3], W[3], z, t;
+  polar_KW(&p->e, p->zt, p->et, r, c, s, K, W, &z, &t);
+  double aK = nrm3(K), iK = 1.0 / aK, aW = nrm3(W), KT = KdT_of(p, K);
+  double complex kR = p->k * r * aK, cb = I * kR - 1.0, ce = cexp(I * kR);
+  double cs_dir = (p->dir == 0) ? c : s;   /* (K.T) multiplies cos for tz, sin for te */
+  double outer = (p->dir == 0) ? s : c;    /* theta factor outside the r integral */
+  for (int n = 0; n < 4; n++) {
+    double Np = Nn(n, z, t);
+    p->dG[n] += w * cb * ce * iK * iK / r * (KT * iK) * Np * aW;
+    p->dH[n] += w * outer * ce / (r * pow(aK, 5)) * ((kR * kR + 3.0 * cb) * KT * cs_dir - cb * aK * aK) * Np;
+  }
+  p->dF += w * outer / (r * pow(aK, 5)) * (3.0 * KT * cs_dir - aK * aK);
+}
+
+/* D-eps disk: theta in 4 quarters of [-pi, pi], r in [0, DEPS] with GLN points */
+static void pv_De(pvctx *p)
+{
+  const or_model *m = p->m;
+  double complex sG[4] = {0}, sH[4] = {0};
+  for (int q = 0; q < 4; q++) {
+    double off = -0.75 * M_PI + 0.5 * M_PI * q;
+    for (int mm = 0; mm < GHN; mm++) {
+      double theta = 0.25 * M_PI * m->xhi[mm] + off, wt = 0.25 * M_PI * m->whi[mm];
+      double c = cos(theta), s = sin(theta);
+      double cs_dir = (p->dir == 0) ? c : s, outer = (p->dir == 0) ? s : c;
+      double complex g[4] = {0}, h0[4] = {0}, h1[4] = {0};
+      for (int i = 0; i < GLN; i++) {
+        double r = 0.5 * DEPS * (m->xli[i] + 1.0), wr = 0.5 * DEPS * m->wli[i];
+        double K[3], W[3], z, t;
+        polar_KW(&p->e, p->zt, p->et, r, c, s, K, W, &z, &t);
+        double aK = nrm3(K), iK = 1.0 / aK, aW = nrm3(W), KT = KdT_of(p, K);
+        double complex kR = p->k * aK * r, cb = I * kR - 1.0;
+        double complex cc = ccos(kR), sn = csin(kR), ce = cexp(I * kR);
+        for (int n = 0; n < 4; n++) {
+          double Np = Nn(n, z, t);
+          g[n] += wr * (cb * sn + kR * cc) * iK * iK / r * (KT * iK) * aW * Np;
+          h0[n] += wr * ce / pow(aK, 4) * ((p->k * kR + 3.0 * I * p->k) * KT * cs_dir - I * p->k * aK * aK) * Np;
+          h1[n] += wr * sn / (pow(aK, 5) * r) * (3.0 * KT * cs_dir - aK * aK) * Np;
+        }
+      }
+      for (int n = 0; n < 4; n++) {
+        sG[n] += wt * g[n];
+        sH[n] += wt * outer * (h0[n] - I * h1[n]);
+      }
+    }
+  }
+  for (int n = 0; n < 4; n++) {
+    p->dG[n] += I * sG[n];
+    p->dH[n] += sH[n];
+  }
+}
+
+/* D-eps principal-value pairing (+r, -r): theta in [0, pi] as two halves */
+static void pv_DP(pvctx *p)
+{
+  const or_model *m = p->m;
+  double complex sG[4] = {0}, sH[4] = {0};
+  double sF = 0;
+  for (int h = 0; h < 2; h++) {
+    for (int mm = 0; mm < GHN; mm++) {
+      double theta = 0.25 * M_PI * m->xhi[mm] + (h == 0 ? 0.25 * M_PI : 0.75 * M_PI);
+      double wt = 0.25 * M_PI * m->whi[mm];
+      double c = cos(theta), s = sin(theta);
+      double cs_dir = (p->dir == 0) ? c : s, outer = (p->dir == 0) ? s : c;
+      double complex g[4] = {0}, hh[4] = {0};
+      double f = 0;
+      for (int i = 0; i < GLN; i++) {
+        double r = 0.5 * DEPS * (m->xli[i] + 1.0), wr = 0.5 * DEPS * m->wli[i];
+        double Kp[3], Km[3], Wp[3], Wm[3];
+        double zp = r * c + p->zt, ep = r * s + p->et, zm = -r * c + p->zt, em = -r * s + p->et;
+        for (int d = 0; d < 3; d++) {
+          Kp[d] = p->e.cr[d][1] * c + p->e.cr[d][2] * s + p->e.cr[d][3] * (r * c * s + p->zt * s + p->et * c);
+          Km[d] = p->e.cr[d][1] * c + p->e.cr[d][2] * s + p->e.cr[d][3] * (-r * c * s + p->zt * s + p->et * c);
+          Wp[d] = p->e.cw[d][0] + p->e.cw[d][1] * zp + p->e.cw[d][2] * ep;
+          Wm[d] = p->e.cw[d][0] + p->e.cw[d][1] * zm + p->e.cw[d][2] * em;
+        }
+        double aKp = nrm3(Kp), aKm = nrm3(Km), aWp = nrm3(Wp), aWm = nrm3(Wm);
+        double KpT = KdT_of(p, Kp), KmT = KdT_of(p, Km);
+        double complex ccp = ccos(p->k * aKp * r), ccm = ccos(-p->k * aKm * r);
+        for (int n = 0; n < 4; n++) {
+          double Np = Nn(n, zp, ep), Nm = Nn(n, zm, em);
+          g[n] += wr / r * (ccp / (aKp * aKp) * (KpT / aKp) * Np * aWp - ccm / (aKm * aKm) * (KmT / aKm) * Nm * aWm);
+          hh[n] += wr / r * (ccp / pow(aKp, 5) * (3.0 * KpT * cs_dir - aKp * aKp) * Np -
+                             ccm / pow(aKm, 5) * (3.0 * KmT * cs_dir - aKm * aKm) * Nm);
+        }
+        f += wr / r * ((3.0 * KpT * cs_dir - aKp * aKp) / pow(aKp, 5) - (3.0 * KmT * cs_dir - aKm * aKm) / pow(aKm, 5));
+      }
+      for (int n = 0; n < 4; n++) {
+        sG[n] += wt * g[n];
+        sH[n] += wt * outer * hh[n];
+      }
+      sF += wt * outer * f;
+    }
+  }
+  for (int n = 0; n < 4; n++) {
+    p->dG[n] -= sG[n];
+    p->dH[n] -= sH[n];
+  }
+  p->dF += sF;
+}
+
+static void fs_pv_deriv(const or_model *m, int s, double zt, double et, double complex k, int dir,
+                        double complex *dCC)
+{
+  pvctx p = {0};
+  p.m = m;
+  get_el(m, s, &p.e);
+  p.zt = zt; p.et = et; p.k = k; p.dir = dir;
+  for (int d = 0; d < 3; d++)
+    p.T[d] = (dir == 0) ? p.e.cr[d][1] + p.e.cr[d][3] * et : p.e.cr[d][2] + p.e.cr[d][3] * zt;
+  double iaT = 1.0 / nrm3(p.T);
+  polar_visit(m, zt, et, DEPS, GHN, m->xhi, m->whi, pv_Db_fn, &p);
+  pv_De(&p);
+  pv_DP(&p);
+  int plane = is_plane(&p.e);
+  double bdC = bdC_of(&p.e);
+  for (int n = 0; n < 4; n++) {
+    dCC[n] = p.dG[n] * (-I_FP * iaT);
+    dCC[n + 4] = plane ? 0.0 : p.dH[n] * bdC * I_FP * iaT;
+  }
+  dCC[8] = plane ? 0.0 : p.dF * bdC * I_FP * iaT;
+}
+
+static void fs_singular(const or_model *m, int ss, int nid, double complex k, double complex *CC,
+                        double complex *dCz, double complex *dCe)
+{
+  int s = abs(ss);
+  pctx p = {0};
+  p.m = m;
+  get_el(m, s, &p.e);
+  p.zt = m->zt44[nid];
+  p.et = m->et44[nid];
+  p.k = k;
+  polar_visit(m, p.zt, p.et, 0.0, GLN, m->xli, m->wli, fs_sing_GHF_fn, &p);
+  int plane = is_plane(&p.e);
+  double bdC = bdC_of(&p.e);
+  for (int n = 0; n < 4; n++) {
+    CC[n] = p.G[n] * I_FP;
+    CC[n + 4] = plane ? 0.0 : p.H[n] * I_FP * bdC;
+  }
+  CC[8] = plane ? 0.0 : p.F * I_FP * bdC;
+  fs_pv_deriv(m, s, p.zt, p.et, k, 0, dCz);
+  fs_pv_deriv(m, s, p.zt, p.et, k, 1, dCe);
+  if (ss < 0) { flip_sign(CC); flip_sign(dCz); flip_sign(dCe); }
+}
+
+int or_pair(const or_model *m, int did, int t, int tn, int s, int type, double complex *CC,
+            double complex *dCz, double complex *dCe)
+{
+  if (m->ed[4 * abs(s) + 3] == 0 || m->ed[4 * abs(t) + 3] == 0) {
+    fprintf(stderr, "oracle: linear triangles are not restated\n");
+    exit(1);
+  }
+  double vtz[3], vte[3];
+  node_tangents(m, abs(t), tn, vtz, vte);
+  const double *rt = m->ren + 12 * abs(t) + 3 * tn;
+  int n = (type >= 3) ? GHN : GLN;
+  const double *x = (type >= 3) ? m->xhi : m->xli, *w = (type >= 3) ? m->whi : m->wli;
+  if (did == 0 && m->ps == 1) {
+    if (t != s) {
+      q0_regular(m, rt, vtz, vte, s, n, x, w, CC, dCz, dCe);
+      if (type < 3 && (fabs(creal(dCz[8])) > CBD_SDF || fabs(creal(dCe[8])) > CBD_SDF)) {
+        q0_regular(m, rt, vtz, vte, s, GHN, m->xhi, m->whi, CC, dCz, dCe);
+        return 1;
+      }
+      return 0;
+    }
+    q0_singular(m, s, tn, CC, dCz, dCe);
+    return 2;
+  }
+  double complex k = m->kn[did];
+  if (t != s) {
+    fs_regular(m, rt, vtz, vte, s, k, n, x, w, CC, dCz, dCe);
+    if (type < 3 && (fabs(creal(dCz[8])) > CBD_SDF || fabs(creal(dCe[8])) > CBD_SDF)) {
+      fs_regular(m, rt, vtz, vte, s, k, GHN, m->xhi, m->whi, CC, dCz, dCe);
+      return 1;
+    }
+    return 0;
+  }
+  fs_singular(m, s, tn, k, CC, dCz, dCe);
+  return 2;
+}
+
+/* ---- field-point coefficients (no derivatives) ------------------------------------- */
+void or_coef_rt(const or_model *m, int did, const double *rt, int ss, int type, double complex *CC)
+{
+  int s = abs(ss);
+  elc e;
+  get_el(m, s, &e);
+  int periodic = (did == 0 && m->ps == 1);
+  double complex k = m->kn[did];
+  double zq[64 * 64], eq[64 * 64], wq[64 * 64];
+  int np = 0;
+  if (type == 0) {
+    for (int i = 0; i < 4; i++) { zq[np] = m->zt44[i]; eq[np] = m->et44[i]; wq[np++] = m->wt44[i]; }
+  } else if (type == 1) {
+    for (int i = 0; i < 9; i++) { zq[np] = m->zt49[i]; eq[np] = m->et49[i]; wq[np++] = m->wt49[i]; }
+  } else {
+    int n = (type == 2) ? GLN : GHN;
+    const double *x = (type == 2) ? m->xli : m->xhi, *w = (type == 2) ? m->wli : m->whi;
+    for (int i = 0; i < n; i++)
+      for (int j = 0; j < n; j++) { zq[np] = x[i]; eq[np] = x[j]; wq[np++] = w[i] * w[j]; }
+  }
+  for (int l = 0; l < 9; l++) CC[l] = 0.0;
+  double F = 0;
+  for (int q = 0; q < np; q++) {
+    double R[3], W[3];
+    rw_at(&e, zq[q], eq[q], R, W);
+    if (type == 0) {
+      /* the 4-point rule reads the stored node data (q0_bil_coef.c:28-30) */
+      int nq = q;
+      for (int d = 0; d < 3; d++) { R[d] = m->ren[12 * s + 3 * nq + d]; W[d] = m->wen[12 * s + 3 * nq + d]; }
+    }
+    for (int d = 0; d < 3; d++) R[d] -= rt[d];
+    double aR = nrm3(R), iR = 1.0 / aR, aW = nrm3(W);
+    double tD = dot3(R, W) * iR * iR * iR;
+    F += wq[q] * tD;
+    if (periodic) {
+      double complex o[10];
+      qpgf_or_die(m, 0, R, o);
+      double complex hn = o[1] * W[0] + o[2] * W[1] + o[3] * W[2];
+      for (int ep = 0; ep < 4; ep++) {
+        double Np = (type == 0) ? (ep == q ? 1.0 : 0.0) : Nn(ep, zq[q], eq[q]);
+        CC[ep] += wq[q] * o[0] * Np * aW;
+        CC[ep + 4] += wq[q] * hn * Np;
+      }
+    } else {
+      double complex ca = I * k * aR, ce = cexp(ca);
+      for (int ep = 0; ep < 4; ep++) {
+        double Np = (type == 0) ? (ep == q ? 1.0 : 0.0) : Nn(ep, zq[q], eq[q]);
+        CC[ep] += wq[q] * I_FP * ce * iR * aW * Np;
+        CC[ep + 4] += wq[q] * I_FP * (ca - 1.0) * ce * tD * Np;
+      }
+    }
+  }
+  CC[8] = F * I_FP;
+  if (!periodic && ss < 0) flip_sign(CC);
+}

@@ -1,0 +1,100 @@
+"""Quasi-periodic Green's function: oracle and GPU kernel against the reference's own
+d3hm_qpgf_d2_v2 (golden vectors from oracle/_ref, built from qgf2_src/d3_qpgf_d2.c), plus
+properties the mathematics fixes: Bloch quasi-periodicity, Ewald/Fourier agreement via
+points on both sides of the switch, gradient by finite differences.
+
+Tolerance: 1e-12 relative to the largest component of each quantity at that point."""
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle_lib
+
+G = Path(__file__).resolve().parent / "golden"
+TOL = 1e-12
+
+
+def _cases():
+    d = np.load(G / "qpgf_ref.npz")
+    for name in d["names"]:
+        name = str(name)
+        yield name, d[name + "_qd"], d[name + "_r"], d[name + "_val"], d[name + "_code"]
+
+
+def _err(a, b):
+    e0 = np.abs(a[:, 0] - b[:, 0]) / np.abs(b[:, 0])
+    e1 = np.max(np.abs(a[:, 1:4] - b[:, 1:4]), 1) / np.max(np.abs(b[:, 1:4]), 1)
+    return np.maximum(e0, e1)
+
+
+@pytest.mark.parametrize("case", [c[0] for c in _cases()])
+def test_oracle_vs_reference(case):
+    for name, qd, r, val, code in _cases():
+        if name != case:
+            continue
+        assert (code > 0).all()
+        got, rc = oracle_lib.qpgf(qd, r, mode=1)
+        assert rc > 0
+        e = _err(got, val)
+        assert e.max() < TOL, (name, r[e.argmax()], e.max())
+        eh = np.max(np.abs(got[:, 4:] - val[:, 4:]), 1) / np.max(np.abs(val[:, 4:]), 1)
+        assert eh.max() < 1e-11, (name, eh.max())
+
+
+def _quasi_periodic_check(fun, qd):
+    rng = np.random.default_rng(5)
+    r = rng.uniform(-0.4, 0.4, (50, 3))
+    d1 = np.array([qd[4], qd[5], 0.0])
+    v0 = fun(qd, r)
+    v1 = fun(qd, r + d1)
+    ph = np.exp(-1j * (qd[1] * d1[0] + qd[2] * d1[1]))
+    return np.max(np.abs(v1[:, 0] - ph * v0[:, 0]) / np.abs(v0[:, 0]))
+
+
+def test_oracle_quasi_periodicity():
+    for name, qd, r, val, code in _cases():
+        e = _quasi_periodic_check(lambda q, x: oracle_lib.qpgf(q, x)[0], qd)
+        assert e < 1e-12, (name, e)
+
+
+def test_oracle_gradient_fd():
+    for name, qd, r, val, code in _cases():
+        rr = r[:20] + 0.05
+        h = 1e-5
+        g = oracle_lib.qpgf(qd, rr)[0]
+        for c in range(3):
+            dp = rr.copy(); dp[:, c] += h
+            dm = rr.copy(); dm[:, c] -= h
+            fd = (oracle_lib.qpgf(qd, dp)[0][:, 0] - oracle_lib.qpgf(qd, dm)[0][:, 0]) / (2 * h)
+            assert np.max(np.abs(fd - g[:, 1 + c]) / np.max(np.abs(g[:, 1:4]), 1)) < 1e-6, (name, c)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", [c[0] for c in _cases()])
+def test_gpu_vs_reference(gpu_lib, case):
+    for name, qd, r, val, code in _cases():
+        if name != case:
+            continue
+        got = gpu_lib.qpgf_v1(qd, r)
+        e = _err(got, val[:, :4])
+        print(name, "gpu qpgf max rel", e.max())
+        assert e.max() < TOL, (name, r[e.argmax()], e.max())
+
+
+@pytest.mark.gpu
+def test_gpu_quasi_periodicity(gpu_lib):
+    for name, qd, r, val, code in _cases():
+        e = _quasi_periodic_check(gpu_lib.qpgf_v1, qd)
+        assert e < 1e-12, (name, e)
+
+
+@pytest.mark.gpu
+def test_gpu_vs_oracle_many(gpu_lib):
+    rng = np.random.default_rng(9)
+    for name, qd, r, val, code in _cases():
+        rr = rng.uniform(-1.2, 1.2, (3000, 3))
+        a = gpu_lib.qpgf_v1(qd, rr)
+        b = oracle_lib.qpgf(qd, rr)[0][:, :4]
+        e = _err(a, b)
+        assert e.max() < TOL, (name, rr[e.argmax()], e.max())
