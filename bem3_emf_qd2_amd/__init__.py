@@ -37,6 +37,7 @@ def _declare(L: ctypes.CDLL) -> None:
     L.bq_device_ok.restype = ctypes.c_int
     L.bq_faddeeva_w_batch.argtypes = [ctypes.c_int, _D, _D]
     L.bq_qpgf_v1_batch.argtypes = [_D, ctypes.c_int, _D, _D]
+    L.bq_qpgf_batch.argtypes = [_D, ctypes.c_int, ctypes.c_int, _D, _D]
 
 
 def _check(rc: int, what: str) -> None:
@@ -61,12 +62,15 @@ def faddeeva_w(z) -> np.ndarray:
     return w[:, 0] + 1j * w[:, 1]
 
 
-def qpgf_v1(qd8, r) -> np.ndarray:
-    """Quasi-periodic Green's function and gradient, shape (n, 4) complex."""
+def qpgf_v1(qd8, r, converged=False) -> np.ndarray:
+    """Quasi-periodic Green's function and gradient, shape (n, 4) complex.
+
+    converged=False reproduces d3hm_qpgf_d2_v1 (whose spectral sum stops early at r_z == 0);
+    converged=True gives the converged sum there, as d3hm_qpgf_d2_qG does for the value."""
     q = _as(qd8)
     assert q.shape == (8,)
     rr = _as(np.asarray(r).reshape(-1, 3))
     out = np.empty((len(rr), 8))
-    _check(lib().bq_qpgf_v1_batch(q.ctypes.data_as(_D), len(rr), rr.ctypes.data_as(_D), out.ctypes.data_as(_D)),
-           "qpgf_v1")
+    _check(lib().bq_qpgf_batch(q.ctypes.data_as(_D), int(bool(converged)), len(rr), rr.ctypes.data_as(_D),
+                               out.ctypes.data_as(_D)), "qpgf")
     return out[:, 0::2] + 1j * out[:, 1::2]

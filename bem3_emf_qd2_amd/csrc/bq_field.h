@@ -1,0 +1,48 @@
+/*
+ * bq_field.h -- device field evaluation (internal, C ABI).
+ */
+#ifndef BQ_FIELD_H
+#define BQ_FIELD_H
+#include "bq_plan.h"
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BQ_MAXD 16 /* domains supported by the field context */
+
+/* device-side view passed to the field kernel by value */
+typedef struct bq_fctx {
+  int ps, MN;
+  bq_qplan plan;
+  const double *E, *ren, *wen, *kn;      /* device */
+  int dNe[BQ_MAXD];
+  const int *sid[BQ_MAXD];               /* device, per domain */
+  const double *Eb[BQ_MAXD], *dEb[BQ_MAXD], *Hb[BQ_MAXD], *dHb[BQ_MAXD]; /* device, [(dNe+1)][4][3] complex */
+} bq_fctx;
+
+/* host-side description of a solved model */
+typedef struct bq_fdata {
+  int Ne, MN, ps;
+  double qd[8];
+  const double *kn;                      /* [MN+1] complex */
+  const double *E, *ren, *wen;           /* [(Ne+1)][24], [(Ne+1)][12], [(Ne+1)][12] */
+  const int *dNe;
+  const int *const *sid;
+  const double *const *Eb, *const *dEb, *const *Hb, *const *dHb;
+  const double *xli, *wli, *xhi, *whi, *z49, *e49, *w49;
+} bq_fdata;
+
+int bq_field_ctx_create(const bq_fdata *f, void **handle);
+/* out[n][13] complex-as-doubles: sum E (3 complex), sum H (3 complex), F (1 real) for each
+ * folded point with known domain id */
+int bq_field_eval(void *handle, int n, const double *pts, const int *did, int type, double *out);
+void bq_field_ctx_destroy(void *handle);
+
+int bq_field_eval_dev(const bq_fctx *C, int npts, const double *d_pts, const int *d_did, int type, double *d_out,
+                      void *stream);
+int bq_field_rules_init(const double *z49, const double *e49, const double *w49);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,56 @@
+/*
+ * bq_solver.h -- device solver session used by solve_bieq (internal, C ABI).
+ *
+ * The host flattens the model (element constants, nodes, sub-domain lists, incident data)
+ * into bq_model; the session uploads it once, assembles every domain's matrices on the
+ * device, solves the global system and provides the two dense products the boundary-field
+ * step needs.  All complex arrays are interleaved (re, im) doubles.
+ */
+#ifndef BQ_SOLVER_H
+#define BQ_SOLVER_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct bq_model {
+  int Ne, NN, MN, ps;
+  double qd[8];          /* QPD2: k, vk[3], vd1[2], vd2[2] */
+  const double *kn;      /* [MN+1] complex */
+  const double *E;       /* [(Ne+1)][24] element constants cr[3][4], cw[3][3], pad */
+  const double *ren;     /* [(Ne+1)][4][3] */
+  const double *nrm;     /* [(Ne+1)][4][3] unit normals at element nodes (main side) */
+  const int *eni;        /* [(Ne+1)][4] */
+  const int *md, *sd;    /* [Ne+1] */
+  const double *Ui, *dUi; /* [(Ne+1)][4][4] complex */
+  const int *dNe;        /* [MN+1] */
+  const int *const *sid; /* [MN+1][dNe+1] */
+  const double *xli, *wli, *xhi, *whi; /* Gauss-Legendre 12 and 64 on [-1, 1] */
+} bq_model;
+
+typedef struct bq_stats {
+  double t_assemble_ms, t_solve_ms, t_fields_ms;
+  int n_gh_pairs;        /* pairs that needed the 64x64 rule, all domains */
+  long long n_qpgf;      /* QPGF evaluations in the assembly (algorithmic count) */
+} bq_stats;
+
+typedef struct bq_sess bq_sess;
+
+bq_sess *bq_sess_create(const bq_model *m);
+void bq_sess_destroy(bq_sess *s);
+int bq_sess_assemble(bq_sess *s);
+/* X[8 NN] complex: the global unknowns in the reference's order (X[c NN + e]) */
+int bq_sess_solve_global(bq_sess *s, double *X);
+/* out[2N][4] complex = dG U' - dH U for domain d (d >= 1 or non-periodic), where
+ * U'[4(s-1)+sn][i] = dU, U = U of the domain's element nodes */
+int bq_sess_dudt(bq_sess *s, int d, const double *U, const double *dU, double *out);
+/* dEH[N][6] = G_d^{-1} (H_d EH), EH[N][6] complex */
+int bq_sess_deh(bq_sess *s, int d, const double *EH, double *dEH);
+/* optional: copy a domain's G, H (row-major N x N) to host */
+int bq_sess_get_GH(bq_sess *s, int d, double *G, double *H);
+int bq_sess_get_dGH(bq_sess *s, int d, double *dG, double *dH, double *tdF);
+const bq_stats *bq_sess_stats(const bq_sess *s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,768 @@
+// bq_assemble.hip -- coefficient-matrix assembly for one domain on the GPU.
+//
+// For collocation node (t, tn) (row 4(t-1)+tn) and element s (columns 4(s-1)..+3) of a
+// domain, the reference computes (d3qd2_src/d3qd2_solve_bieq.c:165-256):
+//   G  = int Gk N_n |W|,  H = int dGk/dn N_n,  F = solid angle / 4 pi,
+// plus for the free-space domains the tangential derivatives dG/dt, dH/dt, dF/dt.
+//   domain 0, periodic:  q0_dcoef_bd_node_t2_cc (d3qd2_src/d3qd2_elem.c:830-886)
+//   other domains:       dcoef_bd_node_t2       (d3qd2_src/d3qd2_elem.c:515-572)
+// with the 12x12 Gauss rule (PREC_DEF_BE = 2), a 64x64 rule when |dF/dt| > 0.5 on the
+// 12x12 rule (CBD_SDF), and polar singular rules when t == s.
+//
+// GPU organisation (one launch per phase, all on one stream):
+//   k_classify   one thread per (row, s): F and dF/dt with the 12x12 rule (no Green's
+//                function needed), flags the pair GL / GH / singular.
+//   k_q0_gl      one wavefront per element pair (t, s): its 4 collocation nodes x 144 Gauss
+//                points = 576 = 9 x 64 QPGF evaluations; one QPGF per Gauss point feeds
+//                all four interpolation functions (the reference evaluates it 4 times).
+//                16-lane butterfly reductions, one row of G and H written per node.
+//   k_q0_gh      one 256-thread block per flagged pair: 4096 points.
+//   k_q0_sing    one block per collocation node: polar rule (4 sub-triangles x 128 angles
+//                x 12 radii), F by the polar rule and dF/dt by central differences.
+//   k_fs_gl/_gh/_sing   same for the free-space Helmholtz kernel with complex k, adding
+//                dG/dt and dH/dt; the singular derivatives use the principal-value split
+//                of d3qd2_src/bil_dcoef.c:369-739.
+//   k_row_diag   fixed-order row sums of F into the diagonal term and tdF.
+// Every output element is written by exactly one thread: no atomics on values, so results
+// do not depend on scheduling.
+#include "bq_runtime.hpp"
+#include "bq_qpgf.hpp"
+#include "bq_elem.hpp"
+#include "bq_reduce.hpp"
+#include "../bq_assemble.h"
+
+using namespace bq;
+
+constexpr double kDEPS = 0.1;     // D_epsilon, reference d3qd2_src/d3qd2_const.h:28
+constexpr double kCDFH = 1.0e-3;  // finite-difference step, d3qd2_const.h:29
+constexpr double kSDF = 0.5;      // GL -> GH switch, d3qd2_const.h:78
+constexpr double kTWOPI = 6.283185307179586476925286766559;
+
+struct ColPt {
+  double rt[3], vz[3], ve[3];
+};
+
+__device__ __forceinline__ void colloc(const double* __restrict__ E, const double* __restrict__ ren, int at, int tn,
+                                       ColPt& c)
+{
+  Elem e;
+  load_elem(E, at, e);
+  node_tangents(e, tn, c.vz, c.ve);
+  const double* p = ren + (size_t)at * 12 + 3 * tn;
+  c.rt[0] = p[0];
+  c.rt[1] = p[1];
+  c.rt[2] = p[2];
+}
+
+// free-space solid-angle integrand at one point (F, dF/dtz, dF/dte), unscaled
+__device__ __forceinline__ void f_terms(const double R[3], const double W[3], const ColPt& c, double& f, double& f0,
+                                        double& f1)
+{
+  double aR = sqrt(dot3(R, R)), iR = 1.0 / aR;
+  double rdW = dot3(R, W) * iR, rdv0 = dot3(R, c.vz) * iR, rdv1 = dot3(R, c.ve) * iR;
+  double iR3 = iR * iR * iR;
+  f = rdW * iR * iR;
+  f0 = iR3 * (3.0 * rdv0 * rdW - dot3(W, c.vz));
+  f1 = iR3 * (3.0 * rdv1 * rdW - dot3(W, c.ve));
+}
+
+// ------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_classify(const double* __restrict__ E, const double* __restrict__ ren,
+                                                  const int* __restrict__ sid, int dNe, int freespace,
+                                                  double* __restrict__ F3, unsigned char* __restrict__ flag)
+{
+  long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long N = 4L * dNe;
+  if (idx >= N * dNe) return;
+  int row = (int)(idx / dNe), sc = (int)(idx % dNe);
+  int td = sid[row / 4 + 1], tn = row & 3, sd = sid[sc + 1];
+  double* o = F3 + 3 * idx;
+  if (td == sd) {
+    flag[idx] = 2;
+    o[0] = o[1] = o[2] = 0.0;
+    return;
+  }
+  ColPt c;
+  colloc(E, ren, abs(td), tn, c);
+  Elem e;
+  load_elem(E, abs(sd), e);
+  double F = 0, F0 = 0, F1 = 0;
+  for (int i = 0; i < 12; ++i) {
+    double tf = 0, t0 = 0, t1 = 0;
+    for (int j = 0; j < 12; ++j) {
+      double R[3], W[3], f, f0, f1;
+      rw_at(e, c_xl[i], c_xl[j], R, W);
+      R[0] -= c.rt[0]; R[1] -= c.rt[1]; R[2] -= c.rt[2];
+      f_terms(R, W, c, f, f0, f1);
+      tf += c_wl[j] * f;
+      t0 += c_wl[j] * f0;
+      t1 += c_wl[j] * f1;
+    }
+    F += c_wl[i] * tf;
+    F0 += c_wl[i] * t0;
+    F1 += c_wl[i] * t1;
+  }
+  double sg = (freespace && sd < 0) ? -kI4PI : kI4PI;
+  o[0] = F * sg;
+  o[1] = F0 * sg;
+  o[2] = F1 * sg;
+  flag[idx] = (fabs(F0 * kI4PI) > kSDF || fabs(F1 * kI4PI) > kSDF) ? 1 : 0;
+}
+
+// ------------------------------------------------------------------------------------
+// domain 0 (periodic), regular pairs, 12x12 rule
+__global__ void __launch_bounds__(256) k_q0_gl(bq_qplan P, const double* __restrict__ E, const double* __restrict__ ren,
+                                               const int* __restrict__ sid, int dNe, const unsigned char* __restrict__ flag,
+                                               double2* __restrict__ G, double2* __restrict__ H)
+{
+  const int lane = threadIdx.x & 63;
+  long pair = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (pair >= (long)dNe * dNe) return;
+  const int t = (int)(pair / dNe) + 1, s = (int)(pair % dNe) + 1;
+  if (t == s) return;
+  const int tn = lane >> 4, sub = lane & 15;
+  const long N = 4L * dNe, row = 4L * (t - 1) + tn;
+  const bool active = flag[row * dNe + (s - 1)] == 0;
+  ColPt c;
+  colloc(E, ren, abs(sid[t]), tn, c);
+  Elem e;
+  load_elem(E, abs(sid[s]), e);
+  double acc[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) acc[k] = 0.0;
+  if (active) {
+    for (int m = 0; m < 9; ++m) {
+      int p = sub + 16 * m, i = p / 12, j = p - 12 * i;
+      double z = c_xl[i], et = c_xl[j], w = c_wl[i] * c_wl[j];
+      double R[3], W[3], Nn[4];
+      rw_at(e, z, et, R, W);
+      R[0] -= c.rt[0]; R[1] -= c.rt[1]; R[2] -= c.rt[2];
+      QG q;
+      qpgf_v1(P, R[0], R[1], R[2], q);
+      double aW = sqrt(dot3(W, W));
+      cplx hn = q.d[0] * W[0] + q.d[1] * W[1] + q.d[2] * W[2];
+      shapeN4(z, et, Nn);
+#pragma unroll
+      for (int n = 0; n < 4; ++n) {
+        double gw = w * Nn[n] * aW, hw = w * Nn[n];
+        acc[2 * n] += gw * q.g.re;
+        acc[2 * n + 1] += gw * q.g.im;
+        acc[8 + 2 * n] += hw * hn.re;
+        acc[8 + 2 * n + 1] += hw * hn.im;
+      }
+    }
+  }
+  group_sum<16, 16>(acc);
+  if (active && sub < 2) {
+    double2* dst = (sub == 0 ? G : H) + row * N + 4L * (s - 1);
+    const int o = sub * 8;
+#pragma unroll
+    for (int n = 0; n < 4; ++n) dst[n] = make_double2(acc[o + 2 * n], acc[o + 2 * n + 1]);
+  }
+}
+
+// domain 0, flagged pairs: 64x64 rule, F and dF/dt recomputed with it
+__global__ void __launch_bounds__(256) k_q0_gh(bq_qplan P, const double* __restrict__ E, const double* __restrict__ ren,
+                                               const int* __restrict__ sid, int dNe, const int* __restrict__ list,
+                                               double2* __restrict__ G, double2* __restrict__ H, double* __restrict__ F3)
+{
+  __shared__ double lds[19 * 4];
+  const long code = list[blockIdx.x];
+  const long N = 4L * dNe, row = code / dNe;
+  const int sc = (int)(code % dNe), t = (int)(row / 4) + 1, tn = (int)(row & 3);
+  ColPt c;
+  colloc(E, ren, abs(sid[t]), tn, c);
+  Elem e;
+  load_elem(E, abs(sid[sc + 1]), e);
+  double acc[19];
+#pragma unroll
+  for (int k = 0; k < 19; ++k) acc[k] = 0.0;
+  for (int m = 0; m < 16; ++m) {
+    int p = threadIdx.x + 256 * m, i = p >> 6, j = p & 63;
+    double z = c_xh[i], et = c_xh[j], w = c_wh[i] * c_wh[j];
+    double R[3], W[3], Nn[4], f, f0, f1;
+    rw_at(e, z, et, R, W);
+    R[0] -= c.rt[0]; R[1] -= c.rt[1]; R[2] -= c.rt[2];
+    f_terms(R, W, c, f, f0, f1);
+    QG q;
+    qpgf_v1(P, R[0], R[1], R[2], q);
+    double aW = sqrt(dot3(W, W));
+    cplx hn = q.d[0] * W[0] + q.d[1] * W[1] + q.d[2] * W[2];
+    shapeN4(z, et, Nn);
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+      double gw = w * Nn[n] * aW, hw = w * Nn[n];
+      acc[2 * n] += gw * q.g.re;
+      acc[2 * n + 1] += gw * q.g.im;
+      acc[8 + 2 * n] += hw * hn.re;
+      acc[8 + 2 * n + 1] += hw * hn.im;
+    }
+    acc[16] += w * f;
+    acc[17] += w * f0;
+    acc[18] += w * f1;
+  }
+  block_sum256<19>(acc, lds);
+  if (threadIdx.x == 0) {
+    double2* g = G + row * N + 4L * sc;
+    double2* h = H + row * N + 4L * sc;
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+      g[n] = make_double2(acc[2 * n], acc[2 * n + 1]);
+      h[n] = make_double2(acc[8 + 2 * n], acc[8 + 2 * n + 1]);
+    }
+    double* o = F3 + 3 * code;
+    o[0] = acc[16] * kI4PI;
+    o[1] = acc[17] * kI4PI;
+    o[2] = acc[18] * kI4PI;
+  }
+}
+
+// polar F of element e at its own parameter point (zt, et), unscaled by bdC/4pi
+__device__ double polar_F_part(const Elem& e, double zt, double et)
+{
+  double r[5], th[5];
+  corner_r_th(zt, et, r, th);
+  double F = 0.0;
+  for (int q = threadIdx.x; q < 512; q += 256) {
+    int j = q >> 7, m = q & 127;
+    double theta, wt;
+    polar_theta(th, j, m, theta, wt);
+    double rm = polar_rm(r, th, j, theta), cth, sth;
+    sincos(theta, &sth, &cth);
+    double acc = 0.0;
+    for (int i = 0; i < 12; ++i) {
+      double rr = 0.5 * rm * (c_xl[i] + 1.0), K[3];
+      polar_K(e, zt, et, rr, cth, sth, K);
+      double aK = sqrt(dot3(K, K));
+      acc += c_wl[i] * cth * sth / (aK * aK * aK);
+    }
+    F += wt * 0.5 * rm * acc;
+  }
+  return F;
+}
+
+// domain 0, singular pairs: one block per collocation node
+__global__ void __launch_bounds__(256) k_q0_sing(bq_qplan P, const double* __restrict__ E, const int* __restrict__ sid,
+                                                 int dNe, double2* __restrict__ G, double2* __restrict__ H,
+                                                 double* __restrict__ F3)
+{
+  __shared__ double lds[21 * 4];
+  const long N = 4L * dNe, row = blockIdx.x;
+  const int t = (int)(row / 4) + 1, tn = (int)(row & 3), as = abs(sid[t]);
+  Elem e;
+  load_elem(E, as, e);
+  const double zt = node_z(tn), et = node_e(tn);
+  double r[5], th[5];
+  corner_r_th(zt, et, r, th);
+  double acc[21];
+#pragma unroll
+  for (int k = 0; k < 21; ++k) acc[k] = 0.0;
+  for (int q = threadIdx.x; q < 512; q += 256) {
+    int j = q >> 7, m = q & 127;
+    double theta, wt;
+    polar_theta(th, j, m, theta, wt);
+    double rm = polar_rm(r, th, j, theta), cth, sth;
+    sincos(theta, &sth, &cth);
+    for (int i = 0; i < 12; ++i) {
+      double rr = 0.5 * rm * (c_xl[i] + 1.0), w = wt * 0.5 * rm * c_wl[i];
+      double K[3], R[3], W[3], Nn[4];
+      polar_K(e, zt, et, rr, cth, sth, K);
+      double z = rr * cth + zt, ee = rr * sth + et;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        R[d] = rr * K[d];
+        W[d] = e.cw[d][0] + e.cw[d][1] * z + e.cw[d][2] * ee;
+      }
+      QG qq;
+      qpgf_v1(P, R[0], R[1], R[2], qq);
+      // the reference takes G here from d3hm_qpgf_d2_qG and the gradient from _dqG
+      // (d3qd2_src/q0_bil_coef.c:366, :399, :444, :489); they differ only at R_z == 0
+      if (R[2] == 0.0) {
+        QG qa;
+        qpgf_v1(P, R[0], R[1], R[2], qa, false);
+        qq.g = qa.g;
+      }
+      double aW = sqrt(dot3(W, W));
+      cplx hn = qq.d[0] * W[0] + qq.d[1] * W[1] + qq.d[2] * W[2];
+      shapeN4(z, ee, Nn);
+#pragma unroll
+      for (int n = 0; n < 4; ++n) {
+        double gw = w * Nn[n] * aW * rr, hw = w * Nn[n] * rr;
+        acc[2 * n] += gw * qq.g.re;
+        acc[2 * n + 1] += gw * qq.g.im;
+        acc[8 + 2 * n] += hw * hn.re;
+        acc[8 + 2 * n + 1] += hw * hn.im;
+      }
+    }
+  }
+  // F at the node and at the four shifted points (central differences, CDFH)
+  acc[16] = polar_F_part(e, zt, et);
+  acc[17] = polar_F_part(e, zt + kCDFH, et);
+  acc[18] = polar_F_part(e, zt - kCDFH, et);
+  acc[19] = polar_F_part(e, zt, et + kCDFH);
+  acc[20] = polar_F_part(e, zt, et - kCDFH);
+  block_sum256<21>(acc, lds);
+  if (threadIdx.x == 0) {
+    double2* g = G + row * N + 4L * (t - 1);
+    double2* h = H + row * N + 4L * (t - 1);
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+      g[n] = make_double2(acc[2 * n], acc[2 * n + 1]);
+      h[n] = make_double2(acc[8 + 2 * n], acc[8 + 2 * n + 1]);
+    }
+    double sc = elem_plane(e) ? 0.0 : elem_bdC(e) * kI4PI;
+    double Tz[3], Te[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      Tz[d] = e.cr[d][1] + e.cr[d][3] * et;
+      Te[d] = e.cr[d][2] + e.cr[d][3] * zt;
+    }
+    double* o = F3 + 3 * (row * dNe + (t - 1));
+    o[0] = acc[16] * sc;
+    o[1] = (acc[17] - acc[18]) * sc / (2.0 * kCDFH) / sqrt(dot3(Tz, Tz));
+    o[2] = (acc[19] - acc[20]) * sc / (2.0 * kCDFH) / sqrt(dot3(Te, Te));
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// free-space kernel terms at one regular point; adds w-weighted contributions of the
+// 6 quantities x 4 shape functions to acc (48 doubles):
+//   [0..7] G, [8..15] H, [16..23] dG/dtz, [24..31] dG/dte, [32..39] dH/dtz, [40..47] dH/dte
+__device__ __forceinline__ void fs_point(const double R[3], const double W[3], const double Nn[4], const ColPt& c,
+                                         cplx k, double w, double (&acc)[48])
+{
+  double aR = sqrt(dot3(R, R)), iR = 1.0 / aR, aW = sqrt(dot3(W, W));
+  cplx ca = cmul_i(k * aR);                // i k R
+  cplx ce = cexp(ca);
+  cplx cb = ca - mk(1.0, 0.0);
+  double rdW = dot3(R, W) * iR, tD = rdW * iR * iR;
+  double rdv0 = dot3(R, c.vz) * iR, rdv1 = dot3(R, c.ve) * iR, Wdv0 = dot3(W, c.vz), Wdv1 = dot3(W, c.ve);
+  cplx g = ce * (iR * aW);
+  cplx h = cb * ce * tD;
+  cplx dg = cb * ce * (iR * iR * aW);
+  cplx k2R2 = k * k * (aR * aR);
+  cplx t3 = k2R2 + 3.0 * cb;
+  cplx ce3 = ce * (iR * iR * iR);
+  cplx dh0 = ce3 * (t3 * (rdv0 * rdW) - cb * Wdv0);
+  cplx dh1 = ce3 * (t3 * (rdv1 * rdW) - cb * Wdv1);
+#pragma unroll
+  for (int n = 0; n < 4; ++n) {
+    double wn = w * Nn[n];
+    acc[2 * n] += wn * g.re;            acc[2 * n + 1] += wn * g.im;
+    acc[8 + 2 * n] += wn * h.re;        acc[8 + 2 * n + 1] += wn * h.im;
+    acc[16 + 2 * n] += wn * rdv0 * dg.re; acc[16 + 2 * n + 1] += wn * rdv0 * dg.im;
+    acc[24 + 2 * n] += wn * rdv1 * dg.re; acc[24 + 2 * n + 1] += wn * rdv1 * dg.im;
+    acc[32 + 2 * n] += wn * dh0.re;     acc[32 + 2 * n + 1] += wn * dh0.im;
+    acc[40 + 2 * n] += wn * dh1.re;     acc[40 + 2 * n + 1] += wn * dh1.im;
+  }
+}
+
+// scale by 1/4pi (dG by -1/4pi), flip the sign of H, dH for elements seen from their
+// sub-domain side, and store G, H rows and the dG/dH derivative rows
+__device__ __forceinline__ void fs_store(const double (&acc)[48], bool neg, long row, long N, long col,
+                                         double2* G, double2* H, double2* dG, double2* dH)
+{
+  double sh = neg ? -kI4PI : kI4PI;
+#pragma unroll
+  for (int n = 0; n < 4; ++n) {
+    G[row * N + col + n] = make_double2(acc[2 * n] * kI4PI, acc[2 * n + 1] * kI4PI);
+    H[row * N + col + n] = make_double2(acc[8 + 2 * n] * sh, acc[8 + 2 * n + 1] * sh);
+    dG[(2 * row) * N + col + n] = make_double2(-acc[16 + 2 * n] * kI4PI, -acc[16 + 2 * n + 1] * kI4PI);
+    dG[(2 * row + 1) * N + col + n] = make_double2(-acc[24 + 2 * n] * kI4PI, -acc[24 + 2 * n + 1] * kI4PI);
+    dH[(2 * row) * N + col + n] = make_double2(acc[32 + 2 * n] * sh, acc[32 + 2 * n + 1] * sh);
+    dH[(2 * row + 1) * N + col + n] = make_double2(acc[40 + 2 * n] * sh, acc[40 + 2 * n + 1] * sh);
+  }
+}
+
+__global__ void __launch_bounds__(256) k_fs_gl(double kr, double ki, const double* __restrict__ E,
+                                               const double* __restrict__ ren, const int* __restrict__ sid, int dNe,
+                                               const unsigned char* __restrict__ flag, double2* __restrict__ G,
+                                               double2* __restrict__ H, double2* __restrict__ dG,
+                                               double2* __restrict__ dH)
+{
+  const int lane = threadIdx.x & 63;
+  long pair = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (pair >= (long)dNe * dNe) return;
+  const int t = (int)(pair / dNe) + 1, s = (int)(pair % dNe) + 1;
+  if (t == s) return;
+  const int tn = lane >> 4, sub = lane & 15;
+  const long N = 4L * dNe, row = 4L * (t - 1) + tn;
+  const bool active = flag[row * dNe + (s - 1)] == 0;
+  const int sd = sid[s];
+  ColPt c;
+  colloc(E, ren, abs(sid[t]), tn, c);
+  Elem e;
+  load_elem(E, abs(sd), e);
+  const cplx k = mk(kr, ki);
+  double acc[48];
+#pragma unroll
+  for (int q = 0; q < 48; ++q) acc[q] = 0.0;
+  if (active) {
+    for (int m = 0; m < 9; ++m) {
+      int p = sub + 16 * m, i = p / 12, j = p - 12 * i;
+      double z = c_xl[i], et = c_xl[j];
+      double R[3], W[3], Nn[4];
+      rw_at(e, z, et, R, W);
+      R[0] -= c.rt[0]; R[1] -= c.rt[1]; R[2] -= c.rt[2];
+      shapeN4(z, et, Nn);
+      fs_point(R, W, Nn, c, k, c_wl[i] * c_wl[j], acc);
+    }
+  }
+  group_sum<16, 48>(acc);
+  if (active && sub == 0) fs_store(acc, sd < 0, row, N, 4L * (s - 1), G, H, dG, dH);
+}
+
+__global__ void __launch_bounds__(256) k_fs_gh(double kr, double ki, const double* __restrict__ E,
+                                               const double* __restrict__ ren, const int* __restrict__ sid, int dNe,
+                                               const int* __restrict__ list, double2* __restrict__ G,
+                                               double2* __restrict__ H, double2* __restrict__ dG,
+                                               double2* __restrict__ dH, double* __restrict__ F3)
+{
+  __shared__ double lds[51 * 4];
+  const long code = list[blockIdx.x];
+  const long N = 4L * dNe, row = code / dNe;
+  const int sc = (int)(code % dNe), t = (int)(row / 4) + 1, tn = (int)(row & 3), sd = sid[sc + 1];
+  ColPt c;
+  colloc(E, ren, abs(sid[t]), tn, c);
+  Elem e;
+  load_elem(E, abs(sd), e);
+  const cplx k = mk(kr, ki);
+  double acc[48];
+#pragma unroll
+  for (int q = 0; q < 48; ++q) acc[q] = 0.0;
+  double fa[3] = {0.0, 0.0, 0.0};
+  for (int m = 0; m < 16; ++m) {
+    int p = threadIdx.x + 256 * m, i = p >> 6, j = p & 63;
+    double z = c_xh[i], et = c_xh[j], w = c_wh[i] * c_wh[j];
+    double R[3], W[3], Nn[4], f, f0, f1;
+    rw_at(e, z, et, R, W);
+    R[0] -= c.rt[0]; R[1] -= c.rt[1]; R[2] -= c.rt[2];
+    f_terms(R, W, c, f, f0, f1);
+    fa[0] += w * f;
+    fa[1] += w * f0;
+    fa[2] += w * f1;
+    shapeN4(z, et, Nn);
+    fs_point(R, W, Nn, c, k, w, acc);
+  }
+  block_sum256<48>(acc, lds);
+  block_sum256<3>(fa, lds);
+  if (threadIdx.x == 0) {
+    fs_store(acc, sd < 0, row, N, 4L * sc, G, H, dG, dH);
+    double sg = sd < 0 ? -kI4PI : kI4PI;
+    double* o = F3 + 3 * code;
+    o[0] = fa[0] * sg;
+    o[1] = fa[1] * sg;
+    o[2] = fa[2] * sg;
+  }
+}
+
+// free-space singular pair: G, H, F by the polar rule (d3qd2_src/bil_coef.c:244-342) and
+// the principal-value tangential derivatives (d3qd2_src/bil_dcoef.c:369-739)
+__global__ void __launch_bounds__(256) k_fs_sing(double kr, double ki, const double* __restrict__ E,
+                                                 const int* __restrict__ sid, int dNe, double2* __restrict__ G,
+                                                 double2* __restrict__ H, double2* __restrict__ dG,
+                                                 double2* __restrict__ dH, double* __restrict__ F3)
+{
+  __shared__ double lds[17 * 4];
+  const long N = 4L * dNe, row = blockIdx.x;
+  const int t = (int)(row / 4) + 1, tn = (int)(row & 3), sd = sid[t];
+  Elem e;
+  load_elem(E, abs(sd), e);
+  const cplx k = mk(kr, ki);
+  const double zt = node_z(tn), et = node_e(tn);
+  double r[5], th[5];
+  corner_r_th(zt, et, r, th);
+  const bool plane = elem_plane(e);
+  const double bdC = elem_bdC(e);
+  const double sgn = (sd < 0) ? -1.0 : 1.0;
+  const long col = 4L * (t - 1);
+
+  // (a) G, H, F
+  {
+    double a[17];
+#pragma unroll
+    for (int q = 0; q < 17; ++q) a[q] = 0.0;
+    for (int q = threadIdx.x; q < 512; q += 256) {
+      int j = q >> 7, m = q & 127;
+      double theta, wt;
+      polar_theta(th, j, m, theta, wt);
+      double rm = polar_rm(r, th, j, theta), cth, sth;
+      sincos(theta, &sth, &cth);
+      for (int i = 0; i < 12; ++i) {
+        double rr = 0.5 * rm * (c_xl[i] + 1.0), w = wt * 0.5 * rm * c_wl[i];
+        double K[3], W[3], Nn[4];
+        polar_K(e, zt, et, rr, cth, sth, K);
+        double z = rr * cth + zt, ee = rr * sth + et;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) W[d] = e.cw[d][0] + e.cw[d][1] * z + e.cw[d][2] * ee;
+        double aK = sqrt(dot3(K, K)), aW = sqrt(dot3(W, W));
+        cplx kR = k * (rr * aK);
+        cplx ce = cexp(cmul_i(kR));
+        cplx g = ce * (aW / aK);
+        cplx h = (cmul_i(kR) - mk(1.0, 0.0)) * ce * (cth * sth / (aK * aK * aK));
+        shapeN4(z, ee, Nn);
+#pragma unroll
+        for (int n = 0; n < 4; ++n) {
+          a[2 * n] += w * Nn[n] * g.re;
+          a[2 * n + 1] += w * Nn[n] * g.im;
+          a[8 + 2 * n] += w * Nn[n] * h.re;
+          a[8 + 2 * n + 1] += w * Nn[n] * h.im;
+        }
+        a[16] += w * cth * sth / (aK * aK * aK);
+      }
+    }
+    block_sum256<17>(a, lds);
+    if (threadIdx.x == 0) {
+      double sh = plane ? 0.0 : bdC * kI4PI * sgn;
+#pragma unroll
+      for (int n = 0; n < 4; ++n) {
+        G[row * N + col + n] = make_double2(a[2 * n] * kI4PI, a[2 * n + 1] * kI4PI);
+        H[row * N + col + n] = make_double2(a[8 + 2 * n] * sh, a[8 + 2 * n + 1] * sh);
+      }
+      F3[3 * (row * dNe + (t - 1))] = a[16] * sh;
+    }
+  }
+
+  // (b) tangential derivatives, direction 0 = zeta, 1 = eta
+  for (int dir = 0; dir < 2; ++dir) {
+    double T[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) T[d] = (dir == 0) ? e.cr[d][1] + e.cr[d][3] * et : e.cr[d][2] + e.cr[d][3] * zt;
+    const double iaT = 1.0 / sqrt(dot3(T, T));
+    double a[17];  // dG (8), dH (8), dF (1)
+#pragma unroll
+    for (int q = 0; q < 17; ++q) a[q] = 0.0;
+    // D-bar region: r in [DEPS, rm], 64 radii
+    for (int q = threadIdx.x; q < 512; q += 256) {
+      int j = q >> 7, m = q & 127;
+      double theta, wt;
+      polar_theta(th, j, m, theta, wt);
+      double rm = polar_rm(r, th, j, theta), cth, sth;
+      sincos(theta, &sth, &cth);
+      const double csd = dir == 0 ? cth : sth, outer = dir == 0 ? sth : cth;
+      for (int i = 0; i < 64; ++i) {
+        double rr = 0.5 * (rm - kDEPS) * c_xh[i] + 0.5 * (rm + kDEPS), w = wt * 0.5 * (rm - kDEPS) * c_wh[i];
+        double K[3], W[3], Nn[4];
+        polar_K(e, zt, et, rr, cth, sth, K);
+        double z = rr * cth + zt, ee = rr * sth + et;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) W[d] = e.cw[d][0] + e.cw[d][1] * z + e.cw[d][2] * ee;
+        double aK = sqrt(dot3(K, K)), iK = 1.0 / aK, aW = sqrt(dot3(W, W)), KT = dot3(K, T);
+        double iK5 = iK * iK * iK * iK * iK;
+        cplx kR = k * (rr * aK);
+        cplx ce = cexp(cmul_i(kR));
+        cplx cb = cmul_i(kR) - mk(1.0, 0.0);
+        cplx g = cb * ce * (iK * iK / rr * (KT * iK) * aW);
+        cplx h = ce * ((kR * kR + 3.0 * cb) * (KT * csd) - cb * (aK * aK)) * (outer * iK5 / rr);
+        shapeN4(z, ee, Nn);
+#pragma unroll
+        for (int n = 0; n < 4; ++n) {
+          a[2 * n] += w * Nn[n] * g.re;
+          a[2 * n + 1] += w * Nn[n] * g.im;
+          a[8 + 2 * n] += w * Nn[n] * h.re;
+          a[8 + 2 * n + 1] += w * Nn[n] * h.im;
+        }
+        a[16] += w * outer * (3.0 * KT * csd - aK * aK) * iK5 / rr;
+      }
+    }
+    // D-eps disk: theta in [-pi, pi] (4 quarters x 64), r in [0, DEPS] (12 radii)
+    for (int q = threadIdx.x; q < 256; q += 256) {
+      int quarter = q >> 6, mm = q & 63;
+      double theta = 0.25 * bq::kPI * c_xh[mm] + (-0.75 * bq::kPI + 0.5 * bq::kPI * quarter);
+      double wt = 0.25 * bq::kPI * c_wh[mm], cth, sth;
+      sincos(theta, &sth, &cth);
+      const double csd = dir == 0 ? cth : sth, outer = dir == 0 ? sth : cth;
+      cplx gs[4], h0s[4], h1s[4];
+#pragma unroll
+      for (int n = 0; n < 4; ++n) gs[n] = h0s[n] = h1s[n] = mk(0, 0);
+      for (int i = 0; i < 12; ++i) {
+        double rr = 0.5 * kDEPS * (c_xl[i] + 1.0), wr = 0.5 * kDEPS * c_wl[i];
+        double K[3], W[3], Nn[4];
+        polar_K(e, zt, et, rr, cth, sth, K);
+        double z = rr * cth + zt, ee = rr * sth + et;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) W[d] = e.cw[d][0] + e.cw[d][1] * z + e.cw[d][2] * ee;
+        double aK = sqrt(dot3(K, K)), iK = 1.0 / aK, aW = sqrt(dot3(W, W)), KT = dot3(K, T);
+        cplx kR = k * (rr * aK);
+        cplx cb = cmul_i(kR) - mk(1.0, 0.0);
+        cplx cc = ccos(kR), sn = csin(kR), ce = cexp(cmul_i(kR));
+        cplx g = (cb * sn + kR * cc) * (iK * iK / rr * (KT * iK) * aW);
+        cplx ik = cmul_i(k);
+        cplx h0 = ce * ((k * kR + 3.0 * ik) * (KT * csd) - ik * (aK * aK)) * (iK * iK * iK * iK);
+        cplx h1 = sn * ((3.0 * KT * csd - aK * aK) * iK * iK * iK * iK * iK / rr);
+        shapeN4(z, ee, Nn);
+#pragma unroll
+        for (int n = 0; n < 4; ++n) {
+          gs[n] += g * (wr * Nn[n]);
+          h0s[n] += h0 * (wr * Nn[n]);
+          h1s[n] += h1 * (wr * Nn[n]);
+        }
+      }
+#pragma unroll
+      for (int n = 0; n < 4; ++n) {
+        cplx ig = cmul_i(gs[n]) * wt;                            // + i * De
+        cplx hh = (h0s[n] - cmul_i(h1s[n])) * (wt * outer);      // + (De0 - i De1)
+        a[2 * n] += ig.re;
+        a[2 * n + 1] += ig.im;
+        a[8 + 2 * n] += hh.re;
+        a[8 + 2 * n + 1] += hh.im;
+      }
+    }
+    // D-eps principal-value pairing of +r and -r: theta in [0, pi] (2 x 64), 12 radii
+    for (int q = threadIdx.x; q < 128; q += 256) {
+      int h = q >> 6, mm = q & 63;
+      double theta = 0.25 * bq::kPI * c_xh[mm] + (h == 0 ? 0.25 * bq::kPI : 0.75 * bq::kPI);
+      double wt = 0.25 * bq::kPI * c_wh[mm], cth, sth;
+      sincos(theta, &sth, &cth);
+      const double csd = dir == 0 ? cth : sth, outer = dir == 0 ? sth : cth;
+      for (int i = 0; i < 12; ++i) {
+        double rr = 0.5 * kDEPS * (c_xl[i] + 1.0), wr = wt * 0.5 * kDEPS * c_wl[i];
+        double Kp[3], Km[3], Wp[3], Wm[3], Np[4], Nm[4];
+        double zp = rr * cth + zt, ep = rr * sth + et, zm = -rr * cth + zt, em = -rr * sth + et;
+        polar_K(e, zt, et, rr, cth, sth, Kp);
+        polar_K(e, zt, et, -rr, cth, sth, Km);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          Wp[d] = e.cw[d][0] + e.cw[d][1] * zp + e.cw[d][2] * ep;
+          Wm[d] = e.cw[d][0] + e.cw[d][1] * zm + e.cw[d][2] * em;
+        }
+        double aKp = sqrt(dot3(Kp, Kp)), aKm = sqrt(dot3(Km, Km));
+        double aWp = sqrt(dot3(Wp, Wp)), aWm = sqrt(dot3(Wm, Wm));
+        double KpT = dot3(Kp, T), KmT = dot3(Km, T);
+        cplx ccp = ccos(k * (aKp * rr)), ccm = ccos(k * (aKm * rr));
+        double ip5 = 1.0 / (aKp * aKp * aKp * aKp * aKp), im5 = 1.0 / (aKm * aKm * aKm * aKm * aKm);
+        double gp = KpT / (aKp * aKp * aKp) * aWp, gm = KmT / (aKm * aKm * aKm) * aWm;
+        double hp = (3.0 * KpT * csd - aKp * aKp) * ip5, hm = (3.0 * KmT * csd - aKm * aKm) * im5;
+        shapeN4(zp, ep, Np);
+        shapeN4(zm, em, Nm);
+        double ir = wr / rr;
+#pragma unroll
+        for (int n = 0; n < 4; ++n) {
+          cplx g = ccp * (gp * Np[n]) - ccm * (gm * Nm[n]);
+          cplx hh = ccp * (hp * Np[n]) - ccm * (hm * Nm[n]);
+          a[2 * n] -= ir * g.re;
+          a[2 * n + 1] -= ir * g.im;
+          a[8 + 2 * n] -= ir * outer * hh.re;
+          a[8 + 2 * n + 1] -= ir * outer * hh.im;
+        }
+        a[16] += ir * outer * (hp - hm);
+      }
+    }
+    block_sum256<17>(a, lds);
+    if (threadIdx.x == 0) {
+      double sg = -kI4PI * iaT;
+      double sh = plane ? 0.0 : bdC * kI4PI * iaT * sgn;
+      long dr = (2 * row + dir) * N + col;
+#pragma unroll
+      for (int n = 0; n < 4; ++n) {
+        dG[dr + n] = make_double2(a[2 * n] * sg, a[2 * n + 1] * sg);
+        dH[dr + n] = make_double2(a[8 + 2 * n] * sh, a[8 + 2 * n + 1] * sh);
+      }
+      F3[3 * (row * dNe + (t - 1)) + 1 + dir] = a[16] * sh;
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------
+__global__ void k_row_diag(int dNe, int open, const double* __restrict__ F3, double2* __restrict__ H,
+                           double* __restrict__ tdF)
+{
+  long row = (long)blockIdx.x * blockDim.x + threadIdx.x, N = 4L * dNe;
+  if (row >= N) return;
+  double F = 0, F0 = 0, F1 = 0;
+  const double* f = F3 + 3 * row * dNe;
+  for (int s = 0; s < dNe; ++s) {
+    F += f[3 * s];
+    F0 += f[3 * s + 1];
+    F1 += f[3 * s + 2];
+  }
+  double C = open ? 1.0 + F : F;
+  H[row * N + row].x += C;
+  if (tdF) {
+    tdF[3 * row] = C;
+    tdF[3 * row + 1] = F0;
+    tdF[3 * row + 2] = F1;
+  }
+}
+
+__global__ void k_collect_gh(long n, const unsigned char* __restrict__ flag, int* __restrict__ list,
+                             int* __restrict__ count)
+{
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || flag[i] != 1) return;
+  int slot = atomicAdd(count, 1);
+  list[slot] = (int)i;
+}
+
+// ------------------------------------------------------------------------------------
+extern "C" {
+
+int bq_set_quadrature(const double* xl, const double* wl, const double* xh, const double* wh)
+{
+  if (bq::set_quad_tables(xl, wl, xh, wh)) {
+    bq::set_error("quadrature table upload failed");
+    return -1;
+  }
+  return 0;
+}
+
+// Assemble domain matrices on the device.  All pointers are device pointers; G, H are
+// N x N, dG, dH are (2N) x N (rows 2*row + {0: zeta, 1: eta}), tdF is N x 3, work must
+// hold bq_assemble_work_bytes(dNe).  dG/dH/tdF may be null for the periodic domain.
+size_t bq_assemble_work_bytes(int dNe)
+{
+  size_t pairs = (size_t)4 * dNe * dNe;
+  return pairs * 3 * sizeof(double) + pairs + pairs * sizeof(int) + 256;
+}
+
+int bq_assemble_domain_dev(const bq_adom* a, void* work, void* stream)
+{
+  hipStream_t s = (hipStream_t)stream;
+  const int dNe = a->dNe;
+  const long N = 4L * dNe, pairs = N * dNe;
+  if (dNe <= 0) return 0;
+  char* w = (char*)work;
+  double* F3 = (double*)w;
+  unsigned char* flag = (unsigned char*)(w + pairs * 3 * sizeof(double));
+  int* list = (int*)(w + pairs * 3 * sizeof(double) + pairs);
+  int* count = (int*)(w + pairs * 3 * sizeof(double) + pairs + pairs * sizeof(int));
+  double2* G = (double2*)a->G;
+  double2* H = (double2*)a->H;
+  BQ_HIP(hipMemsetAsync(G, 0, sizeof(double2) * N * N, s));
+  BQ_HIP(hipMemsetAsync(H, 0, sizeof(double2) * N * N, s));
+  BQ_HIP(hipMemsetAsync(count, 0, sizeof(int), s));
+  const int freespace = a->periodic ? 0 : 1;
+  k_classify<<<(unsigned)((pairs + 255) / 256), 256, 0, s>>>(a->E, a->ren, a->sid, dNe, freespace, F3, flag);
+  BQ_HIP(hipGetLastError());
+  k_collect_gh<<<(unsigned)((pairs + 255) / 256), 256, 0, s>>>(pairs, flag, list, count);
+  BQ_HIP(hipGetLastError());
+  int ngh = 0;
+  BQ_HIP(hipMemcpyAsync(&ngh, count, sizeof(int), hipMemcpyDeviceToHost, s));
+  BQ_HIP(hipStreamSynchronize(s));
+  const unsigned gl_blocks = (unsigned)(((long)dNe * dNe + 3) / 4);
+  if (a->periodic) {
+    k_q0_gl<<<gl_blocks, 256, 0, s>>>(a->plan, a->E, a->ren, a->sid, dNe, flag, G, H);
+    BQ_HIP(hipGetLastError());
+    if (ngh) k_q0_gh<<<ngh, 256, 0, s>>>(a->plan, a->E, a->ren, a->sid, dNe, list, G, H, F3);
+    BQ_HIP(hipGetLastError());
+    k_q0_sing<<<(unsigned)N, 256, 0, s>>>(a->plan, a->E, a->sid, dNe, G, H, F3);
+    BQ_HIP(hipGetLastError());
+  } else {
+    double2* dG = (double2*)a->dG;
+    double2* dH = (double2*)a->dH;
+    BQ_HIP(hipMemsetAsync(dG, 0, sizeof(double2) * 2 * N * N, s));
+    BQ_HIP(hipMemsetAsync(dH, 0, sizeof(double2) * 2 * N * N, s));
+    k_fs_gl<<<gl_blocks, 256, 0, s>>>(a->kr, a->ki, a->E, a->ren, a->sid, dNe, flag, G, H, dG, dH);
+    BQ_HIP(hipGetLastError());
+    if (ngh) k_fs_gh<<<ngh, 256, 0, s>>>(a->kr, a->ki, a->E, a->ren, a->sid, dNe, list, G, H, dG, dH, F3);
+    BQ_HIP(hipGetLastError());
+    k_fs_sing<<<(unsigned)N, 256, 0, s>>>(a->kr, a->ki, a->E, a->sid, dNe, G, H, dG, dH, F3);
+    BQ_HIP(hipGetLastError());
+  }
+  k_row_diag<<<(unsigned)((N + 255) / 256), 256, 0, s>>>(dNe, a->open_domain, F3, H, a->tdF);
+  BQ_HIP(hipGetLastError());
+  if (a->n_gh_out) *a->n_gh_out = ngh;
+  return 0;
+}
+
+}  // extern "C"

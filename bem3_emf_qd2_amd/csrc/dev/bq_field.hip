@@ -1,0 +1,247 @@
+// bq_field.hip -- fields at arbitrary points from the boundary solution (EH_s of
+// d3qd2_src/d3qd2_field.c:203-269):
+//   E(r) = [sum_s sum_n G_n(r, s) dE_s,n - H_n(r, s) E_s,n] / C,  same for H,
+//   C = 1 + F (open region) or F, F = sum_s F(r, s)  (solid angle / 4 pi),
+// with the coefficients of q0_coef_rt / coef_rt (d3qd2_src/d3qd2_elem.c:148-185, :574-609):
+// type 0 = the 4 element nodes, 1 = 9-point rule, 2 = 12x12, 3 = 64x64 Gauss rule.
+// One 256-thread block per point; threads stride over (element, quadrature point) pairs
+// and the block reduces E, H, F in a fixed order.
+#include "bq_runtime.hpp"
+#include "bq_qpgf.hpp"
+#include "bq_elem.hpp"
+#include "bq_reduce.hpp"
+#include <cstring>
+#include <vector>
+#include "../bq_field.h"
+
+using namespace bq;
+
+namespace {
+
+__constant__ double c_z49[9], c_e49[9], c_w49[9];
+
+__device__ __forceinline__ cplx ld2(const double* p, long i) { return mk(p[2 * i], p[2 * i + 1]); }
+
+__device__ __forceinline__ int rule_size(int type) { return type == 0 ? 4 : type == 1 ? 9 : type == 2 ? 144 : 4096; }
+
+__device__ __forceinline__ void rule_point(int type, int q, double& z, double& e, double& w)
+{
+  if (type == 0) {
+    z = node_z(q);
+    e = node_e(q);
+    w = 1.0;
+  } else if (type == 1) {
+    z = c_z49[q];
+    e = c_e49[q];
+    w = c_w49[q];
+  } else if (type == 2) {
+    int i = q / 12, j = q - 12 * i;
+    z = c_xl[i];
+    e = c_xl[j];
+    w = c_wl[i] * c_wl[j];
+  } else {
+    int i = q >> 6, j = q & 63;
+    z = c_xh[i];
+    e = c_xh[j];
+    w = c_wh[i] * c_wh[j];
+  }
+}
+
+__global__ void __launch_bounds__(256) k_field(bq_fctx C, int npts, const double* __restrict__ pts,
+                                               const int* __restrict__ did_of, int type, double* __restrict__ out)
+{
+  __shared__ double lds[13 * 4];
+  const int p = blockIdx.x;
+  const int did = did_of[p];
+  const double* r0 = pts + 3 * p;
+  const int dNe = C.dNe[did];
+  const int* sid = C.sid[did];
+  const double *Eb = C.Eb[did], *dEb = C.dEb[did], *Hb = C.Hb[did], *dHb = C.dHb[did];
+  const bool periodic = (did == 0 && C.ps == 1);
+  const cplx k = mk(C.kn[2 * did], C.kn[2 * did + 1]);
+  const int nq = rule_size(type);
+  double acc[13];
+#pragma unroll
+  for (int i = 0; i < 13; ++i) acc[i] = 0.0;
+  const long total = (long)dNe * nq;
+  for (long it = threadIdx.x; it < total; it += 256) {
+    const int sc = (int)(it / nq), q = (int)(it - (long)sc * nq);
+    const int sd = sid[sc + 1], as = abs(sd);
+    Elem e;
+    load_elem(C.E, as, e);
+    double z, et, w, R[3], W[3], Nn[4];
+    rule_point(type, q, z, et, w);
+    if (type == 0) {
+      // the 4-point rule reads the stored node data (d3qd2_src/q0_bil_coef.c:28-30)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        R[d] = C.ren[12L * as + 3 * q + d];
+        W[d] = C.wen[12L * as + 3 * q + d];
+      }
+#pragma unroll
+      for (int n = 0; n < 4; ++n) Nn[n] = (n == q) ? 1.0 : 0.0;
+    } else {
+      rw_at(e, z, et, R, W);
+      shapeN4(z, et, Nn);
+    }
+    R[0] -= r0[0]; R[1] -= r0[1]; R[2] -= r0[2];
+    double aR = sqrt(dot3(R, R)), iR = 1.0 / aR, aW = sqrt(dot3(W, W));
+    double tD = dot3(R, W) * iR * iR * iR;
+    cplx g, h;
+    double sgn = 1.0;
+    if (periodic) {
+      QG qq;
+      qpgf_v1(C.plan, R[0], R[1], R[2], qq);
+      g = qq.g * aW;
+      h = qq.d[0] * W[0] + qq.d[1] * W[1] + qq.d[2] * W[2];
+    } else {
+      cplx ca = cmul_i(k * aR);
+      cplx ce = cexp(ca);
+      g = ce * (iR * aW * kI4PI);
+      h = (ca - mk(1.0, 0.0)) * ce * (tD * kI4PI);
+      if (sd < 0) sgn = -1.0;
+    }
+    acc[12] += w * tD * kI4PI * sgn;
+    const long base = 12L * as;  // unused for domain data indexing; kept for clarity
+    (void)base;
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+      cplx gn = g * (w * Nn[n]), hn = h * (w * Nn[n] * sgn);
+      long o = 12L * (sc + 1) + 3 * n;
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        cplx ev = gn * ld2(dEb, o + i) - hn * ld2(Eb, o + i);
+        cplx hv = gn * ld2(dHb, o + i) - hn * ld2(Hb, o + i);
+        acc[2 * i] += ev.re;
+        acc[2 * i + 1] += ev.im;
+        acc[6 + 2 * i] += hv.re;
+        acc[6 + 2 * i + 1] += hv.im;
+      }
+    }
+  }
+  block_sum256<13>(acc, lds);
+  if (threadIdx.x == 0) {
+    double* o = out + 13L * p;
+    for (int i = 0; i < 13; ++i) o[i] = acc[i];
+  }
+}
+
+}  // namespace
+
+extern "C" int bq_field_rules_init(const double* z49, const double* e49, const double* w49)
+{
+  BQ_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_z49), z49, 9 * sizeof(double)));
+  BQ_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_e49), e49, 9 * sizeof(double)));
+  BQ_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_w49), w49, 9 * sizeof(double)));
+  return 0;
+}
+
+// out[p][13] = E (3 complex), H (3 complex), F for each folded point
+extern "C" int bq_field_eval_dev(const bq_fctx* C, int npts, const double* d_pts, const int* d_did, int type,
+                                 double* d_out, void* stream)
+{
+  if (npts <= 0) return 0;
+  hipStream_t s = (hipStream_t)stream;
+  k_field<<<npts, 256, 0, s>>>(*C, npts, d_pts, d_did, type, d_out);
+  BQ_HIP(hipGetLastError());
+  return 0;
+}
+
+// ------------------------------------------------------------------------------------
+// context: the solved model resident in HBM for repeated field evaluation
+namespace {
+struct FieldCtx {
+  bq_fctx C;
+  std::vector<void*> owned;
+};
+
+template <class T>
+int up(FieldCtx* F, const T* h, size_t n, const T** out, hipStream_t s)
+{
+  void* p = nullptr;
+  if (n) {
+    BQ_HIP(hipMalloc(&p, n * sizeof(T)));
+    BQ_HIP(hipMemcpyAsync(p, h, n * sizeof(T), hipMemcpyHostToDevice, s));
+    F->owned.push_back(p);
+  }
+  *out = (const T*)p;
+  return 0;
+}
+}  // namespace
+
+extern "C" int bq_field_ctx_create(const bq_fdata* f, void** handle)
+{
+  if (bq::ensure_device()) return -1;
+  if (f->MN + 1 > BQ_MAXD) {
+    bq::set_error("field context: %d domains exceed the supported %d", f->MN + 1, BQ_MAXD);
+    return -2;
+  }
+  hipStream_t s = bq::stream();
+  FieldCtx* F = new FieldCtx();
+  bq_fctx& C = F->C;
+  memset(&C, 0, sizeof C);
+  C.ps = f->ps;
+  C.MN = f->MN;
+  if (bq_qplan_init(&C.plan, f->qd[0], f->qd + 1, f->qd + 4, f->qd + 6, 1.0e-15) && f->ps == 1) {
+    bq::set_error("field context: degenerate lattice");
+    delete F;
+    return -3;
+  }
+  size_t n1 = (size_t)f->Ne + 1;
+  if (up(F, f->E, n1 * 24, &C.E, s) || up(F, f->ren, n1 * 12, &C.ren, s) || up(F, f->wen, n1 * 12, &C.wen, s) ||
+      up(F, f->kn, (size_t)(f->MN + 1) * 2, &C.kn, s)) {
+    bq_field_ctx_destroy(F);
+    return -1;
+  }
+  for (int d = 0; d <= f->MN; ++d) {
+    size_t m = (size_t)f->dNe[d] + 1;
+    C.dNe[d] = f->dNe[d];
+    if (up(F, f->sid[d], m, &C.sid[d], s) || up(F, f->Eb[d], m * 24, &C.Eb[d], s) ||
+        up(F, f->dEb[d], m * 24, &C.dEb[d], s) || up(F, f->Hb[d], m * 24, &C.Hb[d], s) ||
+        up(F, f->dHb[d], m * 24, &C.dHb[d], s)) {
+      bq_field_ctx_destroy(F);
+      return -1;
+    }
+  }
+  if (bq::set_quad_tables(f->xli, f->wli, f->xhi, f->whi) || bq_field_rules_init(f->z49, f->e49, f->w49)) {
+    bq_field_ctx_destroy(F);
+    return -1;
+  }
+  BQ_HIP(hipStreamSynchronize(s));
+  *handle = F;
+  return 0;
+}
+
+extern "C" int bq_field_eval(void* handle, int n, const double* pts, const int* did, int type, double* out)
+{
+  if (n <= 0) return 0;
+  if (type < 0 || type > 3) {
+    bq::set_error("field: type %d not supported (0..3)", type);
+    return -2;
+  }
+  FieldCtx* F = (FieldCtx*)handle;
+  hipStream_t s = bq::stream();
+  double *dp = nullptr, *dout = nullptr;
+  int* dd = nullptr;
+  BQ_HIP(hipMallocAsync((void**)&dp, sizeof(double) * 3 * n, s));
+  BQ_HIP(hipMallocAsync((void**)&dd, sizeof(int) * n, s));
+  BQ_HIP(hipMallocAsync((void**)&dout, sizeof(double) * 13 * n, s));
+  BQ_HIP(hipMemcpyAsync(dp, pts, sizeof(double) * 3 * n, hipMemcpyHostToDevice, s));
+  BQ_HIP(hipMemcpyAsync(dd, did, sizeof(int) * n, hipMemcpyHostToDevice, s));
+  if (bq_field_eval_dev(&F->C, n, dp, dd, type, dout, s)) return -1;
+  BQ_HIP(hipMemcpyAsync(out, dout, sizeof(double) * 13 * n, hipMemcpyDeviceToHost, s));
+  BQ_HIP(hipFreeAsync(dp, s));
+  BQ_HIP(hipFreeAsync(dd, s));
+  BQ_HIP(hipFreeAsync(dout, s));
+  BQ_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+extern "C" void bq_field_ctx_destroy(void* handle)
+{
+  FieldCtx* F = (FieldCtx*)handle;
+  if (!F) return;
+  (void)hipStreamSynchronize(bq::stream());
+  for (void* p : F->owned) (void)hipFree(p);
+  delete F;
+}

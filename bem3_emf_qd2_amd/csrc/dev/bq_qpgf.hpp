@@ -84,13 +84,171 @@ __device__ void qpgf_fourier(const bq_qplan& P, double px, double py, double pz,
   o.d[2] = Sz * (-h * sg);
 }
 
-// Ewald split for the folded point.
-__device__ void qpgf_ewald(const bq_qplan& P, double px, double py, double pz, QG& o)
+// one Ewald spectral term for the reciprocal vector (ax, ay): F (value) and Gz (z part)
+__device__ __forceinline__ void spectral_term(const bq_qplan& P, double ax, double ay, double px, double py, double pz,
+                                              double ve, cplx& F, cplx& Gz)
 {
-  double r2 = px * px + py * py + pz * pz;
+  double b2 = P.k2 - ax * ax - ay * ay;
+  double az = 0.5 * pz / ve;
+  cplx ce = cexpi(ax * px + ay * py);
+  if (b2 > 0.0) {
+    double b = sqrt(b2), ef = exp(b2 * ve * ve - az * az);
+    cplx ff = cexpi(b * pz);
+    cplx w = faddeeva_w(b * ve, az);
+    // F = 2 ce / b (i ef Im w + ff);  Gz = 2 ce (ef Re w - ff)
+    F = ce * (mk(ff.re, ef * w.im + ff.im) * (2.0 / b));
+    Gz = ce * (mk(ef * w.re - ff.re, -ff.im) * 2.0);
+  } else {
+    double b = sqrt(-b2), bv = b * ve;
+    double a1 = bv + az, a2 = bv - az;
+    double e12 = exp(-bv * bv - az * az);
+    double f1 = (a1 > 0.0) ? e12 * erfcx(a1) : exp(b * pz) * erfc(a1);
+    double f2 = (a2 > 0.0) ? e12 * erfcx(a2) : exp(-b * pz) * erfc(a2);
+    // F = ce / (i b) (f1 + f2);  Gz = ce (f1 - f2)
+    F = mk(ce.im, -ce.re) * ((f1 + f2) / b);
+    Gz = ce * (f1 - f2);
+  }
+}
+
+// ---- exactly rounded building blocks for the r_z == 0 path -------------------------------
+// exp(t) rounded to nearest (barring results within 1e-30 relative of a rounding midpoint):
+// t = k ln2 + r with ln2 split in two doubles, then exp(r) by a 22-term Taylor-Horner
+// evaluation in double-double arithmetic (Dekker / Knuth two-sum and fma two-product).
+__device__ __forceinline__ void dd_two_sum(double a, double b, double& s, double& e)
+{
+#pragma clang fp contract(off)
+  s = a + b;
+  const double bb = s - a;
+  e = (a - (s - bb)) + (b - bb);
+}
+__device__ double exp_rn(double t)
+{
+#pragma clang fp contract(off)
+  if (!(fabs(t) < 700.0)) return exp(t);
+  const double L1 = 0x1.62e42fefa39efp-1, L2 = 0x1.abc9e3b39803fp-56;  // ln 2 = L1 + L2
+  const double k = rint(t * 0x1.71547652b82fep0);
+  const double ph = k * L1, pl = fma(k, L1, -ph);  // k L1 = ph + pl exactly
+  double rh, rl;
+  dd_two_sum(t - ph, -(pl + k * L2), rh, rl);      // t - ph is exact (Sterbenz)
+  double sh = 1.0, sl = 0.0;
+  for (int n = 22; n >= 1; --n) {
+    // (sh, sl) <- 1 + r (sh, sl) / n
+    double ph2 = rh * sh, pe = fma(rh, sh, -ph2);
+    pe += rh * sl + rl * sh;
+    double mh, ml;
+    dd_two_sum(ph2, pe, mh, ml);
+    const double q1 = mh / n;
+    const double q2 = (fma(-q1, (double)n, mh) + ml) / n;
+    double th, tl;
+    dd_two_sum(1.0, q1, th, tl);
+    tl += q2;
+    dd_two_sum(th, tl, sh, sl);
+  }
+  return ldexp(sh, (int)k);
+}
+
+// Folding of r into the unit cell and the per-point Ewald parameter, with the reference's
+// operation order and no contraction (d3_qpgf_d2.c:417-423, :1680-1693), so the r_z == 0
+// emulation below sees bit-identical inputs.
+__device__ __forceinline__ void fold_cell(const bq_qplan& P, double rx, double ry, double& l1, double& l2,
+                                          double& px, double& py)
+{
+#pragma clang fp contract(off)
+  l1 = floor((rx * P.d2y - ry * P.d2x) * P.idd + 0.5);
+  l2 = floor(-(rx * P.d1y - ry * P.d1x) * P.idd + 0.5);
+  px = rx - l1 * P.d1x - l2 * P.d2x;
+  py = ry - l1 * P.d1y - l2 * P.d2y;
+}
+__device__ __forceinline__ double ewald_param(const bq_qplan& P, double px, double py, double pz)
+{
+#pragma clang fp contract(off)
+  const double r2 = px * px + py * py + pz * pz;
   double ve = P.veps0;
   if (P.k2 * ve * ve - r2 / (4.0 * ve * ve) > BQ_EW_EXPM)
     ve = sqrt(BQ_EW_EXPM + sqrt(P.k2 * r2 + BQ_EW_EXPM * BQ_EW_EXPM)) / (1.4142135623730950488 * P.k);
+  return ve;
+}
+
+// One Ewald spectral term at z = 0, evaluated the way the reference does it
+// (d3_qpgf_d2.c:1965-2009): products rounded before the subtraction (no fused multiply-add),
+// so that dqG_z of a propagating order, 2 (e^{b^2 v^2} Re w(b v) - 1), is the same rounding
+// residual (often exactly zero) that the reference's stop test sees.  exp is rounded to
+// nearest (exp_rn) as a correctly rounding libm does; a libm off by an ulp at these
+// arguments would make the reference itself stop elsewhere.
+__device__ __forceinline__ void spectral_term_z0(const bq_qplan& P, double ax, double ay, double px, double py,
+                                                 double ve, bool prop, cplx& F, cplx& Gz)
+{
+#pragma clang fp contract(off)
+  const double b2 = P.k2 - ax * ax - ay * ay;
+  const cplx ce = cexpi(ax * px + ay * py);
+  if (prop) {
+    const double b = sqrt(b2), ef = exp_rn(b2 * ve * ve);
+    const double x = b * ve;
+    const double wre = exp_rn(-(x * x)), wim = faddeeva_w(x, 0.0).im;
+    const double pr = ef * wre;
+    F = ce * (mk(1.0, ef * wim) * (2.0 / b));
+    Gz = ce * mk(2.0 * (pr - 1.0), 0.0);
+  } else {
+    const double b = sqrt(-b2), f = erfc(b * ve);
+    F = mk(ce.im, -ce.re) * ((f + f) / b);
+    Gz = mk(0.0, 0.0);
+  }
+}
+
+// Spectral sum over exactly the terms the reference's _v1/_dqG variants add when r_z == 0
+// (qgf2_src/d3_qpgf_d2.c:1840-1963): their stop test compares dqG_z, which is identically
+// zero there for evanescent orders (and zero up to rounding for propagating ones), so each
+// row stops at its first evanescent term past the centre (both directions), and the rows
+// stop after the first row beyond the centre whose non-centre terms are all evanescent and
+// whose dqG_z partial sum compares equal to the one before the row.  Kept for result parity
+// with the reference.
+__device__ void spectral_z0_ref(const bq_qplan& P, double px, double py, double ve, QG& o)
+{
+#pragma clang fp contract(off)
+  const double g22 = P.g2x * P.g2x + P.g2y * P.g2y;
+  const int nc = (int)round((P.vk0 * P.d1x + P.vk1 * P.d1y) / (2.0 * kPI));
+  cplx S = mk(0, 0), Sx = mk(0, 0), Sy = mk(0, 0), Sz = mk(0, 0);
+  for (int dir = 0; dir < 3; ++dir) {
+    int n1 = (dir == 0) ? nc : (dir == 1 ? nc + 1 : nc - 1);
+    const int step = (dir == 2) ? -1 : 1;
+    for (int it = 0; it < 99; ++it, n1 += step) {
+      const cplx Sz0 = Sz;
+      const double ux = -P.vk0 + n1 * P.g1x, uy = -P.vk1 + n1 * P.g1y;
+      const int n2c = (int)round(-(ux * P.g2x + uy * P.g2y) / g22);
+      int nprop = 0;
+      for (int side = 0; side < 3; ++side) {
+        int n2 = (side == 0) ? n2c : (side == 1 ? n2c + 1 : n2c - 1);
+        const int s2 = (side == 2) ? -1 : 1;
+        for (int m = 0; m < 99; ++m, n2 += s2) {
+          const double ax = ux + n2 * P.g2x, ay = uy + n2 * P.g2y;
+          const bool prop = P.k2 - ax * ax - ay * ay > 0.0;
+          cplx F, Gz;
+          spectral_term_z0(P, ax, ay, px, py, ve, prop, F, Gz);
+          S += F;
+          Sx += F * ax;
+          Sy += F * ay;
+          Sz += Gz;
+          if (side == 0) break;
+          if (!prop) break;  // first evanescent term ends the side
+          ++nprop;
+        }
+      }
+      if (dir == 0) break;
+      if (nprop == 0 && 2.0 * fabs(Sz.re - Sz0.re) <= P.eps * (fabs(Sz.re) + fabs(Sz0.re)) &&
+          2.0 * fabs(Sz.im - Sz0.im) <= P.eps * (fabs(Sz.im) + fabs(Sz0.im)))
+        break;
+    }
+  }
+  const double cd = 0.25 * P.aidd;
+  o.g += cmul_i(S) * cd;
+  o.d[0] += Sx * (-cd);
+  o.d[1] += Sy * (-cd);
+}
+
+// Ewald split for the folded point.
+__device__ void qpgf_ewald(const bq_qplan& P, double px, double py, double pz, QG& o, bool z0_ref)
+{
+  const double ve = ewald_param(P, px, py, pz);
   const double ive = 1.0 / ve, ke = P.k * ve, ke2 = ke * ke;
 
   // ---- spatial sum: |r + R_l| <= 2 ve sqrt(ke^2 + cut)
@@ -125,7 +283,12 @@ __device__ void qpgf_ewald(const bq_qplan& P, double px, double py, double pz, Q
     o.d[1] = Sxy[1] * (-kI4PI);
     o.d[2] = Sz * (-pz * kI4PI);
   }
-  // ---- spectral sum: |beta| ve <= sqrt(cut)
+  // ---- spectral sum
+  if (pz == 0.0 && z0_ref) {
+    spectral_z0_ref(P, px, py, ve, o);
+    return;
+  }
+  // |beta| ve <= sqrt(cut)
   {
     double A2 = P.k2 + BQ_EXP_CUT * ive * ive;
     double A = sqrt(A2) * 1.0000001;
@@ -134,34 +297,15 @@ __device__ void qpgf_ewald(const bq_qplan& P, double px, double py, double pz, Q
     double c1 = (P.vk0 * P.d1x + P.vk1 * P.d1y) / (2.0 * kPI), w1 = A * d1n / (2.0 * kPI);
     int n1lo = (int)ceil(c1 - w1), n1hi = (int)floor(c1 + w1);
     double g22 = P.g2x * P.g2x + P.g2y * P.g2y;
-    double az = 0.5 * pz * ive;
     cplx S = mk(0, 0), Sx = mk(0, 0), Sy = mk(0, 0), Sz = mk(0, 0);
     for (int n1 = n1lo; n1 <= n1hi; ++n1) {
       double ux = -P.vk0 + n1 * P.g1x, uy = -P.vk1 + n1 * P.g1y;
       int lo, hi;
       if (!row_range(ux, uy, P.g2x, P.g2y, g22, A2, lo, hi)) continue;
       for (int n2 = lo; n2 <= hi; ++n2) {
-        double ax = ux + n2 * P.g2x, ay = uy + n2 * P.g2y;
-        double b2 = P.k2 - ax * ax - ay * ay;
-        cplx ce = cexpi(ax * px + ay * py);
         cplx F, Gz;
-        if (b2 > 0.0) {
-          double b = sqrt(b2), ef = exp(b2 * ve * ve - az * az);
-          cplx ff = cexpi(b * pz);
-          cplx w = faddeeva_w(b * ve, az);
-          // F = 2 ce / b (i ef Im w + ff);  Gz = 2 ce (ef Re w - ff)
-          F = ce * (mk(ff.re, ef * w.im + ff.im) * (2.0 / b));
-          Gz = ce * (mk(ef * w.re - ff.re, -ff.im) * 2.0);
-        } else {
-          double b = sqrt(-b2), bv = b * ve;
-          double a1 = bv + az, a2 = bv - az;
-          double e12 = exp(-bv * bv - az * az);
-          double f1 = (a1 > 0.0) ? e12 * erfcx(a1) : exp(b * pz) * erfc(a1);
-          double f2 = (a2 > 0.0) ? e12 * erfcx(a2) : exp(-b * pz) * erfc(a2);
-          // F = ce / (i b) (f1 + f2);  Gz = ce (f1 - f2)
-          F = mk(ce.im, -ce.re) * ((f1 + f2) / b);
-          Gz = ce * (f1 - f2);
-        }
+        double ax = ux + n2 * P.g2x, ay = uy + n2 * P.g2y;
+        spectral_term(P, ax, ay, px, py, pz, ve, F, Gz);
         S += F;
         Sx += F * ax;
         Sy += F * ay;
@@ -176,19 +320,21 @@ __device__ void qpgf_ewald(const bq_qplan& P, double px, double py, double pz, Q
   }
 }
 
-// Value and gradient at r (relative position, source minus field point).
-__device__ void qpgf_v1(const bq_qplan& P, double rx, double ry, double rz, QG& o)
+// Value and gradient at r (relative position, source minus field point).  z0_ref = true
+// reproduces the reference's _v1 result at r_z == 0 (see spectral_z0_ref); false gives the
+// fully converged sum there (what the reference's _qG variant returns for the value).
+__device__ void qpgf_v1(const bq_qplan& P, double rx, double ry, double rz, QG& o, bool z0_ref = true)
 {
-  double l1 = floor((rx * P.d2y - ry * P.d2x) * P.idd + 0.5);
-  double l2 = floor(-(rx * P.d1y - ry * P.d1x) * P.idd + 0.5);
-  double px = rx - l1 * P.d1x - l2 * P.d2x, py = ry - l1 * P.d1y - l2 * P.d2y, pz = rz;
+  double l1, l2, px, py;
+  fold_cell(P, rx, ry, l1, l2, px, py);
+  const double pz = rz;
   bool fd = false;
   if (P.fd_allowed) {
     double az = fabs(rz);
     fd = (P.fd_bN / P.fd_bNp * exp(-P.fd_bNp * az) < P.eps) && (P.fd_bN / P.fd_bNm * exp(-P.fd_bNm * az) < P.eps);
   }
   if (fd) qpgf_fourier(P, px, py, pz, o);
-  else qpgf_ewald(P, px, py, pz, o);
+  else qpgf_ewald(P, px, py, pz, o, z0_ref);
   if (l1 != 0.0 || l2 != 0.0) {
     cplx cf = cexpi(-(P.vk0 * (l1 * P.d1x + l2 * P.d2x) + P.vk1 * (l1 * P.d1y + l2 * P.d2y)));
     o.g = o.g * cf;

@@ -74,12 +74,13 @@ __global__ void __launch_bounds__(256) k_faddeeva(int n, const double* __restric
   w[2 * i + 1] = v.im;
 }
 
-__global__ void __launch_bounds__(256) k_qpgf_v1(bq_qplan P, int n, const double* __restrict__ r, double* __restrict__ out)
+__global__ void __launch_bounds__(256) k_qpgf_v1(bq_qplan P, int z0_ref, int n, const double* __restrict__ r,
+                                                 double* __restrict__ out)
 {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   bq::QG q;
-  bq::qpgf_v1(P, r[3 * i], r[3 * i + 1], r[3 * i + 2], q);
+  bq::qpgf_v1(P, r[3 * i], r[3 * i + 1], r[3 * i + 2], q, z0_ref != 0);
   double* o = out + 8 * i;
   o[0] = q.g.re; o[1] = q.g.im;
   for (int c = 0; c < 3; ++c) { o[2 + 2 * c] = q.d[c].re; o[3 + 2 * c] = q.d[c].im; }
@@ -117,19 +118,22 @@ int bq_qpgf_plan(const double* qd8, double eps, bq_qplan* plan)
   return bq_qplan_init(plan, qd8[0], qd8 + 1, qd8 + 4, qd8 + 6, eps);
 }
 
-int bq_qpgf_v1_dev(const bq_qplan* plan, int n, const double* d_r, double* d_out, hipStream_t s)
+int bq_qpgf_v1_dev(const bq_qplan* plan, int variant, int n, const double* d_r, double* d_out, hipStream_t s)
 {
   if (n <= 0) return 0;
-  k_qpgf_v1<<<(n + 255) / 256, 256, 0, s>>>(*plan, n, d_r, d_out);
+  k_qpgf_v1<<<(n + 255) / 256, 256, 0, s>>>(*plan, variant == 0, n, d_r, d_out);
   BQ_HIP(hipGetLastError());
   return 0;
 }
 
-int bq_qpgf_v1_batch(const double* qd8, int n, const double* r, double* out)
+int bq_qpgf_batch(const double* qd8, int variant, int n, const double* r, double* out)
 {
-  if (n < 0 || !qd8 || (n > 0 && (!r || !out))) { bq::set_error("bq_qpgf_v1_batch: bad arguments"); return -2; }
+  if (n < 0 || !qd8 || (variant != 0 && variant != 1) || (n > 0 && (!r || !out))) {
+    bq::set_error("bq_qpgf_batch: bad arguments");
+    return -2;
+  }
   bq_qplan P;
-  if (bq_qpgf_plan(qd8, 1e-15, &P)) { bq::set_error("bq_qpgf_v1_batch: degenerate lattice"); return -3; }
+  if (bq_qpgf_plan(qd8, 1e-15, &P)) { bq::set_error("bq_qpgf_batch: degenerate lattice"); return -3; }
   if (n == 0) return 0;
   if (bq::ensure_device()) return -1;
   hipStream_t s = bq::stream();
@@ -137,12 +141,17 @@ int bq_qpgf_v1_batch(const double* qd8, int n, const double* r, double* out)
   BQ_HIP(hipMallocAsync((void**)&dr, sizeof(double) * 3 * (size_t)n, s));
   BQ_HIP(hipMallocAsync((void**)&dout, sizeof(double) * 8 * (size_t)n, s));
   BQ_HIP(hipMemcpyAsync(dr, r, sizeof(double) * 3 * (size_t)n, hipMemcpyHostToDevice, s));
-  if (bq_qpgf_v1_dev(&P, n, dr, dout, s)) return -1;
+  if (bq_qpgf_v1_dev(&P, variant, n, dr, dout, s)) return -1;
   BQ_HIP(hipMemcpyAsync(out, dout, sizeof(double) * 8 * (size_t)n, hipMemcpyDeviceToHost, s));
   BQ_HIP(hipFreeAsync(dr, s));
   BQ_HIP(hipFreeAsync(dout, s));
   BQ_HIP(hipStreamSynchronize(s));
   return 0;
+}
+
+int bq_qpgf_v1_batch(const double* qd8, int n, const double* r, double* out)
+{
+  return bq_qpgf_batch(qd8, 0, n, r, out);
 }
 
 }  // extern "C"

@@ -1,0 +1,486 @@
+// bq_solver.hip -- device solver session behind solve_bieq.
+//
+// Global system.  The reference assembles one sparse 8NN x 8NN system (unknowns X_0..X_3 =
+// A_x, A_y, A_z, phi and X_4..X_7 = their normal derivatives) and solves it with PARDISO
+// (d3qd2_src/d3qd2_solve_bieq.c:258-454).  Its rows have a fixed block structure: with
+// Y_c = (X_c, X_{c+4}) for c = 0..3 and the rows of component c stacked in domain order,
+//   M Y_c + K diag(n_c) X_3 = b_c          (c = 0, 1, 2)
+//   sum_c L diag(n_c) X_c + M3 Y_3 = b_3
+// where M is [H | -G] on rows of an element's main domain and [H | G] on its sub-domain
+// rows, K = (k1^2 - k2^2) G and L = (1 - k1^2/k2^2) G on sub-domain rows (zero elsewhere),
+// M3 = [H | -G] / [H | (k1/k2)^2 G], and n_c is the c-th component of the unit normal at
+// each element node (create_matrix_csr_dac, :284-421).  Block elimination gives
+//   Z = M^{-1} K,  y_c = M^{-1} b_c,
+//   S = M3 - [ L (Z_top o Nn) | 0 ],  Nn_ij = n_i . n_j,
+//   S Y_3 = b_3 - L (sum_c n_c o y_c,top),   Y_c = y_c - Z (n_c o X_3),
+// two dense LUs of order 2NN instead of one of order 8NN (64x fewer LU flops), all
+// on the device.  The result is the same linear system's solution; only the elimination
+// order differs from a direct factorisation.
+#include <vector>
+#include <hip/hip_runtime.h>
+#include "bq_runtime.hpp"
+#include "bq_complex.hpp"
+#include "../bq_solver.h"
+#include "../bq_assemble.h"
+#include "../bq_linalg.h"
+#include "../bq_plan.h"
+
+struct bq_sess {
+  bq_model m;
+  int Ne, NN, MN, ps;
+  hipStream_t s;
+  double *E = nullptr, *ren = nullptr, *nrm = nullptr, *Ui = nullptr, *dUi = nullptr, *kn = nullptr;
+  int *eni = nullptr, *md = nullptr, *sd = nullptr;
+  std::vector<int> dNe;
+  std::vector<std::vector<int>> sid_h;
+  std::vector<int*> sid;
+  std::vector<double*> G, H, dG, dH, tdF;
+  std::vector<double> kn_h;
+  bq_qplan plan;
+  bq_stats st{};
+  int two_nn = 0;
+};
+
+namespace {
+
+template <class T>
+int dmalloc(T** p, size_t n)
+{
+  if (n == 0) {
+    *p = nullptr;
+    return 0;
+  }
+  BQ_HIP(hipMalloc((void**)p, n * sizeof(T)));
+  return 0;
+}
+
+template <class T>
+int upload(T** p, const T* h, size_t n, hipStream_t s)
+{
+  if (dmalloc(p, n)) return -1;
+  if (n) BQ_HIP(hipMemcpyAsync(*p, h, n * sizeof(T), hipMemcpyHostToDevice, s));
+  return 0;
+}
+
+__device__ __forceinline__ cplx ld2(const double* p, long i) { return mk(p[2 * i], p[2 * i + 1]); }
+__device__ __forceinline__ void st2(double* p, long i, cplx v)
+{
+  p[2 * i] = v.re;
+  p[2 * i + 1] = v.im;
+}
+
+// one thread per (element s of the domain, row of the domain): entries of M, M3, K, L
+__global__ void k_build_blocks(int dNe, int d, int rowbase, int NN, const int* __restrict__ sid,
+                               const int* __restrict__ eni, const int* __restrict__ md, const int* __restrict__ sdm,
+                               const double* __restrict__ kn, const double* __restrict__ G, const double* __restrict__ H,
+                               double* __restrict__ M, double* __restrict__ M3, double* __restrict__ K,
+                               double* __restrict__ L)
+{
+  const long N = 4L * dNe;
+  long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= N * dNe) return;
+  const int sc = (int)(idx / N);
+  const long rl = idx % N;
+  const long r = rowbase + rl, ld = 2L * NN;
+  const int sd = sid[sc + 1], asd = abs(sd), mdid = md[asd], sdid = sdm[asd];
+  const bool main_side = (d == mdid);
+  cplx beps = mk(1, 0), delt = mk(0, 0), sigm = mk(0, 0);
+  if (!main_side) {
+    cplx k1 = ld2(kn, mdid), k2 = ld2(kn, sdid);
+    cplx k12 = k1 * k1, k22 = k2 * k2;
+    delt = k12 - k22;
+    beps = cdiv(k12, k22);
+    sigm = mk(1.0, 0.0) - beps;
+  }
+  for (int l = 0; l < 4; ++l) {
+    const long e = eni[4 * asd + l];
+    if (e < 0) continue;
+    cplx g = ld2(G, rl * N + 4L * sc + l), h = ld2(H, rl * N + 4L * sc + l);
+    st2(M, e * ld + r, h);
+    st2(M3, e * ld + r, h);
+    if (main_side) {
+      st2(M, (NN + e) * ld + r, -g);
+      st2(M3, (NN + e) * ld + r, -g);
+    } else {
+      st2(M, (NN + e) * ld + r, g);
+      st2(M3, (NN + e) * ld + r, beps * g);
+      st2(K, e * ld + r, delt * g);
+      st2(L, e * ld + r, sigm * g);
+    }
+  }
+}
+
+// right-hand sides: b_c (c < 3) into columns NN..NN+2 of R, b_3 into b3
+__global__ void k_build_rhs(int dNe, int d, int rowbase, int NN, const int* __restrict__ sid,
+                            const int* __restrict__ md, const int* __restrict__ sdm, const double* __restrict__ kn,
+                            const double* __restrict__ nrm, const double* __restrict__ Ui,
+                            const double* __restrict__ dUi, const double* __restrict__ G,
+                            const double* __restrict__ H, double* __restrict__ R, double* __restrict__ b3)
+{
+  const long N = 4L * dNe;
+  long rl = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (rl >= N) return;
+  const long r = rowbase + rl, ld = 2L * NN;
+  cplx b[4] = {mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0)};
+  for (int sc = 0; sc < dNe; ++sc) {
+    const int sd = sid[sc + 1], asd = abs(sd), mdid = md[asd], sdid = sdm[asd];
+    if (d == mdid || mdid != 0) continue;
+    cplx k1 = ld2(kn, mdid), k2 = ld2(kn, sdid);
+    cplx sigm = mk(1.0, 0.0) - cdiv(k1 * k1, k2 * k2);
+    for (int l = 0; l < 4; ++l) {
+      cplx g = ld2(G, rl * N + 4L * sc + l), h = ld2(H, rl * N + 4L * sc + l);
+      const double* vn = nrm + 12L * asd + 3 * l;
+      cplx ndu = mk(0, 0);
+      for (int c = 0; c < 3; ++c) {
+        cplx ui = ld2(Ui, 16L * asd + 4 * l + c), dui = ld2(dUi, 16L * asd + 4 * l + c);
+        b[c] = b[c] - h * ui - g * dui;
+        ndu += ui * vn[c];
+      }
+      b[3] = b[3] - sigm * g * ndu;
+    }
+  }
+  for (int c = 0; c < 3; ++c) st2(R, (long)(NN + c) * ld + r, b[c]);
+  st2(b3, r, b[3]);
+}
+
+// W = Z_top o (n_i . n_j), NN x NN column-major; Z has leading dimension 2NN
+__global__ void k_hadamard_normals(int NN, const double* __restrict__ Z, const double* __restrict__ nn3,
+                                   double* __restrict__ W)
+{
+  long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)NN * NN) return;
+  long j = idx / NN, i = idx % NN;
+  double g = nn3[3 * i] * nn3[3 * j] + nn3[3 * i + 1] * nn3[3 * j + 1] + nn3[3 * i + 2] * nn3[3 * j + 2];
+  st2(W, idx, ld2(Z, j * 2L * NN + i) * g);
+}
+
+// v[e] = sum_c n_c(e) y_c[e] (top half of y_c = column NN + c of R)
+__global__ void k_normal_dot(int NN, const double* __restrict__ R, const double* __restrict__ nn3, double* __restrict__ v)
+{
+  long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= NN) return;
+  const long ld = 2L * NN;
+  cplx a = mk(0, 0);
+  for (int c = 0; c < 3; ++c) a += ld2(R, (long)(NN + c) * ld + e) * nn3[3 * e + c];
+  st2(v, e, a);
+}
+
+// T[e][c] = n_c(e) X3[e], NN x 3 column-major
+__global__ void k_scale_normals(int NN, const double* __restrict__ Y3, const double* __restrict__ nn3,
+                                double* __restrict__ T)
+{
+  long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= NN) return;
+  cplx x = ld2(Y3, e);
+  for (int c = 0; c < 3; ++c) st2(T, (long)c * NN + e, x * nn3[3 * e + c]);
+}
+
+__global__ void k_transpose(long n, const double* __restrict__ A, double* __restrict__ B)
+{
+  long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n * n) return;
+  long r = idx / n, c = idx % n;
+  st2(B, c * n + r, ld2(A, idx));
+}
+
+inline unsigned nblk(long n, int b) { return (unsigned)((n + b - 1) / b); }
+
+}  // namespace
+
+extern "C" {
+
+int bq_set_quadrature(const double* xl, const double* wl, const double* xh, const double* wh);
+
+bq_sess* bq_sess_create(const bq_model* m)
+{
+  if (bq::ensure_device()) return nullptr;
+  bq_sess* S = new bq_sess();
+  S->m = *m;
+  S->Ne = m->Ne;
+  S->NN = m->NN;
+  S->MN = m->MN;
+  S->ps = m->ps;
+  S->s = bq::stream();
+  hipStream_t s = S->s;
+  const size_t ne1 = (size_t)m->Ne + 1;
+  if (upload(&S->E, m->E, ne1 * 24, s) || upload(&S->ren, m->ren, ne1 * 12, s) ||
+      upload(&S->nrm, m->nrm, ne1 * 12, s) || upload(&S->Ui, m->Ui, ne1 * 32, s) ||
+      upload(&S->dUi, m->dUi, ne1 * 32, s) || upload(&S->kn, m->kn, (size_t)(m->MN + 1) * 2, s) ||
+      upload(&S->eni, m->eni, ne1 * 4, s) || upload(&S->md, m->md, ne1, s) || upload(&S->sd, m->sd, ne1, s)) {
+    bq_sess_destroy(S);
+    return nullptr;
+  }
+  S->kn_h.assign(m->kn, m->kn + 2 * (m->MN + 1));
+  for (int d = 0; d <= m->MN; ++d) {
+    S->dNe.push_back(m->dNe[d]);
+    S->sid_h.emplace_back(m->sid[d], m->sid[d] + m->dNe[d] + 1);
+    int* p = nullptr;
+    if (upload(&p, m->sid[d], (size_t)m->dNe[d] + 1, s)) {
+      bq_sess_destroy(S);
+      return nullptr;
+    }
+    S->sid.push_back(p);
+    S->two_nn += 4 * m->dNe[d];
+  }
+  S->G.assign(m->MN + 1, nullptr);
+  S->H.assign(m->MN + 1, nullptr);
+  S->dG.assign(m->MN + 1, nullptr);
+  S->dH.assign(m->MN + 1, nullptr);
+  S->tdF.assign(m->MN + 1, nullptr);
+  if (bq_qplan_init(&S->plan, m->qd[0], m->qd + 1, m->qd + 4, m->qd + 6, 1.0e-15) && m->ps == 1) {
+    bq::set_error("degenerate lattice");
+    bq_sess_destroy(S);
+    return nullptr;
+  }
+  // Gauss-Legendre tables as the host computed them for the model
+  if (bq_set_quadrature(m->xli, m->wli, m->xhi, m->whi)) {
+    bq_sess_destroy(S);
+    return nullptr;
+  }
+  return S;
+}
+
+void bq_sess_destroy(bq_sess* S)
+{
+  if (!S) return;
+  (void)hipStreamSynchronize(S->s);
+  auto f = [](void* p) { if (p) (void)hipFree(p); };
+  f(S->E); f(S->ren); f(S->nrm); f(S->Ui); f(S->dUi); f(S->kn); f(S->eni); f(S->md); f(S->sd);
+  for (auto p : S->sid) f(p);
+  for (auto p : S->G) f(p);
+  for (auto p : S->H) f(p);
+  for (auto p : S->dG) f(p);
+  for (auto p : S->dH) f(p);
+  for (auto p : S->tdF) f(p);
+  delete S;
+}
+
+const bq_stats* bq_sess_stats(const bq_sess* S) { return &S->st; }
+
+int bq_sess_assemble(bq_sess* S)
+{
+  hipStream_t s = S->s;
+  hipEvent_t e0, e1;
+  BQ_HIP(hipEventCreate(&e0));
+  BQ_HIP(hipEventCreate(&e1));
+  BQ_HIP(hipEventRecord(e0, s));
+  size_t wmax = 0;
+  for (int d = 0; d <= S->MN; ++d) wmax = std::max(wmax, bq_assemble_work_bytes(S->dNe[d]));
+  void* work = nullptr;
+  BQ_HIP(hipMalloc(&work, wmax));
+  S->st.n_gh_pairs = 0;
+  S->st.n_qpgf = 0;
+  for (int d = 0; d <= S->MN; ++d) {
+    const long dNe = S->dNe[d], N = 4 * dNe;
+    if (dNe == 0) continue;
+    const bool periodic = (d == 0 && S->ps == 1);
+    if (dmalloc(&S->G[d], (size_t)(2 * N * N)) || dmalloc(&S->H[d], (size_t)(2 * N * N))) return -1;
+    if (!periodic) {
+      if (dmalloc(&S->dG[d], (size_t)(4 * N * N)) || dmalloc(&S->dH[d], (size_t)(4 * N * N)) ||
+          dmalloc(&S->tdF[d], (size_t)(3 * N)))
+        return -1;
+    }
+    bq_adom a{};
+    a.dNe = (int)dNe;
+    a.periodic = periodic ? 1 : 0;
+    a.open_domain = (d == 0) ? 1 : 0;
+    a.kr = S->kn_h[2 * d];
+    a.ki = S->kn_h[2 * d + 1];
+    a.plan = S->plan;
+    a.E = S->E;
+    a.ren = S->ren;
+    a.sid = S->sid[d];
+    a.G = S->G[d];
+    a.H = S->H[d];
+    a.dG = S->dG[d];
+    a.dH = S->dH[d];
+    a.tdF = S->tdF[d];
+    int ngh = 0;
+    a.n_gh_out = &ngh;
+    if (bq_assemble_domain_dev(&a, work, s)) {
+      (void)hipFree(work);
+      return -1;
+    }
+    S->st.n_gh_pairs += ngh;
+    if (periodic) S->st.n_qpgf += (N * dNe - N - ngh) * 144LL + ngh * 4096LL + N * 6144LL;
+  }
+  BQ_HIP(hipEventRecord(e1, s));
+  BQ_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  BQ_HIP(hipEventElapsedTime(&ms, e0, e1));
+  S->st.t_assemble_ms = ms;
+  (void)hipFree(work);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return 0;
+}
+
+int bq_sess_solve_global(bq_sess* S, double* Xh)
+{
+  hipStream_t s = S->s;
+  const int NN = S->NN, n2 = 2 * NN;
+  if (S->two_nn != n2) {
+    bq::set_error("solve_global: element-node count mismatch (%d vs %d); linear triangles need row compaction",
+                  S->two_nn, n2);
+    return -2;
+  }
+  hipEvent_t e0, e1;
+  BQ_HIP(hipEventCreate(&e0));
+  BQ_HIP(hipEventCreate(&e1));
+  BQ_HIP(hipEventRecord(e0, s));
+  const size_t sq = (size_t)n2 * n2 * 2, rect = (size_t)n2 * NN * 2;
+  double *M, *M3, *R, *L, *b3, *W, *v, *T;
+  int *ip1, *ip2;
+  double* nn3 = nullptr;
+  if (dmalloc(&M, sq) || dmalloc(&M3, sq) || dmalloc(&R, (size_t)n2 * (NN + 3) * 2) || dmalloc(&L, rect) ||
+      dmalloc(&b3, (size_t)n2 * 2) || dmalloc(&W, (size_t)NN * NN * 2) || dmalloc(&v, (size_t)NN * 2) ||
+      dmalloc(&T, (size_t)NN * 6) || dmalloc(&ip1, (size_t)n2) || dmalloc(&ip2, (size_t)n2) ||
+      dmalloc(&nn3, (size_t)NN * 3))
+    return -1;
+  // unit normals per element node e (main side)
+  {
+    std::vector<double> h((size_t)NN * 3, 0.0);
+    for (int i = 1; i <= S->Ne; ++i)
+      for (int l = 0; l < 4; ++l) {
+        int e = S->m.eni[4 * i + l];
+        if (e < 0) continue;
+        for (int c = 0; c < 3; ++c) h[3 * (size_t)e + c] = S->m.nrm[12 * (size_t)i + 3 * l + c];
+      }
+    BQ_HIP(hipMemcpyAsync(nn3, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    BQ_HIP(hipStreamSynchronize(s));
+  }
+  BQ_HIP(hipMemsetAsync(M, 0, sq * 8, s));
+  BQ_HIP(hipMemsetAsync(M3, 0, sq * 8, s));
+  BQ_HIP(hipMemsetAsync(R, 0, (size_t)n2 * (NN + 3) * 16, s));
+  BQ_HIP(hipMemsetAsync(L, 0, rect * 8, s));
+  int rowbase = 0;
+  for (int d = 0; d <= S->MN; ++d) {
+    const long dNe = S->dNe[d], N = 4 * dNe;
+    if (dNe == 0) continue;
+    k_build_blocks<<<nblk(N * dNe, 256), 256, 0, s>>>((int)dNe, d, rowbase, NN, S->sid[d], S->eni, S->md, S->sd,
+                                                       S->kn, S->G[d], S->H[d], M, M3, R, L);
+    BQ_HIP(hipGetLastError());
+    k_build_rhs<<<nblk(N, 128), 128, 0, s>>>((int)dNe, d, rowbase, NN, S->sid[d], S->md, S->sd, S->kn, S->nrm, S->Ui,
+                                             S->dUi, S->G[d], S->H[d], R, b3);
+    BQ_HIP(hipGetLastError());
+    rowbase += (int)N;
+  }
+  const double one[2] = {1, 0}, mone[2] = {-1, 0};
+  int info = bq_zgetrf_dev(s, n2, M, n2, ip1);
+  if (info) {
+    bq::set_error("solve_global: block matrix M is singular (info %d)", info);
+    return info > 0 ? -3 : info;
+  }
+  if (bq_zgetrs_dev(s, n2, NN + 3, M, n2, ip1, R, n2)) return -1;
+  k_hadamard_normals<<<nblk((long)NN * NN, 256), 256, 0, s>>>(NN, R, nn3, W);
+  BQ_HIP(hipGetLastError());
+  if (bq_zgemm_op_dev(s, 0, 0, n2, NN, NN, mone, L, n2, W, NN, one, M3, n2)) return -1;
+  k_normal_dot<<<nblk(NN, 256), 256, 0, s>>>(NN, R, nn3, v);
+  BQ_HIP(hipGetLastError());
+  if (bq_zgemm_op_dev(s, 0, 0, n2, 1, NN, mone, L, n2, v, NN, one, b3, n2)) return -1;
+  info = bq_zgetrf_dev(s, n2, M3, n2, ip2);
+  if (info) {
+    bq::set_error("solve_global: Schur complement is singular (info %d)", info);
+    return info > 0 ? -3 : info;
+  }
+  if (bq_zgetrs_dev(s, n2, 1, M3, n2, ip2, b3, n2)) return -1;
+  k_scale_normals<<<nblk(NN, 256), 256, 0, s>>>(NN, b3, nn3, T);
+  BQ_HIP(hipGetLastError());
+  // y_c -= Z T_c for c = 0..2 (columns NN..NN+2 of R)
+  if (bq_zgemm_op_dev(s, 0, 0, n2, 3, NN, mone, R, n2, T, NN, one, R + (size_t)NN * n2 * 2, n2)) return -1;
+  std::vector<double> Y((size_t)n2 * 4 * 2);
+  BQ_HIP(hipMemcpyAsync(Y.data(), R + (size_t)NN * n2 * 2, (size_t)n2 * 3 * 16, hipMemcpyDeviceToHost, s));
+  BQ_HIP(hipMemcpyAsync(Y.data() + (size_t)n2 * 3 * 2, b3, (size_t)n2 * 16, hipMemcpyDeviceToHost, s));
+  BQ_HIP(hipEventRecord(e1, s));
+  BQ_HIP(hipStreamSynchronize(s));
+  float ms = 0;
+  BQ_HIP(hipEventElapsedTime(&ms, e0, e1));
+  S->st.t_solve_ms = ms;
+  for (int c = 0; c < 4; ++c)
+    for (int e = 0; e < NN; ++e) {
+      const double* y = Y.data() + 2 * ((size_t)c * n2);
+      Xh[2 * ((size_t)c * NN + e)] = y[2 * e];
+      Xh[2 * ((size_t)c * NN + e) + 1] = y[2 * e + 1];
+      Xh[2 * ((size_t)(c + 4) * NN + e)] = y[2 * (NN + e)];
+      Xh[2 * ((size_t)(c + 4) * NN + e) + 1] = y[2 * (NN + e) + 1];
+    }
+  for (void* p : {(void*)M, (void*)M3, (void*)R, (void*)L, (void*)b3, (void*)W, (void*)v, (void*)T, (void*)ip1,
+                  (void*)ip2, (void*)nn3})
+    (void)hipFree(p);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return 0;
+}
+
+int bq_sess_dudt(bq_sess* S, int d, const double* U, const double* dU, double* out)
+{
+  hipStream_t s = S->s;
+  const long N = 4L * S->dNe[d];
+  if (!S->dG[d]) {
+    bq::set_error("dudt: domain %d has no derivative matrices", d);
+    return -2;
+  }
+  double *du, *u, *o;
+  if (upload(&du, dU, (size_t)N * 8, s) || upload(&u, U, (size_t)N * 8, s) || dmalloc(&o, (size_t)2 * N * 8))
+    return -1;
+  const double one[2] = {1, 0}, mone[2] = {-1, 0}, zero[2] = {0, 0};
+  // dG, dH are row-major (2N x N) = column-major (N x 2N) transposed
+  if (bq_zgemm_op_dev(s, 1, 0, (int)(2 * N), 4, (int)N, one, S->dG[d], N, du, N, zero, o, 2 * N)) return -1;
+  if (bq_zgemm_op_dev(s, 1, 0, (int)(2 * N), 4, (int)N, mone, S->dH[d], N, u, N, one, o, 2 * N)) return -1;
+  BQ_HIP(hipMemcpyAsync(out, o, (size_t)2 * N * 4 * 16, hipMemcpyDeviceToHost, s));
+  BQ_HIP(hipStreamSynchronize(s));
+  (void)hipFree(du);
+  (void)hipFree(u);
+  (void)hipFree(o);
+  return 0;
+}
+
+int bq_sess_deh(bq_sess* S, int d, const double* EH, double* dEH)
+{
+  hipStream_t s = S->s;
+  const long N = 4L * S->dNe[d];
+  double *eh, *b, *gt;
+  int* ip;
+  if (upload(&eh, EH, (size_t)N * 12, s) || dmalloc(&b, (size_t)N * 12) || dmalloc(&gt, (size_t)N * N * 2) ||
+      dmalloc(&ip, (size_t)N))
+    return -1;
+  const double one[2] = {1, 0}, zero[2] = {0, 0};
+  if (bq_zgemm_op_dev(s, 1, 0, (int)N, 6, (int)N, one, S->H[d], N, eh, N, zero, b, N)) return -1;
+  k_transpose<<<nblk(N * N, 256), 256, 0, s>>>(N, S->G[d], gt);
+  BQ_HIP(hipGetLastError());
+  int info = bq_zgetrf_dev(s, (int)N, gt, N, ip);
+  if (info) {
+    bq::set_error("solve_deh: G of domain %d is singular (info %d)", d, info);
+    return info > 0 ? -3 : info;
+  }
+  if (bq_zgetrs_dev(s, (int)N, 6, gt, N, ip, b, N)) return -1;
+  BQ_HIP(hipMemcpyAsync(dEH, b, (size_t)N * 6 * 16, hipMemcpyDeviceToHost, s));
+  BQ_HIP(hipStreamSynchronize(s));
+  (void)hipFree(eh);
+  (void)hipFree(b);
+  (void)hipFree(gt);
+  (void)hipFree(ip);
+  return 0;
+}
+
+int bq_sess_get_GH(bq_sess* S, int d, double* G, double* H)
+{
+  const size_t N = 4 * (size_t)S->dNe[d];
+  if (G) BQ_HIP(hipMemcpyAsync(G, S->G[d], N * N * 16, hipMemcpyDeviceToHost, S->s));
+  if (H) BQ_HIP(hipMemcpyAsync(H, S->H[d], N * N * 16, hipMemcpyDeviceToHost, S->s));
+  BQ_HIP(hipStreamSynchronize(S->s));
+  return 0;
+}
+
+int bq_sess_get_dGH(bq_sess* S, int d, double* dG, double* dH, double* tdF)
+{
+  const size_t N = 4 * (size_t)S->dNe[d];
+  if (!S->dG[d]) return -2;
+  if (dG) BQ_HIP(hipMemcpyAsync(dG, S->dG[d], 2 * N * N * 16, hipMemcpyDeviceToHost, S->s));
+  if (dH) BQ_HIP(hipMemcpyAsync(dH, S->dH[d], 2 * N * N * 16, hipMemcpyDeviceToHost, S->s));
+  if (tdF) BQ_HIP(hipMemcpyAsync(tdF, S->tdF[d], 3 * N * 8, hipMemcpyDeviceToHost, S->s));
+  BQ_HIP(hipStreamSynchronize(S->s));
+  return 0;
+}
+
+}  // extern "C"

@@ -24,8 +24,10 @@ int bq_qplan_init(bq_qplan *p, double k, const double *vk, const double *vd1, co
   p->g2x = -cpd * vd1[1]; p->g2y = cpd * vd1[0];
   p->b1x = p->g1x / (2.0 * M_PI); p->b1y = p->g1y / (2.0 * M_PI);
   p->b2x = p->g2x / (2.0 * M_PI); p->b2y = p->g2y / (2.0 * M_PI);
-  double adp = hypot(vd1[0] + vd2[0], vd1[1] + vd2[1]);
-  double adm = hypot(vd1[0] - vd2[0], vd1[1] - vd2[1]);
+  /* same expression as d3_qpgf_d2.c:1680-1682, so the value is bit-identical */
+  double sp0 = vd1[0] + vd2[0], sp1 = vd1[1] + vd2[1], sm0 = vd1[0] - vd2[0], sm1 = vd1[1] - vd2[1];
+  double adp = sqrt(sp0 * sp0 + sp1 * sp1);
+  double adm = sqrt(sm0 * sm0 + sm1 * sm1);
   p->veps0 = sqrt(fabs(det) / (4.0 * M_PI) * adp / adm);
   p->eps = eps;
 

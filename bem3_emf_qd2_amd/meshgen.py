@@ -1,0 +1,143 @@
+"""Input generation for the benchmark and test scenes: cube-sphere meshes in the Gmsh 2.2
+ASCII paired-element format the reference reads (d3qd2_src/d3qd2_setup.c:662-959), and
+the three text input files (ipw, periodicity, medium; d3qd2_src/m_ipw.c:9-106,
+d3qd2_src/d3qd2_setup.c:557-650).
+
+Cube-sphere: each cube face is split into an n x n grid of equal angles (gnomonic
+projection) and pushed radially onto the sphere, like mesh_sample/sphere_m4.geo does with
+transfinite circle arcs; 6 n^2 bilinear quads, 6 n^2 + 2 nodes.
+
+Mesh convention (checked against mesh_sample/sphere_m4.msh): each surface element is
+written twice, first for the inner domain with its normal pointing out of that domain, then
+for the outer domain with the node order reversed (0, 3, 2, 1); the open region is
+physical 99.
+"""
+from __future__ import annotations
+
+import math
+from pathlib import Path
+
+import numpy as np
+
+OPEN = 99
+
+
+def cube_sphere(n: int, radius: float, center=(0.0, 0.0, 0.0)):
+    """Return (nodes[N,3], quads[M,4]) with 0-based node ids and outward normals."""
+    t = np.tan(np.linspace(-math.pi / 4, math.pi / 4, n + 1))
+    faces = []
+    # (axis, sign): face normal direction; local u, v axes chosen so (u x v) = outward
+    for axis in range(3):
+        for sign in (1.0, -1.0):
+            u_ax, v_ax = [(1, 2), (2, 0), (0, 1)][axis]
+            if sign < 0:
+                u_ax, v_ax = v_ax, u_ax
+            faces.append((axis, sign, u_ax, v_ax))
+    key_to_id: dict[tuple, int] = {}
+    nodes: list[np.ndarray] = []
+    quads = []
+
+    def node_id(p):
+        k = tuple(np.round(p, 12))
+        if k not in key_to_id:
+            key_to_id[k] = len(nodes)
+            nodes.append(p)
+        return key_to_id[k]
+
+    for axis, sign, ua, va in faces:
+        ids = np.zeros((n + 1, n + 1), int)
+        for i in range(n + 1):
+            for j in range(n + 1):
+                p = np.zeros(3)
+                p[axis] = sign
+                p[ua] = t[i]
+                p[va] = t[j]
+                p = p / np.linalg.norm(p)
+                ids[i, j] = node_id(p)
+        for i in range(n):
+            for j in range(n):
+                quads.append([ids[i, j], ids[i + 1, j], ids[i + 1, j + 1], ids[i, j + 1]])
+    X = np.array(nodes) * radius + np.asarray(center, float)
+    Q = np.array(quads, int)
+    # orient every quad outward (robust against the per-face axis bookkeeping)
+    c = X[Q].mean(1) - np.asarray(center, float)
+    nrm = np.cross(X[Q[:, 2]] - X[Q[:, 0]], X[Q[:, 3]] - X[Q[:, 1]])
+    flip = np.einsum("ij,ij->i", nrm, c) < 0
+    Q[flip] = Q[flip][:, ::-1]
+    return X, Q
+
+
+def write_msh(path, nodes: np.ndarray, surfaces) -> None:
+    """surfaces: list of (elements[M,k] 0-based, inner_domain, outer_domain, entity_id);
+    element normals point out of inner_domain.  Domain 0 is written as physical 99."""
+    path = Path(path)
+    lines = ["$MeshFormat", "2.2 0 8", "$EndMeshFormat", "$Nodes", str(len(nodes))]
+    for i, p in enumerate(nodes, 1):
+        lines.append(f"{i} {p[0]:.16g} {p[1]:.16g} {p[2]:.16g}")
+    lines += ["$EndNodes", "$Elements"]
+    body = []
+    eid = 1
+    for elems, din, dout, ent in surfaces:
+        pin, pout = (din or OPEN), (dout or OPEN)
+        for e in elems:
+            e1 = [int(v) + 1 for v in e]
+            typ = 3 if len(e1) == 4 else 2
+            rev = [e1[0], e1[3], e1[2], e1[1]] if typ == 3 else [e1[0], e1[2], e1[1]]
+            body.append(f"{eid} {typ} 2 {pin} {ent} " + " ".join(map(str, e1)))
+            body.append(f"{eid + 1} {typ} 2 {pout} {ent} " + " ".join(map(str, rev)))
+            eid += 2
+    lines.append(str(len(body)))
+    lines += body
+    lines.append("$EndElements")
+    path.write_text("\n".join(lines) + "\n")
+
+
+def write_ipw(path, waves) -> None:
+    """waves: list of dicts with lambda0, ni, power, e0x, e0y (complex), theta, phi."""
+    out = ["## plane wave data", "# number of wave", str(len(waves)),
+           "# lambda ni power real(e0x) imag(e0x) real(e0y) imag(e0y) fx fy fz theta phi"]
+    for w in waves:
+        ex, ey = complex(w.get("e0x", 1.0)), complex(w.get("e0y", 0.0))
+        out.append(f"{w['lambda0']:.17g} {w['ni']:.17g} {w.get('power', 0.5 * w['ni']):.17g} "
+                   f"{ex.real:.17g} {ex.imag:.17g} {ey.real:.17g} {ey.imag:.17g} 0 0 0 "
+                   f"{w['theta']:.17g} {w.get('phi', 0.0):.17g}")
+    Path(path).write_text("\n".join(out) + "\n")
+
+
+def write_periodicity(path, d1, d2, ps=1) -> None:
+    Path(path).write_text("## periodicity datafile\n# ps d1x d1y d2x d2y\n"
+                          f"{ps} {d1[0]:.17g} {d1[1]:.17g} {d2[0]:.17g} {d2[1]:.17g}\n")
+
+
+def write_medium(path, indices) -> None:
+    lines = ["## medium datafile", "# number of medium", str(len(indices)), "# real(n) imag(n)"]
+    lines += [f"{complex(n).real:.17g} {complex(n).imag:.17g}" for n in indices]
+    Path(path).write_text("\n".join(lines) + "\n")
+
+
+def sphere_scene(dirpath, n: int, radius: float, n_sphere, lam: float, n0: float = 1.0,
+                 theta: float = 1e-6, phi: float = 0.0, pol: str = "TM", d1=(1.0, 0.0), d2=(0.0, 1.0),
+                 core=None):
+    """Write a complete input set for a (optionally two-layer) sphere on a lattice.
+
+    core = (radius, index) adds a concentric core (domain 2 inside the shell domain 1).
+    pol: "TM" -> e0x = 1 (p-polarised in the beam frame), "TE" -> e0y = 1
+    (d3qd2_src/ipw.c:139-144).  Amplitude 1 means power = n0 / 2 (d3qd2_src/ipw.c:131).
+    """
+    d = Path(dirpath)
+    d.mkdir(parents=True, exist_ok=True)
+    X, Q = cube_sphere(n, radius)
+    surfaces = [(Q, 1, 0, 1)]
+    media = [n_sphere]
+    if core is not None:
+        Xc, Qc = cube_sphere(n, core[0])
+        Qc = Qc + len(X)
+        X = np.vstack([X, Xc])
+        surfaces.append((Qc, 2, 1, 2))
+        media.append(core[1])
+    write_msh(d / "mesh.msh", X, surfaces)
+    ex, ey = (1.0, 0.0) if pol.upper() == "TM" else (0.0, 1.0)
+    write_ipw(d / "ipw.txt", [dict(lambda0=lam, ni=n0, power=0.5 * n0, e0x=ex, e0y=ey, theta=theta, phi=phi)])
+    write_periodicity(d / "periodicity_data.txt", d1, d2)
+    write_medium(d / "medium_data.txt", media)
+    return d

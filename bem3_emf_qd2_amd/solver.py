@@ -1,0 +1,129 @@
+"""ctypes driver for the drop-in C API (read/solve/write and field evaluation).
+
+Thin plumbing over libbem3qd2.so for tests and benchmarks; the computation is the
+library's.  DQD2 is held as an opaque buffer of the size the library reports.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+import numpy as np
+
+from . import lib as _lib
+
+_D = ctypes.POINTER(ctypes.c_double)
+_I = ctypes.POINTER(ctypes.c_int)
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("t_assemble_ms", ctypes.c_double), ("t_solve_ms", ctypes.c_double),
+                ("t_fields_ms", ctypes.c_double), ("n_gh_pairs", ctypes.c_int), ("n_qpgf", ctypes.c_longlong)]
+
+
+def L():
+    lb = _lib()
+    if not getattr(lb, "_solver_declared", False):
+        lb.bq_sizeof_dqd2.restype = ctypes.c_int
+        for f in ("dat_read", "dat_write"):
+            getattr(lb, f).argtypes = [ctypes.c_char_p, ctypes.c_void_p]
+        for f in ("initialize_dqd2", "finalize_dqd2", "solve_bieq", "print_dqd2"):
+            getattr(lb, f).argtypes = [ctypes.c_void_p]
+        lb.read_dqd2.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_char_p), ctypes.c_void_p]
+        lb.output_node_particles.argtypes = [ctypes.c_char_p, ctypes.c_void_p]
+        lb.bq_EH_t_batch.argtypes = [ctypes.c_void_p, ctypes.c_int, _D, ctypes.c_int, _D, _D, _I]
+        lb.bq_assemble_domain_host.argtypes = [ctypes.c_void_p, ctypes.c_int, _D, _D, _D, _D, _D]
+        lb.bq_solve_last_stats.restype = ctypes.POINTER(Stats)
+        lb._solver_declared = True
+    return lb
+
+
+class Model:
+    """A DQD2 owned by Python; released with close()."""
+
+    def __init__(self):
+        self.buf = ctypes.create_string_buffer(L().bq_sizeof_dqd2())
+        self.ptr = ctypes.cast(self.buf, ctypes.c_void_p)
+        self.live = False
+
+    @classmethod
+    def from_dat(cls, path) -> "Model":
+        m = cls()
+        L().dat_read(str(path).encode(), m.ptr)
+        m.live = True
+        return m
+
+    @classmethod
+    def from_inputs(cls, ipw, per, med, msh, out="out.dat") -> "Model":
+        m = cls()
+        args = [b"bq", *(str(x).encode() for x in (ipw, per, med, msh, out))]
+        argv = (ctypes.c_char_p * len(args))(*args)
+        L().read_dqd2(len(args), argv, m.ptr)
+        L().initialize_dqd2(m.ptr)
+        m.live = True
+        return m
+
+    @classmethod
+    def from_dir(cls, d, out="out.dat") -> "Model":
+        d = Path(d)
+        return cls.from_inputs(d / "ipw.txt", d / "periodicity_data.txt", d / "medium_data.txt", d / "mesh.msh",
+                               d / out)
+
+    def solve(self) -> Stats:
+        L().solve_bieq(self.ptr)
+        return L().bq_solve_last_stats().contents
+
+    def write(self, path) -> None:
+        L().dat_write(str(path).encode(), self.ptr)
+
+    def fields(self, r, type_=1):
+        r = np.ascontiguousarray(np.asarray(r, float).reshape(-1, 3))
+        n = len(r)
+        E = np.zeros((n, 3), complex)
+        H = np.zeros((n, 3), complex)
+        did = np.zeros(n, np.int32)
+        L().bq_EH_t_batch(self.ptr, n, r.ctypes.data_as(_D), type_, E.ctypes.data_as(_D), H.ctypes.data_as(_D),
+                          did.ctypes.data_as(_I))
+        return E, H, did
+
+    def assemble(self, d, dNe, deriv=True):
+        N = 4 * dNe
+        G = np.zeros((N, N), complex)
+        H = np.zeros((N, N), complex)
+        dG = np.zeros((2 * N, N), complex) if deriv else None
+        dH = np.zeros((2 * N, N), complex) if deriv else None
+        tdF = np.zeros((N, 3)) if deriv else None
+        nul = ctypes.cast(None, _D)
+        p = lambda a: a.ctypes.data_as(_D) if a is not None else nul
+        rc = L().bq_assemble_domain_host(self.ptr, d, p(G), p(H), p(dG), p(dH), p(tdF))
+        if rc:
+            raise RuntimeError(f"device assembly failed ({rc}): {_lib().bq_last_error().decode()}")
+        return G, H, dG, dH, tdF
+
+    def close(self):
+        if self.live:
+            L().finalize_dqd2(self.ptr)
+            self.live = False
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def smoke_solve():
+    """Small end-to-end solve (24-quad sphere on a square lattice) checked for sanity."""
+    import tempfile
+    from . import meshgen
+    with tempfile.TemporaryDirectory() as td:
+        meshgen.sphere_scene(td, 2, 0.3, 1.5, 0.6, theta=0.2)
+        m = Model.from_dir(td)
+        st = m.solve()
+        E, H, did = m.fields([[0.1, 0.2, 0.9]], 1)
+        m.close()
+        print(f"smoke: 24-quad solve assembly {st.t_assemble_ms:.2f} ms, solve {st.t_solve_ms:.2f} ms, "
+              f"E(0.1,0.2,0.9) = {E[0]}")
+        if not np.all(np.isfinite(E)):
+            raise RuntimeError("smoke: non-finite field")

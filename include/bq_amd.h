@@ -28,6 +28,31 @@ int bq_faddeeva_w_batch(int n, const double *z, double *w);
  * (qgf2_src/d3_qpgf_d2.h:101-112, d3_qpgf_d2.c:124). */
 int bq_qpgf_v1_batch(const double *qd8, int n, const double *r, double *out);
 
+/* Same with a choice of the reference variant it reproduces at r_z == 0, where the
+ * reference's variants differ: variant 0 = d3hm_qpgf_d2_v1 / _dqG (their spectral sum
+ * stops early there, d3_qpgf_d2.c:1840-1963), variant 1 = d3hm_qpgf_d2_qG (converged,
+ * d3_qpgf_d2.c:1190-1335).  Elsewhere both give the converged sum. */
+int bq_qpgf_batch(const double *qd8, int variant, int n, const double *r, double *out);
+
+/* sizeof(DQD2) as compiled into the library (for callers that allocate it opaquely). */
+int bq_sizeof_dqd2(void);
+
+/* Total fields at n points of a solved model in one GPU launch (batched EH_t of
+ * d3qd2_src/d3qd2_field.c:271).  qd: a DQD2* after solve_bieq or dat_read; r[n][3];
+ * E, H: [n][3] complex out; did[n] out (domain id, -1 if none).  type 0..3 as EH_t. */
+int bq_EH_t_batch(void *qd, int n, const double *r, int type, double *E, double *H, int *did);
+
+/* Device assembly of one domain's coefficient matrices for a set-up model (DQD2* after
+ * initialize_dqd2 or dat_read), copied to host buffers (NULL to skip): G, H [N][N],
+ * dG, dH [2N][N] complex (rows 2 row + {zeta, eta}), tdF [N][3]; N = 4 x elements of the
+ * domain.  Row/column layout of the reference's tmpG/tmpH files
+ * (d3qd2_src/d3qd2_solve_bieq.c:212-234). */
+int bq_assemble_domain_host(void *qd, int d, double *G, double *H, double *dG, double *dH, double *tdF);
+
+/* Counters and phase times (ms, device clock) of the last solve_bieq in this process:
+ * returns a pointer to {t_assemble_ms, t_solve_ms, t_fields_ms, n_gh_pairs, n_qpgf}. */
+const void *bq_solve_last_stats(void);
+
 #ifdef __cplusplus
 }
 #endif

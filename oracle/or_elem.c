@@ -204,13 +204,16 @@ static void q0_sing_fn(void *vp, double r, double c, double s, double w)
   double K[3], W[3], z, t, R[3];
   polar_KW(&p->e, p->zt, p->et, r, c, s, K, W, &z, &t);
   for (int d = 0; d < 3; d++) R[d] = r * K[d];
-  double complex o[10];
-  qpgf_or_die(p->m, 0, R, o);
+  /* the reference evaluates qG and dqG separately here (q0_bil_coef.c:444, :489); their
+   * stop rules differ only for R_z == 0 exactly */
+  double complex o[10], og[10];
+  qpgf_or_die(p->m, 3, R, o);
+  qpgf_or_die(p->m, 2, R, og);
   double aW = nrm3(W);
   double complex hn = o[1] * W[0] + o[2] * W[1] + o[3] * W[2];
   for (int n = 0; n < 4; n++) {
     double Np = Nn(n, z, t);
-    p->G[n] += w * o[0] * Np * aW * r;
+    p->G[n] += w * og[0] * Np * aW * r;
     p->H[n] += w * hn * Np * r;
   }
 }

@@ -12,9 +12,10 @@
  *   Ewald spatial sum and its terms                       :1714-1838, 2052-2200
  *   Ewald spectral sum and its terms                      :1840-2009, 2201-2399
  *   adaptive stopping tests (EFS, comp/diff)              :471-482
- * The five reference variants (_v1, _v2, _qG, _dqG) share one implementation here; the
- * stopping tests of _v1 drive modes 0/2/3 and those of _v2 drive mode 1.  Their results
- * differ from the reference only in terms below eps*EFS of the sum.
+ * The reference variants (_v1, _v2, _qG, _dqG) share one implementation here, each with
+ * its own stopping tests (sp_* / sn_* below).  Results differ from the reference only in
+ * terms below eps*EFS of the sum -- including the reference's early stop of the _v1/_dqG
+ * spectral sum when r_z == 0 exactly, whose stop test then compares zero with zero.
  */
 #include <math.h>
 #include <complex.h>
@@ -158,7 +159,38 @@ static void ew_spatial_term(const or_qpd2 *qd, int l1, int l2, const double *r, 
   t->h[5] = ra[0] * cf3;
 }
 
-static int ew_spatial(int v2, const double *r, double eps, double veps, const or_qpd2 *qd, acc_t *res)
+/* stop rules per reference variant (mode): 0 _v1 and 2 _qG stop the spatial sum on the
+ * value, 3 _dqG on the x/y gradient, 1 _v2 on the xx/yy Hessian (d3_qpgf_d2.c:1082-1168,
+ * :1377-1480, :1714-1812, :2052-2157) */
+static int sp_small(int mode, const acc_t *T, const acc_t *S, double eps)
+{
+  if (mode == 1) return small_z(T->h[0], S->h[0], eps) && small_z(T->h[1], S->h[1], eps);
+  if (mode == 3) return small_z(T->d[0], S->d[0], eps) && small_z(T->d[1], S->d[1], eps);
+  return small_z(T->q, S->q, eps);
+}
+static int sp_same(int mode, const acc_t *S, const acc_t *O, double eps)
+{
+  if (mode == 1) return same_z(S->h[0], O->h[0], eps) && same_z(S->h[1], O->h[1], eps);
+  if (mode == 3) return same_z(S->d[0], O->d[0], eps) && same_z(S->d[1], O->d[1], eps);
+  return same_z(S->q, O->q, eps);
+}
+/* spectral sums: _v1 and _dqG stop on dqG_z (which is exactly zero at z = 0, so those
+ * variants truncate there), _qG on the value, _v2 on the xx/yy Hessian
+ * (d3_qpgf_d2.c:1190-1335, :1505-1670, :1840-2009, :2201-2399) */
+static int sn_small(int mode, const acc_t *T, const acc_t *S, double eps)
+{
+  if (mode == 1) return small_z(T->h[0], S->h[0], eps) && small_z(T->h[1], S->h[1], eps);
+  if (mode == 2) return small_z(T->q, S->q, eps);
+  return small_z(T->d[2], S->d[2], eps);
+}
+static int sn_same(int mode, const acc_t *S, const acc_t *O, double eps)
+{
+  if (mode == 1) return same_z(S->h[0], O->h[0], eps) && same_z(S->h[1], O->h[1], eps);
+  if (mode == 2) return same_z(S->q, O->q, eps);
+  return same_z(S->d[2], O->d[2], eps);
+}
+
+static int ew_spatial(int mode, const double *r, double eps, double veps, const or_qpd2 *qd, acc_t *res)
 {
   acc_t S = {0}, T;
   int cnt = 0;
@@ -178,16 +210,12 @@ static int ew_spatial(int v2, const double *r, double eps, double veps, const or
           ew_spatial_term(qd, l1, l2, r, eps, veps, &T);
           acc_add(&S, &T);
           cnt++;
-          int stop = v2 ? (small_z(T.h[0], S.h[0], eps) && small_z(T.h[1], S.h[1], eps))
-                        : small_z(T.q, S.q, eps);
-          if (stop) break;
+          if (sp_small(mode, &T, &S, eps)) break;
         }
         if (m == EW_LIMIT) return -1;
       }
       if (dir == 0) break;
-      int conv = v2 ? (same_z(S.h[0], old.h[0], eps) && same_z(S.h[1], old.h[1], eps))
-                    : same_z(S.q, old.q, eps);
-      if (conv) break;
+      if (sp_same(mode, &S, &old, eps)) break;
     }
   }
   res->q = S.q / (4.0 * M_PI);
@@ -243,7 +271,7 @@ static int ew_spectral_term(const or_qpd2 *qd, int n1, int n2, const double *g1,
   return kind;
 }
 
-static int ew_spectral(int v2, const double *r, double eps, double veps, const or_qpd2 *qd, acc_t *res)
+static int ew_spectral(int mode, const double *r, double eps, double veps, const or_qpd2 *qd, acc_t *res)
 {
   double det = qd->vd1[0] * qd->vd2[1] - qd->vd2[0] * qd->vd1[1];
   if (det == 0.0) return -4;
@@ -273,19 +301,13 @@ static int ew_spectral(int v2, const double *r, double eps, double veps, const o
           acc_add(&S, &T);
           cnt++;
           if (k > 0) {
-            int stop = v2 ? (small_z(T.h[0], S.h[0], eps) && small_z(T.h[1], S.h[1], eps))
-                          : small_z(T.d[2], S.d[2], eps);
-            if (stop) break;
+            if (sn_small(mode, &T, &S, eps)) break;
           } else nprop++;
         }
         if (m == EW_LIMIT) return -2;
       }
       if (dir == 0) break;
-      if (nprop == 0) {
-        int conv = v2 ? (same_z(S.h[0], old.h[0], eps) && same_z(S.h[1], old.h[1], eps))
-                      : same_z(S.d[2], old.d[2], eps);
-        if (conv) break;
-      }
+      if (nprop == 0 && sn_same(mode, &S, &old, eps)) break;
     }
   }
   double cd = 0.25 * fabs(idd);
@@ -303,7 +325,7 @@ static int ew_spectral(int v2, const double *r, double eps, double veps, const o
   return cnt;
 }
 
-static int ew_sum(int v2, const double *r, double eps, const or_qpd2 *qd, acc_t *res)
+static int ew_sum(int mode, const double *r, double eps, const or_qpd2 *qd, acc_t *res)
 {
   double det = qd->vd1[0] * qd->vd2[1] - qd->vd2[0] * qd->vd1[1];
   if (det == 0.0) return -4;
@@ -314,9 +336,9 @@ static int ew_sum(int v2, const double *r, double eps, const or_qpd2 *qd, acc_t 
   if (qd->k * qd->k * veps * veps - r2 / (4.0 * veps * veps) > EW_EXPM)
     veps = sqrt(EW_EXPM + sqrt(qd->k * qd->k * r2 + EW_EXPM * EW_EXPM)) / (sqrt(2.0) * qd->k);
   acc_t a, b;
-  int c1 = ew_spatial(v2, r, eps, veps, qd, &a);
+  int c1 = ew_spatial(mode, r, eps, veps, qd, &a);
   if (c1 < 0) return c1;
-  int c2 = ew_spectral(v2, r, eps, veps, qd, &b);
+  int c2 = ew_spectral(mode, r, eps, veps, qd, &b);
   if (c2 < 0) return c2;
   *res = a;
   acc_add(res, &b);
@@ -363,7 +385,7 @@ int or_qpgf(int mode, const double *r, double eps, const or_qpd2 *qd, double com
     err = fd_sum(v2, rt, eps, qd, &res);
     if (err < 0) return err - 10;
   } else {
-    err = ew_sum(v2, rt, eps, qd, &res);
+    err = ew_sum(mode, rt, eps, qd, &res);
     if (err < 0) return err - 20;
   }
   if (l1 != 0 || l2 != 0) {

@@ -2,8 +2,10 @@
 qgf2_src/d3_qpgf_d2.c, compiled in place by oracle/Makefile into oracle/_ref/libqpgf_ref.so.
 
 Writes qpgf_ref.npz: for each lattice/incidence case, seeded sample points r and the
-reference's d3hm_qpgf_d2_v2 output (qG, dqG[3], d2qG[6]) with eps = 1e-15, plus its
-return code (term count).  Run where oracle/_ref has been built:
+reference's d3hm_qpgf_d2_v2 output (qG, dqG[3], d2qG[6]) with eps = 1e-15 and its return
+code, plus the _v1 output and the _qG / _dqG outputs at the same points.  The _v1 and _dqG
+variants stop their spectral sums early when r_z == 0 exactly (their stop test compares a
+zero z-derivative with a zero sum), so at the z = 0 sample points they differ from _v2.  Run where oracle/_ref has been built:
     python tests/golden/make_qpgf_golden.py
 """
 import ctypes
@@ -47,6 +49,9 @@ def main():
     D = ctypes.POINTER(ctypes.c_double)
     lib.d3hm_qpgf_d2_v2.argtypes = [D, D, D, D, ctypes.c_double, ctypes.POINTER(QPD2)]
     lib.d3hm_qpgf_d2_v2.restype = ctypes.c_int
+    lib.d3hm_qpgf_d2_v1.argtypes = [D, D, D, ctypes.c_double, ctypes.POINTER(QPD2)]
+    lib.d3hm_qpgf_d2_qG.argtypes = [D, D, ctypes.c_double, ctypes.POINTER(QPD2)]
+    lib.d3hm_qpgf_d2_dqG.argtypes = [D, D, ctypes.c_double, ctypes.POINTER(QPD2)]
     rng = np.random.default_rng(20261203)
     out = {}
     for name, q in CASES.items():
@@ -58,6 +63,8 @@ def main():
             np.c_[rng.uniform(-2, 2, (16, 2)), rng.choice([-1, 1], 16) * rng.uniform(1.5, 3.0, 16)],  # far in z
         ])
         vals = np.zeros((len(R), 10), complex)
+        v1 = np.zeros((len(R), 4), complex)      # _v1: qG, dqG
+        vq = np.zeros((len(R), 4), complex)      # _qG value, _dqG gradient
         codes = np.zeros(len(R), np.int32)
         for i, p in enumerate(R):
             pr = (ctypes.c_double * 3)(*p)
@@ -66,10 +73,19 @@ def main():
             vals[i, 0] = complex(g[0], g[1])
             vals[i, 1:4] = np.array(d[0::2]) + 1j * np.array(d[1::2])
             vals[i, 4:10] = np.array(h[0::2]) + 1j * np.array(h[1::2])
+            lib.d3hm_qpgf_d2_v1(g, d, pr, 1e-15, ctypes.byref(qd))
+            v1[i, 0] = complex(g[0], g[1])
+            v1[i, 1:4] = np.array(d[0::2]) + 1j * np.array(d[1::2])
+            lib.d3hm_qpgf_d2_qG(g, pr, 1e-15, ctypes.byref(qd))
+            lib.d3hm_qpgf_d2_dqG(d, pr, 1e-15, ctypes.byref(qd))
+            vq[i, 0] = complex(g[0], g[1])
+            vq[i, 1:4] = np.array(d[0::2]) + 1j * np.array(d[1::2])
         out[name + "_qd"] = np.array(q)
         out[name + "_r"] = R
         out[name + "_val"] = vals
         out[name + "_code"] = codes
+        out[name + "_v1"] = v1
+        out[name + "_vq"] = vq
     np.savez_compressed(HERE / "qpgf_ref.npz", names=np.array(list(CASES)), **out)
     print("cases", list(CASES), "points per case", len(R))
 
