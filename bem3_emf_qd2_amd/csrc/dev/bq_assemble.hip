@@ -732,7 +732,7 @@ int bq_assemble_domain_dev(const bq_adom* a, void* work, void* stream)
   BQ_HIP(hipMemsetAsync(H, 0, sizeof(double2) * N * N, s));
   BQ_HIP(hipMemsetAsync(count, 0, sizeof(int), s));
   const int freespace = a->periodic ? 0 : 1;
-  k_classify<<<(unsigned)((pairs + 255) / 256), 256, 0, s>>>(a->E, a->ren, a->sid, dNe, freespace, F3, flag);
+  BQ_KTIME("k_classify", s, k_classify<<<(unsigned)((pairs + 255) / 256), 256, 0, s>>>(a->E, a->ren, a->sid, dNe, freespace, F3, flag));
   BQ_HIP(hipGetLastError());
   k_collect_gh<<<(unsigned)((pairs + 255) / 256), 256, 0, s>>>(pairs, flag, list, count);
   BQ_HIP(hipGetLastError());
@@ -741,22 +741,22 @@ int bq_assemble_domain_dev(const bq_adom* a, void* work, void* stream)
   BQ_HIP(hipStreamSynchronize(s));
   const unsigned gl_blocks = (unsigned)(((long)dNe * dNe + 3) / 4);
   if (a->periodic) {
-    k_q0_gl<<<gl_blocks, 256, 0, s>>>(a->plan, a->E, a->ren, a->sid, dNe, flag, G, H);
+    BQ_KTIME("k_q0_gl", s, k_q0_gl<<<gl_blocks, 256, 0, s>>>(a->plan, a->E, a->ren, a->sid, dNe, flag, G, H));
     BQ_HIP(hipGetLastError());
-    if (ngh) k_q0_gh<<<ngh, 256, 0, s>>>(a->plan, a->E, a->ren, a->sid, dNe, list, G, H, F3);
+    if (ngh) BQ_KTIME("k_q0_gh", s, k_q0_gh<<<ngh, 256, 0, s>>>(a->plan, a->E, a->ren, a->sid, dNe, list, G, H, F3));
     BQ_HIP(hipGetLastError());
-    k_q0_sing<<<(unsigned)N, 256, 0, s>>>(a->plan, a->E, a->sid, dNe, G, H, F3);
+    BQ_KTIME("k_q0_sing", s, k_q0_sing<<<(unsigned)N, 256, 0, s>>>(a->plan, a->E, a->sid, dNe, G, H, F3));
     BQ_HIP(hipGetLastError());
   } else {
     double2* dG = (double2*)a->dG;
     double2* dH = (double2*)a->dH;
     BQ_HIP(hipMemsetAsync(dG, 0, sizeof(double2) * 2 * N * N, s));
     BQ_HIP(hipMemsetAsync(dH, 0, sizeof(double2) * 2 * N * N, s));
-    k_fs_gl<<<gl_blocks, 256, 0, s>>>(a->kr, a->ki, a->E, a->ren, a->sid, dNe, flag, G, H, dG, dH);
+    BQ_KTIME("k_fs_gl", s, k_fs_gl<<<gl_blocks, 256, 0, s>>>(a->kr, a->ki, a->E, a->ren, a->sid, dNe, flag, G, H, dG, dH));
     BQ_HIP(hipGetLastError());
-    if (ngh) k_fs_gh<<<ngh, 256, 0, s>>>(a->kr, a->ki, a->E, a->ren, a->sid, dNe, list, G, H, dG, dH, F3);
+    if (ngh) BQ_KTIME("k_fs_gh", s, k_fs_gh<<<ngh, 256, 0, s>>>(a->kr, a->ki, a->E, a->ren, a->sid, dNe, list, G, H, dG, dH, F3));
     BQ_HIP(hipGetLastError());
-    k_fs_sing<<<(unsigned)N, 256, 0, s>>>(a->kr, a->ki, a->E, a->sid, dNe, G, H, dG, dH, F3);
+    BQ_KTIME("k_fs_sing", s, k_fs_sing<<<(unsigned)N, 256, 0, s>>>(a->kr, a->ki, a->E, a->sid, dNe, G, H, dG, dH, F3));
     BQ_HIP(hipGetLastError());
   }
   k_row_diag<<<(unsigned)((N + 255) / 256), 256, 0, s>>>(dNe, a->open_domain, F3, H, a->tdF);

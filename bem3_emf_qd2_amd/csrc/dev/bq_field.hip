@@ -142,7 +142,7 @@ extern "C" int bq_field_eval_dev(const bq_fctx* C, int npts, const double* d_pts
 {
   if (npts <= 0) return 0;
   hipStream_t s = (hipStream_t)stream;
-  k_field<<<npts, 256, 0, s>>>(*C, npts, d_pts, d_did, type, d_out);
+  BQ_KTIME("k_field", s, k_field<<<npts, 256, 0, s>>>(*C, npts, d_pts, d_did, type, d_out));
   BQ_HIP(hipGetLastError());
   return 0;
 }

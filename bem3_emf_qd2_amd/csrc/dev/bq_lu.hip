@@ -222,9 +222,10 @@ int bq_zgetrf_dev(void* stream, int n, double* Ad, long lda, int* ipiv)
   hipStream_t s = (hipStream_t)stream;
   double2* A = (double2*)Ad;
   const double2 one = make_double2(1.0, 0.0), mone = make_double2(-1.0, 0.0);
+  const int kt_all = bq::ktime_begin("zgetrf", s);
   for (int j = 0; j < n; j += kNB) {
     int jb = min(kNB, n - j);
-    k_panel<<<1, 1024, 0, s>>>(n, A, lda, j, jb, ipiv);
+    BQ_KTIME("zgetrf_panel", s, k_panel<<<1, 1024, 0, s>>>(n, A, lda, j, jb, ipiv));
     BQ_HIP(hipGetLastError());
     if (j > 0) k_swap<<<(j + 255) / 256, 256, 0, s>>>(A, lda, j, jb, ipiv, 0, j);
     int rest = n - j - jb;
@@ -233,15 +234,18 @@ int bq_zgetrf_dev(void* stream, int n, double* Ad, long lda, int* ipiv)
       k_trsm_lu<<<(rest + 127) / 128, 128, 0, s>>>(jb, A + (long)j * lda + j, lda, A + (long)(j + jb) * lda + j, lda,
                                                      rest);
       BQ_HIP(hipGetLastError());
-      if (zgemm(s, rest, rest, jb, mone, A + (long)j * lda + (j + jb), lda, A + (long)(j + jb) * lda + j, lda, one,
-                A + (long)(j + jb) * lda + (j + jb), lda))
-        return -1;
+      int rc = 0;
+      BQ_KTIME("zgetrf_update", s,
+               rc = zgemm(s, rest, rest, jb, mone, A + (long)j * lda + (j + jb), lda, A + (long)(j + jb) * lda + j,
+                          lda, one, A + (long)(j + jb) * lda + (j + jb), lda));
+      if (rc) return -1;
     }
   }
   int* dinfo = nullptr;
   BQ_HIP(hipMallocAsync((void**)&dinfo, sizeof(int), s));
   int big = 0x7fffffff;
   BQ_HIP(hipMemcpyAsync(dinfo, &big, sizeof(int), hipMemcpyHostToDevice, s));
+  bq::ktime_end(kt_all, s);
   k_check_diag<<<(n + 255) / 256, 256, 0, s>>>(n, A, lda, dinfo);
   int info = 0;
   BQ_HIP(hipMemcpyAsync(&info, dinfo, sizeof(int), hipMemcpyDeviceToHost, s));

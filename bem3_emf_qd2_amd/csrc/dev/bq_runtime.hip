@@ -62,6 +62,71 @@ hipStream_t stream()
   return g_stream[dev];
 }
 
+// ---- kernel timing ----------------------------------------------------------------
+namespace {
+constexpr int kMaxNames = 64, kMaxPending = 4096;
+struct KStat {
+  char name[48];
+  double ms;
+  long long n;
+};
+struct KPending {
+  int stat;
+  hipEvent_t a, b;
+};
+std::mutex t_mu;
+int t_on = 0, t_nstat = 0, t_npend = 0;
+KStat t_stat[kMaxNames];
+KPending t_pend[kMaxPending];
+
+// resolve pending event pairs into the totals (t_mu held)
+void t_flush()
+{
+  for (int i = 0; i < t_npend; ++i) {
+    float ms = 0.f;
+    if (hipEventSynchronize(t_pend[i].b) == hipSuccess && hipEventElapsedTime(&ms, t_pend[i].a, t_pend[i].b) == hipSuccess) {
+      t_stat[t_pend[i].stat].ms += ms;
+      t_stat[t_pend[i].stat].n += 1;
+    }
+    (void)hipEventDestroy(t_pend[i].a);
+    (void)hipEventDestroy(t_pend[i].b);
+  }
+  t_npend = 0;
+}
+}  // namespace
+
+int ktime_begin(const char* name, hipStream_t s)
+{
+  std::lock_guard<std::mutex> lk(t_mu);
+  if (!t_on) return -1;
+  if (t_npend == kMaxPending) t_flush();
+  int k = 0;
+  while (k < t_nstat && strncmp(t_stat[k].name, name, sizeof t_stat[k].name - 1) != 0) ++k;
+  if (k == t_nstat) {
+    if (t_nstat == kMaxNames) return -1;
+    snprintf(t_stat[k].name, sizeof t_stat[k].name, "%s", name);
+    t_stat[k].ms = 0.0;
+    t_stat[k].n = 0;
+    ++t_nstat;
+  }
+  KPending& p = t_pend[t_npend];
+  if (hipEventCreate(&p.a) != hipSuccess) return -1;
+  if (hipEventCreate(&p.b) != hipSuccess) {
+    (void)hipEventDestroy(p.a);
+    return -1;
+  }
+  p.stat = k;
+  (void)hipEventRecord(p.a, s);
+  return t_npend++;
+}
+
+void ktime_end(int slot, hipStream_t s)
+{
+  if (slot < 0) return;
+  std::lock_guard<std::mutex> lk(t_mu);
+  if (slot < t_npend) (void)hipEventRecord(t_pend[slot].b, s);
+}
+
 }  // namespace bq
 
 // ------------------------------------------------------------------------------------
@@ -152,6 +217,28 @@ int bq_qpgf_batch(const double* qd8, int variant, int n, const double* r, double
 int bq_qpgf_v1_batch(const double* qd8, int n, const double* r, double* out)
 {
   return bq_qpgf_batch(qd8, 0, n, r, out);
+}
+
+int bq_kernel_timing(int on)
+{
+  std::lock_guard<std::mutex> lk(bq::t_mu);
+  bq::t_flush();
+  bq::t_on = on ? 1 : 0;
+  bq::t_nstat = 0;
+  return 0;
+}
+
+int bq_kernel_times(int max, char* names, double* total_ms, long long* count)
+{
+  std::lock_guard<std::mutex> lk(bq::t_mu);
+  bq::t_flush();
+  int n = bq::t_nstat < max ? bq::t_nstat : max;
+  for (int i = 0; i < n; ++i) {
+    if (names) snprintf(names + 48 * i, 48, "%s", bq::t_stat[i].name);
+    if (total_ms) total_ms[i] = bq::t_stat[i].ms;
+    if (count) count[i] = bq::t_stat[i].n;
+  }
+  return n;
 }
 
 }  // extern "C"

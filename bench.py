@@ -1,0 +1,250 @@
+"""Benchmark: solved incidences per second of the plain FP64 GPU path.
+
+One step = one solved incidence, the work the reference's d3qd2_bv_solver + example1 do for
+one plane wave (d3qd2_src/d3qd2_solve_bieq.c:36-84, d3qd2_src/d3qd2_field.c:271): assembly
+of every domain's coefficient matrices, the global solve, boundary E/H and dE/dH, then E and
+H at 100 probe points (EH_t, type 1).  The mesh and incidence stay resident; each step
+recomputes everything above from them.
+
+Default workload: the 384-quad cube-sphere of radius 0.4 um, index 1.5, in vacuum on a
+1.0 um square lattice, lambda0 = 0.6 um, x-polarised, normal incidence (theta = 1e-6 rad,
+the reference rejects theta = 0 on a lattice), 100 probes uniform in the unit cell with
+z in [-1, 1] um from default_rng(1).
+
+Prints ONE JSON line.  `roofline` describes the kernel with the largest share of device time
+(HIP events on the library stream around every launch); `cpu_baseline` times the CPU
+restatement of the same path (oracle/) on a bounded sample of rows, scaled to one incidence.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFS = 78.6
+
+# FP64 VALU flops per QPGF evaluation inside the assembly kernels, per workload, from
+# rocprofv3 --pmc SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 (x64 lanes, FMA counted twice)
+# divided by the evaluations each launch makes (profiles/k_q0_gl_c1_pmc.md), and the HBM
+# bytes per launch from FETCH_SIZE (x2.00) + WRITE_SIZE (x1.00), the factors a 1 GiB copy
+# gives for the same counters.
+FLOPS_PER_EVAL = {"c1": {"k_q0_gl": 85653, "k_q0_gh": 28986, "k_q0_sing": 56502}}
+TRAFFIC_BYTES = {"c1": {"k_q0_gl": (2815.5 * 2.0 + 73320.0) * 1024}}
+
+WORKLOADS = {
+    "c1": dict(n=8, radius=0.4, n_sphere=1.5, lam=0.6, n0=1.0, theta=1e-6, phi=0.0, pol="TM",
+               d1=(1.0, 0.0), d2=(0.0, 1.0), probes=100, seed=1,
+               label="C1: 384-quad cube-sphere r=0.4um n=1.5 in n0=1.0, square lattice 1.0um, "
+                     "lambda0=0.6um, x-pol, normal incidence, 100 probes (EH_t type 1)"),
+}
+
+
+def _probes(w, n, seed):
+    rng = np.random.default_rng(seed)
+    d1, d2 = np.array(w["d1"]), np.array(w["d2"])
+    a = rng.uniform(-0.5, 0.5, (n, 2))
+    xy = a[:, :1] * d1 + a[:, 1:] * d2
+    z = rng.uniform(-1.0, 1.0, (n, 1))
+    return np.ascontiguousarray(np.c_[xy, z])
+
+
+def _scene(w, d):
+    from bem3_emf_qd2_amd import meshgen
+    meshgen.sphere_scene(d, w["n"], w["radius"], w["n_sphere"], w["lam"], n0=w["n0"], theta=w["theta"],
+                         phi=w["phi"], pol=w["pol"], d1=w["d1"], d2=w["d2"])
+
+
+def _kernel_times(L):
+    import ctypes
+    names = ctypes.create_string_buffer(48 * 64)
+    ms = (ctypes.c_double * 64)()
+    cnt = (ctypes.c_longlong * 64)()
+    n = L.bq_kernel_times(64, names, ms, cnt)
+    out = {}
+    for i in range(n):
+        nm = names.raw[48 * i:48 * (i + 1)].split(b"\0")[0].decode()
+        out[nm] = (ms[i], cnt[i])
+    return out
+
+
+def _roofline(ktimes, stats_per_step, steps, n_rows, ne, workload):
+    """Roofline object for the kernel with the largest total device time."""
+    name = max(ktimes, key=lambda k: ktimes[k][0])
+    tot_ms, cnt = ktimes[name]
+    avg_s = tot_ms / cnt / 1e3
+    r = {"kernel": name, "bound": "mfma", "pipe": "fp64-valu", "unit": "TFLOP/s", "peak": FP64_PEAK_TFS,
+         "avg_launch_ms": avg_s * 1e3, "launches_per_step": cnt / steps, "traffic": None, "achieved": None,
+         "frac": None}
+    if name == "zgetrf_update":
+        r["note"] = "rocBLAS zgemm trailing updates of the blocked LU"
+        return r
+    ngh = stats_per_step["n_gh_pairs"]
+    evals = {"k_q0_gl": (n_rows * ne - n_rows - ngh) * 144, "k_q0_gh": ngh * 4096, "k_q0_sing": n_rows * 6144}
+    if name in evals:
+        r["evals_per_launch"] = evals[name]
+        fpe = FLOPS_PER_EVAL.get(workload, {}).get(name)
+        r["traffic"] = TRAFFIC_BYTES.get(workload, {}).get(name)
+        if fpe:
+            ach = evals[name] * fpe / avg_s / 1e12
+            r["achieved"] = ach
+            r["frac"] = ach / FP64_PEAK_TFS
+            r["flops_per_eval"] = fpe
+    return r
+
+
+def _cpu_baseline(model_dat, w, probes, gpu_solve_dims):
+    """Oracle (CPU restatement, OpenMP) on a bounded sample, scaled to one incidence."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    from datfile import read_dat
+    from oracle_pipeline import OracleModel
+    import scipy.linalg as sla
+
+    cores = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
+    d = read_dat(model_dat)
+    o = OracleModel.from_dat(d)
+    rng = np.random.default_rng(7)
+    total = 0.0
+    measured = 0.0
+    parts = []
+    for dom in range(o.MN + 1):
+        ne = o.dNe[dom]
+        rows = min(ne, 2 * cores)
+        t0 = int(rng.integers(1, ne - rows + 2))
+        t = time.perf_counter()
+        o.assemble(dom, t0, t0 + rows - 1, deriv=(dom > 0))
+        dt = time.perf_counter() - t
+        measured += dt
+        total += dt * ne / rows
+        parts.append(f"domain {dom}: {rows}/{ne} element rows in {dt:.1f}s")
+    # dense LU of the global system (order 8 NN): timed at n/4, scaled by 64
+    na = 8 * o.NN
+    n_s = na // 4
+    A = rng.standard_normal((n_s, n_s)) + 1j * rng.standard_normal((n_s, n_s))
+    t = time.perf_counter()
+    sla.lu_factor(A, check_finite=False)
+    dt = time.perf_counter() - t
+    measured += dt
+    total += dt * (na / n_s) ** 3
+    parts.append(f"LU {n_s} in {dt:.1f}s x{(na / n_s) ** 3:.0f}")
+    del A
+    # per-domain zgesv of order 4 Ne_d with 6 right-hand sides
+    for dom in range(o.MN + 1):
+        n = 4 * o.dNe[dom]
+        A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+        t = time.perf_counter()
+        sla.lu_factor(A, check_finite=False)
+        dt = time.perf_counter() - t
+        measured += dt
+        total += dt
+    parts.append("per-domain dE/dH LUs timed in full")
+    return {"value": 1.0 / total, "unit": "incidences/s", "cores": cores, "kind": "port",
+            "seconds_per_incidence": total,
+            "sample": "oracle/ (C restatement, OpenMP) on the same model: " + "; ".join(parts) +
+                      f"; element rows scaled by row count, probes not included; {measured:.0f}s measured"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="c1", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run one device-to-device copy of MIB MiB (calibrates FETCH_SIZE/WRITE_SIZE)")
+    args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        x = torch.ones(args.copy_probe * 262144, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": x.numel() * 4}))
+        return
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(independent incidences would be spread one per GPU with no exchange).", file=sys.stderr)
+        sys.exit(2)
+
+    import bem3_emf_qd2_amd as bq
+    from bem3_emf_qd2_amd.solver import Model
+    L = bq.lib()
+    if not bq.device_ok():
+        raise SystemExit(f"bench.py: no usable gfx950 device: {L.bq_last_error().decode()}")
+    w = WORKLOADS[args.workload]
+    probes = _probes(w, w["probes"], w["seed"])
+    with tempfile.TemporaryDirectory() as td:
+        _scene(w, td)
+        m = Model.from_dir(td)
+        m.write(Path(td) / "setup.dat")
+
+        def step():
+            st = m.solve()
+            E, H, did = m.fields(probes, 1)
+            return st, E, H
+
+        for _ in range(args.warmup):
+            st, E0, H0 = step()
+        L.bq_kernel_timing(1)
+        lat = []
+        t_all = time.perf_counter()
+        for _ in range(args.steps):
+            t = time.perf_counter()
+            st, E, H = step()
+            lat.append(time.perf_counter() - t)
+        elapsed = time.perf_counter() - t_all
+        kt = _kernel_times(L)
+        L.bq_kernel_timing(0)
+        stats = {"t_assemble_ms": st.t_assemble_ms, "t_solve_ms": st.t_solve_ms, "t_fields_ms": st.t_fields_ms,
+                 "n_gh_pairs": st.n_gh_pairs, "n_qpgf": st.n_qpgf}
+        ref = np.r_[E0.ravel(), H0.ravel()] if args.warmup else None
+        rep = None
+        if ref is not None:
+            got = np.r_[E.ravel(), H.ravel()]
+            rep = float(np.abs(got - ref).max() / np.abs(ref).max())
+        ne = 6 * w["n"] ** 2
+        n_rows = 4 * ne
+        roof = _roofline(kt, stats, args.steps, n_rows, ne, args.workload)
+        cpu = None if args.no_cpu_baseline else _cpu_baseline(Path(td) / "setup.dat", w, probes, None)
+        m.close()
+
+    lat_ms = np.array(lat) * 1e3
+    out = {
+        "metric": "solved incidences/sec + p50/p99 latency at <=1e-8 rel err vs FP64 plain path",
+        "value": args.steps / elapsed,
+        "unit": "incidences/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p99": float(np.percentile(lat_ms, 99))},
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (generated cube-sphere mesh and plane wave)",
+        "config": {"workload": w["label"], "quads": ne, "element_nodes": n_rows, "na": 8 * n_rows,
+                   "probes": w["probes"], "parallelism": "single GPU"},
+        "rel_err_vs_fp64": 0.0,
+        "repeat_rel_diff": rep,
+        "phase_ms": stats,
+        "kernels_ms_per_step": {k: v[0] / args.steps for k, v in sorted(kt.items(), key=lambda x: -x[1][0])},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -53,6 +53,14 @@ int bq_assemble_domain_host(void *qd, int d, double *G, double *H, double *dG, d
  * returns a pointer to {t_assemble_ms, t_solve_ms, t_fields_ms, n_gh_pairs, n_qpgf}. */
 const void *bq_solve_last_stats(void);
 
+/* Per-kernel timing (no counterpart in the reference; for benchmarks and profiling):
+ * bq_kernel_timing(1) turns on HIP-event timing of every kernel launch the library makes
+ * (on the stream it launches on) and clears the totals; bq_kernel_timing(0) turns it off.
+ * bq_kernel_times copies up to max entries: names (48 chars each), total_ms and launch
+ * count per kernel name; returns the number of entries. */
+int bq_kernel_timing(int on);
+int bq_kernel_times(int max, char *names, double *total_ms, long long *count);
+
 #ifdef __cplusplus
 }
 #endif
