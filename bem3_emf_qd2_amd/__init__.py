@@ -38,6 +38,7 @@ def _declare(L: ctypes.CDLL) -> None:
     L.bq_faddeeva_w_batch.argtypes = [ctypes.c_int, _D, _D]
     L.bq_qpgf_v1_batch.argtypes = [_D, ctypes.c_int, _D, _D]
     L.bq_qpgf_batch.argtypes = [_D, ctypes.c_int, ctypes.c_int, _D, _D]
+    L.bq_zgesv_host.argtypes = [ctypes.c_int, ctypes.c_int, _D, _D]
 
 
 def _check(rc: int, what: str) -> None:
@@ -74,3 +75,18 @@ def qpgf_v1(qd8, r, converged=False) -> np.ndarray:
     _check(lib().bq_qpgf_batch(q.ctypes.data_as(_D), int(bool(converged)), len(rr), rr.ctypes.data_as(_D),
                                out.ctypes.data_as(_D)), "qpgf")
     return out[:, 0::2] + 1j * out[:, 1::2]
+
+
+def zgesv(A, B):
+    """Solve A X = B on the GPU with the library's LU (complex double); returns X."""
+    A = np.asarray(A, complex)
+    B = np.asarray(B, complex)
+    vec = B.ndim == 1
+    B2 = B.reshape(len(B), -1)
+    n, nrhs = A.shape[0], B2.shape[1]
+    Af = np.asfortranarray(A).copy(order="F")
+    Bf = np.asfortranarray(B2).copy(order="F")
+    rc = lib().bq_zgesv_host(n, nrhs, Af.ctypes.data_as(_D), Bf.ctypes.data_as(_D))
+    if rc != 0:
+        raise RuntimeError(f"zgesv failed ({rc}): {lib().bq_last_error().decode()}")
+    return Bf[:, 0] if vec else Bf

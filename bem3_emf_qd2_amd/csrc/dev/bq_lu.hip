@@ -1,15 +1,18 @@
 // bq_lu.hip -- dense complex FP64 LU with partial pivoting and the matching solves, on
 // the device, column-major (LAPACK zgetrf/zgetrs semantics, 0-based pivots).
 //
-// Right-looking blocked LU: for each panel of nb columns
-//   1. k_panel     one 1024-thread workgroup factors the m x nb panel (pivot search by
-//                  block reduction over |re|+|im|, row swap, scale, rank-1 update),
-//   2. k_swap      applies the panel's row interchanges to the columns left and right,
-//   3. k_trsm_lu   U12 = L11^{-1} A12 (one thread per column, L11 staged in LDS),
-//   4. rocBLAS zgemm for the trailing update A22 -= L21 U12 (MFMA FP64 on gfx950).
-// Solves use the same blocking: a small triangular kernel on each diagonal block and
-// zgemm for the off-diagonal update.  This replaces the reference's MKL PARDISO and
-// LAPACKE_zgesv calls (d3qd2_src/d3qd2_mkl_solver.c:18-148, d3qd2_src/d3qd2_solve_bieq.c:753).
+// Right-looking blocked LU: for each panel of nb = 32 columns
+//   1. k_tslu       chooses the panel's nb pivot rows by tournament (partial pivoting on
+//                   256-row blocks in registers, then 8-way reductions of candidate sets);
+//                   the last level also yields the factored diagonal block,
+//   2. k_swap       applies the interchanges to every column,
+//   3. k_diag_finish  stores L11, U11 and their inverses inv(L11), inv(U11),
+//   4. k_l21        L21 = A21 inv(U11),   k_apply_blk  U12 = inv(L11) A12,
+//   5. rocBLAS zgemm for the trailing update A22 -= L21 U12 (MFMA FP64 on gfx950).
+// Solves: the pivots become one permutation (gather), then per diagonal block
+// X_j = inv(T_jj) X_j (k_apply_blk) and a zgemm for the off-diagonal update.
+// Replaces the reference's MKL PARDISO and LAPACKE_zgesv calls
+// (d3qd2_src/d3qd2_mkl_solver.c:18-148, d3qd2_src/d3qd2_solve_bieq.c:753).
 #include <rocblas/rocblas.h>
 #include "bq_runtime.hpp"
 #include "bq_complex.hpp"
@@ -18,139 +21,269 @@
 namespace {
 
 constexpr int kNB = 32;
+constexpr int kBlkElems = 2 * kNB * kNB;  // inv(L11) then inv(U11), complex, per diagonal block
 
 __device__ __forceinline__ double2 c2(cplx a) { return make_double2(a.re, a.im); }
 __device__ __forceinline__ cplx cc(double2 a) { return mk(a.x, a.y); }
 
-// Factor the panel A[j:n, j:j+jb] in place; ipiv[j..j+jb) receives global row indices.
-__global__ void __launch_bounds__(1024) k_panel(int n, double2* __restrict__ A, long lda, int j, int jb,
-                                                int* __restrict__ ipiv)
+// ---- panel: tournament pivoting ---------------------------------------------------------
+// The m x nb panel is split into blocks of 256 rows.  One 1024-thread workgroup runs partial
+// pivoting on a block held in registers -- 4 row groups of 64 rows (one row per lane) times
+// 4 column groups of 8 columns (one wavefront each) -- and keeps only the nb chosen rows
+// (leaf level).  Up to 8 candidate sets are then reduced the same way on their original rows
+// (merge levels) until nb rows remain; they become the panel's pivots in the order the last
+// reduction chose them, and that reduction's eliminated rows are the factors L11, U11 of the
+// diagonal block.  This replaces a column-by-column pivot search over the whole panel (a
+// global reduction per column) by log8(m/256) + 1 small factorizations; the pivots are those
+// of communication-avoiding LU (Grigori, Demmel, Xiang, SIAM J. Matrix Anal. Appl. 32(4), 2011).
+constexpr int kTsRows = 256;  // rows per tournament workgroup
+constexpr int kTsFan = kTsRows / kNB;  // candidate sets merged per workgroup
+
+// mode 0 (leaf): rows j + 256 blk + slot.  mode 1 (merge): slot -> set 8 blk + slot / nb.
+// cout[blk][nb] receives the chosen rows.  When gridDim.x == 1 the call is the last level:
+// F (nb x nb, column-major) receives the eliminated pivot rows (unit L below the diagonal,
+// U on and above) and ipiv[j..j+jb) the LAPACK-style interchanges for those rows.
+__global__ void __launch_bounds__(1024) k_tslu(int mode, int n, const double2* __restrict__ A, long lda, int j, int jb,
+                                               const int* __restrict__ cin, int nin, int* __restrict__ cout,
+                                               double2* __restrict__ F, int* __restrict__ ipiv)
 {
-  __shared__ double sval[1024];
-  __shared__ int sidx[1024];
+  __shared__ double vbuf[4];
+  __shared__ int ibuf[4];
   __shared__ double2 prow[kNB];
-  const int tid = threadIdx.x;
-  for (int c = 0; c < jb; ++c) {
-    const int col = j + c;
-    double2* colp = A + (long)col * lda;
-    // pivot search over rows col..n-1
-    double best = -1.0;
-    int bi = col;
-    for (int i = col + tid; i < n; i += 1024) {
-      double2 v = colp[i];
-      double a = fabs(v.x) + fabs(v.y);
-      if (a > best) { best = a; bi = i; }
-    }
-    sval[tid] = best;
-    sidx[tid] = bi;
-    __syncthreads();
-    for (int s = 512; s > 0; s >>= 1) {
-      if (tid < s) {
-        double o = sval[tid + s];
-        int oi = sidx[tid + s];
-        if (o > sval[tid] || (o == sval[tid] && oi < sidx[tid])) { sval[tid] = o; sidx[tid] = oi; }
+  __shared__ double2 lbuf[kTsRows];
+  __shared__ int order[kNB];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int g = wave >> 2, q = wave & 3;  // row group, column group
+  const int slot = g * 64 + lane;
+  int row_id = -1;
+  if (mode == 0) {
+    const int r = j + blockIdx.x * kTsRows + slot;
+    row_id = r < n ? r : -1;
+  } else {
+    const int set = kTsFan * blockIdx.x + slot / kNB, k = slot % kNB;
+    row_id = (set < nin && k < jb) ? cin[set * kNB + k] : -1;
+  }
+  cplx a[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int col = q * 8 + k;
+    a[k] = (row_id >= 0 && col < jb) ? cc(A[(long)(j + col) * lda + row_id]) : mk(0.0, 0.0);
+  }
+  bool live = row_id >= 0;
+  int mystep = -1;
+#pragma unroll
+  for (int c = 0; c < kNB; ++c) {
+    if (c < jb) {
+      const int qc = c / 8, kc = c % 8;
+      if (q == qc) {  // column owners: argmax over their 64 rows
+        double v = live ? fabs(a[kc].re) + fabs(a[kc].im) : -1.0;
+        int idx = live ? row_id : 0x7fffffff;
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+          const double ov = __shfl_xor(v, off);
+          const int oi = __shfl_xor(idx, off);
+          if (ov > v || (ov == v && oi < idx)) {
+            v = ov;
+            idx = oi;
+          }
+        }
+        if (lane == 0) {
+          vbuf[g] = v;
+          ibuf[g] = idx;
+        }
       }
       __syncthreads();
-    }
-    const int p = sidx[0];
-    if (tid == 0) ipiv[col] = p;
-    // swap rows col and p across the panel columns
-    if (p != col && tid < jb) {
-      double2* cp = A + (long)(j + tid) * lda;
-      double2 t = cp[col];
-      cp[col] = cp[p];
-      cp[p] = t;
-    }
-    __syncthreads();
-    if (tid < jb) prow[tid] = A[(long)(j + tid) * lda + col];
-    __syncthreads();
-    cplx piv = cc(prow[c]);
-    cplx inv = (piv.re == 0.0 && piv.im == 0.0) ? mk(0, 0) : cdiv(mk(1.0, 0.0), piv);
-    // scale column below the diagonal and update the rest of the panel
-    for (int i = col + 1 + tid; i < n; i += 1024) {
-      cplx l = cc(colp[i]) * inv;
-      colp[i] = c2(l);
-      for (int cc2 = c + 1; cc2 < jb; ++cc2) {
-        double2* q = A + (long)(j + cc2) * lda + i;
-        cplx u = cc(prow[cc2]);
-        *q = c2(cc(*q) - l * u);
+      double pv = vbuf[0];
+      int pi = ibuf[0];
+#pragma unroll
+      for (int h = 1; h < 4; ++h)
+        if (vbuf[h] > pv || (vbuf[h] == pv && ibuf[h] < pi)) {
+          pv = vbuf[h];
+          pi = ibuf[h];
+        }
+      if (tid == 0) order[c] = pv < 0.0 ? -1 : pi;
+      const bool me = pv >= 0.0 && live && row_id == pi;
+      if (me) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) prow[q * 8 + k] = make_double2(a[k].re, a[k].im);
+        live = false;
+        mystep = c;
+      }
+      __syncthreads();
+      if (pv >= 0.0) {
+        if (q == qc && live) {
+          const cplx piv = cc(prow[c]);
+          const cplx l = (piv.re != 0.0 || piv.im != 0.0) ? cdiv(a[kc], piv) : mk(0.0, 0.0);
+          a[kc] = l;
+          lbuf[slot] = make_double2(l.re, l.im);
+        }
+        __syncthreads();
+        if (live) {
+          const cplx l = cc(lbuf[slot]);
+#pragma unroll
+          for (int k = 0; k < 8; ++k)
+            if (q * 8 + k > c) a[k] = a[k] - l * cc(prow[q * 8 + k]);
+        }
       }
     }
-    __syncthreads();
+  }
+  __syncthreads();
+  if (tid < kNB) cout[blockIdx.x * kNB + tid] = tid < jb ? order[tid] : -1;
+  if (gridDim.x == 1) {
+    if (mystep >= 0) {
+#pragma unroll
+      for (int k = 0; k < 8; ++k) F[(q * 8 + k) * kNB + mystep] = make_double2(a[k].re, a[k].im);
+    }
+    if (tid == 0) {
+      int pos[kNB];
+      for (int c = 0; c < kNB; ++c) pos[c] = c < jb ? order[c] : -1;
+      for (int c = 0; c < jb; ++c) {
+        const int t = j + c;
+        const int p = pos[c] >= 0 ? pos[c] : t;
+        ipiv[t] = p;
+        if (p != t)
+          for (int c2 = c + 1; c2 < jb; ++c2)
+            if (pos[c2] == t) pos[c2] = p;
+      }
+    }
   }
 }
 
-// apply ipiv[j..j+jb) to columns [c0, c1)
-__global__ void k_swap(double2* __restrict__ A, long lda, int j, int jb, const int* __restrict__ ipiv, int c0, int c1)
+// Write the factored diagonal block F (from the last tournament level) into A[j, j] (rows
+// already interchanged) and store inv(L11), inv(U11) into inv (kNB x kNB each,
+// column-major, identity beyond jb).  The inverses use recursive doubling: with the inverses
+// X11, X22 of the diagonal s x s blocks of a 2s x 2s lower block, X21 = -X22 L21 X11 (upper:
+// X12 = -X11 U12 X22), one element per thread per level.
+__global__ void __launch_bounds__(1024) k_diag_finish(double2* __restrict__ A, long lda, int j, int jb,
+                                                      const double2* __restrict__ F, double2* __restrict__ inv)
 {
-  int col = c0 + blockIdx.x * blockDim.x + threadIdx.x;
-  if (col >= c1) return;
+  __shared__ double2 Lx[kNB * kNB];
+  __shared__ double2 Ux[kNB * kNB];
+  __shared__ double2 Tm[kNB * kNB];
+  const int tid = threadIdx.x, r = tid % kNB, c = tid / kNB;
+  {
+    const bool in = r < jb && c < jb;
+    const double2 f = in ? F[c * kNB + r] : make_double2(r == c ? 1.0 : 0.0, 0.0);
+    if (in) A[(long)(j + c) * lda + j + r] = f;
+    Lx[c * kNB + r] = (c < r) ? f : make_double2(c == r ? 1.0 : 0.0, 0.0);
+    cplx u = (c >= r) ? cc(f) : mk(0.0, 0.0);
+    if (r == c) u = (u.re != 0.0 || u.im != 0.0) ? cdiv(mk(1.0, 0.0), u) : mk(0.0, 0.0);
+    Ux[c * kNB + r] = c2(u);
+  }
+  __syncthreads();
+  for (int s = 1; s < kNB; s <<= 1) {
+    const int b = (r / (2 * s)) * (2 * s), bc = (c / (2 * s)) * (2 * s);
+    const bool lo = r >= b + s && c >= b && c < b + s;    // lower, 2s block at b
+    const bool up = c >= bc + s && r >= bc && r < bc + s;  // upper, 2s block at bc
+    cplx t = mk(0, 0);
+    if (lo)
+      for (int k = b; k < b + s; ++k) t = t + cc(Lx[k * kNB + r]) * cc(Lx[c * kNB + k]);  // L21 X11
+    if (up)
+      for (int k = bc + s; k < bc + 2 * s; ++k) t = t + cc(Ux[k * kNB + r]) * cc(Ux[c * kNB + k]);  // U12 X22
+    if (lo || up) Tm[c * kNB + r] = c2(t);
+    __syncthreads();
+    t = mk(0, 0);
+    if (lo)
+      for (int k = b + s; k < b + 2 * s; ++k) t = t + cc(Lx[k * kNB + r]) * cc(Tm[c * kNB + k]);  // X22 T
+    if (up)
+      for (int k = bc; k < bc + s; ++k) t = t + cc(Ux[k * kNB + r]) * cc(Tm[c * kNB + k]);  // X11 T
+    __syncthreads();
+    if (lo) Lx[c * kNB + r] = c2(mk(-t.re, -t.im));
+    if (up) Ux[c * kNB + r] = c2(mk(-t.re, -t.im));
+    __syncthreads();
+  }
+  inv[c * kNB + r] = Lx[c * kNB + r];
+  inv[kNB * kNB + c * kNB + r] = Ux[c * kNB + r];
+}
+
+// L21 = A21 inv(U11) in place for rows j+jb..n-1 of the panel; 64-row tiles in LDS
+__global__ void __launch_bounds__(256) k_l21(int n, double2* __restrict__ A, long lda, int j, int jb,
+                                             const double2* __restrict__ invU)
+{
+  __shared__ double2 Us[kNB * kNB];
+  __shared__ double2 As[64 * kNB];
+  const int tid = threadIdx.x;
+  const int r0 = j + jb + blockIdx.x * 64;
+  for (int q = tid; q < kNB * kNB; q += 256) Us[q] = invU[q];
+  for (int q = tid; q < 64 * kNB; q += 256) {
+    const int rr = q % 64, c = q / 64;
+    As[q] = (r0 + rr < n && c < jb) ? A[(long)(j + c) * lda + r0 + rr] : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  for (int q = tid; q < 64 * kNB; q += 256) {
+    const int rr = q % 64, c = q / 64;
+    if (r0 + rr >= n || c >= jb) continue;
+    cplx acc = mk(0, 0);
+#pragma unroll 8
+    for (int k = 0; k < kNB; ++k) acc = acc + cc(As[k * 64 + rr]) * cc(Us[c * kNB + k]);
+    A[(long)(j + c) * lda + r0 + rr] = c2(acc);
+  }
+}
+
+// apply the interchanges ipiv[j..j+jb) to every column
+__global__ void __launch_bounds__(256) k_swap(double2* __restrict__ A, long lda, int n, int j, int jb,
+                                              const int* __restrict__ ipiv)
+{
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= n) return;
   double2* cp = A + (long)col * lda;
   for (int r = j; r < j + jb; ++r) {
-    int p = ipiv[r];
+    const int p = ipiv[r];
     if (p != r) {
-      double2 t = cp[r];
+      const double2 t = cp[r];
       cp[r] = cp[p];
       cp[p] = t;
     }
   }
 }
 
-// B[0:jb, 0:ncol] <- L^{-1} B with L unit lower (jb x jb) at Lp (leading dim ldl)
-__global__ void k_trsm_lu(int jb, const double2* __restrict__ Lp, long ldl, double2* __restrict__ B, long ldb, int ncol)
+// B[0:jb, 0:ncol] <- Tm B for a kNB x kNB matrix Tm (column-major, identity beyond jb);
+// 64 columns per workgroup, both operands staged in LDS.
+__global__ void __launch_bounds__(256) k_apply_blk(const double2* __restrict__ Tm, int jb, double2* __restrict__ B,
+                                                   long ldb, int ncol)
 {
-  __shared__ double2 L[kNB * kNB];
-  for (int q = threadIdx.x; q < jb * jb; q += blockDim.x) {
-    int r = q % jb, c = q / jb;
-    L[c * kNB + r] = Lp[(long)c * ldl + r];
+  __shared__ double2 Ts[kNB * kNB];
+  __shared__ double2 Bs[kNB * 64];
+  const int tid = threadIdx.x, col0 = blockIdx.x * 64;
+  for (int q = tid; q < kNB * kNB; q += 256) Ts[q] = Tm[q];
+  for (int q = tid; q < kNB * 64; q += 256) {
+    const int r = q % kNB, c = q / kNB;
+    Bs[q] = (r < jb && col0 + c < ncol) ? B[(long)(col0 + c) * ldb + r] : make_double2(0.0, 0.0);
   }
   __syncthreads();
-  int col = blockIdx.x * blockDim.x + threadIdx.x;
-  if (col >= ncol) return;
-  double2* b = B + (long)col * ldb;
-  cplx x[kNB];
-  for (int r = 0; r < jb; ++r) x[r] = cc(b[r]);
-  for (int r = 0; r < jb; ++r) {
-    for (int c = 0; c < r; ++c) x[r] = x[r] - cc(L[c * kNB + r]) * x[c];
+  for (int q = tid; q < kNB * 64; q += 256) {
+    const int r = q % kNB, c = q / kNB;
+    if (r >= jb || col0 + c >= ncol) continue;
+    cplx acc = mk(0, 0);
+#pragma unroll 8
+    for (int k = 0; k < kNB; ++k) acc = acc + cc(Ts[k * kNB + r]) * cc(Bs[c * kNB + k]);
+    B[(long)(col0 + c) * ldb + r] = c2(acc);
   }
-  for (int r = 0; r < jb; ++r) b[r] = c2(x[r]);
 }
 
-// B[0:jb, :] <- U^{-1} B with U upper (non-unit) jb x jb
-__global__ void k_trsm_un(int jb, const double2* __restrict__ Up, long ldu, double2* __restrict__ B, long ldb, int ncol)
+// the interchanges ipiv[0..n) applied in order to 0..n-1 give the row permutation
+__global__ void k_ipiv_perm(int n, const int* __restrict__ ipiv, int* __restrict__ perm)
 {
-  __shared__ double2 U[kNB * kNB];
-  for (int q = threadIdx.x; q < jb * jb; q += blockDim.x) {
-    int r = q % jb, c = q / jb;
-    U[c * kNB + r] = Up[(long)c * ldu + r];
-  }
-  __syncthreads();
-  int col = blockIdx.x * blockDim.x + threadIdx.x;
-  if (col >= ncol) return;
-  double2* b = B + (long)col * ldb;
-  cplx x[kNB];
-  for (int r = 0; r < jb; ++r) x[r] = cc(b[r]);
-  for (int r = jb - 1; r >= 0; --r) {
-    for (int c = r + 1; c < jb; ++c) x[r] = x[r] - cc(U[c * kNB + r]) * x[c];
-    x[r] = cdiv(x[r], cc(U[r * kNB + r]));
-  }
-  for (int r = 0; r < jb; ++r) b[r] = c2(x[r]);
-}
-
-// row interchanges on B (n x nrhs): forward (getrs) order
-__global__ void k_laswp_rhs(int n, const int* __restrict__ ipiv, double2* __restrict__ B, long ldb, int nrhs)
-{
-  int col = blockIdx.x * blockDim.x + threadIdx.x;
-  if (col >= nrhs) return;
-  double2* b = B + (long)col * ldb;
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  for (int r = 0; r < n; ++r) perm[r] = r;
   for (int r = 0; r < n; ++r) {
-    int p = ipiv[r];
+    const int p = ipiv[r];
     if (p != r) {
-      double2 t = b[r];
-      b[r] = b[p];
-      b[p] = t;
+      const int t = perm[r];
+      perm[r] = perm[p];
+      perm[p] = t;
     }
   }
+}
+
+// Bt[r, c] = B[perm[r], c]
+__global__ void __launch_bounds__(256) k_gather_rows(int n, int nrhs, const int* __restrict__ perm,
+                                                     const double2* __restrict__ B, long ldb, double2* __restrict__ Bt)
+{
+  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (long)n * nrhs) return;
+  const int r = (int)(q % n), c = (int)(q / n);
+  Bt[(long)c * n + r] = B[(long)c * ldb + perm[r]];
 }
 
 __global__ void k_check_diag(int n, const double2* __restrict__ A, long lda, int* __restrict__ info)
@@ -215,24 +348,51 @@ int bq_zgemm_dev(void* stream, int m, int n, int k, const double* alpha, const d
   return zgemm((hipStream_t)stream, m, n, k, a, (const double2*)A, lda, (const double2*)B, ldb, b, (double2*)C, ldc);
 }
 
-// In-place LU of the n x n column-major matrix A (device).  ipiv: device int[n].
+long bq_lu_inv_doubles(int n)
+{
+  return (long)((n + kNB - 1) / kNB) * kBlkElems * 2;
+}
+
+// In-place LU of the n x n column-major matrix A (device).  ipiv: device int[n]; dinv: device
+// buffer of bq_lu_inv_doubles(n) doubles for the inverted diagonal blocks (used by getrs).
 // Returns 0, a positive k if U(k,k) is exactly zero (1-based, like LAPACK), or < 0.
-int bq_zgetrf_dev(void* stream, int n, double* Ad, long lda, int* ipiv)
+int bq_zgetrf_dev(void* stream, int n, double* Ad, long lda, int* ipiv, double* dinv)
 {
   hipStream_t s = (hipStream_t)stream;
   double2* A = (double2*)Ad;
+  double2* inv = (double2*)dinv;
   const double2 one = make_double2(1.0, 0.0), mone = make_double2(-1.0, 0.0);
+  const int nleaf = (n + kTsRows - 1) / kTsRows;
+  int* cand = nullptr;
+  double2* F = nullptr;
+  BQ_HIP(hipMallocAsync((void**)&cand, sizeof(int) * (size_t)2 * nleaf * kNB, s));
+  BQ_HIP(hipMallocAsync((void**)&F, sizeof(double2) * kNB * kNB, s));
   const int kt_all = bq::ktime_begin("zgetrf", s);
   for (int j = 0; j < n; j += kNB) {
-    int jb = min(kNB, n - j);
-    BQ_KTIME("zgetrf_panel", s, k_panel<<<1, 1024, 0, s>>>(n, A, lda, j, jb, ipiv));
+    const int jb = min(kNB, n - j);
+    double2* invj = inv + (long)(j / kNB) * kBlkElems;
+    // pivots of the panel by tournament; the last level also factors the diagonal block
+    int nset = (n - j + kTsRows - 1) / kTsRows;
+    int* cur = cand;
+    int* nxt = cand + (size_t)nleaf * kNB;
+    const int kt_p = bq::ktime_begin("zgetrf_panel", s);
+    k_tslu<<<nset, 1024, 0, s>>>(0, n, A, lda, j, jb, nullptr, 0, cur, F, ipiv);
+    while (nset > 1) {
+      const int nout = (nset + kTsFan - 1) / kTsFan;
+      k_tslu<<<nout, 1024, 0, s>>>(1, n, A, lda, j, jb, cur, nset, nxt, F, ipiv);
+      int* t = cur;
+      cur = nxt;
+      nxt = t;
+      nset = nout;
+    }
+    k_swap<<<(n + 255) / 256, 256, 0, s>>>(A, lda, n, j, jb, ipiv);
+    k_diag_finish<<<1, 1024, 0, s>>>(A, lda, j, jb, F, invj);
+    const int rest = n - j - jb;
+    if (rest > 0) k_l21<<<(rest + 63) / 64, 256, 0, s>>>(n, A, lda, j, jb, invj + kNB * kNB);
+    bq::ktime_end(kt_p, s);
     BQ_HIP(hipGetLastError());
-    if (j > 0) k_swap<<<(j + 255) / 256, 256, 0, s>>>(A, lda, j, jb, ipiv, 0, j);
-    int rest = n - j - jb;
     if (rest > 0) {
-      k_swap<<<(rest + 255) / 256, 256, 0, s>>>(A, lda, j, jb, ipiv, j + jb, n);
-      k_trsm_lu<<<(rest + 127) / 128, 128, 0, s>>>(jb, A + (long)j * lda + j, lda, A + (long)(j + jb) * lda + j, lda,
-                                                     rest);
+      k_apply_blk<<<(rest + 63) / 64, 256, 0, s>>>(invj, jb, A + (long)(j + jb) * lda + j, lda, rest);
       BQ_HIP(hipGetLastError());
       int rc = 0;
       BQ_KTIME("zgetrf_update", s,
@@ -241,11 +401,13 @@ int bq_zgetrf_dev(void* stream, int n, double* Ad, long lda, int* ipiv)
       if (rc) return -1;
     }
   }
+  bq::ktime_end(kt_all, s);
+  BQ_HIP(hipFreeAsync(cand, s));
+  BQ_HIP(hipFreeAsync(F, s));
   int* dinfo = nullptr;
   BQ_HIP(hipMallocAsync((void**)&dinfo, sizeof(int), s));
   int big = 0x7fffffff;
   BQ_HIP(hipMemcpyAsync(dinfo, &big, sizeof(int), hipMemcpyHostToDevice, s));
-  bq::ktime_end(kt_all, s);
   k_check_diag<<<(n + 255) / 256, 256, 0, s>>>(n, A, lda, dinfo);
   int info = 0;
   BQ_HIP(hipMemcpyAsync(&info, dinfo, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -255,31 +417,46 @@ int bq_zgetrf_dev(void* stream, int n, double* Ad, long lda, int* ipiv)
 }
 
 // Solve A X = B with the factors from bq_zgetrf_dev; B is n x nrhs column-major (device).
-int bq_zgetrs_dev(void* stream, int n, int nrhs, const double* Ad, long lda, const int* ipiv, double* Bd, long ldb)
+int bq_zgetrs_dev(void* stream, int n, int nrhs, const double* Ad, long lda, const int* ipiv, const double* dinv,
+                  double* Bd, long ldb)
 {
   hipStream_t s = (hipStream_t)stream;
   const double2* A = (const double2*)Ad;
+  const double2* inv = (const double2*)dinv;
   double2* B = (double2*)Bd;
   const double2 one = make_double2(1.0, 0.0), mone = make_double2(-1.0, 0.0);
   if (n <= 0 || nrhs <= 0) return 0;
-  k_laswp_rhs<<<(nrhs + 127) / 128, 128, 0, s>>>(n, ipiv, B, ldb, nrhs);
+  const int kt_all = bq::ktime_begin("zgetrs", s);
+  int* perm = nullptr;
+  double2* Bt = nullptr;
+  BQ_HIP(hipMallocAsync((void**)&perm, sizeof(int) * (size_t)n, s));
+  BQ_HIP(hipMallocAsync((void**)&Bt, sizeof(double2) * (size_t)n * nrhs, s));
+  k_ipiv_perm<<<1, 1, 0, s>>>(n, ipiv, perm);
+  const long tot = (long)n * nrhs;
+  k_gather_rows<<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(n, nrhs, perm, B, ldb, Bt);
   BQ_HIP(hipGetLastError());
-  // forward: L (unit lower)
+  BQ_HIP(hipMemcpy2DAsync(B, ldb * sizeof(double2), Bt, (size_t)n * sizeof(double2), (size_t)n * sizeof(double2),
+                          nrhs, hipMemcpyDeviceToDevice, s));
+  BQ_HIP(hipFreeAsync(Bt, s));
+  BQ_HIP(hipFreeAsync(perm, s));
+  const unsigned cblk = (unsigned)((nrhs + 63) / 64);
+  // forward: unit lower L
   for (int j = 0; j < n; j += kNB) {
-    int jb = min(kNB, n - j);
-    k_trsm_lu<<<(nrhs + 127) / 128, 128, 0, s>>>(jb, A + (long)j * lda + j, lda, B + j, ldb, nrhs);
+    const int jb = min(kNB, n - j);
+    k_apply_blk<<<cblk, 256, 0, s>>>(inv + (long)(j / kNB) * kBlkElems, jb, B + j, ldb, nrhs);
     BQ_HIP(hipGetLastError());
-    int rest = n - j - jb;
+    const int rest = n - j - jb;
     if (rest > 0 && zgemm(s, rest, nrhs, jb, mone, A + (long)j * lda + (j + jb), lda, B + j, ldb, one, B + j + jb, ldb))
       return -1;
   }
   // backward: U
   for (int j = ((n - 1) / kNB) * kNB; j >= 0; j -= kNB) {
-    int jb = min(kNB, n - j);
-    k_trsm_un<<<(nrhs + 127) / 128, 128, 0, s>>>(jb, A + (long)j * lda + j, lda, B + j, ldb, nrhs);
+    const int jb = min(kNB, n - j);
+    k_apply_blk<<<cblk, 256, 0, s>>>(inv + (long)(j / kNB) * kBlkElems + kNB * kNB, jb, B + j, ldb, nrhs);
     BQ_HIP(hipGetLastError());
     if (j > 0 && zgemm(s, j, nrhs, jb, mone, A + (long)j * lda, lda, B + j, ldb, one, B, ldb)) return -1;
   }
+  bq::ktime_end(kt_all, s);
   return 0;
 }
 
@@ -288,16 +465,17 @@ int bq_zgesv_host(int n, int nrhs, double* A, double* B)
 {
   if (bq::ensure_device()) return -1;
   hipStream_t s = bq::stream();
-  double *dA = nullptr, *dB = nullptr;
+  double *dA = nullptr, *dB = nullptr, *dv = nullptr;
   int* dp = nullptr;
   size_t an = sizeof(double2) * (size_t)n * n, bn = sizeof(double2) * (size_t)n * nrhs;
   BQ_HIP(hipMalloc((void**)&dA, an));
   BQ_HIP(hipMalloc((void**)&dB, bn));
   BQ_HIP(hipMalloc((void**)&dp, sizeof(int) * n));
+  BQ_HIP(hipMalloc((void**)&dv, sizeof(double) * bq_lu_inv_doubles(n)));
   BQ_HIP(hipMemcpyAsync(dA, A, an, hipMemcpyHostToDevice, s));
   BQ_HIP(hipMemcpyAsync(dB, B, bn, hipMemcpyHostToDevice, s));
-  int info = bq_zgetrf_dev(s, n, dA, n, dp);
-  if (info == 0) info = bq_zgetrs_dev(s, n, nrhs, dA, n, dp, dB, n);
+  int info = bq_zgetrf_dev(s, n, dA, n, dp, dv);
+  if (info == 0) info = bq_zgetrs_dev(s, n, nrhs, dA, n, dp, dv, dB, n);
   if (info == 0) {
     BQ_HIP(hipMemcpyAsync(B, dB, bn, hipMemcpyDeviceToHost, s));
     BQ_HIP(hipMemcpyAsync(A, dA, an, hipMemcpyDeviceToHost, s));
@@ -306,6 +484,7 @@ int bq_zgesv_host(int n, int nrhs, double* A, double* B)
   (void)hipFree(dA);
   (void)hipFree(dB);
   (void)hipFree(dp);
+  (void)hipFree(dv);
   return info;
 }
 

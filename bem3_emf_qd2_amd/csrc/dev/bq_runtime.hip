@@ -63,6 +63,9 @@ hipStream_t stream()
 }
 
 // ---- kernel timing ----------------------------------------------------------------
+// Each timed launch gets a ticket; its start/end events are resolved into per-name totals
+// once both are recorded.  Failed event calls are cleared from HIP's last-error state so
+// that they never surface in a later hipGetLastError() check of the library.
 namespace {
 constexpr int kMaxNames = 64, kMaxPending = 4096;
 struct KStat {
@@ -72,26 +75,36 @@ struct KStat {
 };
 struct KPending {
   int stat;
+  long long ticket;
+  bool ended;
   hipEvent_t a, b;
 };
 std::mutex t_mu;
 int t_on = 0, t_nstat = 0, t_npend = 0;
+long long t_ticket = 0;
 KStat t_stat[kMaxNames];
 KPending t_pend[kMaxPending];
 
-// resolve pending event pairs into the totals (t_mu held)
+// resolve every pending pair whose end was recorded (t_mu held)
 void t_flush()
 {
+  int keep = 0;
   for (int i = 0; i < t_npend; ++i) {
-    float ms = 0.f;
-    if (hipEventSynchronize(t_pend[i].b) == hipSuccess && hipEventElapsedTime(&ms, t_pend[i].a, t_pend[i].b) == hipSuccess) {
-      t_stat[t_pend[i].stat].ms += ms;
-      t_stat[t_pend[i].stat].n += 1;
+    KPending& p = t_pend[i];
+    if (!p.ended) {
+      t_pend[keep++] = p;
+      continue;
     }
-    (void)hipEventDestroy(t_pend[i].a);
-    (void)hipEventDestroy(t_pend[i].b);
+    float ms = 0.f;
+    if (p.stat >= 0 && hipEventSynchronize(p.b) == hipSuccess && hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+      t_stat[p.stat].ms += ms;
+      t_stat[p.stat].n += 1;
+    }
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
   }
-  t_npend = 0;
+  t_npend = keep;
+  (void)hipGetLastError();
 }
 }  // namespace
 
@@ -100,6 +113,7 @@ int ktime_begin(const char* name, hipStream_t s)
   std::lock_guard<std::mutex> lk(t_mu);
   if (!t_on) return -1;
   if (t_npend == kMaxPending) t_flush();
+  if (t_npend == kMaxPending) return -1;
   int k = 0;
   while (k < t_nstat && strncmp(t_stat[k].name, name, sizeof t_stat[k].name - 1) != 0) ++k;
   if (k == t_nstat) {
@@ -110,21 +124,40 @@ int ktime_begin(const char* name, hipStream_t s)
     ++t_nstat;
   }
   KPending& p = t_pend[t_npend];
-  if (hipEventCreate(&p.a) != hipSuccess) return -1;
+  if (hipEventCreate(&p.a) != hipSuccess) {
+    (void)hipGetLastError();
+    return -1;
+  }
   if (hipEventCreate(&p.b) != hipSuccess) {
     (void)hipEventDestroy(p.a);
+    (void)hipGetLastError();
     return -1;
   }
   p.stat = k;
-  (void)hipEventRecord(p.a, s);
-  return t_npend++;
+  p.ended = false;
+  p.ticket = t_ticket++;
+  if (hipEventRecord(p.a, s) != hipSuccess) {
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
+    (void)hipGetLastError();
+    return -1;
+  }
+  ++t_npend;
+  return (int)(p.ticket & 0x7fffffff);
 }
 
-void ktime_end(int slot, hipStream_t s)
+void ktime_end(int ticket, hipStream_t s)
 {
-  if (slot < 0) return;
+  if (ticket < 0) return;
   std::lock_guard<std::mutex> lk(t_mu);
-  if (slot < t_npend) (void)hipEventRecord(t_pend[slot].b, s);
+  for (int i = t_npend - 1; i >= 0; --i) {
+    KPending& p = t_pend[i];
+    if ((int)(p.ticket & 0x7fffffff) == ticket && !p.ended) {
+      if (hipEventRecord(p.b, s) == hipSuccess) p.ended = true;
+      else (void)hipGetLastError();
+      return;
+    }
+  }
 }
 
 }  // namespace bq
@@ -225,6 +258,8 @@ int bq_kernel_timing(int on)
   bq::t_flush();
   bq::t_on = on ? 1 : 0;
   bq::t_nstat = 0;
+  // pairs still open stay pending but no longer count
+  for (int i = 0; i < bq::t_npend; ++i) bq::t_pend[i].stat = -1;
   return 0;
 }
 

@@ -329,13 +329,14 @@ int bq_sess_solve_global(bq_sess* S, double* Xh)
   BQ_HIP(hipEventCreate(&e1));
   BQ_HIP(hipEventRecord(e0, s));
   const size_t sq = (size_t)n2 * n2 * 2, rect = (size_t)n2 * NN * 2;
-  double *M, *M3, *R, *L, *b3, *W, *v, *T;
+  double *M, *M3, *R, *L, *b3, *W, *v, *T, *inv1, *inv2;
   int *ip1, *ip2;
   double* nn3 = nullptr;
   if (dmalloc(&M, sq) || dmalloc(&M3, sq) || dmalloc(&R, (size_t)n2 * (NN + 3) * 2) || dmalloc(&L, rect) ||
       dmalloc(&b3, (size_t)n2 * 2) || dmalloc(&W, (size_t)NN * NN * 2) || dmalloc(&v, (size_t)NN * 2) ||
       dmalloc(&T, (size_t)NN * 6) || dmalloc(&ip1, (size_t)n2) || dmalloc(&ip2, (size_t)n2) ||
-      dmalloc(&nn3, (size_t)NN * 3))
+      dmalloc(&nn3, (size_t)NN * 3) || dmalloc(&inv1, (size_t)bq_lu_inv_doubles(n2)) ||
+      dmalloc(&inv2, (size_t)bq_lu_inv_doubles(n2)))
     return -1;
   // unit normals per element node e (main side)
   {
@@ -366,24 +367,24 @@ int bq_sess_solve_global(bq_sess* S, double* Xh)
     rowbase += (int)N;
   }
   const double one[2] = {1, 0}, mone[2] = {-1, 0};
-  int info = bq_zgetrf_dev(s, n2, M, n2, ip1);
+  int info = bq_zgetrf_dev(s, n2, M, n2, ip1, inv1);
   if (info) {
     bq::set_error("solve_global: block matrix M is singular (info %d)", info);
     return info > 0 ? -3 : info;
   }
-  if (bq_zgetrs_dev(s, n2, NN + 3, M, n2, ip1, R, n2)) return -1;
+  if (bq_zgetrs_dev(s, n2, NN + 3, M, n2, ip1, inv1, R, n2)) return -1;
   k_hadamard_normals<<<nblk((long)NN * NN, 256), 256, 0, s>>>(NN, R, nn3, W);
   BQ_HIP(hipGetLastError());
   if (bq_zgemm_op_dev(s, 0, 0, n2, NN, NN, mone, L, n2, W, NN, one, M3, n2)) return -1;
   k_normal_dot<<<nblk(NN, 256), 256, 0, s>>>(NN, R, nn3, v);
   BQ_HIP(hipGetLastError());
   if (bq_zgemm_op_dev(s, 0, 0, n2, 1, NN, mone, L, n2, v, NN, one, b3, n2)) return -1;
-  info = bq_zgetrf_dev(s, n2, M3, n2, ip2);
+  info = bq_zgetrf_dev(s, n2, M3, n2, ip2, inv2);
   if (info) {
     bq::set_error("solve_global: Schur complement is singular (info %d)", info);
     return info > 0 ? -3 : info;
   }
-  if (bq_zgetrs_dev(s, n2, 1, M3, n2, ip2, b3, n2)) return -1;
+  if (bq_zgetrs_dev(s, n2, 1, M3, n2, ip2, inv2, b3, n2)) return -1;
   k_scale_normals<<<nblk(NN, 256), 256, 0, s>>>(NN, b3, nn3, T);
   BQ_HIP(hipGetLastError());
   // y_c -= Z T_c for c = 0..2 (columns NN..NN+2 of R)
@@ -405,7 +406,7 @@ int bq_sess_solve_global(bq_sess* S, double* Xh)
       Xh[2 * ((size_t)(c + 4) * NN + e) + 1] = y[2 * (NN + e) + 1];
     }
   for (void* p : {(void*)M, (void*)M3, (void*)R, (void*)L, (void*)b3, (void*)W, (void*)v, (void*)T, (void*)ip1,
-                  (void*)ip2, (void*)nn3})
+                  (void*)ip2, (void*)nn3, (void*)inv1, (void*)inv2})
     (void)hipFree(p);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
@@ -439,27 +440,28 @@ int bq_sess_deh(bq_sess* S, int d, const double* EH, double* dEH)
 {
   hipStream_t s = S->s;
   const long N = 4L * S->dNe[d];
-  double *eh, *b, *gt;
+  double *eh, *b, *gt, *inv;
   int* ip;
   if (upload(&eh, EH, (size_t)N * 12, s) || dmalloc(&b, (size_t)N * 12) || dmalloc(&gt, (size_t)N * N * 2) ||
-      dmalloc(&ip, (size_t)N))
+      dmalloc(&ip, (size_t)N) || dmalloc(&inv, (size_t)bq_lu_inv_doubles((int)N)))
     return -1;
   const double one[2] = {1, 0}, zero[2] = {0, 0};
   if (bq_zgemm_op_dev(s, 1, 0, (int)N, 6, (int)N, one, S->H[d], N, eh, N, zero, b, N)) return -1;
   k_transpose<<<nblk(N * N, 256), 256, 0, s>>>(N, S->G[d], gt);
   BQ_HIP(hipGetLastError());
-  int info = bq_zgetrf_dev(s, (int)N, gt, N, ip);
+  int info = bq_zgetrf_dev(s, (int)N, gt, N, ip, inv);
   if (info) {
     bq::set_error("solve_deh: G of domain %d is singular (info %d)", d, info);
     return info > 0 ? -3 : info;
   }
-  if (bq_zgetrs_dev(s, (int)N, 6, gt, N, ip, b, N)) return -1;
+  if (bq_zgetrs_dev(s, (int)N, 6, gt, N, ip, inv, b, N)) return -1;
   BQ_HIP(hipMemcpyAsync(dEH, b, (size_t)N * 6 * 16, hipMemcpyDeviceToHost, s));
   BQ_HIP(hipStreamSynchronize(s));
   (void)hipFree(eh);
   (void)hipFree(b);
   (void)hipFree(gt);
   (void)hipFree(ip);
+  (void)hipFree(inv);
   return 0;
 }
 

@@ -3,7 +3,10 @@
 tests/golden/sample3/ex3.dat and tests/golden/verification/veri.dat are result files the
 reference ships (analysis_sample3/, verification/): a 384-quad cube-sphere (r = 0.125 um,
 n = 1.6) in water (n0 = 1.33) on hexagonal lattices of 0.75 and 2.0 um, lambda0 = 1 um,
-theta = 0.2.  They hold the model and the reference's surface solution U, dU, E, dE, H, dH.
+theta = 0.2.  tests/golden/sphere_m2/ex.dat is the reference's top-level result: a two-layer
+sphere (1086 quads; shell n = 1.6, metallic core n = 0.06 + 4i, so MN = 2 and three domains)
+on a 0.75 um square lattice.  They hold the model and the reference's surface solution
+U, dU, E, dE, H, dH.
 The GPU solve of the same model must reproduce them to 1e-10 (max |diff| / max |ref| per
 array); the oracle reproduces ex3.dat to 4e-12 (tests/oracle_pipeline.py, slow test below).
 example1_out.txt (the output example1 printed, as shipped in example1.log) holds EH_t at r = (0.3, 0, 1.1) for the 4- and 9-point rules.
@@ -44,7 +47,7 @@ def _compare(sol, ref, tol):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("case", ["sample3/ex3.dat", "verification/veri.dat"])
+@pytest.mark.parametrize("case", ["sample3/ex3.dat", "verification/veri.dat", "sphere_m2/ex.dat"])
 def test_gpu_solve_matches_reference_dat(tmp_path, gpu_lib, case):
     from bem3_emf_qd2_amd.solver import Model
     ref = read_dat(G / case)
@@ -58,11 +61,11 @@ def test_gpu_solve_matches_reference_dat(tmp_path, gpu_lib, case):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("case", ["sample3", "verification"])
+@pytest.mark.parametrize("case", ["sample3", "verification", "sphere_m2"])
 def test_gpu_fields_on_reference_solution(gpu_lib, case):
     """EH_t on the reference's own solution vs the output of the reference example1 (15 digits)."""
     from bem3_emf_qd2_amd.solver import Model
-    dat = G / case / ("ex3.dat" if case == "sample3" else "veri.dat")
+    dat = G / case / {"sample3": "ex3.dat", "verification": "veri.dat", "sphere_m2": "ex.dat"}[case]
     exp = _example1(G / case / "example1_out.txt")
     m = Model.from_dat(dat)
     for t in (0, 1):
