@@ -28,7 +28,23 @@ typedef struct bq_qplan {
   double fd_bN, fd_bNp, fd_bNm; /* Fourier-domain selection data (bNp/bNm = 0 -> Ewald only) */
   int fd_allowed;
   double eps;              /* relative accuracy used in the Fourier/Ewald choice */
+  double ph2re, ph2im;     /* exp(i vk . d2): Bloch phase step along d2 */
+  double cell, rcell;      /* |det(d1,d2)| and |det(g1,g2)| (unit-cell areas) */
 } bq_qplan;
+
+/* Relative cost of one lattice term of each kind, for choosing the Fourier-domain or the
+ * Ewald sum per point (both converge to the same value): Fourier-domain term 1, Ewald
+ * spectral term 2.1, Ewald spatial term (one Faddeeva evaluation) 6.1, as timed on MI355X
+ * for the QPGF kernel on 2^20-point batches. */
+#ifndef BQ_COST_FD
+#define BQ_COST_FD 1.0
+#endif
+#ifndef BQ_COST_SPEC
+#define BQ_COST_SPEC 2.1
+#endif
+#ifndef BQ_COST_SPAT
+#define BQ_COST_SPAT 6.1
+#endif
 
 /* Fill a plan from the reference's QPD2 fields (k, vk[3], vd1[2], vd2[2]).
  * Returns 0, or -1 if the lattice is degenerate. */

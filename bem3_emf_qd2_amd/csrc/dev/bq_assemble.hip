@@ -110,33 +110,51 @@ __global__ void __launch_bounds__(256) k_classify(const double* __restrict__ E, 
 }
 
 // ------------------------------------------------------------------------------------
-// domain 0 (periodic), regular pairs, 12x12 rule
-__global__ void __launch_bounds__(256) k_q0_gl(bq_qplan P, const double* __restrict__ E, const double* __restrict__ ren,
-                                               const int* __restrict__ sid, int dNe, const unsigned char* __restrict__ flag,
-                                               double2* __restrict__ G, double2* __restrict__ H)
+// domain 0 (periodic), regular pairs, 12x12 rule.  The per-lane sums live in LDS rather than
+// registers: the lattice sums need nearly 256 registers, and keeping the kernel within 256
+// lets two wavefronts share each SIMD.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_q0_gl(bq_qplan P, const double* __restrict__ E, const double* __restrict__ ren, const int* __restrict__ sid, int dNe,
+        const unsigned char* __restrict__ flag, double2* __restrict__ G, double2* __restrict__ H)
 {
-  const int lane = threadIdx.x & 63;
-  long pair = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  __shared__ double sacc[16][256];
+  __shared__ double selem[4][kElemStride];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  // one wavefront per (t, s): make the pair index wave-uniform so that the element data of
+  // s are fetched with scalar loads and held in scalar registers
+  const long pair = (long)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   if (pair >= (long)dNe * dNe) return;
   const int t = (int)(pair / dNe) + 1, s = (int)(pair % dNe) + 1;
   if (t == s) return;
   const int tn = lane >> 4, sub = lane & 15;
   const long N = 4L * dNe, row = 4L * (t - 1) + tn;
   const bool active = flag[row * dNe + (s - 1)] == 0;
-  ColPt c;
-  colloc(E, ren, abs(sid[t]), tn, c);
-  Elem e;
-  load_elem(E, abs(sid[s]), e);
-  double acc[16];
+  double* acc = &sacc[0][threadIdx.x];  // acc[k * 256]
 #pragma unroll
-  for (int k = 0; k < 16; ++k) acc[k] = 0.0;
+  for (int k = 0; k < 16; ++k) acc[k * 256] = 0.0;
+  // element s staged in LDS and read back per point, so it is not held across the sums
+  if (lane < kElemStride) selem[wv][lane] = E[(size_t)abs(sid[s]) * kElemStride + lane];
+  __builtin_amdgcn_s_waitcnt(0);
+  __builtin_amdgcn_wave_barrier();
   if (active) {
+    double rt[3];
+    {
+      const double* p = ren + (size_t)abs(sid[t]) * 12 + 3 * tn;
+      rt[0] = p[0];
+      rt[1] = p[1];
+      rt[2] = p[2];
+    }
+#pragma unroll 1
     for (int m = 0; m < 9; ++m) {
       int p = sub + 16 * m, i = p / 12, j = p - 12 * i;
       double z = c_xl[i], et = c_xl[j], w = c_wl[i] * c_wl[j];
       double R[3], W[3], Nn[4];
-      rw_at(e, z, et, R, W);
-      R[0] -= c.rt[0]; R[1] -= c.rt[1]; R[2] -= c.rt[2];
+      {
+        Elem e;
+        load_elem(&selem[0][0], wv, e);
+        rw_at(e, z, et, R, W);
+      }
+      R[0] -= rt[0]; R[1] -= rt[1]; R[2] -= rt[2];
       QG q;
       qpgf_v1(P, R[0], R[1], R[2], q);
       double aW = sqrt(dot3(W, W));
@@ -145,28 +163,33 @@ __global__ void __launch_bounds__(256) k_q0_gl(bq_qplan P, const double* __restr
 #pragma unroll
       for (int n = 0; n < 4; ++n) {
         double gw = w * Nn[n] * aW, hw = w * Nn[n];
-        acc[2 * n] += gw * q.g.re;
-        acc[2 * n + 1] += gw * q.g.im;
-        acc[8 + 2 * n] += hw * hn.re;
-        acc[8 + 2 * n + 1] += hw * hn.im;
+        acc[(2 * n) * 256] += gw * q.g.re;
+        acc[(2 * n + 1) * 256] += gw * q.g.im;
+        acc[(8 + 2 * n) * 256] += hw * hn.re;
+        acc[(8 + 2 * n + 1) * 256] += hw * hn.im;
       }
     }
   }
-  group_sum<16, 16>(acc);
+  double a[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) a[k] = acc[k * 256];
+  group_sum<16, 16>(a);
   if (active && sub < 2) {
     double2* dst = (sub == 0 ? G : H) + row * N + 4L * (s - 1);
     const int o = sub * 8;
 #pragma unroll
-    for (int n = 0; n < 4; ++n) dst[n] = make_double2(acc[o + 2 * n], acc[o + 2 * n + 1]);
+    for (int n = 0; n < 4; ++n) dst[n] = make_double2(a[o + 2 * n], a[o + 2 * n + 1]);
   }
 }
 
-// domain 0, flagged pairs: 64x64 rule, F and dF/dt recomputed with it
-__global__ void __launch_bounds__(256) k_q0_gh(bq_qplan P, const double* __restrict__ E, const double* __restrict__ ren,
-                                               const int* __restrict__ sid, int dNe, const int* __restrict__ list,
-                                               double2* __restrict__ G, double2* __restrict__ H, double* __restrict__ F3)
+// domain 0, flagged pairs: 64x64 rule, F and dF/dt recomputed with it (lattice-sum
+// accumulators in LDS, two wavefronts per SIMD, as in k_q0_gl)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_q0_gh(bq_qplan P, const double* __restrict__ E, const double* __restrict__ ren, const int* __restrict__ sid, int dNe,
+        const int* __restrict__ list, double2* __restrict__ G, double2* __restrict__ H, double* __restrict__ F3)
 {
   __shared__ double lds[19 * 4];
+  __shared__ double sacc[16][256];
   const long code = list[blockIdx.x];
   const long N = 4L * dNe, row = code / dNe;
   const int sc = (int)(code % dNe), t = (int)(row / 4) + 1, tn = (int)(row & 3);
@@ -174,9 +197,11 @@ __global__ void __launch_bounds__(256) k_q0_gh(bq_qplan P, const double* __restr
   colloc(E, ren, abs(sid[t]), tn, c);
   Elem e;
   load_elem(E, abs(sid[sc + 1]), e);
-  double acc[19];
+  double* acc = &sacc[0][threadIdx.x];
 #pragma unroll
-  for (int k = 0; k < 19; ++k) acc[k] = 0.0;
+  for (int k = 0; k < 16; ++k) acc[k * 256] = 0.0;
+  double fs[3] = {0.0, 0.0, 0.0};
+#pragma unroll 1
   for (int m = 0; m < 16; ++m) {
     int p = threadIdx.x + 256 * m, i = p >> 6, j = p & 63;
     double z = c_xh[i], et = c_xh[j], w = c_wh[i] * c_wh[j];
@@ -184,6 +209,9 @@ __global__ void __launch_bounds__(256) k_q0_gh(bq_qplan P, const double* __restr
     rw_at(e, z, et, R, W);
     R[0] -= c.rt[0]; R[1] -= c.rt[1]; R[2] -= c.rt[2];
     f_terms(R, W, c, f, f0, f1);
+    fs[0] += w * f;
+    fs[1] += w * f0;
+    fs[2] += w * f1;
     QG q;
     qpgf_v1(P, R[0], R[1], R[2], q);
     double aW = sqrt(dot3(W, W));
@@ -192,28 +220,31 @@ __global__ void __launch_bounds__(256) k_q0_gh(bq_qplan P, const double* __restr
 #pragma unroll
     for (int n = 0; n < 4; ++n) {
       double gw = w * Nn[n] * aW, hw = w * Nn[n];
-      acc[2 * n] += gw * q.g.re;
-      acc[2 * n + 1] += gw * q.g.im;
-      acc[8 + 2 * n] += hw * hn.re;
-      acc[8 + 2 * n + 1] += hw * hn.im;
+      acc[(2 * n) * 256] += gw * q.g.re;
+      acc[(2 * n + 1) * 256] += gw * q.g.im;
+      acc[(8 + 2 * n) * 256] += hw * hn.re;
+      acc[(8 + 2 * n + 1) * 256] += hw * hn.im;
     }
-    acc[16] += w * f;
-    acc[17] += w * f0;
-    acc[18] += w * f1;
   }
-  block_sum256<19>(acc, lds);
+  double a[19];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) a[k] = acc[k * 256];
+  a[16] = fs[0];
+  a[17] = fs[1];
+  a[18] = fs[2];
+  block_sum256<19>(a, lds);
   if (threadIdx.x == 0) {
     double2* g = G + row * N + 4L * sc;
     double2* h = H + row * N + 4L * sc;
 #pragma unroll
     for (int n = 0; n < 4; ++n) {
-      g[n] = make_double2(acc[2 * n], acc[2 * n + 1]);
-      h[n] = make_double2(acc[8 + 2 * n], acc[8 + 2 * n + 1]);
+      g[n] = make_double2(a[2 * n], a[2 * n + 1]);
+      h[n] = make_double2(a[8 + 2 * n], a[8 + 2 * n + 1]);
     }
     double* o = F3 + 3 * code;
-    o[0] = acc[16] * kI4PI;
-    o[1] = acc[17] * kI4PI;
-    o[2] = acc[18] * kI4PI;
+    o[0] = a[16] * kI4PI;
+    o[1] = a[17] * kI4PI;
+    o[2] = a[18] * kI4PI;
   }
 }
 
@@ -241,12 +272,14 @@ __device__ double polar_F_part(const Elem& e, double zt, double et)
   return F;
 }
 
-// domain 0, singular pairs: one block per collocation node
-__global__ void __launch_bounds__(256) k_q0_sing(bq_qplan P, const double* __restrict__ E, const int* __restrict__ sid,
-                                                 int dNe, double2* __restrict__ G, double2* __restrict__ H,
-                                                 double* __restrict__ F3)
+// domain 0, singular pairs: one block per collocation node (lattice-sum accumulators in
+// LDS, two wavefronts per SIMD, as in k_q0_gl)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_q0_sing(bq_qplan P, const double* __restrict__ E, const int* __restrict__ sid, int dNe, double2* __restrict__ G,
+          double2* __restrict__ H, double* __restrict__ F3)
 {
   __shared__ double lds[21 * 4];
+  __shared__ double sacc[16][256];
   const long N = 4L * dNe, row = blockIdx.x;
   const int t = (int)(row / 4) + 1, tn = (int)(row & 3), as = abs(sid[t]);
   Elem e;
@@ -254,9 +287,9 @@ __global__ void __launch_bounds__(256) k_q0_sing(bq_qplan P, const double* __res
   const double zt = node_z(tn), et = node_e(tn);
   double r[5], th[5];
   corner_r_th(zt, et, r, th);
-  double acc[21];
+  double* sa = &sacc[0][threadIdx.x];
 #pragma unroll
-  for (int k = 0; k < 21; ++k) acc[k] = 0.0;
+  for (int k = 0; k < 16; ++k) sa[k * 256] = 0.0;
   for (int q = threadIdx.x; q < 512; q += 256) {
     int j = q >> 7, m = q & 127;
     double theta, wt;
@@ -288,13 +321,16 @@ __global__ void __launch_bounds__(256) k_q0_sing(bq_qplan P, const double* __res
 #pragma unroll
       for (int n = 0; n < 4; ++n) {
         double gw = w * Nn[n] * aW * rr, hw = w * Nn[n] * rr;
-        acc[2 * n] += gw * qq.g.re;
-        acc[2 * n + 1] += gw * qq.g.im;
-        acc[8 + 2 * n] += hw * hn.re;
-        acc[8 + 2 * n + 1] += hw * hn.im;
+        sa[(2 * n) * 256] += gw * qq.g.re;
+        sa[(2 * n + 1) * 256] += gw * qq.g.im;
+        sa[(8 + 2 * n) * 256] += hw * hn.re;
+        sa[(8 + 2 * n + 1) * 256] += hw * hn.im;
       }
     }
   }
+  double acc[21];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) acc[k] = sa[k * 256];
   // F at the node and at the four shifted points (central differences, CDFH)
   acc[16] = polar_F_part(e, zt, et);
   acc[17] = polar_F_part(e, zt + kCDFH, et);
@@ -381,7 +417,9 @@ __global__ void __launch_bounds__(256) k_fs_gl(double kr, double ki, const doubl
                                                double2* __restrict__ dH)
 {
   const int lane = threadIdx.x & 63;
-  long pair = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  // one wavefront per (t, s): make the pair index wave-uniform so that the element data of
+  // s are fetched with scalar loads and held in scalar registers
+  const long pair = (long)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   if (pair >= (long)dNe * dNe) return;
   const int t = (int)(pair / dNe) + 1, s = (int)(pair % dNe) + 1;
   if (t == s) return;

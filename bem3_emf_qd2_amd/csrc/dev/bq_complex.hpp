@@ -44,16 +44,53 @@ __device__ __forceinline__ cplx cdiv(cplx a, cplx b)
   double r = b.re / b.im, d = b.im + b.re * r;
   return cplx{(a.re * r + a.im) / d, (a.im * r - a.re) / d};
 }
+// sin and cos: x = k pi/2 + r with pi/2 split into three doubles (P1 + P2 + P3, rounded
+// from the exact value).  x - k P1 is exact under fma for 1 <= |x| < 2^52 because both
+// operands are multiples of 2^-52 and the difference is below 2, so r keeps full relative
+// accuracy next to multiples of pi/2.  sin r and cos r by their Taylor series to r^19 / r^18
+// on |r| <= pi/4 (truncation below 1e-20).  The quadrant comes from k mod 4 in floating point.
+__device__ __forceinline__ void sincos_fast(double x, double* s, double* c)
+{
+  const double k = rint(x * 0x1.45f306dc9c883p-1);  // x 2/pi
+  double r = fma(-k, 0x1.921fb54442d18p+0, x);
+  r = fma(-k, 0x1.1a62633145c07p-54, r);
+  r = fma(-k, -0x1.f1976b7ed8fbcp-110, r);
+  const double r2 = r * r;
+  double ps = -0x1.2f49b46814157p-57;             // -1/19!
+  ps = fma(ps, r2, 0x1.952c77030ad4ap-49);         //  1/17!
+  ps = fma(ps, r2, -0x1.ae7f3e733b81fp-41);        // -1/15!
+  ps = fma(ps, r2, 0x1.6124613a86d09p-33);         //  1/13!
+  ps = fma(ps, r2, -0x1.ae64567f544e4p-26);        // -1/11!
+  ps = fma(ps, r2, 0x1.71de3a556c734p-19);         //  1/9!
+  ps = fma(ps, r2, -0x1.a01a01a01a01ap-13);        // -1/7!
+  ps = fma(ps, r2, 0x1.1111111111111p-7);          //  1/5!
+  ps = fma(ps, r2, -0x1.5555555555555p-3);         // -1/3!
+  const double sr = fma(ps * r2, r, r);
+  double pc = -0x1.6827863b97d97p-53;              // -1/18!
+  pc = fma(pc, r2, 0x1.ae7f3e733b81fp-45);         //  1/16!
+  pc = fma(pc, r2, -0x1.93974a8c07c9dp-37);        // -1/14!
+  pc = fma(pc, r2, 0x1.1eed8eff8d898p-29);         //  1/12!
+  pc = fma(pc, r2, -0x1.27e4fb7789f5cp-22);        // -1/10!
+  pc = fma(pc, r2, 0x1.a01a01a01a01ap-16);         //  1/8!
+  pc = fma(pc, r2, -0x1.6c16c16c16c17p-10);        // -1/6!
+  pc = fma(pc, r2, 0x1.5555555555555p-5);          //  1/4!
+  pc = fma(pc, r2, -0.5);                          // -1/2!
+  const double cr = fma(pc, r2, 1.0);
+  const int q = (int)(k - 4.0 * floor(k * 0.25));
+  const double sv = (q & 1) ? cr : sr, cv = (q & 1) ? sr : cr;
+  *s = (q & 2) ? -sv : sv;
+  *c = ((q + 1) & 2) ? -cv : cv;
+}
 __device__ __forceinline__ cplx cexpi(double t)
 {
   double s, c;
-  sincos(t, &s, &c);
+  sincos_fast(t, &s, &c);
   return cplx{c, s};
 }
 __device__ __forceinline__ cplx cexp(cplx z)
 {
   double e = exp(z.re), s, c;
-  sincos(z.im, &s, &c);
+  sincos_fast(z.im, &s, &c);
   return cplx{e * c, e * s};
 }
 __device__ __forceinline__ cplx ccos(cplx z)

@@ -47,11 +47,56 @@ __device__ __forceinline__ cplx w_cf_upper(double x, double y)
   return r;
 }
 
-__device__ cplx faddeeva_w(double xs, double y)
+// Series of Algorithm 916 for |y| <= 7 and x = |xs| < 10 outside the continued-fraction region.
+__device__ __forceinline__ cplx w_series(double xs, double y)
+{
+  const double x = fabs(xs);
+  double s1 = 0.0, s2 = 0.0, s3 = 0.0, s4 = 0.0, s5 = 0.0;
+  const double ex2 = exp(-x * x);
+  const double e2ax = exp(2.0 * kFA * x), em2ax = 1.0 / e2ax;
+  const int nterm = min(60, (int)((x + 6.3) * (1.0 / kFA)) + 2);
+  double t = exp(-kFA2), q = exp(-3.0 * kFA2);  // exp(-a^2 n^2), exp(-a^2 (2n+1))
+  const double q2 = exp(-2.0 * kFA2);
+  double p2 = 1.0, pm2 = 1.0, y2 = y * y;
+#pragma unroll 1
+  for (int n = 1; n <= nterm; ++n) {
+    double an = kFA * n;
+    double coef = t * ex2 * rcp_nr(fma(an, an, y2));
+    p2 *= e2ax;
+    pm2 *= em2ax;
+    double cm = coef * pm2, cp = coef * p2;
+    s1 += coef;
+    s2 += cm;
+    s4 = fma(cm, an, s4);
+    s3 += cp;
+    s5 = fma(cp, an, s5);
+    t *= q;
+    q *= q2;
+  }
+  const double ex2erfcx = (y > -6.0) ? ex2 * erfcx(y) : 2.0 * exp(y2 - x * x);
+  double sxy, cxy, s2xy, c2xy;
+  sincos_fast(xs * y, &sxy, &cxy);
+  s2xy = 2.0 * sxy * cxy;
+  c2xy = fma(-2.0 * sxy, sxy, 1.0);
+  const double coef1 = ex2erfcx - kFC * y * s1;
+  const double coef2 = kFC * xs * ex2;
+  double re = coef1 * c2xy + coef2 * sxy * sinc_of(xs * y, sxy) + 0.5 * kFC * y * (s2 + s3);
+  double im = coef2 * sinc_of(2.0 * xs * y, s2xy) - coef1 * s2xy + copysign(0.5 * kFC * (s5 - s4), xs);
+  return mk(re, im);
+}
+
+__device__ __forceinline__ bool w_use_cf(double x, double ya)
+{
+  return ya > 7.0 || (x > 6.0 && (ya > 0.1 || (x > 8.0 && ya > 1e-10) || x > 28.0));
+}
+
+// The rarely taken regions (continued fraction; x >= 10 with tiny |y|; x = 0), kept out of
+// line so that the common series path does not carry their registers.
+__device__ __attribute__((noinline)) cplx w_cold(double xs, double y)
 {
   const double x = fabs(xs), ya = fabs(y);
   if (xs == 0.0) return mk(erfcx(y), 0.0);
-  if (ya > 7.0 || (x > 6.0 && (ya > 0.1 || (x > 8.0 && ya > 1e-10) || x > 28.0))) {
+  if (w_use_cf(x, ya)) {
     double xr = (y < 0.0) ? -xs : xs;
     cplx w = w_cf_upper(xr, ya);
     if (y < 0.0) {
@@ -60,43 +105,10 @@ __device__ cplx faddeeva_w(double xs, double y)
     }
     return w;
   }
-  double s1 = 0.0, s2 = 0.0, s3 = 0.0, s4 = 0.0, s5 = 0.0;
-  if (x < 10.0) {
-    const double ex2 = exp(-x * x);
-    const double e2ax = exp(2.0 * kFA * x), em2ax = 1.0 / e2ax;
-    const int nterm = min(60, (int)((x + 6.3) * (1.0 / kFA)) + 2);
-    double t = exp(-kFA2), q = exp(-3.0 * kFA2);       // exp(-a^2 n^2), exp(-a^2 (2n+1))
-    const double q2 = exp(-2.0 * kFA2);
-    double p2 = 1.0, pm2 = 1.0, y2 = y * y;
-    for (int n = 1; n <= nterm; ++n) {
-      double an = kFA * n;
-      double coef = t * ex2 * rcp_nr(fma(an, an, y2));
-      p2 *= e2ax;
-      pm2 *= em2ax;
-      double cm = coef * pm2, cp = coef * p2;
-      s1 += coef;
-      s2 += cm;
-      s4 = fma(cm, an, s4);
-      s3 += cp;
-      s5 = fma(cp, an, s5);
-      t *= q;
-      q *= q2;
-    }
-    const double ex2erfcx = (y > -6.0) ? ex2 * erfcx(y) : 2.0 * exp(y2 - x * x);
-    double sxy, cxy, s2xy, c2xy;
-    sincos(xs * y, &sxy, &cxy);
-    s2xy = 2.0 * sxy * cxy;
-    c2xy = fma(-2.0 * sxy, sxy, 1.0);
-    const double coef1 = ex2erfcx - kFC * y * s1;
-    const double coef2 = kFC * xs * ex2;
-    double re = coef1 * c2xy + coef2 * sxy * sinc_of(xs * y, sxy) + 0.5 * kFC * y * (s2 + s3);
-    double im = coef2 * sinc_of(2.0 * xs * y, s2xy) - coef1 * s2xy + copysign(0.5 * kFC * (s5 - s4), xs);
-    return mk(re, im);
-  }
   // x >= 10, |y| tiny: series centred on n0 = x/a
   double n0 = floor(x * (1.0 / kFA) + 0.5), dx = kFA * n0 - x;
-  s3 = exp(-dx * dx) / fma(kFA2 * n0, n0, y * y);
-  s5 = kFA * n0 * s3;
+  double s3 = exp(-dx * dx) / fma(kFA2 * n0, n0, y * y);
+  double s5 = kFA * n0 * s3;
   for (int dn = 1; dn < 40; ++dn) {
     double np = n0 + dn, nm = n0 - dn;
     double tp = exp(-(kFA * dn + dx) * (kFA * dn + dx)) / fma(kFA2 * np, np, y * y);
@@ -105,6 +117,13 @@ __device__ cplx faddeeva_w(double xs, double y)
     s5 += kFA * (np * tp + nm * tm);
   }
   return mk(exp(-x * x) + 0.5 * kFC * y * s3, 0.5 * kFC * copysign(s5, xs));
+}
+
+__device__ __forceinline__ cplx faddeeva_w(double xs, double y)
+{
+  const double x = fabs(xs), ya = fabs(y);
+  if (xs == 0.0 || x >= 10.0 || w_use_cf(x, ya)) return w_cold(xs, y);
+  return w_series(xs, y);
 }
 
 }  // namespace bq

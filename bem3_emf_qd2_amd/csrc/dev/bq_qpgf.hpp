@@ -42,6 +42,32 @@ __device__ __forceinline__ bool row_range(double ux, double uy, double ex, doubl
   return lo <= hi;
 }
 
+// Visit the integers lo..hi of a lattice row starting at c (lo <= c <= hi) and walking out
+// both ways, with E = exp(i phase(n)) carried by multiplication with step = exp(i dphase)
+// from E0 = exp(i phase(c)); starting at the row's centre keeps the carried chains short
+// and their rounding smallest where the terms are largest.
+template <class Fn>
+__device__ __forceinline__ void row_walk(int lo, int hi, int c, cplx E0, cplx step, Fn&& f)
+{
+  cplx E = E0;
+  for (int n = c; n <= hi; ++n) {
+    f(n, E);
+    E = E * step;
+  }
+  const cplx back = conj(step);
+  E = E0 * back;
+  for (int n = c - 1; n >= lo; --n) {
+    f(n, E);
+    E = E * back;
+  }
+}
+
+__device__ __forceinline__ int row_centre(double ue, double e2, int lo, int hi)
+{
+  const int c = (int)rint(-ue / e2);
+  return c < lo ? lo : (c > hi ? hi : c);
+}
+
 // Fourier-domain sum for the folded point (px, py, pz), pz != 0.
 __device__ void qpgf_fourier(const bq_qplan& P, double px, double py, double pz, QG& o)
 {
@@ -52,30 +78,33 @@ __device__ void qpgf_fourier(const bq_qplan& P, double px, double py, double pz,
   double d1n = sqrt(P.d1x * P.d1x + P.d1y * P.d1y);
   double c1 = (P.vk0 * P.d1x + P.vk1 * P.d1y) / (2.0 * kPI), w1 = A * d1n / (2.0 * kPI);
   int n1lo = (int)ceil(c1 - w1), n1hi = (int)floor(c1 + w1);
-  double g22 = P.g2x * P.g2x + P.g2y * P.g2y;
+  const double g22 = P.g2x * P.g2x + P.g2y * P.g2y;
+  const double gp1 = P.g1x * px + P.g1y * py, gp2 = P.g2x * px + P.g2y * py, ap = -(P.vk0 * px + P.vk1 * py);
+  const cplx step = cexpi(gp2);
   cplx S = mk(0, 0), Sx = mk(0, 0), Sy = mk(0, 0), Sz = mk(0, 0);
   for (int n1 = n1lo; n1 <= n1hi; ++n1) {
-    double ux = -P.vk0 + n1 * P.g1x, uy = -P.vk1 + n1 * P.g1y;
+    const double ux = -P.vk0 + n1 * P.g1x, uy = -P.vk1 + n1 * P.g1y;
     int lo, hi;
     if (!row_range(ux, uy, P.g2x, P.g2y, g22, A2, lo, hi)) continue;
-    for (int n2 = lo; n2 <= hi; ++n2) {
-      double ax = ux + n2 * P.g2x, ay = uy + n2 * P.g2y;
-      double b2 = P.k2 - ax * ax - ay * ay;
+    const int c = row_centre(ux * P.g2x + uy * P.g2y, g22, lo, hi);
+    row_walk(lo, hi, c, cexpi(ap + n1 * gp1 + c * gp2), step, [&](int n2, cplx E) {
+      const double ax = ux + n2 * P.g2x, ay = uy + n2 * P.g2y;
+      const double b2 = P.k2 - ax * ax - ay * ay;
       cplx dz, F;
       if (b2 > 0.0) {
-        double b = sqrt(b2);
-        dz = cexpi(ax * px + ay * py + b * az);
+        const double b = sqrt(b2);
+        dz = E * cexpi(b * az);
         F = dz * (1.0 / b);
       } else {
-        double b = sqrt(-b2);
-        dz = cexpi(ax * px + ay * py) * exp(-b * az);
+        const double b = sqrt(-b2);
+        dz = E * exp(-b * az);
         F = mk(dz.im, -dz.re) * (1.0 / b);  // -i dz / b
       }
       S += F;
       Sx += F * ax;
       Sy += F * ay;
       Sz += dz;
-    }
+    });
   }
   double h = 0.5 * P.aidd, sg = (pz > 0.0) ? 1.0 : -1.0;
   o.g = cmul_i(S) * h;
@@ -84,13 +113,13 @@ __device__ void qpgf_fourier(const bq_qplan& P, double px, double py, double pz,
   o.d[2] = Sz * (-h * sg);
 }
 
-// one Ewald spectral term for the reciprocal vector (ax, ay): F (value) and Gz (z part)
-__device__ __forceinline__ void spectral_term(const bq_qplan& P, double ax, double ay, double px, double py, double pz,
-                                              double ve, cplx& F, cplx& Gz)
+// one Ewald spectral term for the reciprocal vector (ax, ay) with ce = exp(i a . p):
+// F (value) and Gz (z part)
+__device__ __forceinline__ void spectral_term(const bq_qplan& P, double ax, double ay, cplx ce, double pz, double ve,
+                                              cplx& F, cplx& Gz)
 {
   double b2 = P.k2 - ax * ax - ay * ay;
   double az = 0.5 * pz / ve;
-  cplx ce = cexpi(ax * px + ay * py);
   if (b2 > 0.0) {
     double b = sqrt(b2), ef = exp(b2 * ve * ve - az * az);
     cplx ff = cexpi(b * pz);
@@ -246,9 +275,8 @@ __device__ void spectral_z0_ref(const bq_qplan& P, double px, double py, double 
 }
 
 // Ewald split for the folded point.
-__device__ void qpgf_ewald(const bq_qplan& P, double px, double py, double pz, QG& o, bool z0_ref)
+__device__ void qpgf_ewald(const bq_qplan& P, double px, double py, double pz, double ve, QG& o, bool z0_ref)
 {
-  const double ve = ewald_param(P, px, py, pz);
   const double ive = 1.0 / ve, ke = P.k * ve, ke2 = ke * ke;
 
   // ---- spatial sum: |r + R_l| <= 2 ve sqrt(ke^2 + cut)
@@ -260,23 +288,25 @@ __device__ void qpgf_ewald(const bq_qplan& P, double px, double py, double pz, Q
     double d22 = P.d2x * P.d2x + P.d2y * P.d2y;
     const double gcon = ive / kSQPI;
     cplx S = mk(0, 0), Sxy[2] = {mk(0, 0), mk(0, 0)}, Sz = mk(0, 0);
+    const cplx step = mk(P.ph2re, P.ph2im);
     for (int l1 = l1lo; l1 <= l1hi; ++l1) {
-      double ux = px + l1 * P.d1x, uy = py + l1 * P.d1y;
+      const double ux = px + l1 * P.d1x, uy = py + l1 * P.d1y;
       int lo, hi;
       if (!row_range(ux, uy, P.d2x, P.d2y, d22, R2 - pz * pz, lo, hi)) continue;
-      for (int l2 = lo; l2 <= hi; ++l2) {
-        double Rx = l1 * P.d1x + l2 * P.d2x, Ry = l1 * P.d1y + l2 * P.d2y;
-        double ax = px + Rx, ay = py + Ry;
-        double rl = sqrt(ax * ax + ay * ay + pz * pz), irl = 1.0 / rl;
-        double y = 0.5 * rl * ive;
-        cplx cef = cexpi(P.vk0 * Rx + P.vk1 * Ry) * exp(ke2 - y * y);
-        cplx w = faddeeva_w(ke, y);
+      const int c = row_centre(ux * P.d2x + uy * P.d2y, d22, lo, hi);
+      const cplx E0 = cexpi(P.vk0 * (l1 * P.d1x + c * P.d2x) + P.vk1 * (l1 * P.d1y + c * P.d2y));
+      row_walk(lo, hi, c, E0, step, [&](int l2, cplx Eb) {
+        const double ax = ux + l2 * P.d2x, ay = uy + l2 * P.d2y;
+        const double rl = sqrt(ax * ax + ay * ay + pz * pz), irl = 1.0 / rl;
+        const double y = 0.5 * rl * ive;
+        const cplx cef = Eb * exp(ke2 - y * y);
+        const cplx w = faddeeva_w(ke, y);
         S += cef * (irl * w.re);
-        cplx cf = cef * (irl * irl * (irl * w.re - P.k * w.im + gcon));
+        const cplx cf = cef * (irl * irl * (irl * w.re - P.k * w.im + gcon));
         Sxy[0] += cf * ax;
         Sxy[1] += cf * ay;
         Sz += cf;
-      }
+      });
     }
     o.g = S * kI4PI;
     o.d[0] = Sxy[0] * (-kI4PI);
@@ -298,19 +328,22 @@ __device__ void qpgf_ewald(const bq_qplan& P, double px, double py, double pz, Q
     int n1lo = (int)ceil(c1 - w1), n1hi = (int)floor(c1 + w1);
     double g22 = P.g2x * P.g2x + P.g2y * P.g2y;
     cplx S = mk(0, 0), Sx = mk(0, 0), Sy = mk(0, 0), Sz = mk(0, 0);
+    const double gp1 = P.g1x * px + P.g1y * py, gp2 = P.g2x * px + P.g2y * py, ap = -(P.vk0 * px + P.vk1 * py);
+    const cplx step = cexpi(gp2);
     for (int n1 = n1lo; n1 <= n1hi; ++n1) {
-      double ux = -P.vk0 + n1 * P.g1x, uy = -P.vk1 + n1 * P.g1y;
+      const double ux = -P.vk0 + n1 * P.g1x, uy = -P.vk1 + n1 * P.g1y;
       int lo, hi;
       if (!row_range(ux, uy, P.g2x, P.g2y, g22, A2, lo, hi)) continue;
-      for (int n2 = lo; n2 <= hi; ++n2) {
+      const int c = row_centre(ux * P.g2x + uy * P.g2y, g22, lo, hi);
+      row_walk(lo, hi, c, cexpi(ap + n1 * gp1 + c * gp2), step, [&](int n2, cplx E) {
         cplx F, Gz;
-        double ax = ux + n2 * P.g2x, ay = uy + n2 * P.g2y;
-        spectral_term(P, ax, ay, px, py, pz, ve, F, Gz);
+        const double ax = ux + n2 * P.g2x, ay = uy + n2 * P.g2y;
+        spectral_term(P, ax, ay, E, pz, ve, F, Gz);
         S += F;
         Sx += F * ax;
         Sy += F * ay;
         Sz += Gz;
-      }
+      });
     }
     double cd = 0.25 * P.aidd;
     o.g += cmul_i(S) * cd;
@@ -328,13 +361,20 @@ __device__ void qpgf_v1(const bq_qplan& P, double rx, double ry, double rz, QG& 
   double l1, l2, px, py;
   fold_cell(P, rx, ry, l1, l2, px, py);
   const double pz = rz;
+  const double ve = ewald_param(P, px, py, pz);
+  // Both sums converge to the same value; take the cheaper one.  Term counts are the lattice
+  // points inside each truncation disk (area / cell area).
   bool fd = false;
-  if (P.fd_allowed) {
-    double az = fabs(rz);
-    fd = (P.fd_bN / P.fd_bNp * exp(-P.fd_bNp * az) < P.eps) && (P.fd_bN / P.fd_bNm * exp(-P.fd_bNm * az) < P.eps);
+  if (P.fd_allowed && pz != 0.0) {
+    const double bz = BQ_EXP_CUT / fabs(pz);
+    const double n_fd = kPI * (P.k2 + bz * bz) / P.rcell;
+    const double n_sn = kPI * (P.k2 + BQ_EXP_CUT / (ve * ve)) / P.rcell;
+    const double rs2 = 4.0 * ve * ve * (P.k2 * ve * ve + BQ_EXP_CUT) - pz * pz;
+    const double n_sp = rs2 > 0.0 ? kPI * rs2 / P.cell : 0.0;
+    fd = BQ_COST_FD * n_fd < BQ_COST_SPEC * n_sn + BQ_COST_SPAT * n_sp;
   }
   if (fd) qpgf_fourier(P, px, py, pz, o);
-  else qpgf_ewald(P, px, py, pz, o, z0_ref);
+  else qpgf_ewald(P, px, py, pz, ve, o, z0_ref);
   if (l1 != 0.0 || l2 != 0.0) {
     cplx cf = cexpi(-(P.vk0 * (l1 * P.d1x + l2 * P.d2x) + P.vk1 * (l1 * P.d1y + l2 * P.d2y)));
     o.g = o.g * cf;

@@ -30,6 +30,10 @@ int bq_qplan_init(bq_qplan *p, double k, const double *vk, const double *vd1, co
   double adm = sqrt(sm0 * sm0 + sm1 * sm1);
   p->veps0 = sqrt(fabs(det) / (4.0 * M_PI) * adp / adm);
   p->eps = eps;
+  p->ph2re = cos(vk[0] * vd2[0] + vk[1] * vd2[1]);
+  p->ph2im = sin(vk[0] * vd2[0] + vk[1] * vd2[1]);
+  p->cell = fabs(det);
+  p->rcell = fabs(p->g1x * p->g2y - p->g1y * p->g2x);
 
   /* Fourier-domain sum is allowed only if the orders S_LIMIT away from the nearest one
    * are evanescent on both sides; the per-point test then checks their decay over |z|. */

@@ -38,8 +38,8 @@ FP64_PEAK_TFS = 78.6
 # divided by the evaluations each launch makes (profiles/k_q0_gl_c1_pmc.md), and the HBM
 # bytes per launch from FETCH_SIZE (x2.00) + WRITE_SIZE (x1.00), the factors a 1 GiB copy
 # gives for the same counters.
-FLOPS_PER_EVAL = {"c1": {"k_q0_gl": 85653, "k_q0_gh": 28986, "k_q0_sing": 56502}}
-TRAFFIC_BYTES = {"c1": {"k_q0_gl": (2815.5 * 2.0 + 73320.0) * 1024}}
+FLOPS_PER_EVAL = {"c1": {"k_q0_gl": 54213, "k_q0_gh": 25808, "k_q0_sing": 50261}}
+TRAFFIC_BYTES = {"c1": {"k_q0_gl": (13373882.2 * 2.0 + 17179946.9) * 1024}}
 
 WORKLOADS = {
     "c1": dict(n=8, radius=0.4, n_sphere=1.5, lam=0.6, n0=1.0, theta=1e-6, phi=0.0, pol="TM",
@@ -62,6 +62,28 @@ def _scene(w, d):
     from bem3_emf_qd2_amd import meshgen
     meshgen.sphere_scene(d, w["n"], w["radius"], w["n_sphere"], w["lam"], n0=w["n0"], theta=w["theta"],
                          phi=w["phi"], pol=w["pol"], d1=w["d1"], d2=w["d2"])
+
+
+class _QuietStdout:
+    """Send the library's progress prints (solve_bieq's phase lines) to /dev/null while the
+    steps run, so that stdout carries only the JSON line."""
+
+    def __enter__(self):
+        import ctypes
+        self.libc = ctypes.CDLL(None)
+        sys.stdout.flush()
+        self.libc.fflush(None)
+        self.saved = os.dup(1)
+        self.null = os.open(os.devnull, os.O_WRONLY)
+        os.dup2(self.null, 1)
+        return self
+
+    def __exit__(self, *exc):
+        self.libc.fflush(None)
+        os.dup2(self.saved, 1)
+        os.close(self.saved)
+        os.close(self.null)
+        return False
 
 
 def _kernel_times(L):
@@ -194,16 +216,17 @@ def main():
             E, H, did = m.fields(probes, 1)
             return st, E, H
 
-        for _ in range(args.warmup):
-            st, E0, H0 = step()
-        L.bq_kernel_timing(1)
-        lat = []
-        t_all = time.perf_counter()
-        for _ in range(args.steps):
-            t = time.perf_counter()
-            st, E, H = step()
-            lat.append(time.perf_counter() - t)
-        elapsed = time.perf_counter() - t_all
+        with _QuietStdout():
+            for _ in range(args.warmup):
+                st, E0, H0 = step()
+            L.bq_kernel_timing(1)
+            lat = []
+            t_all = time.perf_counter()
+            for _ in range(args.steps):
+                t = time.perf_counter()
+                st, E, H = step()
+                lat.append(time.perf_counter() - t)
+            elapsed = time.perf_counter() - t_all
         kt = _kernel_times(L)
         L.bq_kernel_timing(0)
         stats = {"t_assemble_ms": st.t_assemble_ms, "t_solve_ms": st.t_solve_ms, "t_fields_ms": st.t_fields_ms,
