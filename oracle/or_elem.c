@@ -1,22 +1,38 @@
 /*
- * or_elem.c -- element-pair boundary-integral coefficients for bilinear quads.
+ * or_elem.c -- element-pair boundary-integral coefficients for bilinear quads and linear
+ * triangles.
  *
  * TEST INFRASTRUCTURE ONLY (see oracle.h).
  *
  * Restates, for PREC_DEF_BE = 2 (d3qd2_src/bem3_emf_qd2.h:13) and SW_* = 0
  * (d3qd2_src/d3qd2_const.h:80-90):
  *   periodic regular pair, GL/GH         q0_bil_dcoef_v2_cc_GL/_GH  d3qd2_src/q0_bil_dcoef_v2_cc.c:129-290
+ *                                        q0_lit_dcoef_v2_cc_GL/_GH  d3qd2_src/q0_lit_dcoef_v2_cc.c:155-301
  *   periodic singular pair (polar + FD)  q0_bil_dcoef_bd_vt2_cc_DV  :292-301, q0_bil_coef_bd
  *                                        d3qd2_src/q0_bil_coef.c:246-338, q0_bil_coef_bd_F
- *                                        q0_bil_dcoef_v2_cc.c:336-454
- *   free-space regular pair, GL/GH       bil_dcoef_v2_GL/_GH  d3qd2_src/bil_dcoef_v2.c:141-325
+ *                                        q0_bil_dcoef_v2_cc.c:336-454;
+ *                                        q0_lit_dcoef_bd_vt2_cc_DV q0_lit_dcoef_v2_cc.c:331-342,
+ *                                        q0_lit_coef_bd d3qd2_src/q0_lit_coef.c:297-371, :549-602
+ *   free-space regular pair, GL/GH       bil_dcoef_v2_GL/_GH  d3qd2_src/bil_dcoef_v2.c:141-325,
+ *                                        lit_dcoef_v2_GL/_GH  d3qd2_src/lit_dcoef_v2.c:183-357, :381-413
  *   free-space singular pair (PV)        bil_coef_bd  d3qd2_src/bil_coef.c:244-342,
  *                                        bil_dcoef_bd_tz0_PV/_te0_PV  d3qd2_src/bil_dcoef.c:369-739,
- *                                        integrands :916-1630
+ *                                        integrands :916-1630;
+ *                                        lit_coef_bd d3qd2_src/lit_coef.c:290-349, :543-578,
+ *                                        lit_dcoef_bd_tz0_PV/_te0_PV d3qd2_src/lit_dcoef.c:461-637,
+ *                                        integrands :845-952
  *   dispatch + GL->GH fallback           d3qd2_src/d3qd2_elem.c:515-572, 830-886
  *   field-point coefficients             q0_bil_coef_{4p,9p,GL,GH} d3qd2_src/q0_bil_coef.c:10-198,
- *                                        bil_coef_{4p,9p,GL,GH} d3qd2_src/bil_coef.c:25-202
- * Linear triangles (lit_*, q0_lit_*) are not restated yet; or_pair aborts on them.
+ *                                        bil_coef_{4p,9p,GL,GH} d3qd2_src/bil_coef.c:25-202,
+ *                                        q0_lit_coef_{4p,7p,GL,GH} d3qd2_src/q0_lit_coef.c:10-242,
+ *                                        lit_coef_{4p,7p,GL,GH} d3qd2_src/lit_coef.c:26-242
+ *
+ * One restatement serves both shapes.  Every rule is a list of (zeta, eta, weight) points
+ * with weights in reference-area units, and every integrand carries the surface Jacobian
+ * |W| js, where W is the element's normal vector field (cw) and js = 1 for the bilinear map
+ * (whose cw already is dr/dzeta x dr/deta) and js = 2/(3 sqrt3) for the triangle (whose cw
+ * is (r1-r0)x(r2-r0) while dr/dzeta x dr/deta = cw 2/(3 sqrt3)).  The reference writes the
+ * same constants as per-rule prefactors (1/24, 0.5, I_EP, I_SPSQ3 ...); they are equal.
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -26,6 +42,8 @@
 
 #define I_FP 7.957747154594766788444e-2 /* 1/(4 pi) */
 #define P44_N 0.5773502691896257645091
+#define P34_N0 0.2
+#define SQ3 1.732050807568877293528
 #define MEPS 1.0e-15
 #define GEPS 1.0e-15
 #define DEPS 0.1
@@ -34,7 +52,13 @@
 #define GLN 12
 #define GHN 64
 
-typedef struct { double cr[3][4], cw[3][3]; } elc;
+typedef struct {
+  double cr[3][4], cw[3][3];
+  int tri;   /* 1 for a linear triangle */
+  double js; /* surface Jacobian per |W|: 1 (quad) or 2/(3 sqrt3) (triangle) */
+} elc;
+
+static int is_tri_id(const or_model *m, int s) { return m->ed[4 * abs(s) + 3] == 0; }
 
 static void get_el(const or_model *m, int s, elc *e)
 {
@@ -42,16 +66,43 @@ static void get_el(const or_model *m, int s, elc *e)
     for (int c = 0; c < 4; c++) e->cr[d][c] = m->cr[12 * s + 4 * d + c];
     for (int c = 0; c < 3; c++) e->cw[d][c] = m->cw[9 * s + 3 * d + c];
   }
+  e->tri = is_tri_id(m, s);
+  e->js = e->tri ? 2.0 / (3.0 * SQ3) : 1.0;
 }
 
-static double Nn(int n, double z, double e)
+static int n_shape(const elc *e) { return e->tri ? 3 : 4; }
+
+/* interpolation functions: bilinear ones equal to 1 at their own 2x2 Gauss point
+ * (d3qd2_elem.c:903-915); linear ones equal to 1 at their own triangle collocation node
+ * (lit_Nn d3qd2_elem.c:1047-1058) */
+static double Nn(const elc *e, int n, double z, double t)
 {
+  if (e->tri) {
+    double ia = 0.5 / P34_N0;
+    switch (n) {
+    case 0: return (1.0 - ia * (z + SQ3 * t)) / 3.0;
+    case 1: return (1.0 + 2.0 * ia * z) / 3.0;
+    default: return (1.0 - ia * (z - SQ3 * t)) / 3.0;
+    }
+  }
   double ia = 1.0 / P44_N;
   switch (n) {
-  case 0: return 0.25 * (1.0 - z * ia) * (1.0 - e * ia);
-  case 1: return 0.25 * (1.0 + z * ia) * (1.0 - e * ia);
-  case 2: return 0.25 * (1.0 + z * ia) * (1.0 + e * ia);
-  default: return 0.25 * (1.0 - z * ia) * (1.0 + e * ia);
+  case 0: return 0.25 * (1.0 - z * ia) * (1.0 - t * ia);
+  case 1: return 0.25 * (1.0 + z * ia) * (1.0 - t * ia);
+  case 2: return 0.25 * (1.0 + z * ia) * (1.0 + t * ia);
+  default: return 0.25 * (1.0 - z * ia) * (1.0 + t * ia);
+  }
+}
+
+/* r * dN/dr of the linear triangle functions along direction (c, s) (lit_dNn_pr_cs,
+ * d3qd2_src/lit_dcoef.c:845-856) */
+static double tri_dN(int n, double c, double s)
+{
+  double ia = 0.5 / P34_N0;
+  switch (n) {
+  case 0: return -2.0 / 3.0 * ia * (c + SQ3 * s);
+  case 1: return 4.0 / 3.0 * ia * c;
+  default: return -2.0 / 3.0 * ia * (c - SQ3 * s);
   }
 }
 
@@ -66,27 +117,40 @@ static void rw_at(const elc *e, double z, double t, double *r, double *w)
 static double dot3(const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
 static double nrm3(const double *a) { return sqrt(dot3(a, a)); }
 
+/* flat element test (bil_check_plane d3qd2_elem.c:987-996); triangles are flat */
 static int is_plane(const elc *e)
 {
+  if (e->tri) return 1;
   double b[3] = {e->cr[0][1], e->cr[1][1], e->cr[2][1]}, c[3] = {e->cw[0][2], e->cw[1][2], e->cw[2][2]};
   return fabs(dot3(b, c) / (nrm3(b) * nrm3(c))) < GEPS;
 }
 
 static double bdC_of(const elc *e) { return e->cr[0][1] * e->cw[0][2] + e->cr[1][1] * e->cw[1][2] + e->cr[2][1] * e->cw[2][2]; }
 
+/* collocation node coordinates of node nid on element s */
+static void node_zeta_eta(const or_model *m, int s, int nid, double *zt, double *et)
+{
+  if (is_tri_id(m, s)) { *zt = m->zt34[nid]; *et = m->et34[nid]; }
+  else { *zt = m->zt44[nid]; *et = m->et44[nid]; }
+}
+
 /* unit tangents at element node nid (d3qd2_elem.c:122-143) */
-static void node_tangents(const or_model *m, int s, int nid, double *vtz, double *vte)
+void or_node_tangents(const or_model *m, int s, int nid, double *vtz, double *vte)
 {
   elc e;
-  get_el(m, s, &e);
+  get_el(m, abs(s), &e);
+  double zt, et;
+  node_zeta_eta(m, abs(s), nid, &zt, &et);
   for (int d = 0; d < 3; d++) {
-    vtz[d] = e.cr[d][1] + e.cr[d][3] * m->et44[nid];
-    vte[d] = e.cr[d][2] + e.cr[d][3] * m->zt44[nid];
+    vtz[d] = e.cr[d][1] + e.cr[d][3] * et;
+    vte[d] = e.cr[d][2] + e.cr[d][3] * zt;
   }
   double a = nrm3(vtz), b = nrm3(vte);
   for (int d = 0; d < 3; d++) { vtz[d] /= a; vte[d] /= b; }
 }
 
+/* H and F change sign for the sub-domain side of an element (bil_convert_CC / lit_convert_CC,
+ * d3qd2_elem.c:927-947, :1075-1093) */
 static void flip_sign(double complex *C) { for (int i = 4; i < 9; i++) C[i] = -C[i]; }
 
 static void qpgf_or_die(const or_model *m, int mode, const double *R, double complex *o)
@@ -98,69 +162,73 @@ static void qpgf_or_die(const or_model *m, int mode, const double *R, double com
   }
 }
 
-/* ---- periodic regular pair: tensor GL rule of order n ------------------------------ */
+static const or_rule *rule_of(const or_model *m, const elc *e, int type) { return &m->rule[e->tri][type]; }
+
+/* ---- periodic regular pair ---------------------------------------------------------- */
 static void q0_regular(const or_model *m, const double *rt, const double *v0, const double *v1,
-                       int s, int n, const double *x, const double *wq, double complex *CC,
-                       double complex *dC0, double complex *dC1)
+                       int s, int type, double complex *CC, double complex *dC0, double complex *dC1)
 {
   elc e;
   get_el(m, s, &e);
+  const or_rule *q = rule_of(m, &e, type);
+  int ns = n_shape(&e);
   for (int l = 0; l < 9; l++) CC[l] = dC0[l] = dC1[l] = 0.0;
   double F = 0, dF0 = 0, dF1 = 0;
-  for (int i = 0; i < n; i++) {
-    double complex tg[4] = {0}, th[4] = {0};
-    double tf = 0, td0 = 0, td1 = 0;
-    for (int j = 0; j < n; j++) {
-      double z = x[i], t = x[j], R[3], W[3];
-      rw_at(&e, z, t, R, W);
-      for (int d = 0; d < 3; d++) R[d] -= rt[d];
-      double aR = nrm3(R), iR = 1.0 / aR, aW = nrm3(W);
-      double rdW = dot3(R, W) * iR, rdv0 = dot3(R, v0) * iR, rdv1 = dot3(R, v1) * iR;
-      double complex o[10];
-      qpgf_or_die(m, 0, R, o);
-      double complex hn = o[1] * W[0] + o[2] * W[1] + o[3] * W[2];
-      for (int ep = 0; ep < 4; ep++) {
-        double Np = Nn(ep, z, t);
-        tg[ep] += wq[j] * o[0] * Np * aW;
-        th[ep] += wq[j] * hn * Np;
-      }
-      tf += wq[j] * rdW * iR * iR;
-      td0 += wq[j] * iR * iR * iR * (3.0 * rdv0 * rdW - dot3(W, v0));
-      td1 += wq[j] * iR * iR * iR * (3.0 * rdv1 * rdW - dot3(W, v1));
+  for (int p = 0; p < q->n; p++) {
+    double z = q->z[p], t = q->e[p], wq = q->w[p], R[3], W[3];
+    rw_at(&e, z, t, R, W);
+    for (int d = 0; d < 3; d++) { R[d] -= rt[d]; W[d] *= e.js; }
+    double aR = nrm3(R), iR = 1.0 / aR, aW = nrm3(W);
+    double rdW = dot3(R, W) * iR, rdv0 = dot3(R, v0) * iR, rdv1 = dot3(R, v1) * iR;
+    double complex o[10];
+    qpgf_or_die(m, 0, R, o);
+    double complex hn = o[1] * W[0] + o[2] * W[1] + o[3] * W[2];
+    for (int ep = 0; ep < ns; ep++) {
+      double Np = Nn(&e, ep, z, t);
+      CC[ep] += wq * o[0] * Np * aW;
+      CC[ep + 4] += wq * hn * Np;
     }
-    for (int ep = 0; ep < 4; ep++) {
-      CC[ep] += wq[i] * tg[ep];
-      CC[ep + 4] += wq[i] * th[ep];
-    }
-    F += wq[i] * tf;
-    dF0 += wq[i] * td0;
-    dF1 += wq[i] * td1;
+    F += wq * rdW * iR * iR;
+    dF0 += wq * iR * iR * iR * (3.0 * rdv0 * rdW - dot3(W, v0));
+    dF1 += wq * iR * iR * iR * (3.0 * rdv1 * rdW - dot3(W, v1));
   }
   CC[8] = F * I_FP;
   dC0[8] = dF0 * I_FP;
   dC1[8] = dF1 * I_FP;
 }
 
-/* ---- polar rule around (zt, et): visits every (r, theta) node of the 4 sub-triangles -- */
+/* ---- polar rule around (zt, et): visits every (r, theta) node of the sub-triangles ---- */
 typedef void (*polar_fn)(void *ctx, double r, double cth, double sth, double wgt);
 
-static void node_r_th(double zt, double et, double *r, double *th)
+/* corners of the reference element seen from (zt, et): distances r[] and angles th[],
+ * closed with th[n] = th[0] + 2 pi (bil_calc_node_r_th d3qd2_elem.c:1014-1033,
+ * lit_calc_node_r_th :1149-1162).  Returns the number of sub-triangles. */
+static int node_r_th(int tri, double zt, double et, double *r, double *th)
 {
+  if (tri) {
+    double ztp = 1.0 - zt, ztm = -0.5 - zt, etp = 0.5 * SQ3 - et, etm = -0.5 * SQ3 - et;
+    r[0] = sqrt(ztm * ztm + etm * etm); r[1] = sqrt(ztp * ztp + et * et); r[2] = sqrt(ztm * ztm + etp * etp);
+    r[3] = r[0];
+    th[0] = atan2(etm, ztm); th[1] = atan2(-et, ztp); th[2] = atan2(etp, ztm);
+    th[3] = th[0] + 2.0 * M_PI;
+    return 3;
+  }
   double ztp = 1.0 - zt, ztm = -1.0 - zt, etp = 1.0 - et, etm = -1.0 - et;
   r[0] = sqrt(ztm * ztm + etm * etm); r[1] = sqrt(ztp * ztp + etm * etm);
   r[2] = sqrt(ztp * ztp + etp * etp); r[3] = sqrt(ztm * ztm + etp * etp);
   r[4] = r[0];
   th[0] = atan2(etm, ztm); th[1] = atan2(etm, ztp); th[2] = atan2(etp, ztp); th[3] = atan2(etp, ztm);
   th[4] = th[0] + 2.0 * M_PI;
+  return 4;
 }
 
 /* theta: two GHN halves per sub-triangle; r: rule (nr, xr, wr) on [r_lo, rm(theta)] */
-static void polar_visit(const or_model *m, double zt, double et, double rlo, int nr,
+static void polar_visit(const or_model *m, int tri, double zt, double et, double rlo, int nr,
                         const double *xr, const double *wr, polar_fn f, void *ctx)
 {
   double r[5], th[5];
-  node_r_th(zt, et, r, th);
-  for (int j = 0; j < 4; j++) {
+  int nsub = node_r_th(tri, zt, et, r, th);
+  for (int j = 0; j < nsub; j++) {
     double r0 = r[j], r1 = r[j + 1], t0 = th[j], t1 = th[j + 1];
     for (int h = 0; h < 2; h++) {
       for (int mm = 0; mm < GHN; mm++) {
@@ -177,7 +245,7 @@ static void polar_visit(const or_model *m, double zt, double et, double rlo, int
   }
 }
 
-/* K = (R(zt + r c, et + r s) - R(zt, et)) / r and W at that point */
+/* K = (R(zt + r c, et + r s) - R(zt, et)) / r and the Jacobian-scaled W at that point */
 static void polar_KW(const elc *e, double zt, double et, double r, double c, double s, double *K, double *W,
                      double *z, double *t)
 {
@@ -185,7 +253,7 @@ static void polar_KW(const elc *e, double zt, double et, double r, double c, dou
   *t = r * s + et;
   for (int d = 0; d < 3; d++) {
     K[d] = e->cr[d][1] * c + e->cr[d][2] * s + e->cr[d][3] * (r * c * s + zt * s + et * c);
-    W[d] = e->cw[d][0] + e->cw[d][1] * (*z) + e->cw[d][2] * (*t);
+    W[d] = (e->cw[d][0] + e->cw[d][1] * (*z) + e->cw[d][2] * (*t)) * e->js;
   }
 }
 
@@ -204,15 +272,15 @@ static void q0_sing_fn(void *vp, double r, double c, double s, double w)
   double K[3], W[3], z, t, R[3];
   polar_KW(&p->e, p->zt, p->et, r, c, s, K, W, &z, &t);
   for (int d = 0; d < 3; d++) R[d] = r * K[d];
-  /* the reference evaluates qG and dqG separately here (q0_bil_coef.c:444, :489); their
-   * stop rules differ only for R_z == 0 exactly */
+  /* the reference evaluates qG and dqG separately here (q0_bil_coef.c:444, :489;
+   * q0_lit_coef.c:569, :596); their stop rules differ only for R_z == 0 exactly */
   double complex o[10], og[10];
   qpgf_or_die(p->m, 3, R, o);
   qpgf_or_die(p->m, 2, R, og);
   double aW = nrm3(W);
   double complex hn = o[1] * W[0] + o[2] * W[1] + o[3] * W[2];
-  for (int n = 0; n < 4; n++) {
-    double Np = Nn(n, z, t);
+  for (int n = 0; n < n_shape(&p->e); n++) {
+    double Np = Nn(&p->e, n, z, t);
     p->G[n] += w * og[0] * Np * aW * r;
     p->H[n] += w * hn * Np * r;
   }
@@ -235,7 +303,7 @@ static double polar_F(const or_model *m, int s, double zt, double et)
   get_el(m, s, &p.e);
   if (is_plane(&p.e)) return 0.0;
   p.zt = zt; p.et = et;
-  polar_visit(m, zt, et, 0.0, GLN, m->xli, m->wli, F_fn, &p);
+  polar_visit(m, p.e.tri, zt, et, 0.0, GLN, m->xli, m->wli, F_fn, &p);
   return p.F * I_FP * bdC_of(&p.e);
 }
 
@@ -245,12 +313,14 @@ static void q0_singular(const or_model *m, int s, int nid, double complex *CC, d
   pctx p = {0};
   p.m = m;
   get_el(m, s, &p.e);
-  p.zt = m->zt44[nid];
-  p.et = m->et44[nid];
-  polar_visit(m, p.zt, p.et, 0.0, GLN, m->xli, m->wli, q0_sing_fn, &p);
-  for (int n = 0; n < 4; n++) { CC[n] = p.G[n]; CC[n + 4] = p.H[n]; }
+  node_zeta_eta(m, s, nid, &p.zt, &p.et);
+  polar_visit(m, p.e.tri, p.zt, p.et, 0.0, GLN, m->xli, m->wli, q0_sing_fn, &p);
+  for (int i = 0; i < 9; i++) CC[i] = dCz[i] = dCe[i] = 0.0;
+  for (int n = 0; n < n_shape(&p.e); n++) { CC[n] = p.G[n]; CC[n + 4] = p.H[n]; }
+  /* a flat element has F = 0 and dF/dt = 0 at its own nodes; the triangle path sets them
+   * so without computing (q0_lit_dcoef_v2_cc.c:331-342) */
+  if (p.e.tri) return;
   CC[8] = polar_F(m, s, p.zt, p.et);
-  for (int i = 0; i < 9; i++) dCz[i] = dCe[i] = 0.0;
   double Tz[3], Te[3];
   for (int d = 0; d < 3; d++) {
     Tz[d] = p.e.cr[d][1] + p.e.cr[d][3] * p.et;
@@ -265,37 +335,36 @@ static void q0_singular(const or_model *m, int s, int nid, double complex *CC, d
 
 /* ---- free-space regular pair ------------------------------------------------------- */
 static void fs_regular(const or_model *m, const double *rt, const double *v0, const double *v1,
-                       int ss, double complex k, int n, const double *x, const double *wq,
-                       double complex *CC, double complex *dC0, double complex *dC1)
+                       int ss, double complex k, int type, double complex *CC, double complex *dC0,
+                       double complex *dC1)
 {
   int s = abs(ss);
   elc e;
   get_el(m, s, &e);
+  const or_rule *q = rule_of(m, &e, type);
+  int ns = n_shape(&e);
   for (int l = 0; l < 9; l++) CC[l] = dC0[l] = dC1[l] = 0.0;
   double F = 0, dF0 = 0, dF1 = 0;
-  for (int i = 0; i < n; i++) {
-    for (int j = 0; j < n; j++) {
-      double z = x[i], t = x[j], R[3], W[3];
-      rw_at(&e, z, t, R, W);
-      for (int d = 0; d < 3; d++) R[d] -= rt[d];
-      double aR = nrm3(R), iR = 1.0 / aR, aW = nrm3(W);
-      double complex ca = I * k * aR, cb = ca - 1.0, ce = cexp(ca);
-      double rdW = dot3(R, W) * iR, tD = rdW * iR * iR;
-      double rdv0 = dot3(R, v0) * iR, rdv1 = dot3(R, v1) * iR, Wdv0 = dot3(W, v0), Wdv1 = dot3(W, v1);
-      double wij = wq[i] * wq[j];
-      for (int ep = 0; ep < 4; ep++) {
-        double Np = Nn(ep, z, t);
-        CC[ep] += wij * ce * iR * aW * Np;
-        CC[ep + 4] += wij * cb * ce * tD * Np;
-        dC0[ep] += wij * cb * ce * iR * iR * rdv0 * aW * Np;
-        dC1[ep] += wij * cb * ce * iR * iR * rdv1 * aW * Np;
-        dC0[ep + 4] += wij * ce * iR * iR * iR * ((k * k * aR * aR + 3.0 * cb) * rdv0 * rdW - cb * Wdv0) * Np;
-        dC1[ep + 4] += wij * ce * iR * iR * iR * ((k * k * aR * aR + 3.0 * cb) * rdv1 * rdW - cb * Wdv1) * Np;
-      }
-      F += wij * tD;
-      dF0 += wij * iR * iR * iR * (3.0 * rdv0 * rdW - Wdv0);
-      dF1 += wij * iR * iR * iR * (3.0 * rdv1 * rdW - Wdv1);
+  for (int p = 0; p < q->n; p++) {
+    double z = q->z[p], t = q->e[p], wij = q->w[p], R[3], W[3];
+    rw_at(&e, z, t, R, W);
+    for (int d = 0; d < 3; d++) { R[d] -= rt[d]; W[d] *= e.js; }
+    double aR = nrm3(R), iR = 1.0 / aR, aW = nrm3(W);
+    double complex ca = I * k * aR, cb = ca - 1.0, ce = cexp(ca);
+    double rdW = dot3(R, W) * iR, tD = rdW * iR * iR;
+    double rdv0 = dot3(R, v0) * iR, rdv1 = dot3(R, v1) * iR, Wdv0 = dot3(W, v0), Wdv1 = dot3(W, v1);
+    for (int ep = 0; ep < ns; ep++) {
+      double Np = Nn(&e, ep, z, t);
+      CC[ep] += wij * ce * iR * aW * Np;
+      CC[ep + 4] += wij * cb * ce * tD * Np;
+      dC0[ep] += wij * cb * ce * iR * iR * rdv0 * aW * Np;
+      dC1[ep] += wij * cb * ce * iR * iR * rdv1 * aW * Np;
+      dC0[ep + 4] += wij * ce * iR * iR * iR * ((k * k * aR * aR + 3.0 * cb) * rdv0 * rdW - cb * Wdv0) * Np;
+      dC1[ep + 4] += wij * ce * iR * iR * iR * ((k * k * aR * aR + 3.0 * cb) * rdv1 * rdW - cb * Wdv1) * Np;
     }
+    F += wij * tD;
+    dF0 += wij * iR * iR * iR * (3.0 * rdv0 * rdW - Wdv0);
+    dF1 += wij * iR * iR * iR * (3.0 * rdv1 * rdW - Wdv1);
   }
   for (int ep = 0; ep < 8; ep++) {
     CC[ep] *= I_FP;
@@ -308,7 +377,7 @@ static void fs_regular(const or_model *m, const double *rt, const double *v0, co
   if (ss < 0) { flip_sign(CC); flip_sign(dC0); flip_sign(dC1); }
 }
 
-/* ---- free-space singular pair: G,H,F by polar rule (bil_coef_bd) ------------------- */
+/* ---- free-space singular pair: G,H,F by polar rule (bil_coef_bd / lit_coef_bd) ------- */
 static void fs_sing_GHF_fn(void *vp, double r, double c, double s, double w)
 {
   pctx *p = vp;
@@ -316,8 +385,8 @@ static void fs_sing_GHF_fn(void *vp, double r, double c, double s, double w)
   polar_KW(&p->e, p->zt, p->et, r, c, s, K, W, &z, &t);
   double aK = nrm3(K), aW = nrm3(W);
   double complex kR = p->k * r * aK, ce = cexp(I * kR);
-  for (int n = 0; n < 4; n++) {
-    double Np = Nn(n, z, t);
+  for (int n = 0; n < n_shape(&p->e); n++) {
+    double Np = Nn(&p->e, n, z, t);
     p->G[n] += w * ce / aK * aW * Np;
     p->H[n] += w * (I * kR - 1.0) * ce / (aK * aK * aK) * c * s * Np;
   }
@@ -347,8 +416,8 @@ static void pv_Db_fn(void *vp, double r, double c, double s, double w)
   double complex kR = p->k * r * aK, cb = I * kR - 1.0, ce = cexp(I * kR);
   double cs_dir = (p->dir == 0) ? c : s;   /* (K.T) multiplies cos for tz, sin for te */
   double outer = (p->dir == 0) ? s : c;    /* theta factor outside the r integral */
-  for (int n = 0; n < 4; n++) {
-    double Np = Nn(n, z, t);
+  for (int n = 0; n < n_shape(&p->e); n++) {
+    double Np = Nn(&p->e, n, z, t);
     p->dG[n] += w * cb * ce * iK * iK / r * (KT * iK) * Np * aW;
     p->dH[n] += w * outer * ce / (r * pow(aK, 5)) * ((kR * kR + 3.0 * cb) * KT * cs_dir - cb * aK * aK) * Np;
   }
@@ -375,7 +444,7 @@ static void pv_De(pvctx *p)
         double complex kR = p->k * aK * r, cb = I * kR - 1.0;
         double complex cc = ccos(kR), sn = csin(kR), ce = cexp(I * kR);
         for (int n = 0; n < 4; n++) {
-          double Np = Nn(n, z, t);
+          double Np = Nn(&p->e, n, z, t);
           g[n] += wr * (cb * sn + kR * cc) * iK * iK / r * (KT * iK) * aW * Np;
           h0[n] += wr * ce / pow(aK, 4) * ((p->k * kR + 3.0 * I * p->k) * KT * cs_dir - I * p->k * aK * aK) * Np;
           h1[n] += wr * sn / (pow(aK, 5) * r) * (3.0 * KT * cs_dir - aK * aK) * Np;
@@ -421,7 +490,7 @@ static void pv_DP(pvctx *p)
         double KpT = KdT_of(p, Kp), KmT = KdT_of(p, Km);
         double complex ccp = ccos(p->k * aKp * r), ccm = ccos(-p->k * aKm * r);
         for (int n = 0; n < 4; n++) {
-          double Np = Nn(n, zp, ep), Nm = Nn(n, zm, em);
+          double Np = Nn(&p->e, n, zp, ep), Nm = Nn(&p->e, n, zm, em);
           g[n] += wr / r * (ccp / (aKp * aKp) * (KpT / aKp) * Np * aWp - ccm / (aKm * aKm) * (KmT / aKm) * Nm * aWm);
           hh[n] += wr / r * (ccp / pow(aKp, 5) * (3.0 * KpT * cs_dir - aKp * aKp) * Np -
                              ccm / pow(aKm, 5) * (3.0 * KmT * cs_dir - aKm * aKm) * Nm);
@@ -442,6 +511,43 @@ static void pv_DP(pvctx *p)
   p->dF += sF;
 }
 
+/* Linear triangle, D-eps disk and its PV pairing.  The interpolation function is
+ * N(zt + r c, et + r s) = N(zt, et) + r tri_dN(c, s); the constant part cancels in the
+ * principal value and the r integrals of the linear part have closed forms, leaving theta
+ * rules only (lit_sdG_De_theta, lit_sdG_DP_theta d3qd2_src/lit_dcoef.c:906-952; the
+ * reference adds i/k times the first and subtracts 1/k times the second, :503-524). */
+static void tri_pv_disk(pvctx *p)
+{
+  const or_model *m = p->m;
+  double aW = nrm3((double[3]){p->e.cw[0][0], p->e.cw[1][0], p->e.cw[2][0]}) * p->e.js;
+  double complex k = p->k;
+  for (int n = 0; n < 3; n++) {
+    double complex tDe = 0.0, tDP = 0.0;
+    for (int q = 0; q < 4; q++) {
+      double off = -0.75 * M_PI + 0.5 * M_PI * q;
+      for (int mm = 0; mm < GHN; mm++) {
+        double theta = 0.25 * M_PI * m->xhi[mm] + off, c = cos(theta), s = sin(theta);
+        double K[3];
+        for (int d = 0; d < 3; d++) K[d] = p->e.cr[d][1] * c + p->e.cr[d][2] * s;
+        double aK = nrm3(K), iK = 1.0 / aK, KT = KdT_of(p, K) * iK;
+        double complex keK = k * DEPS * aK, cs = csin(keK), cc = ccos(keK);
+        tDe += 0.25 * M_PI * m->whi[mm] * tri_dN(n, c, s) * iK * iK * iK * KT *
+               ((keK + I) * cs + (2.0 - I * keK) * cc - 2.0);
+      }
+    }
+    for (int h = 0; h < 2; h++) {
+      for (int mm = 0; mm < GHN; mm++) {
+        double theta = 0.25 * M_PI * m->xhi[mm] + (h == 0 ? 0.25 * M_PI : 0.75 * M_PI);
+        double c = cos(theta), s = sin(theta), K[3];
+        for (int d = 0; d < 3; d++) K[d] = p->e.cr[d][1] * c + p->e.cr[d][2] * s;
+        double aK = nrm3(K), iK = 1.0 / aK, KT = KdT_of(p, K) * iK;
+        tDP += 0.25 * M_PI * m->whi[mm] * tri_dN(n, c, s) * csin(k * aK * DEPS) * iK * iK * iK * KT;
+      }
+    }
+    p->dG[n] += (I * tDe / k - tDP / k) * aW;
+  }
+}
+
 static void fs_pv_deriv(const or_model *m, int s, double zt, double et, double complex k, int dir,
                         double complex *dCC)
 {
@@ -452,7 +558,14 @@ static void fs_pv_deriv(const or_model *m, int s, double zt, double et, double c
   for (int d = 0; d < 3; d++)
     p.T[d] = (dir == 0) ? p.e.cr[d][1] + p.e.cr[d][3] * et : p.e.cr[d][2] + p.e.cr[d][3] * zt;
   double iaT = 1.0 / nrm3(p.T);
-  polar_visit(m, zt, et, DEPS, GHN, m->xhi, m->whi, pv_Db_fn, &p);
+  polar_visit(m, p.e.tri, zt, et, DEPS, GHN, m->xhi, m->whi, pv_Db_fn, &p);
+  for (int i = 0; i < 9; i++) dCC[i] = 0.0;
+  if (p.e.tri) {
+    /* flat: dH/dt = dF/dt = 0 (lit_dcoef_bd_tz0_PV computes dG only) */
+    tri_pv_disk(&p);
+    for (int n = 0; n < 3; n++) dCC[n] = p.dG[n] * (-I_FP * iaT);
+    return;
+  }
   pv_De(&p);
   pv_DP(&p);
   int plane = is_plane(&p.e);
@@ -471,13 +584,13 @@ static void fs_singular(const or_model *m, int ss, int nid, double complex k, do
   pctx p = {0};
   p.m = m;
   get_el(m, s, &p.e);
-  p.zt = m->zt44[nid];
-  p.et = m->et44[nid];
+  node_zeta_eta(m, s, nid, &p.zt, &p.et);
   p.k = k;
-  polar_visit(m, p.zt, p.et, 0.0, GLN, m->xli, m->wli, fs_sing_GHF_fn, &p);
+  polar_visit(m, p.e.tri, p.zt, p.et, 0.0, GLN, m->xli, m->wli, fs_sing_GHF_fn, &p);
   int plane = is_plane(&p.e);
   double bdC = bdC_of(&p.e);
-  for (int n = 0; n < 4; n++) {
+  for (int i = 0; i < 9; i++) CC[i] = 0.0;
+  for (int n = 0; n < n_shape(&p.e); n++) {
     CC[n] = p.G[n] * I_FP;
     CC[n + 4] = plane ? 0.0 : p.H[n] * I_FP * bdC;
   }
@@ -490,20 +603,15 @@ static void fs_singular(const or_model *m, int ss, int nid, double complex k, do
 int or_pair(const or_model *m, int did, int t, int tn, int s, int type, double complex *CC,
             double complex *dCz, double complex *dCe)
 {
-  if (m->ed[4 * abs(s) + 3] == 0 || m->ed[4 * abs(t) + 3] == 0) {
-    fprintf(stderr, "oracle: linear triangles are not restated\n");
-    exit(1);
-  }
   double vtz[3], vte[3];
-  node_tangents(m, abs(t), tn, vtz, vte);
+  or_node_tangents(m, abs(t), tn, vtz, vte);
   const double *rt = m->ren + 12 * abs(t) + 3 * tn;
-  int n = (type >= 3) ? GHN : GLN;
-  const double *x = (type >= 3) ? m->xhi : m->xli, *w = (type >= 3) ? m->whi : m->wli;
+  int rule = (type >= 3) ? 3 : 2;
   if (did == 0 && m->ps == 1) {
     if (t != s) {
-      q0_regular(m, rt, vtz, vte, s, n, x, w, CC, dCz, dCe);
+      q0_regular(m, rt, vtz, vte, s, rule, CC, dCz, dCe);
       if (type < 3 && (fabs(creal(dCz[8])) > CBD_SDF || fabs(creal(dCe[8])) > CBD_SDF)) {
-        q0_regular(m, rt, vtz, vte, s, GHN, m->xhi, m->whi, CC, dCz, dCe);
+        q0_regular(m, rt, vtz, vte, s, 3, CC, dCz, dCe);
         return 1;
       }
       return 0;
@@ -513,9 +621,9 @@ int or_pair(const or_model *m, int did, int t, int tn, int s, int type, double c
   }
   double complex k = m->kn[did];
   if (t != s) {
-    fs_regular(m, rt, vtz, vte, s, k, n, x, w, CC, dCz, dCe);
+    fs_regular(m, rt, vtz, vte, s, k, rule, CC, dCz, dCe);
     if (type < 3 && (fabs(creal(dCz[8])) > CBD_SDF || fabs(creal(dCe[8])) > CBD_SDF)) {
-      fs_regular(m, rt, vtz, vte, s, k, GHN, m->xhi, m->whi, CC, dCz, dCe);
+      fs_regular(m, rt, vtz, vte, s, k, 3, CC, dCz, dCe);
       return 1;
     }
     return 0;
@@ -530,50 +638,38 @@ void or_coef_rt(const or_model *m, int did, const double *rt, int ss, int type, 
   int s = abs(ss);
   elc e;
   get_el(m, s, &e);
-  int periodic = (did == 0 && m->ps == 1);
+  int periodic = (did == 0 && m->ps == 1), ns = n_shape(&e);
   double complex k = m->kn[did];
-  double zq[64 * 64], eq[64 * 64], wq[64 * 64];
-  int np = 0;
-  if (type == 0) {
-    for (int i = 0; i < 4; i++) { zq[np] = m->zt44[i]; eq[np] = m->et44[i]; wq[np++] = m->wt44[i]; }
-  } else if (type == 1) {
-    for (int i = 0; i < 9; i++) { zq[np] = m->zt49[i]; eq[np] = m->et49[i]; wq[np++] = m->wt49[i]; }
-  } else {
-    int n = (type == 2) ? GLN : GHN;
-    const double *x = (type == 2) ? m->xli : m->xhi, *w = (type == 2) ? m->wli : m->whi;
-    for (int i = 0; i < n; i++)
-      for (int j = 0; j < n; j++) { zq[np] = x[i]; eq[np] = x[j]; wq[np++] = w[i] * w[j]; }
-  }
+  const or_rule *q = rule_of(m, &e, type);
   for (int l = 0; l < 9; l++) CC[l] = 0.0;
   double F = 0;
-  for (int q = 0; q < np; q++) {
+  for (int p = 0; p < q->n; p++) {
     double R[3], W[3];
-    rw_at(&e, zq[q], eq[q], R, W);
+    rw_at(&e, q->z[p], q->e[p], R, W);
     if (type == 0) {
-      /* the 4-point rule reads the stored node data (q0_bil_coef.c:28-30) */
-      int nq = q;
-      for (int d = 0; d < 3; d++) { R[d] = m->ren[12 * s + 3 * nq + d]; W[d] = m->wen[12 * s + 3 * nq + d]; }
+      /* the node rule reads the stored node data (q0_bil_coef.c:28-30, q0_lit_coef.c:29-31) */
+      for (int d = 0; d < 3; d++) { R[d] = m->ren[12 * s + 3 * p + d]; W[d] = m->wen[12 * s + 3 * p + d]; }
     }
-    for (int d = 0; d < 3; d++) R[d] -= rt[d];
+    for (int d = 0; d < 3; d++) { R[d] -= rt[d]; W[d] *= e.js; }
     double aR = nrm3(R), iR = 1.0 / aR, aW = nrm3(W);
     double tD = dot3(R, W) * iR * iR * iR;
-    F += wq[q] * tD;
+    F += q->w[p] * tD;
+    double complex g, h;
     if (periodic) {
       double complex o[10];
       qpgf_or_die(m, 0, R, o);
-      double complex hn = o[1] * W[0] + o[2] * W[1] + o[3] * W[2];
-      for (int ep = 0; ep < 4; ep++) {
-        double Np = (type == 0) ? (ep == q ? 1.0 : 0.0) : Nn(ep, zq[q], eq[q]);
-        CC[ep] += wq[q] * o[0] * Np * aW;
-        CC[ep + 4] += wq[q] * hn * Np;
-      }
+      g = o[0] * aW;
+      h = o[1] * W[0] + o[2] * W[1] + o[3] * W[2];
     } else {
       double complex ca = I * k * aR, ce = cexp(ca);
-      for (int ep = 0; ep < 4; ep++) {
-        double Np = (type == 0) ? (ep == q ? 1.0 : 0.0) : Nn(ep, zq[q], eq[q]);
-        CC[ep] += wq[q] * I_FP * ce * iR * aW * Np;
-        CC[ep + 4] += wq[q] * I_FP * (ca - 1.0) * ce * tD * Np;
-      }
+      g = I_FP * ce * iR * aW;
+      h = I_FP * (ca - 1.0) * ce * tD;
+    }
+    for (int ep = 0; ep < ns; ep++) {
+      /* node points carry their own function only; the triangle centroid carries 1/3 each */
+      double Np = (type == 0 && p < ns) ? (ep == p ? 1.0 : 0.0) : Nn(&e, ep, q->z[p], q->e[p]);
+      CC[ep] += q->w[p] * g * Np;
+      CC[ep + 4] += q->w[p] * h * Np;
     }
   }
   CC[8] = F * I_FP;

@@ -20,6 +20,21 @@
 #define P49_W1 0.4938271604938271604938
 #define P49_W2 0.7901234567901234567901
 #define MEPS 1.0e-15
+/* linear-triangle collocation nodes (0.4 of the circumradius towards each vertex) plus the
+ * centroid, and the 7-point rule, on the reference triangle with vertices (-1/2, -sqrt3/2),
+ * (1, 0), (-1/2, sqrt3/2) (reference d3qd2_src/d3qd2_const.h:63-73, d3qd2_setup.c:1208-1217) */
+#define P34_N0 0.2
+#define P34_N1 0.3464101615137754587055
+#define P34_W0 0.5208333333333333333333
+#define P34_W1 0.5625
+#define P37_N0 0.3480702390148154917985
+#define P37_N1 0.2052130961576726346557
+#define P37_N2 0.6028753385763033130539
+#define P37_N3 0.3554395089236065568357
+#define P37_W0 0.1259391805448271525957
+#define P37_W1 0.1323941527885061807376
+#define P37_W2 0.225
+#define SQ3 1.732050807568877293528
 
 /* Gauss-Legendre nodes/weights on [x1,x2] by Newton iteration on P_n (ascending). */
 void or_gauleg(double x1, double x2, double *x, double *w, int n)
@@ -89,6 +104,55 @@ void or_ipw_EH(const or_model *m, const double *x, double complex *e, double com
 {
   double complex de[3], dh[3];
   or_ipw_EH_dv(m, x, NULL, e, h, de, dh);
+}
+
+static void rule_alloc(or_rule *r, int n)
+{
+  r->n = n;
+  r->z = malloc(sizeof(double) * n);
+  r->e = malloc(sizeof(double) * n);
+  r->w = malloc(sizeof(double) * n);
+}
+
+/* Tensor Gauss-Legendre rule on the square [-1,1]^2 (reference bil_*_GL/_GH loops). */
+static void rule_quad_gl(or_rule *r, int n, const double *x, const double *w)
+{
+  rule_alloc(r, n * n);
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < n; j++) {
+      r->z[n * i + j] = x[i];
+      r->e[n * i + j] = x[j];
+      r->w[n * i + j] = w[i] * w[j];
+    }
+}
+
+/* Gauss-Legendre rule on the reference triangle: the triangle is split into three
+ * quadrilaterals at the edge midpoints and the centroid, each mapped bilinearly from
+ * (a, b) in [-1,1]^2 with Jacobian (sqrt3/16)(1 + a/4 + b/4)
+ * (reference lit_dcoef_v2_GL d3qd2_src/lit_dcoef_v2.c:213-236). */
+static void rule_tri_gl(or_rule *r, int n, const double *x, const double *w)
+{
+  rule_alloc(r, 3 * n * n);
+  int q = 0;
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < n; j++) {
+      double a = x[i], b = x[j], wt = 0.0625 * SQ3 * (1.0 + 0.25 * a + 0.25 * b) * w[i] * w[j];
+      r->z[q] = 0.125 * (3.0 + 2.0 * a + 2.0 * b + a * b);
+      r->e[q] = 0.125 * SQ3 * (-a + b);
+      r->w[q++] = wt;
+      r->z[q] = 0.0625 * (-3.0 + a - 5.0 * b - a * b);
+      r->e[q] = 0.0625 * SQ3 * (3.0 + 3.0 * a + b + a * b);
+      r->w[q++] = wt;
+      r->z[q] = 0.0625 * (-3.0 - 5.0 * a + b - a * b);
+      r->e[q] = 0.0625 * SQ3 * (-3.0 - a - 3.0 * b - a * b);
+      r->w[q++] = wt;
+    }
+}
+
+static void rule_list(or_rule *r, int n, const double *z, const double *e, const double *w, double scale)
+{
+  rule_alloc(r, n);
+  for (int i = 0; i < n; i++) { r->z[i] = z[i]; r->e[i] = e[i]; r->w[i] = w[i] * scale; }
 }
 
 or_model *or_model_new(int MN, const double *n_re_im, int ps, const double *vd1, const double *vd2,
@@ -180,8 +244,24 @@ or_model *or_model_new(int MN, const double *n_re_im, int ps, const double *vd1,
   const double w49[9] = {P49_W0, P49_W0, P49_W0, P49_W0, P49_W1, P49_W1, P49_W1, P49_W1, P49_W2};
   for (int j = 0; j < 4; j++) { m->zt44[j] = z44[j]; m->et44[j] = e44[j]; m->wt44[j] = 1.0; }
   for (int j = 0; j < 9; j++) { m->zt49[j] = z49[j]; m->et49[j] = e49[j]; m->wt49[j] = w49[j]; }
+  const double z34[4] = {-P34_N0, 2.0 * P34_N0, -P34_N0, 0.0}, e34[4] = {-P34_N1, 0.0, P34_N1, 0.0};
+  const double w34[4] = {P34_W0, P34_W0, P34_W0, -P34_W1};
+  const double z37[7] = {-P37_N0, 2.0 * P37_N0, -P37_N0, P37_N1, -2.0 * P37_N1, P37_N1, 0.0};
+  const double e37[7] = {-P37_N2, 0.0, P37_N2, P37_N3, 0.0, -P37_N3, 0.0};
+  const double w37[7] = {P37_W0, P37_W0, P37_W0, P37_W1, P37_W1, P37_W1, P37_W2};
+  for (int j = 0; j < 4; j++) { m->zt34[j] = z34[j]; m->et34[j] = e34[j]; m->wt34[j] = w34[j]; }
+  for (int j = 0; j < 7; j++) { m->zt37[j] = z37[j]; m->et37[j] = e37[j]; m->wt37[j] = w37[j]; }
   or_gauleg(-1.0, 1.0, m->xli, m->wli, 12);
   or_gauleg(-1.0, 1.0, m->xhi, m->whi, 64);
+  /* the 4- and 7-point triangle weights are fractions of the area 3 sqrt3 / 4 */
+  rule_list(&m->rule[0][0], 4, m->zt44, m->et44, m->wt44, 1.0);
+  rule_list(&m->rule[0][1], 9, m->zt49, m->et49, m->wt49, 1.0);
+  rule_quad_gl(&m->rule[0][2], 12, m->xli, m->wli);
+  rule_quad_gl(&m->rule[0][3], 64, m->xhi, m->whi);
+  rule_list(&m->rule[1][0], 4, m->zt34, m->et34, m->wt34, 0.75 * SQ3);
+  rule_list(&m->rule[1][1], 7, m->zt37, m->et37, m->wt37, 0.75 * SQ3);
+  rule_tri_gl(&m->rule[1][2], 12, m->xli, m->wli);
+  rule_tri_gl(&m->rule[1][3], 64, m->xhi, m->whi);
 
   /* sub-domain lists: main side positive, sub side negative (d3qd2_setup.c:1290-1307) */
   m->dNe = calloc(MN + 1, sizeof(int));
@@ -196,15 +276,17 @@ or_model *or_model_new(int MN, const double *n_re_im, int ps, const double *vd1,
     }
   }
 
-  /* element nodes and incident field (d3qd2_setup.c:1309-1349; triangles not yet) */
+  /* element nodes and incident field (d3qd2_setup.c:1309-1349): quads at the 2x2 Gauss
+   * points, triangles at the three collocation nodes and the centroid */
   m->ren = calloc(12 * (Ne + 1), sizeof(double));
   m->wen = calloc(12 * (Ne + 1), sizeof(double));
   m->Ui = calloc(16 * (Ne + 1), sizeof(double complex));
   m->dUi = calloc(16 * (Ne + 1), sizeof(double complex));
   for (int i = 1; i <= Ne; i++) {
     const double *cr = m->cr + 12 * i, *cw = m->cw + 9 * i;
+    int tri = (m->ed[4 * i + 3] == 0);
     for (int j = 0; j < 4; j++) {
-      double z = m->zt44[j], e = m->et44[j], r[3], w[3];
+      double z = tri ? m->zt34[j] : m->zt44[j], e = tri ? m->et34[j] : m->et44[j], r[3], w[3];
       for (int d = 0; d < 3; d++) {
         r[d] = cr[4 * d] + cr[4 * d + 1] * z + cr[4 * d + 2] * e + cr[4 * d + 3] * z * e;
         w[d] = cw[3 * d] + cw[3 * d + 1] * z + cw[3 * d + 2] * e;
@@ -233,6 +315,8 @@ void or_model_free(or_model *m)
   free(m->sid); free(m->dNe); free(m->ipw); free(m->n); free(m->kn);
   free(m->rn); free(m->ed); free(m->eni); free(m->md); free(m->sd);
   free(m->cr); free(m->cw); free(m->ren); free(m->wen); free(m->Ui); free(m->dUi);
+  for (int a = 0; a < 2; a++)
+    for (int b = 0; b < 4; b++) { free(m->rule[a][b].z); free(m->rule[a][b].e); free(m->rule[a][b].w); }
   free(m);
 }
 
@@ -249,4 +333,5 @@ void or_model_export(const or_model *m, double *ren, double *wen, double *Ui, do
 }
 
 int or_model_dNe(const or_model *m, int d) { return m->dNe[d]; }
+int or_model_NN(const or_model *m) { return m->NN; }
 int or_model_sid(const or_model *m, int d, int i) { return m->sid[d][i]; }

@@ -178,19 +178,6 @@ static void inv33(double *a, const double *vn, const double *vtz, const double *
   for (int i = 0; i < 9; i++) a[i] *= idet;
 }
 
-static void tangents(const or_model *m, int s, int nid, double *vtz, double *vte)
-{
-  const double *cr = m->cr + 12 * s;
-  double et = m->et44[nid], zt = m->zt44[nid];
-  for (int d = 0; d < 3; d++) {
-    vtz[d] = cr[4 * d + 1] + cr[4 * d + 3] * et;
-    vte[d] = cr[4 * d + 2] + cr[4 * d + 3] * zt;
-  }
-  double a = sqrt(vtz[0] * vtz[0] + vtz[1] * vtz[1] + vtz[2] * vtz[2]);
-  double b = sqrt(vte[0] * vte[0] + vte[1] * vte[1] + vte[2] * vte[2]);
-  for (int d = 0; d < 3; d++) { vtz[d] /= a; vte[d] /= b; }
-}
-
 /* E = -grad(phi) + A, H = (lambda0 / 2 pi i) curl A from U, dU/dn and the tangential
  * derivatives; gradient components recovered through the inverse of [n, tz, te]. */
 static void eh_from(const double *a, const double complex *U, const double complex *dUn,
@@ -220,7 +207,7 @@ void or_boundary_EH(const or_model *m, double complex **U, double complex **dU,
         if (tn == 3 && is_tri(m, atd)) continue;
         size_t row = 4 * (size_t)(t - 1) + tn;
         double vtz[3], vte[3], vn[3], a[9];
-        tangents(m, atd, tn, vtz, vte);
+        or_node_tangents(m, atd, tn, vtz, vte);
         const double *w = m->wen + 12 * atd + 3 * tn;
         double aw = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
         for (int i = 0; i < 3; i++) vn[i] = sig * w[i] / aw;

@@ -40,6 +40,8 @@ typedef struct {
   double complex ex, ey, ez, hx, hy, hz;
 } or_ipw;
 
+typedef struct { int n; double *z, *e, *w; } or_rule;
+
 typedef struct {
   int MN, ps, Nn, Ne, NN, n_ipw;
   double lambda0, n0;
@@ -56,6 +58,12 @@ typedef struct {
   int **sid;                  /* sid[d][1..dNe[d]] signed element ids */
   double xli[12], wli[12], xhi[64], whi[64];
   double zt44[4], et44[4], wt44[4], zt49[9], et49[9], wt49[9];
+  double zt34[4], et34[4], wt34[4], zt37[7], et37[7], wt37[7];
+  /* Quadrature point lists over the reference element in (zeta, eta), weights in
+   * reference-area units (sum = 4 for the square, 3 sqrt(3)/4 for the triangle):
+   * rule[shape][type], shape 0 = bilinear quad, 1 = linear triangle; type 0 = element
+   * nodes (4p), 1 = 9p / 7p, 2 = GL (12), 3 = GH (64). */
+  or_rule rule[2][4];
 } or_model;
 
 or_model *or_model_new(int MN, const double *n_re_im, int ps, const double *vd1, const double *vd2,
@@ -65,6 +73,7 @@ void or_model_free(or_model *m);
 void or_model_export(const or_model *m, double *ren, double *wen, double *Ui, double *dUi,
                      double *qd, double *kn);
 int or_model_dNe(const or_model *m, int d);
+int or_model_NN(const or_model *m);
 int or_model_sid(const or_model *m, int d, int i);
 void or_gauleg(double x1, double x2, double *x, double *w, int n);
 void or_ipw_EH(const or_model *m, const double *x, double complex *e, double complex *h);
@@ -79,6 +88,8 @@ void or_ipw_EH_dv(const or_model *m, const double *x, const double *v, double co
  * used, 2 if singular, 0 otherwise. */
 int or_pair(const or_model *m, int did, int t, int tn, int s, int type, double complex *CC,
             double complex *dCz, double complex *dCe);
+/* Unit tangents at node nid of element s (reference tz_te_bd_node d3qd2_src/d3qd2_elem.c:122-143). */
+void or_node_tangents(const or_model *m, int s, int nid, double *vtz, double *vte);
 /* Coefficients for an arbitrary field point (reference q0_coef_rt / coef_rt). */
 void or_coef_rt(const or_model *m, int did, const double *rt, int s, int type, double complex *CC);
 

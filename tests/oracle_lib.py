@@ -34,6 +34,8 @@ def oracle() -> ctypes.CDLL:
         o.or_model_export.argtypes = [ctypes.c_void_p, D, D, D, D, D, D]
         o.or_model_dNe.argtypes = [ctypes.c_void_p, ctypes.c_int]
         o.or_model_sid.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+        o.or_model_NN.argtypes = [ctypes.c_void_p]
+        o.or_node_tangents.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, D, D]
         o.or_pair.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                               ctypes.c_int, D, D, D]
         o.or_coef_rt.argtypes = [ctypes.c_void_p, ctypes.c_int, D, ctypes.c_int, ctypes.c_int, D]

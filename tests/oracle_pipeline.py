@@ -32,7 +32,8 @@ class OracleModel:
                                 self.sd.ctypes.data_as(Ip))
         self.MN, self.Ne = MN, len(ed) - 1
         self.dNe = [o.or_model_dNe(self.h, d) for d in range(MN + 1)]
-        self.NN = 4 * self.Ne  # quads only in the oracle
+        self.NN = o.or_model_NN(self.h)
+        self.tri = self.ed[:, 3] == 0
 
     @classmethod
     def from_dat(cls, d):
@@ -61,6 +62,14 @@ class OracleModel:
 
     def sid(self, d):
         return np.array([0] + [self.o.or_model_sid(self.h, d, i) for i in range(1, self.dNe[d] + 1)], np.int32)
+
+    def live_rows(self, did):
+        """Rows/columns of the 4-per-element domain layout that hold a node (a triangle's
+        fourth slot is empty; solve_deh_bv d3qd2_src/d3qd2_solve_bieq.c:680-760 drops it)."""
+        sid = np.abs(self.sid(did)[1:])
+        keep = np.ones((len(sid), 4), bool)
+        keep[self.tri[sid], 3] = False
+        return keep.ravel()
 
     def assemble(self, did, t0=1, t1=None, deriv=True):
         N = 4 * self.dNe[did]
@@ -106,7 +115,9 @@ class OracleModel:
             Bd = np.zeros((N, 6), complex)
             self.o.or_deh_rhs(self.h, d, ptr(mats[d][1].view(float)), ptr(E[d].view(float)), ptr(H[d].view(float)),
                               ptr(Bd.view(float)))
-            Xd = sla.solve(mats[d][0], Bd, check_finite=False)
+            keep = self.live_rows(d)
+            Xd = np.zeros((N, 6), complex)
+            Xd[keep] = sla.solve(mats[d][0][np.ix_(keep, keep)], Bd[keep], check_finite=False)
             dE[d][1:] = Xd[:, :3].reshape(-1, 4, 3)
             dH[d][1:] = Xd[:, 3:].reshape(-1, 4, 3)
         return dict(U=U, dU=dU, E=E, H=H, dE=dE, dH=dH, X=X, mats=mats)
