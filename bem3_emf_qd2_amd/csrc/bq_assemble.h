@@ -23,6 +23,8 @@ typedef struct bq_adom {
   double *dG, *dH;    /* device out: 2N x N complex (free-space domains) */
   double *tdF;        /* device out: N x 3 (may be NULL) */
   int *n_gh_out;      /* host out: number of pairs that used the 64x64 rule (may be NULL) */
+  long long *n_points_out; /* host out [3]: integrand points of the 12-point, 64-point and
+                              singular rules (may be NULL) */
 } bq_adom;
 
 size_t bq_assemble_work_bytes(int dNe);

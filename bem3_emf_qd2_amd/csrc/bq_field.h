@@ -13,6 +13,7 @@ extern "C" {
 /* device-side view passed to the field kernel by value */
 typedef struct bq_fctx {
   int ps, MN;
+  int has_tri;                           /* any linear triangles */
   bq_qplan plan;
   const double *E, *ren, *wen, *kn;      /* device */
   int dNe[BQ_MAXD];
@@ -25,11 +26,12 @@ typedef struct bq_fdata {
   int Ne, MN, ps;
   double qd[8];
   const double *kn;                      /* [MN+1] complex */
-  const double *E, *ren, *wen;           /* [(Ne+1)][24], [(Ne+1)][12], [(Ne+1)][12] */
+  const double *E, *ren, *wen;           /* [(Ne+1)][24] (bq_pack_elem), [(Ne+1)][12], [(Ne+1)][12] */
   const int *dNe;
   const int *const *sid;
   const double *const *Eb, *const *dEb, *const *Hb, *const *dHb;
   const double *xli, *wli, *xhi, *whi, *z49, *e49, *w49;
+  const double *w34, *z37, *e37, *w37;   /* triangle node-rule weights and 7-point rule */
 } bq_fdata;
 
 int bq_field_ctx_create(const bq_fdata *f, void **handle);
@@ -40,7 +42,8 @@ void bq_field_ctx_destroy(void *handle);
 
 int bq_field_eval_dev(const bq_fctx *C, int npts, const double *d_pts, const int *d_did, int type, double *d_out,
                       void *stream);
-int bq_field_rules_init(const double *z49, const double *e49, const double *w49);
+int bq_field_rules_init(const double *z49, const double *e49, const double *w49, const double *w34,
+                        const double *z37, const double *e37, const double *w37);
 
 #ifdef __cplusplus
 }

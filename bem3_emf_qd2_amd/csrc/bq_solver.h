@@ -16,7 +16,7 @@ typedef struct bq_model {
   int Ne, NN, MN, ps;
   double qd[8];          /* QPD2: k, vk[3], vd1[2], vd2[2] */
   const double *kn;      /* [MN+1] complex */
-  const double *E;       /* [(Ne+1)][24] element constants cr[3][4], cw[3][3], pad */
+  const double *E;       /* [(Ne+1)][24] element constants, bq_pack_elem layout */
   const double *ren;     /* [(Ne+1)][4][3] */
   const double *nrm;     /* [(Ne+1)][4][3] unit normals at element nodes (main side) */
   const int *eni;        /* [(Ne+1)][4] */
@@ -31,9 +31,17 @@ typedef struct bq_stats {
   double t_assemble_ms, t_solve_ms, t_fields_ms;
   int n_gh_pairs;        /* pairs that needed the 64x64 rule, all domains */
   long long n_qpgf;      /* QPGF evaluations in the assembly (algorithmic count) */
+  long long n_qpgf_gl, n_qpgf_gh, n_qpgf_sing; /* the same split by kernel phase */
 } bq_stats;
 
 typedef struct bq_sess bq_sess;
+
+/* Element constants in the device layout (24 doubles): cr[3][4], cw[3][3] (dimension-major
+ * as the reference's cr[s][d][c], cw[s][d][c]), [21] = shape (0 bilinear quad, 1 linear
+ * triangle), 2 pad.  A triangle's cw is stored times 2/(3 sqrt3), which makes |W| the
+ * surface Jacobian of the map from the reference triangle, as |W| already is for the
+ * bilinear map (reference d3qd2_src/d3qd2_setup.c:1150-1185). */
+void bq_pack_elem(double *const *cr, double *const *cw, int tri, double *dst);
 
 bq_sess *bq_sess_create(const bq_model *m);
 void bq_sess_destroy(bq_sess *s);

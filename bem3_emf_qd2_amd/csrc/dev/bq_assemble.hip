@@ -10,8 +10,9 @@
 // 12x12 rule (CBD_SDF), and polar singular rules when t == s.
 //
 // GPU organisation (one launch per phase, all on one stream):
-//   k_classify   one thread per (row, s): F and dF/dt with the 12x12 rule (no Green's
-//                function needed), flags the pair GL / GH / singular.
+//   k_classify   one thread per (row, s): F and dF/dt with the 12-point rule (no Green's
+//                function needed), flags the pair GL / GH / singular / no such row (the
+//                fourth slot of a linear triangle).
 //   k_q0_gl      one wavefront per element pair (t, s): its 4 collocation nodes x 144 Gauss
 //                points = 576 = 9 x 64 QPGF evaluations; one QPGF per Gauss point feeds
 //                all four interpolation functions (the reference evaluates it 4 times).
@@ -23,6 +24,10 @@
 //                dG/dt and dH/dt; the singular derivatives use the principal-value split
 //                of d3qd2_src/bil_dcoef.c:369-739.
 //   k_row_diag   fixed-order row sums of F into the diagonal term and tdF.
+// Linear triangles use the same kernels: the Gauss rule over a triangle has 3 n^2 points
+// (three mapped quadrilaterals), the polar rules 3 sectors, the interpolation functions
+// are linear, and the inner disk of the principal-value derivative has closed-form radial
+// integrals (lit_* / q0_lit_* of the reference).  A triangle's fourth row and column stay 0.
 // Every output element is written by exactly one thread: no atomics on values, so results
 // do not depend on scheduling.
 #include "bq_runtime.hpp"
@@ -77,6 +82,11 @@ __global__ void __launch_bounds__(256) k_classify(const double* __restrict__ E, 
   int row = (int)(idx / dNe), sc = (int)(idx % dNe);
   int td = sid[row / 4 + 1], tn = row & 3, sd = sid[sc + 1];
   double* o = F3 + 3 * idx;
+  if (tn == 3 && elem_is_tri(E, abs(td))) {
+    flag[idx] = 3;
+    o[0] = o[1] = o[2] = 0.0;
+    return;
+  }
   if (td == sd) {
     flag[idx] = 2;
     o[0] = o[1] = o[2] = 0.0;
@@ -87,20 +97,16 @@ __global__ void __launch_bounds__(256) k_classify(const double* __restrict__ E, 
   Elem e;
   load_elem(E, abs(sd), e);
   double F = 0, F0 = 0, F1 = 0;
-  for (int i = 0; i < 12; ++i) {
-    double tf = 0, t0 = 0, t1 = 0;
-    for (int j = 0; j < 12; ++j) {
-      double R[3], W[3], f, f0, f1;
-      rw_at(e, c_xl[i], c_xl[j], R, W);
-      R[0] -= c.rt[0]; R[1] -= c.rt[1]; R[2] -= c.rt[2];
-      f_terms(R, W, c, f, f0, f1);
-      tf += c_wl[j] * f;
-      t0 += c_wl[j] * f0;
-      t1 += c_wl[j] * f1;
-    }
-    F += c_wl[i] * tf;
-    F0 += c_wl[i] * t0;
-    F1 += c_wl[i] * t1;
+  const int np = gauss_points(e.tri, false);
+  for (int p = 0; p < np; ++p) {
+    double z, et, w, R[3], W[3], f, f0, f1;
+    gauss_point(e.tri, false, p, z, et, w);
+    rw_at(e, z, et, R, W);
+    R[0] -= c.rt[0]; R[1] -= c.rt[1]; R[2] -= c.rt[2];
+    f_terms(R, W, c, f, f0, f1);
+    F += w * f;
+    F0 += w * f0;
+    F1 += w * f1;
   }
   double sg = (freespace && sd < 0) ? -kI4PI : kI4PI;
   o[0] = F * sg;
@@ -129,6 +135,7 @@ k_q0_gl(bq_qplan P, const double* __restrict__ E, const double* __restrict__ ren
   const int tn = lane >> 4, sub = lane & 15;
   const long N = 4L * dNe, row = 4L * (t - 1) + tn;
   const bool active = flag[row * dNe + (s - 1)] == 0;
+  const bool stri = elem_is_tri(E, abs(sid[s]));
   double* acc = &sacc[0][threadIdx.x];  // acc[k * 256]
 #pragma unroll
   for (int k = 0; k < 16; ++k) acc[k * 256] = 0.0;
@@ -144,10 +151,11 @@ k_q0_gl(bq_qplan P, const double* __restrict__ E, const double* __restrict__ ren
       rt[1] = p[1];
       rt[2] = p[2];
     }
+    const int nm = stri ? 27 : 9;  // 144 or 432 points, 16 lanes per node
 #pragma unroll 1
-    for (int m = 0; m < 9; ++m) {
-      int p = sub + 16 * m, i = p / 12, j = p - 12 * i;
-      double z = c_xl[i], et = c_xl[j], w = c_wl[i] * c_wl[j];
+    for (int m = 0; m < nm; ++m) {
+      double z, et, w;
+      gauss_point(stri, false, sub + 16 * m, z, et, w);
       double R[3], W[3], Nn[4];
       {
         Elem e;
@@ -159,7 +167,7 @@ k_q0_gl(bq_qplan P, const double* __restrict__ E, const double* __restrict__ ren
       qpgf_v1(P, R[0], R[1], R[2], q);
       double aW = sqrt(dot3(W, W));
       cplx hn = q.d[0] * W[0] + q.d[1] * W[1] + q.d[2] * W[2];
-      shapeN4(z, et, Nn);
+      shapeN(stri, z, et, Nn);
 #pragma unroll
       for (int n = 0; n < 4; ++n) {
         double gw = w * Nn[n] * aW, hw = w * Nn[n];
@@ -201,10 +209,11 @@ k_q0_gh(bq_qplan P, const double* __restrict__ E, const double* __restrict__ ren
 #pragma unroll
   for (int k = 0; k < 16; ++k) acc[k * 256] = 0.0;
   double fs[3] = {0.0, 0.0, 0.0};
+  const int nm = e.tri ? 48 : 16;  // 4096 or 12288 points
 #pragma unroll 1
-  for (int m = 0; m < 16; ++m) {
-    int p = threadIdx.x + 256 * m, i = p >> 6, j = p & 63;
-    double z = c_xh[i], et = c_xh[j], w = c_wh[i] * c_wh[j];
+  for (int m = 0; m < nm; ++m) {
+    double z, et, w;
+    gauss_point(e.tri, true, threadIdx.x + 256 * m, z, et, w);
     double R[3], W[3], Nn[4], f, f0, f1;
     rw_at(e, z, et, R, W);
     R[0] -= c.rt[0]; R[1] -= c.rt[1]; R[2] -= c.rt[2];
@@ -216,7 +225,7 @@ k_q0_gh(bq_qplan P, const double* __restrict__ E, const double* __restrict__ ren
     qpgf_v1(P, R[0], R[1], R[2], q);
     double aW = sqrt(dot3(W, W));
     cplx hn = q.d[0] * W[0] + q.d[1] * W[1] + q.d[2] * W[2];
-    shapeN4(z, et, Nn);
+    shapeN(e.tri, z, et, Nn);
 #pragma unroll
     for (int n = 0; n < 4; ++n) {
       double gw = w * Nn[n] * aW, hw = w * Nn[n];
@@ -248,11 +257,11 @@ k_q0_gh(bq_qplan P, const double* __restrict__ E, const double* __restrict__ ren
   }
 }
 
-// polar F of element e at its own parameter point (zt, et), unscaled by bdC/4pi
+// polar F of a bilinear element e at its own parameter point (zt, et), unscaled by bdC/4pi
 __device__ double polar_F_part(const Elem& e, double zt, double et)
 {
   double r[5], th[5];
-  corner_r_th(zt, et, r, th);
+  corner_r_th4(zt, et, r, th);
   double F = 0.0;
   for (int q = threadIdx.x; q < 512; q += 256) {
     int j = q >> 7, m = q & 127;
@@ -284,13 +293,14 @@ k_q0_sing(bq_qplan P, const double* __restrict__ E, const int* __restrict__ sid,
   const int t = (int)(row / 4) + 1, tn = (int)(row & 3), as = abs(sid[t]);
   Elem e;
   load_elem(E, as, e);
-  const double zt = node_z(tn), et = node_e(tn);
+  if (e.tri && tn == 3) return;  // no fourth node (whole block)
+  const double zt = elem_node_z(e.tri, tn), et = elem_node_e(e.tri, tn);
   double r[5], th[5];
-  corner_r_th(zt, et, r, th);
+  const int nsec = corner_r_th(e.tri, zt, et, r, th);
   double* sa = &sacc[0][threadIdx.x];
 #pragma unroll
   for (int k = 0; k < 16; ++k) sa[k * 256] = 0.0;
-  for (int q = threadIdx.x; q < 512; q += 256) {
+  for (int q = threadIdx.x; q < 128 * nsec; q += 256) {
     int j = q >> 7, m = q & 127;
     double theta, wt;
     polar_theta(th, j, m, theta, wt);
@@ -317,7 +327,7 @@ k_q0_sing(bq_qplan P, const double* __restrict__ E, const int* __restrict__ sid,
       }
       double aW = sqrt(dot3(W, W));
       cplx hn = qq.d[0] * W[0] + qq.d[1] * W[1] + qq.d[2] * W[2];
-      shapeN4(z, ee, Nn);
+      shapeN(e.tri, z, ee, Nn);
 #pragma unroll
       for (int n = 0; n < 4; ++n) {
         double gw = w * Nn[n] * aW * rr, hw = w * Nn[n] * rr;
@@ -331,12 +341,17 @@ k_q0_sing(bq_qplan P, const double* __restrict__ E, const int* __restrict__ sid,
   double acc[21];
 #pragma unroll
   for (int k = 0; k < 16; ++k) acc[k] = sa[k * 256];
-  // F at the node and at the four shifted points (central differences, CDFH)
-  acc[16] = polar_F_part(e, zt, et);
-  acc[17] = polar_F_part(e, zt + kCDFH, et);
-  acc[18] = polar_F_part(e, zt - kCDFH, et);
-  acc[19] = polar_F_part(e, zt, et + kCDFH);
-  acc[20] = polar_F_part(e, zt, et - kCDFH);
+  // F at the node and at the four shifted points (central differences, CDFH); a flat
+  // triangle has F = dF/dt = 0 at its own nodes (q0_lit_dcoef_v2_cc.c:331-342)
+  if (e.tri) {
+    acc[16] = acc[17] = acc[18] = acc[19] = acc[20] = 0.0;
+  } else {
+    acc[16] = polar_F_part(e, zt, et);
+    acc[17] = polar_F_part(e, zt + kCDFH, et);
+    acc[18] = polar_F_part(e, zt - kCDFH, et);
+    acc[19] = polar_F_part(e, zt, et + kCDFH);
+    acc[20] = polar_F_part(e, zt, et - kCDFH);
+  }
   block_sum256<21>(acc, lds);
   if (threadIdx.x == 0) {
     double2* g = G + row * N + 4L * (t - 1);
@@ -436,14 +451,15 @@ __global__ void __launch_bounds__(256) k_fs_gl(double kr, double ki, const doubl
 #pragma unroll
   for (int q = 0; q < 48; ++q) acc[q] = 0.0;
   if (active) {
-    for (int m = 0; m < 9; ++m) {
-      int p = sub + 16 * m, i = p / 12, j = p - 12 * i;
-      double z = c_xl[i], et = c_xl[j];
+    const int nm = e.tri ? 27 : 9;
+    for (int m = 0; m < nm; ++m) {
+      double z, et, w;
+      gauss_point(e.tri, false, sub + 16 * m, z, et, w);
       double R[3], W[3], Nn[4];
       rw_at(e, z, et, R, W);
       R[0] -= c.rt[0]; R[1] -= c.rt[1]; R[2] -= c.rt[2];
-      shapeN4(z, et, Nn);
-      fs_point(R, W, Nn, c, k, c_wl[i] * c_wl[j], acc);
+      shapeN(e.tri, z, et, Nn);
+      fs_point(R, W, Nn, c, k, w, acc);
     }
   }
   group_sum<16, 48>(acc);
@@ -469,9 +485,10 @@ __global__ void __launch_bounds__(256) k_fs_gh(double kr, double ki, const doubl
 #pragma unroll
   for (int q = 0; q < 48; ++q) acc[q] = 0.0;
   double fa[3] = {0.0, 0.0, 0.0};
-  for (int m = 0; m < 16; ++m) {
-    int p = threadIdx.x + 256 * m, i = p >> 6, j = p & 63;
-    double z = c_xh[i], et = c_xh[j], w = c_wh[i] * c_wh[j];
+  const int nm = e.tri ? 48 : 16;
+  for (int m = 0; m < nm; ++m) {
+    double z, et, w;
+    gauss_point(e.tri, true, threadIdx.x + 256 * m, z, et, w);
     double R[3], W[3], Nn[4], f, f0, f1;
     rw_at(e, z, et, R, W);
     R[0] -= c.rt[0]; R[1] -= c.rt[1]; R[2] -= c.rt[2];
@@ -479,7 +496,7 @@ __global__ void __launch_bounds__(256) k_fs_gh(double kr, double ki, const doubl
     fa[0] += w * f;
     fa[1] += w * f0;
     fa[2] += w * f1;
-    shapeN4(z, et, Nn);
+    shapeN(e.tri, z, et, Nn);
     fs_point(R, W, Nn, c, k, w, acc);
   }
   block_sum256<48>(acc, lds);
@@ -506,10 +523,11 @@ __global__ void __launch_bounds__(256) k_fs_sing(double kr, double ki, const dou
   const int t = (int)(row / 4) + 1, tn = (int)(row & 3), sd = sid[t];
   Elem e;
   load_elem(E, abs(sd), e);
+  if (e.tri && tn == 3) return;  // no fourth node (whole block)
   const cplx k = mk(kr, ki);
-  const double zt = node_z(tn), et = node_e(tn);
+  const double zt = elem_node_z(e.tri, tn), et = elem_node_e(e.tri, tn);
   double r[5], th[5];
-  corner_r_th(zt, et, r, th);
+  const int nsec = corner_r_th(e.tri, zt, et, r, th);
   const bool plane = elem_plane(e);
   const double bdC = elem_bdC(e);
   const double sgn = (sd < 0) ? -1.0 : 1.0;
@@ -520,7 +538,7 @@ __global__ void __launch_bounds__(256) k_fs_sing(double kr, double ki, const dou
     double a[17];
 #pragma unroll
     for (int q = 0; q < 17; ++q) a[q] = 0.0;
-    for (int q = threadIdx.x; q < 512; q += 256) {
+    for (int q = threadIdx.x; q < 128 * nsec; q += 256) {
       int j = q >> 7, m = q & 127;
       double theta, wt;
       polar_theta(th, j, m, theta, wt);
@@ -538,7 +556,7 @@ __global__ void __launch_bounds__(256) k_fs_sing(double kr, double ki, const dou
         cplx ce = cexp(cmul_i(kR));
         cplx g = ce * (aW / aK);
         cplx h = (cmul_i(kR) - mk(1.0, 0.0)) * ce * (cth * sth / (aK * aK * aK));
-        shapeN4(z, ee, Nn);
+        shapeN(e.tri, z, ee, Nn);
 #pragma unroll
         for (int n = 0; n < 4; ++n) {
           a[2 * n] += w * Nn[n] * g.re;
@@ -571,7 +589,7 @@ __global__ void __launch_bounds__(256) k_fs_sing(double kr, double ki, const dou
 #pragma unroll
     for (int q = 0; q < 17; ++q) a[q] = 0.0;
     // D-bar region: r in [DEPS, rm], 64 radii
-    for (int q = threadIdx.x; q < 512; q += 256) {
+    for (int q = threadIdx.x; q < 128 * nsec; q += 256) {
       int j = q >> 7, m = q & 127;
       double theta, wt;
       polar_theta(th, j, m, theta, wt);
@@ -592,7 +610,7 @@ __global__ void __launch_bounds__(256) k_fs_sing(double kr, double ki, const dou
         cplx cb = cmul_i(kR) - mk(1.0, 0.0);
         cplx g = cb * ce * (iK * iK / rr * (KT * iK) * aW);
         cplx h = ce * ((kR * kR + 3.0 * cb) * (KT * csd) - cb * (aK * aK)) * (outer * iK5 / rr);
-        shapeN4(z, ee, Nn);
+        shapeN(e.tri, z, ee, Nn);
 #pragma unroll
         for (int n = 0; n < 4; ++n) {
           a[2 * n] += w * Nn[n] * g.re;
@@ -603,6 +621,39 @@ __global__ void __launch_bounds__(256) k_fs_sing(double kr, double ki, const dou
         a[16] += w * outer * (3.0 * KT * csd - aK * aK) * iK5 / rr;
       }
     }
+    if (e.tri) {
+      // Linear triangle: N(zt + r c, et + r s) = N(zt, et) + r tri_dN(c, s).  The constant
+      // part cancels in the principal value and the radial integrals of the linear part
+      // over [0, DEPS] are closed forms, leaving theta rules (lit_sdG_De_theta and
+      // lit_sdG_DP_theta, d3qd2_src/lit_dcoef.c:906-952, combined as i/k De - 1/k DP).
+      const double aWj = sqrt(e.cw[0][0] * e.cw[0][0] + e.cw[1][0] * e.cw[1][0] + e.cw[2][0] * e.cw[2][0]);
+      const cplx ik = cdiv(mk(aWj, 0.0), k);  // |W| / k
+      for (int q = threadIdx.x; q < 384; q += 256) {
+        const bool disk = q < 256;
+        const int qq = disk ? q : q - 256, part = qq >> 6, mm = qq & 63;
+        double theta = 0.25 * bq::kPI * c_xh[mm] +
+                       (disk ? (-0.75 * bq::kPI + 0.5 * bq::kPI * part) : (part == 0 ? 0.25 * bq::kPI : 0.75 * bq::kPI));
+        double wt = 0.25 * bq::kPI * c_wh[mm], cth, sth;
+        sincos(theta, &sth, &cth);
+        double K[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) K[d] = e.cr[d][1] * cth + e.cr[d][2] * sth;
+        double aK = sqrt(dot3(K, K)), iK = 1.0 / aK, f = iK * iK * iK * (dot3(K, T) * iK) * wt;
+        cplx keK = k * (kDEPS * aK), v;
+        if (disk) {
+          cplx cs = csin(keK), cc = ccos(keK);
+          v = cmul_i(((keK + mk(0.0, 1.0)) * cs + (mk(2.0, 0.0) - cmul_i(keK)) * cc - mk(2.0, 0.0)) * ik) * f;
+        } else {
+          v = csin(keK) * ik * (-f);
+        }
+#pragma unroll
+        for (int n = 0; n < 3; ++n) {
+          double dn = tri_dN(n, cth, sth);
+          a[2 * n] += dn * v.re;
+          a[2 * n + 1] += dn * v.im;
+        }
+      }
+    } else {
     // D-eps disk: theta in [-pi, pi] (4 quarters x 64), r in [0, DEPS] (12 radii)
     for (int q = threadIdx.x; q < 256; q += 256) {
       int quarter = q >> 6, mm = q & 63;
@@ -686,6 +737,7 @@ __global__ void __launch_bounds__(256) k_fs_sing(double kr, double ki, const dou
         a[16] += ir * outer * (hp - hm);
       }
     }
+    }  // bilinear inner disk
     block_sum256<17>(a, lds);
     if (threadIdx.x == 0) {
       double sg = -kI4PI * iaT;
@@ -702,11 +754,15 @@ __global__ void __launch_bounds__(256) k_fs_sing(double kr, double ki, const dou
 }
 
 // ------------------------------------------------------------------------------------
-__global__ void k_row_diag(int dNe, int open, const double* __restrict__ F3, double2* __restrict__ H,
-                           double* __restrict__ tdF)
+__global__ void k_row_diag(int dNe, int open, const double* __restrict__ E, const int* __restrict__ sid,
+                           const double* __restrict__ F3, double2* __restrict__ H, double* __restrict__ tdF)
 {
   long row = (long)blockIdx.x * blockDim.x + threadIdx.x, N = 4L * dNe;
   if (row >= N) return;
+  if ((row & 3) == 3 && elem_is_tri(E, abs(sid[row / 4 + 1]))) {  // no such node: row stays 0
+    if (tdF) tdF[3 * row] = tdF[3 * row + 1] = tdF[3 * row + 2] = 0.0;
+    return;
+  }
   double F = 0, F0 = 0, F1 = 0;
   const double* f = F3 + 3 * row * dNe;
   for (int s = 0; s < dNe; ++s) {
@@ -721,6 +777,27 @@ __global__ void k_row_diag(int dNe, int open, const double* __restrict__ F3, dou
     tdF[3 * row + 1] = F0;
     tdF[3 * row + 2] = F1;
   }
+}
+
+// algorithmic point counts per phase: [0] 12-point rule, [1] 64-point rule, [2] singular
+// polar rules (QPGF evaluations for the periodic domain)
+__global__ void __launch_bounds__(256) k_count_points(long n, int dNe, const double* __restrict__ E,
+                                                      const int* __restrict__ sid,
+                                                      const unsigned char* __restrict__ flag,
+                                                      unsigned long long* __restrict__ cnt)
+{
+  __shared__ double lds[3 * 4];
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  double v[3] = {0.0, 0.0, 0.0};
+  if (i < n) {
+    const int f = flag[i];
+    const bool tri = elem_is_tri(E, abs(sid[(int)(i % dNe) + 1]));
+    if (f == 0) v[0] = tri ? 432.0 : 144.0;
+    else if (f == 1) v[1] = tri ? 12288.0 : 4096.0;
+    else if (f == 2) v[2] = tri ? 4608.0 : 6144.0;  // sectors x 128 angles x 12 radii
+  }
+  block_sum256<3>(v, lds);
+  if (threadIdx.x < 3 && v[threadIdx.x] != 0.0) atomicAdd(&cnt[threadIdx.x], (unsigned long long)v[threadIdx.x]);
 }
 
 __global__ void k_collect_gh(long n, const unsigned char* __restrict__ flag, int* __restrict__ list,
@@ -750,7 +827,7 @@ int bq_set_quadrature(const double* xl, const double* wl, const double* xh, cons
 size_t bq_assemble_work_bytes(int dNe)
 {
   size_t pairs = (size_t)4 * dNe * dNe;
-  return pairs * 3 * sizeof(double) + pairs + pairs * sizeof(int) + 256;
+  return 256 + pairs * 3 * sizeof(double) + pairs + pairs * sizeof(int);
 }
 
 int bq_assemble_domain_dev(const bq_adom* a, void* work, void* stream)
@@ -759,23 +836,30 @@ int bq_assemble_domain_dev(const bq_adom* a, void* work, void* stream)
   const int dNe = a->dNe;
   const long N = 4L * dNe, pairs = N * dNe;
   if (dNe <= 0) return 0;
+  // work (256-byte aligned): counters, then F3 [pairs][3], flag [pairs], GH list [pairs]
   char* w = (char*)work;
-  double* F3 = (double*)w;
-  unsigned char* flag = (unsigned char*)(w + pairs * 3 * sizeof(double));
-  int* list = (int*)(w + pairs * 3 * sizeof(double) + pairs);
-  int* count = (int*)(w + pairs * 3 * sizeof(double) + pairs + pairs * sizeof(int));
+  int* count = (int*)w;
+  unsigned long long* npts = (unsigned long long*)(w + 64);
+  double* F3 = (double*)(w + 256);
+  unsigned char* flag = (unsigned char*)(w + 256 + pairs * 3 * sizeof(double));
+  int* list = (int*)(w + 256 + pairs * 3 * sizeof(double) + pairs);
   double2* G = (double2*)a->G;
   double2* H = (double2*)a->H;
   BQ_HIP(hipMemsetAsync(G, 0, sizeof(double2) * N * N, s));
   BQ_HIP(hipMemsetAsync(H, 0, sizeof(double2) * N * N, s));
   BQ_HIP(hipMemsetAsync(count, 0, sizeof(int), s));
+  BQ_HIP(hipMemsetAsync(npts, 0, 3 * sizeof(unsigned long long), s));
   const int freespace = a->periodic ? 0 : 1;
   BQ_KTIME("k_classify", s, k_classify<<<(unsigned)((pairs + 255) / 256), 256, 0, s>>>(a->E, a->ren, a->sid, dNe, freespace, F3, flag));
   BQ_HIP(hipGetLastError());
   k_collect_gh<<<(unsigned)((pairs + 255) / 256), 256, 0, s>>>(pairs, flag, list, count);
   BQ_HIP(hipGetLastError());
+  k_count_points<<<(unsigned)((pairs + 255) / 256), 256, 0, s>>>(pairs, dNe, a->E, a->sid, flag, npts);
+  BQ_HIP(hipGetLastError());
   int ngh = 0;
+  unsigned long long np3[3] = {0, 0, 0};
   BQ_HIP(hipMemcpyAsync(&ngh, count, sizeof(int), hipMemcpyDeviceToHost, s));
+  BQ_HIP(hipMemcpyAsync(np3, npts, sizeof np3, hipMemcpyDeviceToHost, s));
   BQ_HIP(hipStreamSynchronize(s));
   const unsigned gl_blocks = (unsigned)(((long)dNe * dNe + 3) / 4);
   if (a->periodic) {
@@ -797,9 +881,11 @@ int bq_assemble_domain_dev(const bq_adom* a, void* work, void* stream)
     BQ_KTIME("k_fs_sing", s, k_fs_sing<<<(unsigned)N, 256, 0, s>>>(a->kr, a->ki, a->E, a->sid, dNe, G, H, dG, dH, F3));
     BQ_HIP(hipGetLastError());
   }
-  k_row_diag<<<(unsigned)((N + 255) / 256), 256, 0, s>>>(dNe, a->open_domain, F3, H, a->tdF);
+  k_row_diag<<<(unsigned)((N + 255) / 256), 256, 0, s>>>(dNe, a->open_domain, a->E, a->sid, F3, H, a->tdF);
   BQ_HIP(hipGetLastError());
   if (a->n_gh_out) *a->n_gh_out = ngh;
+  if (a->n_points_out)
+    for (int i = 0; i < 3; ++i) a->n_points_out[i] = (long long)np3[i];
   return 0;
 }
 

@@ -3,7 +3,10 @@
 //   E(r) = [sum_s sum_n G_n(r, s) dE_s,n - H_n(r, s) E_s,n] / C,  same for H,
 //   C = 1 + F (open region) or F, F = sum_s F(r, s)  (solid angle / 4 pi),
 // with the coefficients of q0_coef_rt / coef_rt (d3qd2_src/d3qd2_elem.c:148-185, :574-609):
-// type 0 = the 4 element nodes, 1 = 9-point rule, 2 = 12x12, 3 = 64x64 Gauss rule.
+// type 0 = the element nodes (a triangle's 3 nodes and centroid), 1 = 9-point (quad) or
+// 7-point (triangle) rule, 2 = 12-point, 3 = 64-point Gauss rule (n^2 points on a quad,
+// 3 n^2 on a triangle).  Weights are in reference-area units and |W| is the Jacobian
+// (bq_elem.hpp), so one integrand serves both shapes.
 // One 256-thread block per point; threads stride over (element, quadrature point) pairs
 // and the block reduces E, H, F in a fixed order.
 #include "bq_runtime.hpp"
@@ -19,31 +22,29 @@ using namespace bq;
 namespace {
 
 __constant__ double c_z49[9], c_e49[9], c_w49[9];
+// triangle node-rule weights and the 7-point rule, weights times the reference-triangle
+// area 3 sqrt3 / 4 (the reference's weights sum to 1)
+__constant__ double c_w34[4], c_z37[7], c_e37[7], c_w37[7];
 
 __device__ __forceinline__ cplx ld2(const double* p, long i) { return mk(p[2 * i], p[2 * i + 1]); }
 
-__device__ __forceinline__ int rule_size(int type) { return type == 0 ? 4 : type == 1 ? 9 : type == 2 ? 144 : 4096; }
+__device__ __forceinline__ int rule_size(int type, bool tri)
+{
+  return type == 0 ? 4 : type == 1 ? (tri ? 7 : 9) : gauss_points(tri, type == 3);
+}
 
-__device__ __forceinline__ void rule_point(int type, int q, double& z, double& e, double& w)
+__device__ __forceinline__ void rule_point(int type, bool tri, int q, double& z, double& e, double& w)
 {
   if (type == 0) {
-    z = node_z(q);
-    e = node_e(q);
-    w = 1.0;
+    z = elem_node_z(tri, q);
+    e = elem_node_e(tri, q);
+    w = tri ? c_w34[q] : 1.0;
   } else if (type == 1) {
-    z = c_z49[q];
-    e = c_e49[q];
-    w = c_w49[q];
-  } else if (type == 2) {
-    int i = q / 12, j = q - 12 * i;
-    z = c_xl[i];
-    e = c_xl[j];
-    w = c_wl[i] * c_wl[j];
+    z = tri ? c_z37[q] : c_z49[q];
+    e = tri ? c_e37[q] : c_e49[q];
+    w = tri ? c_w37[q] : c_w49[q];
   } else {
-    int i = q >> 6, j = q & 63;
-    z = c_xh[i];
-    e = c_xh[j];
-    w = c_wh[i] * c_wh[j];
+    gauss_point(tri, type == 3, q, z, e, w);
   }
 }
 
@@ -59,7 +60,7 @@ __global__ void __launch_bounds__(256) k_field(bq_fctx C, int npts, const double
   const double *Eb = C.Eb[did], *dEb = C.dEb[did], *Hb = C.Hb[did], *dHb = C.dHb[did];
   const bool periodic = (did == 0 && C.ps == 1);
   const cplx k = mk(C.kn[2 * did], C.kn[2 * did + 1]);
-  const int nq = rule_size(type);
+  const int nq = C.has_tri ? max(rule_size(type, false), rule_size(type, true)) : rule_size(type, false);
   double acc[13];
 #pragma unroll
   for (int i = 0; i < 13; ++i) acc[i] = 0.0;
@@ -69,20 +70,26 @@ __global__ void __launch_bounds__(256) k_field(bq_fctx C, int npts, const double
     const int sd = sid[sc + 1], as = abs(sd);
     Elem e;
     load_elem(C.E, as, e);
+    if (q >= rule_size(type, e.tri)) continue;
     double z, et, w, R[3], W[3], Nn[4];
-    rule_point(type, q, z, et, w);
+    rule_point(type, e.tri, q, z, et, w);
     if (type == 0) {
-      // the 4-point rule reads the stored node data (d3qd2_src/q0_bil_coef.c:28-30)
+      // the node rule reads the stored node data (d3qd2_src/q0_bil_coef.c:28-30,
+      // q0_lit_coef.c:29-31); a triangle's normal is constant and taken from the element
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
         R[d] = C.ren[12L * as + 3 * q + d];
-        W[d] = C.wen[12L * as + 3 * q + d];
+        W[d] = e.tri ? e.cw[d][0] : C.wen[12L * as + 3 * q + d];
       }
+      if (e.tri && q == 3) {
+        shapeN3(z, et, Nn);  // the centroid carries 1/3 of each function
+      } else {
 #pragma unroll
-      for (int n = 0; n < 4; ++n) Nn[n] = (n == q) ? 1.0 : 0.0;
+        for (int n = 0; n < 4; ++n) Nn[n] = (n == q) ? 1.0 : 0.0;
+      }
     } else {
       rw_at(e, z, et, R, W);
-      shapeN4(z, et, Nn);
+      shapeN(e.tri, z, et, Nn);
     }
     R[0] -= r0[0]; R[1] -= r0[1]; R[2] -= r0[2];
     double aR = sqrt(dot3(R, R)), iR = 1.0 / aR, aW = sqrt(dot3(W, W));
@@ -128,11 +135,20 @@ __global__ void __launch_bounds__(256) k_field(bq_fctx C, int npts, const double
 
 }  // namespace
 
-extern "C" int bq_field_rules_init(const double* z49, const double* e49, const double* w49)
+extern "C" int bq_field_rules_init(const double* z49, const double* e49, const double* w49, const double* w34,
+                                   const double* z37, const double* e37, const double* w37)
 {
+  const double area = 0.75 * sqrt(3.0);
+  double a34[4], a37[7];
+  for (int i = 0; i < 4; ++i) a34[i] = w34[i] * area;
+  for (int i = 0; i < 7; ++i) a37[i] = w37[i] * area;
   BQ_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_z49), z49, 9 * sizeof(double)));
   BQ_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_e49), e49, 9 * sizeof(double)));
   BQ_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_w49), w49, 9 * sizeof(double)));
+  BQ_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_w34), a34, 4 * sizeof(double)));
+  BQ_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_z37), z37, 7 * sizeof(double)));
+  BQ_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_e37), e37, 7 * sizeof(double)));
+  BQ_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_w37), a37, 7 * sizeof(double)));
   return 0;
 }
 
@@ -188,6 +204,8 @@ extern "C" int bq_field_ctx_create(const bq_fdata* f, void** handle)
     return -3;
   }
   size_t n1 = (size_t)f->Ne + 1;
+  for (size_t i = 1; i < n1; ++i)
+    if (f->E[24 * i + 21] != 0.0) C.has_tri = 1;
   if (up(F, f->E, n1 * 24, &C.E, s) || up(F, f->ren, n1 * 12, &C.ren, s) || up(F, f->wen, n1 * 12, &C.wen, s) ||
       up(F, f->kn, (size_t)(f->MN + 1) * 2, &C.kn, s)) {
     bq_field_ctx_destroy(F);
@@ -203,7 +221,8 @@ extern "C" int bq_field_ctx_create(const bq_fdata* f, void** handle)
       return -1;
     }
   }
-  if (bq::set_quad_tables(f->xli, f->wli, f->xhi, f->whi) || bq_field_rules_init(f->z49, f->e49, f->w49)) {
+  if (bq::set_quad_tables(f->xli, f->wli, f->xhi, f->whi) ||
+      bq_field_rules_init(f->z49, f->e49, f->w49, f->w34, f->z37, f->e37, f->w37)) {
     bq_field_ctx_destroy(F);
     return -1;
   }

@@ -16,6 +16,9 @@
 // two dense LUs of order 2NN instead of one of order 8NN (64x fewer LU flops), all
 // on the device.  The result is the same linear system's solution; only the elimination
 // order differs from a direct factorisation.
+// Domain matrices keep 4 rows and columns per element; a linear triangle's fourth row and
+// column are empty, and the global system and the dE/dH systems take the live rows only, in
+// the reference's order (create_matrix_csr_dac and solve_deh_bv skip them the same way).
 #include <vector>
 #include <hip/hip_runtime.h>
 #include "bq_runtime.hpp"
@@ -34,6 +37,9 @@ struct bq_sess {
   std::vector<int> dNe;
   std::vector<std::vector<int>> sid_h;
   std::vector<int*> sid;
+  std::vector<int*> rowmap;  // per domain, device: row 4(t-1)+tn -> live index or -1
+  std::vector<int*> live;    // per domain, device: live index -> row
+  std::vector<int> nlive;
   std::vector<double*> G, H, dG, dH, tdF;
   std::vector<double> kn_h;
   bq_qplan plan;
@@ -70,18 +76,20 @@ __device__ __forceinline__ void st2(double* p, long i, cplx v)
 }
 
 // one thread per (element s of the domain, row of the domain): entries of M, M3, K, L
-__global__ void k_build_blocks(int dNe, int d, int rowbase, int NN, const int* __restrict__ sid,
-                               const int* __restrict__ eni, const int* __restrict__ md, const int* __restrict__ sdm,
-                               const double* __restrict__ kn, const double* __restrict__ G, const double* __restrict__ H,
-                               double* __restrict__ M, double* __restrict__ M3, double* __restrict__ K,
-                               double* __restrict__ L)
+__global__ void k_build_blocks(int dNe, int d, int rowbase, int NN, const int* __restrict__ rowmap,
+                               const int* __restrict__ sid, const int* __restrict__ eni, const int* __restrict__ md,
+                               const int* __restrict__ sdm, const double* __restrict__ kn,
+                               const double* __restrict__ G, const double* __restrict__ H, double* __restrict__ M,
+                               double* __restrict__ M3, double* __restrict__ K, double* __restrict__ L)
 {
   const long N = 4L * dNe;
   long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= N * dNe) return;
   const int sc = (int)(idx / N);
   const long rl = idx % N;
-  const long r = rowbase + rl, ld = 2L * NN;
+  const int rc = rowmap[rl];
+  if (rc < 0) return;
+  const long r = rowbase + rc, ld = 2L * NN;
   const int sd = sid[sc + 1], asd = abs(sd), mdid = md[asd], sdid = sdm[asd];
   const bool main_side = (d == mdid);
   cplx beps = mk(1, 0), delt = mk(0, 0), sigm = mk(0, 0);
@@ -111,16 +119,19 @@ __global__ void k_build_blocks(int dNe, int d, int rowbase, int NN, const int* _
 }
 
 // right-hand sides: b_c (c < 3) into columns NN..NN+2 of R, b_3 into b3
-__global__ void k_build_rhs(int dNe, int d, int rowbase, int NN, const int* __restrict__ sid,
-                            const int* __restrict__ md, const int* __restrict__ sdm, const double* __restrict__ kn,
-                            const double* __restrict__ nrm, const double* __restrict__ Ui,
-                            const double* __restrict__ dUi, const double* __restrict__ G,
-                            const double* __restrict__ H, double* __restrict__ R, double* __restrict__ b3)
+__global__ void k_build_rhs(int dNe, int d, int rowbase, int NN, const int* __restrict__ rowmap,
+                            const int* __restrict__ sid, const int* __restrict__ md, const int* __restrict__ sdm,
+                            const double* __restrict__ kn, const double* __restrict__ nrm,
+                            const double* __restrict__ Ui, const double* __restrict__ dUi,
+                            const double* __restrict__ G, const double* __restrict__ H, double* __restrict__ R,
+                            double* __restrict__ b3)
 {
   const long N = 4L * dNe;
   long rl = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (rl >= N) return;
-  const long r = rowbase + rl, ld = 2L * NN;
+  const int rc = rowmap[rl];
+  if (rc < 0) return;
+  const long r = rowbase + rc, ld = 2L * NN;
   cplx b[4] = {mk(0, 0), mk(0, 0), mk(0, 0), mk(0, 0)};
   for (int sc = 0; sc < dNe; ++sc) {
     const int sd = sid[sc + 1], asd = abs(sd), mdid = md[asd], sdid = sdm[asd];
@@ -175,12 +186,26 @@ __global__ void k_scale_normals(int NN, const double* __restrict__ Y3, const dou
   for (int c = 0; c < 3; ++c) st2(T, (long)c * NN + e, x * nn3[3 * e + c]);
 }
 
-__global__ void k_transpose(long n, const double* __restrict__ A, double* __restrict__ B)
+// B (n x n, column-major) = A[live, live], A row-major N x N
+__global__ void k_gather_live(long N, int n, const int* __restrict__ live, const double* __restrict__ A,
+                              double* __restrict__ B)
 {
   long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= n * n) return;
-  long r = idx / n, c = idx % n;
-  st2(B, c * n + r, ld2(A, idx));
+  if (idx >= (long)n * n) return;
+  long j = idx / n, i = idx % n;
+  st2(B, idx, ld2(A, (long)live[i] * N + live[j]));
+}
+
+// B[c][i] = A[c][live[i]] (to_live) or A[c][live[i]] = B[c][i] (back), c < ncol; columns of
+// length N (A) and n (B)
+__global__ void k_live_rows(long N, int n, int ncol, const int* __restrict__ live, double* __restrict__ A,
+                           double* __restrict__ B, int to_live)
+{
+  long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)n * ncol) return;
+  long c = idx / n, i = idx % n;
+  if (to_live) st2(B, idx, ld2(A, c * N + live[i]));
+  else st2(A, c * N + live[i], ld2(B, idx));
 }
 
 inline unsigned nblk(long n, int b) { return (unsigned)((n + b - 1) / b); }
@@ -220,7 +245,35 @@ bq_sess* bq_sess_create(const bq_model* m)
       return nullptr;
     }
     S->sid.push_back(p);
-    S->two_nn += 4 * m->dNe[d];
+    // live rows: all 4 per quad, 3 per triangle (shape flag at E[21])
+    const long N = 4L * m->dNe[d];
+    std::vector<int> map(N), lv;
+    for (int t = 1; t <= m->dNe[d]; ++t) {
+      const bool tri = m->E[24 * (size_t)abs(m->sid[d][t]) + 21] != 0.0;
+      for (int tn = 0; tn < 4; ++tn) {
+        const long r = 4L * (t - 1) + tn;
+        if (tn == 3 && tri) {
+          map[r] = -1;
+        } else {
+          map[r] = (int)lv.size();
+          lv.push_back((int)r);
+        }
+      }
+    }
+    int *pm = nullptr, *pl = nullptr;
+    if (upload(&pm, map.data(), map.size(), s) || upload(&pl, lv.data(), lv.size(), s)) {
+      bq_sess_destroy(S);
+      return nullptr;
+    }
+    S->rowmap.push_back(pm);
+    S->live.push_back(pl);
+    S->nlive.push_back((int)lv.size());
+    S->two_nn += (int)lv.size();
+    if (hipStreamSynchronize(s) != hipSuccess) {  // map and lv are freed at the end of the loop body
+      bq::set_error("session set-up: stream synchronisation failed");
+      bq_sess_destroy(S);
+      return nullptr;
+    }
   }
   S->G.assign(m->MN + 1, nullptr);
   S->H.assign(m->MN + 1, nullptr);
@@ -247,6 +300,8 @@ void bq_sess_destroy(bq_sess* S)
   auto f = [](void* p) { if (p) (void)hipFree(p); };
   f(S->E); f(S->ren); f(S->nrm); f(S->Ui); f(S->dUi); f(S->kn); f(S->eni); f(S->md); f(S->sd);
   for (auto p : S->sid) f(p);
+  for (auto p : S->rowmap) f(p);
+  for (auto p : S->live) f(p);
   for (auto p : S->G) f(p);
   for (auto p : S->H) f(p);
   for (auto p : S->dG) f(p);
@@ -269,7 +324,7 @@ int bq_sess_assemble(bq_sess* S)
   void* work = nullptr;
   BQ_HIP(hipMalloc(&work, wmax));
   S->st.n_gh_pairs = 0;
-  S->st.n_qpgf = 0;
+  S->st.n_qpgf = S->st.n_qpgf_gl = S->st.n_qpgf_gh = S->st.n_qpgf_sing = 0;
   for (int d = 0; d <= S->MN; ++d) {
     const long dNe = S->dNe[d], N = 4 * dNe;
     if (dNe == 0) continue;
@@ -296,13 +351,20 @@ int bq_sess_assemble(bq_sess* S)
     a.dH = S->dH[d];
     a.tdF = S->tdF[d];
     int ngh = 0;
+    long long npts[3] = {0, 0, 0};
     a.n_gh_out = &ngh;
+    a.n_points_out = npts;
     if (bq_assemble_domain_dev(&a, work, s)) {
       (void)hipFree(work);
       return -1;
     }
     S->st.n_gh_pairs += ngh;
-    if (periodic) S->st.n_qpgf += (N * dNe - N - ngh) * 144LL + ngh * 4096LL + N * 6144LL;
+    if (periodic) {
+      S->st.n_qpgf_gl += npts[0];
+      S->st.n_qpgf_gh += npts[1];
+      S->st.n_qpgf_sing += npts[2];
+      S->st.n_qpgf += npts[0] + npts[1] + npts[2];
+    }
   }
   BQ_HIP(hipEventRecord(e1, s));
   BQ_HIP(hipEventSynchronize(e1));
@@ -320,8 +382,7 @@ int bq_sess_solve_global(bq_sess* S, double* Xh)
   hipStream_t s = S->s;
   const int NN = S->NN, n2 = 2 * NN;
   if (S->two_nn != n2) {
-    bq::set_error("solve_global: element-node count mismatch (%d vs %d); linear triangles need row compaction",
-                  S->two_nn, n2);
+    bq::set_error("solve_global: element-node count mismatch (%d live rows vs 2 NN = %d)", S->two_nn, n2);
     return -2;
   }
   hipEvent_t e0, e1;
@@ -358,13 +419,13 @@ int bq_sess_solve_global(bq_sess* S, double* Xh)
   for (int d = 0; d <= S->MN; ++d) {
     const long dNe = S->dNe[d], N = 4 * dNe;
     if (dNe == 0) continue;
-    k_build_blocks<<<nblk(N * dNe, 256), 256, 0, s>>>((int)dNe, d, rowbase, NN, S->sid[d], S->eni, S->md, S->sd,
-                                                       S->kn, S->G[d], S->H[d], M, M3, R, L);
+    k_build_blocks<<<nblk(N * dNe, 256), 256, 0, s>>>((int)dNe, d, rowbase, NN, S->rowmap[d], S->sid[d], S->eni,
+                                                       S->md, S->sd, S->kn, S->G[d], S->H[d], M, M3, R, L);
     BQ_HIP(hipGetLastError());
-    k_build_rhs<<<nblk(N, 128), 128, 0, s>>>((int)dNe, d, rowbase, NN, S->sid[d], S->md, S->sd, S->kn, S->nrm, S->Ui,
-                                             S->dUi, S->G[d], S->H[d], R, b3);
+    k_build_rhs<<<nblk(N, 128), 128, 0, s>>>((int)dNe, d, rowbase, NN, S->rowmap[d], S->sid[d], S->md, S->sd, S->kn,
+                                             S->nrm, S->Ui, S->dUi, S->G[d], S->H[d], R, b3);
     BQ_HIP(hipGetLastError());
-    rowbase += (int)N;
+    rowbase += S->nlive[d];
   }
   const double one[2] = {1, 0}, mone[2] = {-1, 0};
   int info = bq_zgetrf_dev(s, n2, M, n2, ip1, inv1);
@@ -440,25 +501,33 @@ int bq_sess_deh(bq_sess* S, int d, const double* EH, double* dEH)
 {
   hipStream_t s = S->s;
   const long N = 4L * S->dNe[d];
-  double *eh, *b, *gt, *inv;
+  const int n = S->nlive[d];
+  double *eh, *b, *bl, *gt, *inv;
   int* ip;
-  if (upload(&eh, EH, (size_t)N * 12, s) || dmalloc(&b, (size_t)N * 12) || dmalloc(&gt, (size_t)N * N * 2) ||
-      dmalloc(&ip, (size_t)N) || dmalloc(&inv, (size_t)bq_lu_inv_doubles((int)N)))
+  if (upload(&eh, EH, (size_t)N * 12, s) || dmalloc(&b, (size_t)N * 12) || dmalloc(&bl, (size_t)n * 12) ||
+      dmalloc(&gt, (size_t)n * n * 2) || dmalloc(&ip, (size_t)n) || dmalloc(&inv, (size_t)bq_lu_inv_doubles(n)))
     return -1;
   const double one[2] = {1, 0}, zero[2] = {0, 0};
+  // H [E H] over all rows (empty rows and columns contribute 0), then the live rows only
   if (bq_zgemm_op_dev(s, 1, 0, (int)N, 6, (int)N, one, S->H[d], N, eh, N, zero, b, N)) return -1;
-  k_transpose<<<nblk(N * N, 256), 256, 0, s>>>(N, S->G[d], gt);
+  k_live_rows<<<nblk((long)n * 6, 256), 256, 0, s>>>(N, n, 6, S->live[d], b, bl, 1);
   BQ_HIP(hipGetLastError());
-  int info = bq_zgetrf_dev(s, (int)N, gt, N, ip, inv);
+  k_gather_live<<<nblk((long)n * n, 256), 256, 0, s>>>(N, n, S->live[d], S->G[d], gt);
+  BQ_HIP(hipGetLastError());
+  int info = bq_zgetrf_dev(s, n, gt, n, ip, inv);
   if (info) {
     bq::set_error("solve_deh: G of domain %d is singular (info %d)", d, info);
     return info > 0 ? -3 : info;
   }
-  if (bq_zgetrs_dev(s, (int)N, 6, gt, N, ip, inv, b, N)) return -1;
+  if (bq_zgetrs_dev(s, n, 6, gt, n, ip, inv, bl, n)) return -1;
+  BQ_HIP(hipMemsetAsync(b, 0, (size_t)N * 6 * 16, s));
+  k_live_rows<<<nblk((long)n * 6, 256), 256, 0, s>>>(N, n, 6, S->live[d], b, bl, 0);
+  BQ_HIP(hipGetLastError());
   BQ_HIP(hipMemcpyAsync(dEH, b, (size_t)N * 6 * 16, hipMemcpyDeviceToHost, s));
   BQ_HIP(hipStreamSynchronize(s));
   (void)hipFree(eh);
   (void)hipFree(b);
+  (void)hipFree(bl);
   (void)hipFree(gt);
   (void)hipFree(ip);
   (void)hipFree(inv);

@@ -16,6 +16,7 @@
 #include "bem3_emf_qd2.h"
 #include "bq_amd.h"
 #include "../bq_field.h"
+#include "../bq_solver.h"
 
 static pthread_mutex_t g_lock = PTHREAD_MUTEX_INITIALIZER;
 static struct {
@@ -33,10 +34,7 @@ static void field_fail(const char *what)
 static void flat_elem(BOUD *bd, double *E, double *ren, double *wen)
 {
   for (int i = 1; i <= bd->Ne; i++) {
-    for (int d = 0; d < 3; d++) {
-      for (int c = 0; c < 4; c++) E[24 * i + 4 * d + c] = bd->cr[i][d][c];
-      for (int c = 0; c < 3; c++) E[24 * i + 12 + 3 * d + c] = bd->cw[i][d][c];
-    }
+    bq_pack_elem(bd->cr[i], bd->cw[i], bd->ed[i][3] == 0, E + 24 * (size_t)i);
     for (int j = 0; j < 4; j++)
       for (int d = 0; d < 3; d++) {
         ren[12 * i + 3 * j + d] = bd->ren[i][j][d];
@@ -109,6 +107,7 @@ static void *context_for(DQD2 *qd)
   f.dHb = (const double *const *)dHb;
   f.xli = bd->xli; f.wli = bd->wli; f.xhi = bd->xhi; f.whi = bd->whi;
   f.z49 = bd->zt_49; f.e49 = bd->et_49; f.w49 = bd->wt_49;
+  f.w34 = bd->wt_34; f.z37 = bd->zt_37; f.e37 = bd->et_37; f.w37 = bd->wt_37;
   void *ctx = NULL;
   int rc = bq_field_ctx_create(&f, &ctx);
   for (int d = 0; d <= MN; d++) {

@@ -61,10 +61,7 @@ static void flatten(DQD2 *qd, flat_model *f)
   f->dNe = calloc(qd->MN + 1, sizeof(int));
   f->sid = calloc(qd->MN + 1, sizeof(int *));
   for (int i = 1; i <= Ne; i++) {
-    for (int d = 0; d < 3; d++) {
-      for (int c = 0; c < 4; c++) f->E[24 * i + 4 * d + c] = bd->cr[i][d][c];
-      for (int c = 0; c < 3; c++) f->E[24 * i + 12 + 3 * d + c] = bd->cw[i][d][c];
-    }
+    bq_pack_elem(bd->cr[i], bd->cw[i], bd->ed[i][3] == 0, f->E + 24 * (size_t)i);
     for (int j = 0; j < 4; j++) {
       double w[3] = {bd->wen[i][j][0], bd->wen[i][j][1], bd->wen[i][j][2]};
       vuni_d(w);
@@ -305,11 +302,6 @@ void solve_bieq(DQD2 *qd)
   printf("  coefficient matrix          ");
   fflush(stdout);
   ts = now_s();
-  for (int i = 1; i <= qd->bd.Ne; i++)
-    if (qd->bd.ed[i][3] == 0) {
-      printf("\nsolve_bieq(): linear triangular elements are not supported by the GPU solver yet. Exit...\n");
-      exit(1);
-    }
   flat_model f;
   memset(&f, 0, sizeof f);
   flatten(qd, &f);

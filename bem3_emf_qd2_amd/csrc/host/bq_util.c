@@ -6,6 +6,7 @@
 #include <stdlib.h>
 #include <math.h>
 #include "bem3_emf_qd2.h"
+#include "../bq_solver.h"
 
 void vadd_d(double *r, double *v0, double *v1) { for (int i = 0; i < 3; i++) r[i] = v0[i] + v1[i]; }
 void vsub_d(double *r, double *v0, double *v1) { for (int i = 0; i < 3; i++) r[i] = v0[i] - v1[i]; }
@@ -50,3 +51,14 @@ void *m_alloc2(size_t num, size_t size, char *txt)
 }
 
 int bq_sizeof_dqd2(void) { return (int)sizeof(DQD2); }
+
+void bq_pack_elem(double *const *cr, double *const *cw, int tri, double *dst)
+{
+  const double js = tri ? 2.0 / (3.0 * sqrt(3.0)) : 1.0;
+  for (int d = 0; d < 3; d++) {
+    for (int c = 0; c < 4; c++) dst[4 * d + c] = cr[d][c];
+    for (int c = 0; c < 3; c++) dst[12 + 3 * d + c] = cw[d][c] * js;
+  }
+  dst[21] = tri ? 1.0 : 0.0;
+  dst[22] = dst[23] = 0.0;
+}

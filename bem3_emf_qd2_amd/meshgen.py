@@ -67,6 +67,101 @@ def cube_sphere(n: int, radius: float, center=(0.0, 0.0, 0.0)):
     return X, Q
 
 
+def _ring(radius: float, z: float, m: int, phase: float):
+    ang = phase + 2.0 * math.pi * np.arange(m) / m
+    return np.c_[radius * np.cos(ang), radius * np.sin(ang), np.full(m, z)], ang
+
+
+def _strip(ia, aa, ib, ab):
+    """Triangulate the band between two closed rings of vertex ids ia (angles aa) and ib
+    (angles ab), merging by angle; a ring of one vertex is a pole."""
+    tris = []
+    if len(ia) == 1 or len(ib) == 1:
+        pole, ring = (ia[0], ib) if len(ia) == 1 else (ib[0], ia)
+        return [[pole, ring[k], ring[(k + 1) % len(ring)]] for k in range(len(ring))]
+    na, nb = len(ia), len(ib)
+    i = j = 0
+    ua = np.unwrap(np.r_[aa, aa[0] + 2 * math.pi])
+    ub = np.unwrap(np.r_[ab, ab[0] + 2 * math.pi])
+    while i < na or j < nb:
+        if j >= nb or (i < na and ua[i + 1] <= ub[j + 1]):
+            tris.append([ia[i % na], ia[(i + 1) % na], ib[j % nb]])
+            i += 1
+        else:
+            tris.append([ia[i % na], ib[(j + 1) % nb], ib[j % nb]])
+            j += 1
+    return tris
+
+
+def cone(radius: float, height: float, h: float, z0: float = 0.0, jitter: float = 0.0, seed: int = 0):
+    """Right circular cone (base disk at z0, apex at z0 + height) as linear triangles of edge
+    about h: rings of nearly uniform spacing on the lateral surface and the base, bands
+    between rings triangulated by angle.  jitter moves every vertex off the apex, the base
+    centre and the rim by jitter * h in a seeded random direction.  Returns (nodes[N, 3],
+    tris[M, 3]) with 0-based ids and outward normals."""
+    slant = math.hypot(radius, height)
+    n_l = max(2, round(slant / h))
+    n_b = max(2, round(radius / h))
+    m_rim = max(6, round(2 * math.pi * radius / h))
+    X = [np.array([[0.0, 0.0, z0 + height]])]
+    rings = [(np.array([0]), np.array([0.0]))]
+    nid = 1
+    for i in range(1, n_l + 1):  # lateral, apex -> rim
+        s = i / n_l
+        m = m_rim if i == n_l else max(3, round(2 * math.pi * radius * s / h))
+        P, ang = _ring(radius * s, z0 + height * (1 - s), m, 0.5 * (i % 2) * 2 * math.pi / m)
+        X.append(P)
+        rings.append((np.arange(nid, nid + m), ang))
+        nid += m
+    rim = rings[-1]
+    tris = []
+    for a, b in zip(rings[:-1], rings[1:]):
+        tris += _strip(a[0], a[1], b[0], b[1])
+    base = [rim]
+    for j in range(n_b - 1, 0, -1):  # base, rim -> centre
+        s = j / n_b
+        m = max(3, round(2 * math.pi * radius * s / h))
+        P, ang = _ring(radius * s, z0, m, 0.5 * (j % 2) * 2 * math.pi / m)
+        X.append(P)
+        base.append((np.arange(nid, nid + m), ang))
+        nid += m
+    X.append(np.array([[0.0, 0.0, z0]]))
+    base.append((np.array([nid]), np.array([0.0])))
+    nid += 1
+    for a, b in zip(base[:-1], base[1:]):
+        tris += _strip(a[0], a[1], b[0], b[1])
+    X = np.vstack(X)
+    T = np.array(tris, int)
+    if jitter:
+        fixed = np.zeros(len(X), bool)
+        fixed[0] = fixed[nid - 1] = True
+        fixed[rim[0]] = True
+        rng = np.random.default_rng(seed)
+        v = rng.normal(size=(len(X), 3))
+        v /= np.linalg.norm(v, axis=1)[:, None]
+        X = X + np.where(fixed[:, None], 0.0, jitter * h * v)
+    # outward orientation: the cone is convex, so a face normal points away from its
+    # interior point (0, 0, z0 + height / 4)
+    c = X[T].mean(1) - np.array([0.0, 0.0, z0 + height / 4])
+    nrm = np.cross(X[T[:, 1]] - X[T[:, 0]], X[T[:, 2]] - X[T[:, 0]])
+    flip = np.einsum("ij,ij->i", nrm, c) < 0
+    T[flip] = T[flip][:, ::-1]
+    return X, T
+
+
+def split_quads(Q: np.ndarray, every: int):
+    """Mixed mesh: every `every`-th quad (a, b, c, d) becomes the triangles (a, b, c) and
+    (a, c, d), which keep its orientation.  Returns a list of 4- and 3-node rows."""
+    out = []
+    for i, q in enumerate(Q):
+        if every and i % every == 0:
+            out.append([q[0], q[1], q[2]])
+            out.append([q[0], q[2], q[3]])
+        else:
+            out.append(list(q))
+    return out
+
+
 def write_msh(path, nodes: np.ndarray, surfaces) -> None:
     """surfaces: list of (elements[M,k] 0-based, inner_domain, outer_domain, entity_id);
     element normals point out of inner_domain.  Domain 0 is written as physical 99."""
@@ -117,17 +212,18 @@ def write_medium(path, indices) -> None:
 
 def sphere_scene(dirpath, n: int, radius: float, n_sphere, lam: float, n0: float = 1.0,
                  theta: float = 1e-6, phi: float = 0.0, pol: str = "TM", d1=(1.0, 0.0), d2=(0.0, 1.0),
-                 core=None):
+                 core=None, tri_every: int = 0):
     """Write a complete input set for a (optionally two-layer) sphere on a lattice.
 
     core = (radius, index) adds a concentric core (domain 2 inside the shell domain 1).
+    tri_every = k > 0 splits every k-th quad of the outer surface into two linear triangles.
     pol: "TM" -> e0x = 1 (p-polarised in the beam frame), "TE" -> e0y = 1
     (d3qd2_src/ipw.c:139-144).  Amplitude 1 means power = n0 / 2 (d3qd2_src/ipw.c:131).
     """
     d = Path(dirpath)
     d.mkdir(parents=True, exist_ok=True)
     X, Q = cube_sphere(n, radius)
-    surfaces = [(Q, 1, 0, 1)]
+    surfaces = [(split_quads(Q, tri_every) if tri_every else Q, 1, 0, 1)]
     media = [n_sphere]
     if core is not None:
         Xc, Qc = cube_sphere(n, core[0])
@@ -140,4 +236,28 @@ def sphere_scene(dirpath, n: int, radius: float, n_sphere, lam: float, n0: float
     write_ipw(d / "ipw.txt", [dict(lambda0=lam, ni=n0, power=0.5 * n0, e0x=ex, e0y=ey, theta=theta, phi=phi)])
     write_periodicity(d / "periodicity_data.txt", d1, d2)
     write_medium(d / "medium_data.txt", media)
+    return d
+
+
+def sphere_cone_scene(dirpath, n: int = 16, core=(0.3, 1.6 + 0.01j), shell=(0.40, 1.45),
+                      cone_=(0.3, 0.6, 2.0), cone_h: float = 0.03, cone_z0: float = 0.5, jitter: float = 0.01,
+                      seed: int = 2, lam: float = 0.55, n0: float = 1.0, theta: float = math.radians(30),
+                      phi: float = math.radians(15), pol: str = "TM", d1=(0.9, 0.0), d2=(0.45, 0.779)):
+    """Two-layer cube-sphere (6 n^2 quads per surface) centred at the origin and a linear-
+    triangle cone (base radius, height, index) standing on the plane z = cone_z0 above it, in
+    one hexagonal lattice cell.  Domains: 1 shell, 2 core, 3 cone.  The default shell radius
+    0.40 um keeps the sphere off its own periodic images: with |a2| = 0.8996 um a 0.45 um
+    shell would overlap them."""
+    d = Path(dirpath)
+    d.mkdir(parents=True, exist_ok=True)
+    Xs, Qs = cube_sphere(n, shell[0])
+    Xc, Qc = cube_sphere(n, core[0])
+    Xk, Tk = cone(cone_[0], cone_[1], cone_h, z0=cone_z0, jitter=jitter, seed=seed)
+    X = np.vstack([Xs, Xc, Xk])
+    surfaces = [(Qs, 1, 0, 1), (Qc + len(Xs), 2, 1, 2), (Tk + len(Xs) + len(Xc), 3, 0, 3)]
+    write_msh(d / "mesh.msh", X, surfaces)
+    ex, ey = (1.0, 0.0) if pol.upper() == "TM" else (0.0, 1.0)
+    write_ipw(d / "ipw.txt", [dict(lambda0=lam, ni=n0, power=0.5 * n0, e0x=ex, e0y=ey, theta=theta, phi=phi)])
+    write_periodicity(d / "periodicity_data.txt", d1, d2)
+    write_medium(d / "medium_data.txt", [shell[1], core[1], cone_[2]])
     return d

@@ -19,7 +19,8 @@ _I = ctypes.POINTER(ctypes.c_int)
 
 class Stats(ctypes.Structure):
     _fields_ = [("t_assemble_ms", ctypes.c_double), ("t_solve_ms", ctypes.c_double),
-                ("t_fields_ms", ctypes.c_double), ("n_gh_pairs", ctypes.c_int), ("n_qpgf", ctypes.c_longlong)]
+                ("t_fields_ms", ctypes.c_double), ("n_gh_pairs", ctypes.c_int), ("n_qpgf", ctypes.c_longlong),
+                ("n_qpgf_gl", ctypes.c_longlong), ("n_qpgf_gh", ctypes.c_longlong), ("n_qpgf_sing", ctypes.c_longlong)]
 
 
 def L():

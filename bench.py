@@ -6,10 +6,17 @@ of every domain's coefficient matrices, the global solve, boundary E/H and dE/dH
 H at 100 probe points (EH_t, type 1).  The mesh and incidence stay resident; each step
 recomputes everything above from them.
 
-Default workload: the 384-quad cube-sphere of radius 0.4 um, index 1.5, in vacuum on a
-1.0 um square lattice, lambda0 = 0.6 um, x-polarised, normal incidence (theta = 1e-6 rad,
-the reference rejects theta = 0 on a lattice), 100 probes uniform in the unit cell with
-z in [-1, 1] um from default_rng(1).
+Default workload (c2, the scene the metric is quoted on): in one hexagonal lattice cell
+(a1 = (0.9, 0), a2 = (0.45, 0.779) um) a two-layer cube-sphere (core r 0.3 um, n 1.6 + 0.01i;
+shell n 1.45; 6x16x16 = 1536 bilinear quads per surface) and a right circular cone (base
+radius 0.3 um, height 0.6 um, n 2.0) of 2012 linear triangles with a seeded 1 %-of-edge jitter
+of its non-feature vertices; oblique incidence theta = 30 deg, phi = 15 deg, TM, lambda0 =
+0.55 um; 100 probes uniform in the cell with z in [-1, 1] um from default_rng(1).  The shell
+radius is 0.40 um, not 0.45: on this lattice |a2| = 0.8996 um, so a 0.45 um shell would
+overlap its own periodic image; the cone stands on z = 0.5 um above the sphere.
+Workload c1: the 384-quad cube-sphere of radius 0.4 um, index 1.5, in vacuum on a 1.0 um
+square lattice, lambda0 = 0.6 um, x-polarised, normal incidence (theta = 1e-6 rad, the
+reference rejects theta = 0 on a lattice), same probes.
 
 Prints ONE JSON line.  `roofline` describes the kernel with the largest share of device time
 (HIP events on the library stream around every launch); `cpu_baseline` times the CPU
@@ -42,10 +49,17 @@ FLOPS_PER_EVAL = {"c1": {"k_q0_gl": 54213, "k_q0_gh": 25808, "k_q0_sing": 50261}
 TRAFFIC_BYTES = {"c1": {"k_q0_gl": (13373882.2 * 2.0 + 17179946.9) * 1024}}
 
 WORKLOADS = {
-    "c1": dict(n=8, radius=0.4, n_sphere=1.5, lam=0.6, n0=1.0, theta=1e-6, phi=0.0, pol="TM",
-               d1=(1.0, 0.0), d2=(0.0, 1.0), probes=100, seed=1,
+    "c1": dict(kind="sphere", n=8, radius=0.4, n_sphere=1.5, lam=0.6, n0=1.0, theta=1e-6, phi=0.0, pol="TM",
+               d1=(1.0, 0.0), d2=(0.0, 1.0), probes=100, seed=1, steps=10, warmup=2,
                label="C1: 384-quad cube-sphere r=0.4um n=1.5 in n0=1.0, square lattice 1.0um, "
                      "lambda0=0.6um, x-pol, normal incidence, 100 probes (EH_t type 1)"),
+    "c2": dict(kind="sphere_cone", n=16, lam=0.55, n0=1.0, theta=np.radians(30), phi=np.radians(15), pol="TM",
+               d1=(0.9, 0.0), d2=(0.45, 0.779), probes=100, seed=1, steps=2, warmup=1,
+               label="C2: two-layer cube-sphere (core r=0.3um n=1.6+0.01i, shell r=0.40um n=1.45, 1536 quads "
+                     "per surface) + cone (base r=0.3um, h=0.6um, n=2.0, 2012 linear triangles, 1% edge jitter, "
+                     "seed 2) in one hexagonal cell a1=(0.9,0) a2=(0.45,0.779)um, lambda0=0.55um, theta=30deg, "
+                     "phi=15deg, TM, 100 probes (EH_t type 1); shell 0.40 instead of 0.45um so that it does "
+                     "not overlap its periodic image (|a2| = 0.8996um)"),
 }
 
 
@@ -60,8 +74,12 @@ def _probes(w, n, seed):
 
 def _scene(w, d):
     from bem3_emf_qd2_amd import meshgen
-    meshgen.sphere_scene(d, w["n"], w["radius"], w["n_sphere"], w["lam"], n0=w["n0"], theta=w["theta"],
-                         phi=w["phi"], pol=w["pol"], d1=w["d1"], d2=w["d2"])
+    if w["kind"] == "sphere_cone":
+        meshgen.sphere_cone_scene(d, n=w["n"], lam=w["lam"], n0=w["n0"], theta=w["theta"], phi=w["phi"],
+                                  pol=w["pol"], d1=w["d1"], d2=w["d2"])
+    else:
+        meshgen.sphere_scene(d, w["n"], w["radius"], w["n_sphere"], w["lam"], n0=w["n0"], theta=w["theta"],
+                             phi=w["phi"], pol=w["pol"], d1=w["d1"], d2=w["d2"])
 
 
 class _QuietStdout:
@@ -99,7 +117,7 @@ def _kernel_times(L):
     return out
 
 
-def _roofline(ktimes, stats_per_step, steps, n_rows, ne, workload):
+def _roofline(ktimes, stats_per_step, steps, workload):
     """Roofline object for the kernel with the largest total device time."""
     name = max(ktimes, key=lambda k: ktimes[k][0])
     tot_ms, cnt = ktimes[name]
@@ -110,8 +128,9 @@ def _roofline(ktimes, stats_per_step, steps, n_rows, ne, workload):
     if name == "zgetrf_update":
         r["note"] = "rocBLAS zgemm trailing updates of the blocked LU"
         return r
-    ngh = stats_per_step["n_gh_pairs"]
-    evals = {"k_q0_gl": (n_rows * ne - n_rows - ngh) * 144, "k_q0_gh": ngh * 4096, "k_q0_sing": n_rows * 6144}
+    # QPGF evaluations per launch, as the device counted them from the pair classification
+    evals = {"k_q0_gl": stats_per_step["n_qpgf_gl"], "k_q0_gh": stats_per_step["n_qpgf_gh"],
+             "k_q0_sing": stats_per_step["n_qpgf_sing"]}
     if name in evals:
         r["evals_per_launch"] = evals[name]
         fpe = FLOPS_PER_EVAL.get(workload, {}).get(name)
@@ -148,27 +167,26 @@ def _cpu_baseline(model_dat, w, probes, gpu_solve_dims):
         measured += dt
         total += dt * ne / rows
         parts.append(f"domain {dom}: {rows}/{ne} element rows in {dt:.1f}s")
-    # dense LU of the global system (order 8 NN): timed at n/4, scaled by 64
-    na = 8 * o.NN
-    n_s = na // 4
-    A = rng.standard_normal((n_s, n_s)) + 1j * rng.standard_normal((n_s, n_s))
-    t = time.perf_counter()
-    sla.lu_factor(A, check_finite=False)
-    dt = time.perf_counter() - t
-    measured += dt
-    total += dt * (na / n_s) ** 3
-    parts.append(f"LU {n_s} in {dt:.1f}s x{(na / n_s) ** 3:.0f}")
-    del A
-    # per-domain zgesv of order 4 Ne_d with 6 right-hand sides
-    for dom in range(o.MN + 1):
-        n = 4 * o.dNe[dom]
-        A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    # dense LUs (the global system of order 8 NN, and per domain the dE/dH system of order
+    # N_d): timed at order min(n, 3072) and scaled by (n / n_s)^3
+    def lu_time(n):
+        n_s = min(n, 3072)
+        A = rng.standard_normal((n_s, n_s)) + 1j * rng.standard_normal((n_s, n_s))
         t = time.perf_counter()
         sla.lu_factor(A, check_finite=False)
-        dt = time.perf_counter() - t
+        return time.perf_counter() - t, (n / n_s) ** 3, n_s
+
+    na = 8 * o.NN
+    dt, sc, n_s = lu_time(na)
+    measured += dt
+    total += dt * sc
+    parts.append(f"global LU {na} timed at {n_s} in {dt:.1f}s x{sc:.0f}")
+    for dom in range(o.MN + 1):
+        n = int(o.live_rows(dom).sum())
+        dt, sc, n_s = lu_time(n)
         measured += dt
-        total += dt
-    parts.append("per-domain dE/dH LUs timed in full")
+        total += dt * sc
+    parts.append("per-domain dE/dH LUs timed at <= 3072 and scaled")
     return {"value": 1.0 / total, "unit": "incidences/s", "cores": cores, "kind": "port",
             "seconds_per_incidence": total,
             "sample": "oracle/ (C restatement, OpenMP) on the same model: " + "; ".join(parts) +
@@ -178,9 +196,9 @@ def _cpu_baseline(model_dat, w, probes, gpu_solve_dims):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="c1", choices=sorted(WORKLOADS))
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (default per workload: c2 2, c1 10)")
+    ap.add_argument("--warmup", type=int, default=None, help="untimed steps (default per workload: c2 1, c1 2)")
+    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run one device-to-device copy of MIB MiB (calibrates FETCH_SIZE/WRITE_SIZE)")
@@ -205,6 +223,10 @@ def main():
     if not bq.device_ok():
         raise SystemExit(f"bench.py: no usable gfx950 device: {L.bq_last_error().decode()}")
     w = WORKLOADS[args.workload]
+    if args.steps is None:
+        args.steps = w["steps"]
+    if args.warmup is None:
+        args.warmup = w["warmup"]
     probes = _probes(w, w["probes"], w["seed"])
     with tempfile.TemporaryDirectory() as td:
         _scene(w, td)
@@ -230,15 +252,19 @@ def main():
         kt = _kernel_times(L)
         L.bq_kernel_timing(0)
         stats = {"t_assemble_ms": st.t_assemble_ms, "t_solve_ms": st.t_solve_ms, "t_fields_ms": st.t_fields_ms,
-                 "n_gh_pairs": st.n_gh_pairs, "n_qpgf": st.n_qpgf}
+                 "n_gh_pairs": st.n_gh_pairs, "n_qpgf": st.n_qpgf, "n_qpgf_gl": st.n_qpgf_gl,
+                 "n_qpgf_gh": st.n_qpgf_gh, "n_qpgf_sing": st.n_qpgf_sing}
         ref = np.r_[E0.ravel(), H0.ravel()] if args.warmup else None
         rep = None
         if ref is not None:
             got = np.r_[E.ravel(), H.ravel()]
             rep = float(np.abs(got - ref).max() / np.abs(ref).max())
-        ne = 6 * w["n"] ** 2
-        n_rows = 4 * ne
-        roof = _roofline(kt, stats, args.steps, n_rows, ne, args.workload)
+        sys.path.insert(0, str(ROOT / "tests"))
+        from datfile import read_dat  # the ex.dat reader (layout only)
+        dd = read_dat(Path(td) / "setup.dat")
+        tris = int(np.sum(dd["ed"][1:, 3] == 0))
+        quads, NN = int(dd["Ne"]) - tris, int(dd["NN"])
+        roof = _roofline(kt, stats, args.steps, args.workload)
         cpu = None if args.no_cpu_baseline else _cpu_baseline(Path(td) / "setup.dat", w, probes, None)
         m.close()
 
@@ -257,7 +283,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (generated cube-sphere mesh and plane wave)",
-        "config": {"workload": w["label"], "quads": ne, "element_nodes": n_rows, "na": 8 * n_rows,
+        "config": {"workload": w["label"], "quads": quads, "triangles": tris, "element_nodes": NN, "na": 8 * NN,
                    "probes": w["probes"], "parallelism": "single GPU"},
         "rel_err_vs_fp64": 0.0,
         "repeat_rel_diff": rep,

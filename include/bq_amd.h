@@ -50,7 +50,9 @@ int bq_EH_t_batch(void *qd, int n, const double *r, int type, double *E, double 
 int bq_assemble_domain_host(void *qd, int d, double *G, double *H, double *dG, double *dH, double *tdF);
 
 /* Counters and phase times (ms, device clock) of the last solve_bieq in this process:
- * returns a pointer to {t_assemble_ms, t_solve_ms, t_fields_ms, n_gh_pairs, n_qpgf}. */
+ * returns a pointer to {double t_assemble_ms, t_solve_ms, t_fields_ms; int n_gh_pairs;
+ * long long n_qpgf, n_qpgf_gl, n_qpgf_gh, n_qpgf_sing} (QPGF evaluations of the periodic
+ * domain, in total and by phase: 12-point rule, 64-point fallback rule, singular rules). */
 const void *bq_solve_last_stats(void);
 
 /* Per-kernel timing (no counterpart in the reference; for benchmarks and profiling):

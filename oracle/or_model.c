@@ -332,6 +332,18 @@ void or_model_export(const or_model *m, double *ren, double *wen, double *Ui, do
   if (kn) memcpy(kn, m->kn, sizeof(double complex) * (m->MN + 1));
 }
 
+/* Replace the element node data by stored arrays (same layout as or_model_export): a model
+ * read back from a result file carries the node points, normals and incident field the run
+ * that wrote it computed (dat_read, d3qd2_src/d3qd2_setup.c:420-512). */
+void or_model_set_nodes(or_model *m, const double *ren, const double *wen, const double *Ui, const double *dUi)
+{
+  int n = m->Ne + 1;
+  memcpy(m->ren, ren, sizeof(double) * 12 * n);
+  memcpy(m->wen, wen, sizeof(double) * 12 * n);
+  memcpy(m->Ui, Ui, sizeof(double complex) * 16 * n);
+  memcpy(m->dUi, dUi, sizeof(double complex) * 16 * n);
+}
+
 int or_model_dNe(const or_model *m, int d) { return m->dNe[d]; }
 int or_model_NN(const or_model *m) { return m->NN; }
 int or_model_sid(const or_model *m, int d, int i) { return m->sid[d][i]; }

@@ -72,6 +72,7 @@ or_model *or_model_new(int MN, const double *n_re_im, int ps, const double *vd1,
 void or_model_free(or_model *m);
 void or_model_export(const or_model *m, double *ren, double *wen, double *Ui, double *dUi,
                      double *qd, double *kn);
+void or_model_set_nodes(or_model *m, const double *ren, const double *wen, const double *Ui, const double *dUi);
 int or_model_dNe(const or_model *m, int d);
 int or_model_NN(const or_model *m);
 int or_model_sid(const or_model *m, int d, int i);

@@ -35,6 +35,7 @@ def oracle() -> ctypes.CDLL:
         o.or_model_dNe.argtypes = [ctypes.c_void_p, ctypes.c_int]
         o.or_model_sid.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
         o.or_model_NN.argtypes = [ctypes.c_void_p]
+        o.or_model_set_nodes.argtypes = [ctypes.c_void_p, D, D, D, D]
         o.or_node_tangents.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, D, D]
         o.or_pair.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                               ctypes.c_int, D, D, D]

@@ -36,9 +36,16 @@ class OracleModel:
         self.tri = self.ed[:, 3] == 0
 
     @classmethod
-    def from_dat(cls, d):
+    def from_dat(cls, d, stored_nodes=False):
+        """Model of a result file.  stored_nodes: take the element node points, normals and
+        incident field from the file (as the library's dat_read does) instead of recomputing
+        them from the mesh; they can differ in the last bit from this restatement's own."""
         q = d["qd"]
-        return cls(d["MN"], d["n"][1:], d["ps"], q[4:6], q[6:8], ipw_params(d), d["rn"], d["ed"], d["md"], d["sd"])
+        m = cls(d["MN"], d["n"][1:], d["ps"], q[4:6], q[6:8], ipw_params(d), d["rn"], d["ed"], d["md"], d["sd"])
+        if stored_nodes:
+            keep = [np.ascontiguousarray(d[k]) for k in ("ren", "wen", "Ui", "dUi")]
+            m.o.or_model_set_nodes(m.h, ptr(keep[0]), ptr(keep[1]), ptr(keep[2].view(float)), ptr(keep[3].view(float)))
+        return m
 
     def __del__(self):
         try:

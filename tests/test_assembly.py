@@ -18,9 +18,10 @@ ROOT = Path(__file__).resolve().parent.parent
 TOL = 1e-10
 
 
-def _scene(tmp_path, n, radius, lam, theta, phi=0.0, d1=(1.0, 0.0), d2=(0.0, 1.0), nsph=1.5):
+def _scene(tmp_path, n, radius, lam, theta, phi=0.0, d1=(1.0, 0.0), d2=(0.0, 1.0), nsph=1.5, tri_every=0):
     from bem3_emf_qd2_amd import meshgen
-    return meshgen.sphere_scene(tmp_path, n, radius, nsph, lam, theta=theta, phi=phi, d1=d1, d2=d2)
+    return meshgen.sphere_scene(tmp_path, n, radius, nsph, lam, theta=theta, phi=phi, d1=d1, d2=d2,
+                                tri_every=tri_every)
 
 
 def _oracle_from_inputs(tmp_path):
@@ -32,11 +33,13 @@ def _oracle_from_inputs(tmp_path):
     return m, d, OracleModel.from_dat(d)
 
 
-def test_mesh_reader_matches_oracle_setup(tmp_path):
-    """Host set-up (CPU, product code) vs oracle: element nodes, normals, incident field."""
+@pytest.mark.parametrize("tri_every", [0, 3])
+def test_mesh_reader_matches_oracle_setup(tmp_path, tri_every):
+    """Host set-up (CPU, product code) vs oracle: element nodes, normals, incident field
+    (with tri_every = 3, a third of the quads split into linear triangles)."""
     from bem3_emf_qd2_amd import lib
     lib()
-    _scene(tmp_path, 3, 0.3, 0.6, 0.3, phi=0.4)
+    _scene(tmp_path, 3, 0.3, 0.6, 0.3, phi=0.4, tri_every=tri_every)
     m, d, o = _oracle_from_inputs(tmp_path)
     ex = o.export()
     assert np.abs(ex["ren"] - d["ren"]).max() < 1e-15
@@ -52,18 +55,29 @@ def _rel(a, b):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("case", ["sq_l06", "hex_oblique"])
+@pytest.mark.parametrize("case", ["sq_l06", "hex_oblique", "mixed_tri"])
 def test_gpu_assembly_vs_oracle(tmp_path, gpu_lib, case):
+    """mixed_tri: a cube-sphere with every third quad split into two linear triangles
+    (16 quads + 16 triangles), exercising quad/triangle pairs both ways, triangle GH
+    fallbacks and the triangle singular and principal-value rules; a triangle's fourth row
+    and column must stay exactly 0."""
     if case == "sq_l06":
         _scene(tmp_path, 2, 0.3, 0.6, 0.2)
+    elif case == "mixed_tri":
+        _scene(tmp_path, 2, 0.3, 0.55, np.radians(25), phi=np.radians(10), d1=(0.9, 0.0), d2=(0.45, 0.779),
+               nsph=1.45 + 0.02j, tri_every=3)
     else:
         _scene(tmp_path, 3, 0.35, 0.55, np.radians(30), phi=np.radians(15), d1=(0.9, 0.0), d2=(0.45, 0.779),
                nsph=1.6 + 0.05j)
     m, d, o = _oracle_from_inputs(tmp_path)
+    if case == "mixed_tri":
+        assert o.tri[1:].sum() == 16
     for dom in (0, 1):
         dNe = o.dNe[dom]
         G, H, dG, dH, tdF = m.assemble(dom, dNe, deriv=(dom > 0))
         oG, oH, odG, odH, otdF = o.assemble(dom)
+        dead = ~o.live_rows(dom)
+        assert not np.any(G[dead]) and not np.any(H[dead]) and not np.any(G[:, dead]) and not np.any(H[:, dead])
         eg, eh = _rel(G, oG), _rel(H, oH)
         print(case, dom, "G", eg, "H", eh)
         assert eg < TOL and eh < TOL, (case, dom, eg, eh)

@@ -1,5 +1,8 @@
 """End-to-end parity with the reference's own results.
 
+tests/golden/cone_m1/ex2.dat (analysis_sample2/) is the reference's only shipped result with
+linear triangles: a cone of 553 quads and 28 triangles (n = 0.06 + 4i) on a hexagonal
+0.75 um lattice, rotated by pi about x.
 tests/golden/sample3/ex3.dat and tests/golden/verification/veri.dat are result files the
 reference ships (analysis_sample3/, verification/): a 384-quad cube-sphere (r = 0.125 um,
 n = 1.6) in water (n0 = 1.33) on hexagonal lattices of 0.75 and 2.0 um, lambda0 = 1 um,
@@ -21,6 +24,15 @@ from datfile import read_dat
 
 G = Path(__file__).resolve().parent / "golden"
 TOL = 1e-10
+# cone_m1: the cone's flat base was rotated by pi about x, which left its node z at
+# 0.125 +- 1e-11, and a few Gauss points of base pairs land at |R_z| = 2^-56 (one ulp of
+# 0.125).  There the reference's d3hm_qpgf_d2_v1 stops its spectral sum on a comparison of
+# erfc(b v + a) and erfc(b v - a) that is decided by libm rounding (its value is then neither
+# the truncated r_z == 0 one nor the converged one; tests/golden/qpgf_ref.npz and
+# oracle/_ref show the band 2^-58 < |R_z| < 2^-52).  The library takes the converged value
+# there.  The reference's own algorithm restated in C (oracle/) differs from ex2.dat by up to
+# 6e-10 for the same reason; the GPU solution differs by up to 9.6e-10 (H), 1.5e-10 (U).
+TOL_CASE = {"cone_m1/ex2.dat": 2e-9}
 KEYS = ["U", "dU", "E", "dE", "H", "dH"]
 
 
@@ -47,7 +59,7 @@ def _compare(sol, ref, tol):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("case", ["sample3/ex3.dat", "verification/veri.dat", "sphere_m2/ex.dat"])
+@pytest.mark.parametrize("case", ["sample3/ex3.dat", "verification/veri.dat", "sphere_m2/ex.dat", "cone_m1/ex2.dat"])
 def test_gpu_solve_matches_reference_dat(tmp_path, gpu_lib, case):
     from bem3_emf_qd2_amd.solver import Model
     ref = read_dat(G / case)
@@ -57,15 +69,16 @@ def test_gpu_solve_matches_reference_dat(tmp_path, gpu_lib, case):
     m.close()
     print(case, f"assembly {st.t_assemble_ms:.1f} ms solve {st.t_solve_ms:.1f} ms gh pairs {st.n_gh_pairs}")
     sol = read_dat(tmp_path / "gpu.dat")
-    _compare(sol, ref, TOL)
+    _compare(sol, ref, TOL_CASE.get(case, TOL))
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("case", ["sample3", "verification", "sphere_m2"])
+@pytest.mark.parametrize("case", ["sample3", "verification", "sphere_m2", "cone_m1"])
 def test_gpu_fields_on_reference_solution(gpu_lib, case):
     """EH_t on the reference's own solution vs the output of the reference example1 (15 digits)."""
     from bem3_emf_qd2_amd.solver import Model
-    dat = G / case / {"sample3": "ex3.dat", "verification": "veri.dat", "sphere_m2": "ex.dat"}[case]
+    dat = G / case / {"sample3": "ex3.dat", "verification": "veri.dat", "sphere_m2": "ex.dat",
+                      "cone_m1": "ex2.dat"}[case]
     exp = _example1(G / case / "example1_out.txt")
     m = Model.from_dat(dat)
     for t in (0, 1):
@@ -89,3 +102,26 @@ def test_oracle_pipeline_matches_ex3():
     sol = o.solve()
     fake = {"dom": [{k: sol[k][d] for k in KEYS} for d in range(ref["MN"] + 1)]}
     _compare(fake, ref, 1e-11)
+
+
+@pytest.mark.gpu
+def test_gpu_mini_sphere_cone_vs_oracle(tmp_path, gpu_lib):
+    """The benchmark scene in miniature: two-layer cube-sphere (2x2 quads per face) and a
+    coarse linear-triangle cone with jittered vertices in one hexagonal cell, oblique TM
+    incidence; four domains (open, shell, core, cone), quads and triangles in one open
+    region.  GPU solve vs the oracle's complete solve of the same stored model."""
+    from bem3_emf_qd2_amd import meshgen
+    from bem3_emf_qd2_amd.solver import Model
+    from oracle_pipeline import OracleModel
+    meshgen.sphere_cone_scene(tmp_path, n=2, cone_h=0.2)
+    m = Model.from_dir(tmp_path)
+    m.write(tmp_path / "setup.dat")
+    m.solve()
+    m.write(tmp_path / "gpu.dat")
+    m.close()
+    d = read_dat(tmp_path / "setup.dat")
+    assert d["MN"] == 3 and np.sum(d["ed"][1:, 3] == 0) > 40
+    o = OracleModel.from_dat(d, stored_nodes=True)
+    sol = o.solve(log=lambda *a: None)
+    fake = {"dom": [{k: sol[k][dd] for k in KEYS} for dd in range(d["MN"] + 1)]}
+    _compare(read_dat(tmp_path / "gpu.dat"), fake, TOL)
