@@ -1,16 +1,22 @@
 // bq_lu.hip -- dense complex FP64 LU with partial pivoting and the matching solves, on
 // the device, column-major (LAPACK zgetrf/zgetrs semantics, 0-based pivots).
 //
-// Right-looking blocked LU: for each panel of nb = 32 columns
-//   1. k_tslu       chooses the panel's nb pivot rows by tournament (partial pivoting on
+// Right-looking LU blocked on two levels.  An outer panel of nb2 = 256 columns is factored
+// in steps of nb = 32 columns:
+//   1. k_tslu       chooses the step's nb pivot rows by tournament (partial pivoting on
 //                   256-row blocks in registers, then 8-way reductions of candidate sets);
 //                   the last level also yields the factored diagonal block,
-//   2. k_swap       applies the interchanges to every column,
+//   2. k_swap       applies the interchanges to the outer panel's columns,
 //   3. k_diag_finish  stores L11, U11 and their inverses inv(L11), inv(U11),
-//   4. k_l21        L21 = A21 inv(U11),   k_apply_blk  U12 = inv(L11) A12,
-//   5. rocBLAS zgemm for the trailing update A22 -= L21 U12 (MFMA FP64 on gfx950).
-// Solves: the pivots become one permutation (gather), then per diagonal block
-// X_j = inv(T_jj) X_j (k_apply_blk) and a zgemm for the off-diagonal update.
+//   4. k_l21        L21 = A21 inv(U11),   k_apply_blk  U12 = inv(L11) A12 inside the panel,
+//   5. rocBLAS zgemm updates the rest of the outer panel.
+// Then the panel's interchanges go to all other columns, U12 of the outer block row is
+// solved with the panel's unit L11 (32-column steps), and one rank-256 zgemm updates the
+// trailing matrix (MFMA FP64 on gfx950).  A rank-32 update per 32 columns would stream the
+// trailing matrix through HBM 8x as often; at order 36 648 that traffic alone is ~10 s.
+// Solves: the pivots become one permutation (gather), then, per 256-row block, 32-row
+// diagonal steps X_j = inv(T_jj) X_j (k_apply_blk) with in-block zgemm updates, and one
+// rank-256 zgemm for the rows outside the block.
 // Replaces the reference's MKL PARDISO and LAPACKE_zgesv calls
 // (d3qd2_src/d3qd2_mkl_solver.c:18-148, d3qd2_src/d3qd2_solve_bieq.c:753).
 #include <rocblas/rocblas.h>
@@ -21,6 +27,7 @@
 namespace {
 
 constexpr int kNB = 32;
+constexpr int kNB2 = 256;  // outer panel (multiple of kNB)
 constexpr int kBlkElems = 2 * kNB * kNB;  // inv(L11) then inv(U11), complex, per diagonal block
 
 __device__ __forceinline__ double2 c2(cplx a) { return make_double2(a.re, a.im); }
@@ -220,13 +227,13 @@ __global__ void __launch_bounds__(256) k_l21(int n, double2* __restrict__ A, lon
   }
 }
 
-// apply the interchanges ipiv[j..j+jb) to every column
-__global__ void __launch_bounds__(256) k_swap(double2* __restrict__ A, long lda, int n, int j, int jb,
+// apply the interchanges ipiv[j..j+jb) to columns col0 .. col0+ncol-1
+__global__ void __launch_bounds__(256) k_swap(double2* __restrict__ A, long lda, int col0, int ncol, int j, int jb,
                                               const int* __restrict__ ipiv)
 {
-  const int col = blockIdx.x * blockDim.x + threadIdx.x;
-  if (col >= n) return;
-  double2* cp = A + (long)col * lda;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncol) return;
+  double2* cp = A + (long)(col0 + c) * lda;
   for (int r = j; r < j + jb; ++r) {
     const int p = ipiv[r];
     if (p != r) {
@@ -368,36 +375,61 @@ int bq_zgetrf_dev(void* stream, int n, double* Ad, long lda, int* ipiv, double* 
   BQ_HIP(hipMallocAsync((void**)&cand, sizeof(int) * (size_t)2 * nleaf * kNB, s));
   BQ_HIP(hipMallocAsync((void**)&F, sizeof(double2) * kNB * kNB, s));
   const int kt_all = bq::ktime_begin("zgetrf", s);
-  for (int j = 0; j < n; j += kNB) {
-    const int jb = min(kNB, n - j);
-    double2* invj = inv + (long)(j / kNB) * kBlkElems;
-    // pivots of the panel by tournament; the last level also factors the diagonal block
-    int nset = (n - j + kTsRows - 1) / kTsRows;
-    int* cur = cand;
-    int* nxt = cand + (size_t)nleaf * kNB;
+  for (int j0 = 0; j0 < n; j0 += kNB2) {
+    const int jb2 = min(kNB2, n - j0), je = j0 + jb2;
     const int kt_p = bq::ktime_begin("zgetrf_panel", s);
-    k_tslu<<<nset, 1024, 0, s>>>(0, n, A, lda, j, jb, nullptr, 0, cur, F, ipiv);
-    while (nset > 1) {
-      const int nout = (nset + kTsFan - 1) / kTsFan;
-      k_tslu<<<nout, 1024, 0, s>>>(1, n, A, lda, j, jb, cur, nset, nxt, F, ipiv);
-      int* t = cur;
-      cur = nxt;
-      nxt = t;
-      nset = nout;
+    for (int j = j0; j < je; j += kNB) {
+      const int jb = min(kNB, je - j);
+      double2* invj = inv + (long)(j / kNB) * kBlkElems;
+      // pivots of the step by tournament; the last level also factors the diagonal block
+      int nset = (n - j + kTsRows - 1) / kTsRows;
+      int* cur = cand;
+      int* nxt = cand + (size_t)nleaf * kNB;
+      k_tslu<<<nset, 1024, 0, s>>>(0, n, A, lda, j, jb, nullptr, 0, cur, F, ipiv);
+      while (nset > 1) {
+        const int nout = (nset + kTsFan - 1) / kTsFan;
+        k_tslu<<<nout, 1024, 0, s>>>(1, n, A, lda, j, jb, cur, nset, nxt, F, ipiv);
+        int* t = cur;
+        cur = nxt;
+        nxt = t;
+        nset = nout;
+      }
+      k_swap<<<(jb2 + 255) / 256, 256, 0, s>>>(A, lda, j0, jb2, j, jb, ipiv);
+      k_diag_finish<<<1, 1024, 0, s>>>(A, lda, j, jb, F, invj);
+      const int rest = n - j - jb, wide = je - j - jb;
+      if (rest > 0) k_l21<<<(rest + 63) / 64, 256, 0, s>>>(n, A, lda, j, jb, invj + kNB * kNB);
+      BQ_HIP(hipGetLastError());
+      if (wide > 0) {  // the rest of the outer panel
+        k_apply_blk<<<(wide + 63) / 64, 256, 0, s>>>(invj, jb, A + (long)(j + jb) * lda + j, lda, wide);
+        BQ_HIP(hipGetLastError());
+        if (zgemm(s, rest, wide, jb, mone, A + (long)j * lda + (j + jb), lda, A + (long)(j + jb) * lda + j, lda, one,
+                  A + (long)(j + jb) * lda + (j + jb), lda))
+          return -1;
+      }
     }
-    k_swap<<<(n + 255) / 256, 256, 0, s>>>(A, lda, n, j, jb, ipiv);
-    k_diag_finish<<<1, 1024, 0, s>>>(A, lda, j, jb, F, invj);
-    const int rest = n - j - jb;
-    if (rest > 0) k_l21<<<(rest + 63) / 64, 256, 0, s>>>(n, A, lda, j, jb, invj + kNB * kNB);
+    // the panel's interchanges on the columns left and right of it
+    if (j0 > 0) k_swap<<<(j0 + 255) / 256, 256, 0, s>>>(A, lda, 0, j0, j0, jb2, ipiv);
+    const int right = n - je;
+    if (right > 0) {
+      k_swap<<<(right + 255) / 256, 256, 0, s>>>(A, lda, je, right, j0, jb2, ipiv);
+      // U12 of the block row: unit-lower solve with the panel's L11, 32 rows at a time
+      for (int j = j0; j < je; j += kNB) {
+        const int jb = min(kNB, je - j);
+        k_apply_blk<<<(right + 63) / 64, 256, 0, s>>>(inv + (long)(j / kNB) * kBlkElems, jb, A + (long)je * lda + j,
+                                                      lda, right);
+        BQ_HIP(hipGetLastError());
+        if (je - j - jb > 0 && zgemm(s, je - j - jb, right, jb, mone, A + (long)j * lda + (j + jb), lda,
+                                     A + (long)je * lda + j, lda, one, A + (long)je * lda + (j + jb), lda))
+          return -1;
+      }
+    }
     bq::ktime_end(kt_p, s);
     BQ_HIP(hipGetLastError());
-    if (rest > 0) {
-      k_apply_blk<<<(rest + 63) / 64, 256, 0, s>>>(invj, jb, A + (long)(j + jb) * lda + j, lda, rest);
-      BQ_HIP(hipGetLastError());
+    if (right > 0) {
       int rc = 0;
       BQ_KTIME("zgetrf_update", s,
-               rc = zgemm(s, rest, rest, jb, mone, A + (long)j * lda + (j + jb), lda, A + (long)(j + jb) * lda + j,
-                          lda, one, A + (long)(j + jb) * lda + (j + jb), lda));
+               rc = zgemm(s, right, right, jb2, mone, A + (long)j0 * lda + je, lda, A + (long)je * lda + j0, lda, one,
+                          A + (long)je * lda + je, lda));
       if (rc) return -1;
     }
   }
@@ -440,21 +472,32 @@ int bq_zgetrs_dev(void* stream, int n, int nrhs, const double* Ad, long lda, con
   BQ_HIP(hipFreeAsync(Bt, s));
   BQ_HIP(hipFreeAsync(perm, s));
   const unsigned cblk = (unsigned)((nrhs + 63) / 64);
-  // forward: unit lower L
-  for (int j = 0; j < n; j += kNB) {
-    const int jb = min(kNB, n - j);
-    k_apply_blk<<<cblk, 256, 0, s>>>(inv + (long)(j / kNB) * kBlkElems, jb, B + j, ldb, nrhs);
-    BQ_HIP(hipGetLastError());
-    const int rest = n - j - jb;
-    if (rest > 0 && zgemm(s, rest, nrhs, jb, mone, A + (long)j * lda + (j + jb), lda, B + j, ldb, one, B + j + jb, ldb))
+  // forward: unit lower L, 256-row blocks of 32-row steps
+  for (int j0 = 0; j0 < n; j0 += kNB2) {
+    const int je = min(n, j0 + kNB2);
+    for (int j = j0; j < je; j += kNB) {
+      const int jb = min(kNB, je - j);
+      k_apply_blk<<<cblk, 256, 0, s>>>(inv + (long)(j / kNB) * kBlkElems, jb, B + j, ldb, nrhs);
+      BQ_HIP(hipGetLastError());
+      if (je - j - jb > 0 &&
+          zgemm(s, je - j - jb, nrhs, jb, mone, A + (long)j * lda + (j + jb), lda, B + j, ldb, one, B + j + jb, ldb))
+        return -1;
+    }
+    if (n - je > 0 &&
+        zgemm(s, n - je, nrhs, je - j0, mone, A + (long)j0 * lda + je, lda, B + j0, ldb, one, B + je, ldb))
       return -1;
   }
-  // backward: U
-  for (int j = ((n - 1) / kNB) * kNB; j >= 0; j -= kNB) {
-    const int jb = min(kNB, n - j);
-    k_apply_blk<<<cblk, 256, 0, s>>>(inv + (long)(j / kNB) * kBlkElems + kNB * kNB, jb, B + j, ldb, nrhs);
-    BQ_HIP(hipGetLastError());
-    if (j > 0 && zgemm(s, j, nrhs, jb, mone, A + (long)j * lda, lda, B + j, ldb, one, B, ldb)) return -1;
+  // backward: U, 256-row blocks from the bottom, 32-row steps upwards inside each
+  for (int j0 = ((n - 1) / kNB2) * kNB2; j0 >= 0; j0 -= kNB2) {
+    const int je = min(n, j0 + kNB2);
+    for (int j = ((je - 1 - j0) / kNB) * kNB + j0; j >= j0; j -= kNB) {
+      const int jb = min(kNB, je - j);
+      k_apply_blk<<<cblk, 256, 0, s>>>(inv + (long)(j / kNB) * kBlkElems + kNB * kNB, jb, B + j, ldb, nrhs);
+      BQ_HIP(hipGetLastError());
+      if (j > j0 && zgemm(s, j - j0, nrhs, jb, mone, A + (long)j * lda + j0, lda, B + j, ldb, one, B + j0, ldb))
+        return -1;
+    }
+    if (j0 > 0 && zgemm(s, j0, nrhs, je - j0, mone, A + (long)j0 * lda, lda, B + j0, ldb, one, B, ldb)) return -1;
   }
   bq::ktime_end(kt_all, s);
   return 0;

@@ -239,15 +239,17 @@ def main():
             return st, E, H
 
         with _QuietStdout():
-            for _ in range(args.warmup):
+            for i in range(args.warmup):
                 st, E0, H0 = step()
+                print(f"bench.py: warm-up step {i + 1}/{args.warmup} done", file=sys.stderr, flush=True)
             L.bq_kernel_timing(1)
             lat = []
             t_all = time.perf_counter()
-            for _ in range(args.steps):
+            for i in range(args.steps):
                 t = time.perf_counter()
                 st, E, H = step()
                 lat.append(time.perf_counter() - t)
+                print(f"bench.py: timed step {i + 1}/{args.steps}: {lat[-1]:.2f} s", file=sys.stderr, flush=True)
             elapsed = time.perf_counter() - t_all
         kt = _kernel_times(L)
         L.bq_kernel_timing(0)
@@ -292,7 +294,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
     }
-    print(json.dumps(out))
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

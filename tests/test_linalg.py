@@ -34,3 +34,21 @@ def test_gpu_zgesv_needs_pivoting(gpu_lib):
     b = rng.standard_normal(n) + 0j
     x = gpu_lib.zgesv(A, b)
     assert np.abs(A @ x - b).max() < 1e-11 * np.abs(b).max() * np.abs(A).max() * n
+
+
+@pytest.mark.gpu
+def test_gpu_zgesv_pivots_across_outer_panels(gpu_lib):
+    """Zero diagonal blocks straddling the 256-column outer panel boundary and the 32-column
+    steps: the interchanges chosen inside one outer panel must reach the columns on both
+    sides of it."""
+    rng = np.random.default_rng(11)
+    n = 600
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    A[250:262, 250:262] = 0.0
+    A[510:515, :] *= 1e-6
+    B = rng.standard_normal((n, 40)) + 1j * rng.standard_normal((n, 40))
+    X = gpu_lib.zgesv(A, B)
+    ref = np.linalg.solve(A, B)
+    bw = np.abs(A @ X - B).max() / (np.abs(A).max() * np.abs(X).max() * n)
+    assert bw < 1e-13
+    assert np.abs(X - ref).max() / np.abs(ref).max() < 1e-8
