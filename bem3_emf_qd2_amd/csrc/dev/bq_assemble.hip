@@ -116,9 +116,11 @@ __global__ void __launch_bounds__(256) k_classify(const double* __restrict__ E, 
 }
 
 // ------------------------------------------------------------------------------------
-// domain 0 (periodic), regular pairs, 12x12 rule.  The per-lane sums live in LDS rather than
-// registers: the lattice sums need nearly 256 registers, and keeping the kernel within 256
-// lets two wavefronts share each SIMD.
+// domain 0 (periodic), regular pairs, 12-point rule.  One wavefront per pair (t, s): the
+// target's nodes share the 64 lanes (4 x 16 for a quad, 3 x 21 for a triangle, whose fourth
+// row does not exist), each lane walks the source's Gauss points with stride 16 or 21.
+// The per-lane sums live in LDS rather than registers: the lattice sums need nearly 256
+// registers, and keeping the kernel within 256 lets two wavefronts share each SIMD.
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
 k_q0_gl(bq_qplan P, const double* __restrict__ E, const double* __restrict__ ren, const int* __restrict__ sid, int dNe,
         const unsigned char* __restrict__ flag, double2* __restrict__ G, double2* __restrict__ H)
@@ -132,10 +134,11 @@ k_q0_gl(bq_qplan P, const double* __restrict__ E, const double* __restrict__ ren
   if (pair >= (long)dNe * dNe) return;
   const int t = (int)(pair / dNe) + 1, s = (int)(pair % dNe) + 1;
   if (t == s) return;
-  const int tn = lane >> 4, sub = lane & 15;
-  const long N = 4L * dNe, row = 4L * (t - 1) + tn;
-  const bool active = flag[row * dNe + (s - 1)] == 0;
-  const bool stri = elem_is_tri(E, abs(sid[s]));
+  const bool ttri = elem_is_tri(E, abs(sid[t])), stri = elem_is_tri(E, abs(sid[s]));
+  const int gw = ttri ? 21 : 16, nodes = ttri ? 3 : 4;
+  const int tn = lane / gw, sub = lane - tn * gw;
+  const long N = 4L * dNe, row = 4L * (t - 1) + (tn < nodes ? tn : 0);
+  const bool active = tn < nodes && flag[row * dNe + (s - 1)] == 0;
   double* acc = &sacc[0][threadIdx.x];  // acc[k * 256]
 #pragma unroll
   for (int k = 0; k < 16; ++k) acc[k * 256] = 0.0;
@@ -151,11 +154,14 @@ k_q0_gl(bq_qplan P, const double* __restrict__ E, const double* __restrict__ ren
       rt[1] = p[1];
       rt[2] = p[2];
     }
-    const int nm = stri ? 27 : 9;  // 144 or 432 points, 16 lanes per node
+    const int np = stri ? 432 : 144;
+    const int nm = (np + gw - 1) / gw;
 #pragma unroll 1
     for (int m = 0; m < nm; ++m) {
+      const int p = sub + gw * m;
+      if (p >= np) break;
       double z, et, w;
-      gauss_point(stri, false, sub + 16 * m, z, et, w);
+      gauss_point(stri, false, p, z, et, w);
       double R[3], W[3], Nn[4];
       {
         Elem e;
@@ -170,23 +176,26 @@ k_q0_gl(bq_qplan P, const double* __restrict__ E, const double* __restrict__ ren
       shapeN(stri, z, et, Nn);
 #pragma unroll
       for (int n = 0; n < 4; ++n) {
-        double gw = w * Nn[n] * aW, hw = w * Nn[n];
-        acc[(2 * n) * 256] += gw * q.g.re;
-        acc[(2 * n + 1) * 256] += gw * q.g.im;
+        double gw2 = w * Nn[n] * aW, hw = w * Nn[n];
+        acc[(2 * n) * 256] += gw2 * q.g.re;
+        acc[(2 * n + 1) * 256] += gw2 * q.g.im;
         acc[(8 + 2 * n) * 256] += hw * hn.re;
         acc[(8 + 2 * n + 1) * 256] += hw * hn.im;
       }
     }
   }
-  double a[16];
-#pragma unroll
-  for (int k = 0; k < 16; ++k) a[k] = acc[k * 256];
-  group_sum<16, 16>(a);
-  if (active && sub < 2) {
-    double2* dst = (sub == 0 ? G : H) + row * N + 4L * (s - 1);
-    const int o = sub * 8;
-#pragma unroll
-    for (int n = 0; n < 4; ++n) dst[n] = make_double2(a[o + 2 * n], a[o + 2 * n + 1]);
+  __builtin_amdgcn_wave_barrier();
+  // per node, lane k < 16 of the node's first 16 lanes sums value k over the node's lanes
+  // in lane order (fixed order: the result does not depend on scheduling)
+  if (tn < nodes && sub < 16) {
+    const bool act_n = flag[row * dNe + (s - 1)] == 0;
+    if (act_n) {
+      const int base = wv * 64 + tn * gw;
+      double v = 0.0;
+      for (int j = 0; j < gw; ++j) v += sacc[sub][base + j];
+      double* dst = (double*)((sub < 8 ? G : H) + row * N + 4L * (s - 1));
+      dst[sub & 7] = v;
+    }
   }
 }
 
