@@ -19,6 +19,7 @@ typedef struct bq_fctx {
   int dNe[BQ_MAXD];
   const int *sid[BQ_MAXD];               /* device, per domain */
   const double *Eb[BQ_MAXD], *dEb[BQ_MAXD], *Hb[BQ_MAXD], *dHb[BQ_MAXD]; /* device, [(dNe+1)][4][3] complex */
+  const double *Ub[BQ_MAXD], *dUb[BQ_MAXD]; /* device, [(dNe+1)][4][4] complex */
 } bq_fctx;
 
 /* host-side description of a solved model */
@@ -29,19 +30,22 @@ typedef struct bq_fdata {
   const double *E, *ren, *wen;           /* [(Ne+1)][24] (bq_pack_elem), [(Ne+1)][12], [(Ne+1)][12] */
   const int *dNe;
   const int *const *sid;
-  const double *const *Eb, *const *dEb, *const *Hb, *const *dHb;
+  const double *const *Eb, *const *dEb, *const *Hb, *const *dHb, *const *Ub, *const *dUb;
   const double *xli, *wli, *xhi, *whi, *z49, *e49, *w49;
   const double *w34, *z37, *e37, *w37;   /* triangle node-rule weights and 7-point rule */
 } bq_fdata;
 
 int bq_field_ctx_create(const bq_fdata *f, void **handle);
-/* out[n][13] complex-as-doubles: sum E (3 complex), sum H (3 complex), F (1 real) for each
- * folded point with known domain id */
-int bq_field_eval(void *handle, int n, const double *pts, const int *did, int type, double *out);
+/* kind 0: out[n][13] = sum E (3 complex), sum H (3 complex), F; kind 1: out[n][9] = sum U
+ * (4 complex), F; for folded points pts[n][3] with known domain ids, or, when bd_t is not
+ * NULL, for the boundary points (bd_ze[2p], bd_ze[2p+1]) of elements bd_t[p] (1-based
+ * positions in the domain's list; pts unused). */
+int bq_field_eval(void *handle, int kind, int n, const double *pts, const int *did, const int *bd_t,
+                  const double *bd_ze, int type, double *out);
 void bq_field_ctx_destroy(void *handle);
 
-int bq_field_eval_dev(const bq_fctx *C, int npts, const double *d_pts, const int *d_did, int type, double *d_out,
-                      void *stream);
+int bq_field_eval_dev(const bq_fctx *C, int kind, int npts, const double *d_pts, const int *d_did, const int *d_bd_t,
+                      const double *d_bd_ze, int type, double *d_out, void *stream);
 int bq_field_rules_init(const double *z49, const double *e49, const double *w49, const double *w34,
                         const double *z37, const double *e37, const double *w37);
 

@@ -48,25 +48,85 @@ __device__ __forceinline__ void rule_point(int type, bool tri, int q, double& z,
   }
 }
 
-__global__ void __launch_bounds__(256) k_field(bq_fctx C, int npts, const double* __restrict__ pts,
-                                               const int* __restrict__ did_of, int type, double* __restrict__ out)
+// Node data of domain did: KIND 0 -> E, dE, H, dH ([(dNe+1)][4][3] complex), 13 outputs per
+// point (E, H, F); KIND 1 -> U, dU ([(dNe+1)][4][4] complex), 9 outputs (U, F).
+template <int KIND>
+struct Acc {
+  static constexpr int NV = KIND == 0 ? 13 : 9;
+};
+
+// add g dX - h X over the element's nodes (weights already in g, h per node)
+template <int KIND>
+__device__ __forceinline__ void add_node_terms(const bq_fctx& C, int did, int sc, const cplx (&gn)[4],
+                                               const cplx (&hn)[4], double (&acc)[Acc<KIND>::NV])
 {
-  __shared__ double lds[13 * 4];
+  if (KIND == 0) {
+    const double *Eb = C.Eb[did], *dEb = C.dEb[did], *Hb = C.Hb[did], *dHb = C.dHb[did];
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+      const long o = 12L * (sc + 1) + 3 * n;
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        cplx ev = gn[n] * ld2(dEb, o + i) - hn[n] * ld2(Eb, o + i);
+        cplx hv = gn[n] * ld2(dHb, o + i) - hn[n] * ld2(Hb, o + i);
+        acc[2 * i] += ev.re;
+        acc[2 * i + 1] += ev.im;
+        acc[6 + 2 * i] += hv.re;
+        acc[6 + 2 * i + 1] += hv.im;
+      }
+    }
+  } else {
+    const double *Ub = C.Ub[did], *dUb = C.dUb[did];
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+      const long o = 16L * (sc + 1) + 4 * n;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        cplx uv = gn[n] * ld2(dUb, o + i) - hn[n] * ld2(Ub, o + i);
+        acc[2 * i] += uv.re;
+        acc[2 * i + 1] += uv.im;
+      }
+    }
+  }
+}
+
+// One 256-thread block per point.  Free points: pts[3p].  Boundary points (bd_t != null):
+// the point (zeta, eta) = bd_ze[2p] of element t = bd_t[p] (position in the domain's list);
+// every other element uses the rule `type` at that point, the element itself the polar rule
+// around it (q0_coef_bd / coef_bd, d3qd2_src/d3qd2_elem.c:684-770, :187-230).
+template <int KIND>
+__global__ void __launch_bounds__(256) k_field(bq_fctx C, int npts, const double* __restrict__ pts,
+                                               const int* __restrict__ did_of, const int* __restrict__ bd_t,
+                                               const double* __restrict__ bd_ze, int type, double* __restrict__ out)
+{
+  constexpr int NV = Acc<KIND>::NV;
+  __shared__ double lds[NV * 4];
   const int p = blockIdx.x;
   const int did = did_of[p];
-  const double* r0 = pts + 3 * p;
   const int dNe = C.dNe[did];
   const int* sid = C.sid[did];
-  const double *Eb = C.Eb[did], *dEb = C.dEb[did], *Hb = C.Hb[did], *dHb = C.dHb[did];
   const bool periodic = (did == 0 && C.ps == 1);
   const cplx k = mk(C.kn[2 * did], C.kn[2 * did + 1]);
+  const int self = bd_t ? bd_t[p] : 0;  // 1-based position of the point's element, 0 if none
+  double r0[3];
+  if (self) {
+    Elem et;
+    load_elem(C.E, abs(sid[self]), et);
+    double W[3];
+    rw_at(et, bd_ze[2 * p], bd_ze[2 * p + 1], r0, W);
+  } else {
+    r0[0] = pts[3 * p];
+    r0[1] = pts[3 * p + 1];
+    r0[2] = pts[3 * p + 2];
+  }
   const int nq = C.has_tri ? max(rule_size(type, false), rule_size(type, true)) : rule_size(type, false);
-  double acc[13];
+  double acc[NV];
 #pragma unroll
-  for (int i = 0; i < 13; ++i) acc[i] = 0.0;
+  for (int i = 0; i < NV; ++i) acc[i] = 0.0;
   const long total = (long)dNe * nq;
   for (long it = threadIdx.x; it < total; it += 256) {
     const int sc = (int)(it / nq), q = (int)(it - (long)sc * nq);
+    if (sc + 1 == self) continue;
     const int sd = sid[sc + 1], as = abs(sd);
     Elem e;
     load_elem(C.E, as, e);
@@ -108,28 +168,77 @@ __global__ void __launch_bounds__(256) k_field(bq_fctx C, int npts, const double
       h = (ca - mk(1.0, 0.0)) * ce * (tD * kI4PI);
       if (sd < 0) sgn = -1.0;
     }
-    acc[12] += w * tD * kI4PI * sgn;
-    const long base = 12L * as;  // unused for domain data indexing; kept for clarity
-    (void)base;
+    acc[NV - 1] += w * tD * kI4PI * sgn;
+    cplx gn[4], hn[4];
 #pragma unroll
     for (int n = 0; n < 4; ++n) {
-      cplx gn = g * (w * Nn[n]), hn = h * (w * Nn[n] * sgn);
-      long o = 12L * (sc + 1) + 3 * n;
+      gn[n] = g * (w * Nn[n]);
+      hn[n] = h * (w * Nn[n] * sgn);
+    }
+    add_node_terms<KIND>(C, did, sc, gn, hn, acc);
+  }
+  if (self) {
+    // the point's own element: polar rule around (zeta, eta), 128 angles per sector in two
+    // 64-point halves, 12 radii (q0_bil_coef_bd d3qd2_src/q0_bil_coef.c:246-338,
+    // q0_lit_coef_bd q0_lit_coef.c:297-371, bil_coef_bd bil_coef.c:244-342, lit_coef_bd)
+    const int sd = sid[self];
+    Elem e;
+    load_elem(C.E, abs(sd), e);
+    const double zt = bd_ze[2 * p], ett = bd_ze[2 * p + 1];
+    double rr5[5], th[5];
+    const int nsec = corner_r_th(e.tri, zt, ett, rr5, th);
+    const bool plane = elem_plane(e);
+    const double sF = plane ? 0.0 : elem_bdC(e) * kI4PI;  // polar F and free-space H factor
+    const double sg = (!periodic && sd < 0) ? -1.0 : 1.0;
+    for (int q = threadIdx.x; q < nsec * 1536; q += 256) {
+      const int j = q / 1536, m = (q / 12) % 128, i = q % 12;
+      double theta, wt;
+      polar_theta(th, j, m, theta, wt);
+      double rm = polar_rm(rr5, th, j, theta), cth, sth;
+      sincos(theta, &sth, &cth);
+      const double rr = 0.5 * rm * (c_xl[i] + 1.0), w = wt * 0.5 * rm * c_wl[i];
+      double K[3], R[3], W[3], Nn[4];
+      polar_K(e, zt, ett, rr, cth, sth, K);
+      const double z = rr * cth + zt, ee = rr * sth + ett;
 #pragma unroll
-      for (int i = 0; i < 3; ++i) {
-        cplx ev = gn * ld2(dEb, o + i) - hn * ld2(Eb, o + i);
-        cplx hv = gn * ld2(dHb, o + i) - hn * ld2(Hb, o + i);
-        acc[2 * i] += ev.re;
-        acc[2 * i + 1] += ev.im;
-        acc[6 + 2 * i] += hv.re;
-        acc[6 + 2 * i + 1] += hv.im;
+      for (int d = 0; d < 3; ++d) {
+        R[d] = rr * K[d];
+        W[d] = e.cw[d][0] + e.cw[d][1] * z + e.cw[d][2] * ee;
       }
+      const double aK = sqrt(dot3(K, K)), aW = sqrt(dot3(W, W));
+      cplx g, h;
+      if (periodic) {
+        QG qq;
+        qpgf_v1(C.plan, R[0], R[1], R[2], qq);
+        // the reference takes the value from d3hm_qpgf_d2_qG here (q0_bil_coef.c:444,
+        // q0_lit_coef.c:569); it differs from _v1 only at R_z == 0
+        if (R[2] == 0.0) {
+          QG qa;
+          qpgf_v1(C.plan, R[0], R[1], R[2], qa, false);
+          qq.g = qa.g;
+        }
+        g = qq.g * (aW * rr);
+        h = (qq.d[0] * W[0] + qq.d[1] * W[1] + qq.d[2] * W[2]) * rr;
+      } else {
+        const cplx kR = k * (rr * aK), ce = cexp(cmul_i(kR));
+        g = ce * (aW / aK * kI4PI);
+        h = (cmul_i(kR) - mk(1.0, 0.0)) * ce * (cth * sth / (aK * aK * aK) * sF * sg);
+      }
+      acc[NV - 1] += w * cth * sth / (aK * aK * aK) * sF * sg;
+      shapeN(e.tri, z, ee, Nn);
+      cplx gn[4], hn[4];
+#pragma unroll
+      for (int n = 0; n < 4; ++n) {
+        gn[n] = g * (w * Nn[n]);
+        hn[n] = h * (w * Nn[n]);
+      }
+      add_node_terms<KIND>(C, did, self - 1, gn, hn, acc);
     }
   }
-  block_sum256<13>(acc, lds);
+  block_sum256<NV>(acc, lds);
   if (threadIdx.x == 0) {
-    double* o = out + 13L * p;
-    for (int i = 0; i < 13; ++i) o[i] = acc[i];
+    double* o = out + (long)NV * p;
+    for (int i = 0; i < NV; ++i) o[i] = acc[i];
   }
 }
 
@@ -153,12 +262,17 @@ extern "C" int bq_field_rules_init(const double* z49, const double* e49, const d
 }
 
 // out[p][13] = E (3 complex), H (3 complex), F for each folded point
-extern "C" int bq_field_eval_dev(const bq_fctx* C, int npts, const double* d_pts, const int* d_did, int type,
-                                 double* d_out, void* stream)
+// kind 0: out[p][13] = E (3 complex), H (3 complex), F; kind 1: out[p][9] = U (4 complex), F.
+// d_bd_t / d_bd_ze (may be null): boundary queries instead of free points.
+extern "C" int bq_field_eval_dev(const bq_fctx* C, int kind, int npts, const double* d_pts, const int* d_did,
+                                 const int* d_bd_t, const double* d_bd_ze, int type, double* d_out, void* stream)
 {
   if (npts <= 0) return 0;
   hipStream_t s = (hipStream_t)stream;
-  BQ_KTIME("k_field", s, k_field<<<npts, 256, 0, s>>>(*C, npts, d_pts, d_did, type, d_out));
+  if (kind == 0)
+    BQ_KTIME("k_field", s, k_field<0><<<npts, 256, 0, s>>>(*C, npts, d_pts, d_did, d_bd_t, d_bd_ze, type, d_out));
+  else
+    BQ_KTIME("k_field_u", s, k_field<1><<<npts, 256, 0, s>>>(*C, npts, d_pts, d_did, d_bd_t, d_bd_ze, type, d_out));
   BQ_HIP(hipGetLastError());
   return 0;
 }
@@ -216,7 +330,8 @@ extern "C" int bq_field_ctx_create(const bq_fdata* f, void** handle)
     C.dNe[d] = f->dNe[d];
     if (up(F, f->sid[d], m, &C.sid[d], s) || up(F, f->Eb[d], m * 24, &C.Eb[d], s) ||
         up(F, f->dEb[d], m * 24, &C.dEb[d], s) || up(F, f->Hb[d], m * 24, &C.Hb[d], s) ||
-        up(F, f->dHb[d], m * 24, &C.dHb[d], s)) {
+        up(F, f->dHb[d], m * 24, &C.dHb[d], s) || up(F, f->Ub[d], m * 32, &C.Ub[d], s) ||
+        up(F, f->dUb[d], m * 32, &C.dUb[d], s)) {
       bq_field_ctx_destroy(F);
       return -1;
     }
@@ -231,7 +346,8 @@ extern "C" int bq_field_ctx_create(const bq_fdata* f, void** handle)
   return 0;
 }
 
-extern "C" int bq_field_eval(void* handle, int n, const double* pts, const int* did, int type, double* out)
+extern "C" int bq_field_eval(void* handle, int kind, int n, const double* pts, const int* did, const int* bd_t,
+                             const double* bd_ze, int type, double* out)
 {
   if (n <= 0) return 0;
   if (type < 0 || type > 3) {
@@ -240,18 +356,25 @@ extern "C" int bq_field_eval(void* handle, int n, const double* pts, const int* 
   }
   FieldCtx* F = (FieldCtx*)handle;
   hipStream_t s = bq::stream();
-  double *dp = nullptr, *dout = nullptr;
-  int* dd = nullptr;
-  BQ_HIP(hipMallocAsync((void**)&dp, sizeof(double) * 3 * n, s));
+  const int nv = kind == 0 ? 13 : 9;
+  double *dp = nullptr, *dout = nullptr, *dze = nullptr;
+  int *dd = nullptr, *dt = nullptr;
   BQ_HIP(hipMallocAsync((void**)&dd, sizeof(int) * n, s));
-  BQ_HIP(hipMallocAsync((void**)&dout, sizeof(double) * 13 * n, s));
-  BQ_HIP(hipMemcpyAsync(dp, pts, sizeof(double) * 3 * n, hipMemcpyHostToDevice, s));
+  BQ_HIP(hipMallocAsync((void**)&dout, sizeof(double) * nv * n, s));
   BQ_HIP(hipMemcpyAsync(dd, did, sizeof(int) * n, hipMemcpyHostToDevice, s));
-  if (bq_field_eval_dev(&F->C, n, dp, dd, type, dout, s)) return -1;
-  BQ_HIP(hipMemcpyAsync(out, dout, sizeof(double) * 13 * n, hipMemcpyDeviceToHost, s));
-  BQ_HIP(hipFreeAsync(dp, s));
-  BQ_HIP(hipFreeAsync(dd, s));
-  BQ_HIP(hipFreeAsync(dout, s));
+  if (bd_t) {
+    BQ_HIP(hipMallocAsync((void**)&dt, sizeof(int) * n, s));
+    BQ_HIP(hipMallocAsync((void**)&dze, sizeof(double) * 2 * n, s));
+    BQ_HIP(hipMemcpyAsync(dt, bd_t, sizeof(int) * n, hipMemcpyHostToDevice, s));
+    BQ_HIP(hipMemcpyAsync(dze, bd_ze, sizeof(double) * 2 * n, hipMemcpyHostToDevice, s));
+  } else {
+    BQ_HIP(hipMallocAsync((void**)&dp, sizeof(double) * 3 * n, s));
+    BQ_HIP(hipMemcpyAsync(dp, pts, sizeof(double) * 3 * n, hipMemcpyHostToDevice, s));
+  }
+  if (bq_field_eval_dev(&F->C, kind, n, dp, dd, dt, dze, type, dout, s)) return -1;
+  BQ_HIP(hipMemcpyAsync(out, dout, sizeof(double) * nv * n, hipMemcpyDeviceToHost, s));
+  for (void* q : {(void*)dp, (void*)dd, (void*)dout, (void*)dt, (void*)dze})
+    if (q) BQ_HIP(hipFreeAsync(q, s));
   BQ_HIP(hipStreamSynchronize(s));
   return 0;
 }

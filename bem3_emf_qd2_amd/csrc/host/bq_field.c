@@ -1,11 +1,13 @@
 /*
- * bq_field.c -- field API of the reference (d3qd2_src/d3qd2_field.c) on the GPU.
+ * bq_field.c -- field and force API of the reference (d3qd2_src/d3qd2_field.c,
+ * d3qd2_src/d3qd2_force.c) on the GPU.
  *
  * The solved model is uploaded once per DQD2 (the first field call after solve_bieq or
  * dat_read) and kept in HBM until finalize_dqd2.  The scalar entry points (EH_s, EH_t ...)
  * are safe to call from several host threads at once, as the reference's examples do from
  * OpenMP loops: calls are serialised on one lock and each evaluates its point on the GPU.
- * bq_EH_t_batch evaluates many points in one launch.
+ * bq_EH_t_batch and bq_EH_t_bd_batch evaluate many points in one launch; force_FN is one
+ * launch of boundary queries followed by the stress sums on the host.
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -43,9 +45,11 @@ static void flat_elem(BOUD *bd, double *E, double *ren, double *wen)
   }
 }
 
+/* [n1][4][m] complex -> interleaved doubles; zeros when the array is absent */
 static double *flat_z3(double complex ***a, int n1, int m)
 {
   double *o = calloc((size_t)n1 * 4 * m * 2, sizeof(double));
+  if (!a) return o;
   for (int i = 0; i < n1; i++)
     for (int j = 0; j < 4; j++)
       for (int c = 0; c < m; c++) {
@@ -73,6 +77,7 @@ static void *context_for(DQD2 *qd)
   const int **sid = calloc(MN + 1, sizeof(int *));
   double **Eb = calloc(MN + 1, sizeof(double *)), **dEb = calloc(MN + 1, sizeof(double *));
   double **Hb = calloc(MN + 1, sizeof(double *)), **dHb = calloc(MN + 1, sizeof(double *));
+  double **Ub = calloc(MN + 1, sizeof(double *)), **dUb = calloc(MN + 1, sizeof(double *));
   for (int d = 0; d <= MN; d++) {
     kn[2 * d] = creal(qd->kn[d]);
     kn[2 * d + 1] = cimag(qd->kn[d]);
@@ -83,6 +88,8 @@ static void *context_for(DQD2 *qd)
     dEb[d] = flat_z3(s->dE, s->Ne + 1, 3);
     Hb[d] = flat_z3(s->H, s->Ne + 1, 3);
     dHb[d] = flat_z3(s->dH, s->Ne + 1, 3);
+    Ub[d] = flat_z3(s->U, s->Ne + 1, 4);
+    dUb[d] = flat_z3(s->dU, s->Ne + 1, 4);
   }
   bq_fdata f;
   memset(&f, 0, sizeof f);
@@ -105,15 +112,18 @@ static void *context_for(DQD2 *qd)
   f.dEb = (const double *const *)dEb;
   f.Hb = (const double *const *)Hb;
   f.dHb = (const double *const *)dHb;
+  f.Ub = (const double *const *)Ub;
+  f.dUb = (const double *const *)dUb;
   f.xli = bd->xli; f.wli = bd->wli; f.xhi = bd->xhi; f.whi = bd->whi;
   f.z49 = bd->zt_49; f.e49 = bd->et_49; f.w49 = bd->wt_49;
   f.w34 = bd->wt_34; f.z37 = bd->zt_37; f.e37 = bd->et_37; f.w37 = bd->wt_37;
   void *ctx = NULL;
   int rc = bq_field_ctx_create(&f, &ctx);
   for (int d = 0; d <= MN; d++) {
-    free(Eb[d]); free(dEb[d]); free(Hb[d]); free(dHb[d]);
+    free(Eb[d]); free(dEb[d]); free(Hb[d]); free(dHb[d]); free(Ub[d]); free(dUb[d]);
   }
-  free(E); free(ren); free(wen); free(kn); free(dNe); free(sid); free(Eb); free(dEb); free(Hb); free(dHb);
+  free(E); free(ren); free(wen); free(kn); free(dNe); free(sid);
+  free(Eb); free(dEb); free(Hb); free(dHb); free(Ub); free(dUb);
   if (rc) field_fail("field set-up");
   g_cache.qd = qd;
   g_cache.key = key;
@@ -133,6 +143,21 @@ void bq_release_device_state(DQD2 *qd)
   pthread_mutex_unlock(&g_lock);
 }
 
+/* one launch over m queries (free points r0 or boundary points bt/bze); out [m][13 | 9] */
+static double *launch(DQD2 *qd, int kind, int m, const double *r0, const int *dd, const int *bt, const double *bze,
+                      int type)
+{
+  double *out = calloc((size_t)(kind == 0 ? 13 : 9) * (m > 0 ? m : 1), sizeof(double));
+  if (m > 0) {
+    pthread_mutex_lock(&g_lock);
+    void *ctx = context_for(qd);
+    int rc = bq_field_eval(ctx, kind, m, r0, dd, bt, bze, type, out);
+    pthread_mutex_unlock(&g_lock);
+    if (rc) field_fail("field evaluation");
+  }
+  return out;
+}
+
 static void fold(DQD2 *qd, const double *rt, double *r0, int *l1, int *l2)
 {
   if (qd->bd.ps == 1) {
@@ -147,41 +172,43 @@ static void fold(DQD2 *qd, const double *rt, double *r0, int *l1, int *l2)
   r0[2] = rt[2];
 }
 
-/* Scattered (or interior) E, H at n points; did[] receives the domain ids. */
-static void eh_scattered(DQD2 *qd, int n, const double *r, int type, double complex *E, double complex *H, int *did)
+static double complex bloch(DQD2 *qd, int l1, int l2)
 {
-  double *r0 = malloc(sizeof(double) * 3 * (n > 0 ? n : 1));
-  int *l1 = malloc(sizeof(int) * (n > 0 ? n : 1)), *l2 = malloc(sizeof(int) * (n > 0 ? n : 1));
-  int *dd = malloc(sizeof(int) * (n > 0 ? n : 1));
+  if (l1 == 0 && l2 == 0) return 1.0;
+  double arg = qd->bd.qd.vk[0] * (l1 * qd->bd.qd.vd1[0] + l2 * qd->bd.qd.vd2[0]) +
+               qd->bd.qd.vk[1] * (l1 * qd->bd.qd.vd1[1] + l2 * qd->bd.qd.vd2[1]);
+  return cos(arg) + I * sin(arg);
+}
+
+/* Scattered (or interior) fields at n points (EH_s, mEMP_s, d3qd2_src/d3qd2_field.c:10-59,
+ * :203-270).  kind 0: X = E [n][3], Y = H [n][3]; kind 1: X = U [n][4], Y unused.  did[]
+ * receives the domain ids; points outside every domain get zeros. */
+static void scattered(DQD2 *qd, int kind, int n, const double *r, int type, double complex *X, double complex *Y,
+                      int *did)
+{
+  const int nx = kind == 0 ? 3 : 4, nv = kind == 0 ? 13 : 9;
+  size_t nn = n > 0 ? n : 1;
+  double *r0 = malloc(sizeof(double) * 3 * nn);
+  int *l1 = malloc(sizeof(int) * nn), *l2 = malloc(sizeof(int) * nn), *dd = malloc(sizeof(int) * nn);
   int m = 0;
   for (int p = 0; p < n; p++) {
     fold(qd, r + 3 * p, r0 + 3 * m, &l1[p], &l2[p]);
     did[p] = domain_id(r0 + 3 * m, qd);
     if (did[p] >= 0) dd[m++] = did[p];
   }
-  double *out = calloc((size_t)13 * (m > 0 ? m : 1), sizeof(double));
-  if (m > 0) {
-    pthread_mutex_lock(&g_lock);
-    void *ctx = context_for(qd);
-    int rc = bq_field_eval(ctx, m, r0, dd, type, out);
-    pthread_mutex_unlock(&g_lock);
-    if (rc) field_fail("field evaluation");
-  }
+  double *out = launch(qd, kind, m, r0, dd, NULL, NULL, type);
   for (int p = 0, q = 0; p < n; p++) {
-    for (int i = 0; i < 3; i++) E[3 * p + i] = H[3 * p + i] = 0.0;
+    for (int i = 0; i < nx; i++) {
+      X[nx * p + i] = 0.0;
+      if (kind == 0) Y[nx * p + i] = 0.0;
+    }
     if (did[p] < 0) continue;
-    const double *o = out + 13 * q++;
-    double C = (did[p] == 0) ? 1.0 + o[12] : o[12];
-    double complex ce = 1.0;
-    if (l1[p] != 0 || l2[p] != 0) {
-      double arg = qd->bd.qd.vk[0] * (l1[p] * qd->bd.qd.vd1[0] + l2[p] * qd->bd.qd.vd2[0]) +
-                   qd->bd.qd.vk[1] * (l1[p] * qd->bd.qd.vd1[1] + l2[p] * qd->bd.qd.vd2[1]);
-      ce = cos(arg) + I * sin(arg);
-    }
-    for (int i = 0; i < 3; i++) {
-      E[3 * p + i] = (o[2 * i] + I * o[2 * i + 1]) / C * ce;
-      H[3 * p + i] = (o[6 + 2 * i] + I * o[6 + 2 * i + 1]) / C * ce;
-    }
+    const double *o = out + (size_t)nv * q++;
+    double C = (did[p] == 0) ? 1.0 + o[nv - 1] : o[nv - 1];
+    double complex ce = bloch(qd, l1[p], l2[p]);
+    for (int i = 0; i < nx; i++) X[nx * p + i] = (o[2 * i] + I * o[2 * i + 1]) / C * ce;
+    if (kind == 0)
+      for (int i = 0; i < 3; i++) Y[3 * p + i] = (o[6 + 2 * i] + I * o[6 + 2 * i + 1]) / C * ce;
   }
   free(r0); free(l1); free(l2); free(dd); free(out);
 }
@@ -189,7 +216,7 @@ static void eh_scattered(DQD2 *qd, int n, const double *r, int type, double comp
 int EH_s(double complex *E, double complex *H, double *rt, int type, DQD2 *qd)
 {
   int did;
-  eh_scattered(qd, 1, rt, type, E, H, &did);
+  scattered(qd, 0, 1, rt, type, E, H, &did);
   return did;
 }
 
@@ -220,7 +247,7 @@ int bq_EH_t_batch(void *qdv, int n, const double *r, int type, double *E, double
 {
   DQD2 *qd = (DQD2 *)qdv;
   double complex *Ez = (double complex *)E, *Hz = (double complex *)H;
-  eh_scattered(qd, n, r, type, Ez, Hz, did);
+  scattered(qd, 0, n, r, type, Ez, Hz, did);
   for (int p = 0; p < n; p++) {
     if (did[p] != 0) continue;
     double complex e[3], h[3];
@@ -231,6 +258,24 @@ int bq_EH_t_batch(void *qdv, int n, const double *r, int type, double *E, double
     }
   }
   return 0;
+}
+
+int mEMP_s(double complex *U, double *rt, int type, DQD2 *qd)
+{
+  int did;
+  scattered(qd, 1, 1, rt, type, U, NULL, &did);
+  return did;
+}
+
+int mEMP_t(double complex *U, double *rt, int type, DQD2 *qd)
+{
+  int did = mEMP_s(U, rt, type, qd);
+  if (did == 0) {
+    double complex e[3], h[3];
+    calc_mipw_EH(e, h, rt, &qd->pw);
+    for (int i = 0; i < 3; i++) U[i] += e[i];
+  }
+  return did;
 }
 
 int mEMP_i(double complex *U, double *rt, int type, DQD2 *qd)
@@ -249,16 +294,202 @@ int EH_mEMP_i(double complex *E, double complex *H, double *rt, int type, DQD2 *
   return EH_i(E, H, rt, type, qd);
 }
 
+/* position on element seid at (zeta, eta) (r_bd, d3qd2_src/d3qd2_elem.c:18-29) */
+static void r_on(const BOUD *bd, int seid, double zt, double et, double *r)
+{
+  int s = abs(seid);
+  for (int i = 0; i < 3; i++) {
+    r[i] = bd->cr[s][i][0] + bd->cr[s][i][1] * zt + bd->cr[s][i][2] * et;
+    if (bd->ed[s][3] != 0) r[i] += bd->cr[s][i][3] * zt * et;
+  }
+}
+
+static int bd_query_ok(DQD2 *qd, int did, int t)
+{
+  return did >= 0 && did <= qd->MN && t >= 1 && t <= qd->bd.sb[did].Ne;
+}
+
+/* Fields at n boundary points (EH_s_bd / EH_t_bd, d3qd2_src/d3qd2_field.c:299-391): point p
+ * is (zt[p], et[p]) on the t[p]-th element (1-based) of domain did[p]; total adds the
+ * incident wave in domain 0.  Returns -1 (nothing evaluated) on an invalid query. */
+static int bd_fields(DQD2 *qd, int n, const int *did, const int *t, const double *zt, const double *et, int type,
+                     int total, double complex *E, double complex *H)
+{
+  for (int p = 0; p < n; p++)
+    if (!bd_query_ok(qd, did[p], t[p])) return -1;
+  size_t nn = n > 0 ? n : 1;
+  double *ze = malloc(sizeof(double) * 2 * nn);
+  for (int p = 0; p < n; p++) {
+    ze[2 * p] = zt[p];
+    ze[2 * p + 1] = et[p];
+  }
+  double *out = launch(qd, 0, n, NULL, did, t, ze, type);
+  for (int p = 0; p < n; p++) {
+    const double *o = out + 13 * (size_t)p;
+    double C = (did[p] == 0) ? 1.0 + o[12] : o[12];
+    for (int i = 0; i < 3; i++) {
+      E[3 * p + i] = (o[2 * i] + I * o[2 * i + 1]) / C;
+      H[3 * p + i] = (o[6 + 2 * i] + I * o[6 + 2 * i + 1]) / C;
+    }
+    if (total && did[p] == 0) {
+      double r[3];
+      double complex e[3], h[3];
+      r_on(&qd->bd, qd->bd.sb[0].sid[t[p]], zt[p], et[p], r);
+      calc_mipw_EH(e, h, r, &qd->pw);
+      for (int i = 0; i < 3; i++) {
+        E[3 * p + i] += e[i];
+        H[3 * p + i] += h[i];
+      }
+    }
+  }
+  free(ze);
+  free(out);
+  return 0;
+}
+
+static void bd_one(DQD2 *qd, const char *name, double complex *E, double complex *H, int did, int t, double zt,
+                   double et, int type, int total)
+{
+  if (bd_fields(qd, 1, &did, &t, &zt, &et, type, total, E, H)) {
+    printf("%s(): element %d of domain %d does not exist. Exit...\n", name, t, did);
+    exit(1);
+  }
+}
+
+void EH_s_bd(double complex *E, double complex *H, int did, int t, double zeta_t, double eta_t, int type, DQD2 *qd)
+{
+  bd_one(qd, "EH_s_bd", E, H, did, t, zeta_t, eta_t, type, 0);
+}
+
+void EH_t_bd(double complex *E, double complex *H, int did, int t, double zeta_t, double eta_t, int type, DQD2 *qd)
+{
+  bd_one(qd, "EH_t_bd", E, H, did, t, zeta_t, eta_t, type, 1);
+}
+
+int bq_EH_t_bd_batch(void *qdv, int n, const int *did, const int *t, const double *zt, const double *et, int type,
+                     double *E, double *H)
+{
+  if (n < 0 || type < 0 || type > 3) return -1;
+  return bd_fields((DQD2 *)qdv, n, did, t, zt, et, type, 1, (double complex *)E, (double complex *)H);
+}
+
 void EH_i_bd(double complex *E, double complex *H, int did, int t, double zeta_t, double eta_t, int type, DQD2 *qd)
 {
   (void)type;
-  int td = qd->bd.sb[did].sid[t], s = abs(td);
   double r[3];
-  for (int i = 0; i < 3; i++)
-    r[i] = qd->bd.cr[s][i][0] + qd->bd.cr[s][i][1] * zeta_t + qd->bd.cr[s][i][2] * eta_t +
-           qd->bd.cr[s][i][3] * zeta_t * eta_t;
+  r_on(&qd->bd, qd->bd.sb[did].sid[t], zeta_t, eta_t, r);
   if (did == 0) calc_mipw_EH(E, H, r, &qd->pw);
   else for (int i = 0; i < 3; i++) E[i] = H[i] = 0.0;
+}
+
+/* Maxwell stress tensor T = 1/2 Re(eps E E* + H H*) - 1/4 (eps |E|^2 + |H|^2) I with
+ * eps = Re(n0)^2, applied to the surface element w: adds wt T w to F and wt (r - rc) x (T w)
+ * to N (the stress sums of bil_force_4p, d3qd2_src/d3qd2_force.c:56-110). */
+static void stress_add(double epd, const double complex *e, const double complex *h, const double *r,
+                       const double *w, const double *rc, double wt, double *F, double *N)
+{
+  double aE2 = 0.0, aH2 = 0.0, T[9];
+  for (int j = 0; j < 3; j++) {
+    aE2 += creal(e[j] * conj(e[j]));
+    aH2 += creal(h[j] * conj(h[j]));
+  }
+  double Tc = 0.25 * (epd * aE2 + aH2);
+  for (int j = 0; j < 3; j++)
+    for (int k = 0; k < 3; k++) T[3 * j + k] = 0.5 * (epd * creal(e[j] * conj(e[k])) + creal(h[j] * conj(h[k])));
+  T[0] -= Tc;
+  T[4] -= Tc;
+  T[8] -= Tc;
+  double d[3] = {r[0] - rc[0], r[1] - rc[1], r[2] - rc[2]};
+  for (int j = 0; j < 3; j++) {
+    F[j] += (T[3 * j] * w[0] + T[3 * j + 1] * w[1] + T[3 * j + 2] * w[2]) * wt;
+    N[j] += ((d[1] * T[3 * j + 2] - d[2] * T[3 * j + 1]) * w[0] + (d[2] * T[3 * j] - d[0] * T[3 * j + 2]) * w[1] +
+             (d[0] * T[3 * j + 1] - d[1] * T[3 * j]) * w[2]) * wt;
+  }
+}
+
+/* Force F and torque N about rc on the objects (force_FN, d3qd2_src/d3qd2_force.c:10-53):
+ * Maxwell stress integrated over domain 0's boundary.  type 0: the stored node fields
+ * plus the incident wave at the element nodes (a triangle's centroid point needs a boundary
+ * query); type 1: boundary queries at the 9 (quad) or 7 (triangle) point rule.  Every
+ * boundary query is evaluated in one launch with the node rule (PREC_DEF_FN) for the other
+ * elements.  Element sums are added in element order; returns -1 if domain 0 is empty. */
+int force_FN(double *F, double *N, double *rc, int type, DQD2 *qd)
+{
+  BOUD *bd = &qd->bd;
+  SUBD *s0 = &bd->sb[0];
+  int ne = s0->Ne;
+  for (int j = 0; j < 3; j++) F[j] = N[j] = 0.0;
+  if (ne <= 0) return -1;
+  /* the queries */
+  int nq = 0;
+  for (int t = 1; t <= ne; t++) {
+    int tri = bd->ed[abs(s0->sid[t])][3] == 0;
+    nq += type == 0 ? (tri ? 1 : 0) : (tri ? 7 : 9);
+  }
+  size_t qn = nq > 0 ? nq : 1;
+  int *qd_ = malloc(sizeof(int) * qn), *qt = malloc(sizeof(int) * qn);
+  double *qz = malloc(sizeof(double) * qn), *qe = malloc(sizeof(double) * qn);
+  double complex *qE = malloc(sizeof(double complex) * 3 * qn), *qH = malloc(sizeof(double complex) * 3 * qn);
+  for (int t = 1, q = 0; t <= ne; t++) {
+    int tri = bd->ed[abs(s0->sid[t])][3] == 0;
+    if (type == 0) {
+      if (!tri) continue;
+      qd_[q] = 0; qt[q] = t; qz[q] = bd->zt_34[3]; qe[q] = bd->et_34[3]; q++;
+    } else {
+      int np = tri ? 7 : 9;
+      for (int i = 0; i < np; i++, q++) {
+        qd_[q] = 0;
+        qt[q] = t;
+        qz[q] = tri ? bd->zt_37[i] : bd->zt_49[i];
+        qe[q] = tri ? bd->et_37[i] : bd->et_49[i];
+      }
+    }
+  }
+  if (nq > 0 && bd_fields(qd, nq, qd_, qt, qz, qe, PREC_DEF_FN, 1, qE, qH)) {
+    printf("force_FN(): invalid boundary query. Exit...\n");
+    exit(1);
+  }
+  double epd = creal(qd->n[0]) * creal(qd->n[0]);
+  for (int t = 1, q = 0; t <= ne; t++) {
+    int as = abs(s0->sid[t]), tri = bd->ed[as][3] == 0;
+    int np = type == 0 ? 4 : (tri ? 7 : 9);
+    double f[3] = {0.0, 0.0, 0.0}, nn[3] = {0.0, 0.0, 0.0};
+    for (int i = 0; i < np; i++) {
+      double r[3], w[3], wt;
+      double complex e[3], h[3];
+      if (type == 0 && !(tri && i == 3)) {
+        double complex te[3], th[3];
+        for (int j = 0; j < 3; j++) {
+          r[j] = bd->ren[as][i][j];
+          w[j] = bd->wen[as][i][j];
+        }
+        calc_mipw_EH(te, th, r, &qd->pw);
+        for (int j = 0; j < 3; j++) {
+          e[j] = s0->E[t][i][j] + te[j];
+          h[j] = s0->H[t][i][j] + th[j];
+        }
+        wt = tri ? bd->wt_34[i] : bd->wt_44[i];
+      } else {
+        double zt = qz[q], et = qe[q];
+        wt = type == 0 ? bd->wt_34[3] : (tri ? bd->wt_37[i] : bd->wt_49[i]);
+        r_on(bd, as, zt, et, r);
+        for (int j = 0; j < 3; j++) {
+          w[j] = bd->cw[as][j][0] + bd->cw[as][j][1] * zt + bd->cw[as][j][2] * et;
+          e[j] = qE[3 * q + j];
+          h[j] = qH[3 * q + j];
+        }
+        q++;
+      }
+      stress_add(epd, e, h, r, w, rc, wt, f, nn);
+    }
+    double sc = tri ? -0.5 : -1.0;
+    for (int j = 0; j < 3; j++) {
+      F[j] += sc * f[j];
+      N[j] += sc * nn[j];
+    }
+  }
+  free(qd_); free(qt); free(qz); free(qe); free(qE); free(qH);
+  return 0;
 }
 
 static void not_yet(const char *name)
@@ -267,13 +498,7 @@ static void not_yet(const char *name)
   exit(1);
 }
 
-int mEMP_s(double complex *U, double *rt, int type, DQD2 *qd) { (void)U; (void)rt; (void)type; (void)qd; not_yet("mEMP_s"); return -1; }
-int mEMP_t(double complex *U, double *rt, int type, DQD2 *qd) { (void)U; (void)rt; (void)type; (void)qd; not_yet("mEMP_t"); return -1; }
 int EH_mEMP_s(double complex *E, double complex *H, double *rt, int type, DQD2 *qd)
 { (void)E; (void)H; (void)rt; (void)type; (void)qd; not_yet("EH_mEMP_s"); return -1; }
 int EH_mEMP_t(double complex *E, double complex *H, double *rt, int type, DQD2 *qd)
 { (void)E; (void)H; (void)rt; (void)type; (void)qd; not_yet("EH_mEMP_t"); return -1; }
-void EH_s_bd(double complex *E, double complex *H, int did, int t, double zeta_t, double eta_t, int type, DQD2 *qd)
-{ (void)E; (void)H; (void)did; (void)t; (void)zeta_t; (void)eta_t; (void)type; (void)qd; not_yet("EH_s_bd"); }
-void EH_t_bd(double complex *E, double complex *H, int did, int t, double zeta_t, double eta_t, int type, DQD2 *qd)
-{ (void)E; (void)H; (void)did; (void)t; (void)zeta_t; (void)eta_t; (void)type; (void)qd; not_yet("EH_t_bd"); }

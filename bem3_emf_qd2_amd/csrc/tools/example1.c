@@ -1,5 +1,5 @@
-/* example1 -- total field at one point from an ex.dat result, 4-point and 9-point rules
- * (the field part of the reference's example1.c). */
+/* example1 -- total field at one point from an ex.dat result and the radiation force and
+ * torque, 4-point and 9-point rules (the output of the reference's example1.c). */
 #include "bem3_emf_qd2.h"
 
 static void show(double complex *E, double complex *H)
@@ -28,6 +28,15 @@ int main(int argc, char **argv)
   EH_t(E, H, r, 1, &qd);
   printf("type=%d setting ( 9 or 7 point Gauss-Legendre )\n", 1);
   show(E, H);
+  double F[3], N[3], rc[3] = {0.0, 0.0, 0.0};
+  printf("\nRadiation force and torque\n");
+  for (int type = 0; type < 2; type++) {
+    force_FN(F, N, rc, type, &qd);
+    printf("type=%d setting ( %s point Gauss-Legendre ) \n", type, type == 0 ? "4" : "9 or 7");
+    printf("F = (% 15.14e,% 15.14e,% 15.14e)\n", F[0], F[1], F[2]);
+    printf("N = (% 15.14e,% 15.14e,% 15.14e), center of rotation (% g,% g,% g)\n", N[0], N[1], N[2], rc[0], rc[1],
+           rc[2]);
+  }
   finalize_dqd2(&qd);
   return 0;
 }

@@ -34,6 +34,12 @@ def L():
         lb.read_dqd2.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_char_p), ctypes.c_void_p]
         lb.output_node_particles.argtypes = [ctypes.c_char_p, ctypes.c_void_p]
         lb.bq_EH_t_batch.argtypes = [ctypes.c_void_p, ctypes.c_int, _D, ctypes.c_int, _D, _D, _I]
+        lb.bq_EH_t_bd_batch.argtypes = [ctypes.c_void_p, ctypes.c_int, _I, _I, _D, _D, ctypes.c_int, _D, _D]
+        lb.force_FN.argtypes = [_D, _D, _D, ctypes.c_int, ctypes.c_void_p]
+        lb.mEMP_s.argtypes = [_D, _D, ctypes.c_int, ctypes.c_void_p]
+        lb.mEMP_t.argtypes = [_D, _D, ctypes.c_int, ctypes.c_void_p]
+        lb.EH_s_bd.argtypes = [_D, _D, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                               ctypes.c_void_p]
         lb.bq_assemble_domain_host.argtypes = [ctypes.c_void_p, ctypes.c_int, _D, _D, _D, _D, _D]
         lb.bq_solve_last_stats.restype = ctypes.POINTER(Stats)
         lb._solver_declared = True
@@ -87,6 +93,46 @@ class Model:
         L().bq_EH_t_batch(self.ptr, n, r.ctypes.data_as(_D), type_, E.ctypes.data_as(_D), H.ctypes.data_as(_D),
                           did.ctypes.data_as(_I))
         return E, H, did
+
+    def fields_bd(self, did, t, zt, et, type_=1):
+        """Total E, H at boundary points (zt[p], et[p]) of the t[p]-th element of domain did[p]."""
+        did = np.ascontiguousarray(np.atleast_1d(did), np.int32)
+        t = np.ascontiguousarray(np.atleast_1d(t), np.int32)
+        zt = np.ascontiguousarray(np.atleast_1d(zt), float)
+        et = np.ascontiguousarray(np.atleast_1d(et), float)
+        n = len(t)
+        E = np.zeros((n, 3), complex)
+        H = np.zeros((n, 3), complex)
+        rc = L().bq_EH_t_bd_batch(self.ptr, n, did.ctypes.data_as(_I), t.ctypes.data_as(_I), zt.ctypes.data_as(_D),
+                                  et.ctypes.data_as(_D), type_, E.ctypes.data_as(_D), H.ctypes.data_as(_D))
+        if rc:
+            raise ValueError("invalid boundary query")
+        return E, H
+
+    def fields_s_bd(self, did, t, zt, et, type_=1):
+        """Scattered (or interior) E, H at one boundary point through EH_s_bd."""
+        E = np.zeros(3, complex)
+        H = np.zeros(3, complex)
+        L().EH_s_bd(E.ctypes.data_as(_D), H.ctypes.data_as(_D), did, t, zt, et, type_, self.ptr)
+        return E, H
+
+    def potential(self, r, type_=1, total=True):
+        """Modified electromagnetic potential U = (A, phi) at one point (mEMP_t / mEMP_s)."""
+        r = np.ascontiguousarray(np.asarray(r, float).reshape(3))
+        U = np.zeros(4, complex)
+        f = L().mEMP_t if total else L().mEMP_s
+        did = f(U.ctypes.data_as(_D), r.ctypes.data_as(_D), type_, self.ptr)
+        return U, did
+
+    def force(self, rc=(0.0, 0.0, 0.0), type_=0):
+        """Force and torque about rc on the objects (force_FN)."""
+        F = np.zeros(3)
+        N = np.zeros(3)
+        c = np.ascontiguousarray(rc, float)
+        rcode = L().force_FN(F.ctypes.data_as(_D), N.ctypes.data_as(_D), c.ctypes.data_as(_D), type_, self.ptr)
+        if rcode:
+            raise RuntimeError("force_FN: domain 0 has no element")
+        return F, N
 
     def assemble(self, d, dNe, deriv=True):
         N = 4 * dNe

@@ -145,6 +145,11 @@ void EH_s_bd(double complex *E, double complex *H, int did, int t, double zeta_t
 void EH_t_bd(double complex *E, double complex *H, int did, int t, double zeta_t, double eta_t, int type, DQD2 *qd);
 void EH_i_bd(double complex *E, double complex *H, int did, int t, double zeta_t, double eta_t, int type, DQD2 *qd);
 
+/* -- d3qd2_force.c (reference bem3_emf_qd2.h:75-80): force F and torque N about rc on the
+ * objects from the Maxwell stress on domain 0's boundary; type 0: 4-point, 1: 9-/7-point
+ * rule; returns -1 if domain 0 has no element -- */
+int force_FN(double *F, double *N, double *rc, int type, DQD2 *qd);
+
 /* -- incident field helpers (d3qd2_src/m_ipw.h:30-38) -- */
 void read_data_mipw(char *fname, MIPW *obj);
 void print_data_mipw(MIPW *obj);

@@ -42,6 +42,13 @@ int bq_sizeof_dqd2(void);
  * E, H: [n][3] complex out; did[n] out (domain id, -1 if none).  type 0..3 as EH_t. */
 int bq_EH_t_batch(void *qd, int n, const double *r, int type, double *E, double *H, int *did);
 
+/* Total fields at n boundary points in one GPU launch (batched EH_t_bd of
+ * d3qd2_src/d3qd2_field.c:343-391): point p is (zt[p], et[p]) on the t[p]-th element
+ * (1-based) of domain did[p].  E, H: [n][3] complex out.  Returns -1 without evaluating if a
+ * query names no element or type is outside 0..3. */
+int bq_EH_t_bd_batch(void *qd, int n, const int *did, const int *t, const double *zt, const double *et, int type,
+                     double *E, double *H);
+
 /* Device assembly of one domain's coefficient matrices for a set-up model (DQD2* after
  * initialize_dqd2 or dat_read), copied to host buffers (NULL to skip): G, H [N][N],
  * dG, dH [2N][N] complex (rows 2 row + {zeta, eta}), tdF [N][3]; N = 4 x elements of the

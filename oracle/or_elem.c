@@ -22,6 +22,9 @@
  *                                        lit_dcoef_bd_tz0_PV/_te0_PV d3qd2_src/lit_dcoef.c:461-637,
  *                                        integrands :845-952
  *   dispatch + GL->GH fallback           d3qd2_src/d3qd2_elem.c:515-572, 830-886
+ *   boundary-point coefficients          q0_coef_bd / coef_bd d3qd2_src/d3qd2_elem.c:684-770, :187-230:
+ *                                        q0_bil_coef_bd q0_bil_coef.c:246-338, q0_lit_coef_bd
+ *                                        q0_lit_coef.c:297-371, bil_coef_bd, lit_coef_bd
  *   field-point coefficients             q0_bil_coef_{4p,9p,GL,GH} d3qd2_src/q0_bil_coef.c:10-198,
  *                                        bil_coef_{4p,9p,GL,GH} d3qd2_src/bil_coef.c:25-202,
  *                                        q0_lit_coef_{4p,7p,GL,GH} d3qd2_src/q0_lit_coef.c:10-242,
@@ -674,4 +677,49 @@ void or_coef_rt(const or_model *m, int did, const double *rt, int ss, int type, 
   }
   CC[8] = F * I_FP;
   if (!periodic && ss < 0) flip_sign(CC);
+}
+
+/* ---- boundary-point coefficients: (zt, et) on element t, source s ---------------------- */
+void or_coef_bd(const or_model *m, int did, int t, double zt, double et, int ss, int type, double complex *CC)
+{
+  int s = abs(ss), at = abs(t);
+  elc e;
+  get_el(m, at, &e);
+  if (s != at) {
+    double rt[3], W[3];
+    rw_at(&e, zt, et, rt, W);
+    or_coef_rt(m, did, rt, ss, type, CC);
+    return;
+  }
+  pctx p = {0};
+  p.m = m;
+  p.e = e;
+  p.zt = zt;
+  p.et = et;
+  for (int i = 0; i < 9; i++) CC[i] = 0.0;
+  if (did == 0 && m->ps == 1) {
+    polar_visit(m, e.tri, zt, et, 0.0, GLN, m->xli, m->wli, q0_sing_fn, &p);
+    for (int n = 0; n < n_shape(&e); n++) { CC[n] = p.G[n]; CC[n + 4] = p.H[n]; }
+    if (!e.tri) CC[8] = polar_F(m, s, zt, et);
+    return;
+  }
+  p.k = m->kn[did];
+  polar_visit(m, e.tri, zt, et, 0.0, GLN, m->xli, m->wli, fs_sing_GHF_fn, &p);
+  int plane = is_plane(&e);
+  double bdC = bdC_of(&e);
+  for (int n = 0; n < n_shape(&e); n++) {
+    CC[n] = p.G[n] * I_FP;
+    CC[n + 4] = plane ? 0.0 : p.H[n] * I_FP * bdC;
+  }
+  CC[8] = plane ? 0.0 : p.F * I_FP * bdC;
+  if (ss < 0) flip_sign(CC);
+}
+
+/* position on element s at (zt, et) (r_bd, d3qd2_src/d3qd2_elem.c:18-29) */
+void or_r_bd(const or_model *m, int s, double zt, double et, double *r)
+{
+  elc e;
+  double W[3];
+  get_el(m, abs(s), &e);
+  rw_at(&e, zt, et, r, W);
 }

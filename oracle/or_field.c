@@ -4,8 +4,8 @@
  * TEST INFRASTRUCTURE ONLY (see oracle.h).
  *
  * Follows d3qd2_src/d3qd2_setup.c (domain_id :157-202, q0_domain_id_l :238-245,
- * fid_calc_solid_angle :1351-1422) and d3qd2_src/d3qd2_field.c (EH_s :203-269,
- * EH_t :271-285).
+ * fid_calc_solid_angle :1351-1422), d3qd2_src/d3qd2_field.c (mEMP_s/t :10-73, EH_s :203-269,
+ * EH_t :271-285, EH_s_bd/EH_t_bd :299-391) and d3qd2_src/d3qd2_force.c (force_FN).
  */
 #include <math.h>
 #include <complex.h>
@@ -143,4 +143,143 @@ int or_EH_t(const or_model *m, double complex **E, double complex **dE, double c
     for (int i = 0; i < 3; i++) { Eo[i] += e[i]; Ho[i] += h[i]; }
   }
   return did;
+}
+
+/* modified electromagnetic potential U = (A, phi) at r (mEMP_s / mEMP_t,
+ * d3qd2_src/d3qd2_field.c:10-73): the representation sum of EH_s over the stored U, dU
+ * ([Ne+1][4][4] per domain); total adds the incident E to A in domain 0 */
+int or_mEMP(const or_model *m, double complex **U, double complex **dU, const double *r, int type, int total,
+            double complex *Uo)
+{
+  int l1 = 0, l2 = 0;
+  double r0[3] = {r[0], r[1], r[2]};
+  if (m->ps == 1) {
+    const or_qpd2 *q = &m->qd;
+    double idd = 1.0 / (q->vd1[0] * q->vd2[1] - q->vd2[0] * q->vd1[1]);
+    l1 = (int)floor((r[0] * q->vd2[1] - r[1] * q->vd2[0]) * idd + 0.5);
+    l2 = (int)floor(-(r[0] * q->vd1[1] - r[1] * q->vd1[0]) * idd + 0.5);
+    r0[0] = r[0] - l1 * q->vd1[0] - l2 * q->vd2[0];
+    r0[1] = r[1] - l1 * q->vd1[1] - l2 * q->vd2[1];
+  }
+  int did = or_domain_id(m, r0);
+  for (int i = 0; i < 4; i++) Uo[i] = 0.0;
+  if (did < 0) return did;
+  double F = 0;
+  for (int s = 1; s <= m->dNe[did]; s++) {
+    double complex CC[9];
+    or_coef_rt(m, did, r0, m->sid[did][s], type, CC);
+    for (int n = 0; n < 4; n++)
+      for (int i = 0; i < 4; i++) Uo[i] += CC[n] * dU[did][16 * s + 4 * n + i] - CC[n + 4] * U[did][16 * s + 4 * n + i];
+    F += creal(CC[8]);
+  }
+  double C = (did == 0) ? 1.0 + F : F;
+  for (int i = 0; i < 4; i++) Uo[i] /= C;
+  if (l1 != 0 || l2 != 0) {
+    const or_qpd2 *q = &m->qd;
+    double arg = q->vk[0] * (l1 * q->vd1[0] + l2 * q->vd2[0]) + q->vk[1] * (l1 * q->vd1[1] + l2 * q->vd2[1]);
+    double complex ce = cos(arg) + I * sin(arg);
+    for (int i = 0; i < 4; i++) Uo[i] *= ce;
+  }
+  if (total && did == 0) {
+    double complex e[3], h[3];
+    or_ipw_EH(m, r, e, h);
+    for (int i = 0; i < 3; i++) Uo[i] += e[i];
+  }
+  return did;
+}
+
+void or_EH_bd(const or_model *m, double complex **E, double complex **dE, double complex **H,
+              double complex **dH, int did, int t, double zt, double et, int type, int total,
+              double complex *Eo, double complex *Ho)
+{
+  int td = m->sid[did][t];
+  double F = 0.0;
+  for (int i = 0; i < 3; i++) Eo[i] = Ho[i] = 0.0;
+  for (int s = 1; s <= m->dNe[did]; s++) {
+    double complex CC[9];
+    or_coef_bd(m, did, td, zt, et, m->sid[did][s], type, CC);
+    for (int n = 0; n < 4; n++)
+      for (int i = 0; i < 3; i++) {
+        Eo[i] += CC[n] * dE[did][12 * s + 3 * n + i] - CC[n + 4] * E[did][12 * s + 3 * n + i];
+        Ho[i] += CC[n] * dH[did][12 * s + 3 * n + i] - CC[n + 4] * H[did][12 * s + 3 * n + i];
+      }
+    F += creal(CC[8]);
+  }
+  double C = (did == 0) ? 1.0 + F : F;
+  for (int i = 0; i < 3; i++) { Eo[i] /= C; Ho[i] /= C; }
+  if (total && did == 0) {
+    double r[3];
+    double complex e[3], h[3];
+    or_r_bd(m, td, zt, et, r);
+    or_ipw_EH(m, r, e, h);
+    for (int i = 0; i < 3; i++) { Eo[i] += e[i]; Ho[i] += h[i]; }
+  }
+}
+
+/* Maxwell stress T = 1/2 Re(eps E E* + H H*) - 1/4 (eps |E|^2 + |H|^2) I contracted with the
+ * surface element w and its moment about rc, eps = Re(n0)^2 (d3qd2_force.c:60-98) */
+static void stress_add(const or_model *m, const double complex *e, const double complex *h, const double *r,
+                       const double *w, const double *rc, double wt, double *F, double *N)
+{
+  double epd = creal(m->n[0]) * creal(m->n[0]);
+  double aE2 = 0, aH2 = 0, T[9];
+  for (int j = 0; j < 3; j++) { aE2 += creal(e[j] * conj(e[j])); aH2 += creal(h[j] * conj(h[j])); }
+  double Tc = 0.25 * (epd * aE2 + aH2);
+  for (int j = 0; j < 3; j++)
+    for (int k = 0; k < 3; k++) T[3 * j + k] = 0.5 * (epd * creal(e[j] * conj(e[k])) + creal(h[j] * conj(h[k])));
+  T[0] -= Tc; T[4] -= Tc; T[8] -= Tc;
+  double d[3] = {r[0] - rc[0], r[1] - rc[1], r[2] - rc[2]};
+  for (int j = 0; j < 3; j++) {
+    F[j] += (T[3 * j] * w[0] + T[3 * j + 1] * w[1] + T[3 * j + 2] * w[2]) * wt;
+    N[j] += ((d[1] * T[3 * j + 2] - d[2] * T[3 * j + 1]) * w[0] + (d[2] * T[3 * j] - d[0] * T[3 * j + 2]) * w[1] +
+             (d[0] * T[3 * j + 1] - d[1] * T[3 * j]) * w[2]) * wt;
+  }
+}
+
+int or_force_FN(const or_model *m, double complex **E, double complex **dE, double complex **H,
+                double complex **dH, const double *rc, int type, double *F, double *N)
+{
+  for (int j = 0; j < 3; j++) F[j] = N[j] = 0.0;
+  if (m->dNe[0] == 0) return -1;
+  int ne = m->dNe[0];
+  double *part = calloc((size_t)6 * (ne + 1), sizeof(double));
+#pragma omp parallel for schedule(dynamic)
+  for (int t = 1; t <= ne; t++) {
+    int at = abs(m->sid[0][t]), tri = (m->ed[4 * at + 3] == 0);
+    double f[3] = {0, 0, 0}, nn[3] = {0, 0, 0};
+    int np = type == 0 ? 4 : (tri ? 7 : 9);
+    for (int i = 0; i < np; i++) {
+      double r[3], w[3], wt, zt, et;
+      double complex e[3], h[3];
+      const double *cw = m->cw + 9 * at;
+      if (type == 0 && !(tri && i == 3)) {
+        /* stored node fields plus the incident wave */
+        for (int j = 0; j < 3; j++) {
+          r[j] = m->ren[12 * at + 3 * i + j];
+          w[j] = m->wen[12 * at + 3 * i + j];
+          e[j] = E[0][12 * t + 3 * i + j];
+          h[j] = H[0][12 * t + 3 * i + j];
+        }
+        double complex te[3], th[3];
+        or_ipw_EH(m, r, te, th);
+        for (int j = 0; j < 3; j++) { e[j] += te[j]; h[j] += th[j]; }
+        wt = tri ? m->wt34[i] : m->wt44[i];
+      } else {
+        zt = type == 0 ? m->zt34[i] : (tri ? m->zt37[i] : m->zt49[i]);
+        et = type == 0 ? m->et34[i] : (tri ? m->et37[i] : m->et49[i]);
+        wt = type == 0 ? m->wt34[i] : (tri ? m->wt37[i] : m->wt49[i]);
+        or_r_bd(m, at, zt, et, r);
+        for (int j = 0; j < 3; j++) w[j] = cw[3 * j] + cw[3 * j + 1] * zt + cw[3 * j + 2] * et;
+        /* PREC_DEF_FN = 0: node rule for the other elements (bem3_emf_qd2.h:14) */
+        or_EH_bd(m, E, dE, H, dH, 0, t, zt, et, 0, 1, e, h);
+      }
+      stress_add(m, e, h, r, w, rc, wt, f, nn);
+    }
+    double sc = tri ? -0.5 : -1.0;
+    for (int j = 0; j < 3; j++) { part[6 * t + j] = sc * f[j]; part[6 * t + 3 + j] = sc * nn[j]; }
+  }
+  for (int t = 1; t <= ne; t++)
+    for (int j = 0; j < 3; j++) { F[j] += part[6 * t + j]; N[j] += part[6 * t + 3 + j]; }
+  free(part);
+  return 0;
 }

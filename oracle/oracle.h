@@ -94,6 +94,12 @@ void or_node_tangents(const or_model *m, int s, int nid, double *vtz, double *vt
 /* Coefficients for an arbitrary field point (reference q0_coef_rt / coef_rt). */
 void or_coef_rt(const or_model *m, int did, const double *rt, int s, int type, double complex *CC);
 
+/* Coefficients for the point (zt, et) of element t (signed ids as in sid lists): the polar
+ * singular rule when s is t, else the rule `type` at that point (reference q0_coef_bd /
+ * coef_bd, d3qd2_src/d3qd2_elem.c:684-770, :187-230). */
+void or_coef_bd(const or_model *m, int did, int t, double zt, double et, int s, int type, double complex *CC);
+void or_r_bd(const or_model *m, int s, double zt, double et, double *r);
+
 /* ---- or_system.c ------------------------------------------------------------------ */
 /* Assemble rows [t0, t1] (1-based element indices of domain did, inclusive) of the domain
  * matrices; N = 4*dNe.  tG,tH: [N][N]; tdG,tdH: [N][2][N]; tdF: [N][3]
@@ -120,5 +126,19 @@ void or_deh_rhs(const or_model *m, int d, const double complex *tH, double compl
 int or_domain_id(const or_model *m, const double *rt);
 int or_EH_t(const or_model *m, double complex **E, double complex **dE, double complex **H,
             double complex **dH, const double *r, int type, double complex *Eo, double complex *Ho);
+/* Modified electromagnetic potential (A, phi) at r from the stored U, dU ([Ne+1][4][4] complex
+ * per domain); total = 1 adds the incident E to A in the open region (mEMP_s / mEMP_t,
+ * d3qd2_src/d3qd2_field.c:10-73).  Returns the domain id. */
+int or_mEMP(const or_model *m, double complex **U, double complex **dU, const double *r, int type, int total,
+            double complex *Uo);
+/* Field at the point (zt, et) of element t (1-based position in domain did's list); total = 1
+ * adds the incident field in the open region (EH_s_bd / EH_t_bd, d3qd2_src/d3qd2_field.c:299-403). */
+void or_EH_bd(const or_model *m, double complex **E, double complex **dE, double complex **H,
+              double complex **dH, int did, int t, double zt, double et, int type, int total,
+              double complex *Eo, double complex *Ho);
+/* Radiation force and torque about rc on the objects in the open region from the Maxwell
+ * stress tensor (force_FN, d3qd2_src/d3qd2_force.c:10-240). */
+int or_force_FN(const or_model *m, double complex **E, double complex **dE, double complex **H,
+                double complex **dH, const double *rc, int type, double *F, double *N);
 
 #endif

@@ -137,3 +137,24 @@ class OracleModel:
         did = self.o.or_EH_t(self.h, P(sol["E"]), P(sol["dE"]), P(sol["H"]), P(sol["dH"]), ptr(rr), type_,
                              ptr(E.view(float)), ptr(H.view(float)))
         return did, E, H
+
+    def mEMP(self, sol, r, type_=1, total=True):
+        """(A, phi) at r from sol["U"], sol["dU"] ([Ne+1][4][4] complex per domain)."""
+        P = lambda L: (D * len(L))(*[ptr(x.view(float)) for x in L])
+        U = np.zeros(4, complex)
+        rr = np.ascontiguousarray(r, float)
+        did = self.o.or_mEMP(self.h, P(sol["U"]), P(sol["dU"]), ptr(rr), type_, int(total), ptr(U.view(float)))
+        return did, U
+
+    def EH_bd(self, sol, did, t, zt, et, type_=0, total=True):
+        P = lambda L: (D * len(L))(*[ptr(x.view(float)) for x in L])
+        E = np.zeros(3, complex); H = np.zeros(3, complex)
+        self.o.or_EH_bd(self.h, P(sol["E"]), P(sol["dE"]), P(sol["H"]), P(sol["dH"]), did, t, zt, et, type_,
+                        int(total), ptr(E.view(float)), ptr(H.view(float)))
+        return E, H
+
+    def force_FN(self, sol, rc=(0.0, 0.0, 0.0), type_=0):
+        P = lambda L: (D * len(L))(*[ptr(x.view(float)) for x in L])
+        F = np.zeros(3); N = np.zeros(3); r = np.ascontiguousarray(rc, float)
+        rc_ = self.o.or_force_FN(self.h, P(sol["E"]), P(sol["dE"]), P(sol["H"]), P(sol["dH"]), ptr(r), type_, ptr(F), ptr(N))
+        return rc_, F, N
