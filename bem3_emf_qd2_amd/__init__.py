@@ -77,6 +77,18 @@ def qpgf_v1(qd8, r, converged=False) -> np.ndarray:
     return out[:, 0::2] + 1j * out[:, 1::2]
 
 
+def qpgf_v2(qd8, r) -> np.ndarray:
+    """d3hm_qpgf_d2_v2 on the GPU: shape (n, 10) complex = qG, dqG[3], d2qG[6] (xx, yy, zz, xy,
+    yz, zx)."""
+    q = _as(qd8)
+    assert q.shape == (8,)
+    rr = _as(np.asarray(r).reshape(-1, 3))
+    out = np.empty((len(rr), 20))
+    _check(lib().bq_qpgf_batch(q.ctypes.data_as(_D), 2, len(rr), rr.ctypes.data_as(_D), out.ctypes.data_as(_D)),
+           "qpgf_v2")
+    return out[:, 0::2] + 1j * out[:, 1::2]
+
+
 def zgesv(A, B):
     """Solve A X = B on the GPU with the library's LU (complex double); returns X."""
     A = np.asarray(A, complex)

@@ -37,9 +37,10 @@ typedef struct bq_fdata {
 
 int bq_field_ctx_create(const bq_fdata *f, void **handle);
 /* kind 0: out[n][13] = sum E (3 complex), sum H (3 complex), F; kind 1: out[n][9] = sum U
- * (4 complex), F; for folded points pts[n][3] with known domain ids, or, when bd_t is not
- * NULL, for the boundary points (bd_ze[2p], bd_ze[2p+1]) of elements bd_t[p] (1-based
- * positions in the domain's list; pts unused). */
+ * (4 complex), F; kind 2: out[n][36] = sum U, sum dU/dr_j (3 x 4 complex), F, dF/dr_j; for
+ * folded points pts[n][3] with known domain ids, or, when bd_t is not NULL (kinds 0 and 1),
+ * for the boundary points (bd_ze[2p], bd_ze[2p+1]) of elements bd_t[p] (1-based positions
+ * in the domain's list; pts unused). */
 int bq_field_eval(void *handle, int kind, int n, const double *pts, const int *did, const int *bd_t,
                   const double *bd_ze, int type, double *out);
 void bq_field_ctx_destroy(void *handle);

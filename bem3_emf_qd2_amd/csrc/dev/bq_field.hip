@@ -48,6 +48,33 @@ __device__ __forceinline__ void rule_point(int type, bool tri, int q, double& z,
   }
 }
 
+// Point q of rule `type` on element e (id as): position R, normal W, interpolation weights Nn
+// and rule weight w.  The node rule reads the stored node data (d3qd2_src/q0_bil_coef.c:28-30,
+// q0_lit_coef.c:29-31); a triangle's normal is constant and taken from the element, and its
+// centroid point carries 1/3 of each function.
+__device__ __forceinline__ void field_rule_point(const bq_fctx& C, const Elem& e, int as, int type, int q,
+                                                 double (&R)[3], double (&W)[3], double (&Nn)[4], double& w)
+{
+  double z, et;
+  rule_point(type, e.tri, q, z, et, w);
+  if (type == 0) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      R[d] = C.ren[12L * as + 3 * q + d];
+      W[d] = e.tri ? e.cw[d][0] : C.wen[12L * as + 3 * q + d];
+    }
+    if (e.tri && q == 3) {
+      shapeN3(z, et, Nn);
+    } else {
+#pragma unroll
+      for (int n = 0; n < 4; ++n) Nn[n] = (n == q) ? 1.0 : 0.0;
+    }
+  } else {
+    rw_at(e, z, et, R, W);
+    shapeN(e.tri, z, et, Nn);
+  }
+}
+
 // Node data of domain did: KIND 0 -> E, dE, H, dH ([(dNe+1)][4][3] complex), 13 outputs per
 // point (E, H, F); KIND 1 -> U, dU ([(dNe+1)][4][4] complex), 9 outputs (U, F).
 template <int KIND>
@@ -131,26 +158,8 @@ __global__ void __launch_bounds__(256) k_field(bq_fctx C, int npts, const double
     Elem e;
     load_elem(C.E, as, e);
     if (q >= rule_size(type, e.tri)) continue;
-    double z, et, w, R[3], W[3], Nn[4];
-    rule_point(type, e.tri, q, z, et, w);
-    if (type == 0) {
-      // the node rule reads the stored node data (d3qd2_src/q0_bil_coef.c:28-30,
-      // q0_lit_coef.c:29-31); a triangle's normal is constant and taken from the element
-#pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        R[d] = C.ren[12L * as + 3 * q + d];
-        W[d] = e.tri ? e.cw[d][0] : C.wen[12L * as + 3 * q + d];
-      }
-      if (e.tri && q == 3) {
-        shapeN3(z, et, Nn);  // the centroid carries 1/3 of each function
-      } else {
-#pragma unroll
-        for (int n = 0; n < 4; ++n) Nn[n] = (n == q) ? 1.0 : 0.0;
-      }
-    } else {
-      rw_at(e, z, et, R, W);
-      shapeN(e.tri, z, et, Nn);
-    }
+    double w, R[3], W[3], Nn[4];
+    field_rule_point(C, e, as, type, q, R, W, Nn, w);
     R[0] -= r0[0]; R[1] -= r0[1]; R[2] -= r0[2];
     double aR = sqrt(dot3(R, R)), iR = 1.0 / aR, aW = sqrt(dot3(W, W));
     double tD = dot3(R, W) * iR * iR * iR;
@@ -242,6 +251,98 @@ __global__ void __launch_bounds__(256) k_field(bq_fctx C, int npts, const double
   }
 }
 
+
+// E, H from the potentials (EH_mEMP_s, d3qd2_src/d3qd2_field.c:90-173): one block per point,
+// out[p][36] = U (4 complex), dU/dr_j (3 x 4 complex), F, dF/dr_j.  Coefficients and their
+// field-point gradients as q0_dcoef_rt_grad / dcoef_rt_grad (d3qd2_src/d3qd2_elem.c:316-349,
+// :650-683): the periodic kernel takes the Hessian of the lattice sum (qpgf_v2).
+__global__ void __launch_bounds__(256) k_field_grad(bq_fctx C, int npts, const double* __restrict__ pts,
+                                                    const int* __restrict__ did_of, int type,
+                                                    double* __restrict__ out)
+{
+  constexpr int NV = 36;
+  __shared__ double lds[NV * 4];
+  const int p = blockIdx.x;
+  const int did = did_of[p];
+  const int dNe = C.dNe[did];
+  const int* sid = C.sid[did];
+  const bool periodic = (did == 0 && C.ps == 1);
+  const cplx k = mk(C.kn[2 * did], C.kn[2 * did + 1]), k2 = k * k;
+  const double r0[3] = {pts[3 * p], pts[3 * p + 1], pts[3 * p + 2]};
+  const double *Ub = C.Ub[did], *dUb = C.dUb[did];
+  const int nq = C.has_tri ? max(rule_size(type, false), rule_size(type, true)) : rule_size(type, false);
+  double acc[NV];
+#pragma unroll
+  for (int i = 0; i < NV; ++i) acc[i] = 0.0;
+  const long total = (long)dNe * nq;
+  for (long it = threadIdx.x; it < total; it += 256) {
+    const int sc = (int)(it / nq), q = (int)(it - (long)sc * nq);
+    const int sd = sid[sc + 1], as = abs(sd);
+    Elem e;
+    load_elem(C.E, as, e);
+    if (q >= rule_size(type, e.tri)) continue;
+    double w, R[3], W[3], Nn[4];
+    field_rule_point(C, e, as, type, q, R, W, Nn, w);
+    R[0] -= r0[0]; R[1] -= r0[1]; R[2] -= r0[2];
+    const double aR = sqrt(dot3(R, R)), iR = 1.0 / aR, aW = sqrt(dot3(W, W));
+    const double rdW = dot3(R, W) * iR, tD = rdW * iR * iR;
+    cplx g, h, dg[3], dh[3];
+    double sgn = 1.0;
+    if (periodic) {
+      QG2 qq;
+      qpgf_v2(C.plan, R[0], R[1], R[2], qq);
+      g = qq.g * aW;
+      h = qq.d[0] * W[0] + qq.d[1] * W[1] + qq.d[2] * W[2];
+      // rows of the Hessian: (xx, xy, zx), (xy, yy, yz), (zx, yz, zz)
+      dg[0] = qq.d[0] * (-aW);
+      dg[1] = qq.d[1] * (-aW);
+      dg[2] = qq.d[2] * (-aW);
+      dh[0] = (qq.h[0] * W[0] + qq.h[3] * W[1] + qq.h[5] * W[2]) * -1.0;
+      dh[1] = (qq.h[3] * W[0] + qq.h[1] * W[1] + qq.h[4] * W[2]) * -1.0;
+      dh[2] = (qq.h[5] * W[0] + qq.h[4] * W[1] + qq.h[2] * W[2]) * -1.0;
+    } else {
+      const cplx ca = cmul_i(k * aR), ce = cexp(ca), cb = ca - mk(1.0, 0.0);
+      g = ce * (iR * aW * kI4PI);
+      h = cb * ce * (tD * kI4PI);
+      const cplx gg = cb * ce * (-kI4PI * iR * iR * iR * aW);
+      const cplx t1 = (k2 * (aR * aR) + 3.0 * cb) * ce * (kI4PI * iR * iR * iR * iR * rdW);
+      const cplx t2 = cb * ce * (kI4PI * iR * iR * iR);
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        dg[d] = gg * R[d];
+        dh[d] = t1 * R[d] - t2 * W[d];
+      }
+      if (sd < 0) sgn = -1.0;
+    }
+    const double fw = w * kI4PI * sgn;
+    acc[32] += fw * tD;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) acc[33 + d] += fw * iR * iR * iR * (3.0 * R[d] * iR * rdW - W[d]);
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+      const double wn = w * Nn[n], wh = wn * sgn;
+      const long o = 16L * (sc + 1) + 4 * n;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const cplx u = ld2(Ub, o + c), du = ld2(dUb, o + c);
+        const cplx uv = (g * du - h * sgn * u) * wn;
+        acc[2 * c] += uv.re;
+        acc[2 * c + 1] += uv.im;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          const cplx dv = dg[d] * du * wn - dh[d] * u * wh;
+          acc[8 + 8 * d + 2 * c] += dv.re;
+          acc[8 + 8 * d + 2 * c + 1] += dv.im;
+        }
+      }
+    }
+  }
+  block_sum256<NV>(acc, lds);
+  if (threadIdx.x == 0) {
+    double* o = out + (long)NV * p;
+    for (int i = 0; i < NV; ++i) o[i] = acc[i];
+  }
+}
 }  // namespace
 
 extern "C" int bq_field_rules_init(const double* z49, const double* e49, const double* w49, const double* w34,
@@ -262,7 +363,8 @@ extern "C" int bq_field_rules_init(const double* z49, const double* e49, const d
 }
 
 // out[p][13] = E (3 complex), H (3 complex), F for each folded point
-// kind 0: out[p][13] = E (3 complex), H (3 complex), F; kind 1: out[p][9] = U (4 complex), F.
+// kind 0: out[p][13] = E (3 complex), H (3 complex), F; kind 1: out[p][9] = U (4 complex), F;
+// kind 2: out[p][36] = U, dU/dr (3 x 4 complex), F, dF/dr (free points only).
 // d_bd_t / d_bd_ze (may be null): boundary queries instead of free points.
 extern "C" int bq_field_eval_dev(const bq_fctx* C, int kind, int npts, const double* d_pts, const int* d_did,
                                  const int* d_bd_t, const double* d_bd_ze, int type, double* d_out, void* stream)
@@ -271,8 +373,10 @@ extern "C" int bq_field_eval_dev(const bq_fctx* C, int kind, int npts, const dou
   hipStream_t s = (hipStream_t)stream;
   if (kind == 0)
     BQ_KTIME("k_field", s, k_field<0><<<npts, 256, 0, s>>>(*C, npts, d_pts, d_did, d_bd_t, d_bd_ze, type, d_out));
-  else
+  else if (kind == 1)
     BQ_KTIME("k_field_u", s, k_field<1><<<npts, 256, 0, s>>>(*C, npts, d_pts, d_did, d_bd_t, d_bd_ze, type, d_out));
+  else
+    BQ_KTIME("k_field_grad", s, k_field_grad<<<npts, 256, 0, s>>>(*C, npts, d_pts, d_did, type, d_out));
   BQ_HIP(hipGetLastError());
   return 0;
 }
@@ -356,7 +460,11 @@ extern "C" int bq_field_eval(void* handle, int kind, int n, const double* pts, c
   }
   FieldCtx* F = (FieldCtx*)handle;
   hipStream_t s = bq::stream();
-  const int nv = kind == 0 ? 13 : 9;
+  if (kind < 0 || kind > 2 || (kind == 2 && bd_t)) {
+    bq::set_error("field: kind %d not supported here", kind);
+    return -2;
+  }
+  const int nv = kind == 0 ? 13 : (kind == 1 ? 9 : 36);
   double *dp = nullptr, *dout = nullptr, *dze = nullptr;
   int *dd = nullptr, *dt = nullptr;
   BQ_HIP(hipMallocAsync((void**)&dd, sizeof(int) * n, s));

@@ -28,6 +28,12 @@ struct QG {
   cplx g;
   cplx d[3];
 };
+// value, gradient and Hessian (xx, yy, zz, xy, yz, zx), the _v2 output
+struct QG2 {
+  cplx g;
+  cplx d[3];
+  cplx h[6];
+};
 
 // row bounds of { t : |u + t e|^2 <= R2 } for integer t
 __device__ __forceinline__ bool row_range(double ux, double uy, double ex, double ey, double e2, double R2,
@@ -68,8 +74,11 @@ __device__ __forceinline__ int row_centre(double ue, double e2, int lo, int hi)
   return c < lo ? lo : (c > hi ? hi : c);
 }
 
-// Fourier-domain sum for the folded point (px, py, pz), pz != 0.
-__device__ void qpgf_fourier(const bq_qplan& P, double px, double py, double pz, QG& o)
+// Fourier-domain sum for the folded point (px, py, pz), pz != 0.  With HESS (Q = QG2) the
+// second derivatives are summed too: each order is exp(i a.p + i b |z|) / b, so d2/dxdy brings
+// -a_x a_y, d2/dz2 brings -b^2 and d2/dx dz brings i a_x (i b sgn z).
+template <bool HESS, class Q>
+__device__ void qpgf_fourier_t(const bq_qplan& P, double px, double py, double pz, Q& o)
 {
   double az = fabs(pz), bcut = BQ_EXP_CUT / az;
   double A2 = P.k2 + bcut * bcut;
@@ -82,6 +91,7 @@ __device__ void qpgf_fourier(const bq_qplan& P, double px, double py, double pz,
   const double gp1 = P.g1x * px + P.g1y * py, gp2 = P.g2x * px + P.g2y * py, ap = -(P.vk0 * px + P.vk1 * py);
   const cplx step = cexpi(gp2);
   cplx S = mk(0, 0), Sx = mk(0, 0), Sy = mk(0, 0), Sz = mk(0, 0);
+  cplx Hxx = mk(0, 0), Hyy = mk(0, 0), Hzz = mk(0, 0), Hxy = mk(0, 0), Hxz = mk(0, 0), Hyz = mk(0, 0);
   for (int n1 = n1lo; n1 <= n1hi; ++n1) {
     const double ux = -P.vk0 + n1 * P.g1x, uy = -P.vk1 + n1 * P.g1y;
     int lo, hi;
@@ -104,6 +114,14 @@ __device__ void qpgf_fourier(const bq_qplan& P, double px, double py, double pz,
       Sx += F * ax;
       Sy += F * ay;
       Sz += dz;
+      if constexpr (HESS) {
+        Hxx += F * (ax * ax);
+        Hyy += F * (ay * ay);
+        Hxy += F * (ax * ay);
+        Hzz += F * b2;
+        Hxz += dz * ax;
+        Hyz += dz * ay;
+      }
     });
   }
   double h = 0.5 * P.aidd, sg = (pz > 0.0) ? 1.0 : -1.0;
@@ -111,17 +129,32 @@ __device__ void qpgf_fourier(const bq_qplan& P, double px, double py, double pz,
   o.d[0] = Sx * (-h);
   o.d[1] = Sy * (-h);
   o.d[2] = Sz * (-h * sg);
+  if constexpr (HESS) {
+    o.h[0] = cmul_i(Hxx) * (-h);
+    o.h[1] = cmul_i(Hyy) * (-h);
+    o.h[2] = cmul_i(Hzz) * (-h);
+    o.h[3] = cmul_i(Hxy) * (-h);
+    o.h[4] = cmul_i(Hyz) * (-h * sg);
+    o.h[5] = cmul_i(Hxz) * (-h * sg);
+  }
+}
+__device__ __forceinline__ void qpgf_fourier(const bq_qplan& P, double px, double py, double pz, QG& o)
+{
+  qpgf_fourier_t<false>(P, px, py, pz, o);
 }
 
 // one Ewald spectral term for the reciprocal vector (ax, ay) with ce = exp(i a . p):
-// F (value) and Gz (z part)
+// F (value) and Gz (z part, i dF/dz)
+// and g0 = exp(b^2 v^2 - z^2 / 4v^2), which the second z derivative needs:
+// dGz/dz = -i b^2 F - 2 ce g0 / (v sqrt(pi)).
 __device__ __forceinline__ void spectral_term(const bq_qplan& P, double ax, double ay, cplx ce, double pz, double ve,
-                                              cplx& F, cplx& Gz)
+                                              cplx& F, cplx& Gz, double& g0)
 {
   double b2 = P.k2 - ax * ax - ay * ay;
   double az = 0.5 * pz / ve;
   if (b2 > 0.0) {
     double b = sqrt(b2), ef = exp(b2 * ve * ve - az * az);
+    g0 = ef;
     cplx ff = cexpi(b * pz);
     cplx w = faddeeva_w(b * ve, az);
     // F = 2 ce / b (i ef Im w + ff);  Gz = 2 ce (ef Re w - ff)
@@ -131,6 +164,7 @@ __device__ __forceinline__ void spectral_term(const bq_qplan& P, double ax, doub
     double b = sqrt(-b2), bv = b * ve;
     double a1 = bv + az, a2 = bv - az;
     double e12 = exp(-bv * bv - az * az);
+    g0 = e12;
     double f1 = (a1 > 0.0) ? e12 * erfcx(a1) : exp(b * pz) * erfc(a1);
     double f2 = (a2 > 0.0) ? e12 * erfcx(a2) : exp(-b * pz) * erfc(a2);
     // F = ce / (i b) (f1 + f2);  Gz = ce (f1 - f2)
@@ -274,8 +308,15 @@ __device__ void spectral_z0_ref(const bq_qplan& P, double px, double py, double 
   o.d[1] += Sy * (-cd);
 }
 
-// Ewald split for the folded point.
-__device__ void qpgf_ewald(const bq_qplan& P, double px, double py, double pz, double ve, QG& o, bool z0_ref)
+// Ewald split for the folded point.  With HESS (Q = QG2) the second derivatives are summed
+// too; z0_ref applies to the first-derivative form only.
+//   spatial term f(rho) = ef Re w / rho (rho = |p + R_l|): grad = q a, Hessian = q I + a a^T q'/rho
+//     with q = -ef (Re w / rho - k Im w + 1/(v sqrt(pi))) / rho^2 and
+//     q'/rho = -k^2 ef Re w / rho^3 + ef / (2 v^3 sqrt(pi) rho^2) - 3 q / rho^2;
+//   spectral term i F e^{i a.p}: d2/dxdy -> -a_x a_y, d2/dx dz -> i a_x Gz,
+//     d2/dz2 -> -i b^2 F - 2 ce g0 / (v sqrt(pi)).
+template <bool HESS, class Q>
+__device__ void qpgf_ewald_t(const bq_qplan& P, double px, double py, double pz, double ve, Q& o, bool z0_ref)
 {
   const double ive = 1.0 / ve, ke = P.k * ve, ke2 = ke * ke;
 
@@ -288,6 +329,8 @@ __device__ void qpgf_ewald(const bq_qplan& P, double px, double py, double pz, d
     double d22 = P.d2x * P.d2x + P.d2y * P.d2y;
     const double gcon = ive / kSQPI;
     cplx S = mk(0, 0), Sxy[2] = {mk(0, 0), mk(0, 0)}, Sz = mk(0, 0);
+    cplx Hd = mk(0, 0), Hxx = mk(0, 0), Hyy = mk(0, 0), Hzz = mk(0, 0), Hxy = mk(0, 0), Hyz = mk(0, 0),
+         Hzx = mk(0, 0);
     const cplx step = mk(P.ph2re, P.ph2im);
     for (int l1 = l1lo; l1 <= l1hi; ++l1) {
       const double ux = px + l1 * P.d1x, uy = py + l1 * P.d1y;
@@ -306,17 +349,38 @@ __device__ void qpgf_ewald(const bq_qplan& P, double px, double py, double pz, d
         Sxy[0] += cf * ax;
         Sxy[1] += cf * ay;
         Sz += cf;
+        if constexpr (HESS) {
+          const double i2 = irl * irl;
+          const cplx qr = cef * (i2 * (0.5 * gcon * ive * ive - P.k2 * w.re * irl)) + cf * (3.0 * i2);
+          Hd += cf;
+          Hxx += qr * (ax * ax);
+          Hyy += qr * (ay * ay);
+          Hzz += qr * (pz * pz);
+          Hxy += qr * (ax * ay);
+          Hyz += qr * (ay * pz);
+          Hzx += qr * (ax * pz);
+        }
       });
     }
     o.g = S * kI4PI;
     o.d[0] = Sxy[0] * (-kI4PI);
     o.d[1] = Sxy[1] * (-kI4PI);
     o.d[2] = Sz * (-pz * kI4PI);
+    if constexpr (HESS) {
+      o.h[0] = (Hxx - Hd) * kI4PI;
+      o.h[1] = (Hyy - Hd) * kI4PI;
+      o.h[2] = (Hzz - Hd) * kI4PI;
+      o.h[3] = Hxy * kI4PI;
+      o.h[4] = Hyz * kI4PI;
+      o.h[5] = Hzx * kI4PI;
+    }
   }
   // ---- spectral sum
-  if (pz == 0.0 && z0_ref) {
-    spectral_z0_ref(P, px, py, ve, o);
-    return;
+  if constexpr (!HESS) {
+    if (pz == 0.0 && z0_ref) {
+      spectral_z0_ref(P, px, py, ve, o);
+      return;
+    }
   }
   // |beta| ve <= sqrt(cut)
   {
@@ -328,6 +392,8 @@ __device__ void qpgf_ewald(const bq_qplan& P, double px, double py, double pz, d
     int n1lo = (int)ceil(c1 - w1), n1hi = (int)floor(c1 + w1);
     double g22 = P.g2x * P.g2x + P.g2y * P.g2y;
     cplx S = mk(0, 0), Sx = mk(0, 0), Sy = mk(0, 0), Sz = mk(0, 0);
+    cplx Hxx = mk(0, 0), Hyy = mk(0, 0), Hxy = mk(0, 0), Hxz = mk(0, 0), Hyz = mk(0, 0), Hb = mk(0, 0),
+         Hg = mk(0, 0);
     const double gp1 = P.g1x * px + P.g1y * py, gp2 = P.g2x * px + P.g2y * py, ap = -(P.vk0 * px + P.vk1 * py);
     const cplx step = cexpi(gp2);
     for (int n1 = n1lo; n1 <= n1hi; ++n1) {
@@ -337,12 +403,22 @@ __device__ void qpgf_ewald(const bq_qplan& P, double px, double py, double pz, d
       const int c = row_centre(ux * P.g2x + uy * P.g2y, g22, lo, hi);
       row_walk(lo, hi, c, cexpi(ap + n1 * gp1 + c * gp2), step, [&](int n2, cplx E) {
         cplx F, Gz;
+        double g0;
         const double ax = ux + n2 * P.g2x, ay = uy + n2 * P.g2y;
-        spectral_term(P, ax, ay, E, pz, ve, F, Gz);
+        spectral_term(P, ax, ay, E, pz, ve, F, Gz, g0);
         S += F;
         Sx += F * ax;
         Sy += F * ay;
         Sz += Gz;
+        if constexpr (HESS) {
+          Hxx += F * (ax * ax);
+          Hyy += F * (ay * ay);
+          Hxy += F * (ax * ay);
+          Hxz += Gz * ax;
+          Hyz += Gz * ay;
+          Hb += F * (P.k2 - ax * ax - ay * ay);
+          Hg += E * g0;
+        }
       });
     }
     double cd = 0.25 * P.aidd;
@@ -350,7 +426,20 @@ __device__ void qpgf_ewald(const bq_qplan& P, double px, double py, double pz, d
     o.d[0] += Sx * (-cd);
     o.d[1] += Sy * (-cd);
     o.d[2] += Sz * cd;
+    if constexpr (HESS) {
+      o.h[0] += cmul_i(Hxx) * (-cd);
+      o.h[1] += cmul_i(Hyy) * (-cd);
+      o.h[2] += cmul_i(Hb) * (-cd) + Hg * (-2.0 * cd * ive / kSQPI);
+      o.h[3] += cmul_i(Hxy) * (-cd);
+      o.h[4] += cmul_i(Hyz) * cd;
+      o.h[5] += cmul_i(Hxz) * cd;
+    }
   }
+}
+__device__ __forceinline__ void qpgf_ewald(const bq_qplan& P, double px, double py, double pz, double ve, QG& o,
+                                           bool z0_ref)
+{
+  qpgf_ewald_t<false>(P, px, py, pz, ve, o, z0_ref);
 }
 
 // Value and gradient at r (relative position, source minus field point).  z0_ref = true
@@ -379,6 +468,38 @@ __device__ void qpgf_v1(const bq_qplan& P, double rx, double ry, double rz, QG& 
     cplx cf = cexpi(-(P.vk0 * (l1 * P.d1x + l2 * P.d2x) + P.vk1 * (l1 * P.d1y + l2 * P.d2y)));
     o.g = o.g * cf;
     for (int i = 0; i < 3; ++i) o.d[i] = o.d[i] * cf;
+  }
+}
+
+// Value, gradient and Hessian at r (the reference's _v2, qgf2_src/d3_qpgf_d2.c:178-230,
+// :887-1043, :2011-2399).  Its stop tests watch the xx and yy second derivatives, which do
+// not vanish at r_z == 0, so every sum here is the converged one.
+__device__ void qpgf_v2(const bq_qplan& P, double rx, double ry, double rz, QG2& o)
+{
+  double l1, l2, px, py;
+  fold_cell(P, rx, ry, l1, l2, px, py);
+  const double pz = rz;
+  const double ve = ewald_param(P, px, py, pz);
+  bool fd = false;
+  if (P.fd_allowed && pz != 0.0) {
+    const double bz = BQ_EXP_CUT / fabs(pz);
+    const double n_fd = kPI * (P.k2 + bz * bz) / P.rcell;
+    const double n_sn = kPI * (P.k2 + BQ_EXP_CUT / (ve * ve)) / P.rcell;
+    const double rs2 = 4.0 * ve * ve * (P.k2 * ve * ve + BQ_EXP_CUT) - pz * pz;
+    const double n_sp = rs2 > 0.0 ? kPI * rs2 / P.cell : 0.0;
+    fd = BQ_COST_FD * n_fd < BQ_COST_SPEC * n_sn + BQ_COST_SPAT * n_sp;
+  }
+  if (fd) {
+    qpgf_fourier_t<true>(P, px, py, pz, o);
+  } else {
+    for (int i = 0; i < 6; ++i) o.h[i] = mk(0, 0);
+    qpgf_ewald_t<true>(P, px, py, pz, ve, o, false);
+  }
+  if (l1 != 0.0 || l2 != 0.0) {
+    cplx cf = cexpi(-(P.vk0 * (l1 * P.d1x + l2 * P.d2x) + P.vk1 * (l1 * P.d1y + l2 * P.d2y)));
+    o.g = o.g * cf;
+    for (int i = 0; i < 3; ++i) o.d[i] = o.d[i] * cf;
+    for (int i = 0; i < 6; ++i) o.h[i] = o.h[i] * cf;
   }
 }
 

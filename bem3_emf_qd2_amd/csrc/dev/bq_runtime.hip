@@ -184,6 +184,19 @@ __global__ void __launch_bounds__(256) k_qpgf_v1(bq_qplan P, int z0_ref, int n, 
   for (int c = 0; c < 3; ++c) { o[2 + 2 * c] = q.d[c].re; o[3 + 2 * c] = q.d[c].im; }
 }
 
+__global__ void __launch_bounds__(256) k_qpgf_v2(bq_qplan P, int n, const double* __restrict__ r,
+                                                 double* __restrict__ out)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  bq::QG2 q;
+  bq::qpgf_v2(P, r[3 * i], r[3 * i + 1], r[3 * i + 2], q);
+  double* o = out + 20 * i;
+  o[0] = q.g.re; o[1] = q.g.im;
+  for (int c = 0; c < 3; ++c) { o[2 + 2 * c] = q.d[c].re; o[3 + 2 * c] = q.d[c].im; }
+  for (int c = 0; c < 6; ++c) { o[8 + 2 * c] = q.h[c].re; o[9 + 2 * c] = q.h[c].im; }
+}
+
 extern "C" {
 
 const char* bq_last_error(void) { return bq::g_err; }
@@ -219,14 +232,17 @@ int bq_qpgf_plan(const double* qd8, double eps, bq_qplan* plan)
 int bq_qpgf_v1_dev(const bq_qplan* plan, int variant, int n, const double* d_r, double* d_out, hipStream_t s)
 {
   if (n <= 0) return 0;
-  k_qpgf_v1<<<(n + 255) / 256, 256, 0, s>>>(*plan, variant == 0, n, d_r, d_out);
+  if (variant == 2)
+    k_qpgf_v2<<<(n + 255) / 256, 256, 0, s>>>(*plan, n, d_r, d_out);
+  else
+    k_qpgf_v1<<<(n + 255) / 256, 256, 0, s>>>(*plan, variant == 0, n, d_r, d_out);
   BQ_HIP(hipGetLastError());
   return 0;
 }
 
 int bq_qpgf_batch(const double* qd8, int variant, int n, const double* r, double* out)
 {
-  if (n < 0 || !qd8 || (variant != 0 && variant != 1) || (n > 0 && (!r || !out))) {
+  if (n < 0 || !qd8 || variant < 0 || variant > 2 || (n > 0 && (!r || !out))) {
     bq::set_error("bq_qpgf_batch: bad arguments");
     return -2;
   }
@@ -237,10 +253,11 @@ int bq_qpgf_batch(const double* qd8, int variant, int n, const double* r, double
   hipStream_t s = bq::stream();
   double *dr = nullptr, *dout = nullptr;
   BQ_HIP(hipMallocAsync((void**)&dr, sizeof(double) * 3 * (size_t)n, s));
-  BQ_HIP(hipMallocAsync((void**)&dout, sizeof(double) * 8 * (size_t)n, s));
+  const size_t nv = variant == 2 ? 20 : 8;
+  BQ_HIP(hipMallocAsync((void**)&dout, sizeof(double) * nv * (size_t)n, s));
   BQ_HIP(hipMemcpyAsync(dr, r, sizeof(double) * 3 * (size_t)n, hipMemcpyHostToDevice, s));
   if (bq_qpgf_v1_dev(&P, variant, n, dr, dout, s)) return -1;
-  BQ_HIP(hipMemcpyAsync(out, dout, sizeof(double) * 8 * (size_t)n, hipMemcpyDeviceToHost, s));
+  BQ_HIP(hipMemcpyAsync(out, dout, sizeof(double) * nv * (size_t)n, hipMemcpyDeviceToHost, s));
   BQ_HIP(hipFreeAsync(dr, s));
   BQ_HIP(hipFreeAsync(dout, s));
   BQ_HIP(hipStreamSynchronize(s));

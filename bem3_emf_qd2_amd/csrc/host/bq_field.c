@@ -6,8 +6,8 @@
  * dat_read) and kept in HBM until finalize_dqd2.  The scalar entry points (EH_s, EH_t ...)
  * are safe to call from several host threads at once, as the reference's examples do from
  * OpenMP loops: calls are serialised on one lock and each evaluates its point on the GPU.
- * bq_EH_t_batch and bq_EH_t_bd_batch evaluate many points in one launch; force_FN is one
- * launch of boundary queries followed by the stress sums on the host.
+ * bq_EH_t_batch, bq_EH_t_bd_batch and bq_EH_mEMP_t_batch evaluate many points in one launch;
+ * force_FN is one launch of boundary queries followed by the stress sums on the host.
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -143,11 +143,13 @@ void bq_release_device_state(DQD2 *qd)
   pthread_mutex_unlock(&g_lock);
 }
 
-/* one launch over m queries (free points r0 or boundary points bt/bze); out [m][13 | 9] */
+/* one launch over m queries (free points r0 or boundary points bt/bze); out [m][13 | 9 | 36]
+ * for kinds 0, 1, 2 */
 static double *launch(DQD2 *qd, int kind, int m, const double *r0, const int *dd, const int *bt, const double *bze,
                       int type)
 {
-  double *out = calloc((size_t)(kind == 0 ? 13 : 9) * (m > 0 ? m : 1), sizeof(double));
+  const size_t nv = kind == 0 ? 13 : (kind == 1 ? 9 : 36);
+  double *out = calloc(nv * (m > 0 ? m : 1), sizeof(double));
   if (m > 0) {
     pthread_mutex_lock(&g_lock);
     void *ctx = context_for(qd);
@@ -492,13 +494,74 @@ int force_FN(double *F, double *N, double *rc, int type, DQD2 *qd)
   return 0;
 }
 
-static void not_yet(const char *name)
+/* E, H from the modified potential and its gradient at n points (EH_mEMP_s,
+ * d3qd2_src/d3qd2_field.c:90-173): E = A - grad phi, H = lambda0 / (2 pi i) curl A.  A point
+ * where a solid-angle derivative exceeds CBD_CDF = 10 (too close to the boundary) gets zero
+ * fields and ret = -did (-99 for the open region); otherwise ret = did (-1 outside every
+ * domain).  total adds the incident wave in the open region (EH_mEMP_t). */
+static void eh_memp(DQD2 *qd, int n, const double *r, int type, int total, double complex *E, double complex *H,
+                    int *ret)
 {
-  printf("%s(): not available in this build of the GPU library. Exit...\n", name);
-  exit(1);
+  size_t nn = n > 0 ? n : 1;
+  double *r0 = malloc(sizeof(double) * 3 * nn);
+  int *l1 = malloc(sizeof(int) * nn), *l2 = malloc(sizeof(int) * nn), *dd = malloc(sizeof(int) * nn);
+  int m = 0;
+  for (int p = 0; p < n; p++) {
+    fold(qd, r + 3 * p, r0 + 3 * m, &l1[p], &l2[p]);
+    ret[p] = domain_id(r0 + 3 * m, qd);
+    if (ret[p] >= 0) dd[m++] = ret[p];
+  }
+  double *out = launch(qd, 2, m, r0, dd, NULL, NULL, type);
+  const double complex ch = qd->pw.lambda_0 / (2.0 * M_PI * I);
+  for (int p = 0, q = 0; p < n; p++) {
+    for (int i = 0; i < 3; i++) E[3 * p + i] = H[3 * p + i] = 0.0;
+    if (ret[p] < 0) continue;
+    const double *o = out + 36 * (size_t)q++;
+    const int did = ret[p];
+    if (fabs(o[33]) > 10.0 || fabs(o[34]) > 10.0 || fabs(o[35]) > 10.0) {
+      ret[p] = did == 0 ? -99 : -did;
+      continue;
+    }
+    double iC = (did == 0) ? 1.0 / (1.0 + o[32]) : 1.0 / o[32];
+    double complex U[4], dU[3][4];
+    for (int c = 0; c < 4; c++) U[c] = (o[2 * c] + I * o[2 * c + 1]) * iC;
+    for (int d = 0; d < 3; d++)
+      for (int c = 0; c < 4; c++)
+        dU[d][c] = ((o[8 + 8 * d + 2 * c] + I * o[9 + 8 * d + 2 * c]) - U[c] * o[33 + d]) * iC;
+    double complex ce = bloch(qd, l1[p], l2[p]);
+    for (int d = 0; d < 3; d++) E[3 * p + d] = (-dU[d][3] + U[d]) * ce;
+    H[3 * p + 0] = ch * (dU[1][2] - dU[2][1]) * ce;
+    H[3 * p + 1] = ch * (dU[2][0] - dU[0][2]) * ce;
+    H[3 * p + 2] = ch * (dU[0][1] - dU[1][0]) * ce;
+    if (total && did == 0) {
+      double complex e[3], h[3];
+      calc_mipw_EH(e, h, (double *)(r + 3 * p), &qd->pw);
+      for (int i = 0; i < 3; i++) {
+        E[3 * p + i] += e[i];
+        H[3 * p + i] += h[i];
+      }
+    }
+  }
+  free(r0); free(l1); free(l2); free(dd); free(out);
 }
 
 int EH_mEMP_s(double complex *E, double complex *H, double *rt, int type, DQD2 *qd)
-{ (void)E; (void)H; (void)rt; (void)type; (void)qd; not_yet("EH_mEMP_s"); return -1; }
+{
+  int ret;
+  eh_memp(qd, 1, rt, type, 0, E, H, &ret);
+  return ret;
+}
+
 int EH_mEMP_t(double complex *E, double complex *H, double *rt, int type, DQD2 *qd)
-{ (void)E; (void)H; (void)rt; (void)type; (void)qd; not_yet("EH_mEMP_t"); return -1; }
+{
+  int ret;
+  eh_memp(qd, 1, rt, type, 1, E, H, &ret);
+  return ret;
+}
+
+int bq_EH_mEMP_t_batch(void *qdv, int n, const double *r, int type, double *E, double *H, int *ret)
+{
+  if (n < 0 || type < 0 || type > 3) return -1;
+  eh_memp((DQD2 *)qdv, n, r, type, 1, (double complex *)E, (double complex *)H, ret);
+  return 0;
+}

@@ -35,6 +35,8 @@ def L():
         lb.output_node_particles.argtypes = [ctypes.c_char_p, ctypes.c_void_p]
         lb.bq_EH_t_batch.argtypes = [ctypes.c_void_p, ctypes.c_int, _D, ctypes.c_int, _D, _D, _I]
         lb.bq_EH_t_bd_batch.argtypes = [ctypes.c_void_p, ctypes.c_int, _I, _I, _D, _D, ctypes.c_int, _D, _D]
+        lb.bq_EH_mEMP_t_batch.argtypes = [ctypes.c_void_p, ctypes.c_int, _D, ctypes.c_int, _D, _D, _I]
+        lb.EH_mEMP_s.argtypes = [_D, _D, _D, ctypes.c_int, ctypes.c_void_p]
         lb.force_FN.argtypes = [_D, _D, _D, ctypes.c_int, ctypes.c_void_p]
         lb.mEMP_s.argtypes = [_D, _D, ctypes.c_int, ctypes.c_void_p]
         lb.mEMP_t.argtypes = [_D, _D, ctypes.c_int, ctypes.c_void_p]
@@ -93,6 +95,28 @@ class Model:
         L().bq_EH_t_batch(self.ptr, n, r.ctypes.data_as(_D), type_, E.ctypes.data_as(_D), H.ctypes.data_as(_D),
                           did.ctypes.data_as(_I))
         return E, H, did
+
+    def fields_memp(self, r, type_=1):
+        """Total E, H from the modified potential (EH_mEMP_t) at points r[n][3]; ret[n] = domain
+        id, or -did (-99 for the open region) for a point too close to the boundary."""
+        r = np.ascontiguousarray(np.asarray(r, float).reshape(-1, 3))
+        n = len(r)
+        E = np.zeros((n, 3), complex)
+        H = np.zeros((n, 3), complex)
+        ret = np.zeros(n, np.int32)
+        rc = L().bq_EH_mEMP_t_batch(self.ptr, n, r.ctypes.data_as(_D), type_, E.ctypes.data_as(_D),
+                                    H.ctypes.data_as(_D), ret.ctypes.data_as(_I))
+        if rc:
+            raise ValueError("bq_EH_mEMP_t_batch: bad arguments")
+        return E, H, ret
+
+    def fields_memp_s(self, r, type_=1):
+        """Scattered E, H from the modified potential at one point through EH_mEMP_s."""
+        r = np.ascontiguousarray(np.asarray(r, float).reshape(3))
+        E = np.zeros(3, complex)
+        H = np.zeros(3, complex)
+        ret = L().EH_mEMP_s(E.ctypes.data_as(_D), H.ctypes.data_as(_D), r.ctypes.data_as(_D), type_, self.ptr)
+        return E, H, ret
 
     def fields_bd(self, did, t, zt, et, type_=1):
         """Total E, H at boundary points (zt[p], et[p]) of the t[p]-th element of domain did[p]."""

@@ -173,6 +173,8 @@ void *m_alloc2(size_t num, size_t size, char *txt);
 
 /* -- quasi-periodic Green's function (qgf2_src/d3_qpgf_d2.h:101-126), one point each -- */
 int d3hm_qpgf_d2_v1(double complex *qG, double complex *dqG, double *r, double eps, QPD2 *qd);
+/* d2qG[6] = xx, yy, zz, xy, yz, zx second derivatives (qgf2_src/d3_qpgf_d2.h:124) */
+int d3hm_qpgf_d2_v2(double complex *qG, double complex *dqG, double complex *d2qG, double *r, double eps, QPD2 *qd);
 int d3hm_qpgf_d2_qG(double complex *qG, double *r, double eps, QPD2 *qd);
 int d3hm_qpgf_d2_dqG(double complex *dqG, double *r, double eps, QPD2 *qd);
 

@@ -31,7 +31,8 @@ int bq_qpgf_v1_batch(const double *qd8, int n, const double *r, double *out);
 /* Same with a choice of the reference variant it reproduces at r_z == 0, where the
  * reference's variants differ: variant 0 = d3hm_qpgf_d2_v1 / _dqG (their spectral sum
  * stops early there, d3_qpgf_d2.c:1840-1963), variant 1 = d3hm_qpgf_d2_qG (converged,
- * d3_qpgf_d2.c:1190-1335).  Elsewhere both give the converged sum. */
+ * d3_qpgf_d2.c:1190-1335).  Elsewhere both give the converged sum.  variant 2 =
+ * d3hm_qpgf_d2_v2 (d3_qpgf_d2.c:178-230): out[n][20] adds d2qG xx, yy, zz, xy, yz, zx. */
 int bq_qpgf_batch(const double *qd8, int variant, int n, const double *r, double *out);
 
 /* sizeof(DQD2) as compiled into the library (for callers that allocate it opaquely). */
@@ -41,6 +42,12 @@ int bq_sizeof_dqd2(void);
  * d3qd2_src/d3qd2_field.c:271).  qd: a DQD2* after solve_bieq or dat_read; r[n][3];
  * E, H: [n][3] complex out; did[n] out (domain id, -1 if none).  type 0..3 as EH_t. */
 int bq_EH_t_batch(void *qd, int n, const double *r, int type, double *E, double *H, int *did);
+
+/* Total fields from the modified potential at n points in one GPU launch (batched
+ * EH_mEMP_t of d3qd2_src/d3qd2_field.c:175-190).  E, H: [n][3] complex out; ret[n] = domain id,
+ * -1 outside every domain, or -did (-99 for the open region) with zero fields for a point too
+ * close to the boundary.  Returns -1 if type is outside 0..3. */
+int bq_EH_mEMP_t_batch(void *qd, int n, const double *r, int type, double *E, double *H, int *ret);
 
 /* Total fields at n boundary points in one GPU launch (batched EH_t_bd of
  * d3qd2_src/d3qd2_field.c:343-391): point p is (zt[p], et[p]) on the t[p]-th element

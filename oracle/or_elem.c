@@ -679,6 +679,68 @@ void or_coef_rt(const or_model *m, int did, const double *rt, int ss, int type, 
   if (!periodic && ss < 0) flip_sign(CC);
 }
 
+/* ---- field-point coefficients and their gradient in the field point ----------------
+ * dC[p][.] = d CC / d rt_p (q0_dcoef_rt_grad / dcoef_rt_grad, d3qd2_src/d3qd2_elem.c:316-349,
+ * :650-683; kernels q0_bil_dcoef_grad.c:10-328, q0_lit_dcoef_grad.c:10-389,
+ * bil_dcoef_grad.c:10-298, lit_dcoef_grad.c:10-352).  Same rules and weights as or_coef_rt;
+ * the periodic kernel takes the Hessian from the _v2 lattice sum. */
+void or_coef_rt_grad(const or_model *m, int did, const double *rt, int ss, int type, double complex *CC,
+                     double complex (*dC)[9])
+{
+  int s = abs(ss);
+  elc e;
+  get_el(m, s, &e);
+  int periodic = (did == 0 && m->ps == 1), ns = n_shape(&e);
+  double complex k = m->kn[did];
+  const or_rule *q = rule_of(m, &e, type);
+  for (int l = 0; l < 9; l++) CC[l] = dC[0][l] = dC[1][l] = dC[2][l] = 0.0;
+  double F = 0, dF[3] = {0, 0, 0};
+  for (int p = 0; p < q->n; p++) {
+    double R[3], W[3];
+    rw_at(&e, q->z[p], q->e[p], R, W);
+    if (type == 0)
+      for (int d = 0; d < 3; d++) { R[d] = m->ren[12 * s + 3 * p + d]; W[d] = m->wen[12 * s + 3 * p + d]; }
+    for (int d = 0; d < 3; d++) { R[d] -= rt[d]; W[d] *= e.js; }
+    double aR = nrm3(R), iR = 1.0 / aR, aW = nrm3(W);
+    double rdW = dot3(R, W) * iR, tD = rdW * iR * iR;
+    F += q->w[p] * tD;
+    for (int d = 0; d < 3; d++) dF[d] += q->w[p] * iR * iR * iR * (3.0 * R[d] * iR * rdW - W[d]);
+    double complex g, h, dg[3], dh[3];
+    if (periodic) {
+      double complex o[10];
+      qpgf_or_die(m, 1, R, o);
+      /* Hessian order xx, yy, zz, xy, yz, zx */
+      double complex Hm[3][3] = {{o[4], o[7], o[9]}, {o[7], o[5], o[8]}, {o[9], o[8], o[6]}};
+      g = o[0] * aW;
+      h = o[1] * W[0] + o[2] * W[1] + o[3] * W[2];
+      for (int d = 0; d < 3; d++) {
+        dg[d] = -o[1 + d] * aW;
+        dh[d] = -(Hm[d][0] * W[0] + Hm[d][1] * W[1] + Hm[d][2] * W[2]);
+      }
+    } else {
+      double complex ca = I * k * aR, cb = ca - 1.0, ce = cexp(ca);
+      g = I_FP * ce * iR * aW;
+      h = I_FP * cb * ce * tD;
+      for (int d = 0; d < 3; d++) {
+        dg[d] = -I_FP * cb * ce * iR * iR * (R[d] * iR) * aW;
+        dh[d] = I_FP * ce * iR * iR * iR * ((k * k * aR * aR + 3.0 * cb) * (R[d] * iR) * rdW - cb * W[d]);
+      }
+    }
+    for (int ep = 0; ep < ns; ep++) {
+      double Np = (type == 0 && p < ns) ? (ep == p ? 1.0 : 0.0) : Nn(&e, ep, q->z[p], q->e[p]);
+      CC[ep] += q->w[p] * g * Np;
+      CC[ep + 4] += q->w[p] * h * Np;
+      for (int d = 0; d < 3; d++) {
+        dC[d][ep] += q->w[p] * dg[d] * Np;
+        dC[d][ep + 4] += q->w[p] * dh[d] * Np;
+      }
+    }
+  }
+  CC[8] = F * I_FP;
+  for (int d = 0; d < 3; d++) dC[d][8] = dF[d] * I_FP;
+  if (!periodic && ss < 0) { flip_sign(CC); flip_sign(dC[0]); flip_sign(dC[1]); flip_sign(dC[2]); }
+}
+
 /* ---- boundary-point coefficients: (zt, et) on element t, source s ---------------------- */
 void or_coef_bd(const or_model *m, int did, int t, double zt, double et, int ss, int type, double complex *CC)
 {

@@ -4,7 +4,8 @@
  * TEST INFRASTRUCTURE ONLY (see oracle.h).
  *
  * Follows d3qd2_src/d3qd2_setup.c (domain_id :157-202, q0_domain_id_l :238-245,
- * fid_calc_solid_angle :1351-1422), d3qd2_src/d3qd2_field.c (mEMP_s/t :10-73, EH_s :203-269,
+ * fid_calc_solid_angle :1351-1422), d3qd2_src/d3qd2_field.c (mEMP_s/t :10-73, EH_mEMP_s/t :90-190,
+ * EH_s :203-269,
  * EH_t :271-285, EH_s_bd/EH_t_bd :299-391) and d3qd2_src/d3qd2_force.c (force_FN).
  */
 #include <math.h>
@@ -184,6 +185,64 @@ int or_mEMP(const or_model *m, double complex **U, double complex **dU, const do
     double complex e[3], h[3];
     or_ipw_EH(m, r, e, h);
     for (int i = 0; i < 3; i++) Uo[i] += e[i];
+  }
+  return did;
+}
+
+/* E, H from the modified potential and its gradient (EH_mEMP_s / EH_mEMP_t,
+ * d3qd2_src/d3qd2_field.c:90-190): E = A - grad phi, H = lambda0 / (2 pi i) curl A.  A point
+ * where a solid-angle derivative exceeds CBD_CDF = 10 (too close to the boundary) gives zero
+ * fields and returns -did (-99 for the open region). */
+int or_EH_mEMP(const or_model *m, double complex **U, double complex **dU, const double *r, int type, int total,
+               double complex *Eo, double complex *Ho)
+{
+  int l1 = 0, l2 = 0;
+  double r0[3] = {r[0], r[1], r[2]};
+  if (m->ps == 1) {
+    const or_qpd2 *q = &m->qd;
+    double idd = 1.0 / (q->vd1[0] * q->vd2[1] - q->vd2[0] * q->vd1[1]);
+    l1 = (int)floor((r[0] * q->vd2[1] - r[1] * q->vd2[0]) * idd + 0.5);
+    l2 = (int)floor(-(r[0] * q->vd1[1] - r[1] * q->vd1[0]) * idd + 0.5);
+    r0[0] = r[0] - l1 * q->vd1[0] - l2 * q->vd2[0];
+    r0[1] = r[1] - l1 * q->vd1[1] - l2 * q->vd2[1];
+  }
+  int did = or_domain_id(m, r0);
+  for (int i = 0; i < 3; i++) Eo[i] = Ho[i] = 0.0;
+  if (did < 0) return did;
+  double complex Us[4] = {0, 0, 0, 0}, dUs[3][4] = {{0}};
+  double F = 0, dF[3] = {0, 0, 0};
+  for (int s = 1; s <= m->dNe[did]; s++) {
+    double complex CC[9], dC[3][9];
+    or_coef_rt_grad(m, did, r0, m->sid[did][s], type, CC, dC);
+    for (int n = 0; n < 4; n++)
+      for (int c = 0; c < 4; c++) {
+        double complex u = U[did][16 * s + 4 * n + c], du = dU[did][16 * s + 4 * n + c];
+        Us[c] += CC[n] * du - CC[n + 4] * u;
+        for (int d = 0; d < 3; d++) dUs[d][c] += dC[d][n] * du - dC[d][n + 4] * u;
+      }
+    F += creal(CC[8]);
+    for (int d = 0; d < 3; d++) dF[d] += creal(dC[d][8]);
+  }
+  if (fabs(dF[0]) > 10.0 || fabs(dF[1]) > 10.0 || fabs(dF[2]) > 10.0) return did == 0 ? -99 : -did;
+  double iC = (did == 0) ? 1.0 / (1.0 + F) : 1.0 / F;
+  for (int c = 0; c < 4; c++) Us[c] *= iC;
+  for (int d = 0; d < 3; d++)
+    for (int c = 0; c < 4; c++) dUs[d][c] = (dUs[d][c] - Us[c] * dF[d]) * iC;
+  double complex ch = m->lambda0 / (2.0 * M_PI * I);
+  for (int d = 0; d < 3; d++) Eo[d] = -dUs[d][3] + Us[d];
+  Ho[0] = ch * (dUs[1][2] - dUs[2][1]);
+  Ho[1] = ch * (dUs[2][0] - dUs[0][2]);
+  Ho[2] = ch * (dUs[0][1] - dUs[1][0]);
+  if (l1 != 0 || l2 != 0) {
+    const or_qpd2 *q = &m->qd;
+    double arg = q->vk[0] * (l1 * q->vd1[0] + l2 * q->vd2[0]) + q->vk[1] * (l1 * q->vd1[1] + l2 * q->vd2[1]);
+    double complex ce = cos(arg) + I * sin(arg);
+    for (int i = 0; i < 3; i++) { Eo[i] *= ce; Ho[i] *= ce; }
+  }
+  if (total && did == 0) {
+    double complex e[3], h[3];
+    or_ipw_EH(m, r, e, h);
+    for (int i = 0; i < 3; i++) { Eo[i] += e[i]; Ho[i] += h[i]; }
   }
   return did;
 }

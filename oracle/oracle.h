@@ -93,6 +93,10 @@ int or_pair(const or_model *m, int did, int t, int tn, int s, int type, double c
 void or_node_tangents(const or_model *m, int s, int nid, double *vtz, double *vte);
 /* Coefficients for an arbitrary field point (reference q0_coef_rt / coef_rt). */
 void or_coef_rt(const or_model *m, int did, const double *rt, int s, int type, double complex *CC);
+/* as or_coef_rt, plus dC[p][.] = d CC / d rt_p (q0_dcoef_rt_grad / dcoef_rt_grad,
+ * d3qd2_src/d3qd2_elem.c:316-349, :650-683) */
+void or_coef_rt_grad(const or_model *m, int did, const double *rt, int s, int type, double complex *CC,
+                     double complex (*dC)[9]);
 
 /* Coefficients for the point (zt, et) of element t (signed ids as in sid lists): the polar
  * singular rule when s is t, else the rule `type` at that point (reference q0_coef_bd /
@@ -126,6 +130,11 @@ void or_deh_rhs(const or_model *m, int d, const double complex *tH, double compl
 int or_domain_id(const or_model *m, const double *rt);
 int or_EH_t(const or_model *m, double complex **E, double complex **dE, double complex **H,
             double complex **dH, const double *r, int type, double complex *Eo, double complex *Ho);
+/* E, H from (A, phi) and its gradient at r (EH_mEMP_s / EH_mEMP_t, d3qd2_src/d3qd2_field.c:90-190);
+ * returns the domain id, or -did (-99 for the open region) with zero fields when the point is
+ * too close to the boundary. */
+int or_EH_mEMP(const or_model *m, double complex **U, double complex **dU, const double *r, int type, int total,
+               double complex *Eo, double complex *Ho);
 /* Modified electromagnetic potential (A, phi) at r from the stored U, dU ([Ne+1][4][4] complex
  * per domain); total = 1 adds the incident E to A in the open region (mEMP_s / mEMP_t,
  * d3qd2_src/d3qd2_field.c:10-73).  Returns the domain id. */

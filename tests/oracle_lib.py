@@ -49,6 +49,9 @@ def oracle() -> ctypes.CDLL:
         o.or_EH_t.argtypes = [ctypes.c_void_p] + [ctypes.POINTER(D)] * 4 + [D, ctypes.c_int, D, D]
         o.or_EH_bd.argtypes = [ctypes.c_void_p] + [ctypes.POINTER(D)] * 4 + [ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                                                              ctypes.c_double, ctypes.c_int, ctypes.c_int, D, D]
+        o.or_EH_mEMP.argtypes = [ctypes.c_void_p, ctypes.POINTER(D), ctypes.POINTER(D), D, ctypes.c_int, ctypes.c_int,
+                                 D, D]
+        o.or_coef_rt_grad.argtypes = [ctypes.c_void_p, ctypes.c_int, D, ctypes.c_int, ctypes.c_int, D, D]
         o.or_mEMP.argtypes = [ctypes.c_void_p, ctypes.POINTER(D), ctypes.POINTER(D), D, ctypes.c_int, ctypes.c_int, D]
         o.or_force_FN.argtypes = [ctypes.c_void_p] + [ctypes.POINTER(D)] * 4 + [D, ctypes.c_int, D, D]
         o.or_coef_bd.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,

@@ -146,6 +146,26 @@ class OracleModel:
         did = self.o.or_mEMP(self.h, P(sol["U"]), P(sol["dU"]), ptr(rr), type_, int(total), ptr(U.view(float)))
         return did, U
 
+    def EH_mEMP(self, sol, r, type_=1, total=True):
+        """E, H at r from (A, phi) and its gradient; returns (did or the near-boundary code, E, H)."""
+        P = lambda L: (D * len(L))(*[ptr(x.view(float)) for x in L])
+        E = np.zeros(3, complex); H = np.zeros(3, complex)
+        rr = np.ascontiguousarray(r, float)
+        did = self.o.or_EH_mEMP(self.h, P(sol["U"]), P(sol["dU"]), ptr(rr), type_, int(total), ptr(E.view(float)),
+                                ptr(H.view(float)))
+        return did, E, H
+
+    def coef_rt(self, did, r, s, type_=1, grad=False):
+        """CC[9] (and dC[3][9] = d CC / d r) of signed element s at field point r."""
+        CC = np.zeros(9, complex)
+        rr = np.ascontiguousarray(r, float)
+        if not grad:
+            self.o.or_coef_rt(self.h, did, ptr(rr), int(s), type_, ptr(CC.view(float)))
+            return CC
+        dC = np.zeros((3, 9), complex)
+        self.o.or_coef_rt_grad(self.h, did, ptr(rr), int(s), type_, ptr(CC.view(float)), ptr(dC.view(float)))
+        return CC, dC
+
     def EH_bd(self, sol, did, t, zt, et, type_=0, total=True):
         P = lambda L: (D * len(L))(*[ptr(x.view(float)) for x in L])
         E = np.zeros(3, complex); H = np.zeros(3, complex)

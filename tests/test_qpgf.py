@@ -1,5 +1,5 @@
-"""Quasi-periodic Green's function: oracle and GPU kernel against the reference's own
-d3hm_qpgf_d2_v2, _v1, _qG and _dqG (golden vectors from oracle/_ref, built from
+"""Quasi-periodic Green's function: oracle and GPU kernels (first-derivative form and the
+_v2 form with the Hessian) against the reference's own d3hm_qpgf_d2_v2, _v1, _qG and _dqG (golden vectors from oracle/_ref, built from
 qgf2_src/d3_qpgf_d2.c), plus properties the mathematics fixes: Bloch quasi-periodicity,
 Ewald/Fourier agreement via points on both sides of the switch, gradient by finite
 differences.
@@ -118,6 +118,35 @@ def test_gpu_vs_reference(gpu_lib, case):
         e = _err(got, val[:, :4])
         print(name, "gpu qpgf (converged) max rel vs v2", e.max())
         assert e.max() < TOL, (name, r[e.argmax()], e.max())
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", [c[0] for c in _cases()])
+def test_gpu_v2_hessian_vs_reference(gpu_lib, case):
+    """d3hm_qpgf_d2_v2 (value, gradient, Hessian) against the reference's own _v2 output."""
+    for name, qd, r, val, code in _cases():
+        if name != case:
+            continue
+        got = gpu_lib.qpgf_v2(qd, r)
+        e = _err(got[:, :4], val[:, :4])
+        eh = np.max(np.abs(got[:, 4:] - val[:, 4:]), 1) / np.max(np.abs(val[:, 4:]), 1)
+        print(name, "gpu v2 max rel", e.max(), "hessian", eh.max())
+        assert e.max() < TOL, (name, r[e.argmax()], e.max())
+        assert eh.max() < 1e-11, (name, r[eh.argmax()], eh.max())
+
+
+@pytest.mark.gpu
+def test_gpu_v2_hessian_vs_oracle_many(gpu_lib):
+    """Random points on both sides of the Ewald / Fourier switch, incl. neighbour cells."""
+    rng = np.random.default_rng(10)
+    for name, qd, r, val, code in _cases():
+        rr = rng.uniform(-1.2, 1.2, (2000, 3))
+        a = gpu_lib.qpgf_v2(qd, rr)
+        b = oracle_lib.qpgf(qd, rr, mode=1)[0]
+        e = _err(a[:, :4], b[:, :4])
+        eh = np.max(np.abs(a[:, 4:] - b[:, 4:]), 1) / np.max(np.abs(b[:, 4:]), 1)
+        assert e.max() < TOL, (name, rr[e.argmax()], e.max())
+        assert eh.max() < 1e-11, (name, rr[eh.argmax()], eh.max())
 
 
 @pytest.mark.gpu
