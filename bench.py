@@ -40,13 +40,18 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFS = 78.6
 
-# FP64 VALU flops per QPGF evaluation inside the assembly kernels, per workload, from
-# rocprofv3 --pmc SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 (x64 lanes, FMA counted twice)
-# divided by the evaluations each launch makes (profiles/k_q0_gl_c1_pmc.md), and the HBM
-# bytes per launch from FETCH_SIZE (x2.00) + WRITE_SIZE (x1.00), the factors a 1 GiB copy
-# gives for the same counters.
-FLOPS_PER_EVAL = {"c1": {"k_q0_gl": 54213, "k_q0_gh": 25808, "k_q0_sing": 50261}}
-TRAFFIC_BYTES = {"c1": {"k_q0_gl": (13373882.2 * 2.0 + 17179946.9) * 1024}}
+# FP64 flops per QPGF evaluation, 64 x (ADD + MUL + TRANS + 2 FMA) instructions of one launch
+# (rocprofv3 PMC) over its device-counted evaluations; C2's figures are read on the same scene
+# at reduced resolution (workload c2s: same lattice, wavelength, materials and incidence).
+# Files under profiles/ hold the counter values.
+FLOPS_PER_EVAL = {"c1": {"k_q0_gl": 54213, "k_q0_gh": 25808, "k_q0_sing": 50261},
+                  "c2": {"k_q0_gl": 44823, "k_q0_gh": 50007, "k_q0_sing": 50175}}
+FLOPS_PER_EVAL["c2s"] = FLOPS_PER_EVAL["c2"]
+# HBM bytes per evaluation of k_q0_gl: (FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00) KiB x 1024 of one
+# launch over its evaluations (factors from a 1 GiB copy read with the same counters)
+TRAFFIC_PER_EVAL = {"c1": {"k_q0_gl": (13373882.2 * 2.0 + 17179946.9) * 1024 / 84215808},
+                    "c2": {"k_q0_gl": (761168.875 * 2.0 + 87269516.4375) * 1024 / 829861632}}
+TRAFFIC_PER_EVAL["c2s"] = TRAFFIC_PER_EVAL["c2"]
 
 WORKLOADS = {
     "c1": dict(kind="sphere", n=8, radius=0.4, n_sphere=1.5, lam=0.6, n0=1.0, theta=1e-6, phi=0.0, pol="TM",
@@ -60,6 +65,14 @@ WORKLOADS = {
                      "seed 2) in one hexagonal cell a1=(0.9,0) a2=(0.45,0.779)um, lambda0=0.55um, theta=30deg, "
                      "phi=15deg, TM, 100 probes (EH_t type 1); shell 0.40 instead of 0.45um so that it does "
                      "not overlap its periodic image (|a2| = 0.8996um)"),
+    # the C2 scene at a quarter of the element count (384 quads per sphere surface, ~500
+    # cone triangles), same lattice, wavelength, materials and incidence: used to read the
+    # QPGF kernels' per-evaluation counters, which depend on the lattice and the spread of
+    # source-target offsets, not on the element count
+    "c2s": dict(kind="sphere_cone", n=8, cone_h=0.06, lam=0.55, n0=1.0, theta=np.radians(30), phi=np.radians(15),
+                pol="TM", d1=(0.9, 0.0), d2=(0.45, 0.779), probes=100, seed=1, steps=3, warmup=1,
+                label="C2 geometry at reduced resolution (384 quads per sphere surface, cone edge 0.06um), "
+                      "for per-evaluation kernel counters"),
 }
 
 
@@ -75,8 +88,8 @@ def _probes(w, n, seed):
 def _scene(w, d):
     from bem3_emf_qd2_amd import meshgen
     if w["kind"] == "sphere_cone":
-        meshgen.sphere_cone_scene(d, n=w["n"], lam=w["lam"], n0=w["n0"], theta=w["theta"], phi=w["phi"],
-                                  pol=w["pol"], d1=w["d1"], d2=w["d2"])
+        meshgen.sphere_cone_scene(d, n=w["n"], cone_h=w.get("cone_h", 0.03), lam=w["lam"], n0=w["n0"],
+                                  theta=w["theta"], phi=w["phi"], pol=w["pol"], d1=w["d1"], d2=w["d2"])
     else:
         meshgen.sphere_scene(d, w["n"], w["radius"], w["n_sphere"], w["lam"], n0=w["n0"], theta=w["theta"],
                              phi=w["phi"], pol=w["pol"], d1=w["d1"], d2=w["d2"])
@@ -134,7 +147,8 @@ def _roofline(ktimes, stats_per_step, steps, workload):
     if name in evals:
         r["evals_per_launch"] = evals[name]
         fpe = FLOPS_PER_EVAL.get(workload, {}).get(name)
-        r["traffic"] = TRAFFIC_BYTES.get(workload, {}).get(name)
+        tpe = TRAFFIC_PER_EVAL.get(workload, {}).get(name)
+        r["traffic"] = tpe * evals[name] if tpe else None
         if fpe:
             ach = evals[name] * fpe / avg_s / 1e12
             r["achieved"] = ach
