@@ -5,7 +5,7 @@
  * The solved model is uploaded once per DQD2 (the first field call after solve_bieq or
  * dat_read) and kept in HBM until finalize_dqd2.  The scalar entry points (EH_s, EH_t ...)
  * are safe to call from several host threads at once, as the reference's examples do from
- * OpenMP loops: calls are serialised on one lock and each evaluates its point on the GPU.
+ * OpenMP loops: concurrent calls are gathered into one launch (point_call below).
  * bq_EH_t_batch, bq_EH_t_bd_batch and bq_EH_mEMP_t_batch evaluate many points in one launch;
  * force_FN is one launch of boundary queries followed by the stress sums on the host.
  */
@@ -215,11 +215,106 @@ static void scattered(DQD2 *qd, int kind, int n, const double *r, int type, doub
   free(r0); free(l1); free(l2); free(dd); free(out);
 }
 
+/* ---- concurrent single-point calls ------------------------------------------------
+ * The reference's examples call EH_t from OpenMP loops, one point per thread
+ * (example2.c:30-36, example3.c:120).  Each call joins a queue; a caller that finds no
+ * evaluation in flight becomes the leader, takes every queued request like the one at the
+ * head (same model, entry point and rule), evaluates them in one launch and wakes the
+ * others.  Every point is computed by its own block in a fixed order, so a result does not
+ * depend on which requests shared its launch. */
+typedef struct preq {
+  DQD2 *qd;
+  int op, type; /* op 0: EH_s, 1: mEMP_s, 2: EH_mEMP_s */
+  double r[3];
+  double complex x[4], y[3];
+  int ret, done;
+  struct preq *next;
+} preq;
+
+static pthread_mutex_t q_mu = PTHREAD_MUTEX_INITIALIZER;
+static pthread_cond_t q_cv = PTHREAD_COND_INITIALIZER;
+static preq *q_head, *q_tail;
+static int q_busy;
+
+static void eh_memp(DQD2 *qd, int n, const double *r, int type, int total, double complex *E, double complex *H,
+                    int *ret);
+
+static void run_batch(preq **b, int n)
+{
+  DQD2 *qd = b[0]->qd;
+  const int op = b[0]->op, type = b[0]->type, nx = op == 1 ? 4 : 3;
+  double *r = malloc(sizeof(double) * 3 * n);
+  int *ret = malloc(sizeof(int) * n);
+  double complex *X = malloc(sizeof(double complex) * 4 * n), *Y = malloc(sizeof(double complex) * 3 * n);
+  for (int i = 0; i < n; i++) memcpy(r + 3 * i, b[i]->r, sizeof(double) * 3);
+  if (op == 2) eh_memp(qd, n, r, type, 0, X, Y, ret);
+  else scattered(qd, op, n, r, type, X, Y, ret);
+  for (int i = 0; i < n; i++) {
+    for (int c = 0; c < nx; c++) b[i]->x[c] = X[nx * i + c];
+    if (op != 1)
+      for (int c = 0; c < 3; c++) b[i]->y[c] = Y[3 * i + c];
+    b[i]->ret = ret[i];
+  }
+  free(r); free(ret); free(X); free(Y);
+}
+
+static int point_call(DQD2 *qd, int op, int type, const double *rt, double complex *X, double complex *Y)
+{
+  preq me;
+  memset(&me, 0, sizeof me);
+  me.qd = qd;
+  me.op = op;
+  me.type = type;
+  memcpy(me.r, rt, sizeof(double) * 3);
+  pthread_mutex_lock(&q_mu);
+  if (q_tail) q_tail->next = &me;
+  else q_head = &me;
+  q_tail = &me;
+  while (!me.done) {
+    if (q_busy) {
+      pthread_cond_wait(&q_cv, &q_mu);
+      continue;
+    }
+    q_busy = 1;
+    const preq *h = q_head;
+    int n = 0;
+    for (preq *p = q_head; p; p = p->next)
+      if (p->qd == h->qd && p->op == h->op && p->type == h->type) n++;
+    preq **b = malloc(sizeof(preq *) * n);
+    DQD2 *hq = h->qd;
+    int hop = h->op, hty = h->type, m = 0;
+    preq *keep = NULL, *kt = NULL;
+    for (preq *p = q_head, *nx; p; p = nx) {
+      nx = p->next;
+      p->next = NULL;
+      if (p->qd == hq && p->op == hop && p->type == hty) {
+        b[m++] = p;
+      } else {
+        if (kt) kt->next = p;
+        else keep = p;
+        kt = p;
+      }
+    }
+    q_head = keep;
+    q_tail = kt;
+    pthread_mutex_unlock(&q_mu);
+    run_batch(b, m);
+    pthread_mutex_lock(&q_mu);
+    for (int i = 0; i < m; i++) b[i]->done = 1;
+    free(b);
+    q_busy = 0;
+    pthread_cond_broadcast(&q_cv);
+  }
+  pthread_mutex_unlock(&q_mu);
+  for (int c = 0; c < (op == 1 ? 4 : 3); c++) X[c] = me.x[c];
+  if (op != 1 && Y)
+    for (int c = 0; c < 3; c++) Y[c] = me.y[c];
+  return me.ret;
+}
+
 int EH_s(double complex *E, double complex *H, double *rt, int type, DQD2 *qd)
 {
-  int did;
-  scattered(qd, 0, 1, rt, type, E, H, &did);
-  return did;
+  return point_call(qd, 0, type, rt, E, H);
 }
 
 int EH_t(double complex *E, double complex *H, double *rt, int type, DQD2 *qd)
@@ -264,9 +359,7 @@ int bq_EH_t_batch(void *qdv, int n, const double *r, int type, double *E, double
 
 int mEMP_s(double complex *U, double *rt, int type, DQD2 *qd)
 {
-  int did;
-  scattered(qd, 1, 1, rt, type, U, NULL, &did);
-  return did;
+  return point_call(qd, 1, type, rt, U, NULL);
 }
 
 int mEMP_t(double complex *U, double *rt, int type, DQD2 *qd)
@@ -547,15 +640,20 @@ static void eh_memp(DQD2 *qd, int n, const double *r, int type, int total, doubl
 
 int EH_mEMP_s(double complex *E, double complex *H, double *rt, int type, DQD2 *qd)
 {
-  int ret;
-  eh_memp(qd, 1, rt, type, 0, E, H, &ret);
-  return ret;
+  return point_call(qd, 2, type, rt, E, H);
 }
 
 int EH_mEMP_t(double complex *E, double complex *H, double *rt, int type, DQD2 *qd)
 {
-  int ret;
-  eh_memp(qd, 1, rt, type, 1, E, H, &ret);
+  int ret = EH_mEMP_s(E, H, rt, type, qd);
+  if (ret == 0) {
+    double complex e[3], h[3];
+    calc_mipw_EH(e, h, rt, &qd->pw);
+    for (int i = 0; i < 3; i++) {
+      E[i] += e[i];
+      H[i] += h[i];
+    }
+  }
   return ret;
 }
 

@@ -212,8 +212,9 @@ def write_medium(path, indices) -> None:
 
 def sphere_scene(dirpath, n: int, radius: float, n_sphere, lam: float, n0: float = 1.0,
                  theta: float = 1e-6, phi: float = 0.0, pol: str = "TM", d1=(1.0, 0.0), d2=(0.0, 1.0),
-                 core=None, tri_every: int = 0):
-    """Write a complete input set for a (optionally two-layer) sphere on a lattice.
+                 core=None, tri_every: int = 0, ps: int = 1):
+    """Write a complete input set for a (optionally two-layer) sphere on a lattice, or alone
+    in free space with ps = 0.
 
     core = (radius, index) adds a concentric core (domain 2 inside the shell domain 1).
     tri_every = k > 0 splits every k-th quad of the outer surface into two linear triangles.
@@ -234,7 +235,7 @@ def sphere_scene(dirpath, n: int, radius: float, n_sphere, lam: float, n0: float
     write_msh(d / "mesh.msh", X, surfaces)
     ex, ey = (1.0, 0.0) if pol.upper() == "TM" else (0.0, 1.0)
     write_ipw(d / "ipw.txt", [dict(lambda0=lam, ni=n0, power=0.5 * n0, e0x=ex, e0y=ey, theta=theta, phi=phi)])
-    write_periodicity(d / "periodicity_data.txt", d1, d2)
+    write_periodicity(d / "periodicity_data.txt", d1, d2, ps=ps)
     write_medium(d / "medium_data.txt", media)
     return d
 

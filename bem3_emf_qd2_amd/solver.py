@@ -36,7 +36,8 @@ def L():
         lb.bq_EH_t_batch.argtypes = [ctypes.c_void_p, ctypes.c_int, _D, ctypes.c_int, _D, _D, _I]
         lb.bq_EH_t_bd_batch.argtypes = [ctypes.c_void_p, ctypes.c_int, _I, _I, _D, _D, ctypes.c_int, _D, _D]
         lb.bq_EH_mEMP_t_batch.argtypes = [ctypes.c_void_p, ctypes.c_int, _D, ctypes.c_int, _D, _D, _I]
-        lb.EH_mEMP_s.argtypes = [_D, _D, _D, ctypes.c_int, ctypes.c_void_p]
+        for f in ("EH_mEMP_s", "EH_mEMP_t", "EH_t", "EH_s"):
+            getattr(lb, f).argtypes = [_D, _D, _D, ctypes.c_int, ctypes.c_void_p]
         lb.force_FN.argtypes = [_D, _D, _D, ctypes.c_int, ctypes.c_void_p]
         lb.mEMP_s.argtypes = [_D, _D, ctypes.c_int, ctypes.c_void_p]
         lb.mEMP_t.argtypes = [_D, _D, ctypes.c_int, ctypes.c_void_p]

@@ -125,3 +125,71 @@ def test_gpu_mini_sphere_cone_vs_oracle(tmp_path, gpu_lib):
     sol = o.solve(log=lambda *a: None)
     fake = {"dom": [{k: sol[k][dd] for k in KEYS} for dd in range(d["MN"] + 1)]}
     _compare(read_dat(tmp_path / "gpu.dat"), fake, TOL)
+
+
+# ---- non-periodic mode (ps = 0): the open region with the free-space Green's function ----
+
+def _nonperiodic_sphere(tmp_path, n):
+    from bem3_emf_qd2_amd import meshgen
+    from bem3_emf_qd2_amd.solver import Model
+    meshgen.sphere_scene(tmp_path, n, 0.4, 1.5, 0.6, ps=0)
+    m = Model.from_dir(tmp_path)
+    m.write(tmp_path / "setup.dat")
+    return m
+
+
+MIE_POINTS = [[0.0, 0.0, 1.0], [0.5, 0.3, -0.8], [-0.6, 0.2, 0.3], [0.0, 0.0, -0.7]]
+
+
+def _mie_error(fields):
+    """Largest relative difference of total E and Z0 H from the Mie series over MIE_POINTS."""
+    import mie
+    eE = eH = 0.0
+    for r, (E, H) in zip(MIE_POINTS, fields):
+        Es, Hs = mie.scattered(r, 0.4, 1.5, 1.0, 0.6)
+        Ei, Hi = mie.incident(np.array(r), 1.0, 0.6)
+        eE = max(eE, np.abs(E - Es - Ei).max() / np.abs(Es + Ei).max())
+        eH = max(eH, np.abs(H - Hs - Hi).max() / np.abs(Hs + Hi).max())
+    return eE, eH
+
+
+def test_oracle_nonperiodic_sphere_converges_to_mie(tmp_path):
+    """A lone dielectric sphere (r 0.4 um, n 1.5, lambda0 0.6 um, ps = 0): the oracle's BIE
+    solution approaches the Mie series at second order in the element size (measured E
+    0.191, 0.086, 0.048 and H 0.101, 0.048, 0.028 with 4, 6, 8 quads per cube-face edge), an
+    independent check of the whole formulation in the non-periodic mode."""
+    from oracle_pipeline import OracleModel
+    errs = []
+    for n in (4, 6):
+        m = _nonperiodic_sphere(tmp_path / f"n{n}", n)
+        m.close()
+        d = read_dat(tmp_path / f"n{n}" / "setup.dat")
+        assert d["ps"] == 0
+        o = OracleModel.from_dat(d, stored_nodes=True)
+        sol = o.solve(log=lambda *a: None)
+        s4 = {k: sol[k] for k in ("E", "dE", "H", "dH")}
+        errs.append(_mie_error([o.EH_t(s4, r, 1)[1:] for r in MIE_POINTS]))
+    print(errs)
+    assert errs[1][0] < 0.1 and errs[1][1] < 0.06
+    assert 1.8 < errs[0][0] / errs[1][0] < 2.8  # (6/4)^2 = 2.25
+
+
+@pytest.mark.gpu
+def test_gpu_nonperiodic_sphere_vs_oracle_and_mie(tmp_path, gpu_lib):
+    """ps = 0 end to end on the GPU: the solution matches the oracle's complete solve of the
+    same stored model (1e-10) and the fields approach the Mie series as the oracle's do."""
+    from oracle_pipeline import OracleModel
+    m = _nonperiodic_sphere(tmp_path, 6)
+    m.solve()
+    m.write(tmp_path / "gpu.dat")
+    E, H, did = m.fields(MIE_POINTS, 1)
+    m.close()
+    d = read_dat(tmp_path / "setup.dat")
+    o = OracleModel.from_dat(d, stored_nodes=True)
+    sol = o.solve(log=lambda *a: None)
+    fake = {"dom": [{k: sol[k][dd] for k in KEYS} for dd in range(d["MN"] + 1)]}
+    _compare(read_dat(tmp_path / "gpu.dat"), fake, TOL)
+    assert list(did) == [0, 0, 0, 0]
+    eE, eH = _mie_error(list(zip(E, H)))
+    print("GPU vs Mie (6 quads per edge): E", eE, "H", eH)
+    assert eE < 0.1 and eH < 0.06
