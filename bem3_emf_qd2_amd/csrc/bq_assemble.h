@@ -23,13 +23,17 @@ typedef struct bq_adom {
   double *dG, *dH;    /* device out: 2N x N complex (free-space domains) */
   double *tdF;        /* device out: N x 3 (may be NULL) */
   int *n_gh_out;      /* host out: number of pairs that used the 64x64 rule (may be NULL) */
-  long long *n_points_out; /* host out [3]: integrand points of the 12-point, 64-point and
-                              singular rules (may be NULL) */
+  long long *n_points_out; /* host out [3]: integrand points of the regular (6- to 12-point),
+                              64-point and singular rules (may be NULL) */
+  int full_rule;      /* 1: every regular periodic pair takes the 12-point rule */
 } bq_adom;
 
 size_t bq_assemble_work_bytes(int dNe);
 int bq_assemble_domain_dev(const bq_adom *a, void *work, void *stream);
 int bq_set_quadrature(const double *xl, const double *wl, const double *xh, const double *wh);
+
+/* Gauss-Legendre nodes and weights of order n on [-1, 1] (Newton iteration on P_n). */
+void bq_gauss_legendre(int n, double *x, double *w);
 
 #ifdef __cplusplus
 }

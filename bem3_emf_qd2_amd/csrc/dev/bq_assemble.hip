@@ -116,6 +116,66 @@ __global__ void __launch_bounds__(256) k_classify(const double* __restrict__ E, 
 }
 
 // ------------------------------------------------------------------------------------
+// Gauss order of a regular periodic pair (t, s).  The reference integrates every regular pair
+// with the 12-point rule (d3qd2_src/q0_bil_dcoef_v2_cc.c:129-204).  Away from the target the
+// integrand is analytic in a Bernstein ellipse E_rho of the source's parameter square, and the
+// n-point Gauss error is at most 4.27 M(rho) rho^-2n / (rho^2 - 1) (M: the integrand's bound on
+// the ellipse).  Here rho is at most half of rho_s, the ellipse through the nearest image of
+// the target element's bounding sphere (D = sqrt2 (1 + gap / a), rho_s = D + sqrt(D^2 - 1),
+// a the source's bounding radius), M grows like exp(k a rho / (2 sqrt2)) from the wave factor,
+// and a further factor 4 covers the growth of 1/R on the ellipse.  The pair takes the
+// smallest order of 6, 8, 10 whose bound is below 1e-18 of the integrand, else 12; with
+// full != 0 every pair takes 12.
+__device__ __forceinline__ void elem_sphere(const Elem& e, double (&c)[3], double& rad)
+{
+  const double zc[4] = {e.tri ? -0.5 : -1.0, 1.0, e.tri ? -0.5 : 1.0, -1.0};
+  const double ec[4] = {e.tri ? -0.5 * kSQ3 : -1.0, e.tri ? 0.0 : -1.0, e.tri ? 0.5 * kSQ3 : 1.0, 1.0};
+  const int nc = e.tri ? 3 : 4;
+  double r[4][3];
+  c[0] = c[1] = c[2] = 0.0;
+  for (int i = 0; i < nc; ++i)
+    for (int d = 0; d < 3; ++d) {
+      r[i][d] = e.cr[d][0] + e.cr[d][1] * zc[i] + e.cr[d][2] * ec[i] + e.cr[d][3] * zc[i] * ec[i];
+      c[d] += r[i][d] / nc;
+    }
+  rad = 0.0;
+  for (int i = 0; i < nc; ++i) {
+    const double dx = r[i][0] - c[0], dy = r[i][1] - c[1], dz = r[i][2] - c[2];
+    rad = fmax(rad, sqrt(dx * dx + dy * dy + dz * dz));
+  }
+}
+
+__device__ __forceinline__ int pair_rule(const bq_qplan& P, const Elem& et, const Elem& es, int full)
+{
+  if (full) return 12;
+  double ct[3], cs[3], rt, rs;
+  elem_sphere(et, ct, rt);
+  elem_sphere(es, cs, rs);
+  double dx = ct[0] - cs[0], dy = ct[1] - cs[1];
+  const double dz = ct[2] - cs[2];
+  const double l1 = rint(dx * P.b1x + dy * P.b1y), l2 = rint(dx * P.b2x + dy * P.b2y);
+  dx -= l1 * P.d1x + l2 * P.d2x;
+  dy -= l1 * P.d1y + l2 * P.d2y;
+  double d2 = 1e300;
+  for (int i = -1; i <= 1; ++i)
+    for (int j = -1; j <= 1; ++j) {
+      const double ex = dx + i * P.d1x + j * P.d2x, ey = dy + i * P.d1y + j * P.d2y;
+      d2 = fmin(d2, ex * ex + ey * ey + dz * dz);
+    }
+  const double gap = sqrt(d2) - rt - rs;
+  if (!(gap > 0.0) || !(rs > 0.0)) return 12;
+  const double D = 1.4142135623730950488 * (1.0 + gap / rs), rho_s = D + sqrt(D * D - 1.0);
+  const double c = P.k * rs * 0.35355339059327376220;  // k a / (2 sqrt 2)
+  for (int n = 6; n <= 10; n += 2) {
+    const double rho = fmin((2.0 * n + 2.0) / fmax(c, 1e-300), 0.5 * rho_s);
+    if (!(rho > 1.05)) continue;
+    // log(4 x 4.27) + c rho - 2n log rho - log(rho^2 - 1) <= log(1e-18)
+    if (2.8379 + c * rho - 2.0 * n * log(rho) - log(rho * rho - 1.0) <= -41.4465) return n;
+  }
+  return 12;
+}
+
+// ------------------------------------------------------------------------------------
 // domain 0 (periodic), regular pairs, 12-point rule.  One wavefront per pair (t, s): the
 // target's nodes share the 64 lanes (4 x 16 for a quad, 3 x 21 for a triangle, whose fourth
 // row does not exist), each lane walks the source's Gauss points with stride 16 or 21.
@@ -123,7 +183,7 @@ __global__ void __launch_bounds__(256) k_classify(const double* __restrict__ E, 
 // registers, and keeping the kernel within 256 lets two wavefronts share each SIMD.
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
 k_q0_gl(bq_qplan P, const double* __restrict__ E, const double* __restrict__ ren, const int* __restrict__ sid, int dNe,
-        const unsigned char* __restrict__ flag, double2* __restrict__ G, double2* __restrict__ H)
+        const unsigned char* __restrict__ flag, int full_rule, double2* __restrict__ G, double2* __restrict__ H)
 {
   __shared__ double sacc[16][256];
   __shared__ double selem[4][kElemStride];
@@ -146,6 +206,13 @@ k_q0_gl(bq_qplan P, const double* __restrict__ E, const double* __restrict__ ren
   if (lane < kElemStride) selem[wv][lane] = E[(size_t)abs(sid[s]) * kElemStride + lane];
   __builtin_amdgcn_s_waitcnt(0);
   __builtin_amdgcn_wave_barrier();
+  int ng;
+  {
+    Elem et, es;
+    load_elem(E, abs(sid[t]), et);
+    load_elem(&selem[0][0], wv, es);
+    ng = __builtin_amdgcn_readfirstlane(pair_rule(P, et, es, full_rule));
+  }
   if (active) {
     double rt[3];
     {
@@ -154,14 +221,14 @@ k_q0_gl(bq_qplan P, const double* __restrict__ E, const double* __restrict__ ren
       rt[1] = p[1];
       rt[2] = p[2];
     }
-    const int np = stri ? 432 : 144;
+    const int np = (stri ? 3 : 1) * ng * ng;
     const int nm = (np + gw - 1) / gw;
 #pragma unroll 1
     for (int m = 0; m < nm; ++m) {
       const int p = sub + gw * m;
       if (p >= np) break;
       double z, et, w;
-      gauss_point(stri, false, p, z, et, w);
+      gauss_point_n(stri, ng, p, z, et, w);
       double R[3], W[3], Nn[4];
       {
         Elem e;
@@ -792,7 +859,8 @@ __global__ void k_row_diag(int dNe, int open, const double* __restrict__ E, cons
 // polar rules (QPGF evaluations for the periodic domain)
 __global__ void __launch_bounds__(256) k_count_points(long n, int dNe, const double* __restrict__ E,
                                                       const int* __restrict__ sid,
-                                                      const unsigned char* __restrict__ flag,
+                                                      const unsigned char* __restrict__ flag, bq_qplan P,
+                                                      int periodic, int full_rule,
                                                       unsigned long long* __restrict__ cnt)
 {
   __shared__ double lds[3 * 4];
@@ -800,8 +868,18 @@ __global__ void __launch_bounds__(256) k_count_points(long n, int dNe, const dou
   double v[3] = {0.0, 0.0, 0.0};
   if (i < n) {
     const int f = flag[i];
-    const bool tri = elem_is_tri(E, abs(sid[(int)(i % dNe) + 1]));
-    if (f == 0) v[0] = tri ? 432.0 : 144.0;
+    const int as = abs(sid[(int)(i % dNe) + 1]);
+    const bool tri = elem_is_tri(E, as);
+    if (f == 0) {
+      int ng = 12;
+      if (periodic) {
+        Elem et, es;
+        load_elem(E, abs(sid[(int)(i / dNe) / 4 + 1]), et);
+        load_elem(E, as, es);
+        ng = pair_rule(P, et, es, full_rule);
+      }
+      v[0] = (tri ? 3.0 : 1.0) * ng * ng;
+    }
     else if (f == 1) v[1] = tri ? 12288.0 : 4096.0;
     else if (f == 2) v[2] = tri ? 4608.0 : 6144.0;  // sectors x 128 angles x 12 radii
   }
@@ -823,7 +901,9 @@ extern "C" {
 
 int bq_set_quadrature(const double* xl, const double* wl, const double* xh, const double* wh)
 {
-  if (bq::set_quad_tables(xl, wl, xh, wh)) {
+  double xv[3][10] = {{0}}, wv[3][10] = {{0}};
+  for (int r = 0; r < 3; ++r) bq_gauss_legendre(6 + 2 * r, xv[r], wv[r]);
+  if (bq::set_quad_tables(xl, wl, xh, wh) || bq::set_vrule_tables(&xv[0][0], &wv[0][0])) {
     bq::set_error("quadrature table upload failed");
     return -1;
   }
@@ -863,7 +943,8 @@ int bq_assemble_domain_dev(const bq_adom* a, void* work, void* stream)
   BQ_HIP(hipGetLastError());
   k_collect_gh<<<(unsigned)((pairs + 255) / 256), 256, 0, s>>>(pairs, flag, list, count);
   BQ_HIP(hipGetLastError());
-  k_count_points<<<(unsigned)((pairs + 255) / 256), 256, 0, s>>>(pairs, dNe, a->E, a->sid, flag, npts);
+  k_count_points<<<(unsigned)((pairs + 255) / 256), 256, 0, s>>>(pairs, dNe, a->E, a->sid, flag, a->plan,
+                                                                  a->periodic, a->full_rule, npts);
   BQ_HIP(hipGetLastError());
   int ngh = 0;
   unsigned long long np3[3] = {0, 0, 0};
@@ -872,7 +953,8 @@ int bq_assemble_domain_dev(const bq_adom* a, void* work, void* stream)
   BQ_HIP(hipStreamSynchronize(s));
   const unsigned gl_blocks = (unsigned)(((long)dNe * dNe + 3) / 4);
   if (a->periodic) {
-    BQ_KTIME("k_q0_gl", s, k_q0_gl<<<gl_blocks, 256, 0, s>>>(a->plan, a->E, a->ren, a->sid, dNe, flag, G, H));
+    BQ_KTIME("k_q0_gl", s,
+             k_q0_gl<<<gl_blocks, 256, 0, s>>>(a->plan, a->E, a->ren, a->sid, dNe, flag, a->full_rule, G, H));
     BQ_HIP(hipGetLastError());
     if (ngh) BQ_KTIME("k_q0_gh", s, k_q0_gh<<<ngh, 256, 0, s>>>(a->plan, a->E, a->ren, a->sid, dNe, list, G, H, F3));
     BQ_HIP(hipGetLastError());

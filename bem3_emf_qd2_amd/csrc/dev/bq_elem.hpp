@@ -23,6 +23,8 @@ constexpr double kSQ3 = 1.7320508075688772935274;  // sqrt(3)
 // includes this header owns a copy (no relocatable device code); the host fills it through
 // that unit's set_quad_tables() before launching.
 static __constant__ double c_xl[12], c_wl[12], c_xh[64], c_wh[64];
+// Gauss-Legendre 6, 8 and 10 (rows 0..2, padded to 10) for the distance-adaptive regular rule
+static __constant__ double c_xv[3][10], c_wv[3][10];
 
 static inline int set_quad_tables(const double* xl, const double* wl, const double* xh, const double* wh)
 {
@@ -30,6 +32,14 @@ static inline int set_quad_tables(const double* xl, const double* wl, const doub
   if (hipMemcpyToSymbol(HIP_SYMBOL(c_wl), wl, 12 * sizeof(double)) != hipSuccess) return -1;
   if (hipMemcpyToSymbol(HIP_SYMBOL(c_xh), xh, 64 * sizeof(double)) != hipSuccess) return -1;
   if (hipMemcpyToSymbol(HIP_SYMBOL(c_wh), wh, 64 * sizeof(double)) != hipSuccess) return -1;
+  return 0;
+}
+
+// xv, wv: [3][10], the rules of order 6, 8 and 10 in the first n entries of each row
+static inline int set_vrule_tables(const double* xv, const double* wv)
+{
+  if (hipMemcpyToSymbol(HIP_SYMBOL(c_xv), xv, 30 * sizeof(double)) != hipSuccess) return -1;
+  if (hipMemcpyToSymbol(HIP_SYMBOL(c_wv), wv, 30 * sizeof(double)) != hipSuccess) return -1;
   return 0;
 }
 
@@ -131,6 +141,22 @@ __device__ __forceinline__ void gauss_point(bool tri, bool gh, int p, double& z,
 }
 
 __device__ __forceinline__ int gauss_points(bool tri, bool gh) { return (gh ? 4096 : 144) * (tri ? 3 : 1); }
+
+// point p of the Gauss rule of order n (6, 8, 10 or 12) over the element: n^2 tensor points
+// on the square, 3 n^2 on the triangle
+__device__ __forceinline__ void gauss_point_n(bool tri, int n, int p, double& z, double& e, double& w)
+{
+  const double* x = (n == 12) ? c_xl : c_xv[(n - 6) >> 1];
+  const double* wq = (n == 12) ? c_wl : c_wv[(n - 6) >> 1];
+  if (tri) {
+    tri_rule_point(p, n, x, wq, z, e, w);
+  } else {
+    const int i = p / n, j = p - n * i;
+    z = x[i];
+    e = x[j];
+    w = wq[i] * wq[j];
+  }
+}
 
 // Positions are evaluated without fused multiply-adds, in the reference's operation order
 // (bil_r_zeta_eta, d3qd2_src/d3qd2_elem.c:949-954): whether R_z comes out exactly 0 decides

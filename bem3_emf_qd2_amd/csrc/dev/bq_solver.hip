@@ -27,6 +27,7 @@
 #include "../bq_assemble.h"
 #include "../bq_linalg.h"
 #include "../bq_plan.h"
+#include <cstdlib>
 
 struct bq_sess {
   bq_model m;
@@ -354,6 +355,8 @@ int bq_sess_assemble(bq_sess* S)
     long long npts[3] = {0, 0, 0};
     a.n_gh_out = &ngh;
     a.n_points_out = npts;
+    // BQ_FULL_RULE=1: the reference's 12-point rule for every regular periodic pair
+    a.full_rule = (getenv("BQ_FULL_RULE") && atoi(getenv("BQ_FULL_RULE")) != 0) ? 1 : 0;
     if (bq_assemble_domain_dev(&a, work, s)) {
       (void)hipFree(work);
       return -1;

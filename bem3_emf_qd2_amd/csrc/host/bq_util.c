@@ -7,6 +7,7 @@
 #include <math.h>
 #include "bem3_emf_qd2.h"
 #include "../bq_solver.h"
+#include "../bq_assemble.h"
 
 void vadd_d(double *r, double *v0, double *v1) { for (int i = 0; i < 3; i++) r[i] = v0[i] + v1[i]; }
 void vsub_d(double *r, double *v0, double *v1) { for (int i = 0; i < 3; i++) r[i] = v0[i] - v1[i]; }
@@ -61,4 +62,28 @@ void bq_pack_elem(double *const *cr, double *const *cw, int tri, double *dst)
   }
   dst[21] = tri ? 1.0 : 0.0;
   dst[22] = dst[23] = 0.0;
+}
+
+/* Gauss-Legendre rule of order n on [-1, 1]: roots of P_n by Newton iteration from the
+ * Chebyshev-like start cos(pi (i + 3/4) / (n + 1/2)), weights 2 / ((1 - x^2) P_n'(x)^2). */
+void bq_gauss_legendre(int n, double *x, double *w)
+{
+  for (int i = 0; i < (n + 1) / 2; i++) {
+    double z = cos(M_PI * (i + 0.75) / (n + 0.5)), dp = 1.0;
+    for (int it = 0; it < 100; it++) {
+      double p0 = 1.0, p1 = z;
+      for (int j = 2; j <= n; j++) {
+        double p2 = ((2.0 * j - 1.0) * z * p1 - (j - 1.0) * p0) / j;
+        p0 = p1;
+        p1 = p2;
+      }
+      dp = n * (z * p1 - p0) / (z * z - 1.0);
+      double dz = p1 / dp;
+      z -= dz;
+      if (fabs(dz) < 1e-17) break;
+    }
+    x[i] = -z;
+    x[n - 1 - i] = z;
+    w[i] = w[n - 1 - i] = 2.0 / ((1.0 - z * z) * dp * dp);
+  }
 }
