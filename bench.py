@@ -43,14 +43,14 @@ FP64_PEAK_TFS = 78.6
 # FP64 flops per QPGF evaluation, 64 x (ADD + MUL + TRANS + 2 FMA) instructions of one launch
 # (rocprofv3 PMC) over its device-counted evaluations; C2's figures are read on the same scene
 # at reduced resolution (workload c2s: same lattice, wavelength, materials and incidence).
-# Files under profiles/ hold the counter values.
-FLOPS_PER_EVAL = {"c1": {"k_q0_gl": 54213, "k_q0_gh": 25808, "k_q0_sing": 50261},
-                  "c2": {"k_q0_gl": 44823, "k_q0_gh": 50007, "k_q0_sing": 50175}}
+# profiles/k_q0_gl_c1_pmc.md and profiles/k_q0_gl_c2s_pmc.md hold the counter values.
+FLOPS_PER_EVAL = {"c1": {"k_q0_gl": 57359, "k_q0_gh": 51625, "k_q0_sing": 50274},
+                  "c2": {"k_q0_gl": 49048, "k_q0_gh": 50007, "k_q0_sing": 50175}}
 FLOPS_PER_EVAL["c2s"] = FLOPS_PER_EVAL["c2"]
 # HBM bytes per evaluation of k_q0_gl: (FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00) KiB x 1024 of one
 # launch over its evaluations (factors from a 1 GiB copy read with the same counters)
-TRAFFIC_PER_EVAL = {"c1": {"k_q0_gl": (13373882.2 * 2.0 + 17179946.9) * 1024 / 84215808},
-                    "c2": {"k_q0_gl": (761168.875 * 2.0 + 87269516.4375) * 1024 / 829861632}}
+TRAFFIC_PER_EVAL = {"c1": {"k_q0_gl": (2049625.8 * 2.0 + 12094762.1) * 1024 / 58796160},
+                    "c2": {"k_q0_gl": (16364526.0 * 2.0 + 89938885.0) * 1024 / 509034196}}
 TRAFFIC_PER_EVAL["c2s"] = TRAFFIC_PER_EVAL["c2"]
 
 WORKLOADS = {
