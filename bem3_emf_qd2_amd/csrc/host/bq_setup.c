@@ -528,8 +528,13 @@ static void init_boundary_data(DQD2 *md)
   }
 }
 
+/* Drops the field evaluator's device copy of qd's model and solution (bq_field.c); called
+ * wherever qd's arrays are replaced, freed or rewritten. */
+void bq_release_device_state(DQD2 *qd);
+
 void initialize_dqd2(DQD2 *qd)
 {
+  bq_release_device_state(qd);
   setup_mipw(&qd->pw);
   check_mipw(qd);
   const IpwD *d = &qd->pw.bd.ipw[0].data;
@@ -547,8 +552,6 @@ void initialize_dqd2(DQD2 *qd)
   init_sub_domain(qd);
   init_boundary_data(qd);
 }
-
-void bq_release_device_state(DQD2 *qd); /* bq_field.c */
 
 void finalize_dqd2(DQD2 *qd)
 {
@@ -717,6 +720,7 @@ void dat_write(char *fname, DQD2 *qd)
 
 void dat_read(char *fname, DQD2 *qd)
 {
+  bq_release_device_state(qd);
   FILE *fp = fopen(fname, "rb");
   if (!fp) {
     printf("dat_read(), Failed to open the %s file.\n", fname);

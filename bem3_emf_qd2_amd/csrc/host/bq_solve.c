@@ -295,9 +295,14 @@ static void boundary_dEH(DQD2 *md, bq_sess *S)
   }
 }
 
+void bq_release_device_state(DQD2 *qd); /* bq_field.c */
+
 void solve_bieq(DQD2 *qd)
 {
   double t0 = now_s(), ts;
+  /* the solution arrays are rewritten in place: the field evaluator's copy of the previous
+     solution must not be reused */
+  bq_release_device_state(qd);
   printf("\nsolve modified electromagnetic potential boundary value \n");
   printf("  coefficient matrix          ");
   fflush(stdout);
