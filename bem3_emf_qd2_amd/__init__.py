@@ -39,6 +39,10 @@ def _declare(L: ctypes.CDLL) -> None:
     L.bq_qpgf_v1_batch.argtypes = [_D, ctypes.c_int, _D, _D]
     L.bq_qpgf_batch.argtypes = [_D, ctypes.c_int, ctypes.c_int, _D, _D]
     L.bq_zgesv_host.argtypes = [ctypes.c_int, ctypes.c_int, _D, _D]
+    L.bq_cgesv_host.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_float),
+                                ctypes.POINTER(ctypes.c_float)]
+    L.bq_solve_precision.argtypes = [ctypes.c_int]
+    L.bq_solve_precision.restype = ctypes.c_int
 
 
 def _check(rc: int, what: str) -> None:
@@ -102,3 +106,27 @@ def zgesv(A, B):
     if rc != 0:
         raise RuntimeError(f"zgesv failed ({rc}): {lib().bq_last_error().decode()}")
     return Bf[:, 0] if vec else Bf
+
+
+def cgesv(A, B):
+    """Solve A X = B on the GPU with the library's LU in complex single precision (the
+    correction solver of the mixed-precision solve); returns X as complex64."""
+    A = np.asarray(A, np.complex64)
+    B = np.asarray(B, np.complex64)
+    vec = B.ndim == 1
+    B2 = B.reshape(len(B), -1)
+    n, nrhs = A.shape[0], B2.shape[1]
+    Af = np.asfortranarray(A).copy(order="F")
+    Bf = np.asfortranarray(B2).copy(order="F")
+    _F = ctypes.POINTER(ctypes.c_float)
+    rc = lib().bq_cgesv_host(n, nrhs, Af.ctypes.data_as(_F), Bf.ctypes.data_as(_F))
+    if rc != 0:
+        raise RuntimeError(f"cgesv failed ({rc}): {lib().bq_last_error().decode()}")
+    return Bf[:, 0] if vec else Bf
+
+
+def solve_precision(mode=None):
+    """Query (mode None) or set the precision of the global solve: "fp64" (complex FP64 LU,
+    the default) or "mixed" (complex FP32 LU + FP64 iterative refinement)."""
+    m = {None: -1, "fp64": 0, "mixed": 1}[mode]
+    return ["fp64", "mixed"][lib().bq_solve_precision(m)]

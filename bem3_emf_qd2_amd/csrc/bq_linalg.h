@@ -1,7 +1,7 @@
 /*
  * bq_linalg.h -- device dense complex linear algebra (internal, C ABI).
- * Matrices are column-major interleaved complex doubles in device memory unless the
- * function name says _host.  Streams are passed as void* (a hipStream_t).
+ * Matrices are column-major interleaved complex doubles (z) or floats (c) in device memory
+ * unless the function name says _host.  Streams are passed as void* (a hipStream_t).
  */
 #ifndef BQ_LINALG_H
 #define BQ_LINALG_H
@@ -20,6 +20,15 @@ int bq_zgemm_dev(void *stream, int m, int n, int k, const double *alpha, const d
 int bq_zgemm_op_dev(void *stream, int ta, int tb, int m, int n, int k, const double *alpha, const double *A,
                     long lda, const double *B, long ldb, const double *beta, double *C, long ldc);
 int bq_zgesv_host(int n, int nrhs, double *A, double *B);
+
+/* The same in complex FP32 (interleaved complex floats; dinv holds bq_lu_inv_doubles(n)
+ * floats). */
+int bq_cgetrf_dev(void *stream, int n, float *A, long lda, int *ipiv, float *dinv);
+int bq_cgetrs_dev(void *stream, int n, int nrhs, const float *A, long lda, const int *ipiv, const float *dinv,
+                  float *B, long ldb);
+int bq_cgemm_op_dev(void *stream, int ta, int tb, int m, int n, int k, const float *alpha, const float *A,
+                    long lda, const float *B, long ldb, const float *beta, float *C, long ldc);
+int bq_cgesv_host(int n, int nrhs, float *A, float *B);
 
 #ifdef __cplusplus
 }

@@ -32,6 +32,9 @@ typedef struct bq_stats {
   int n_gh_pairs;        /* pairs that needed the 64x64 rule, all domains */
   long long n_qpgf;      /* QPGF evaluations in the assembly (algorithmic count) */
   long long n_qpgf_gl, n_qpgf_gh, n_qpgf_sing; /* the same split by kernel phase */
+  int mixed;             /* 1: the global system was solved by FP32 factors + FP64 refinement */
+  int ir_iterations;     /* refinement steps taken (mixed) */
+  double ir_correction;  /* last correction, max |dx| / max |x| (mixed) */
 } bq_stats;
 
 typedef struct bq_sess bq_sess;

@@ -1,5 +1,7 @@
-// bq_lu.hip -- dense complex FP64 LU with partial pivoting and the matching solves, on
-// the device, column-major (LAPACK zgetrf/zgetrs semantics, 0-based pivots).
+// bq_lu.hip -- dense complex LU with partial pivoting and the matching solves, on the
+// device, column-major (LAPACK zgetrf/zgetrs and cgetrf/cgetrs semantics, 0-based pivots),
+// in FP64 and FP32 from one templated set of kernels (the FP32 factors precondition the
+// mixed-precision solve).
 //
 // Right-looking LU blocked on two levels.  An outer panel of nb2 = 256 columns is factored
 // in steps of nb = 32 columns:
@@ -12,7 +14,7 @@
 //   5. rocBLAS zgemm updates the rest of the outer panel.
 // Then the panel's interchanges go to all other columns, U12 of the outer block row is
 // solved with the panel's unit L11 (32-column steps), and one rank-256 zgemm updates the
-// trailing matrix (MFMA FP64 on gfx950).  A rank-32 update per 32 columns would stream the
+// trailing matrix (MFMA on gfx950: FP64, or FP32 for the single-precision factors).  A rank-32 update per 32 columns would stream the
 // trailing matrix through HBM 8x as often; at order 36 648 that traffic alone is ~10 s.
 // Solves: the pivots become one permutation (gather), then, per 256-row block, 32-row
 // diagonal steps X_j = inv(T_jj) X_j (k_apply_blk) with in-block zgemm updates, and one
@@ -26,12 +28,76 @@
 
 namespace {
 
+// complex float with the operations the kernels use (as cplx for double)
+struct cplxf {
+  float re, im;
+};
+__device__ __forceinline__ cplxf operator+(cplxf a, cplxf b) { return cplxf{a.re + b.re, a.im + b.im}; }
+__device__ __forceinline__ cplxf operator-(cplxf a, cplxf b) { return cplxf{a.re - b.re, a.im - b.im}; }
+__device__ __forceinline__ cplxf operator*(cplxf a, cplxf b)
+{
+  return cplxf{fmaf(a.re, b.re, -a.im * b.im), fmaf(a.re, b.im, a.im * b.re)};
+}
+__device__ __forceinline__ cplxf cdivf(cplxf a, cplxf b)
+{
+  if (fabsf(b.re) >= fabsf(b.im)) {
+    float r = b.im / b.re, d = b.re + b.im * r;
+    return cplxf{(a.re + a.im * r) / d, (a.im - a.re * r) / d};
+  }
+  float r = b.re / b.im, d = b.im + b.re * r;
+  return cplxf{(a.re * r + a.im) / d, (a.im * r - a.re) / d};
+}
+
+// precision traits: storage pair V, register complex C, conversions and the rocBLAS gemm
+template <class R>
+struct P;
+template <>
+struct P<double> {
+  using V = double2;
+  using C = cplx;
+  static __device__ __forceinline__ C mkc(double r, double i) { return mk(r, i); }
+  static __device__ __forceinline__ C c(V a) { return mk(a.x, a.y); }
+  static __host__ __device__ __forceinline__ V v(double r, double i) { return make_double2(r, i); }
+  static __device__ __forceinline__ V v(C a) { return make_double2(a.re, a.im); }
+  static __device__ __forceinline__ C div(C a, C b) { return cdiv(a, b); }
+  static rocblas_status gemm(rocblas_handle h, rocblas_operation ta, rocblas_operation tb, int m, int n, int k,
+                             const V* al, const V* A, int lda, const V* B, int ldb, const V* be, V* Cm, int ldc)
+  {
+    return rocblas_zgemm(h, ta, tb, m, n, k, (const rocblas_double_complex*)al, (const rocblas_double_complex*)A, lda,
+                         (const rocblas_double_complex*)B, ldb, (const rocblas_double_complex*)be,
+                         (rocblas_double_complex*)Cm, ldc);
+  }
+  static constexpr const char* kt_f = "zgetrf";
+  static constexpr const char* kt_p = "zgetrf_panel";
+  static constexpr const char* kt_u = "zgetrf_update";
+  static constexpr const char* kt_s = "zgetrs";
+};
+template <>
+struct P<float> {
+  using V = float2;
+  using C = cplxf;
+  static __device__ __forceinline__ C mkc(float r, float i) { return cplxf{r, i}; }
+  static __device__ __forceinline__ C c(V a) { return cplxf{a.x, a.y}; }
+  static __host__ __device__ __forceinline__ V v(float r, float i) { return make_float2(r, i); }
+  static __device__ __forceinline__ V v(C a) { return make_float2(a.re, a.im); }
+  static __device__ __forceinline__ C div(C a, C b) { return cdivf(a, b); }
+  static rocblas_status gemm(rocblas_handle h, rocblas_operation ta, rocblas_operation tb, int m, int n, int k,
+                             const V* al, const V* A, int lda, const V* B, int ldb, const V* be, V* Cm, int ldc)
+  {
+    return rocblas_cgemm(h, ta, tb, m, n, k, (const rocblas_float_complex*)al, (const rocblas_float_complex*)A, lda,
+                         (const rocblas_float_complex*)B, ldb, (const rocblas_float_complex*)be,
+                         (rocblas_float_complex*)Cm, ldc);
+  }
+  static constexpr const char* kt_f = "cgetrf";
+  static constexpr const char* kt_p = "cgetrf_panel";
+  static constexpr const char* kt_u = "cgetrf_update";
+  static constexpr const char* kt_s = "cgetrs";
+};
+
 constexpr int kNB = 32;
 constexpr int kNB2 = 256;  // outer panel (multiple of kNB)
 constexpr int kBlkElems = 2 * kNB * kNB;  // inv(L11) then inv(U11), complex, per diagonal block
 
-__device__ __forceinline__ double2 c2(cplx a) { return make_double2(a.re, a.im); }
-__device__ __forceinline__ cplx cc(double2 a) { return mk(a.x, a.y); }
 
 // ---- panel: tournament pivoting ---------------------------------------------------------
 // The m x nb panel is split into blocks of 256 rows.  One 1024-thread workgroup runs partial
@@ -50,14 +116,18 @@ constexpr int kTsFan = kTsRows / kNB;  // candidate sets merged per workgroup
 // cout[blk][nb] receives the chosen rows.  When gridDim.x == 1 the call is the last level:
 // F (nb x nb, column-major) receives the eliminated pivot rows (unit L below the diagonal,
 // U on and above) and ipiv[j..j+jb) the LAPACK-style interchanges for those rows.
-__global__ void __launch_bounds__(1024) k_tslu(int mode, int n, const double2* __restrict__ A, long lda, int j, int jb,
+template <class R>
+__global__ void __launch_bounds__(1024) k_tslu(int mode, int n, const typename P<R>::V* __restrict__ A, long lda, int j, int jb,
                                                const int* __restrict__ cin, int nin, int* __restrict__ cout,
-                                               double2* __restrict__ F, int* __restrict__ ipiv)
+                                               typename P<R>::V* __restrict__ F, int* __restrict__ ipiv)
 {
-  __shared__ double vbuf[4];
+  using V = typename P<R>::V;
+  using C = typename P<R>::C;
+  using T = P<R>;
+  __shared__ R vbuf[4];
   __shared__ int ibuf[4];
-  __shared__ double2 prow[kNB];
-  __shared__ double2 lbuf[kTsRows];
+  __shared__ V prow[kNB];
+  __shared__ V lbuf[kTsRows];
   __shared__ int order[kNB];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int g = wave >> 2, q = wave & 3;  // row group, column group
@@ -70,11 +140,11 @@ __global__ void __launch_bounds__(1024) k_tslu(int mode, int n, const double2* _
     const int set = kTsFan * blockIdx.x + slot / kNB, k = slot % kNB;
     row_id = (set < nin && k < jb) ? cin[set * kNB + k] : -1;
   }
-  cplx a[8];
+  C a[8];
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
     const int col = q * 8 + k;
-    a[k] = (row_id >= 0 && col < jb) ? cc(A[(long)(j + col) * lda + row_id]) : mk(0.0, 0.0);
+    a[k] = (row_id >= 0 && col < jb) ? T::c(A[(long)(j + col) * lda + row_id]) : T::mkc(0.0, 0.0);
   }
   bool live = row_id >= 0;
   int mystep = -1;
@@ -83,11 +153,11 @@ __global__ void __launch_bounds__(1024) k_tslu(int mode, int n, const double2* _
     if (c < jb) {
       const int qc = c / 8, kc = c % 8;
       if (q == qc) {  // column owners: argmax over their 64 rows
-        double v = live ? fabs(a[kc].re) + fabs(a[kc].im) : -1.0;
+        R v = live ? fabs(a[kc].re) + fabs(a[kc].im) : -1.0;
         int idx = live ? row_id : 0x7fffffff;
 #pragma unroll
         for (int off = 32; off >= 1; off >>= 1) {
-          const double ov = __shfl_xor(v, off);
+          const R ov = __shfl_xor(v, off);
           const int oi = __shfl_xor(idx, off);
           if (ov > v || (ov == v && oi < idx)) {
             v = ov;
@@ -100,7 +170,7 @@ __global__ void __launch_bounds__(1024) k_tslu(int mode, int n, const double2* _
         }
       }
       __syncthreads();
-      double pv = vbuf[0];
+      R pv = vbuf[0];
       int pi = ibuf[0];
 #pragma unroll
       for (int h = 1; h < 4; ++h)
@@ -112,24 +182,24 @@ __global__ void __launch_bounds__(1024) k_tslu(int mode, int n, const double2* _
       const bool me = pv >= 0.0 && live && row_id == pi;
       if (me) {
 #pragma unroll
-        for (int k = 0; k < 8; ++k) prow[q * 8 + k] = make_double2(a[k].re, a[k].im);
+        for (int k = 0; k < 8; ++k) prow[q * 8 + k] = T::v(a[k].re, a[k].im);
         live = false;
         mystep = c;
       }
       __syncthreads();
       if (pv >= 0.0) {
         if (q == qc && live) {
-          const cplx piv = cc(prow[c]);
-          const cplx l = (piv.re != 0.0 || piv.im != 0.0) ? cdiv(a[kc], piv) : mk(0.0, 0.0);
+          const C piv = T::c(prow[c]);
+          const C l = (piv.re != 0.0 || piv.im != 0.0) ? T::div(a[kc], piv) : T::mkc(0.0, 0.0);
           a[kc] = l;
-          lbuf[slot] = make_double2(l.re, l.im);
+          lbuf[slot] = T::v(l.re, l.im);
         }
         __syncthreads();
         if (live) {
-          const cplx l = cc(lbuf[slot]);
+          const C l = T::c(lbuf[slot]);
 #pragma unroll
           for (int k = 0; k < 8; ++k)
-            if (q * 8 + k > c) a[k] = a[k] - l * cc(prow[q * 8 + k]);
+            if (q * 8 + k > c) a[k] = a[k] - l * T::c(prow[q * 8 + k]);
         }
       }
     }
@@ -139,7 +209,7 @@ __global__ void __launch_bounds__(1024) k_tslu(int mode, int n, const double2* _
   if (gridDim.x == 1) {
     if (mystep >= 0) {
 #pragma unroll
-      for (int k = 0; k < 8; ++k) F[(q * 8 + k) * kNB + mystep] = make_double2(a[k].re, a[k].im);
+      for (int k = 0; k < 8; ++k) F[(q * 8 + k) * kNB + mystep] = T::v(a[k].re, a[k].im);
     }
     if (tid == 0) {
       int pos[kNB];
@@ -161,42 +231,46 @@ __global__ void __launch_bounds__(1024) k_tslu(int mode, int n, const double2* _
 // column-major, identity beyond jb).  The inverses use recursive doubling: with the inverses
 // X11, X22 of the diagonal s x s blocks of a 2s x 2s lower block, X21 = -X22 L21 X11 (upper:
 // X12 = -X11 U12 X22), one element per thread per level.
-__global__ void __launch_bounds__(1024) k_diag_finish(double2* __restrict__ A, long lda, int j, int jb,
-                                                      const double2* __restrict__ F, double2* __restrict__ inv)
+template <class R>
+__global__ void __launch_bounds__(1024) k_diag_finish(typename P<R>::V* __restrict__ A, long lda, int j, int jb,
+                                                      const typename P<R>::V* __restrict__ F, typename P<R>::V* __restrict__ inv)
 {
-  __shared__ double2 Lx[kNB * kNB];
-  __shared__ double2 Ux[kNB * kNB];
-  __shared__ double2 Tm[kNB * kNB];
+  using V = typename P<R>::V;
+  using C = typename P<R>::C;
+  using T = P<R>;
+  __shared__ V Lx[kNB * kNB];
+  __shared__ V Ux[kNB * kNB];
+  __shared__ V Tm[kNB * kNB];
   const int tid = threadIdx.x, r = tid % kNB, c = tid / kNB;
   {
     const bool in = r < jb && c < jb;
-    const double2 f = in ? F[c * kNB + r] : make_double2(r == c ? 1.0 : 0.0, 0.0);
+    const V f = in ? F[c * kNB + r] : T::v(r == c ? 1.0 : 0.0, 0.0);
     if (in) A[(long)(j + c) * lda + j + r] = f;
-    Lx[c * kNB + r] = (c < r) ? f : make_double2(c == r ? 1.0 : 0.0, 0.0);
-    cplx u = (c >= r) ? cc(f) : mk(0.0, 0.0);
-    if (r == c) u = (u.re != 0.0 || u.im != 0.0) ? cdiv(mk(1.0, 0.0), u) : mk(0.0, 0.0);
-    Ux[c * kNB + r] = c2(u);
+    Lx[c * kNB + r] = (c < r) ? f : T::v(c == r ? 1.0 : 0.0, 0.0);
+    C u = (c >= r) ? T::c(f) : T::mkc(0.0, 0.0);
+    if (r == c) u = (u.re != 0.0 || u.im != 0.0) ? T::div(T::mkc(1.0, 0.0), u) : T::mkc(0.0, 0.0);
+    Ux[c * kNB + r] = T::v(u);
   }
   __syncthreads();
   for (int s = 1; s < kNB; s <<= 1) {
     const int b = (r / (2 * s)) * (2 * s), bc = (c / (2 * s)) * (2 * s);
     const bool lo = r >= b + s && c >= b && c < b + s;    // lower, 2s block at b
     const bool up = c >= bc + s && r >= bc && r < bc + s;  // upper, 2s block at bc
-    cplx t = mk(0, 0);
+    C t = T::mkc(0, 0);
     if (lo)
-      for (int k = b; k < b + s; ++k) t = t + cc(Lx[k * kNB + r]) * cc(Lx[c * kNB + k]);  // L21 X11
+      for (int k = b; k < b + s; ++k) t = t + T::c(Lx[k * kNB + r]) * T::c(Lx[c * kNB + k]);  // L21 X11
     if (up)
-      for (int k = bc + s; k < bc + 2 * s; ++k) t = t + cc(Ux[k * kNB + r]) * cc(Ux[c * kNB + k]);  // U12 X22
-    if (lo || up) Tm[c * kNB + r] = c2(t);
+      for (int k = bc + s; k < bc + 2 * s; ++k) t = t + T::c(Ux[k * kNB + r]) * T::c(Ux[c * kNB + k]);  // U12 X22
+    if (lo || up) Tm[c * kNB + r] = T::v(t);
     __syncthreads();
-    t = mk(0, 0);
+    t = T::mkc(0, 0);
     if (lo)
-      for (int k = b + s; k < b + 2 * s; ++k) t = t + cc(Lx[k * kNB + r]) * cc(Tm[c * kNB + k]);  // X22 T
+      for (int k = b + s; k < b + 2 * s; ++k) t = t + T::c(Lx[k * kNB + r]) * T::c(Tm[c * kNB + k]);  // X22 T
     if (up)
-      for (int k = bc; k < bc + s; ++k) t = t + cc(Ux[k * kNB + r]) * cc(Tm[c * kNB + k]);  // X11 T
+      for (int k = bc; k < bc + s; ++k) t = t + T::c(Ux[k * kNB + r]) * T::c(Tm[c * kNB + k]);  // X11 T
     __syncthreads();
-    if (lo) Lx[c * kNB + r] = c2(mk(-t.re, -t.im));
-    if (up) Ux[c * kNB + r] = c2(mk(-t.re, -t.im));
+    if (lo) Lx[c * kNB + r] = T::v(-t.re, -t.im);
+    if (up) Ux[c * kNB + r] = T::v(-t.re, -t.im);
     __syncthreads();
   }
   inv[c * kNB + r] = Lx[c * kNB + r];
@@ -204,40 +278,48 @@ __global__ void __launch_bounds__(1024) k_diag_finish(double2* __restrict__ A, l
 }
 
 // L21 = A21 inv(U11) in place for rows j+jb..n-1 of the panel; 64-row tiles in LDS
-__global__ void __launch_bounds__(256) k_l21(int n, double2* __restrict__ A, long lda, int j, int jb,
-                                             const double2* __restrict__ invU)
+template <class R>
+__global__ void __launch_bounds__(256) k_l21(int n, typename P<R>::V* __restrict__ A, long lda, int j, int jb,
+                                             const typename P<R>::V* __restrict__ invU)
 {
-  __shared__ double2 Us[kNB * kNB];
-  __shared__ double2 As[64 * kNB];
+  using V = typename P<R>::V;
+  using C = typename P<R>::C;
+  using T = P<R>;
+  __shared__ V Us[kNB * kNB];
+  __shared__ V As[64 * kNB];
   const int tid = threadIdx.x;
   const int r0 = j + jb + blockIdx.x * 64;
   for (int q = tid; q < kNB * kNB; q += 256) Us[q] = invU[q];
   for (int q = tid; q < 64 * kNB; q += 256) {
     const int rr = q % 64, c = q / 64;
-    As[q] = (r0 + rr < n && c < jb) ? A[(long)(j + c) * lda + r0 + rr] : make_double2(0.0, 0.0);
+    As[q] = (r0 + rr < n && c < jb) ? A[(long)(j + c) * lda + r0 + rr] : T::v(0.0, 0.0);
   }
   __syncthreads();
   for (int q = tid; q < 64 * kNB; q += 256) {
     const int rr = q % 64, c = q / 64;
     if (r0 + rr >= n || c >= jb) continue;
-    cplx acc = mk(0, 0);
+    C acc = T::mkc(0, 0);
 #pragma unroll 8
-    for (int k = 0; k < kNB; ++k) acc = acc + cc(As[k * 64 + rr]) * cc(Us[c * kNB + k]);
-    A[(long)(j + c) * lda + r0 + rr] = c2(acc);
+    for (int k = 0; k < kNB; ++k) acc = acc + T::c(As[k * 64 + rr]) * T::c(Us[c * kNB + k]);
+    A[(long)(j + c) * lda + r0 + rr] = T::v(acc);
   }
 }
 
 // apply the interchanges ipiv[j..j+jb) to columns col0 .. col0+ncol-1
-__global__ void __launch_bounds__(256) k_swap(double2* __restrict__ A, long lda, int col0, int ncol, int j, int jb,
+template <class R>
+__global__ void __launch_bounds__(256) k_swap(typename P<R>::V* __restrict__ A, long lda, int col0, int ncol, int j, int jb,
                                               const int* __restrict__ ipiv)
 {
+  using V = typename P<R>::V;
+  using C = typename P<R>::C;
+  using T = P<R>;
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncol) return;
-  double2* cp = A + (long)(col0 + c) * lda;
+  V* cp = A + (long)(col0 + c) * lda;
   for (int r = j; r < j + jb; ++r) {
     const int p = ipiv[r];
     if (p != r) {
-      const double2 t = cp[r];
+      const V t = cp[r];
       cp[r] = cp[p];
       cp[p] = t;
     }
@@ -246,25 +328,29 @@ __global__ void __launch_bounds__(256) k_swap(double2* __restrict__ A, long lda,
 
 // B[0:jb, 0:ncol] <- Tm B for a kNB x kNB matrix Tm (column-major, identity beyond jb);
 // 64 columns per workgroup, both operands staged in LDS.
-__global__ void __launch_bounds__(256) k_apply_blk(const double2* __restrict__ Tm, int jb, double2* __restrict__ B,
+template <class R>
+__global__ void __launch_bounds__(256) k_apply_blk(const typename P<R>::V* __restrict__ Tm, int jb, typename P<R>::V* __restrict__ B,
                                                    long ldb, int ncol)
 {
-  __shared__ double2 Ts[kNB * kNB];
-  __shared__ double2 Bs[kNB * 64];
+  using V = typename P<R>::V;
+  using C = typename P<R>::C;
+  using T = P<R>;
+  __shared__ V Ts[kNB * kNB];
+  __shared__ V Bs[kNB * 64];
   const int tid = threadIdx.x, col0 = blockIdx.x * 64;
   for (int q = tid; q < kNB * kNB; q += 256) Ts[q] = Tm[q];
   for (int q = tid; q < kNB * 64; q += 256) {
     const int r = q % kNB, c = q / kNB;
-    Bs[q] = (r < jb && col0 + c < ncol) ? B[(long)(col0 + c) * ldb + r] : make_double2(0.0, 0.0);
+    Bs[q] = (r < jb && col0 + c < ncol) ? B[(long)(col0 + c) * ldb + r] : T::v(0.0, 0.0);
   }
   __syncthreads();
   for (int q = tid; q < kNB * 64; q += 256) {
     const int r = q % kNB, c = q / kNB;
     if (r >= jb || col0 + c >= ncol) continue;
-    cplx acc = mk(0, 0);
+    C acc = T::mkc(0, 0);
 #pragma unroll 8
-    for (int k = 0; k < kNB; ++k) acc = acc + cc(Ts[k * kNB + r]) * cc(Bs[c * kNB + k]);
-    B[(long)(col0 + c) * ldb + r] = c2(acc);
+    for (int k = 0; k < kNB; ++k) acc = acc + T::c(Ts[k * kNB + r]) * T::c(Bs[c * kNB + k]);
+    B[(long)(col0 + c) * ldb + r] = T::v(acc);
   }
 }
 
@@ -284,20 +370,28 @@ __global__ void k_ipiv_perm(int n, const int* __restrict__ ipiv, int* __restrict
 }
 
 // Bt[r, c] = B[perm[r], c]
+template <class R>
 __global__ void __launch_bounds__(256) k_gather_rows(int n, int nrhs, const int* __restrict__ perm,
-                                                     const double2* __restrict__ B, long ldb, double2* __restrict__ Bt)
+                                                     const typename P<R>::V* __restrict__ B, long ldb, typename P<R>::V* __restrict__ Bt)
 {
+  using V = typename P<R>::V;
+  using C = typename P<R>::C;
+  using T = P<R>;
   const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= (long)n * nrhs) return;
   const int r = (int)(q % n), c = (int)(q / n);
   Bt[(long)c * n + r] = B[(long)c * ldb + perm[r]];
 }
 
-__global__ void k_check_diag(int n, const double2* __restrict__ A, long lda, int* __restrict__ info)
+template <class R>
+__global__ void k_check_diag(int n, const typename P<R>::V* __restrict__ A, long lda, int* __restrict__ info)
 {
+  using V = typename P<R>::V;
+  using C = typename P<R>::C;
+  using T = P<R>;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  double2 d = A[(long)i * lda + i];
+  V d = A[(long)i * lda + i];
   if (d.x == 0.0 && d.y == 0.0) atomicMin(info, i + 1);
 }
 
@@ -312,27 +406,166 @@ rocblas_handle handle_for(hipStream_t s)
   return g_handle[dev];
 }
 
-int zgemm_op(hipStream_t s, int ta, int tb, int m, int n, int k, double2 alpha, const double2* A, long lda,
-             const double2* B, long ldb, double2 beta, double2* C, long ldc)
+template <class R>
+int gemm_op(hipStream_t s, int ta, int tb, int m, int n, int k, typename P<R>::V alpha, const typename P<R>::V* A,
+            long lda, const typename P<R>::V* B, long ldb, typename P<R>::V beta, typename P<R>::V* C, long ldc)
 {
   if (m <= 0 || n <= 0 || k <= 0) return 0;
   rocblas_handle h = handle_for(s);
-  rocblas_status st = rocblas_zgemm(h, ta ? rocblas_operation_transpose : rocblas_operation_none,
-                                    tb ? rocblas_operation_transpose : rocblas_operation_none, m, n, k,
-                                    (const rocblas_double_complex*)&alpha, (const rocblas_double_complex*)A, (int)lda,
-                                    (const rocblas_double_complex*)B, (int)ldb, (const rocblas_double_complex*)&beta,
-                                    (rocblas_double_complex*)C, (int)ldc);
+  rocblas_status st = P<R>::gemm(h, ta ? rocblas_operation_transpose : rocblas_operation_none,
+                                 tb ? rocblas_operation_transpose : rocblas_operation_none, m, n, k, &alpha, A,
+                                 (int)lda, B, (int)ldb, &beta, C, (int)ldc);
   if (st != rocblas_status_success) {
-    bq::set_error("rocblas_zgemm failed (%d)", (int)st);
+    bq::set_error("rocblas complex gemm failed (%d)", (int)st);
     return -1;
   }
   return 0;
 }
 
-int zgemm(hipStream_t s, int m, int n, int k, double2 alpha, const double2* A, long lda, const double2* B, long ldb,
-          double2 beta, double2* C, long ldc)
+template <class R>
+int gemm(hipStream_t s, int m, int n, int k, typename P<R>::V alpha, const typename P<R>::V* A, long lda,
+         const typename P<R>::V* B, long ldb, typename P<R>::V beta, typename P<R>::V* C, long ldc)
 {
-  return zgemm_op(s, 0, 0, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+  return gemm_op<R>(s, 0, 0, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+// In-place LU of the n x n column-major matrix A (device); inv receives the inverted
+// diagonal blocks.  Returns 0, a positive k if U(k,k) is exactly zero (1-based), or < 0.
+template <class R>
+int getrf(hipStream_t s, int n, typename P<R>::V* A, long lda, int* ipiv, typename P<R>::V* inv)
+{
+  using V = typename P<R>::V;
+  const V one = P<R>::v(1.0, 0.0), mone = P<R>::v(-1.0, 0.0);
+  const int nleaf = (n + kTsRows - 1) / kTsRows;
+  int* cand = nullptr;
+  V* F = nullptr;
+  BQ_HIP(hipMallocAsync((void**)&cand, sizeof(int) * (size_t)2 * nleaf * kNB, s));
+  BQ_HIP(hipMallocAsync((void**)&F, sizeof(V) * kNB * kNB, s));
+  const int kt_all = bq::ktime_begin(P<R>::kt_f, s);
+  for (int j0 = 0; j0 < n; j0 += kNB2) {
+    const int jb2 = min(kNB2, n - j0), je = j0 + jb2;
+    const int kt_p = bq::ktime_begin(P<R>::kt_p, s);
+    for (int j = j0; j < je; j += kNB) {
+      const int jb = min(kNB, je - j);
+      V* invj = inv + (long)(j / kNB) * kBlkElems;
+      // pivots of the step by tournament; the last level also factors the diagonal block
+      int nset = (n - j + kTsRows - 1) / kTsRows;
+      int* cur = cand;
+      int* nxt = cand + (size_t)nleaf * kNB;
+      k_tslu<R><<<nset, 1024, 0, s>>>(0, n, A, lda, j, jb, nullptr, 0, cur, F, ipiv);
+      while (nset > 1) {
+        const int nout = (nset + kTsFan - 1) / kTsFan;
+        k_tslu<R><<<nout, 1024, 0, s>>>(1, n, A, lda, j, jb, cur, nset, nxt, F, ipiv);
+        int* t = cur;
+        cur = nxt;
+        nxt = t;
+        nset = nout;
+      }
+      k_swap<R><<<(jb2 + 255) / 256, 256, 0, s>>>(A, lda, j0, jb2, j, jb, ipiv);
+      k_diag_finish<R><<<1, 1024, 0, s>>>(A, lda, j, jb, F, invj);
+      const int rest = n - j - jb, wide = je - j - jb;
+      if (rest > 0) k_l21<R><<<(rest + 63) / 64, 256, 0, s>>>(n, A, lda, j, jb, invj + kNB * kNB);
+      BQ_HIP(hipGetLastError());
+      if (wide > 0) {  // the rest of the outer panel
+        k_apply_blk<R><<<(wide + 63) / 64, 256, 0, s>>>(invj, jb, A + (long)(j + jb) * lda + j, lda, wide);
+        BQ_HIP(hipGetLastError());
+        if (gemm<R>(s, rest, wide, jb, mone, A + (long)j * lda + (j + jb), lda, A + (long)(j + jb) * lda + j, lda, one,
+                    A + (long)(j + jb) * lda + (j + jb), lda))
+          return -1;
+      }
+    }
+    // the panel's interchanges on the columns left and right of it
+    if (j0 > 0) k_swap<R><<<(j0 + 255) / 256, 256, 0, s>>>(A, lda, 0, j0, j0, jb2, ipiv);
+    const int right = n - je;
+    if (right > 0) {
+      k_swap<R><<<(right + 255) / 256, 256, 0, s>>>(A, lda, je, right, j0, jb2, ipiv);
+      // U12 of the block row: unit-lower solve with the panel's L11, 32 rows at a time
+      for (int j = j0; j < je; j += kNB) {
+        const int jb = min(kNB, je - j);
+        k_apply_blk<R><<<(right + 63) / 64, 256, 0, s>>>(inv + (long)(j / kNB) * kBlkElems, jb,
+                                                         A + (long)je * lda + j, lda, right);
+        BQ_HIP(hipGetLastError());
+        if (je - j - jb > 0 && gemm<R>(s, je - j - jb, right, jb, mone, A + (long)j * lda + (j + jb), lda,
+                                       A + (long)je * lda + j, lda, one, A + (long)je * lda + (j + jb), lda))
+          return -1;
+      }
+    }
+    bq::ktime_end(kt_p, s);
+    BQ_HIP(hipGetLastError());
+    if (right > 0) {
+      int rc = 0;
+      BQ_KTIME(P<R>::kt_u, s,
+               rc = gemm<R>(s, right, right, jb2, mone, A + (long)j0 * lda + je, lda, A + (long)je * lda + j0, lda, one,
+                            A + (long)je * lda + je, lda));
+      if (rc) return -1;
+    }
+  }
+  bq::ktime_end(kt_all, s);
+  BQ_HIP(hipFreeAsync(cand, s));
+  BQ_HIP(hipFreeAsync(F, s));
+  int* dinfo = nullptr;
+  BQ_HIP(hipMallocAsync((void**)&dinfo, sizeof(int), s));
+  int big = 0x7fffffff;
+  BQ_HIP(hipMemcpyAsync(dinfo, &big, sizeof(int), hipMemcpyHostToDevice, s));
+  k_check_diag<R><<<(n + 255) / 256, 256, 0, s>>>(n, A, lda, dinfo);
+  int info = 0;
+  BQ_HIP(hipMemcpyAsync(&info, dinfo, sizeof(int), hipMemcpyDeviceToHost, s));
+  BQ_HIP(hipFreeAsync(dinfo, s));
+  BQ_HIP(hipStreamSynchronize(s));
+  return info == big ? 0 : info;
+}
+
+// Solve A X = B with the factors from getrf; B is n x nrhs column-major (device).
+template <class R>
+int getrs(hipStream_t s, int n, int nrhs, const typename P<R>::V* A, long lda, const int* ipiv,
+          const typename P<R>::V* inv, typename P<R>::V* B, long ldb)
+{
+  using V = typename P<R>::V;
+  const V one = P<R>::v(1.0, 0.0), mone = P<R>::v(-1.0, 0.0);
+  if (n <= 0 || nrhs <= 0) return 0;
+  const int kt_all = bq::ktime_begin(P<R>::kt_s, s);
+  int* perm = nullptr;
+  V* Bt = nullptr;
+  BQ_HIP(hipMallocAsync((void**)&perm, sizeof(int) * (size_t)n, s));
+  BQ_HIP(hipMallocAsync((void**)&Bt, sizeof(V) * (size_t)n * nrhs, s));
+  k_ipiv_perm<<<1, 1, 0, s>>>(n, ipiv, perm);
+  const long tot = (long)n * nrhs;
+  k_gather_rows<R><<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(n, nrhs, perm, B, ldb, Bt);
+  BQ_HIP(hipGetLastError());
+  BQ_HIP(hipMemcpy2DAsync(B, ldb * sizeof(V), Bt, (size_t)n * sizeof(V), (size_t)n * sizeof(V), nrhs,
+                          hipMemcpyDeviceToDevice, s));
+  BQ_HIP(hipFreeAsync(Bt, s));
+  BQ_HIP(hipFreeAsync(perm, s));
+  const unsigned cblk = (unsigned)((nrhs + 63) / 64);
+  // forward: unit lower L, 256-row blocks of 32-row steps
+  for (int j0 = 0; j0 < n; j0 += kNB2) {
+    const int je = min(n, j0 + kNB2);
+    for (int j = j0; j < je; j += kNB) {
+      const int jb = min(kNB, je - j);
+      k_apply_blk<R><<<cblk, 256, 0, s>>>(inv + (long)(j / kNB) * kBlkElems, jb, B + j, ldb, nrhs);
+      BQ_HIP(hipGetLastError());
+      if (je - j - jb > 0 &&
+          gemm<R>(s, je - j - jb, nrhs, jb, mone, A + (long)j * lda + (j + jb), lda, B + j, ldb, one, B + j + jb, ldb))
+        return -1;
+    }
+    if (n - je > 0 &&
+        gemm<R>(s, n - je, nrhs, je - j0, mone, A + (long)j0 * lda + je, lda, B + j0, ldb, one, B + je, ldb))
+      return -1;
+  }
+  // backward: U, 256-row blocks from the bottom, 32-row steps upwards inside each
+  for (int j0 = ((n - 1) / kNB2) * kNB2; j0 >= 0; j0 -= kNB2) {
+    const int je = min(n, j0 + kNB2);
+    for (int j = ((je - 1 - j0) / kNB) * kNB + j0; j >= j0; j -= kNB) {
+      const int jb = min(kNB, je - j);
+      k_apply_blk<R><<<cblk, 256, 0, s>>>(inv + (long)(j / kNB) * kBlkElems + kNB * kNB, jb, B + j, ldb, nrhs);
+      BQ_HIP(hipGetLastError());
+      if (j > j0 && gemm<R>(s, j - j0, nrhs, jb, mone, A + (long)j * lda + j0, lda, B + j, ldb, one, B + j0, ldb))
+        return -1;
+    }
+    if (j0 > 0 && gemm<R>(s, j0, nrhs, je - j0, mone, A + (long)j0 * lda, lda, B + j0, ldb, one, B, ldb)) return -1;
+  }
+  bq::ktime_end(kt_all, s);
+  return 0;
 }
 
 }  // namespace
@@ -344,15 +577,24 @@ int bq_zgemm_op_dev(void* stream, int ta, int tb, int m, int n, int k, const dou
                     const double* B, long ldb, const double* beta, double* C, long ldc)
 {
   double2 a = make_double2(alpha[0], alpha[1]), b = make_double2(beta[0], beta[1]);
-  return zgemm_op((hipStream_t)stream, ta, tb, m, n, k, a, (const double2*)A, lda, (const double2*)B, ldb, b,
-                  (double2*)C, ldc);
+  return gemm_op<double>((hipStream_t)stream, ta, tb, m, n, k, a, (const double2*)A, lda, (const double2*)B, ldb, b,
+                         (double2*)C, ldc);
 }
 
 int bq_zgemm_dev(void* stream, int m, int n, int k, const double* alpha, const double* A, long lda, const double* B,
                  long ldb, const double* beta, double* C, long ldc)
 {
   double2 a = make_double2(alpha[0], alpha[1]), b = make_double2(beta[0], beta[1]);
-  return zgemm((hipStream_t)stream, m, n, k, a, (const double2*)A, lda, (const double2*)B, ldb, b, (double2*)C, ldc);
+  return gemm<double>((hipStream_t)stream, m, n, k, a, (const double2*)A, lda, (const double2*)B, ldb, b,
+                      (double2*)C, ldc);
+}
+
+int bq_cgemm_op_dev(void* stream, int ta, int tb, int m, int n, int k, const float* alpha, const float* A, long lda,
+                    const float* B, long ldb, const float* beta, float* C, long ldc)
+{
+  float2 a = make_float2(alpha[0], alpha[1]), b = make_float2(beta[0], beta[1]);
+  return gemm_op<float>((hipStream_t)stream, ta, tb, m, n, k, a, (const float2*)A, lda, (const float2*)B, ldb, b,
+                        (float2*)C, ldc);
 }
 
 long bq_lu_inv_doubles(int n)
@@ -365,142 +607,29 @@ long bq_lu_inv_doubles(int n)
 // Returns 0, a positive k if U(k,k) is exactly zero (1-based, like LAPACK), or < 0.
 int bq_zgetrf_dev(void* stream, int n, double* Ad, long lda, int* ipiv, double* dinv)
 {
-  hipStream_t s = (hipStream_t)stream;
-  double2* A = (double2*)Ad;
-  double2* inv = (double2*)dinv;
-  const double2 one = make_double2(1.0, 0.0), mone = make_double2(-1.0, 0.0);
-  const int nleaf = (n + kTsRows - 1) / kTsRows;
-  int* cand = nullptr;
-  double2* F = nullptr;
-  BQ_HIP(hipMallocAsync((void**)&cand, sizeof(int) * (size_t)2 * nleaf * kNB, s));
-  BQ_HIP(hipMallocAsync((void**)&F, sizeof(double2) * kNB * kNB, s));
-  const int kt_all = bq::ktime_begin("zgetrf", s);
-  for (int j0 = 0; j0 < n; j0 += kNB2) {
-    const int jb2 = min(kNB2, n - j0), je = j0 + jb2;
-    const int kt_p = bq::ktime_begin("zgetrf_panel", s);
-    for (int j = j0; j < je; j += kNB) {
-      const int jb = min(kNB, je - j);
-      double2* invj = inv + (long)(j / kNB) * kBlkElems;
-      // pivots of the step by tournament; the last level also factors the diagonal block
-      int nset = (n - j + kTsRows - 1) / kTsRows;
-      int* cur = cand;
-      int* nxt = cand + (size_t)nleaf * kNB;
-      k_tslu<<<nset, 1024, 0, s>>>(0, n, A, lda, j, jb, nullptr, 0, cur, F, ipiv);
-      while (nset > 1) {
-        const int nout = (nset + kTsFan - 1) / kTsFan;
-        k_tslu<<<nout, 1024, 0, s>>>(1, n, A, lda, j, jb, cur, nset, nxt, F, ipiv);
-        int* t = cur;
-        cur = nxt;
-        nxt = t;
-        nset = nout;
-      }
-      k_swap<<<(jb2 + 255) / 256, 256, 0, s>>>(A, lda, j0, jb2, j, jb, ipiv);
-      k_diag_finish<<<1, 1024, 0, s>>>(A, lda, j, jb, F, invj);
-      const int rest = n - j - jb, wide = je - j - jb;
-      if (rest > 0) k_l21<<<(rest + 63) / 64, 256, 0, s>>>(n, A, lda, j, jb, invj + kNB * kNB);
-      BQ_HIP(hipGetLastError());
-      if (wide > 0) {  // the rest of the outer panel
-        k_apply_blk<<<(wide + 63) / 64, 256, 0, s>>>(invj, jb, A + (long)(j + jb) * lda + j, lda, wide);
-        BQ_HIP(hipGetLastError());
-        if (zgemm(s, rest, wide, jb, mone, A + (long)j * lda + (j + jb), lda, A + (long)(j + jb) * lda + j, lda, one,
-                  A + (long)(j + jb) * lda + (j + jb), lda))
-          return -1;
-      }
-    }
-    // the panel's interchanges on the columns left and right of it
-    if (j0 > 0) k_swap<<<(j0 + 255) / 256, 256, 0, s>>>(A, lda, 0, j0, j0, jb2, ipiv);
-    const int right = n - je;
-    if (right > 0) {
-      k_swap<<<(right + 255) / 256, 256, 0, s>>>(A, lda, je, right, j0, jb2, ipiv);
-      // U12 of the block row: unit-lower solve with the panel's L11, 32 rows at a time
-      for (int j = j0; j < je; j += kNB) {
-        const int jb = min(kNB, je - j);
-        k_apply_blk<<<(right + 63) / 64, 256, 0, s>>>(inv + (long)(j / kNB) * kBlkElems, jb, A + (long)je * lda + j,
-                                                      lda, right);
-        BQ_HIP(hipGetLastError());
-        if (je - j - jb > 0 && zgemm(s, je - j - jb, right, jb, mone, A + (long)j * lda + (j + jb), lda,
-                                     A + (long)je * lda + j, lda, one, A + (long)je * lda + (j + jb), lda))
-          return -1;
-      }
-    }
-    bq::ktime_end(kt_p, s);
-    BQ_HIP(hipGetLastError());
-    if (right > 0) {
-      int rc = 0;
-      BQ_KTIME("zgetrf_update", s,
-               rc = zgemm(s, right, right, jb2, mone, A + (long)j0 * lda + je, lda, A + (long)je * lda + j0, lda, one,
-                          A + (long)je * lda + je, lda));
-      if (rc) return -1;
-    }
-  }
-  bq::ktime_end(kt_all, s);
-  BQ_HIP(hipFreeAsync(cand, s));
-  BQ_HIP(hipFreeAsync(F, s));
-  int* dinfo = nullptr;
-  BQ_HIP(hipMallocAsync((void**)&dinfo, sizeof(int), s));
-  int big = 0x7fffffff;
-  BQ_HIP(hipMemcpyAsync(dinfo, &big, sizeof(int), hipMemcpyHostToDevice, s));
-  k_check_diag<<<(n + 255) / 256, 256, 0, s>>>(n, A, lda, dinfo);
-  int info = 0;
-  BQ_HIP(hipMemcpyAsync(&info, dinfo, sizeof(int), hipMemcpyDeviceToHost, s));
-  BQ_HIP(hipFreeAsync(dinfo, s));
-  BQ_HIP(hipStreamSynchronize(s));
-  return info == big ? 0 : info;
+  return getrf<double>((hipStream_t)stream, n, (double2*)Ad, lda, ipiv, (double2*)dinv);
 }
 
 // Solve A X = B with the factors from bq_zgetrf_dev; B is n x nrhs column-major (device).
 int bq_zgetrs_dev(void* stream, int n, int nrhs, const double* Ad, long lda, const int* ipiv, const double* dinv,
                   double* Bd, long ldb)
 {
-  hipStream_t s = (hipStream_t)stream;
-  const double2* A = (const double2*)Ad;
-  const double2* inv = (const double2*)dinv;
-  double2* B = (double2*)Bd;
-  const double2 one = make_double2(1.0, 0.0), mone = make_double2(-1.0, 0.0);
-  if (n <= 0 || nrhs <= 0) return 0;
-  const int kt_all = bq::ktime_begin("zgetrs", s);
-  int* perm = nullptr;
-  double2* Bt = nullptr;
-  BQ_HIP(hipMallocAsync((void**)&perm, sizeof(int) * (size_t)n, s));
-  BQ_HIP(hipMallocAsync((void**)&Bt, sizeof(double2) * (size_t)n * nrhs, s));
-  k_ipiv_perm<<<1, 1, 0, s>>>(n, ipiv, perm);
-  const long tot = (long)n * nrhs;
-  k_gather_rows<<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(n, nrhs, perm, B, ldb, Bt);
-  BQ_HIP(hipGetLastError());
-  BQ_HIP(hipMemcpy2DAsync(B, ldb * sizeof(double2), Bt, (size_t)n * sizeof(double2), (size_t)n * sizeof(double2),
-                          nrhs, hipMemcpyDeviceToDevice, s));
-  BQ_HIP(hipFreeAsync(Bt, s));
-  BQ_HIP(hipFreeAsync(perm, s));
-  const unsigned cblk = (unsigned)((nrhs + 63) / 64);
-  // forward: unit lower L, 256-row blocks of 32-row steps
-  for (int j0 = 0; j0 < n; j0 += kNB2) {
-    const int je = min(n, j0 + kNB2);
-    for (int j = j0; j < je; j += kNB) {
-      const int jb = min(kNB, je - j);
-      k_apply_blk<<<cblk, 256, 0, s>>>(inv + (long)(j / kNB) * kBlkElems, jb, B + j, ldb, nrhs);
-      BQ_HIP(hipGetLastError());
-      if (je - j - jb > 0 &&
-          zgemm(s, je - j - jb, nrhs, jb, mone, A + (long)j * lda + (j + jb), lda, B + j, ldb, one, B + j + jb, ldb))
-        return -1;
-    }
-    if (n - je > 0 &&
-        zgemm(s, n - je, nrhs, je - j0, mone, A + (long)j0 * lda + je, lda, B + j0, ldb, one, B + je, ldb))
-      return -1;
-  }
-  // backward: U, 256-row blocks from the bottom, 32-row steps upwards inside each
-  for (int j0 = ((n - 1) / kNB2) * kNB2; j0 >= 0; j0 -= kNB2) {
-    const int je = min(n, j0 + kNB2);
-    for (int j = ((je - 1 - j0) / kNB) * kNB + j0; j >= j0; j -= kNB) {
-      const int jb = min(kNB, je - j);
-      k_apply_blk<<<cblk, 256, 0, s>>>(inv + (long)(j / kNB) * kBlkElems + kNB * kNB, jb, B + j, ldb, nrhs);
-      BQ_HIP(hipGetLastError());
-      if (j > j0 && zgemm(s, j - j0, nrhs, jb, mone, A + (long)j * lda + j0, lda, B + j, ldb, one, B + j0, ldb))
-        return -1;
-    }
-    if (j0 > 0 && zgemm(s, j0, nrhs, je - j0, mone, A + (long)j0 * lda, lda, B + j0, ldb, one, B, ldb)) return -1;
-  }
-  bq::ktime_end(kt_all, s);
-  return 0;
+  return getrs<double>((hipStream_t)stream, n, nrhs, (const double2*)Ad, lda, ipiv, (const double2*)dinv,
+                       (double2*)Bd, ldb);
+}
+
+// The same in complex FP32: A, B interleaved complex floats; dinv holds bq_lu_inv_doubles(n)
+// floats.
+int bq_cgetrf_dev(void* stream, int n, float* Ad, long lda, int* ipiv, float* dinv)
+{
+  return getrf<float>((hipStream_t)stream, n, (float2*)Ad, lda, ipiv, (float2*)dinv);
+}
+
+int bq_cgetrs_dev(void* stream, int n, int nrhs, const float* Ad, long lda, const int* ipiv, const float* dinv,
+                  float* Bd, long ldb)
+{
+  return getrs<float>((hipStream_t)stream, n, nrhs, (const float2*)Ad, lda, ipiv, (const float2*)dinv, (float2*)Bd,
+                      ldb);
 }
 
 // Host-memory convenience: solve A X = B (A n x n, B n x nrhs, both column-major host).
@@ -523,6 +652,31 @@ int bq_zgesv_host(int n, int nrhs, double* A, double* B)
     BQ_HIP(hipMemcpyAsync(B, dB, bn, hipMemcpyDeviceToHost, s));
     BQ_HIP(hipMemcpyAsync(A, dA, an, hipMemcpyDeviceToHost, s));
   }
+  BQ_HIP(hipStreamSynchronize(s));
+  (void)hipFree(dA);
+  (void)hipFree(dB);
+  (void)hipFree(dp);
+  (void)hipFree(dv);
+  return info;
+}
+
+// The same in complex FP32 (A, B interleaved complex floats, host).
+int bq_cgesv_host(int n, int nrhs, float* A, float* B)
+{
+  if (bq::ensure_device()) return -1;
+  hipStream_t s = bq::stream();
+  float *dA = nullptr, *dB = nullptr, *dv = nullptr;
+  int* dp = nullptr;
+  size_t an = sizeof(float2) * (size_t)n * n, bn = sizeof(float2) * (size_t)n * nrhs;
+  BQ_HIP(hipMalloc((void**)&dA, an));
+  BQ_HIP(hipMalloc((void**)&dB, bn));
+  BQ_HIP(hipMalloc((void**)&dp, sizeof(int) * n));
+  BQ_HIP(hipMalloc((void**)&dv, sizeof(float) * bq_lu_inv_doubles(n)));
+  BQ_HIP(hipMemcpyAsync(dA, A, an, hipMemcpyHostToDevice, s));
+  BQ_HIP(hipMemcpyAsync(dB, B, bn, hipMemcpyHostToDevice, s));
+  int info = bq_cgetrf_dev(s, n, dA, n, dp, dv);
+  if (info == 0) info = bq_cgetrs_dev(s, n, nrhs, dA, n, dp, dv, dB, n);
+  if (info == 0) BQ_HIP(hipMemcpyAsync(B, dB, bn, hipMemcpyDeviceToHost, s));
   BQ_HIP(hipStreamSynchronize(s));
   (void)hipFree(dA);
   (void)hipFree(dB);

@@ -28,6 +28,7 @@
 #include "../bq_linalg.h"
 #include "../bq_plan.h"
 #include <cstdlib>
+#include <cfloat>
 
 struct bq_sess {
   bq_model m;
@@ -185,6 +186,60 @@ __global__ void k_scale_normals(int NN, const double* __restrict__ Y3, const dou
   if (e >= NN) return;
   cplx x = ld2(Y3, e);
   for (int c = 0; c < 3; ++c) st2(T, (long)c * NN + e, x * nn3[3 * e + c]);
+}
+
+// ---- mixed-precision solve helpers (F = double or float, interleaved complex) ----------
+__global__ void k_d2f(long n, const double* __restrict__ a, float* __restrict__ b)
+{
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (float)a[i];
+}
+
+__global__ void k_add_f2d(long n, const float* __restrict__ d, double* __restrict__ x)
+{
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] += (double)d[i];
+}
+
+// W[i, j] = Z[i, j] (n_i . n_j), Z the top NN rows of an n2 x NN matrix (ld 2 NN)
+template <class F>
+__global__ void k_hadamard_t(int NN, const F* __restrict__ Z, const double* __restrict__ nn3, F* __restrict__ W)
+{
+  long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)NN * NN) return;
+  long j = idx / NN, i = idx % NN;
+  const double g = nn3[3 * i] * nn3[3 * j] + nn3[3 * i + 1] * nn3[3 * j + 1] + nn3[3 * i + 2] * nn3[3 * j + 2];
+  const long z = j * 2L * NN + i;
+  W[2 * idx] = (F)(Z[2 * z] * g);
+  W[2 * idx + 1] = (F)(Z[2 * z + 1] * g);
+}
+
+// v[e] = sum_c n_c(e) Y[e, c], Y n2 x 3 (ld 2 NN), top NN rows
+template <class F>
+__global__ void k_normal_dot_t(int NN, const F* __restrict__ Y, const double* __restrict__ nn3, F* __restrict__ v)
+{
+  long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= NN) return;
+  const long ld = 2L * NN;
+  double re = 0.0, im = 0.0;
+  for (int c = 0; c < 3; ++c) {
+    re += (double)Y[2 * ((long)c * ld + e)] * nn3[3 * e + c];
+    im += (double)Y[2 * ((long)c * ld + e) + 1] * nn3[3 * e + c];
+  }
+  v[2 * e] = (F)re;
+  v[2 * e + 1] = (F)im;
+}
+
+// T[e, c] = n_c(e) y3[e], NN x 3 column-major
+template <class F>
+__global__ void k_scale_normals_t(int NN, const F* __restrict__ y3, const double* __restrict__ nn3, F* __restrict__ T)
+{
+  long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= NN) return;
+  for (int c = 0; c < 3; ++c) {
+    T[2 * ((long)c * NN + e)] = (F)(y3[2 * e] * nn3[3 * e + c]);
+    T[2 * ((long)c * NN + e) + 1] = (F)(y3[2 * e + 1] * nn3[3 * e + c]);
+  }
 }
 
 // B (n x n, column-major) = A[live, live], A row-major N x N
@@ -380,6 +435,165 @@ int bq_sess_assemble(bq_sess* S)
   return 0;
 }
 
+// Precision of the global solve: 0 = FP64 LU (default), 1 = FP32 LU + FP64 refinement.
+// The environment variable BQ_MIXED=1 sets the initial mode.
+static int g_precision = -1;
+
+extern "C" int bq_solve_precision(int mode)
+{
+  if (g_precision < 0) {
+    const char* e = getenv("BQ_MIXED");
+    g_precision = (e && atoi(e) != 0) ? 1 : 0;
+  }
+  if (mode == 0 || mode == 1) g_precision = mode;
+  return g_precision;
+}
+
+// Y = (y_0, y_1, y_2, y_3), each of length 2 NN (unknowns, then normal derivatives) -> the
+// reference's X layout: X[c NN + e] (c = 0..3) and X[(c + 4) NN + e]
+static void unpack_global(int NN, const std::vector<double>& Y, double* Xh)
+{
+  const size_t n2 = 2 * (size_t)NN;
+  for (int c = 0; c < 4; ++c)
+    for (int e = 0; e < NN; ++e) {
+      const double* y = Y.data() + 2 * ((size_t)c * n2);
+      Xh[2 * ((size_t)c * NN + e)] = y[2 * e];
+      Xh[2 * ((size_t)c * NN + e) + 1] = y[2 * e + 1];
+      Xh[2 * ((size_t)(c + 4) * NN + e)] = y[2 * (NN + e)];
+      Xh[2 * ((size_t)(c + 4) * NN + e) + 1] = y[2 * (NN + e) + 1];
+    }
+}
+
+// The block system of bq_sess_solve_global,
+//   M y_c + K T_c(y_3) = b_c (c < 3),  L sum_c n_c o (y_c)_top + M3 y_3 = b_3,
+// solved by iterative refinement in FP64 with the same block elimination in complex FP32
+// as the correction solver: FP32 LUs of M and of the Schur complement S = M3 - L W
+// (W = Z_top o n n^T, Z = M^-1 K), residuals with the FP64 blocks (Wilkinson; Carson and
+// Higham, SIAM J. Sci. Comput. 40(2), 2018).  M, M3, R = [K | b_0 b_1 b_2], L, b3 are the
+// FP64 blocks as built (left unchanged).  Returns 0 with Y (4 x 2 NN complex) once the
+// correction max|dx| / max|x| is below 1e-15 or stops shrinking below 1e-10; 1 if it stops
+// shrinking above that, an FP32 factor is singular or overflows (the caller falls back to
+// FP64 factors); < 0 on error.
+static int solve_global_mixed(bq_sess* S, const double* M, const double* M3, const double* R, const double* L,
+                              const double* b3, const double* nn3, std::vector<double>& Y)
+{
+  hipStream_t s = S->s;
+  const int NN = S->NN, n2 = 2 * NN;
+  const size_t sq = (size_t)n2 * n2 * 2, rect = (size_t)n2 * NN * 2, vec4 = (size_t)n2 * 4 * 2;
+  float *M32, *S32, *R32, *L32, *W32, *r32, *v32, *T32, *inv1, *inv2;
+  double *x, *r, *Tx, *vx;
+  int *ip1, *ip2;
+  struct Frees {
+    std::vector<void*> p;
+    ~Frees()
+    {
+      for (void* q : p) (void)hipFree(q);
+    }
+  } held;  // every buffer of this function is released on each return
+  auto fmal = [&](float** p, size_t n) {
+    *p = nullptr;
+    if (hipMalloc((void**)p, n * sizeof(float)) != hipSuccess) return true;
+    held.p.push_back(*p);
+    return false;
+  };
+  auto dmal = [&](auto** p, size_t n) {
+    *p = nullptr;
+    if (dmalloc(p, n)) return true;
+    held.p.push_back((void*)*p);
+    return false;
+  };
+  if (fmal(&M32, sq) || fmal(&S32, sq) || fmal(&R32, rect + (size_t)n2 * 2) || fmal(&L32, rect) ||
+      fmal(&W32, (size_t)NN * NN * 2) || fmal(&r32, vec4) || fmal(&v32, (size_t)NN * 2) || fmal(&T32, (size_t)NN * 6) ||
+      fmal(&inv1, (size_t)bq_lu_inv_doubles(n2)) || fmal(&inv2, (size_t)bq_lu_inv_doubles(n2)) ||
+      dmal(&x, vec4) || dmal(&r, vec4) || dmal(&Tx, (size_t)NN * 6) || dmal(&vx, (size_t)NN * 2) ||
+      dmal(&ip1, (size_t)n2) || dmal(&ip2, (size_t)n2)) {
+    bq::set_error("solve_global: device memory for the mixed-precision solve");
+    return -1;
+  }
+  const float onef[2] = {1, 0}, monef[2] = {-1, 0};
+  const double one[2] = {1, 0}, mone[2] = {-1, 0};
+  // FP32 factors: M, then Z = M^-1 K and S = M3 - L W
+  k_d2f<<<nblk((long)sq, 256), 256, 0, s>>>((long)sq, M, M32);
+  int info = bq_cgetrf_dev(s, n2, M32, n2, ip1, inv1);
+  if (info) return 1;
+  k_d2f<<<nblk((long)rect, 256), 256, 0, s>>>((long)rect, R, R32);
+  if (bq_cgetrs_dev(s, n2, NN, M32, n2, ip1, inv1, R32, n2)) return -1;
+  k_hadamard_t<float><<<nblk((long)NN * NN, 256), 256, 0, s>>>(NN, R32, nn3, W32);
+  k_d2f<<<nblk((long)rect, 256), 256, 0, s>>>((long)rect, L, L32);
+  k_d2f<<<nblk((long)sq, 256), 256, 0, s>>>((long)sq, M3, S32);
+  BQ_HIP(hipGetLastError());
+  if (bq_cgemm_op_dev(s, 0, 0, n2, NN, NN, monef, L32, n2, W32, NN, onef, S32, n2)) return -1;
+  info = bq_cgetrf_dev(s, n2, S32, n2, ip2, inv2);
+  if (info) return 1;
+  // refinement: x = 0, r = b
+  BQ_HIP(hipMemsetAsync(x, 0, vec4 * 8, s));
+  BQ_HIP(hipMemcpyAsync(r, R + rect, (size_t)n2 * 3 * 16, hipMemcpyDeviceToDevice, s));
+  BQ_HIP(hipMemcpyAsync(r + (size_t)n2 * 6, b3, (size_t)n2 * 16, hipMemcpyDeviceToDevice, s));
+  std::vector<double> hx(vec4), hd(vec4);
+  const int kMaxIt = 12;
+  const double kStall = 1e-10;
+  double prev = 1e300;
+  int rc = 1;
+  for (int it = 0; it < kMaxIt; ++it) {
+    // correction d = P r in FP32 (the block elimination), written over r32
+    k_d2f<<<nblk((long)vec4, 256), 256, 0, s>>>((long)vec4, r, r32);
+    if (bq_cgetrs_dev(s, n2, 3, M32, n2, ip1, inv1, r32, n2)) return -1;
+    k_normal_dot_t<float><<<nblk(NN, 256), 256, 0, s>>>(NN, r32, nn3, v32);
+    if (bq_cgemm_op_dev(s, 0, 0, n2, 1, NN, monef, L32, n2, v32, NN, onef, r32 + (size_t)n2 * 6, n2)) return -1;
+    if (bq_cgetrs_dev(s, n2, 1, S32, n2, ip2, inv2, r32 + (size_t)n2 * 6, n2)) return -1;
+    k_scale_normals_t<float><<<nblk(NN, 256), 256, 0, s>>>(NN, r32 + (size_t)n2 * 6, nn3, T32);
+    if (bq_cgemm_op_dev(s, 0, 0, n2, 3, NN, monef, R32, n2, T32, NN, onef, r32, n2)) return -1;
+    k_add_f2d<<<nblk((long)vec4, 256), 256, 0, s>>>((long)vec4, r32, x);
+    BQ_HIP(hipGetLastError());
+    // sizes of the correction and of x
+    std::vector<float> hdf(vec4);
+    BQ_HIP(hipMemcpyAsync(hdf.data(), r32, vec4 * sizeof(float), hipMemcpyDeviceToHost, s));
+    BQ_HIP(hipMemcpyAsync(hx.data(), x, vec4 * 8, hipMemcpyDeviceToHost, s));
+    BQ_HIP(hipStreamSynchronize(s));
+    double dmax = 0.0, xmax = 0.0;
+    bool finite = true;
+    for (size_t i = 0; i < vec4; ++i) {
+      const double d = fabs((double)hdf[i]), a = fabs(hx[i]);
+      finite = finite && d <= DBL_MAX && a <= DBL_MAX;
+      dmax = d > dmax ? d : dmax;
+      xmax = a > xmax ? a : xmax;
+    }
+    if (!finite) {  // an FP32 factor overflowed: the FP64 factors take over
+      rc = 1;
+      break;
+    }
+    const double corr = xmax > 0.0 ? dmax / xmax : 0.0;
+    S->st.ir_iterations = it + 1;
+    S->st.ir_correction = corr;
+    if (it > 0 && corr <= 1e-15) {
+      rc = 0;
+      break;
+    }
+    if (it > 1 && corr > 0.5 * prev) {
+      // Stalled: the correction stopped shrinking.  At a small level that is the rounding
+      // noise of the FP64 residual (the limiting accuracy of refinement, no worse than the
+      // FP64 factors' own); at a large one the FP32 factors are too inaccurate.
+      rc = corr <= kStall ? 0 : 1;
+      break;
+    }
+    prev = corr;
+    // residual r = b - A x with the FP64 blocks
+    BQ_HIP(hipMemcpyAsync(r, R + rect, (size_t)n2 * 3 * 16, hipMemcpyDeviceToDevice, s));
+    BQ_HIP(hipMemcpyAsync(r + (size_t)n2 * 6, b3, (size_t)n2 * 16, hipMemcpyDeviceToDevice, s));
+    if (bq_zgemm_op_dev(s, 0, 0, n2, 3, n2, mone, M, n2, x, n2, one, r, n2)) return -1;
+    k_scale_normals_t<double><<<nblk(NN, 256), 256, 0, s>>>(NN, x + (size_t)n2 * 6, nn3, Tx);
+    if (bq_zgemm_op_dev(s, 0, 0, n2, 3, NN, mone, R, n2, Tx, NN, one, r, n2)) return -1;
+    k_normal_dot_t<double><<<nblk(NN, 256), 256, 0, s>>>(NN, x, nn3, vx);
+    if (bq_zgemm_op_dev(s, 0, 0, n2, 1, NN, mone, L, n2, vx, NN, one, r + (size_t)n2 * 6, n2)) return -1;
+    if (bq_zgemm_op_dev(s, 0, 0, n2, 1, n2, mone, M3, n2, x + (size_t)n2 * 6, n2, one, r + (size_t)n2 * 6, n2))
+      return -1;
+    BQ_HIP(hipGetLastError());
+  }
+  if (rc == 1 && S->st.ir_correction <= kStall && S->st.ir_iterations == kMaxIt) rc = 0;
+  if (rc == 0) Y.assign(hx.begin(), hx.end());
+  return rc;
+}
+
 int bq_sess_solve_global(bq_sess* S, double* Xh)
 {
   hipStream_t s = S->s;
@@ -431,6 +645,30 @@ int bq_sess_solve_global(bq_sess* S, double* Xh)
     rowbase += S->nlive[d];
   }
   const double one[2] = {1, 0}, mone[2] = {-1, 0};
+  S->st.mixed = 0;
+  S->st.ir_iterations = 0;
+  S->st.ir_correction = 0.0;
+  if (bq_solve_precision(-1) == 1) {
+    std::vector<double> Y;
+    int rc = solve_global_mixed(S, M, M3, R, L, b3, nn3, Y);
+    if (rc < 0) return rc;
+    if (rc == 0) {
+      BQ_HIP(hipEventRecord(e1, s));
+      BQ_HIP(hipStreamSynchronize(s));
+      float ms = 0;
+      BQ_HIP(hipEventElapsedTime(&ms, e0, e1));
+      S->st.t_solve_ms = ms;
+      S->st.mixed = 1;
+      unpack_global(NN, Y, Xh);
+      for (void* p : {(void*)M, (void*)M3, (void*)R, (void*)L, (void*)b3, (void*)W, (void*)v, (void*)T, (void*)ip1,
+                      (void*)ip2, (void*)nn3, (void*)inv1, (void*)inv2})
+        (void)hipFree(p);
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      return 0;
+    }
+    // rc > 0: the refinement did not converge; the FP64 factorization below solves it
+  }
   int info = bq_zgetrf_dev(s, n2, M, n2, ip1, inv1);
   if (info) {
     bq::set_error("solve_global: block matrix M is singular (info %d)", info);
@@ -461,14 +699,7 @@ int bq_sess_solve_global(bq_sess* S, double* Xh)
   float ms = 0;
   BQ_HIP(hipEventElapsedTime(&ms, e0, e1));
   S->st.t_solve_ms = ms;
-  for (int c = 0; c < 4; ++c)
-    for (int e = 0; e < NN; ++e) {
-      const double* y = Y.data() + 2 * ((size_t)c * n2);
-      Xh[2 * ((size_t)c * NN + e)] = y[2 * e];
-      Xh[2 * ((size_t)c * NN + e) + 1] = y[2 * e + 1];
-      Xh[2 * ((size_t)(c + 4) * NN + e)] = y[2 * (NN + e)];
-      Xh[2 * ((size_t)(c + 4) * NN + e) + 1] = y[2 * (NN + e) + 1];
-    }
+  unpack_global(NN, Y, Xh);
   for (void* p : {(void*)M, (void*)M3, (void*)R, (void*)L, (void*)b3, (void*)W, (void*)v, (void*)T, (void*)ip1,
                   (void*)ip2, (void*)nn3, (void*)inv1, (void*)inv2})
     (void)hipFree(p);

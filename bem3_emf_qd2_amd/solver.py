@@ -20,7 +20,8 @@ _I = ctypes.POINTER(ctypes.c_int)
 class Stats(ctypes.Structure):
     _fields_ = [("t_assemble_ms", ctypes.c_double), ("t_solve_ms", ctypes.c_double),
                 ("t_fields_ms", ctypes.c_double), ("n_gh_pairs", ctypes.c_int), ("n_qpgf", ctypes.c_longlong),
-                ("n_qpgf_gl", ctypes.c_longlong), ("n_qpgf_gh", ctypes.c_longlong), ("n_qpgf_sing", ctypes.c_longlong)]
+                ("n_qpgf_gl", ctypes.c_longlong), ("n_qpgf_gh", ctypes.c_longlong), ("n_qpgf_sing", ctypes.c_longlong),
+                ("mixed", ctypes.c_int), ("ir_iterations", ctypes.c_int), ("ir_correction", ctypes.c_double)]
 
 
 def L():

@@ -214,6 +214,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=None, help="untimed steps (default per workload: c2 1, c1 2)")
     ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"],
+                    help="global solve: complex FP32 LU + FP64 iterative refinement (default; one "
+                         "untimed FP64 step first gives rel_err_vs_fp64), or the plain complex FP64 LU")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run one device-to-device copy of MIB MiB (calibrates FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -253,6 +256,13 @@ def main():
             return st, E, H
 
         with _QuietStdout():
+            fp64_ref = None
+            if args.precision != "fp64":
+                bq.solve_precision("fp64")
+                _, Er, Hr = step()
+                fp64_ref = np.r_[Er.ravel(), Hr.ravel()]
+                print("bench.py: FP64 reference step done", file=sys.stderr, flush=True)
+            bq.solve_precision(args.precision)
             for i in range(args.warmup):
                 st, E0, H0 = step()
                 print(f"bench.py: warm-up step {i + 1}/{args.warmup} done", file=sys.stderr, flush=True)
@@ -267,9 +277,16 @@ def main():
             elapsed = time.perf_counter() - t_all
         kt = _kernel_times(L)
         L.bq_kernel_timing(0)
+        bq.solve_precision("fp64")
         stats = {"t_assemble_ms": st.t_assemble_ms, "t_solve_ms": st.t_solve_ms, "t_fields_ms": st.t_fields_ms,
                  "n_gh_pairs": st.n_gh_pairs, "n_qpgf": st.n_qpgf, "n_qpgf_gl": st.n_qpgf_gl,
                  "n_qpgf_gh": st.n_qpgf_gh, "n_qpgf_sing": st.n_qpgf_sing}
+        if st.mixed:
+            stats.update({"ir_iterations": st.ir_iterations, "ir_correction": st.ir_correction})
+        rel_fp64 = 0.0
+        if fp64_ref is not None:
+            got = np.r_[E.ravel(), H.ravel()]
+            rel_fp64 = float(np.abs(got - fp64_ref).max() / np.abs(fp64_ref).max())
         ref = np.r_[E0.ravel(), H0.ravel()] if args.warmup else None
         rep = None
         if ref is not None:
@@ -300,8 +317,9 @@ def main():
         "dtype": "f64",
         "data": "synthetic (generated cube-sphere mesh and plane wave)",
         "config": {"workload": w["label"], "quads": quads, "triangles": tris, "element_nodes": NN, "na": 8 * NN,
-                   "probes": w["probes"], "parallelism": "single GPU"},
-        "rel_err_vs_fp64": 0.0,
+                   "probes": w["probes"], "parallelism": "single GPU",
+                   "solve_precision": args.precision},
+        "rel_err_vs_fp64": rel_fp64,
         "repeat_rel_diff": rep,
         "phase_ms": stats,
         "kernels_ms_per_step": {k: v[0] / args.steps for k, v in sorted(kt.items(), key=lambda x: -x[1][0])},

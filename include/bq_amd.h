@@ -65,9 +65,21 @@ int bq_assemble_domain_host(void *qd, int d, double *G, double *H, double *dG, d
 
 /* Counters and phase times (ms, device clock) of the last solve_bieq in this process:
  * returns a pointer to {double t_assemble_ms, t_solve_ms, t_fields_ms; int n_gh_pairs;
- * long long n_qpgf, n_qpgf_gl, n_qpgf_gh, n_qpgf_sing} (QPGF evaluations of the periodic
- * domain, in total and by phase: 12-point rule, 64-point fallback rule, singular rules). */
+ * long long n_qpgf, n_qpgf_gl, n_qpgf_gh, n_qpgf_sing; int mixed, ir_iterations;
+ * double ir_correction} (QPGF evaluations of the periodic domain, in total and by phase:
+ * regular pairs with their distance-chosen Gauss rule, 64-point fallback rule, singular
+ * rules; whether the global system took the mixed-precision path, its refinement steps and
+ * the last correction max|dx| / max|x|). */
 const void *bq_solve_last_stats(void);
+
+/* Precision of the global (open-region) linear solve (no counterpart in the reference,
+ * whose solver is PARDISO / LAPACKE_zgesv in FP64, d3qd2_src/d3qd2_mkl_solver.c:18-148):
+ * mode 0 = complex FP64 LU (default); mode 1 = complex FP32 LU of the same blocks used as
+ * the correction solver of iterative refinement with FP64 residuals, which stops when the
+ * correction is below 1e-15 of the solution or stops shrinking (the rounding floor of the
+ * FP64 residual) below 1e-10, and falls back to the FP64 LU when it stalls above that.  Any other mode only queries.  Returns the mode in force.  The environment
+ * variable BQ_MIXED=1 sets the initial mode.  Process-wide; set it between solves. */
+int bq_solve_precision(int mode);
 
 /* Per-kernel timing (no counterpart in the reference; for benchmarks and profiling):
  * bq_kernel_timing(1) turns on HIP-event timing of every kernel launch the library makes

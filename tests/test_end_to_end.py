@@ -73,6 +73,54 @@ def test_gpu_solve_matches_reference_dat(tmp_path, gpu_lib, case):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("case", ["sample3/ex3.dat", "sphere_m2/ex.dat", "cone_m1/ex2.dat"])
+def test_gpu_mixed_precision_solve_matches_reference_dat(tmp_path, gpu_lib, case):
+    """The global system solved with FP32 LU factors and FP64 iterative refinement reaches
+    the same 1e-10 agreement with the reference's results as the FP64 LU, and agrees with
+    the FP64 LU solution itself to 1e-11."""
+    from bem3_emf_qd2_amd.solver import Model
+    ref = read_dat(G / case)
+    sols = {}
+    for mode in ("fp64", "mixed"):
+        gpu_lib.solve_precision(mode)
+        try:
+            m = Model.from_dat(G / case)
+            st = m.solve()
+            m.write(tmp_path / f"{mode}.dat")
+            m.close()
+        finally:
+            gpu_lib.solve_precision("fp64")
+        print(case, mode, f"solve {st.t_solve_ms:.1f} ms mixed {st.mixed} it {st.ir_iterations} "
+              f"corr {st.ir_correction:.2e}")
+        assert st.mixed == (mode == "mixed")
+        sols[mode] = read_dat(tmp_path / f"{mode}.dat")
+    assert gpu_lib.solve_precision() == "fp64"
+    _compare(sols["mixed"], ref, TOL_CASE.get(case, TOL))
+    _compare(sols["mixed"], sols["fp64"], 1e-11)
+
+
+@pytest.mark.gpu
+def test_gpu_fields_follow_a_new_solve(gpu_lib):
+    """Fields evaluated after solve_bieq use the new solution, not the device copy of the
+    solution the model held before (the arrays are rewritten in place): a model read with the
+    reference's solution, evaluated, then solved again must give exactly the fields of a
+    freshly read and solved model."""
+    from bem3_emf_qd2_amd.solver import Model
+    pts = [[0.3, 0.0, 1.1], [0.1, 0.2, -0.9]]
+    m = Model.from_dat(G / "sample3" / "ex3.dat")
+    E_ref, H_ref, _ = m.fields(pts, 1)
+    m.solve()
+    E_re, H_re, _ = m.fields(pts, 1)
+    m.close()
+    m2 = Model.from_dat(G / "sample3" / "ex3.dat")
+    m2.solve()
+    E_new, H_new, _ = m2.fields(pts, 1)
+    m2.close()
+    assert not np.array_equal(E_ref, E_new)  # the two solutions differ in the last digits
+    assert np.array_equal(E_re, E_new) and np.array_equal(H_re, H_new)
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("case", ["sample3", "verification", "sphere_m2", "cone_m1"])
 def test_gpu_fields_on_reference_solution(gpu_lib, case):
     """EH_t on the reference's own solution vs the output of the reference example1 (15 digits)."""

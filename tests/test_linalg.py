@@ -52,3 +52,34 @@ def test_gpu_zgesv_pivots_across_outer_panels(gpu_lib):
     bw = np.abs(A @ X - B).max() / (np.abs(A).max() * np.abs(X).max() * n)
     assert bw < 1e-13
     assert np.abs(X - ref).max() / np.abs(ref).max() < 1e-8
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n,nrhs", [(1, 1), (31, 3), (100, 7), (333, 130), (2053, 64)])
+def test_gpu_cgesv_random(gpu_lib, n, nrhs):
+    """The complex FP32 LU (the correction solver of the mixed-precision global solve):
+    backward error at the level of single precision, forward error vs numpy's complex128
+    solve of the same (FP32-rounded) system within cond(A) x 2^-24 x (n + 4)."""
+    rng = np.random.default_rng(7 * n + nrhs)
+    A = (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))).astype(np.complex64).astype(complex)
+    B = (rng.standard_normal((n, nrhs)) + 1j * rng.standard_normal((n, nrhs))).astype(np.complex64).astype(complex)
+    X = gpu_lib.cgesv(A, B).astype(complex)
+    ref = np.linalg.solve(A, B)
+    bw = np.abs(A @ X - B).max() / (np.abs(A).max() * np.abs(X).max() * n)
+    fw = np.abs(X - ref).max() / np.abs(ref).max()
+    cond = np.linalg.cond(A)
+    print(n, nrhs, "backward", bw, "forward", fw, "cond", cond)
+    assert bw < 1e-6
+    assert fw < cond * (n + 4) * 6e-8
+
+
+@pytest.mark.gpu
+def test_gpu_cgesv_needs_pivoting(gpu_lib):
+    rng = np.random.default_rng(5)
+    n = 600
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    A[250:262, 250:262] = 0.0
+    A[0, 0] = 0.0
+    b = rng.standard_normal(n) + 0j
+    x = gpu_lib.cgesv(A, b).astype(complex)
+    assert np.abs(A @ x - b).max() < 1e-5 * np.abs(A).max() * np.abs(x).max() * n
