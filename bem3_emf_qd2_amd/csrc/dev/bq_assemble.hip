@@ -124,8 +124,9 @@ __global__ void __launch_bounds__(256) k_classify(const double* __restrict__ E, 
 // the target element's bounding sphere (D = sqrt2 (1 + gap / a), rho_s = D + sqrt(D^2 - 1),
 // a the source's bounding radius), M grows like exp(k a rho / (2 sqrt2)) from the wave factor,
 // and a further factor 4 covers the growth of 1/R on the ellipse.  The pair takes the
-// smallest order of 6, 8, 10 whose bound is below 1e-18 of the integrand, else 12; with
-// full != 0 every pair takes 12.
+// smallest order of 6, 8, 10 whose bound is below 1e-15 of the integrand (under the
+// rounding of the 12-point sum and the ~1e-14 truncation noise of the lattice sums
+// themselves), else 12; with full != 0 every pair takes 12.
 __device__ __forceinline__ void elem_sphere(const Elem& e, double (&c)[3], double& rad)
 {
   const double zc[4] = {e.tri ? -0.5 : -1.0, 1.0, e.tri ? -0.5 : 1.0, -1.0};
@@ -164,13 +165,22 @@ __device__ __forceinline__ int pair_rule(const bq_qplan& P, const Elem& et, cons
     }
   const double gap = sqrt(d2) - rt - rs;
   if (!(gap > 0.0) || !(rs > 0.0)) return 12;
+  // Horizontal coplanar pairs (flat source in a plane z = const through the target's nodes,
+  // to 1e-6 of the element size): there R_z is 0 or within a few ulps of it at every Gauss
+  // point, where the reference's spectral sum stops on a different test than elsewhere (its
+  // r_z == 0 branch), so its integrand is not the smooth function the bound assumes.  They
+  // keep the reference's 12-point rule and its Gauss points.
+  const double zs = fabs(es.cr[2][1]) + fabs(es.cr[2][2]) + fabs(es.cr[2][3]);
+  const double zt = fabs(et.cr[2][1]) + fabs(et.cr[2][2]) + fabs(et.cr[2][3]);
+  if (zs <= 1e-6 * rs && fabs(et.cr[2][0] - es.cr[2][0]) <= zs + zt + 1e-6 * rs) return 12;
   const double D = 1.4142135623730950488 * (1.0 + gap / rs), rho_s = D + sqrt(D * D - 1.0);
   const double c = P.k * rs * 0.35355339059327376220;  // k a / (2 sqrt 2)
   for (int n = 6; n <= 10; n += 2) {
     const double rho = fmin((2.0 * n + 2.0) / fmax(c, 1e-300), 0.5 * rho_s);
     if (!(rho > 1.05)) continue;
-    // log(4 x 4.27) + c rho - 2n log rho - log(rho^2 - 1) <= log(1e-18)
-    if (2.8379 + c * rho - 2.0 * n * log(rho) - log(rho * rho - 1.0) <= -41.4465) return n;
+    // log(4 x 4.27) + c rho - 2n log rho - log(rho^2 - 1) <= log(1e-15): below the rounding
+    // of the 144-point sum the 12-point rule forms (sqrt(144) x 2^-53 = 1.3e-15)
+    if (2.8379 + c * rho - 2.0 * n * log(rho) - log(rho * rho - 1.0) <= -34.5388) return n;
   }
   return 12;
 }
