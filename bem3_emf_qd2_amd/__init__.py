@@ -127,6 +127,7 @@ def cgesv(A, B):
 
 def solve_precision(mode=None):
     """Query (mode None) or set the precision of the global solve: "fp64" (complex FP64 LU,
-    the default) or "mixed" (complex FP32 LU + FP64 iterative refinement)."""
-    m = {None: -1, "fp64": 0, "mixed": 1}[mode]
-    return ["fp64", "mixed"][lib().bq_solve_precision(m)]
+    the default), "mixed" (complex FP32 LU + FP64 iterative refinement) or "auto" (mixed for
+    block systems of order 2 NN >= 16384, FP64 below)."""
+    m = {None: -1, "fp64": 0, "mixed": 1, "auto": 2}[mode]
+    return ["fp64", "mixed", "auto"][lib().bq_solve_precision(m)]

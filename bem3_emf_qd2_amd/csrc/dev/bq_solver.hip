@@ -435,17 +435,21 @@ int bq_sess_assemble(bq_sess* S)
   return 0;
 }
 
-// Precision of the global solve: 0 = FP64 LU (default), 1 = FP32 LU + FP64 refinement.
-// The environment variable BQ_MIXED=1 sets the initial mode.
+// Precision of the global solve: 0 = FP64 LU (default), 1 = FP32 LU + FP64 refinement,
+// 2 = mode 1 for block systems of order 2 NN >= kMixedMinOrder, else mode 0 (below it the LUs
+// are bound by the latency of their panel chain, not by flops, and the refinement's host
+// round trips cost more than the FP32 factors save).  BQ_MIXED=1 or 2 sets the initial mode.
 static int g_precision = -1;
+static const int kMixedMinOrder = 16384;
 
 extern "C" int bq_solve_precision(int mode)
 {
   if (g_precision < 0) {
     const char* e = getenv("BQ_MIXED");
-    g_precision = (e && atoi(e) != 0) ? 1 : 0;
+    const int v = e ? atoi(e) : 0;
+    g_precision = (v == 1 || v == 2) ? v : 0;
   }
-  if (mode == 0 || mode == 1) g_precision = mode;
+  if (mode >= 0 && mode <= 2) g_precision = mode;
   return g_precision;
 }
 
@@ -648,7 +652,8 @@ int bq_sess_solve_global(bq_sess* S, double* Xh)
   S->st.mixed = 0;
   S->st.ir_iterations = 0;
   S->st.ir_correction = 0.0;
-  if (bq_solve_precision(-1) == 1) {
+  const int prec = bq_solve_precision(-1);
+  if (prec == 1 || (prec == 2 && n2 >= kMixedMinOrder)) {
     std::vector<double> Y;
     int rc = solve_global_mixed(S, M, M3, R, L, b3, nn3, Y);
     if (rc < 0) return rc;

@@ -44,13 +44,13 @@ FP64_PEAK_TFS = 78.6
 # (rocprofv3 PMC) over its device-counted evaluations; C2's figures are read on the same scene
 # at reduced resolution (workload c2s: same lattice, wavelength, materials and incidence).
 # profiles/k_q0_gl_c1_pmc.md and profiles/k_q0_gl_c2s_pmc.md hold the counter values.
-FLOPS_PER_EVAL = {"c1": {"k_q0_gl": 57359, "k_q0_gh": 51625, "k_q0_sing": 50274},
-                  "c2": {"k_q0_gl": 49048, "k_q0_gh": 50007, "k_q0_sing": 50175}}
+FLOPS_PER_EVAL = {"c1": {"k_q0_gl": 55999, "k_q0_gh": 51625, "k_q0_sing": 50274},
+                  "c2": {"k_q0_gl": 51253, "k_q0_gh": 50007, "k_q0_sing": 50175}}
 FLOPS_PER_EVAL["c2s"] = FLOPS_PER_EVAL["c2"]
 # HBM bytes per evaluation of k_q0_gl: (FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00) KiB x 1024 of one
 # launch over its evaluations (factors from a 1 GiB copy read with the same counters)
-TRAFFIC_PER_EVAL = {"c1": {"k_q0_gl": (2049625.8 * 2.0 + 12094762.1) * 1024 / 58796160},
-                    "c2": {"k_q0_gl": (16364526.0 * 2.0 + 89938885.0) * 1024 / 509034196}}
+TRAFFIC_PER_EVAL = {"c1": {"k_q0_gl": (1172551.7 * 2.0 + 9548425.0) * 1024 / 46912896},
+                    "c2": {"k_q0_gl": (12422251.1 * 2.0 + 72211034.4) * 1024 / 385592752}}
 TRAFFIC_PER_EVAL["c2s"] = TRAFFIC_PER_EVAL["c2"]
 
 WORKLOADS = {
@@ -214,9 +214,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=None, help="untimed steps (default per workload: c2 1, c1 2)")
     ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"],
-                    help="global solve: complex FP32 LU + FP64 iterative refinement (default; one "
-                         "untimed FP64 step first gives rel_err_vs_fp64), or the plain complex FP64 LU")
+    ap.add_argument("--precision", default="auto", choices=["auto", "fp64", "mixed"],
+                    help="global solve: complex FP32 LU + FP64 iterative refinement ('mixed'; one "
+                         "untimed FP64 step first gives rel_err_vs_fp64), the plain complex FP64 LU "
+                         "('fp64'), or 'auto' (default): mixed for block systems of order >= 16384")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run one device-to-device copy of MIB MiB (calibrates FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -281,7 +282,8 @@ def main():
         stats = {"t_assemble_ms": st.t_assemble_ms, "t_solve_ms": st.t_solve_ms, "t_fields_ms": st.t_fields_ms,
                  "n_gh_pairs": st.n_gh_pairs, "n_qpgf": st.n_qpgf, "n_qpgf_gl": st.n_qpgf_gl,
                  "n_qpgf_gh": st.n_qpgf_gh, "n_qpgf_sing": st.n_qpgf_sing}
-        if st.mixed:
+        mixed_used = bool(st.mixed)
+        if mixed_used:
             stats.update({"ir_iterations": st.ir_iterations, "ir_correction": st.ir_correction})
         rel_fp64 = 0.0
         if fp64_ref is not None:
@@ -318,7 +320,7 @@ def main():
         "data": "synthetic (generated cube-sphere mesh and plane wave)",
         "config": {"workload": w["label"], "quads": quads, "triangles": tris, "element_nodes": NN, "na": 8 * NN,
                    "probes": w["probes"], "parallelism": "single GPU",
-                   "solve_precision": args.precision},
+                   "solve_precision": "mixed" if mixed_used else "fp64"},
         "rel_err_vs_fp64": rel_fp64,
         "repeat_rel_diff": rep,
         "phase_ms": stats,

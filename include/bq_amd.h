@@ -77,8 +77,11 @@ const void *bq_solve_last_stats(void);
  * mode 0 = complex FP64 LU (default); mode 1 = complex FP32 LU of the same blocks used as
  * the correction solver of iterative refinement with FP64 residuals, which stops when the
  * correction is below 1e-15 of the solution or stops shrinking (the rounding floor of the
- * FP64 residual) below 1e-10, and falls back to the FP64 LU when it stalls above that.  Any other mode only queries.  Returns the mode in force.  The environment
- * variable BQ_MIXED=1 sets the initial mode.  Process-wide; set it between solves. */
+ * FP64 residual) below 1e-10, and falls back to the FP64 LU when it stalls above that;
+ * mode 2 = mode 1 when the block system's order 2 NN is at least 16384, else mode 0 (smaller
+ * LUs are latency bound and gain nothing from FP32).  Any other mode only queries.  Returns
+ * the mode in force.  The environment variable BQ_MIXED=1 or 2 sets the initial mode.
+ * Process-wide; set it between solves. */
 int bq_solve_precision(int mode);
 
 /* Per-kernel timing (no counterpart in the reference; for benchmarks and profiling):

@@ -102,6 +102,21 @@ def test_gpu_mixed_precision_solve_matches_reference_dat(tmp_path, gpu_lib, case
 
 
 @pytest.mark.gpu
+def test_gpu_auto_precision_keeps_small_systems_in_fp64(gpu_lib):
+    """'auto' takes the FP64 LU for a block system of order 2 NN = 3072 (below 16384)."""
+    from bem3_emf_qd2_amd.solver import Model
+    gpu_lib.solve_precision("auto")
+    try:
+        m = Model.from_dat(G / "sample3" / "ex3.dat")
+        st = m.solve()
+        mixed = st.mixed
+        m.close()
+    finally:
+        gpu_lib.solve_precision("fp64")
+    assert mixed == 0
+
+
+@pytest.mark.gpu
 def test_gpu_fields_follow_a_new_solve(gpu_lib):
     """Fields evaluated after solve_bieq use the new solution, not the device copy of the
     solution the model held before (the arrays are rewritten in place): a model read with the
