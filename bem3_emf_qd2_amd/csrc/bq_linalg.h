@@ -10,7 +10,7 @@ extern "C" {
 #endif
 
 /* LU with partial pivoting; dinv (device, bq_lu_inv_doubles(n) doubles) receives the
- * inverted diagonal blocks that bq_zgetrs_dev uses. */
+ * inverted diagonal blocks and the row permutation that bq_zgetrs_dev uses. */
 long bq_lu_inv_doubles(int n);
 int bq_zgetrf_dev(void *stream, int n, double *A, long lda, int *ipiv, double *dinv);
 int bq_zgetrs_dev(void *stream, int n, int nrhs, const double *A, long lda, const int *ipiv, const double *dinv,

@@ -16,9 +16,10 @@
 // solved with the panel's unit L11 (32-column steps), and one rank-256 zgemm updates the
 // trailing matrix (MFMA on gfx950: FP64, or FP32 for the single-precision factors).  A rank-32 update per 32 columns would stream the
 // trailing matrix through HBM 8x as often; at order 36 648 that traffic alone is ~10 s.
-// Solves: the pivots become one permutation (gather), then, per 256-row block, 32-row
-// diagonal steps X_j = inv(T_jj) X_j (k_apply_blk) with in-block zgemm updates, and one
-// rank-256 zgemm for the rows outside the block.
+// Solves: the permutation getrf stored applies the pivots in one gather, then, per 256-row
+// block, 32-row diagonal steps X_j = inv(T_jj) X_j (k_apply_blk) with in-block zgemm
+// updates, and one rank-256 zgemm for the rows outside the block; with at most 8 right-hand
+// sides one kernel runs a whole diagonal block and one skinny kernel the rows outside it.
 // Replaces the reference's MKL PARDISO and LAPACKE_zgesv calls
 // (d3qd2_src/d3qd2_mkl_solver.c:18-148, d3qd2_src/d3qd2_solve_bieq.c:753).
 #include <rocblas/rocblas.h>
@@ -395,6 +396,129 @@ __global__ void k_check_diag(int n, const typename P<R>::V* __restrict__ A, long
   if (d.x == 0.0 && d.y == 0.0) atomicMin(info, i + 1);
 }
 
+// the row permutation getrf leaves after the inverted diagonal blocks
+template <class V>
+int* perm_of(V* inv, int n)
+{
+  return (int*)(inv + (long)((n + kNB - 1) / kNB) * kBlkElems);
+}
+
+// ---- solves with few right-hand sides (nrhs <= kSmallRhs) --------------------------------
+// The blocked solve below issues two launches per 32-row step; with a handful of columns each
+// launch does almost nothing and the solve is bound by launch latency.  Here one workgroup
+// runs all 32-row steps of a 256-row diagonal block with its rows of B in LDS, and one kernel
+// applies the block's columns of L (or U) to the rows outside it, reading them once.
+constexpr int kSmallRhs = 8;
+
+// rows [j0, je) of B (nrhs columns): LOWER: X = inv(L_blk) X (unit lower, steps downwards);
+// else X = inv(U_blk) X (steps upwards).  inv: the 32x32 inverses getrf stored.
+template <class R, bool LOWER>
+__global__ void __launch_bounds__(256) k_trsv_blk(int n, const typename P<R>::V* __restrict__ A, long lda,
+                                                  const typename P<R>::V* __restrict__ inv,
+                                                  typename P<R>::V* __restrict__ B, long ldb, int nrhs, int j0)
+{
+  using V = typename P<R>::V;
+  using C = typename P<R>::C;
+  using T = P<R>;
+  __shared__ V xs[kSmallRhs * kNB2];
+  __shared__ V ts[kNB * kNB];
+  const int tid = threadIdx.x, je = min(n, j0 + kNB2), nb = je - j0;
+  for (int q = tid; q < nb * nrhs; q += 256) {
+    const int r = q % nb, c = q / nb;
+    xs[c * kNB2 + r] = B[(long)c * ldb + j0 + r];
+  }
+  const int nst = (nb + kNB - 1) / kNB;
+  for (int t = 0; t < nst; ++t) {
+    const int st = LOWER ? t : nst - 1 - t;
+    const int j = j0 + st * kNB, jb = min(kNB, je - j);
+    const V* Tm = inv + (long)(j / kNB) * kBlkElems + (LOWER ? 0 : kNB * kNB);
+    // this step's inverse to LDS and this thread's row of the step's columns to registers,
+    // all loads issued before any is used (the row update has at most 224 rows: one per thread)
+    const int lo = LOWER ? j + jb : j0, hi = LOWER ? je : j;
+    const int rr = lo + tid;
+    const bool rowact = rr < hi;
+    C a[kNB];
+#pragma unroll
+    for (int k = 0; k < kNB; ++k) a[k] = (rowact && k < jb) ? T::c(A[(long)(j + k) * lda + rr]) : T::mkc(0, 0);
+#pragma unroll
+    for (int q = 0; q < kNB * kNB / 256; ++q) ts[q * 256 + tid] = Tm[q * 256 + tid];
+    __syncthreads();
+    const int r = tid % kNB, c = tid / kNB;
+    const bool act = c < nrhs && r < jb;
+    C acc = T::mkc(0, 0);
+    if (act)
+#pragma unroll 8
+      for (int k = 0; k < jb; ++k) acc = acc + T::c(ts[k * kNB + r]) * T::c(xs[c * kNB2 + (j - j0) + k]);
+    __syncthreads();
+    if (act) xs[c * kNB2 + (j - j0) + r] = T::v(acc);
+    __syncthreads();
+    if (rowact) {
+#pragma unroll
+      for (int cc = 0; cc < kSmallRhs; ++cc) {
+        if (cc >= nrhs) continue;
+        C v = T::c(xs[cc * kNB2 + (rr - j0)]);
+#pragma unroll
+        for (int k = 0; k < kNB; ++k) v = v - a[k] * T::c(xs[cc * kNB2 + (j - j0) + min(k, jb - 1)]);
+        xs[cc * kNB2 + (rr - j0)] = T::v(v);
+      }
+    }
+    __syncthreads();
+  }
+  for (int q = tid; q < nb * nrhs; q += 256) {
+    const int r = q % nb, c = q / nb;
+    B[(long)c * ldb + j0 + r] = xs[c * kNB2 + r];
+  }
+}
+
+// Bo[r, c] -= sum_k A[r, k] X[k, c] for r < m, k < kk (<= 256), c < nrhs.  64 rows per
+// workgroup, the k range split over 4 wavefronts and reduced in LDS.  X and Bo are disjoint.
+template <class R>
+__global__ void __launch_bounds__(256) k_skinny_update(int m, int kk, const typename P<R>::V* __restrict__ A, long lda,
+                                                       const typename P<R>::V* __restrict__ X, long ldx, int nrhs,
+                                                       typename P<R>::V* __restrict__ Bo, long ldb)
+{
+  using V = typename P<R>::V;
+  using C = typename P<R>::C;
+  using T = P<R>;
+  __shared__ V xs[kSmallRhs * kNB2];
+  __shared__ V part[3 * 64 * kSmallRhs];
+  const int tid = threadIdx.x, lane = tid & 63, q = tid >> 6;
+  for (int e = tid; e < kk * nrhs; e += 256) {
+    const int k = e % kk, c = e / kk;
+    xs[c * kNB2 + k] = X[(long)c * ldx + k];
+  }
+  __syncthreads();
+  const int r = blockIdx.x * 64 + lane;
+  C acc[kSmallRhs];
+#pragma unroll
+  for (int cc = 0; cc < kSmallRhs; ++cc) acc[cc] = T::mkc(0, 0);
+  if (r < m) {
+    const int k0 = q * 64, k1 = min(kk, k0 + 64);
+#pragma unroll 16
+    for (int k = k0; k < k1; ++k) {
+      const C a = T::c(A[(long)k * lda + r]);
+#pragma unroll
+      for (int cc = 0; cc < kSmallRhs; ++cc)
+        if (cc < nrhs) acc[cc] = acc[cc] + a * T::c(xs[cc * kNB2 + k]);
+    }
+  }
+  if (q > 0) {
+#pragma unroll
+    for (int cc = 0; cc < kSmallRhs; ++cc) part[((q - 1) * kSmallRhs + cc) * 64 + lane] = T::v(acc[cc]);
+  }
+  __syncthreads();
+  if (q == 0 && r < m) {
+#pragma unroll
+    for (int cc = 0; cc < kSmallRhs; ++cc) {
+      if (cc >= nrhs) continue;
+      C t = acc[cc];
+      for (int w = 0; w < 3; ++w) t = t + T::c(part[(w * kSmallRhs + cc) * 64 + lane]);
+      V* o = Bo + (long)cc * ldb + r;
+      *o = T::v(T::c(*o) - t);
+    }
+  }
+}
+
 rocblas_handle g_handle[64];
 
 rocblas_handle handle_for(hipStream_t s)
@@ -500,6 +624,8 @@ int getrf(hipStream_t s, int n, typename P<R>::V* A, long lda, int* ipiv, typena
       if (rc) return -1;
     }
   }
+  k_ipiv_perm<<<1, 1, 0, s>>>(n, ipiv, perm_of(inv, n));
+  BQ_HIP(hipGetLastError());
   bq::ktime_end(kt_all, s);
   BQ_HIP(hipFreeAsync(cand, s));
   BQ_HIP(hipFreeAsync(F, s));
@@ -524,18 +650,36 @@ int getrs(hipStream_t s, int n, int nrhs, const typename P<R>::V* A, long lda, c
   const V one = P<R>::v(1.0, 0.0), mone = P<R>::v(-1.0, 0.0);
   if (n <= 0 || nrhs <= 0) return 0;
   const int kt_all = bq::ktime_begin(P<R>::kt_s, s);
-  int* perm = nullptr;
+  const int* perm = perm_of(inv, n);  // stored by getrf (ipiv is kept for the interface)
+  (void)ipiv;
   V* Bt = nullptr;
-  BQ_HIP(hipMallocAsync((void**)&perm, sizeof(int) * (size_t)n, s));
   BQ_HIP(hipMallocAsync((void**)&Bt, sizeof(V) * (size_t)n * nrhs, s));
-  k_ipiv_perm<<<1, 1, 0, s>>>(n, ipiv, perm);
   const long tot = (long)n * nrhs;
   k_gather_rows<R><<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(n, nrhs, perm, B, ldb, Bt);
   BQ_HIP(hipGetLastError());
   BQ_HIP(hipMemcpy2DAsync(B, ldb * sizeof(V), Bt, (size_t)n * sizeof(V), (size_t)n * sizeof(V), nrhs,
                           hipMemcpyDeviceToDevice, s));
   BQ_HIP(hipFreeAsync(Bt, s));
-  BQ_HIP(hipFreeAsync(perm, s));
+  if (nrhs <= kSmallRhs) {
+    for (int j0 = 0; j0 < n; j0 += kNB2) {
+      const int je = min(n, j0 + kNB2);
+      k_trsv_blk<R, true><<<1, 256, 0, s>>>(n, A, lda, inv, B, ldb, nrhs, j0);
+      if (n - je > 0)
+        k_skinny_update<R><<<(n - je + 63) / 64, 256, 0, s>>>(n - je, je - j0, A + (long)j0 * lda + je, lda, B + j0,
+                                                               ldb, nrhs, B + je, ldb);
+      BQ_HIP(hipGetLastError());
+    }
+    for (int j0 = ((n - 1) / kNB2) * kNB2; j0 >= 0; j0 -= kNB2) {
+      const int je = min(n, j0 + kNB2);
+      k_trsv_blk<R, false><<<1, 256, 0, s>>>(n, A, lda, inv, B, ldb, nrhs, j0);
+      if (j0 > 0)
+        k_skinny_update<R><<<(j0 + 63) / 64, 256, 0, s>>>(j0, je - j0, A + (long)j0 * lda, lda, B + j0, ldb, nrhs, B,
+                                                           ldb);
+      BQ_HIP(hipGetLastError());
+    }
+    bq::ktime_end(kt_all, s);
+    return 0;
+  }
   const unsigned cblk = (unsigned)((nrhs + 63) / 64);
   // forward: unit lower L, 256-row blocks of 32-row steps
   for (int j0 = 0; j0 < n; j0 += kNB2) {
@@ -599,7 +743,8 @@ int bq_cgemm_op_dev(void* stream, int ta, int tb, int m, int n, int k, const flo
 
 long bq_lu_inv_doubles(int n)
 {
-  return (long)((n + kNB - 1) / kNB) * kBlkElems * 2;
+  // inverted diagonal blocks, then the row permutation (n int32, which fit in n reals)
+  return (long)((n + kNB - 1) / kNB) * kBlkElems * 2 + n;
 }
 
 // In-place LU of the n x n column-major matrix A (device).  ipiv: device int[n]; dinv: device

@@ -526,7 +526,11 @@ static int solve_global_mixed(bq_sess* S, const double* M, const double* M3, con
   k_d2f<<<nblk((long)rect, 256), 256, 0, s>>>((long)rect, L, L32);
   k_d2f<<<nblk((long)sq, 256), 256, 0, s>>>((long)sq, M3, S32);
   BQ_HIP(hipGetLastError());
-  if (bq_cgemm_op_dev(s, 0, 0, n2, NN, NN, monef, L32, n2, W32, NN, onef, S32, n2)) return -1;
+  {
+    const int kt = bq::ktime_begin("schur_cgemm", s);
+    if (bq_cgemm_op_dev(s, 0, 0, n2, NN, NN, monef, L32, n2, W32, NN, onef, S32, n2)) return -1;
+    bq::ktime_end(kt, s);
+  }
   info = bq_cgetrf_dev(s, n2, S32, n2, ip2, inv2);
   if (info) return 1;
   // refinement: x = 0, r = b
@@ -538,6 +542,7 @@ static int solve_global_mixed(bq_sess* S, const double* M, const double* M3, con
   const double kStall = 1e-10;
   double prev = 1e300;
   int rc = 1;
+  const int kt_ir = bq::ktime_begin("refinement", s);
   for (int it = 0; it < kMaxIt; ++it) {
     // correction d = P r in FP32 (the block elimination), written over r32
     k_d2f<<<nblk((long)vec4, 256), 256, 0, s>>>((long)vec4, r, r32);
@@ -593,6 +598,7 @@ static int solve_global_mixed(bq_sess* S, const double* M, const double* M3, con
       return -1;
     BQ_HIP(hipGetLastError());
   }
+  bq::ktime_end(kt_ir, s);
   if (rc == 1 && S->st.ir_correction <= kStall && S->st.ir_iterations == kMaxIt) rc = 0;
   if (rc == 0) Y.assign(hx.begin(), hx.end());
   return rc;
@@ -637,6 +643,7 @@ int bq_sess_solve_global(bq_sess* S, double* Xh)
   BQ_HIP(hipMemsetAsync(R, 0, (size_t)n2 * (NN + 3) * 16, s));
   BQ_HIP(hipMemsetAsync(L, 0, rect * 8, s));
   int rowbase = 0;
+  const int kt_b = bq::ktime_begin("build_blocks", s);
   for (int d = 0; d <= S->MN; ++d) {
     const long dNe = S->dNe[d], N = 4 * dNe;
     if (dNe == 0) continue;
@@ -648,6 +655,7 @@ int bq_sess_solve_global(bq_sess* S, double* Xh)
     BQ_HIP(hipGetLastError());
     rowbase += S->nlive[d];
   }
+  bq::ktime_end(kt_b, s);
   const double one[2] = {1, 0}, mone[2] = {-1, 0};
   S->st.mixed = 0;
   S->st.ir_iterations = 0;
