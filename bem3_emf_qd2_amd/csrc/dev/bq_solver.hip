@@ -545,6 +545,7 @@ static int solve_global_mixed(bq_sess* S, const double* M, const double* M3, con
   const int kt_ir = bq::ktime_begin("refinement", s);
   for (int it = 0; it < kMaxIt; ++it) {
     // correction d = P r in FP32 (the block elimination), written over r32
+    const int kt_c = bq::ktime_begin("ir_correction", s);
     k_d2f<<<nblk((long)vec4, 256), 256, 0, s>>>((long)vec4, r, r32);
     if (bq_cgetrs_dev(s, n2, 3, M32, n2, ip1, inv1, r32, n2)) return -1;
     k_normal_dot_t<float><<<nblk(NN, 256), 256, 0, s>>>(NN, r32, nn3, v32);
@@ -554,6 +555,7 @@ static int solve_global_mixed(bq_sess* S, const double* M, const double* M3, con
     if (bq_cgemm_op_dev(s, 0, 0, n2, 3, NN, monef, R32, n2, T32, NN, onef, r32, n2)) return -1;
     k_add_f2d<<<nblk((long)vec4, 256), 256, 0, s>>>((long)vec4, r32, x);
     BQ_HIP(hipGetLastError());
+    bq::ktime_end(kt_c, s);
     // sizes of the correction and of x
     std::vector<float> hdf(vec4);
     BQ_HIP(hipMemcpyAsync(hdf.data(), r32, vec4 * sizeof(float), hipMemcpyDeviceToHost, s));
@@ -587,6 +589,7 @@ static int solve_global_mixed(bq_sess* S, const double* M, const double* M3, con
     }
     prev = corr;
     // residual r = b - A x with the FP64 blocks
+    const int kt_r = bq::ktime_begin("ir_residual", s);
     BQ_HIP(hipMemcpyAsync(r, R + rect, (size_t)n2 * 3 * 16, hipMemcpyDeviceToDevice, s));
     BQ_HIP(hipMemcpyAsync(r + (size_t)n2 * 6, b3, (size_t)n2 * 16, hipMemcpyDeviceToDevice, s));
     if (bq_zgemm_op_dev(s, 0, 0, n2, 3, n2, mone, M, n2, x, n2, one, r, n2)) return -1;
@@ -597,6 +600,7 @@ static int solve_global_mixed(bq_sess* S, const double* M, const double* M3, con
     if (bq_zgemm_op_dev(s, 0, 0, n2, 1, n2, mone, M3, n2, x + (size_t)n2 * 6, n2, one, r + (size_t)n2 * 6, n2))
       return -1;
     BQ_HIP(hipGetLastError());
+    bq::ktime_end(kt_r, s);
   }
   bq::ktime_end(kt_ir, s);
   if (rc == 1 && S->st.ir_correction <= kStall && S->st.ir_iterations == kMaxIt) rc = 0;
