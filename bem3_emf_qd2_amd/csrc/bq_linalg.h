@@ -20,6 +20,9 @@ int bq_zgemm_dev(void *stream, int m, int n, int k, const double *alpha, const d
 int bq_zgemm_op_dev(void *stream, int ta, int tb, int m, int n, int k, const double *alpha, const double *A,
                     long lda, const double *B, long ldb, const double *beta, double *C, long ldc);
 int bq_zgesv_host(int n, int nrhs, double *A, double *B);
+/* B -= A X for a few columns (nrhs <= 8): A m x k read once; X and B must not overlap. */
+int bq_zgemm_skinny_dev(void *stream, int m, int nrhs, long k, const double *A, long lda, const double *X, long ldx,
+                        double *B, long ldb);
 
 /* The same in complex FP32 (interleaved complex floats; dinv holds bq_lu_inv_doubles(n)
  * floats). */

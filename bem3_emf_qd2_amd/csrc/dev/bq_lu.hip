@@ -519,6 +519,50 @@ __global__ void __launch_bounds__(256) k_skinny_update(int m, int kk, const type
   }
 }
 
+// Bo[r, c] -= sum_k A[r, k] X[k, c] for r < m, k < kk (any length), c < nrhs <= kSmallRhs:
+// a matrix-times-few-vectors product that reads A once.  64 rows per workgroup, the k range
+// split over its 4 wavefronts (reduced in LDS); X is read at the same address by every lane.
+template <class R>
+__global__ void __launch_bounds__(256) k_skinny_gemm(int m, long kk, const typename P<R>::V* __restrict__ A, long lda,
+                                                     const typename P<R>::V* __restrict__ X, long ldx, int nrhs,
+                                                     typename P<R>::V* __restrict__ Bo, long ldb)
+{
+  using V = typename P<R>::V;
+  using C = typename P<R>::C;
+  using T = P<R>;
+  __shared__ V part[3 * 64 * kSmallRhs];
+  const int tid = threadIdx.x, lane = tid & 63, q = tid >> 6;
+  const int r = blockIdx.x * 64 + lane;
+  C acc[kSmallRhs];
+#pragma unroll
+  for (int cc = 0; cc < kSmallRhs; ++cc) acc[cc] = T::mkc(0, 0);
+  const long kq = (kk + 3) / 4, k0 = q * kq, k1 = min(kk, k0 + kq);
+  if (r < m) {
+#pragma unroll 8
+    for (long k = k0; k < k1; ++k) {
+      const C a = T::c(A[k * lda + r]);
+#pragma unroll
+      for (int cc = 0; cc < kSmallRhs; ++cc)
+        if (cc < nrhs) acc[cc] = acc[cc] + a * T::c(X[(long)cc * ldx + k]);
+    }
+  }
+  if (q > 0) {
+#pragma unroll
+    for (int cc = 0; cc < kSmallRhs; ++cc) part[((q - 1) * kSmallRhs + cc) * 64 + lane] = T::v(acc[cc]);
+  }
+  __syncthreads();
+  if (q == 0 && r < m) {
+#pragma unroll
+    for (int cc = 0; cc < kSmallRhs; ++cc) {
+      if (cc >= nrhs) continue;
+      C t = acc[cc];
+      for (int w = 0; w < 3; ++w) t = t + T::c(part[(w * kSmallRhs + cc) * 64 + lane]);
+      V* o = Bo + (long)cc * ldb + r;
+      *o = T::v(T::c(*o) - t);
+    }
+  }
+}
+
 rocblas_handle g_handle[64];
 
 rocblas_handle handle_for(hipStream_t s)
@@ -739,6 +783,22 @@ int bq_cgemm_op_dev(void* stream, int ta, int tb, int m, int n, int k, const flo
   float2 a = make_float2(alpha[0], alpha[1]), b = make_float2(beta[0], beta[1]);
   return gemm_op<float>((hipStream_t)stream, ta, tb, m, n, k, a, (const float2*)A, lda, (const float2*)B, ldb, b,
                         (float2*)C, ldc);
+}
+
+// B -= A X with A m x k (column-major, lda), X k x nrhs (ldx), B m x nrhs (ldb), nrhs <= 8,
+// complex double; X and B must not overlap.  Returns 0, or -1 for nrhs out of range.
+int bq_zgemm_skinny_dev(void* stream, int m, int nrhs, long k, const double* A, long lda, const double* X, long ldx,
+                        double* B, long ldb)
+{
+  if (nrhs < 1 || nrhs > kSmallRhs) {
+    bq::set_error("bq_zgemm_skinny_dev: nrhs %d outside 1..%d", nrhs, kSmallRhs);
+    return -1;
+  }
+  if (m <= 0 || k <= 0) return 0;
+  k_skinny_gemm<double><<<(m + 63) / 64, 256, 0, (hipStream_t)stream>>>(m, k, (const double2*)A, lda,
+                                                                       (const double2*)X, ldx, nrhs, (double2*)B, ldb);
+  BQ_HIP(hipGetLastError());
+  return 0;
 }
 
 long bq_lu_inv_doubles(int n)

@@ -592,13 +592,12 @@ static int solve_global_mixed(bq_sess* S, const double* M, const double* M3, con
     const int kt_r = bq::ktime_begin("ir_residual", s);
     BQ_HIP(hipMemcpyAsync(r, R + rect, (size_t)n2 * 3 * 16, hipMemcpyDeviceToDevice, s));
     BQ_HIP(hipMemcpyAsync(r + (size_t)n2 * 6, b3, (size_t)n2 * 16, hipMemcpyDeviceToDevice, s));
-    if (bq_zgemm_op_dev(s, 0, 0, n2, 3, n2, mone, M, n2, x, n2, one, r, n2)) return -1;
+    if (bq_zgemm_skinny_dev(s, n2, 3, n2, M, n2, x, n2, r, n2)) return -1;
     k_scale_normals_t<double><<<nblk(NN, 256), 256, 0, s>>>(NN, x + (size_t)n2 * 6, nn3, Tx);
-    if (bq_zgemm_op_dev(s, 0, 0, n2, 3, NN, mone, R, n2, Tx, NN, one, r, n2)) return -1;
+    if (bq_zgemm_skinny_dev(s, n2, 3, NN, R, n2, Tx, NN, r, n2)) return -1;
     k_normal_dot_t<double><<<nblk(NN, 256), 256, 0, s>>>(NN, x, nn3, vx);
-    if (bq_zgemm_op_dev(s, 0, 0, n2, 1, NN, mone, L, n2, vx, NN, one, r + (size_t)n2 * 6, n2)) return -1;
-    if (bq_zgemm_op_dev(s, 0, 0, n2, 1, n2, mone, M3, n2, x + (size_t)n2 * 6, n2, one, r + (size_t)n2 * 6, n2))
-      return -1;
+    if (bq_zgemm_skinny_dev(s, n2, 1, NN, L, n2, vx, NN, r + (size_t)n2 * 6, n2)) return -1;
+    if (bq_zgemm_skinny_dev(s, n2, 1, n2, M3, n2, x + (size_t)n2 * 6, n2, r + (size_t)n2 * 6, n2)) return -1;
     BQ_HIP(hipGetLastError());
     bq::ktime_end(kt_r, s);
   }

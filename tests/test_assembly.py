@@ -94,9 +94,10 @@ def test_gpu_assembly_vs_oracle(tmp_path, gpu_lib, case):
 @pytest.mark.gpu
 @pytest.mark.parametrize("case", ["c1_sphere", "hex_two_layer"])
 def test_gpu_distance_adaptive_rule_matches_full_rule(tmp_path, gpu_lib, monkeypatch, case):
-    """Regular periodic pairs far from their target take a 6-, 8- or 10-point Gauss rule
-    where an analytic error bound puts the rule's error below 1e-15 of the integrand; the
-    reference takes 12 points for all of them (BQ_FULL_RULE=1 restores that).  The matrices
+    """Regular pairs far from their target (periodic in the open region, free space in the
+    bounded domains) take a 6-, 8- or 10-point Gauss rule where an analytic error bound puts
+    the rule's error below 1e-15 of the integrand; the reference takes 12 points for all of
+    them (BQ_FULL_RULE=1 restores that).  The matrices
     of both must agree to rounding: 1e-13 of the largest entry, and 1e-11 entry by entry."""
     from bem3_emf_qd2_amd import meshgen
     from bem3_emf_qd2_amd.solver import Model
@@ -106,16 +107,23 @@ def test_gpu_distance_adaptive_rule_matches_full_rule(tmp_path, gpu_lib, monkeyp
         meshgen.sphere_scene(tmp_path, 6, 0.40, 1.45, 0.55, theta=np.radians(30), phi=np.radians(15),
                              d1=(0.9, 0.0), d2=(0.45, 0.779), core=(0.3, 1.6 + 0.01j))
     m = Model.from_dir(tmp_path)
+    pairs = []
     try:
         m.write(tmp_path / "setup.dat")
-        dNe = read_dat(tmp_path / "setup.dat")["dom"][0]["Ne"]
-        monkeypatch.setenv("BQ_FULL_RULE", "1")
-        G12, H12, _, _, _ = m.assemble(0, dNe, deriv=False)
-        monkeypatch.setenv("BQ_FULL_RULE", "0")
-        G, H, _, _, _ = m.assemble(0, dNe, deriv=False)
+        dd = read_dat(tmp_path / "setup.dat")["dom"]
+        # domain 0: the periodic kernel; domain 1: the free-space kernel of a bounded domain,
+        # with its tangential-derivative rows
+        for d, deriv in ((0, False), (1, True)):
+            dNe = dd[d]["Ne"]
+            monkeypatch.setenv("BQ_FULL_RULE", "1")
+            full = m.assemble(d, dNe, deriv=deriv)
+            monkeypatch.setenv("BQ_FULL_RULE", "0")
+            got = m.assemble(d, dNe, deriv=deriv)
+            names = ("G", "H", "dG", "dH") if deriv else ("G", "H")
+            pairs += [(got[i], full[i], f"{nm}{d}") for i, nm in enumerate(names)]
     finally:
         m.close()
-    for a, b, nm in ((G, G12, "G"), (H, H12, "H")):
+    for a, b, nm in pairs:
         glob = _rel(a, b)
         big = np.abs(b) > 1e-6 * np.abs(b).max()
         ent = (np.abs(a - b)[big] / np.abs(b)[big]).max()
