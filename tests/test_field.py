@@ -86,3 +86,92 @@ def test_gpu_example2_field_maps(tmp_path, gpu_lib):
         assert np.abs(Ie[:, 2] - ie).max() <= 1e-13 * ie.max()
         assert np.abs(Ih[:, 2] - ih).max() <= 1e-13 * ih.max()
     m.close()
+
+
+def _read_png(path):
+    """Minimal PNG reader for the example3 images (8-bit RGB, filter 0 rows): checks every
+    chunk's CRC and returns the pixel array [h][w][3]."""
+    import struct
+    import zlib
+    b = Path(path).read_bytes()
+    assert b[:8] == b"\x89PNG\r\n\x1a\n"
+    o, idat, w, h = 8, b"", None, None
+    while o < len(b):
+        n = struct.unpack(">I", b[o:o + 4])[0]
+        typ, data = b[o + 4:o + 8], b[o + 8:o + 8 + n]
+        assert struct.unpack(">I", b[o + 8 + n:o + 12 + n])[0] == zlib.crc32(typ + data)
+        if typ == b"IHDR":
+            w, h, depth, ctype = struct.unpack(">IIBB", data[:10])
+            assert depth == 8 and ctype == 2
+        elif typ == b"IDAT":
+            idat += data
+        o += 12 + n
+    raw = np.frombuffer(zlib.decompress(idat), np.uint8).reshape(h, 3 * w + 1)
+    assert (raw[:, 0] == 0).all()
+    return raw[:, 1:].reshape(h, w, 3)
+
+
+_CT1 = np.array([[0x00, 0x00, 0x90], [0x00, 0x0f, 0xff], [0x00, 0x90, 0xff], [0x0f, 0xff, 0xee], [0xff, 0xff, 0xff],
+                 [0xff, 0xee, 0x00], [0xff, 0x70, 0x00], [0xee, 0x00, 0x00], [0x7f, 0x00, 0x00]])
+
+
+def _color(x):
+    """The reference's colour scale (example3.c:403-439): 2^23 steps over [-1, 1]."""
+    if not -1.0 <= x <= 1.0:
+        return np.zeros(3, int)
+    n = int(np.floor(2.0 ** 23 * (x + 1.0)))
+    nc = 2 ** 21
+    i = n // nc + 1
+    if i > 8:
+        return _CT1[8]
+    nd = n - nc * (i - 1)
+    return np.array([int(np.floor(_CT1[i - 1][c] + (_CT1[i][c] - _CT1[i - 1][c]) * (1.0 / nc) * nd))
+                     for c in range(3)])
+
+
+@pytest.mark.gpu
+def test_gpu_example3_field_images(tmp_path, gpu_lib):
+    """The example3 tool (the reference's example3.c) writes, per plane and phase, the real
+    part of each E and H component times exp(-i omega t) as PNG images in the reference's
+    colour scale, normalised by the component's largest modulus, plus the range files and
+    the colour bar.  The images are decoded here and compared pixel by pixel with the scale
+    applied to EH_t at the same points."""
+    import shutil
+    import subprocess
+    from bem3_emf_qd2_amd.solver import Model
+    exe = Path(__file__).resolve().parent.parent / "bem3_emf_qd2_amd" / "bin" / "example3"
+    dat = tmp_path / "ex3.dat"
+    shutil.copy(Path(__file__).resolve().parent / "golden" / "sample3" / "ex3.dat", dat)
+    mx, ts = 7, 3
+    p = subprocess.run([str(exe), str(dat), str(mx), str(ts)], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=300)
+    assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-2000:]
+    out = tmp_path / "ex3_images"
+    assert len(list(out.glob("*.png"))) == 3 * ts * 6 + 1
+    d = read_dat(dat)
+    lam, omega = d["lambda_0"], d["omega"]
+    m = Model.from_dat(dat)
+    g = np.arange(mx) * (2 * lam / (mx - 1))
+    for pl, (fixed, row, col) in {"yz": (0, 2, 1), "xz": (1, 2, 0), "xy": (2, 1, 0)}.items():
+        r = np.zeros((mx * mx, 3))
+        r[:, row] = np.repeat(lam - g, mx)
+        r[:, col] = np.tile(-lam + g, mx)
+        E, H, _ = m.fields(r, 1)
+        info = (out / f"{pl}_info.txt").read_text().splitlines()
+        for f, V in (("E", E), ("H", H)):
+            mxv = np.abs(V).max(0)
+            for dd, cn in enumerate("xyz"):
+                line = next(l for l in info if l.startswith(f"{f}{cn} is"))
+                assert abs(float(line.split()[-1]) - mxv[dd]) <= 1e-6 * mxv[dd]
+                for nt in (0, ts - 1):
+                    img = _read_png(out / f"{pl}_{f}{cn}_{nt:03d}.png")
+                    assert img.shape == (2 * mx, 2 * mx, 3)
+                    cet = np.exp(-1j * omega * (lam / ts) * nt)
+                    want = np.array([_color((cet * V[k, dd]).real / mxv[dd]) for k in range(mx * mx)])
+                    got = img[::2, ::2].reshape(-1, 3)
+                    # a value within rounding of a colour step may land on either side of it
+                    assert np.abs(got.astype(int) - want).max() <= 1
+                    assert (img[::2, ::2] == img[1::2, 1::2]).all()
+    bar = _read_png(out / "color_bar.png")
+    assert bar.shape == (2 * mx, max(2 * mx // 16, 1), 3)
+    m.close()
