@@ -262,3 +262,76 @@ def sphere_cone_scene(dirpath, n: int = 16, core=(0.3, 1.6 + 0.01j), shell=(0.40
     write_periodicity(d / "periodicity_data.txt", d1, d2)
     write_medium(d / "medium_data.txt", [shell[1], core[1], cone_[2]])
     return d
+
+
+# ---- transforms of an existing Gmsh 2.2 ASCII mesh (the reference's mesh_sample/tools) ----
+
+def _msh_map_nodes(src, dst, fn) -> None:
+    """Copy a Gmsh 2.2 ASCII file, replacing every node's coordinates by fn(x, y, z); node
+    lines are written as the reference's tools write them ('%d' then ' %16.15f' x 3), all
+    other lines unchanged."""
+    lines = Path(src).read_text().splitlines(keepends=True)
+    out = []
+    i = 0
+    while i < len(lines):
+        out.append(lines[i])
+        if lines[i].strip() == "$Nodes":
+            n = int(lines[i + 1])
+            out.append(lines[i + 1])
+            for k in range(n):
+                f = lines[i + 2 + k].split()
+                v = fn(float(f[1]), float(f[2]), float(f[3]))
+                out.append(f[0] + "".join(f" {c:16.15f}" for c in v) + "\n")
+            i += 2 + n
+            continue
+        i += 1
+    Path(dst).write_text("".join(out))
+
+
+def msh_rotate(src, dst, axis, angle: float) -> None:
+    """Rotate every node by `angle` (rad) about the unit vector along `axis` (Rodrigues'
+    formula), as mesh_sample/tools/rotation/main.c:41-93 does."""
+    n = [float(a) for a in axis]
+    a = math.sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2])
+    if a == 0.0:
+        raise ValueError("msh_rotate: the rotation axis is the zero vector")
+    n = [n[0] / a, n[1] / a, n[2] / a]
+    st, ct = math.sin(angle), math.cos(angle)
+    R = [ct + n[0] * n[0] * (1.0 - ct), n[0] * n[1] * (1.0 - ct) - n[2] * st, n[2] * n[0] * (1.0 - ct) + n[1] * st,
+         n[0] * n[1] * (1.0 - ct) + n[2] * st, ct + n[1] * n[1] * (1.0 - ct), n[1] * n[2] * (1.0 - ct) - n[0] * st,
+         n[2] * n[0] * (1.0 - ct) - n[1] * st, n[1] * n[2] * (1.0 - ct) + n[0] * st, ct + n[2] * n[2] * (1.0 - ct)]
+    _msh_map_nodes(src, dst, lambda x, y, z: [R[3 * j] * x + R[3 * j + 1] * y + R[3 * j + 2] * z for j in range(3)])
+
+
+def msh_stretch(src, dst, vector) -> None:
+    """Scale every node by |vector| along the direction of `vector` (the perpendicular part is
+    kept), as mesh_sample/tools/stretch/main.c:27-84 does."""
+    v = [float(c) for c in vector]
+    s = math.sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2])
+    if s == 0.0:
+        raise ValueError("msh_stretch: the stretch vector is the zero vector")
+    u = [v[0] / s, v[1] / s, v[2] / s]
+
+    def f(x, y, z):
+        p = (x, y, z)
+        d = p[0] * u[0] + p[1] * u[1] + p[2] * u[2]
+        return [s * (d * u[j]) + (p[j] - d * u[j]) for j in range(3)]
+
+    _msh_map_nodes(src, dst, f)
+
+
+if __name__ == "__main__":
+    import sys
+    a = sys.argv[1:]
+    if len(a) == 7 and a[0] == "rotate":
+        # python -m bem3_emf_qd2_amd.meshgen rotate in.msh nx ny nz angle_rad out.msh
+        msh_rotate(a[1], a[6], (float(a[2]), float(a[3]), float(a[4])), float(a[5]))
+        print(f"Done writing {a[6]}")
+    elif len(a) == 6 and a[0] == "stretch":
+        # python -m bem3_emf_qd2_amd.meshgen stretch in.msh sx sy sz out.msh
+        msh_stretch(a[1], a[5], (float(a[2]), float(a[3]), float(a[4])))
+        print(f"Done writing {a[5]}")
+    else:
+        print("usage: python -m bem3_emf_qd2_amd.meshgen rotate in.msh nx ny nz angle_rad out.msh\n"
+              "       python -m bem3_emf_qd2_amd.meshgen stretch in.msh sx sy sz out.msh")
+        sys.exit(1)
