@@ -214,10 +214,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=None, help="untimed steps (default per workload: c2 1, c1 2)")
     ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--precision", default="auto", choices=["auto", "fp64", "mixed"],
-                    help="global solve: complex FP32 LU + FP64 iterative refinement ('mixed'; one "
-                         "untimed FP64 step first gives rel_err_vs_fp64), the plain complex FP64 LU "
-                         "('fp64'), or 'auto' (default): mixed for block systems of order >= 16384")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed", "auto"],
+                    help="global solve: the plain complex FP64 LU ('fp64', default: the C1 and C2 "
+                         "configurations are the plain FP64 path), complex FP32 LU + FP64 iterative "
+                         "refinement ('mixed'; one untimed FP64 step first gives rel_err_vs_fp64), or "
+                         "'auto': mixed for block systems of order >= 16384")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run one device-to-device copy of MIB MiB (calibrates FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
