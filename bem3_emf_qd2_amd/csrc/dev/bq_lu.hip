@@ -3,8 +3,8 @@
 // in FP64 and FP32 from one templated set of kernels (the FP32 factors precondition the
 // mixed-precision solve).
 //
-// Right-looking LU blocked on two levels.  An outer panel of nb2 = 256 columns is factored
-// in steps of nb = 32 columns:
+// Right-looking LU blocked on two levels.  An outer panel of 512 columns is factored in
+// steps of nb = 32 columns:
 //   1. k_tslu       chooses the step's nb pivot rows by tournament (partial pivoting on
 //                   256-row blocks in registers, then 8-way reductions of candidate sets);
 //                   the last level also yields the factored diagonal block,
@@ -13,7 +13,7 @@
 //   4. k_l21        L21 = A21 inv(U11),   k_apply_blk  U12 = inv(L11) A12 inside the panel,
 //   5. rocBLAS zgemm updates the rest of the outer panel.
 // Then the panel's interchanges go to all other columns, U12 of the outer block row is
-// solved with the panel's unit L11 (32-column steps), and one rank-256 zgemm updates the
+// solved with the panel's unit L11 (32-column steps), and one rank-512 zgemm updates the
 // trailing matrix (MFMA on gfx950: FP64, or FP32 for the single-precision factors).  A rank-32 update per 32 columns would stream the
 // trailing matrix through HBM 8x as often; at order 36 648 that traffic alone is ~10 s.
 // Solves: the permutation getrf stored applies the pivots in one gather, then, per 256-row
@@ -96,7 +96,8 @@ struct P<float> {
 };
 
 constexpr int kNB = 32;
-constexpr int kNB2 = 256;  // outer panel (multiple of kNB)
+constexpr int kNB2 = 256;  // row blocks of the solves (multiple of kNB)
+constexpr int kNBF = 512;  // outer panel of the factorization (multiple of kNB)
 constexpr int kBlkElems = 2 * kNB * kNB;  // inv(L11) then inv(U11), complex, per diagonal block
 
 
@@ -610,8 +611,8 @@ int getrf(hipStream_t s, int n, typename P<R>::V* A, long lda, int* ipiv, typena
   BQ_HIP(hipMallocAsync((void**)&cand, sizeof(int) * (size_t)2 * nleaf * kNB, s));
   BQ_HIP(hipMallocAsync((void**)&F, sizeof(V) * kNB * kNB, s));
   const int kt_all = bq::ktime_begin(P<R>::kt_f, s);
-  for (int j0 = 0; j0 < n; j0 += kNB2) {
-    const int jb2 = min(kNB2, n - j0), je = j0 + jb2;
+  for (int j0 = 0; j0 < n; j0 += kNBF) {
+    const int jb2 = min(kNBF, n - j0), je = j0 + jb2;
     const int kt_p = bq::ktime_begin(P<R>::kt_p, s);
     for (int j = j0; j < je; j += kNB) {
       const int jb = min(kNB, je - j);
