@@ -16,10 +16,11 @@
 // solved with the panel's unit L11 (32-column steps), and one rank-512 zgemm updates the
 // trailing matrix (MFMA on gfx950: FP64, or FP32 for the single-precision factors).  A rank-32 update per 32 columns would stream the
 // trailing matrix through HBM 8x as often; at order 36 648 that traffic alone is ~10 s.
-// Solves: the permutation getrf stored applies the pivots in one gather, then, per 256-row
+// Solves: the permutation getrf stored applies the pivots in one gather, then, per 512-row
 // block, 32-row diagonal steps X_j = inv(T_jj) X_j (k_apply_blk) with in-block zgemm
-// updates, and one rank-256 zgemm for the rows outside the block; with at most 8 right-hand
-// sides one kernel runs a whole diagonal block and one skinny kernel the rows outside it.
+// updates, and one rank-512 zgemm for the rows outside the block; with at most 8 right-hand
+// sides one kernel runs a whole 256-row diagonal block and one skinny kernel the rows
+// outside it.
 // Replaces the reference's MKL PARDISO and LAPACKE_zgesv calls
 // (d3qd2_src/d3qd2_mkl_solver.c:18-148, d3qd2_src/d3qd2_solve_bieq.c:753).
 #include <rocblas/rocblas.h>
@@ -726,9 +727,9 @@ int getrs(hipStream_t s, int n, int nrhs, const typename P<R>::V* A, long lda, c
     return 0;
   }
   const unsigned cblk = (unsigned)((nrhs + 63) / 64);
-  // forward: unit lower L, 256-row blocks of 32-row steps
-  for (int j0 = 0; j0 < n; j0 += kNB2) {
-    const int je = min(n, j0 + kNB2);
+  // forward: unit lower L, 512-row blocks of 32-row steps
+  for (int j0 = 0; j0 < n; j0 += kNBF) {
+    const int je = min(n, j0 + kNBF);
     for (int j = j0; j < je; j += kNB) {
       const int jb = min(kNB, je - j);
       k_apply_blk<R><<<cblk, 256, 0, s>>>(inv + (long)(j / kNB) * kBlkElems, jb, B + j, ldb, nrhs);
@@ -741,9 +742,9 @@ int getrs(hipStream_t s, int n, int nrhs, const typename P<R>::V* A, long lda, c
         gemm<R>(s, n - je, nrhs, je - j0, mone, A + (long)j0 * lda + je, lda, B + j0, ldb, one, B + je, ldb))
       return -1;
   }
-  // backward: U, 256-row blocks from the bottom, 32-row steps upwards inside each
-  for (int j0 = ((n - 1) / kNB2) * kNB2; j0 >= 0; j0 -= kNB2) {
-    const int je = min(n, j0 + kNB2);
+  // backward: U, 512-row blocks from the bottom, 32-row steps upwards inside each
+  for (int j0 = ((n - 1) / kNBF) * kNBF; j0 >= 0; j0 -= kNBF) {
+    const int je = min(n, j0 + kNBF);
     for (int j = ((je - 1 - j0) / kNB) * kNB + j0; j >= j0; j -= kNB) {
       const int jb = min(kNB, je - j);
       k_apply_blk<R><<<cblk, 256, 0, s>>>(inv + (long)(j / kNB) * kBlkElems + kNB * kNB, jb, B + j, ldb, nrhs);
