@@ -474,12 +474,12 @@ static void unpack_global(int NN, const std::vector<double>& Y, double* Xh)
 // as the correction solver: FP32 LUs of M and of the Schur complement S = M3 - L W
 // (W = Z_top o n n^T, Z = M^-1 K), residuals with the FP64 blocks (Wilkinson; Carson and
 // Higham, SIAM J. Sci. Comput. 40(2), 2018).  M, M3, R = [K | b_0 b_1 b_2], L, b3 are the
-// FP64 blocks as built (left unchanged).  Returns 0 with Y (4 x 2 NN complex) once the
+// FP64 blocks as built (left unchanged); L's first l0 rows are zero.  Returns 0 with Y (4 x 2 NN complex) once the
 // correction max|dx| / max|x| is below 1e-15 or stops shrinking below 1e-10; 1 if it stops
 // shrinking above that, an FP32 factor is singular or overflows (the caller falls back to
 // FP64 factors); < 0 on error.
 static int solve_global_mixed(bq_sess* S, const double* M, const double* M3, const double* R, const double* L,
-                              const double* b3, const double* nn3, std::vector<double>& Y)
+                              const double* b3, const double* nn3, int l0, std::vector<double>& Y)
 {
   hipStream_t s = S->s;
   const int NN = S->NN, n2 = 2 * NN;
@@ -528,7 +528,9 @@ static int solve_global_mixed(bq_sess* S, const double* M, const double* M3, con
   BQ_HIP(hipGetLastError());
   {
     const int kt = bq::ktime_begin("schur_cgemm", s);
-    if (bq_cgemm_op_dev(s, 0, 0, n2, NN, NN, monef, L32, n2, W32, NN, onef, S32, n2)) return -1;
+    if (bq_cgemm_op_dev(s, 0, 0, n2 - l0, NN, NN, monef, L32 + 2 * (size_t)l0, n2, W32, NN, onef,
+                        S32 + 2 * (size_t)l0, n2))
+      return -1;
     bq::ktime_end(kt, s);
   }
   info = bq_cgetrf_dev(s, n2, S32, n2, ip2, inv2);
@@ -659,6 +661,15 @@ int bq_sess_solve_global(bq_sess* S, double* Xh)
     rowbase += S->nlive[d];
   }
   bq::ktime_end(kt_b, s);
+  // L couples a row to X_3 only on the sub-domain side of an element; when the open region is
+  // the main side of all its elements, its rows (the first block) are zero in L and the Schur
+  // product skips them (their update would add exact zeros)
+  int l0 = S->nlive[0];
+  for (int i = 1; i <= S->m.dNe[0]; ++i)
+    if (S->m.md[abs(S->m.sid[0][i])] != 0) {
+      l0 = 0;
+      break;
+    }
   const double one[2] = {1, 0}, mone[2] = {-1, 0};
   S->st.mixed = 0;
   S->st.ir_iterations = 0;
@@ -666,7 +677,7 @@ int bq_sess_solve_global(bq_sess* S, double* Xh)
   const int prec = bq_solve_precision(-1);
   if (prec == 1 || (prec == 2 && n2 >= kMixedMinOrder)) {
     std::vector<double> Y;
-    int rc = solve_global_mixed(S, M, M3, R, L, b3, nn3, Y);
+    int rc = solve_global_mixed(S, M, M3, R, L, b3, nn3, l0, Y);
     if (rc < 0) return rc;
     if (rc == 0) {
       BQ_HIP(hipEventRecord(e1, s));
@@ -693,7 +704,12 @@ int bq_sess_solve_global(bq_sess* S, double* Xh)
   if (bq_zgetrs_dev(s, n2, NN + 3, M, n2, ip1, inv1, R, n2)) return -1;
   k_hadamard_normals<<<nblk((long)NN * NN, 256), 256, 0, s>>>(NN, R, nn3, W);
   BQ_HIP(hipGetLastError());
-  if (bq_zgemm_op_dev(s, 0, 0, n2, NN, NN, mone, L, n2, W, NN, one, M3, n2)) return -1;
+  {
+    const int kt = bq::ktime_begin("schur_zgemm", s);
+    if (bq_zgemm_op_dev(s, 0, 0, n2 - l0, NN, NN, mone, L + 2 * (size_t)l0, n2, W, NN, one, M3 + 2 * (size_t)l0, n2))
+      return -1;
+    bq::ktime_end(kt, s);
+  }
   k_normal_dot<<<nblk(NN, 256), 256, 0, s>>>(NN, R, nn3, v);
   BQ_HIP(hipGetLastError());
   if (bq_zgemm_op_dev(s, 0, 0, n2, 1, NN, mone, L, n2, v, NN, one, b3, n2)) return -1;
