@@ -29,6 +29,8 @@
 #include "../bq_plan.h"
 #include <cstdlib>
 #include <cfloat>
+#include <climits>
+#include <algorithm>
 
 struct bq_sess {
   bq_model m;
@@ -453,6 +455,11 @@ extern "C" int bq_solve_precision(int mode)
   return g_precision;
 }
 
+// rows [row0, row0 + rows) of L against its columns [col0, col0 + ncol)
+struct SchurBlock {
+  int row0, rows, col0, ncol;
+};
+
 // Y = (y_0, y_1, y_2, y_3), each of length 2 NN (unknowns, then normal derivatives) -> the
 // reference's X layout: X[c NN + e] (c = 0..3) and X[(c + 4) NN + e]
 static void unpack_global(int NN, const std::vector<double>& Y, double* Xh)
@@ -474,12 +481,14 @@ static void unpack_global(int NN, const std::vector<double>& Y, double* Xh)
 // as the correction solver: FP32 LUs of M and of the Schur complement S = M3 - L W
 // (W = Z_top o n n^T, Z = M^-1 K), residuals with the FP64 blocks (Wilkinson; Carson and
 // Higham, SIAM J. Sci. Comput. 40(2), 2018).  M, M3, R = [K | b_0 b_1 b_2], L, b3 are the
-// FP64 blocks as built (left unchanged); L's first l0 rows are zero.  Returns 0 with Y (4 x 2 NN complex) once the
+// FP64 blocks as built (left unchanged); lblk: the blocks of L that hold its nonzeros.
+// Returns 0 with Y (4 x 2 NN complex) once the
 // correction max|dx| / max|x| is below 1e-15 or stops shrinking below 1e-10; 1 if it stops
 // shrinking above that, an FP32 factor is singular or overflows (the caller falls back to
 // FP64 factors); < 0 on error.
 static int solve_global_mixed(bq_sess* S, const double* M, const double* M3, const double* R, const double* L,
-                              const double* b3, const double* nn3, int l0, std::vector<double>& Y)
+                              const double* b3, const double* nn3, const std::vector<SchurBlock>& lblk,
+                              std::vector<double>& Y)
 {
   hipStream_t s = S->s;
   const int NN = S->NN, n2 = 2 * NN;
@@ -528,9 +537,10 @@ static int solve_global_mixed(bq_sess* S, const double* M, const double* M3, con
   BQ_HIP(hipGetLastError());
   {
     const int kt = bq::ktime_begin("schur_cgemm", s);
-    if (bq_cgemm_op_dev(s, 0, 0, n2 - l0, NN, NN, monef, L32 + 2 * (size_t)l0, n2, W32, NN, onef,
-                        S32 + 2 * (size_t)l0, n2))
-      return -1;
+    for (const SchurBlock& b : lblk)
+      if (bq_cgemm_op_dev(s, 0, 0, b.rows, NN, b.ncol, monef, L32 + 2 * ((size_t)b.col0 * n2 + b.row0), n2,
+                          W32 + 2 * (size_t)b.col0, NN, onef, S32 + 2 * (size_t)b.row0, n2))
+        return -1;
     bq::ktime_end(kt, s);
   }
   info = bq_cgetrf_dev(s, n2, S32, n2, ip2, inv2);
@@ -661,15 +671,28 @@ int bq_sess_solve_global(bq_sess* S, double* Xh)
     rowbase += S->nlive[d];
   }
   bq::ktime_end(kt_b, s);
-  // L couples a row to X_3 only on the sub-domain side of an element; when the open region is
-  // the main side of all its elements, its rows (the first block) are zero in L and the Schur
-  // product skips them (their update would add exact zeros)
-  int l0 = S->nlive[0];
-  for (int i = 1; i <= S->m.dNe[0]; ++i)
-    if (S->m.md[abs(S->m.sid[0][i])] != 0) {
-      l0 = 0;
-      break;
+  // L couples a row of domain d to X_3 of the nodes of the elements whose sub-domain is d (the
+  // only entries k_build_blocks writes): per domain, its rows against one column range.  The
+  // Schur product L W runs on those blocks only; the rest of L is exactly zero.
+  std::vector<SchurBlock> lblk;
+  {
+    int rb = 0;
+    for (int d = 0; d <= S->MN; ++d) {
+      int cmin = INT_MAX, cmax = -1;
+      for (int i = 1; i <= S->m.dNe[d]; ++i) {
+        const int e = abs(S->m.sid[d][i]);
+        if (S->m.md[e] == d) continue;
+        for (int l = 0; l < 4; ++l) {
+          const int c = S->m.eni[4 * e + l];
+          if (c < 0) continue;
+          cmin = std::min(cmin, c);
+          cmax = std::max(cmax, c);
+        }
+      }
+      if (cmax >= 0 && S->nlive[d] > 0) lblk.push_back(SchurBlock{rb, S->nlive[d], cmin, cmax - cmin + 1});
+      rb += S->nlive[d];
     }
+  }
   const double one[2] = {1, 0}, mone[2] = {-1, 0};
   S->st.mixed = 0;
   S->st.ir_iterations = 0;
@@ -677,7 +700,7 @@ int bq_sess_solve_global(bq_sess* S, double* Xh)
   const int prec = bq_solve_precision(-1);
   if (prec == 1 || (prec == 2 && n2 >= kMixedMinOrder)) {
     std::vector<double> Y;
-    int rc = solve_global_mixed(S, M, M3, R, L, b3, nn3, l0, Y);
+    int rc = solve_global_mixed(S, M, M3, R, L, b3, nn3, lblk, Y);
     if (rc < 0) return rc;
     if (rc == 0) {
       BQ_HIP(hipEventRecord(e1, s));
@@ -706,8 +729,10 @@ int bq_sess_solve_global(bq_sess* S, double* Xh)
   BQ_HIP(hipGetLastError());
   {
     const int kt = bq::ktime_begin("schur_zgemm", s);
-    if (bq_zgemm_op_dev(s, 0, 0, n2 - l0, NN, NN, mone, L + 2 * (size_t)l0, n2, W, NN, one, M3 + 2 * (size_t)l0, n2))
-      return -1;
+    for (const SchurBlock& b : lblk)
+      if (bq_zgemm_op_dev(s, 0, 0, b.rows, NN, b.ncol, mone, L + 2 * ((size_t)b.col0 * n2 + b.row0), n2,
+                          W + 2 * (size_t)b.col0, NN, one, M3 + 2 * (size_t)b.row0, n2))
+        return -1;
     bq::ktime_end(kt, s);
   }
   k_normal_dot<<<nblk(NN, 256), 256, 0, s>>>(NN, R, nn3, v);
