@@ -33,6 +33,17 @@ int bq_cgemm_op_dev(void *stream, int ta, int tb, int m, int n, int k, const flo
                     long lda, const float *B, long ldb, const float *beta, float *C, long ldc);
 int bq_cgesv_host(int n, int nrhs, float *A, float *B);
 
+/* The row permutation the factorization stored in dinv (perm[i] = original row placed at
+ * row i; device int[n]); single != 0 for the FP32 buffer. */
+const int *bq_lu_perm_dev(const void *dinv, int n, int single);
+/* Solves whose right-hand sides come in column groups (host arrays): group g, columns
+ * g0[g] .. g0[g]+gn[g]-1, is zero in permuted rows 0 .. gf[g]-1, and its forward solve starts
+ * there.  The groups cover 0 .. nrhs-1. */
+int bq_zgetrs_groups_dev(void *stream, int n, int nrhs, const double *A, long lda, const int *ipiv, const double *dinv,
+                         double *B, long ldb, int ng, const int *g0, const int *gn, const int *gf);
+int bq_cgetrs_groups_dev(void *stream, int n, int nrhs, const float *A, long lda, const int *ipiv, const float *dinv,
+                         float *B, long ldb, int ng, const int *g0, const int *gn, const int *gf);
+
 #ifdef __cplusplus
 }
 #endif

@@ -690,7 +690,8 @@ int getrf(hipStream_t s, int n, typename P<R>::V* A, long lda, int* ipiv, typena
 // Solve A X = B with the factors from getrf; B is n x nrhs column-major (device).
 template <class R>
 int getrs(hipStream_t s, int n, int nrhs, const typename P<R>::V* A, long lda, const int* ipiv,
-          const typename P<R>::V* inv, typename P<R>::V* B, long ldb)
+          const typename P<R>::V* inv, typename P<R>::V* B, long ldb, int ng = 0, const int* g0 = nullptr,
+          const int* gn = nullptr, const int* gf = nullptr)
 {
   using V = typename P<R>::V;
   const V one = P<R>::v(1.0, 0.0), mone = P<R>::v(-1.0, 0.0);
@@ -727,20 +728,30 @@ int getrs(hipStream_t s, int n, int nrhs, const typename P<R>::V* A, long lda, c
     return 0;
   }
   const unsigned cblk = (unsigned)((nrhs + 63) / 64);
-  // forward: unit lower L, 512-row blocks of 32-row steps
-  for (int j0 = 0; j0 < n; j0 += kNBF) {
-    const int je = min(n, j0 + kNBF);
-    for (int j = j0; j < je; j += kNB) {
-      const int jb = min(kNB, je - j);
-      k_apply_blk<R><<<cblk, 256, 0, s>>>(inv + (long)(j / kNB) * kBlkElems, jb, B + j, ldb, nrhs);
-      BQ_HIP(hipGetLastError());
-      if (je - j - jb > 0 &&
-          gemm<R>(s, je - j - jb, nrhs, jb, mone, A + (long)j * lda + (j + jb), lda, B + j, ldb, one, B + j + jb, ldb))
+  // forward: unit lower L, 512-row blocks of 32-row steps.  With column groups, group g
+  // (columns g0[g] .. g0[g] + gn[g]) starts at the block holding row gf[g]: the caller states
+  // that its permuted right-hand sides are zero above that row, so are its solutions there.
+  const int ngrp = ng > 0 ? ng : 1;
+  for (int g = 0; g < ngrp; ++g) {
+    const int c0 = ng > 0 ? g0[g] : 0, nc = ng > 0 ? gn[g] : nrhs;
+    const int first = ng > 0 ? (max(0, gf[g]) / kNBF) * kNBF : 0;
+    if (nc <= 0) continue;
+    V* Bg = B + (long)c0 * ldb;
+    const unsigned gblk = (unsigned)((nc + 63) / 64);
+    for (int j0 = first; j0 < n; j0 += kNBF) {
+      const int je = min(n, j0 + kNBF);
+      for (int j = j0; j < je; j += kNB) {
+        const int jb = min(kNB, je - j);
+        k_apply_blk<R><<<gblk, 256, 0, s>>>(inv + (long)(j / kNB) * kBlkElems, jb, Bg + j, ldb, nc);
+        BQ_HIP(hipGetLastError());
+        if (je - j - jb > 0 && gemm<R>(s, je - j - jb, nc, jb, mone, A + (long)j * lda + (j + jb), lda, Bg + j, ldb,
+                                       one, Bg + j + jb, ldb))
+          return -1;
+      }
+      if (n - je > 0 &&
+          gemm<R>(s, n - je, nc, je - j0, mone, A + (long)j0 * lda + je, lda, Bg + j0, ldb, one, Bg + je, ldb))
         return -1;
     }
-    if (n - je > 0 &&
-        gemm<R>(s, n - je, nrhs, je - j0, mone, A + (long)j0 * lda + je, lda, B + j0, ldb, one, B + je, ldb))
-      return -1;
   }
   // backward: U, 512-row blocks from the bottom, 32-row steps upwards inside each
   for (int j0 = ((n - 1) / kNBF) * kNBF; j0 >= 0; j0 -= kNBF) {
@@ -801,6 +812,29 @@ int bq_zgemm_skinny_dev(void* stream, int m, int nrhs, long k, const double* A, 
                                                                        (const double2*)X, ldx, nrhs, (double2*)B, ldb);
   BQ_HIP(hipGetLastError());
   return 0;
+}
+
+// The row permutation getrf stored in its inverse-block buffer: perm[i] = the original row
+// that the factorization placed at row i.  dinv as passed to getrf (doubles or floats).
+const int* bq_lu_perm_dev(const void* dinv, int n, int single)
+{
+  return single ? perm_of((const float2*)dinv, n) : perm_of((const double2*)dinv, n);
+}
+
+// bq_zgetrs_dev / bq_cgetrs_dev with column groups: group g (columns g0[g] .. g0[g]+gn[g]-1,
+// host arrays) has permuted right-hand sides that are zero in rows 0 .. gf[g]-1.
+int bq_zgetrs_groups_dev(void* stream, int n, int nrhs, const double* Ad, long lda, const int* ipiv,
+                         const double* dinv, double* Bd, long ldb, int ng, const int* g0, const int* gn, const int* gf)
+{
+  return getrs<double>((hipStream_t)stream, n, nrhs, (const double2*)Ad, lda, ipiv, (const double2*)dinv,
+                       (double2*)Bd, ldb, ng, g0, gn, gf);
+}
+
+int bq_cgetrs_groups_dev(void* stream, int n, int nrhs, const float* Ad, long lda, const int* ipiv,
+                         const float* dinv, float* Bd, long ldb, int ng, const int* g0, const int* gn, const int* gf)
+{
+  return getrs<float>((hipStream_t)stream, n, nrhs, (const float2*)Ad, lda, ipiv, (const float2*)dinv,
+                      (float2*)Bd, ldb, ng, g0, gn, gf);
 }
 
 long bq_lu_inv_doubles(int n)

@@ -460,6 +460,46 @@ struct SchurBlock {
   int row0, rows, col0, ncol;
 };
 
+// Column groups of K = R[:, :NN] for the forward solve of Z = M^-1 K.  K has the nonzero
+// blocks of L (k_build_blocks writes both under the same condition), so a block's columns are
+// zero outside its rows; after the factorization's row permutation they are zero above the
+// first position any of those rows was moved to.  Gaps between blocks and the columns from
+// ncol_blocks to ncols (right-hand sides b_c) form groups that start at row 0.  Returns false
+// (no groups) if the blocks' column ranges overlap.
+static bool k_groups(hipStream_t s, const int* perm_dev, int n2, int NN, int ncols, const std::vector<SchurBlock>& lblk,
+                     std::vector<int>& g0, std::vector<int>& gn, std::vector<int>& gf)
+{
+  std::vector<int> perm(n2);
+  if (hipMemcpyAsync(perm.data(), perm_dev, (size_t)n2 * sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return false;
+  std::vector<SchurBlock> b = lblk;
+  std::sort(b.begin(), b.end(), [](const SchurBlock& x, const SchurBlock& y) { return x.col0 < y.col0; });
+  std::vector<int> first(b.size(), n2);
+  for (int i = 0; i < n2; ++i)
+    for (size_t q = 0; q < b.size(); ++q)
+      if (perm[i] >= b[q].row0 && perm[i] < b[q].row0 + b[q].rows && i < first[q]) first[q] = i;
+  int c = 0;
+  for (size_t q = 0; q < b.size(); ++q) {
+    if (b[q].col0 < c || b[q].col0 + b[q].ncol > NN) return false;
+    if (b[q].col0 > c) {
+      g0.push_back(c);
+      gn.push_back(b[q].col0 - c);
+      gf.push_back(0);
+    }
+    g0.push_back(b[q].col0);
+    gn.push_back(b[q].ncol);
+    gf.push_back(first[q]);
+    c = b[q].col0 + b[q].ncol;
+  }
+  if (c < ncols) {
+    g0.push_back(c);
+    gn.push_back(ncols - c);
+    gf.push_back(0);
+  }
+  return true;
+}
+
 // Y = (y_0, y_1, y_2, y_3), each of length 2 NN (unknowns, then normal derivatives) -> the
 // reference's X layout: X[c NN + e] (c = 0..3) and X[(c + 4) NN + e]
 static void unpack_global(int NN, const std::vector<double>& Y, double* Xh)
@@ -530,7 +570,13 @@ static int solve_global_mixed(bq_sess* S, const double* M, const double* M3, con
   int info = bq_cgetrf_dev(s, n2, M32, n2, ip1, inv1);
   if (info) return 1;
   k_d2f<<<nblk((long)rect, 256), 256, 0, s>>>((long)rect, R, R32);
-  if (bq_cgetrs_dev(s, n2, NN, M32, n2, ip1, inv1, R32, n2)) return -1;
+  {
+    std::vector<int> g0, gn, gf;
+    const bool grp = k_groups(s, bq_lu_perm_dev(inv1, n2, 1), n2, NN, NN, lblk, g0, gn, gf);
+    if (bq_cgetrs_groups_dev(s, n2, NN, M32, n2, ip1, inv1, R32, n2, grp ? (int)g0.size() : 0, g0.data(), gn.data(),
+                             gf.data()))
+      return -1;
+  }
   k_hadamard_t<float><<<nblk((long)NN * NN, 256), 256, 0, s>>>(NN, R32, nn3, W32);
   k_d2f<<<nblk((long)rect, 256), 256, 0, s>>>((long)rect, L, L32);
   k_d2f<<<nblk((long)sq, 256), 256, 0, s>>>((long)sq, M3, S32);
@@ -724,7 +770,13 @@ int bq_sess_solve_global(bq_sess* S, double* Xh)
     bq::set_error("solve_global: block matrix M is singular (info %d)", info);
     return info > 0 ? -3 : info;
   }
-  if (bq_zgetrs_dev(s, n2, NN + 3, M, n2, ip1, inv1, R, n2)) return -1;
+  {
+    std::vector<int> g0, gn, gf;
+    const bool grp = k_groups(s, bq_lu_perm_dev(inv1, n2, 0), n2, NN, NN + 3, lblk, g0, gn, gf);
+    if (bq_zgetrs_groups_dev(s, n2, NN + 3, M, n2, ip1, inv1, R, n2, grp ? (int)g0.size() : 0, g0.data(), gn.data(),
+                             gf.data()))
+      return -1;
+  }
   k_hadamard_normals<<<nblk((long)NN * NN, 256), 256, 0, s>>>(NN, R, nn3, W);
   BQ_HIP(hipGetLastError());
   {
