@@ -49,8 +49,8 @@ FLOPS_PER_EVAL = {"c1": {"k_q0_gl": 55999, "k_q0_gh": 51625, "k_q0_sing": 50274}
 FLOPS_PER_EVAL["c2s"] = FLOPS_PER_EVAL["c2"]
 # HBM bytes per evaluation of k_q0_gl: (FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00) KiB x 1024 of one
 # launch over its evaluations (factors from a 1 GiB copy read with the same counters)
-TRAFFIC_PER_EVAL = {"c1": {"k_q0_gl": (1164843.7 * 2.0 + 9543634.9) * 1024 / 46912896},
-                    "c2": {"k_q0_gl": (12408683.2 * 2.0 + 72198050.1) * 1024 / 385592752}}
+TRAFFIC_PER_EVAL = {"c1": {"k_q0_gl": (1163230.7 * 2.0 + 9545073.7) * 1024 / 46912896},
+                    "c2": {"k_q0_gl": (12430181.2 * 2.0 + 72190715.0) * 1024 / 385592752}}
 TRAFFIC_PER_EVAL["c2s"] = TRAFFIC_PER_EVAL["c2"]
 
 WORKLOADS = {
