@@ -8,6 +8,7 @@
  */
 #ifndef BQ_SOLVER_H
 #define BQ_SOLVER_H
+#include <stddef.h>
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -33,8 +34,18 @@ typedef struct bq_stats {
   long long n_qpgf;      /* QPGF evaluations in the assembly (algorithmic count) */
   long long n_qpgf_gl, n_qpgf_gh, n_qpgf_sing; /* the same split by kernel phase */
   int mixed;             /* 1: the global system was solved by FP32 factors + FP64 refinement */
-  int ir_iterations;     /* refinement steps taken (mixed) */
-  double ir_correction;  /* last correction, max |dx| / max |x| (mixed) */
+  int ir_iterations;     /* refinement steps taken */
+  double ir_correction;  /* last correction, max |dx| / max |x| (block elimination, mixed) */
+  int path;              /* 0: block elimination (two LUs of order 2 NN); 1: single-object
+                            reduction (LUs of order NN) */
+  int reused;            /* 1: matrices and factors of the previous solve were reused (same
+                            wavelength, direction and lattice: only the polarisation or
+                            amplitude changed) */
+  double backward_error; /* single-object reduction: normwise backward error of the unreduced
+                            system, max|b - A x| / (||A||_inf ||x||_inf + ||b||_inf) */
+  double t_factor_ms;    /* of t_solve_ms: the factorisation (0 when reused) */
+  int gmres_iterations;  /* FGMRES steps of the mixed-precision single-object solve */
+  int qpgf_error;        /* 0, or the reference's negative QPGF code met by the assembly */
 } bq_stats;
 
 typedef struct bq_sess bq_sess;
@@ -48,6 +59,15 @@ void bq_pack_elem(double *const *cr, double *const *cw, int tri, double *dst);
 
 bq_sess *bq_sess_create(const bq_model *m);
 void bq_sess_destroy(bq_sess *s);
+/* New incident wave for a resident session (same mesh and media; m's arrays as flattened for
+ * the new wave).  The matrices and factors stay valid when k, the Bloch vector, the lattice
+ * and every medium's wave number are unchanged (polarisation or amplitude only).  Returns 1
+ * if they stay valid, 0 if the next solve reassembles, < 0 if m is not the same mesh. */
+int bq_sess_update(bq_sess *s, const bq_model *m);
+/* 1 if the resident matrices belong to the current wave */
+int bq_sess_assembled(const bq_sess *s);
+/* device bytes held by the session */
+size_t bq_sess_bytes(const bq_sess *s);
 int bq_sess_assemble(bq_sess *s);
 /* X[8 NN] complex: the global unknowns in the reference's order (X[c NN + e]) */
 int bq_sess_solve_global(bq_sess *s, double *X);

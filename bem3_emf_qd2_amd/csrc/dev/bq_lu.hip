@@ -24,6 +24,8 @@
 // Replaces the reference's MKL PARDISO and LAPACKE_zgesv calls
 // (d3qd2_src/d3qd2_mkl_solver.c:18-148, d3qd2_src/d3qd2_solve_bieq.c:753).
 #include <rocblas/rocblas.h>
+#include <mutex>
+#include <vector>
 #include "bq_runtime.hpp"
 #include "bq_complex.hpp"
 #include "../bq_linalg.h"
@@ -395,7 +397,8 @@ __global__ void k_check_diag(int n, const typename P<R>::V* __restrict__ A, long
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   V d = A[(long)i * lda + i];
-  if (d.x == 0.0 && d.y == 0.0) atomicMin(info, i + 1);
+  // an exactly zero pivot, or one that is Inf / NaN (a non-finite matrix entry upstream)
+  if ((d.x == 0.0 && d.y == 0.0) || !isfinite((double)d.x) || !isfinite((double)d.y)) atomicMin(info, i + 1);
 }
 
 // the row permutation getrf leaves after the inverted diagonal blocks
@@ -411,6 +414,7 @@ int* perm_of(V* inv, int n)
 // runs all 32-row steps of a 256-row diagonal block with its rows of B in LDS, and one kernel
 // applies the block's columns of L (or U) to the rows outside it, reading them once.
 constexpr int kSmallRhs = 8;
+constexpr int kPermChunk = 512;  // columns per pass of the in-place row permutation
 
 // rows [j0, je) of B (nrhs columns): LOWER: X = inv(L_blk) X (unit lower, steps downwards);
 // else X = inv(U_blk) X (steps upwards).  inv: the 32x32 inverses getrf stored.
@@ -565,15 +569,29 @@ __global__ void __launch_bounds__(256) k_skinny_gemm(int m, long kk, const typen
   }
 }
 
-rocblas_handle g_handle[64];
+// One rocBLAS handle per (device, stream), created on first use and kept for the process:
+// sessions on different streams (concurrent requests) never share a handle, so no caller
+// changes the stream of a handle another thread is using.
+struct HandleSlot {
+  int dev;
+  hipStream_t s;
+  rocblas_handle h;
+};
+std::mutex g_hmu;
+std::vector<HandleSlot> g_handles;
 
 rocblas_handle handle_for(hipStream_t s)
 {
   int dev = 0;
   (void)hipGetDevice(&dev);
-  if (!g_handle[dev]) rocblas_create_handle(&g_handle[dev]);
-  rocblas_set_stream(g_handle[dev], s);
-  return g_handle[dev];
+  std::lock_guard<std::mutex> lk(g_hmu);
+  for (const HandleSlot& q : g_handles)
+    if (q.dev == dev && q.s == s) return q.h;
+  rocblas_handle h = nullptr;
+  if (rocblas_create_handle(&h) != rocblas_status_success) return nullptr;
+  rocblas_set_stream(h, s);
+  g_handles.push_back(HandleSlot{dev, s, h});
+  return h;
 }
 
 template <class R>
@@ -582,6 +600,10 @@ int gemm_op(hipStream_t s, int ta, int tb, int m, int n, int k, typename P<R>::V
 {
   if (m <= 0 || n <= 0 || k <= 0) return 0;
   rocblas_handle h = handle_for(s);
+  if (!h) {
+    bq::set_error("rocblas handle creation failed");
+    return -1;
+  }
   rocblas_status st = P<R>::gemm(h, ta ? rocblas_operation_transpose : rocblas_operation_none,
                                  tb ? rocblas_operation_transpose : rocblas_operation_none, m, n, k, &alpha, A,
                                  (int)lda, B, (int)ldb, &beta, C, (int)ldc);
@@ -699,14 +721,22 @@ int getrs(hipStream_t s, int n, int nrhs, const typename P<R>::V* A, long lda, c
   const int kt_all = bq::ktime_begin(P<R>::kt_s, s);
   const int* perm = perm_of(inv, n);  // stored by getrf (ipiv is kept for the interface)
   (void)ipiv;
-  V* Bt = nullptr;
-  BQ_HIP(hipMallocAsync((void**)&Bt, sizeof(V) * (size_t)n * nrhs, s));
-  const long tot = (long)n * nrhs;
-  k_gather_rows<R><<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(n, nrhs, perm, B, ldb, Bt);
-  BQ_HIP(hipGetLastError());
-  BQ_HIP(hipMemcpy2DAsync(B, ldb * sizeof(V), Bt, (size_t)n * sizeof(V), (size_t)n * sizeof(V), nrhs,
-                          hipMemcpyDeviceToDevice, s));
-  BQ_HIP(hipFreeAsync(Bt, s));
+  // rows permuted in place, kPermChunk columns at a time through one n x kPermChunk buffer
+  // (not a copy of all of B: with n right-hand sides that would double the solve's memory)
+  {
+    const int chunk = min(nrhs, kPermChunk);
+    V* Bt = nullptr;
+    BQ_HIP(hipMallocAsync((void**)&Bt, sizeof(V) * (size_t)n * chunk, s));
+    for (int c0 = 0; c0 < nrhs; c0 += chunk) {
+      const int nc = min(chunk, nrhs - c0);
+      const long tot = (long)n * nc;
+      k_gather_rows<R><<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(n, nc, perm, B + (long)c0 * ldb, ldb, Bt);
+      BQ_HIP(hipGetLastError());
+      BQ_HIP(hipMemcpy2DAsync(B + (long)c0 * ldb, ldb * sizeof(V), Bt, (size_t)n * sizeof(V), (size_t)n * sizeof(V),
+                              nc, hipMemcpyDeviceToDevice, s));
+    }
+    BQ_HIP(hipFreeAsync(Bt, s));
+  }
   if (nrhs <= kSmallRhs) {
     for (int j0 = 0; j0 < n; j0 += kNB2) {
       const int je = min(n, j0 + kNB2);

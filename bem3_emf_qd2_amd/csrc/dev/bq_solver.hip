@@ -23,6 +23,7 @@
 #include <hip/hip_runtime.h>
 #include "bq_runtime.hpp"
 #include "bq_complex.hpp"
+#include "bq_sess.hpp"
 #include "../bq_solver.h"
 #include "../bq_assemble.h"
 #include "../bq_linalg.h"
@@ -31,25 +32,6 @@
 #include <cfloat>
 #include <climits>
 #include <algorithm>
-
-struct bq_sess {
-  bq_model m;
-  int Ne, NN, MN, ps;
-  hipStream_t s;
-  double *E = nullptr, *ren = nullptr, *nrm = nullptr, *Ui = nullptr, *dUi = nullptr, *kn = nullptr;
-  int *eni = nullptr, *md = nullptr, *sd = nullptr;
-  std::vector<int> dNe;
-  std::vector<std::vector<int>> sid_h;
-  std::vector<int*> sid;
-  std::vector<int*> rowmap;  // per domain, device: row 4(t-1)+tn -> live index or -1
-  std::vector<int*> live;    // per domain, device: live index -> row
-  std::vector<int> nlive;
-  std::vector<double*> G, H, dG, dH, tdF;
-  std::vector<double> kn_h;
-  bq_qplan plan;
-  bq_stats st{};
-  int two_nn = 0;
-};
 
 namespace {
 
@@ -268,6 +250,16 @@ __global__ void k_live_rows(long N, int n, int ncol, const int* __restrict__ liv
 
 inline unsigned nblk(long n, int b) { return (unsigned)((n + b - 1) / b); }
 
+// what the matrices depend on besides the mesh: QPD2 (k, Bloch vector, lattice), every
+// medium's wave number and the periodicity switch
+std::vector<double> wave_key(const bq_model* m)
+{
+  std::vector<double> k(m->qd, m->qd + 8);
+  k.insert(k.end(), m->kn, m->kn + 2 * (m->MN + 1));
+  k.push_back((double)m->ps);
+  return k;
+}
+
 }  // namespace
 
 extern "C" {
@@ -283,7 +275,14 @@ bq_sess* bq_sess_create(const bq_model* m)
   S->NN = m->NN;
   S->MN = m->MN;
   S->ps = m->ps;
-  S->s = bq::stream();
+  // each session runs on a stream of its own (default priority), so that sessions of
+  // concurrent requests overlap on the device
+  if (hipStreamCreateWithFlags(&S->s, hipStreamNonBlocking) != hipSuccess) {
+    bq::set_error("session stream creation failed");
+    delete S;
+    return nullptr;
+  }
+  S->own_stream = true;
   hipStream_t s = S->s;
   const size_t ne1 = (size_t)m->Ne + 1;
   if (upload(&S->E, m->E, ne1 * 24, s) || upload(&S->ren, m->ren, ne1 * 12, s) ||
@@ -348,13 +347,44 @@ bq_sess* bq_sess_create(const bq_model* m)
     bq_sess_destroy(S);
     return nullptr;
   }
+  S->key_now = wave_key(m);
   return S;
 }
+
+int bq_sess_update(bq_sess* S, const bq_model* m)
+{
+  if (m->Ne != S->Ne || m->NN != S->NN || m->MN != S->MN || m->ps != S->ps) {
+    bq::set_error("session update: a different mesh (Ne %d/%d, NN %d/%d)", m->Ne, S->Ne, m->NN, S->NN);
+    return -2;
+  }
+  hipStream_t s = S->s;
+  const size_t ne1 = (size_t)m->Ne + 1;
+  S->m = *m;
+  BQ_HIP(hipMemcpyAsync(S->Ui, m->Ui, ne1 * 32 * 8, hipMemcpyHostToDevice, s));
+  BQ_HIP(hipMemcpyAsync(S->dUi, m->dUi, ne1 * 32 * 8, hipMemcpyHostToDevice, s));
+  BQ_HIP(hipMemcpyAsync(S->kn, m->kn, (size_t)(m->MN + 1) * 16, hipMemcpyHostToDevice, s));
+  BQ_HIP(hipStreamSynchronize(s));
+  S->kn_h.assign(m->kn, m->kn + 2 * (m->MN + 1));
+  if (bq_qplan_init(&S->plan, m->qd[0], m->qd + 1, m->qd + 4, m->qd + 6, 1.0e-15) && m->ps == 1) {
+    bq::set_error("degenerate lattice");
+    return -1;
+  }
+  S->key_now = wave_key(m);
+  if (S->assembled && S->key_now == S->key_asm) return 1;
+  S->red.prec = -1;  // the factors belong to the old matrices
+  return 0;
+}
+
+int bq_sess_assembled(const bq_sess* S) { return (S->assembled && S->key_now == S->key_asm) ? 1 : 0; }
+
+size_t bq_sess_bytes(const bq_sess* S) { return S->bytes + S->red.bytes; }
 
 void bq_sess_destroy(bq_sess* S)
 {
   if (!S) return;
   (void)hipStreamSynchronize(S->s);
+  bq::reduced_free(S);
+  if (S->work) (void)hipFree(S->work);
   auto f = [](void* p) { if (p) (void)hipFree(p); };
   f(S->E); f(S->ren); f(S->nrm); f(S->Ui); f(S->dUi); f(S->kn); f(S->eni); f(S->md); f(S->sd);
   for (auto p : S->sid) f(p);
@@ -365,6 +395,7 @@ void bq_sess_destroy(bq_sess* S)
   for (auto p : S->dG) f(p);
   for (auto p : S->dH) f(p);
   for (auto p : S->tdF) f(p);
+  if (S->own_stream) (void)hipStreamDestroy(S->s);
   delete S;
 }
 
@@ -379,19 +410,29 @@ int bq_sess_assemble(bq_sess* S)
   BQ_HIP(hipEventRecord(e0, s));
   size_t wmax = 0;
   for (int d = 0; d <= S->MN; ++d) wmax = std::max(wmax, bq_assemble_work_bytes(S->dNe[d]));
-  void* work = nullptr;
-  BQ_HIP(hipMalloc(&work, wmax));
+  if (!S->work) {
+    BQ_HIP(hipMalloc(&S->work, wmax));
+    S->work_bytes = wmax;
+    S->bytes += wmax;
+  }
+  void* work = S->work;
+  S->assembled = false;
+  S->red.prec = -1;
   S->st.n_gh_pairs = 0;
   S->st.n_qpgf = S->st.n_qpgf_gl = S->st.n_qpgf_gh = S->st.n_qpgf_sing = 0;
   for (int d = 0; d <= S->MN; ++d) {
     const long dNe = S->dNe[d], N = 4 * dNe;
     if (dNe == 0) continue;
     const bool periodic = (d == 0 && S->ps == 1);
-    if (dmalloc(&S->G[d], (size_t)(2 * N * N)) || dmalloc(&S->H[d], (size_t)(2 * N * N))) return -1;
-    if (!periodic) {
-      if (dmalloc(&S->dG[d], (size_t)(4 * N * N)) || dmalloc(&S->dH[d], (size_t)(4 * N * N)) ||
-          dmalloc(&S->tdF[d], (size_t)(3 * N)))
-        return -1;
+    if (!S->G[d]) {
+      if (dmalloc(&S->G[d], (size_t)(2 * N * N)) || dmalloc(&S->H[d], (size_t)(2 * N * N))) return -1;
+      S->bytes += (size_t)32 * N * N;
+      if (!periodic) {
+        if (dmalloc(&S->dG[d], (size_t)(4 * N * N)) || dmalloc(&S->dH[d], (size_t)(4 * N * N)) ||
+            dmalloc(&S->tdF[d], (size_t)(3 * N)))
+          return -1;
+        S->bytes += (size_t)64 * N * N + 24 * N;
+      }
     }
     bq_adom a{};
     a.dNe = (int)dNe;
@@ -414,10 +455,7 @@ int bq_sess_assemble(bq_sess* S)
     a.n_points_out = npts;
     // BQ_FULL_RULE=1: the reference's 12-point rule for every regular periodic pair
     a.full_rule = (getenv("BQ_FULL_RULE") && atoi(getenv("BQ_FULL_RULE")) != 0) ? 1 : 0;
-    if (bq_assemble_domain_dev(&a, work, s)) {
-      (void)hipFree(work);
-      return -1;
-    }
+    if (bq_assemble_domain_dev(&a, work, s)) return -1;
     S->st.n_gh_pairs += ngh;
     if (periodic) {
       S->st.n_qpgf_gl += npts[0];
@@ -431,7 +469,8 @@ int bq_sess_assemble(bq_sess* S)
   float ms = 0;
   BQ_HIP(hipEventElapsedTime(&ms, e0, e1));
   S->st.t_assemble_ms = ms;
-  (void)hipFree(work);
+  S->assembled = true;
+  S->key_asm = S->key_now;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   return 0;
@@ -665,6 +704,38 @@ static int solve_global_mixed(bq_sess* S, const double* M, const double* M3, con
   return rc;
 }
 
+// Single object: the reduction of bq_reduced.hip (factors kept for later solves of the same
+// matrices); FP64, or FP32 factors with FP64 GMRES refinement, falling back to FP64 factors
+// when the refinement stalls above a backward error of 1e-10.
+static int solve_single(bq_sess* S, double* Xh, bool mixed)
+{
+  hipStream_t s = S->s;
+  hipEvent_t e0, e1;
+  BQ_HIP(hipEventCreate(&e0));
+  BQ_HIP(hipEventCreate(&e1));
+  BQ_HIP(hipEventRecord(e0, s));
+  S->st.path = 1;
+  S->st.t_factor_ms = 0.0;
+  const int want = mixed ? 1 : 0;
+  int rc = 0;
+  if (S->red.prec != want) rc = bq::reduced_factor(S, want);
+  if (rc == 0) rc = bq::reduced_solve(S, Xh);
+  if (rc > 0 && want == 1) {  // refinement stalled: FP64 factors
+    rc = bq::reduced_factor(S, 0);
+    if (rc == 0) rc = bq::reduced_solve(S, Xh);
+  }
+  S->st.mixed = S->red.prec == 1 ? 1 : 0;
+  BQ_HIP(hipEventRecord(e1, s));
+  BQ_HIP(hipEventSynchronize(e1));
+  float ms = 0.f;
+  BQ_HIP(hipEventElapsedTime(&ms, e0, e1));
+  S->st.t_solve_ms = ms;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc > 0) rc = -3;
+  return rc;
+}
+
 int bq_sess_solve_global(bq_sess* S, double* Xh)
 {
   hipStream_t s = S->s;
@@ -673,6 +744,18 @@ int bq_sess_solve_global(bq_sess* S, double* Xh)
     bq::set_error("solve_global: element-node count mismatch (%d live rows vs 2 NN = %d)", S->two_nn, n2);
     return -2;
   }
+  S->st.mixed = 0;
+  S->st.ir_iterations = 0;
+  S->st.ir_correction = 0.0;
+  S->st.gmres_iterations = 0;
+  S->st.backward_error = 0.0;
+  {
+    const int prec = bq_solve_precision(-1);
+    const bool mixed = prec == 1 || (prec == 2 && n2 >= kMixedMinOrder);
+    if (bq::single_object(S) && !getenv("BQ_BLOCK_ELIM")) return solve_single(S, Xh, mixed);
+  }
+  S->st.path = 0;
+  S->st.t_factor_ms = 0.0;
   hipEvent_t e0, e1;
   BQ_HIP(hipEventCreate(&e0));
   BQ_HIP(hipEventCreate(&e1));
@@ -842,6 +925,7 @@ int bq_sess_dudt(bq_sess* S, int d, const double* U, const double* dU, double* o
 
 int bq_sess_deh(bq_sess* S, int d, const double* EH, double* dEH)
 {
+  if (S->st.path == 1 && S->red.prec >= 0 && d <= 1) return bq::reduced_deh(S, d, EH, dEH);
   hipStream_t s = S->s;
   const long N = 4L * S->dNe[d];
   const int n = S->nlive[d];

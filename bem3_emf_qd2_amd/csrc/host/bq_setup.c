@@ -531,12 +531,12 @@ static void init_boundary_data(DQD2 *md)
 /* Drops the field evaluator's device copy of qd's model and solution (bq_field.c); called
  * wherever qd's arrays are replaced, freed or rewritten. */
 void bq_release_device_state(DQD2 *qd);
+void bq_release_solver(void *qd); /* bq_solve.c */
 
-void initialize_dqd2(DQD2 *qd)
+/* QPD2 and the wave numbers from the (first) incident wave, as initialize_dqd2 sets them
+ * (reference d3qd2_src/d3qd2_setup.c:121-128) */
+static void wave_constants(DQD2 *qd)
 {
-  bq_release_device_state(qd);
-  setup_mipw(&qd->pw);
-  check_mipw(qd);
   const IpwD *d = &qd->pw.bd.ipw[0].data;
   qd->bd.qd.k = d->ki;
   qd->bd.qd.vk[0] = d->ki * d->sin_t * d->cos_p;
@@ -546,6 +546,41 @@ void initialize_dqd2(DQD2 *qd)
   if (qd->bd.qd.vk[1] == 0.0) qd->bd.qd.vk[1] = qd->bd.qd.k * MEPS;
   qd->n[0] = qd->pw.n_0;
   for (int i = 0; i <= qd->MN; i++) qd->kn[i] = 2.0 * M_PI / qd->pw.lambda_0 * qd->n[i];
+}
+
+int bq_set_incidence(void *qdv, double lambda0, double theta, double phi, double e0x_re, double e0x_im,
+                     double e0y_re, double e0y_im, double power)
+{
+  DQD2 *qd = (DQD2 *)qdv;
+  if (!qd || !qd->pw.bd.ipw || qd->pw.n_ipw < 1 || !qd->bd.sb) return -1;
+  if (power == 0.0) power = 0.5 * qd->pw.n_0; /* amplitude 1 (E0 = sqrt(2 power / n0)) */
+  if (!(lambda0 > 0.0) || !(power > 0.0)) return -1;
+  if (e0x_re == 0.0 && e0x_im == 0.0 && e0y_re == 0.0 && e0y_im == 0.0) return -1;
+  const double kz = cos(theta);
+  if (qd->bd.ps == 1 && (kz == 0.0 || fabs(kz) == 1.0)) return -1;
+  bq_release_device_state(qd);
+  Ipw *p = &qd->pw.bd.ipw[0];
+  qd->pw.n_ipw = 1;
+  p->lambda0 = lambda0;
+  p->power = power;
+  p->e0x = e0x_re + I * e0x_im;
+  p->e0y = e0y_re + I * e0y_im;
+  p->theta = theta;
+  p->phi = phi;
+  setup_mipw(&qd->pw);
+  check_mipw(qd);
+  wave_constants(qd);
+  init_boundary_data(qd);
+  return 0;
+}
+
+void initialize_dqd2(DQD2 *qd)
+{
+  bq_release_device_state(qd);
+  bq_release_solver(qd);
+  setup_mipw(&qd->pw);
+  check_mipw(qd);
+  wave_constants(qd);
   if (qd->th != 0.0 || vabs_d(qd->tv) != 0.0) rotation_translation(qd);
   init_elem_const(&qd->bd);
   malloc_sub_domain(qd);
@@ -556,6 +591,7 @@ void initialize_dqd2(DQD2 *qd)
 void finalize_dqd2(DQD2 *qd)
 {
   bq_release_device_state(qd);
+  bq_release_solver(qd);
   mfree_sub_domain(qd);
   mfree_elem(&qd->bd);
   free_2((void **)qd->bd.rn, qd->bd.Nn + 1);
@@ -721,6 +757,7 @@ void dat_write(char *fname, DQD2 *qd)
 void dat_read(char *fname, DQD2 *qd)
 {
   bq_release_device_state(qd);
+  bq_release_solver(qd);
   FILE *fp = fopen(fname, "rb");
   if (!fp) {
     printf("dat_read(), Failed to open the %s file.\n", fname);

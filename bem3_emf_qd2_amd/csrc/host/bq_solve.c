@@ -16,6 +16,7 @@
 #include <math.h>
 #include <complex.h>
 #include <time.h>
+#include <pthread.h>
 #include "bem3_emf_qd2.h"
 #include "bq_amd.h"
 #include "../bq_solver.h"
@@ -297,6 +298,91 @@ static void boundary_dEH(DQD2 *md, bq_sess *S)
 
 void bq_release_device_state(DQD2 *qd); /* bq_field.c */
 
+/* Device sessions kept per model between solves: the mesh, element tables, matrices, factors
+ * and work buffers stay resident, so that a sweep over incidences (bq_set_incidence between
+ * solves) allocates nothing after its first point and a polarisation-only change reuses the
+ * matrices and factors.  Keyed by the DQD2 address and its per-domain arrays (replaced when the
+ * model is set up again); released by finalize_dqd2 or bq_release_solver. */
+typedef struct sess_entry {
+  DQD2 *qd;
+  const void *key;
+  bq_sess *S;
+  flat_model f;
+  bq_stats last; /* of the model's last solve_bieq */
+  struct sess_entry *next;
+} sess_entry;
+static sess_entry *g_sess;
+static pthread_mutex_t g_smu = PTHREAD_MUTEX_INITIALIZER;
+
+static const void *model_key(DQD2 *qd) { return qd->bd.sb ? (const void *)qd->bd.sb[0].E : NULL; }
+
+static void entry_free(sess_entry *e)
+{
+  if (e->S) bq_sess_destroy(e->S);
+  free_flat(&e->f);
+  free(e);
+}
+
+/* the model's session, created or brought up to date with its current incident wave */
+static bq_sess *session_for(DQD2 *qd)
+{
+  pthread_mutex_lock(&g_smu);
+  sess_entry **pp = &g_sess, *e = NULL;
+  for (; *pp; pp = &(*pp)->next)
+    if ((*pp)->qd == qd) break;
+  if (*pp && (*pp)->key != model_key(qd)) { /* set up again since: a new session */
+    sess_entry *old = *pp;
+    *pp = old->next;
+    entry_free(old);
+  }
+  e = *pp;
+  pthread_mutex_unlock(&g_smu);
+  if (e) {
+    free_flat(&e->f);
+    memset(&e->f, 0, sizeof e->f);
+    flatten(qd, &e->f);
+    if (bq_sess_update(e->S, &e->f.m) >= 0) return e->S;
+    bq_release_solver(qd);
+  }
+  e = calloc(1, sizeof *e);
+  e->qd = qd;
+  e->key = model_key(qd);
+  flatten(qd, &e->f);
+  e->S = bq_sess_create(&e->f.m);
+  if (!e->S) {
+    free_flat(&e->f);
+    free(e);
+    return NULL;
+  }
+  pthread_mutex_lock(&g_smu);
+  e->next = g_sess;
+  g_sess = e;
+  pthread_mutex_unlock(&g_smu);
+  return e->S;
+}
+
+void bq_release_solver(void *qdv)
+{
+  DQD2 *qd = (DQD2 *)qdv;
+  pthread_mutex_lock(&g_smu);
+  sess_entry **pp = &g_sess;
+  while (*pp && (*pp)->qd != qd) pp = &(*pp)->next;
+  sess_entry *e = *pp;
+  if (e) *pp = e->next;
+  pthread_mutex_unlock(&g_smu);
+  if (e) entry_free(e);
+}
+
+long long bq_solver_bytes(void *qdv)
+{
+  long long b = 0;
+  pthread_mutex_lock(&g_smu);
+  for (sess_entry *e = g_sess; e; e = e->next)
+    if (!qdv || e->qd == (DQD2 *)qdv) b += (long long)bq_sess_bytes(e->S);
+  pthread_mutex_unlock(&g_smu);
+  return b;
+}
+
 void solve_bieq(DQD2 *qd)
 {
   double t0 = now_s(), ts;
@@ -307,12 +393,10 @@ void solve_bieq(DQD2 *qd)
   printf("  coefficient matrix          ");
   fflush(stdout);
   ts = now_s();
-  flat_model f;
-  memset(&f, 0, sizeof f);
-  flatten(qd, &f);
-  bq_sess *S = bq_sess_create(&f.m);
+  bq_sess *S = session_for(qd);
   if (!S) gpu_fail("device set-up");
-  if (bq_sess_assemble(S)) gpu_fail("coefficient matrix assembly");
+  int reused = bq_sess_assembled(S);
+  if (!reused && bq_sess_assemble(S)) gpu_fail("coefficient matrix assembly");
   printf("finished. Elapsed time : %5g (sec)\n", now_s() - ts);
 
   printf("  solve mEMP boundary value   ");
@@ -332,11 +416,31 @@ void solve_bieq(DQD2 *qd)
   ts = now_s();
   boundary_EH(qd, S);
   boundary_dEH(qd, S);
-  printf("finished. Elapsed time : %5g (sec)\n", now_s() - ts);
-  g_last = *bq_sess_stats(S);
-  bq_sess_destroy(S);
-  free_flat(&f);
+  double tf = now_s() - ts;
+  printf("finished. Elapsed time : %5g (sec)\n", tf);
+  bq_stats st = *bq_sess_stats(S);
+  st.t_fields_ms = tf * 1e3;
+  st.reused = reused;
+  if (reused) st.t_assemble_ms = 0.0;
+  pthread_mutex_lock(&g_smu);
+  g_last = st;
+  for (sess_entry *e = g_sess; e; e = e->next)
+    if (e->qd == qd) e->last = st;
+  pthread_mutex_unlock(&g_smu);
   printf("Total elapsed time : %g (sec)\n", now_s() - t0);
+}
+
+int bq_solve_stats(void *qdv, void *out)
+{
+  int rc = -1;
+  pthread_mutex_lock(&g_smu);
+  for (sess_entry *e = g_sess; e; e = e->next)
+    if (e->qd == (DQD2 *)qdv) {
+      memcpy(out, &e->last, sizeof e->last);
+      rc = 0;
+    }
+  pthread_mutex_unlock(&g_smu);
+  return rc;
 }
 
 /* Assemble the matrices of domain d for a set-up model and copy them to host buffers

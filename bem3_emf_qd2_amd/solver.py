@@ -21,7 +21,12 @@ class Stats(ctypes.Structure):
     _fields_ = [("t_assemble_ms", ctypes.c_double), ("t_solve_ms", ctypes.c_double),
                 ("t_fields_ms", ctypes.c_double), ("n_gh_pairs", ctypes.c_int), ("n_qpgf", ctypes.c_longlong),
                 ("n_qpgf_gl", ctypes.c_longlong), ("n_qpgf_gh", ctypes.c_longlong), ("n_qpgf_sing", ctypes.c_longlong),
-                ("mixed", ctypes.c_int), ("ir_iterations", ctypes.c_int), ("ir_correction", ctypes.c_double)]
+                ("mixed", ctypes.c_int), ("ir_iterations", ctypes.c_int), ("ir_correction", ctypes.c_double),
+                ("path", ctypes.c_int), ("reused", ctypes.c_int), ("backward_error", ctypes.c_double),
+                ("t_factor_ms", ctypes.c_double), ("gmres_iterations", ctypes.c_int), ("qpgf_error", ctypes.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 def L():
@@ -46,6 +51,11 @@ def L():
                                ctypes.c_void_p]
         lb.bq_assemble_domain_host.argtypes = [ctypes.c_void_p, ctypes.c_int, _D, _D, _D, _D, _D]
         lb.bq_solve_last_stats.restype = ctypes.POINTER(Stats)
+        lb.bq_set_incidence.argtypes = [ctypes.c_void_p] + [ctypes.c_double] * 8
+        lb.bq_release_solver.argtypes = [ctypes.c_void_p]
+        lb.bq_solver_bytes.argtypes = [ctypes.c_void_p]
+        lb.bq_solver_bytes.restype = ctypes.c_longlong
+        lb.bq_solve_stats.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
         lb._solver_declared = True
     return lb
 
@@ -83,7 +93,26 @@ class Model:
 
     def solve(self) -> Stats:
         L().solve_bieq(self.ptr)
-        return L().bq_solve_last_stats().contents
+        st = Stats()
+        if L().bq_solve_stats(self.ptr, ctypes.byref(st)) != 0:
+            raise RuntimeError("solve_bieq left no statistics")
+        return st
+
+    def set_incidence(self, lambda0, theta, phi, e0x=1.0, e0y=0.0, power=None):
+        """Replace the incident wave (bq_set_incidence); power None means amplitude 1."""
+        if power is None:
+            power = 0.0
+        e0x, e0y = complex(e0x), complex(e0y)
+        rc = L().bq_set_incidence(self.ptr, lambda0, theta, phi, e0x.real, e0x.imag, e0y.real, e0y.imag, power)
+        if rc:
+            raise ValueError(f"bq_set_incidence rejected lambda0={lambda0} theta={theta} phi={phi}")
+
+    def release_device(self) -> None:
+        """Free the resident solver session (bq_release_solver)."""
+        L().bq_release_solver(self.ptr)
+
+    def device_bytes(self) -> int:
+        return int(L().bq_solver_bytes(self.ptr))
 
     def write(self, path) -> None:
         L().dat_write(str(path).encode(), self.ptr)

@@ -1,25 +1,27 @@
-"""Benchmark: solved incidences per second of the plain FP64 GPU path.
+"""Benchmark: solved incidences per second on one MI355X, with p50/p99 latency and the error of
+every result against the plain FP64 GPU path.
 
 One step = one solved incidence, the work the reference's d3qd2_bv_solver + example1 do for
 one plane wave (d3qd2_src/d3qd2_solve_bieq.c:36-84, d3qd2_src/d3qd2_field.c:271): assembly
 of every domain's coefficient matrices, the global solve, boundary E/H and dE/dH, then E and
-H at 100 probe points (EH_t, type 1).  The mesh and incidence stay resident; each step
-recomputes everything above from them.
+H at 100 probe points (EH_t, type 1).  The mesh stays resident; each step recomputes
+everything above for its incident wave.
 
-Default workload (c2, the scene the metric is quoted on): in one hexagonal lattice cell
-(a1 = (0.9, 0), a2 = (0.45, 0.779) um) a two-layer cube-sphere (core r 0.3 um, n 1.6 + 0.01i;
-shell n 1.45; 6x16x16 = 1536 bilinear quads per surface) and a right circular cone (base
-radius 0.3 um, height 0.6 um, n 2.0) of 2012 linear triangles with a seeded 1 %-of-edge jitter
-of its non-feature vertices; oblique incidence theta = 30 deg, phi = 15 deg, TM, lambda0 =
-0.55 um; 100 probes uniform in the cell with z in [-1, 1] um from default_rng(1).  The shell
-radius is 0.40 um, not 0.45: on this lattice |a2| = 0.8996 um, so a 0.45 um shell would
-overlap its own periodic image; the cone stands on z = 0.5 um above the sphere.
-Workload c1: the 384-quad cube-sphere of radius 0.4 um, index 1.5, in vacuum on a 1.0 um
-square lattice, lambda0 = 0.6 um, x-polarised, normal incidence (theta = 1e-6 rad, the
-reference rejects theta = 0 on a lattice), same probes.
+Default workload (c4, the sweep the metric is quoted on): a cube-sphere of 6 x 32 x 32 = 6144
+bilinear quads (radius 0.45 um, n 1.5 + 0.001i) in vacuum on a 1.0 um square lattice; 16
+incidence points from default_rng(4): lambda0 uniform in [0.4, 0.8] um, theta in [0, 40] deg,
+phi in [0, 90] deg, TE or TM; 100 probes uniform in the cell with z in [-1, 1] um.  The model
+is set up once; each point moves it to a new wave (bq_set_incidence) and solves on the
+resident device session.  The same points are solved first on the plain FP64 path (timed:
+fp64_value) and then on the mixed-precision path (timed: value): complex FP32 factors of the
+single-object reduction as the preconditioner of FP64 GMRES on the unreduced system.  Each
+mixed result is compared with the FP64 result of the same point: surface solution (U, dU, E,
+dE, H, dH) and probe E, H, max |diff| / max |FP64| per array.
+Workload c4s: the same at 6 x 8 x 8 quads (per-evaluation counters); c1: the 384-quad C1
+sphere, one incidence (plain FP64); c2: the C2 two-layer sphere + cone scene (plain FP64).
 
 Prints ONE JSON line.  `roofline` describes the kernel with the largest share of device time
-(HIP events on the library stream around every launch); `cpu_baseline` times the CPU
+(HIP events on the session stream around every launch); `cpu_baseline` times the CPU
 restatement of the same path (oracle/) on a bounded sample of rows, scaled to one incidence.
 """
 from __future__ import annotations
@@ -69,6 +71,16 @@ WORKLOADS = {
     # cone triangles), same lattice, wavelength, materials and incidence: used to read the
     # QPGF kernels' per-evaluation counters, which depend on the lattice and the spread of
     # source-target offsets, not on the element count
+    # the 6144-quad sweep: one resident model, 16 seeded incidence points, FP64 and mixed
+    "c4": dict(kind="sweep", n=32, radius=0.45, n_sphere=1.5 + 0.001j, n0=1.0, d1=(1.0, 0.0), d2=(0.0, 1.0),
+               points=16, seed=4, probes=100, steps=3, warmup=1,
+               label="C4: 6x32x32 = 6144-quad cube-sphere r=0.45um n=1.5+0.001i in n0=1.0, square lattice "
+                     "1.0um; sweep of 16 points from default_rng(4): lambda0 U[0.4,0.8]um, theta U[0,40]deg, "
+                     "phi U[0,90]deg, TE/TM; 100 probes (EH_t type 1)"),
+    # the same sweep at 6x8x8 quads (per-evaluation counters, quick checks)
+    "c4s": dict(kind="sweep", n=8, radius=0.45, n_sphere=1.5 + 0.001j, n0=1.0, d1=(1.0, 0.0), d2=(0.0, 1.0),
+                points=16, seed=4, probes=100, steps=6, warmup=1,
+                label="C4 sweep physics on a 6x8x8 = 384-quad cube-sphere (counters and checks)"),
     "c2s": dict(kind="sphere_cone", n=8, cone_h=0.06, lam=0.55, n0=1.0, theta=np.radians(30), phi=np.radians(15),
                 pol="TM", d1=(0.9, 0.0), d2=(0.45, 0.779), probes=100, seed=1, steps=3, warmup=1,
                 label="C2 geometry at reduced resolution (384 quads per sphere surface, cone edge 0.06um), "
@@ -85,9 +97,46 @@ def _probes(w, n, seed):
     return np.ascontiguousarray(np.c_[xy, z])
 
 
+def sweep_points(w):
+    """The seeded (lambda0, theta, phi, polarisation) points of a sweep workload."""
+    rng = np.random.default_rng(w["seed"])
+    pts = []
+    for _ in range(w["points"]):
+        lam = float(rng.uniform(0.4, 0.8))
+        th = float(np.radians(rng.uniform(0.0, 40.0)))
+        ph = float(np.radians(rng.uniform(0.0, 90.0)))
+        pol = "TE" if rng.integers(2) == 0 else "TM"
+        pts.append((lam, th, ph, pol))
+    return pts
+
+
+def _set_point(m, p):
+    lam, th, ph, pol = p
+    ex, ey = (1.0, 0.0) if pol == "TM" else (0.0, 1.0)
+    m.set_incidence(lam, th, ph, e0x=ex, e0y=ey)
+
+
+def _surface(path):
+    from datfile import read_dat
+    d = read_dat(path)
+    return [{k: d["dom"][q][k][1:].copy() for k in ("U", "dU", "E", "dE", "H", "dH")} for q in range(d["MN"] + 1)]
+
+
+def _surface_err(a, b):
+    e = 0.0
+    for da, db in zip(a, b):
+        for k in da:
+            e = max(e, float(np.abs(da[k] - db[k]).max() / np.abs(db[k]).max()))
+    return e
+
+
 def _scene(w, d):
     from bem3_emf_qd2_amd import meshgen
-    if w["kind"] == "sphere_cone":
+    if w["kind"] == "sweep":
+        lam, th, ph, pol = sweep_points(w)[0]
+        meshgen.sphere_scene(d, w["n"], w["radius"], w["n_sphere"], lam, n0=w["n0"], theta=th, phi=ph, pol=pol,
+                             d1=w["d1"], d2=w["d2"])
+    elif w["kind"] == "sphere_cone":
         meshgen.sphere_cone_scene(d, n=w["n"], cone_h=w.get("cone_h", 0.03), lam=w["lam"], n0=w["n0"],
                                   theta=w["theta"], phi=w["phi"], pol=w["pol"], d1=w["d1"], d2=w["d2"])
     else:
@@ -142,8 +191,8 @@ def _roofline(ktimes, stats_per_step, steps, workload):
         r["note"] = "rocBLAS zgemm trailing updates of the blocked LU"
         return r
     # QPGF evaluations per launch, as the device counted them from the pair classification
-    evals = {"k_q0_gl": stats_per_step["n_qpgf_gl"], "k_q0_gh": stats_per_step["n_qpgf_gh"],
-             "k_q0_sing": stats_per_step["n_qpgf_sing"]}
+    evals = {"k_q0_gl": stats_per_step.get("n_qpgf_gl", 0), "k_q0_gh": stats_per_step.get("n_qpgf_gh", 0),
+             "k_q0_sing": stats_per_step.get("n_qpgf_sing", 0)}
     if name in evals:
         r["evals_per_launch"] = evals[name]
         fpe = FLOPS_PER_EVAL.get(workload, {}).get(name)
@@ -207,14 +256,193 @@ def _cpu_baseline(model_dat, w, probes, gpu_solve_dims):
                       f"; element rows scaled by row count, probes not included; {measured:.0f}s measured"}
 
 
+def run_sweep(args, w, bq, L):
+    """Sweep workload: one resident model, each step a new incidence point.  The FP64 path
+    runs the points first (warm-up, then timed), then the mixed path the same points; every
+    mixed result is compared with the FP64 result of its point."""
+    from bem3_emf_qd2_amd.solver import Model
+    pts = sweep_points(w)
+    if args.warmup + args.steps > len(pts):
+        raise SystemExit(f"bench.py: {w['points']} sweep points; --warmup + --steps must not exceed them")
+    warm, timed = pts[:args.warmup], pts[args.warmup:args.warmup + args.steps]
+    probes = _probes(w, w["probes"], w["seed"])
+    res = {"fp64": [], "mixed": []}
+    with tempfile.TemporaryDirectory() as td:
+        _scene(w, td)
+        m = Model.from_dir(td)
+        m.write(Path(td) / "setup.dat")
+
+        def step(p):
+            _set_point(m, p)
+            st = m.solve()
+            E, H, did = m.fields(probes, 1)
+            return st, np.r_[E.ravel(), H.ravel()]
+
+        kt = {}
+        modes = ["fp64", "mixed"] if args.precision == "mixed" else [args.precision]
+        with _QuietStdout():
+            for mode in modes:
+                bq.solve_precision(mode)
+                for i, p in enumerate(warm):
+                    step(p)
+                    print(f"bench.py: {mode} warm-up point {i + 1}/{len(warm)} done", file=sys.stderr, flush=True)
+                if mode == modes[-1]:
+                    L.bq_kernel_timing(1)
+                for i, p in enumerate(timed):
+                    t = time.perf_counter()
+                    st, f = step(p)
+                    dt = time.perf_counter() - t
+                    m.write(Path(td) / "pt.dat")
+                    res[mode].append(dict(lat=dt, st=st.as_dict(), probes=f, surf=_surface(Path(td) / "pt.dat")))
+                    print(f"bench.py: {mode} point {i + 1}/{len(timed)} lambda0={p[0]:.4f} theta="
+                          f"{np.degrees(p[1]):.2f} phi={np.degrees(p[2]):.2f} {p[3]}: {dt:.2f} s "
+                          f"(assembly {st.t_assemble_ms:.0f} ms, solve {st.t_solve_ms:.0f} ms, "
+                          f"GMRES {st.gmres_iterations}, eta {st.backward_error:.1e})", file=sys.stderr, flush=True)
+                if mode == modes[-1]:
+                    kt = _kernel_times(L)
+                    L.bq_kernel_timing(0)
+            bq.solve_precision("fp64")
+        sys.path.insert(0, str(ROOT / "tests"))
+        from datfile import read_dat
+        dd = read_dat(Path(td) / "setup.dat")
+        tris = int(np.sum(dd["ed"][1:, 3] == 0))
+        quads, NN = int(dd["Ne"]) - tris, int(dd["NN"])
+        main_mode = modes[-1]
+        rows = []
+        for i, p in enumerate(timed):
+            r = {"lambda0": p[0], "theta_deg": float(np.degrees(p[1])), "phi_deg": float(np.degrees(p[2])),
+                 "pol": p[3]}
+            for mode in modes:
+                q = res[mode][i]
+                r[f"ms_{mode}"] = q["lat"] * 1e3
+                r[f"assemble_ms_{mode}"] = q["st"]["t_assemble_ms"]
+                r[f"solve_ms_{mode}"] = q["st"]["t_solve_ms"]
+                r[f"backward_error_{mode}"] = q["st"]["backward_error"]
+            if "fp64" in res and main_mode != "fp64":
+                a, b = res[main_mode][i], res["fp64"][i]
+                n3 = len(b["probes"]) // 2
+                r["surface_rel_err_vs_fp64"] = _surface_err(a["surf"], b["surf"])
+                r["probe_E_rel_err_vs_fp64"] = float(np.abs(a["probes"][:n3] - b["probes"][:n3]).max() /
+                                                     np.abs(b["probes"][:n3]).max())
+                r["probe_H_rel_err_vs_fp64"] = float(np.abs(a["probes"][n3:] - b["probes"][n3:]).max() /
+                                                     np.abs(b["probes"][n3:]).max())
+                r["gmres_iterations"] = a["st"]["gmres_iterations"]
+            rows.append(r)
+        st_last = res[main_mode][-1]["st"]
+        roof = _roofline(kt, st_last, args.steps, args.workload)
+        cpu = None
+        if not args.no_cpu_baseline:
+            _set_point(m, timed[0])
+            m.write(Path(td) / "point.dat")
+            cpu = _cpu_baseline_pairs(Path(td) / "point.dat")
+        nbytes = m.device_bytes()
+        m.close()
+    lat = np.array([q["lat"] for q in res[main_mode]]) * 1e3
+    total = float(lat.sum() / 1e3)
+    out = {
+        "metric": "solved incidences/sec + p50/p99 latency at <=1e-8 rel err vs FP64 plain path",
+        "value": args.steps / total,
+        "unit": "incidences/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": total / args.steps * 1e3,
+        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64 factors + f64 (assembly, GMRES residual)" if main_mode == "mixed" else "f64",
+        "data": "synthetic (generated cube-sphere mesh and seeded plane-wave sweep)",
+        "config": {"workload": w["label"], "quads": quads, "triangles": tris, "element_nodes": NN,
+                   "na": 8 * NN, "probes": w["probes"], "sweep_points_timed": args.steps,
+                   "parallelism": "single GPU", "solve_precision": main_mode},
+        "max_rel_err_vs_fp64": {k: max(r.get(k, 0.0) for r in rows) for k in
+                                ("surface_rel_err_vs_fp64", "probe_E_rel_err_vs_fp64",
+                                 "probe_H_rel_err_vs_fp64")} if main_mode != "fp64" else None,
+        "points": rows,
+        "phase_ms_last_point": st_last,
+        "device_bytes": nbytes,
+        "kernels_ms_per_step": {k: v[0] / args.steps for k, v in sorted(kt.items(), key=lambda x: -x[1][0])},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    if "fp64" in res and main_mode != "fp64":
+        fl = np.array([q["lat"] for q in res["fp64"]])
+        out["fp64_value"] = args.steps / float(fl.sum())
+        out["fp64_latency_ms"] = {"p50": float(np.percentile(fl * 1e3, 50)),
+                                  "p99": float(np.percentile(fl * 1e3, 99))}
+        out["speedup_vs_fp64"] = out["value"] / out["fp64_value"]
+    print(json.dumps(out), flush=True)
+
+
+def _cpu_baseline_pairs(model_dat, budget_s=8.0):
+    """Oracle (CPU restatement of the reference's element-pair routines) on the box's host
+    cores: random (collocation node, element) pairs of every domain for about budget_s each,
+    scaled to all pairs; plus the reference's dense-equivalent solve of the 8 NN system (LU
+    timed at order 3072 and scaled by n^3).  One element row of the 6144-quad model alone
+    takes the oracle ~45 s per core, so pairs, not rows, are the sample."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    from concurrent.futures import ThreadPoolExecutor
+    from datfile import read_dat
+    from oracle_pipeline import OracleModel
+    import scipy.linalg as sla
+
+    cores = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
+    d = read_dat(model_dat)
+    o = OracleModel.from_dat(d)
+    rng = np.random.default_rng(7)
+    total, measured, parts = 0.0, 0.0, []
+    with ThreadPoolExecutor(cores) as ex:
+        for dom in range(o.MN + 1):
+            ne = o.dNe[dom]
+            sid = o.sid(dom)
+            live = o.live_rows(dom)
+            done, t0 = 0, time.perf_counter()
+            while time.perf_counter() - t0 < budget_s:
+                batch = []
+                for _ in range(4 * cores):
+                    while True:
+                        row = int(rng.integers(0, 4 * ne))
+                        if live[row]:
+                            break
+                    s = int(rng.integers(1, ne + 1))
+                    batch.append((sid[row // 4 + 1], row % 4, sid[s]))
+                list(ex.map(lambda q: o.pair(dom, q[0], q[1], q[2]), batch))
+                done += len(batch)
+            dt = time.perf_counter() - t0
+            npairs = int(live.sum()) * ne
+            measured += dt
+            total += dt * npairs / done
+            parts.append(f"domain {dom}: {done} of {npairs} pairs in {dt:.1f}s")
+    n_s = 3072
+    A = rng.standard_normal((n_s, n_s)) + 1j * rng.standard_normal((n_s, n_s))
+    t = time.perf_counter()
+    sla.lu_factor(A, check_finite=False)
+    dt = time.perf_counter() - t
+    na = 8 * o.NN
+    measured += dt
+    lu = dt * (na / n_s) ** 3
+    asm = total
+    total += lu
+    parts.append(f"dense LU of the 8NN = {na} system timed at {n_s} in {dt:.1f}s, scaled x{(na / n_s) ** 3:.0f} "
+                 f"= {lu:.0f}s (the reference's PARDISO solve of this dense-block system; its factors would "
+                 f"need {na * na * 16 / 2**30:.0f} GiB)")
+    return {"value": 1.0 / total, "unit": "incidences/s", "cores": cores, "kind": "port",
+            "seconds_per_incidence": total, "assembly_seconds": asm, "solve_seconds": lu,
+            "measured_seconds": measured,
+            "sample": "oracle/ (C restatement of the reference's pair routines, 12-point rule everywhere) on "
+                      "the same model at the first timed point's wave: " + "; ".join(parts) +
+                      "; pairs scaled by count; probes not included"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed steps (default per workload: c2 2, c1 10)")
     ap.add_argument("--warmup", type=int, default=None, help="untimed steps (default per workload: c2 1, c1 2)")
-    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="c4", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed", "auto"],
+    ap.add_argument("--precision", default=None, choices=["fp64", "mixed", "auto"],
                     help="global solve: the plain complex FP64 LU ('fp64', default: the C1 and C2 "
                          "configurations are the plain FP64 path), complex FP32 LU + FP64 iterative "
                          "refinement ('mixed'; one untimed FP64 step first gives rel_err_vs_fp64), or "
@@ -246,6 +474,11 @@ def main():
         args.steps = w["steps"]
     if args.warmup is None:
         args.warmup = w["warmup"]
+    if args.precision is None:
+        args.precision = "mixed" if w["kind"] == "sweep" else "fp64"
+    if w["kind"] == "sweep":
+        run_sweep(args, w, bq, L)
+        return
     probes = _probes(w, w["probes"], w["seed"])
     with tempfile.TemporaryDirectory() as td:
         _scene(w, td)

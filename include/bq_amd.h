@@ -66,11 +66,39 @@ int bq_assemble_domain_host(void *qd, int d, double *G, double *H, double *dG, d
 /* Counters and phase times (ms, device clock) of the last solve_bieq in this process:
  * returns a pointer to {double t_assemble_ms, t_solve_ms, t_fields_ms; int n_gh_pairs;
  * long long n_qpgf, n_qpgf_gl, n_qpgf_gh, n_qpgf_sing; int mixed, ir_iterations;
- * double ir_correction} (QPGF evaluations of the periodic domain, in total and by phase:
- * regular pairs with their distance-chosen Gauss rule, 64-point fallback rule, singular
+ * double ir_correction; int path, reused; double backward_error, t_factor_ms;
+ * int gmres_iterations, qpgf_error} (QPGF evaluations of the periodic domain, in total and by
+ * phase: regular pairs with their distance-chosen Gauss rule, 64-point fallback rule, singular
  * rules; whether the global system took the mixed-precision path, its refinement steps and
- * the last correction max|dx| / max|x|). */
+ * the last correction max|dx| / max|x|; the solve path (0 block elimination of order 2 NN,
+ * 1 single-object reduction of order NN); whether the matrices and factors were reused;
+ * the single-object path's normwise backward error of the unreduced system; the
+ * factorisation time; FGMRES steps; t_fields_ms = boundary E, H, dE, dH (host clock)). */
 const void *bq_solve_last_stats(void);
+
+/* Replace the incident plane wave of a set-up model (DQD2* after initialize_dqd2 or
+ * dat_read) by one wave: vacuum wavelength lambda0, direction theta, phi (rad), polarisation
+ * (e0x, e0y) in the wave's frame (complex, as in ipw.txt), power; the medium of the open
+ * region is kept.  Recomputes what the reference derives from the wave in initialize_dqd2
+ * (d3qd2_src/d3qd2_setup.c:121-128, 1309-1349): QPD2 (k, Bloch vector), every domain's wave
+ * number and the incident field Ui, dUi at the element nodes.  The next solve_bieq reuses the
+ * model's resident device session; when only the polarisation or the power changed it also
+ * reuses the coefficient matrices and their factors.  power = 0 means amplitude 1
+ * (power = n0 / 2, reference d3qd2_src/ipw.c:131).  Returns 0, or -1 for invalid arguments
+ * (non-positive lambda0, negative power, zero polarisation, or on a lattice a direction with
+ * cos theta = 0 or |cos theta| = 1, which the reference rejects, d3qd2_setup.c:1083-1104).
+ * No counterpart in the reference, which reads the wave from ipw.txt once. */
+int bq_set_incidence(void *qd, double lambda0, double theta, double phi, double e0x_re, double e0x_im,
+                     double e0y_re, double e0y_im, double power);
+
+/* Device memory of a model's resident solver session is released by finalize_dqd2 or here
+ * (no counterpart in the reference; for callers that keep many models). */
+void bq_release_solver(void *qd);
+/* Device bytes held by the resident solver session of qd (all sessions if qd is NULL). */
+long long bq_solver_bytes(void *qd);
+/* Statistics of qd's last solve_bieq (the bq_solve_last_stats layout) into out; -1 if qd
+ * has no session. */
+int bq_solve_stats(void *qd, void *out);
 
 /* Precision of the global (open-region) linear solve (no counterpart in the reference,
  * whose solver is PARDISO / LAPACKE_zgesv in FP64, d3qd2_src/d3qd2_mkl_solver.c:18-148):
