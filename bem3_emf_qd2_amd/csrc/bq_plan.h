@@ -30,6 +30,11 @@ typedef struct bq_qplan {
   double eps;              /* relative accuracy used in the Fourier/Ewald choice */
   double ph2re, ph2im;     /* exp(i vk . d2): Bloch phase step along d2 */
   double cell, rcell;      /* |det(d1,d2)| and |det(g1,g2)| (unit-cell areas) */
+  int *err;                /* device word (may be NULL): a kernel stores the reference's
+                              negative QPGF code there (atomic minimum) when a reciprocal
+                              order has beta_n^2 = k^2 - |k_t + G|^2 == 0 exactly (a Wood
+                              anomaly): -12 in the Fourier-domain sum, -23 in the Ewald
+                              spectral sum (qgf2_src/d3_qpgf_d2.h:113-123) */
 } bq_qplan;
 
 /* Relative cost of one lattice term of each kind, for choosing the Fourier-domain or the

@@ -124,7 +124,7 @@ __global__ void __launch_bounds__(256) k_classify(const double* __restrict__ E, 
 // the target element's bounding sphere (D = sqrt2 (1 + gap / a), rho_s = D + sqrt(D^2 - 1),
 // a the source's bounding radius), M grows like exp(k a rho / (2 sqrt2)) from the wave factor,
 // and a further factor 4 covers the growth of 1/R on the ellipse.  The pair takes the
-// smallest order of 6, 8, 10 whose bound is below 1e-15 of the integrand (under the
+// smallest order from 3 to 10 whose bound is below 1e-15 of the integrand (under the
 // rounding of the 12-point sum and the ~1e-14 truncation noise of the lattice sums
 // themselves), else 12; with full != 0 every pair takes 12.
 __device__ __forceinline__ void elem_sphere(const Elem& e, double (&c)[3], double& rad)
@@ -150,7 +150,7 @@ __device__ __forceinline__ int rule_from_gap(double gap, double rs, double k)
 {
   const double D = 1.4142135623730950488 * (1.0 + gap / rs), rho_s = D + sqrt(D * D - 1.0);
   const double c = k * rs * 0.35355339059327376220;  // k a / (2 sqrt 2)
-  for (int n = 6; n <= 10; n += 2) {
+  for (int n = kVruleMin; n <= kVruleMax; ++n) {
     const double rho = fmin((2.0 * n + 2.0) / fmax(c, 1e-300), 0.5 * rho_s);
     if (!(rho > 1.05)) continue;
     // log(4 x 4.27) + c rho - 2n log rho - log(rho^2 - 1) <= log(1e-15): below the rounding
@@ -939,8 +939,8 @@ extern "C" {
 
 int bq_set_quadrature(const double* xl, const double* wl, const double* xh, const double* wh)
 {
-  double xv[3][10] = {{0}}, wv[3][10] = {{0}};
-  for (int r = 0; r < 3; ++r) bq_gauss_legendre(6 + 2 * r, xv[r], wv[r]);
+  double xv[kVruleRows][10] = {{0}}, wv[kVruleRows][10] = {{0}};
+  for (int r = 0; r < kVruleRows; ++r) bq_gauss_legendre(kVruleMin + r, xv[r], wv[r]);
   if (bq::set_quad_tables(xl, wl, xh, wh) || bq::set_vrule_tables(&xv[0][0], &wv[0][0])) {
     bq::set_error("quadrature table upload failed");
     return -1;

@@ -23,8 +23,10 @@ constexpr double kSQ3 = 1.7320508075688772935274;  // sqrt(3)
 // includes this header owns a copy (no relocatable device code); the host fills it through
 // that unit's set_quad_tables() before launching.
 static __constant__ double c_xl[12], c_wl[12], c_xh[64], c_wh[64];
-// Gauss-Legendre 6, 8 and 10 (rows 0..2, padded to 10) for the distance-adaptive regular rule
-static __constant__ double c_xv[3][10], c_wv[3][10];
+// Gauss-Legendre of orders 3 to 10 (row n - 3, padded to 10) for the distance-adaptive
+// regular rule
+constexpr int kVruleMin = 3, kVruleMax = 10, kVruleRows = kVruleMax - kVruleMin + 1;
+static __constant__ double c_xv[kVruleRows][10], c_wv[kVruleRows][10];
 
 static inline int set_quad_tables(const double* xl, const double* wl, const double* xh, const double* wh)
 {
@@ -35,11 +37,11 @@ static inline int set_quad_tables(const double* xl, const double* wl, const doub
   return 0;
 }
 
-// xv, wv: [3][10], the rules of order 6, 8 and 10 in the first n entries of each row
+// xv, wv: [kVruleRows][10], the rule of order kVruleMin + row in the first entries of each row
 static inline int set_vrule_tables(const double* xv, const double* wv)
 {
-  if (hipMemcpyToSymbol(HIP_SYMBOL(c_xv), xv, 30 * sizeof(double)) != hipSuccess) return -1;
-  if (hipMemcpyToSymbol(HIP_SYMBOL(c_wv), wv, 30 * sizeof(double)) != hipSuccess) return -1;
+  if (hipMemcpyToSymbol(HIP_SYMBOL(c_xv), xv, kVruleRows * 10 * sizeof(double)) != hipSuccess) return -1;
+  if (hipMemcpyToSymbol(HIP_SYMBOL(c_wv), wv, kVruleRows * 10 * sizeof(double)) != hipSuccess) return -1;
   return 0;
 }
 
@@ -142,12 +144,12 @@ __device__ __forceinline__ void gauss_point(bool tri, bool gh, int p, double& z,
 
 __device__ __forceinline__ int gauss_points(bool tri, bool gh) { return (gh ? 4096 : 144) * (tri ? 3 : 1); }
 
-// point p of the Gauss rule of order n (6, 8, 10 or 12) over the element: n^2 tensor points
+// point p of the Gauss rule of order n (3 to 10, or 12) over the element: n^2 tensor points
 // on the square, 3 n^2 on the triangle
 __device__ __forceinline__ void gauss_point_n(bool tri, int n, int p, double& z, double& e, double& w)
 {
-  const double* x = (n == 12) ? c_xl : c_xv[(n - 6) >> 1];
-  const double* wq = (n == 12) ? c_wl : c_wv[(n - 6) >> 1];
+  const double* x = (n == 12) ? c_xl : c_xv[n - kVruleMin];
+  const double* wq = (n == 12) ? c_wl : c_wv[n - kVruleMin];
   if (tri) {
     tri_rule_point(p, n, x, wq, z, e, w);
   } else {

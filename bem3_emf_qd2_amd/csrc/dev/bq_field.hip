@@ -421,6 +421,16 @@ extern "C" int bq_field_ctx_create(const bq_fdata* f, void** handle)
     delete F;
     return -3;
   }
+  {
+    // the QPGF error word of this context's launches
+    const int zero = 0;
+    const int* perr = nullptr;
+    if (up(F, &zero, 1, &perr, s)) {
+      bq_field_ctx_destroy(F);
+      return -1;
+    }
+    C.plan.err = (int*)perr;
+  }
   size_t n1 = (size_t)f->Ne + 1;
   for (size_t i = 1; i < n1; ++i)
     if (f->E[24 * i + 21] != 0.0) C.has_tri = 1;
@@ -479,11 +489,18 @@ extern "C" int bq_field_eval(void* handle, int kind, int n, const double* pts, c
     BQ_HIP(hipMallocAsync((void**)&dp, sizeof(double) * 3 * n, s));
     BQ_HIP(hipMemcpyAsync(dp, pts, sizeof(double) * 3 * n, hipMemcpyHostToDevice, s));
   }
+  BQ_HIP(hipMemsetAsync(F->C.plan.err, 0, sizeof(int), s));
   if (bq_field_eval_dev(&F->C, kind, n, dp, dd, dt, dze, type, dout, s)) return -1;
+  int err = 0;
   BQ_HIP(hipMemcpyAsync(out, dout, sizeof(double) * nv * n, hipMemcpyDeviceToHost, s));
+  BQ_HIP(hipMemcpyAsync(&err, F->C.plan.err, sizeof(int), hipMemcpyDeviceToHost, s));
   for (void* q : {(void*)dp, (void*)dd, (void*)dout, (void*)dt, (void*)dze})
     if (q) BQ_HIP(hipFreeAsync(q, s));
   BQ_HIP(hipStreamSynchronize(s));
+  if (err) {
+    bq::set_error("field: quasi-periodic Green's function, a reciprocal order has beta_n == 0 (code %d)", err);
+    return err;
+  }
   return 0;
 }
 

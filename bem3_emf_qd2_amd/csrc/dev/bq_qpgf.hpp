@@ -68,6 +68,14 @@ __device__ __forceinline__ void row_walk(int lo, int hi, int c, cplx E0, cplx st
   }
 }
 
+// A reciprocal order with beta_n = 0 has no finite term: the reference returns -12 / -23 for it
+// (d3_qpgf_d2.c, Fourier-domain and Ewald spectral sums).  Here the term is taken as 0 and the
+// code stored in the plan's error word, which the host checks after the launch.
+__device__ __forceinline__ void qpgf_flag(const bq_qplan& P, int code)
+{
+  if (P.err) atomicMin(P.err, code);
+}
+
 __device__ __forceinline__ int row_centre(double ue, double e2, int lo, int hi)
 {
   const int c = (int)rint(-ue / e2);
@@ -105,10 +113,13 @@ __device__ void qpgf_fourier_t(const bq_qplan& P, double px, double py, double p
         const double b = sqrt(b2);
         dz = E * cexpi(b * az);
         F = dz * (1.0 / b);
-      } else {
+      } else if (b2 < 0.0) {
         const double b = sqrt(-b2);
         dz = E * exp(-b * az);
         F = mk(dz.im, -dz.re) * (1.0 / b);  // -i dz / b
+      } else {
+        qpgf_flag(P, -12);
+        dz = F = mk(0.0, 0.0);
       }
       S += F;
       Sx += F * ax;
@@ -160,6 +171,10 @@ __device__ __forceinline__ void spectral_term(const bq_qplan& P, double ax, doub
     // F = 2 ce / b (i ef Im w + ff);  Gz = 2 ce (ef Re w - ff)
     F = ce * (mk(ff.re, ef * w.im + ff.im) * (2.0 / b));
     Gz = ce * (mk(ef * w.re - ff.re, -ff.im) * 2.0);
+  } else if (b2 == 0.0) {
+    qpgf_flag(P, -23);
+    F = Gz = mk(0.0, 0.0);
+    g0 = 0.0;
   } else {
     double b = sqrt(-b2), bv = b * ve;
     double a1 = bv + az, a2 = bv - az;
@@ -244,7 +259,10 @@ __device__ __forceinline__ void spectral_term_z0(const bq_qplan& P, double ax, d
 #pragma clang fp contract(off)
   const double b2 = P.k2 - ax * ax - ay * ay;
   const cplx ce = cexpi(ax * px + ay * py);
-  if (prop) {
+  if (b2 == 0.0) {
+    qpgf_flag(P, -23);
+    F = Gz = mk(0.0, 0.0);
+  } else if (prop) {
     const double b = sqrt(b2), ef = exp_rn(b2 * ve * ve);
     const double x = b * ve;
     const double wre = exp_rn(-(x * x)), wim = faddeeva_w(x, 0.0).im;

@@ -3,6 +3,7 @@
 #include <cstdarg>
 #include <cstring>
 #include <mutex>
+#include <algorithm>
 #include "bq_runtime.hpp"
 #include "bq_qpgf.hpp"
 #include "../../../include/bq_amd.h"
@@ -252,15 +253,26 @@ int bq_qpgf_batch(const double* qd8, int variant, int n, const double* r, double
   if (bq::ensure_device()) return -1;
   hipStream_t s = bq::stream();
   double *dr = nullptr, *dout = nullptr;
+  int* derr = nullptr;
   BQ_HIP(hipMallocAsync((void**)&dr, sizeof(double) * 3 * (size_t)n, s));
   const size_t nv = variant == 2 ? 20 : 8;
   BQ_HIP(hipMallocAsync((void**)&dout, sizeof(double) * nv * (size_t)n, s));
+  BQ_HIP(hipMallocAsync((void**)&derr, sizeof(int), s));
+  BQ_HIP(hipMemsetAsync(derr, 0, sizeof(int), s));
+  P.err = derr;
   BQ_HIP(hipMemcpyAsync(dr, r, sizeof(double) * 3 * (size_t)n, hipMemcpyHostToDevice, s));
   if (bq_qpgf_v1_dev(&P, variant, n, dr, dout, s)) return -1;
+  int err = 0;
   BQ_HIP(hipMemcpyAsync(out, dout, sizeof(double) * nv * (size_t)n, hipMemcpyDeviceToHost, s));
+  BQ_HIP(hipMemcpyAsync(&err, derr, sizeof(int), hipMemcpyDeviceToHost, s));
   BQ_HIP(hipFreeAsync(dr, s));
   BQ_HIP(hipFreeAsync(dout, s));
+  BQ_HIP(hipFreeAsync(derr, s));
   BQ_HIP(hipStreamSynchronize(s));
+  if (err) {
+    bq::set_error("quasi-periodic Green's function: a reciprocal order has beta_n == 0 (code %d)", err);
+    return err;
+  }
   return 0;
 }
 

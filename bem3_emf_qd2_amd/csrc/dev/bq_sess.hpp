@@ -62,6 +62,7 @@ struct bq_sess {
   bool assembled = false;
   std::vector<double> key_asm;
   std::vector<double> key_now;
+  int* qerr = nullptr;       // QPGF error word of the assembly launches (device)
   void* work = nullptr;      // assembly work buffer
   size_t work_bytes = 0;
   bq_reduced red;

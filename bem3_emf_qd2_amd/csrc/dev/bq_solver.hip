@@ -385,6 +385,7 @@ void bq_sess_destroy(bq_sess* S)
   (void)hipStreamSynchronize(S->s);
   bq::reduced_free(S);
   if (S->work) (void)hipFree(S->work);
+  if (S->qerr) (void)hipFree(S->qerr);
   auto f = [](void* p) { if (p) (void)hipFree(p); };
   f(S->E); f(S->ren); f(S->nrm); f(S->Ui); f(S->dUi); f(S->kn); f(S->eni); f(S->md); f(S->sd);
   for (auto p : S->sid) f(p);
@@ -415,6 +416,10 @@ int bq_sess_assemble(bq_sess* S)
     S->work_bytes = wmax;
     S->bytes += wmax;
   }
+  if (!S->qerr) BQ_HIP(hipMalloc((void**)&S->qerr, sizeof(int)));
+  BQ_HIP(hipMemsetAsync(S->qerr, 0, sizeof(int), s));
+  S->plan.err = S->qerr;
+  S->st.qpgf_error = 0;
   void* work = S->work;
   S->assembled = false;
   S->red.prec = -1;
@@ -469,6 +474,19 @@ int bq_sess_assemble(bq_sess* S)
   float ms = 0;
   BQ_HIP(hipEventElapsedTime(&ms, e0, e1));
   S->st.t_assemble_ms = ms;
+  {
+    int qe = 0;
+    BQ_HIP(hipMemcpyAsync(&qe, S->qerr, sizeof(int), hipMemcpyDeviceToHost, s));
+    BQ_HIP(hipStreamSynchronize(s));
+    if (qe) {
+      // the reference's callers print the QPGF code and exit (q0_bil_dcoef_v2_cc.c:177-180)
+      S->st.qpgf_error = qe;
+      bq::set_error("d3hm_qpgf_d2_v1() error. err=%d (a reciprocal order has beta_n == 0)", qe);
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      return qe;
+    }
+  }
   S->assembled = true;
   S->key_asm = S->key_now;
   (void)hipEventDestroy(e0);

@@ -13,6 +13,7 @@ int bq_qplan_init(bq_qplan *p, double k, const double *vk, const double *vd1, co
   if (det == 0.0 || !(k > 0.0)) return -1;
   p->k = k;
   p->k2 = k * k;
+  p->err = 0;
   p->vk0 = vk[0];
   p->vk1 = vk[1];
   p->d1x = vd1[0]; p->d1y = vd1[1];

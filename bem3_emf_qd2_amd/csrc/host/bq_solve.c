@@ -204,10 +204,16 @@ static void eh_from(const double *a, const double complex *U, const double compl
 #undef GR
 }
 
-static void boundary_EH(DQD2 *md, bq_sess *S)
+static int boundary_EH(DQD2 *md, bq_sess *S)
 {
   double complex ch = md->pw.lambda_0 / (2.0 * M_PI * I);
   BOUD *bd = &md->bd;
+  /* position of each element in the open region's list (0: not there) */
+  int *pos0 = calloc((size_t)bd->Ne + 1, sizeof(int));
+  for (int s = 1; s <= bd->sb[0].Ne; s++) {
+    int e = bd->sb[0].sid[s];
+    if (e > 0 && e <= bd->Ne && pos0[e] == 0) pos0[e] = s;
+  }
   for (int d = 1; d <= md->MN; d++) {
     SUBD *sb = &bd->sb[d];
     int Ne = sb->Ne;
@@ -221,8 +227,14 @@ static void boundary_EH(DQD2 *md, bq_sess *S)
           u[(size_t)i * N + 4 * (s - 1) + sn] = sb->U[s][sn][i];
           du[(size_t)i * N + 4 * (s - 1) + sn] = sb->dU[s][sn][i];
         }
-    if (bq_sess_dudt(S, d, (double *)u, (double *)du, (double *)raw)) gpu_fail("boundary field products");
-    if (bq_sess_get_dGH(S, d, NULL, NULL, tdF)) gpu_fail("tdF download");
+    if (bq_sess_dudt(S, d, (double *)u, (double *)du, (double *)raw) || bq_sess_get_dGH(S, d, NULL, NULL, tdF)) {
+      free(pos0);
+      free(u);
+      free(du);
+      free(raw);
+      free(tdF);
+      return -1;
+    }
     for (int t = 1; t <= Ne; t++) {
       int at = sb->sid[t], atd = abs(at);
       double sig = at < 0 ? -1.0 : 1.0;
@@ -246,10 +258,8 @@ static void boundary_EH(DQD2 *md, bq_sess *S)
         double complex e[3], h[3], dUiz[3], dUie[3], dh[3];
         calc_mipw_EH_dEHdv(e, h, dUiz, dh, bd->ren[atd][tn], vtz, &md->pw);
         calc_mipw_EH_dEHdv(e, h, dUie, dh, bd->ren[atd][tn], vte, &md->pw);
-        int s;
-        for (s = 1; s <= bd->sb[0].Ne; s++)
-          if (atd == bd->sb[0].sid[s]) break;
-        if (s > bd->sb[0].Ne) {
+        int s = pos0[atd];
+        if (s <= 0) {
           printf("solve_bieq(), open region id error! id=%d. Exit...\n", s);
           exit(0);
         }
@@ -267,9 +277,11 @@ static void boundary_EH(DQD2 *md, bq_sess *S)
     free(raw);
     free(tdF);
   }
+  free(pos0);
+  return 0;
 }
 
-static void boundary_dEH(DQD2 *md, bq_sess *S)
+static int boundary_dEH(DQD2 *md, bq_sess *S)
 {
   BOUD *bd = &md->bd;
   for (int d = 0; d <= md->MN; d++) {
@@ -284,7 +296,11 @@ static void boundary_dEH(DQD2 *md, bq_sess *S)
           eh[(size_t)i * N + 4 * (s - 1) + sn] = sb->E[s][sn][i];
           eh[(size_t)(i + 3) * N + 4 * (s - 1) + sn] = sb->H[s][sn][i];
         }
-    if (bq_sess_deh(S, d, (double *)eh, (double *)x)) gpu_fail("dE/dH solve");
+    if (bq_sess_deh(S, d, (double *)eh, (double *)x)) {
+      free(eh);
+      free(x);
+      return -1;
+    }
     for (int s = 1; s <= Ne; s++)
       for (int sn = 0; sn < 4; sn++)
         for (int i = 0; i < 3; i++) {
@@ -294,6 +310,7 @@ static void boundary_dEH(DQD2 *md, bq_sess *S)
     free(eh);
     free(x);
   }
+  return 0;
 }
 
 void bq_release_device_state(DQD2 *qd); /* bq_field.c */
@@ -383,41 +400,54 @@ long long bq_solver_bytes(void *qdv)
   return b;
 }
 
-void solve_bieq(DQD2 *qd)
+/* The reference's solve_bieq pipeline; progress lines when verbose.  Returns 0, or a negative
+ * code (the reference's QPGF codes -12 / -23, -1 device failure, -3 singular system) with the
+ * reason in bq_last_error(). */
+static int solve_core(DQD2 *qd, int verbose, const char **stage)
 {
   double t0 = now_s(), ts;
   /* the solution arrays are rewritten in place: the field evaluator's copy of the previous
      solution must not be reused */
   bq_release_device_state(qd);
-  printf("\nsolve modified electromagnetic potential boundary value \n");
-  printf("  coefficient matrix          ");
-  fflush(stdout);
+  if (verbose) {
+    printf("\nsolve modified electromagnetic potential boundary value \n");
+    printf("  coefficient matrix          ");
+    fflush(stdout);
+  }
   ts = now_s();
+  *stage = "device set-up";
   bq_sess *S = session_for(qd);
-  if (!S) gpu_fail("device set-up");
-  int reused = bq_sess_assembled(S);
-  if (!reused && bq_sess_assemble(S)) gpu_fail("coefficient matrix assembly");
-  printf("finished. Elapsed time : %5g (sec)\n", now_s() - ts);
-
-  printf("  solve mEMP boundary value   ");
-  fflush(stdout);
+  if (!S) return -1;
+  int reused = bq_sess_assembled(S), rc = 0;
+  *stage = "coefficient matrix assembly";
+  if (!reused && (rc = bq_sess_assemble(S)) != 0) return rc < 0 ? rc : -1;
+  if (verbose) {
+    printf("finished. Elapsed time : %5g (sec)\n", now_s() - ts);
+    printf("  solve mEMP boundary value   ");
+    fflush(stdout);
+  }
   ts = now_s();
   size_t na = 8 * (size_t)qd->bd.NN;
   double complex *X = calloc(na, sizeof(double complex));
-  int rc = bq_sess_solve_global(S, (double *)X);
-  if (rc) gpu_fail("global system solve");
+  *stage = "global system solve";
+  rc = bq_sess_solve_global(S, (double *)X);
+  if (rc) {
+    free(X);
+    return rc < 0 ? rc : -1;
+  }
   store_solution(qd, X);
   free(X);
-  printf("finished. Elapsed time : %5g (sec)\n", now_s() - ts);
-
-  printf("solve electromagnteic field boundary value \n");
-  printf("  solve EH boundary value     ");
-  fflush(stdout);
+  if (verbose) {
+    printf("finished. Elapsed time : %5g (sec)\n", now_s() - ts);
+    printf("solve electromagnteic field boundary value \n");
+    printf("  solve EH boundary value     ");
+    fflush(stdout);
+  }
   ts = now_s();
-  boundary_EH(qd, S);
-  boundary_dEH(qd, S);
+  *stage = "boundary field";
+  if (boundary_EH(qd, S) || boundary_dEH(qd, S)) return -1;
   double tf = now_s() - ts;
-  printf("finished. Elapsed time : %5g (sec)\n", tf);
+  if (verbose) printf("finished. Elapsed time : %5g (sec)\n", tf);
   bq_stats st = *bq_sess_stats(S);
   st.t_fields_ms = tf * 1e3;
   st.reused = reused;
@@ -427,7 +457,20 @@ void solve_bieq(DQD2 *qd)
   for (sess_entry *e = g_sess; e; e = e->next)
     if (e->qd == qd) e->last = st;
   pthread_mutex_unlock(&g_smu);
-  printf("Total elapsed time : %g (sec)\n", now_s() - t0);
+  if (verbose) printf("Total elapsed time : %g (sec)\n", now_s() - t0);
+  return 0;
+}
+
+void solve_bieq(DQD2 *qd)
+{
+  const char *stage = "";
+  if (solve_core(qd, 1, &stage)) gpu_fail(stage);
+}
+
+int bq_solve_bieq_rc(void *qd)
+{
+  const char *stage = "";
+  return solve_core((DQD2 *)qd, 0, &stage);
 }
 
 int bq_solve_stats(void *qdv, void *out)

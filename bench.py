@@ -261,6 +261,7 @@ def run_sweep(args, w, bq, L):
     runs the points first (warm-up, then timed), then the mixed path the same points; every
     mixed result is compared with the FP64 result of its point."""
     from bem3_emf_qd2_amd.solver import Model
+    sys.path.insert(0, str(ROOT / "tests"))
     pts = sweep_points(w)
     if args.warmup + args.steps > len(pts):
         raise SystemExit(f"bench.py: {w['points']} sweep points; --warmup + --steps must not exceed them")
@@ -282,6 +283,10 @@ def run_sweep(args, w, bq, L):
         modes = ["fp64", "mixed"] if args.precision == "mixed" else [args.precision]
         with _QuietStdout():
             for mode in modes:
+                # each pass starts from a fresh session, so that no timed point reuses the
+                # matrices the other pass assembled for the same wave (allocation happens in
+                # the untimed warm-up)
+                m.release_device()
                 bq.solve_precision(mode)
                 for i, p in enumerate(warm):
                     step(p)

@@ -32,7 +32,10 @@ int bq_qpgf_v1_batch(const double *qd8, int n, const double *r, double *out);
  * reference's variants differ: variant 0 = d3hm_qpgf_d2_v1 / _dqG (their spectral sum
  * stops early there, d3_qpgf_d2.c:1840-1963), variant 1 = d3hm_qpgf_d2_qG (converged,
  * d3_qpgf_d2.c:1190-1335).  Elsewhere both give the converged sum.  variant 2 =
- * d3hm_qpgf_d2_v2 (d3_qpgf_d2.c:178-230): out[n][20] adds d2qG xx, yy, zz, xy, yz, zx. */
+ * d3hm_qpgf_d2_v2 (d3_qpgf_d2.c:178-230): out[n][20] adds d2qG xx, yy, zz, xy, yz, zx.
+ * Returns the reference's code -12 (Fourier-domain sum) or -23 (Ewald sum) when a reciprocal
+ * order has beta_n == 0 exactly (qgf2_src/d3_qpgf_d2.h:113-123); out then holds finite values
+ * without that order's term. */
 int bq_qpgf_batch(const double *qd8, int variant, int n, const double *r, double *out);
 
 /* sizeof(DQD2) as compiled into the library (for callers that allocate it opaquely). */
@@ -90,6 +93,13 @@ const void *bq_solve_last_stats(void);
  * No counterpart in the reference, which reads the wave from ipw.txt once. */
 int bq_set_incidence(void *qd, double lambda0, double theta, double phi, double e0x_re, double e0x_im,
                      double e0y_re, double e0y_im, double power);
+
+/* solve_bieq without its progress lines and without exit(): returns 0, or a negative code
+ * with the reason in bq_last_error() (the reference's QPGF codes -12 / -23 when a reciprocal
+ * order has beta_n == 0, -3 for a singular system, -1 for a device failure).  For request loops
+ * that must survive a failed request; solve_bieq itself prints and exits as the reference does
+ * (d3qd2_src/q0_bil_dcoef_v2_cc.c:177-180). */
+int bq_solve_bieq_rc(void *qd);
 
 /* Device memory of a model's resident solver session is released by finalize_dqd2 or here
  * (no counterpart in the reference; for callers that keep many models). */

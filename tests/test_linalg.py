@@ -99,3 +99,61 @@ def test_gpu_small_rhs_solve_matches_blocked_solve(gpu_lib, n):
     assert np.abs(X8 - X9[:, :8]).max() <= 1e-12 * np.abs(X9).max()
     bw = np.abs(A @ X8 - B[:, :8]).max() / (np.abs(A).max() * np.abs(X8).max() * n)
     assert bw < 1e-13
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("single", [0, 1])
+def test_gpu_getrs_column_groups(gpu_lib, single):
+    """Solves whose right-hand sides come in column groups that are zero in the permuted rows
+    above gf (bq_zgetrs_groups_dev / bq_cgetrs_groups_dev, used for Z = M^-1 K in the block
+    elimination): the forward solve of a group starts at the 512-row block holding gf.  With
+    n = 2048, groups starting at gf = 0, 700 and 1500 (so the skip branch runs for two of
+    them) and right-hand sides built zero in the permuted rows above their gf, the grouped
+    solve must equal the plain one: 1e-12 (FP64), 1e-5 (FP32) of the largest entry."""
+    import ctypes
+    from hipmem import DeviceArray
+    L = gpu_lib.lib()
+    n, ng, per = 2048, 3, 40
+    rng = np.random.default_rng(5 + single)
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)) + 8.0 * np.sqrt(n) * np.eye(n)[::-1]
+    cdt, rdt = (np.complex64, np.float32) if single else (np.complex128, np.float64)
+    dA = DeviceArray(np.asfortranarray(A).T.astype(cdt))  # column-major storage
+    L.bq_lu_inv_doubles.restype = ctypes.c_long
+    ninv = L.bq_lu_inv_doubles(n)
+    dinv = DeviceArray(np.zeros(ninv, rdt))
+    dip = DeviceArray(np.zeros(n, np.int32))
+    getrf = L.bq_cgetrf_dev if single else L.bq_zgetrf_dev
+    getrs = L.bq_cgetrs_dev if single else L.bq_zgetrs_dev
+    getrs_g = L.bq_cgetrs_groups_dev if single else L.bq_zgetrs_groups_dev
+    V = ctypes.c_void_p
+    getrf.argtypes = [V, ctypes.c_int, V, ctypes.c_long, V, V]
+    getrs.argtypes = [V, ctypes.c_int, ctypes.c_int, V, ctypes.c_long, V, V, V, ctypes.c_long]
+    I = ctypes.POINTER(ctypes.c_int)
+    getrs_g.argtypes = getrs.argtypes + [ctypes.c_int, I, I, I]
+    L.bq_lu_perm_dev.argtypes = [V, ctypes.c_int, ctypes.c_int]
+    L.bq_lu_perm_dev.restype = V
+    assert getrf(None, n, dA.ptr, n, dip.ptr, dinv.ptr) == 0
+    off = L.bq_lu_perm_dev(dinv.ptr, n, single) - dinv.ptr.value
+    raw = dinv.get().view(np.uint8)[off:off + 4 * n].view(np.int32)
+    assert sorted(raw.tolist()) == list(range(n))
+    assert not np.array_equal(raw, np.arange(n))  # the anti-diagonal forces interchanges
+    gfs = [0, 700, 1500]
+    B = rng.standard_normal((n, ng * per)) + 1j * rng.standard_normal((n, ng * per))
+    for g, gf in enumerate(gfs):
+        B[raw[:gf], g * per:(g + 1) * per] = 0.0  # zero in permuted rows 0 .. gf-1
+    dB1 = DeviceArray(np.asfortranarray(B).T.astype(cdt))
+    dB2 = DeviceArray(np.asfortranarray(B).T.astype(cdt))
+    assert getrs(None, n, ng * per, dA.ptr, n, dip.ptr, dinv.ptr, dB1.ptr, n) == 0
+    g0 = (ctypes.c_int * ng)(*[g * per for g in range(ng)])
+    gn = (ctypes.c_int * ng)(*([per] * ng))
+    gf = (ctypes.c_int * ng)(*gfs)
+    assert getrs_g(None, n, ng * per, dA.ptr, n, dip.ptr, dinv.ptr, dB2.ptr, n, ng, g0, gn, gf) == 0
+    X1, X2 = dB1.get(), dB2.get()
+    for d in (dA, dinv, dip, dB1, dB2):
+        d.free()
+    err = np.abs(X1 - X2).max() / np.abs(X1).max()
+    ref = np.linalg.solve(A, B).T
+    fw = np.abs(X1 - ref).max() / np.abs(ref).max()
+    print("single" if single else "double", "grouped vs plain", err, "plain vs numpy", fw)
+    assert err < (1e-5 if single else 1e-12)
+    assert fw < (1e-4 if single else 1e-10)

@@ -165,3 +165,31 @@ def test_gpu_vs_oracle_many(gpu_lib):
         b = oracle_lib.qpgf(qd, rr)[0][:, :4]
         e = _err(a, b)
         assert e.max() < TOL, (name, rr[e.argmax()], e.max())
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("variant", [0, 1, 2])
+def test_gpu_qpgf_wood_anomaly_returns_reference_code(gpu_lib, variant):
+    """A reciprocal order with beta_n^2 = k^2 - |k_t + G|^2 == 0 exactly has no finite lattice
+    term.  The reference returns -12 (Fourier-domain sum) or -23 (Ewald sum) and its callers
+    exit (qgf2_src/d3_qpgf_d2.h:113-123, d3qd2_src/q0_bil_dcoef_v2_cc.c:177-180).  Here:
+    k = 1.5 on the unit square lattice with k_t = (2 pi - 1.5, 0), so the order (1, 0) has
+    k_t + G = (1.5, 0) in exact arithmetic and in floating point.  The batch returns that code
+    and its outputs stay finite (no Inf or NaN reaches a caller that ignores it); a plan one
+    ulp away returns 0."""
+    import ctypes
+    D = ctypes.POINTER(ctypes.c_double)
+    L = gpu_lib.lib()
+    r = np.array([[0.1, 0.2, 0.5], [0.3, -0.2, 0.05], [0.2, 0.1, 0.0], [-0.4, 0.3, -1.2]])
+    nv = 20 if variant == 2 else 8
+    for vk0, expect_err in ((2 * np.pi - 1.5, True), (np.nextafter(2 * np.pi - 1.5, 10.0), False)):
+        qd = np.array([1.5, vk0, 0.0, 0.1, 1.0, 0.0, 0.0, 1.0])
+        out = np.zeros((len(r), nv))
+        rc = L.bq_qpgf_batch(qd.ctypes.data_as(D), variant, len(r), np.ascontiguousarray(r).ctypes.data_as(D),
+                             out.ctypes.data_as(D))
+        print(variant, vk0, rc, L.bq_last_error().decode() if rc else "")
+        assert np.all(np.isfinite(out))
+        if expect_err:
+            assert rc in (-12, -23)
+        else:
+            assert rc == 0
