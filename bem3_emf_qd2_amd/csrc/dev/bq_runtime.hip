@@ -198,6 +198,20 @@ __global__ void __launch_bounds__(256) k_qpgf_v2(bq_qplan P, int n, const double
   for (int c = 0; c < 6; ++c) { o[8 + 2 * c] = q.h[c].re; o[9 + 2 * c] = q.h[c].im; }
 }
 
+// ---- allocation-free QPGF batch ------------------------------------------------------
+// One error word per chunk of kNoAllocChunk points in a file-scope device array (each device
+// holds its own copy); its device address is looked up once per device and used for every
+// launch and copy.  Calls that overlap on one device share the array and race.
+constexpr int kNoAllocChunk = 1 << 20, kNoAllocWords = 64;
+static __device__ __align__(256) int scratch[kNoAllocWords];
+static int* g_scratch_ptr[64];
+
+__global__ void k_fill_int(int* __restrict__ p, int n, int v)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
 extern "C" {
 
 const char* bq_last_error(void) { return bq::g_err; }
@@ -269,6 +283,62 @@ int bq_qpgf_batch(const double* qd8, int variant, int n, const double* r, double
   BQ_HIP(hipFreeAsync(dout, s));
   BQ_HIP(hipFreeAsync(derr, s));
   BQ_HIP(hipStreamSynchronize(s));
+  if (err) {
+    bq::set_error("quasi-periodic Green's function: a reciprocal order has beta_n == 0 (code %d)", err);
+    return err;
+  }
+  return 0;
+}
+
+int bq_qpgf_batch_noalloc(const double* qd8, int variant, int n, const double* d_r, double* d_out, void* stream)
+{
+  if (n < 0 || !qd8 || variant < 0 || variant > 2 || (n > 0 && (!d_r || !d_out))) {
+    bq::set_error("bq_qpgf_batch_noalloc: bad arguments");
+    return -2;
+  }
+  if ((long)n > (long)kNoAllocChunk * kNoAllocWords) {
+    bq::set_error("bq_qpgf_batch_noalloc: %d points exceed the maximum of %ld (%d chunks of %d)", n,
+                  (long)kNoAllocChunk * kNoAllocWords, kNoAllocWords, kNoAllocChunk);
+    return -2;
+  }
+  bq_qplan P;
+  if (bq_qpgf_plan(qd8, 1e-15, &P)) {
+    bq::set_error("bq_qpgf_batch_noalloc: degenerate lattice");
+    return -3;
+  }
+  if (n == 0) return 0;
+  if (bq::ensure_device()) return -1;
+  int dev = 0;
+  BQ_HIP(hipGetDevice(&dev));
+  {
+    std::lock_guard<std::mutex> lk(bq::g_mu);
+    if (!g_scratch_ptr[dev]) {
+      void* p = nullptr;
+      const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+      if (e != hipSuccess || !p) {
+        bq::set_error("bq_qpgf_batch_noalloc: hipGetSymbolAddress failed (%s)", hipGetErrorString(e));
+        return -1;
+      }
+      g_scratch_ptr[dev] = (int*)p;
+    }
+  }
+  int* ptr = g_scratch_ptr[dev];
+  hipStream_t s = (hipStream_t)stream;
+  const int nch = (int)(((long)n + kNoAllocChunk - 1) / kNoAllocChunk);
+  const size_t nv = variant == 2 ? 20 : 8;
+  k_fill_int<<<1, kNoAllocWords, 0, s>>>(ptr, nch, 0);
+  BQ_HIP(hipGetLastError());
+  for (int c = 0; c < nch; ++c) {
+    const long p0 = (long)c * kNoAllocChunk;
+    const int m = (int)std::min((long)kNoAllocChunk, (long)n - p0);
+    P.err = ptr + c;
+    if (bq_qpgf_v1_dev(&P, variant, m, d_r + 3 * p0, d_out + nv * p0, s)) return -1;
+  }
+  int h[kNoAllocWords];
+  BQ_HIP(hipMemcpyAsync(h, ptr, sizeof(int) * nch, hipMemcpyDeviceToHost, s));
+  BQ_HIP(hipStreamSynchronize(s));
+  int err = 0;
+  for (int c = 0; c < nch; ++c) err = std::min(err, h[c]);
   if (err) {
     bq::set_error("quasi-periodic Green's function: a reciprocal order has beta_n == 0 (code %d)", err);
     return err;

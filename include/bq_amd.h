@@ -38,6 +38,16 @@ int bq_qpgf_v1_batch(const double *qd8, int n, const double *r, double *out);
  * without that order's term. */
 int bq_qpgf_batch(const double *qd8, int variant, int n, const double *r, double *out);
 
+/* bq_qpgf_batch without any device allocation (no counterpart in the reference: for callers
+ * that cannot allocate, such as code under HIP graph capture or a serving loop that forbids
+ * hipMalloc).  d_r [n][3] and d_out [n][8] (variant 2: [n][20]) are device buffers of the
+ * caller; the work is enqueued on stream (a hipStream_t, NULL for the default stream), which
+ * is synchronized before returning.  The only scratch is one error word per chunk of 2^20
+ * points in a fixed 256-byte device array of the library: n is limited to 64 x 2^20 points
+ * (larger n returns -2 without launching).  Every call on a device shares that array, so calls
+ * that overlap on one device race.  Returns 0, -12 / -23 as bq_qpgf_batch, or < 0 on error. */
+int bq_qpgf_batch_noalloc(const double *qd8, int variant, int n, const double *d_r, double *d_out, void *stream);
+
 /* sizeof(DQD2) as compiled into the library (for callers that allocate it opaquely). */
 int bq_sizeof_dqd2(void);
 
