@@ -131,3 +131,13 @@ def solve_precision(mode=None):
     block systems of order 2 NN >= 16384, FP64 below)."""
     m = {None: -1, "fp64": 0, "mixed": 1, "auto": 2}[mode]
     return ["fp64", "mixed", "auto"][lib().bq_solve_precision(m)]
+
+
+def assembly_tolerance(tol=None) -> float:
+    """Query (None) or set the assembly tolerance (bq_assembly_tolerance): 0 = the reference's
+    accuracy; 0 < tol < 1e-3 = matrices to about tol of their largest entries, with the
+    lattice-sum cut tuned per incidence."""
+    L = lib()
+    L.bq_assembly_tolerance.argtypes = [ctypes.c_double]
+    L.bq_assembly_tolerance.restype = ctypes.c_double
+    return float(L.bq_assembly_tolerance(-1.0 if tol is None else float(tol)))

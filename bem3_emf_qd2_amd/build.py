@@ -42,9 +42,12 @@ def _run(cmd: list[str]) -> tuple[list[str], int, str]:
     return cmd, p.returncode, p.stdout + p.stderr
 
 
+CXX_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-Wall", "-pthread", f"-I{ROOT / 'include'}", f"-I{CSRC}"]
+
+
 def sources() -> tuple[list[Path], list[Path]]:
     hip = sorted((CSRC / "dev").glob("*.hip"))
-    c = sorted((CSRC / "host").glob("*.c"))
+    c = sorted((CSRC / "host").glob("*.c")) + sorted((CSRC / "host").glob("*.cpp"))
     return hip, c
 
 
@@ -58,7 +61,10 @@ def build(verbose: bool = False) -> Path:
     for s in hip:
         jobs.append([HIPCC, *HIP_FLAGS, "-c", str(s), "-o", str(BUILD / (s.stem + ".hip.o"))])
     for s in c:
-        jobs.append(["gcc", *C_FLAGS, "-c", str(s), "-o", str(BUILD / (s.stem + ".c.o"))])
+        if s.suffix == ".cpp":
+            jobs.append(["g++", *CXX_FLAGS, "-c", str(s), "-o", str(BUILD / (s.stem + ".cpp.o"))])
+        else:
+            jobs.append(["gcc", *C_FLAGS, "-c", str(s), "-o", str(BUILD / (s.stem + ".c.o"))])
     workers = min(8, os.cpu_count() or 4)
     failed = []
     with cf.ThreadPoolExecutor(workers) as ex:

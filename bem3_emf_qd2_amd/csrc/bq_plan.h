@@ -11,9 +11,11 @@
 extern "C" {
 #endif
 
-/* Exponent cut for truncated lattice sums: terms below exp(-BQ_EXP_CUT) (4e-18) of the
- * leading term are dropped. */
+/* Default exponent cut for truncated lattice sums: terms below exp(-BQ_EXP_CUT) (4e-18) of
+ * the leading term are dropped (the plan's cut; a tuned plan may take a smaller one). */
 #define BQ_EXP_CUT 40.0
+/* Default error bound of the distance-chosen Gauss rule relative to the integrand: log(1e-15) */
+#define BQ_RULE_LTOL (-34.538776394910684)
 #define BQ_EW_EXPM 3.0 /* Ewald amplification limit, reference qgf2_src/d3_qpgf_d2.h:26 */
 #define BQ_S_LIMIT 15  /* reference qgf2_src/d3_qpgf_d2.h:19 */
 
@@ -30,6 +32,8 @@ typedef struct bq_qplan {
   double eps;              /* relative accuracy used in the Fourier/Ewald choice */
   double ph2re, ph2im;     /* exp(i vk . d2): Bloch phase step along d2 */
   double cell, rcell;      /* |det(d1,d2)| and |det(g1,g2)| (unit-cell areas) */
+  double cut;              /* lattice-sum truncation: terms below exp(-cut) are dropped */
+  double rule_ltol;        /* log of the error bound of the distance-chosen Gauss rule */
   int *err;                /* device word (may be NULL): a kernel stores the reference's
                               negative QPGF code there (atomic minimum) when a reciprocal
                               order has beta_n^2 = k^2 - |k_t + G|^2 == 0 exactly (a Wood

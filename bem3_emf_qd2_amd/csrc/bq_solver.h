@@ -46,6 +46,8 @@ typedef struct bq_stats {
   double t_factor_ms;    /* of t_solve_ms: the factorisation (0 when reused) */
   int gmres_iterations;  /* FGMRES steps of the mixed-precision single-object solve */
   int qpgf_error;        /* 0, or the reference's negative QPGF code met by the assembly */
+  double asm_tol;        /* assembly tolerance in force (0: the reference's accuracy) */
+  double asm_cut;        /* lattice-sum cut the assembly took (terms below exp(-cut) dropped) */
 } bq_stats;
 
 typedef struct bq_sess bq_sess;
@@ -80,6 +82,10 @@ int bq_sess_deh(bq_sess *s, int d, const double *EH, double *dEH);
 int bq_sess_get_GH(bq_sess *s, int d, double *G, double *H);
 int bq_sess_get_dGH(bq_sess *s, int d, double *dG, double *dH, double *tdF);
 const bq_stats *bq_sess_stats(const bq_sess *s);
+
+/* solve_bieq's phase gate (bq_solve.c): fn(phase, 1) before and fn(phase, 0) after the
+ * assembly (phase 0) and the factorisation + solve + boundary fields (phase 1); NULL: none */
+void bq_set_phase_gate(void (*fn)(int phase, int enter));
 
 #ifdef __cplusplus
 }

@@ -146,16 +146,16 @@ __device__ __forceinline__ void elem_sphere(const Elem& e, double (&c)[3], doubl
   }
 }
 
-__device__ __forceinline__ int rule_from_gap(double gap, double rs, double k)
+__device__ __forceinline__ int rule_from_gap(double gap, double rs, double k, double ltol)
 {
   const double D = 1.4142135623730950488 * (1.0 + gap / rs), rho_s = D + sqrt(D * D - 1.0);
   const double c = k * rs * 0.35355339059327376220;  // k a / (2 sqrt 2)
   for (int n = kVruleMin; n <= kVruleMax; ++n) {
     const double rho = fmin((2.0 * n + 2.0) / fmax(c, 1e-300), 0.5 * rho_s);
     if (!(rho > 1.05)) continue;
-    // log(4 x 4.27) + c rho - 2n log rho - log(rho^2 - 1) <= log(1e-15): below the rounding
-    // of the 144-point sum the 12-point rule forms (sqrt(144) x 2^-53 = 1.3e-15)
-    if (2.8379 + c * rho - 2.0 * n * log(rho) - log(rho * rho - 1.0) <= -34.5388) return n;
+    // log(4 x 4.27) + c rho - 2n log rho - log(rho^2 - 1) <= ltol; by default log(1e-15): below
+    // the rounding of the 144-point sum the 12-point rule forms (sqrt(144) x 2^-53 = 1.3e-15)
+    if (2.8379 + c * rho - 2.0 * n * log(rho) - log(rho * rho - 1.0) <= ltol) return n;
   }
   return 12;
 }
@@ -187,12 +187,13 @@ __device__ __forceinline__ int pair_rule(const bq_qplan& P, const Elem& et, cons
   const double zs = fabs(es.cr[2][1]) + fabs(es.cr[2][2]) + fabs(es.cr[2][3]);
   const double zt = fabs(et.cr[2][1]) + fabs(et.cr[2][2]) + fabs(et.cr[2][3]);
   if (zs <= 1e-6 * rs && fabs(et.cr[2][0] - es.cr[2][0]) <= zs + zt + 1e-6 * rs) return 12;
-  return rule_from_gap(gap, rs, P.k);
+  return rule_from_gap(gap, rs, P.k, P.rule_ltol);
 }
 
 // The same for a free-space pair of a bounded domain (complex k: |k| bounds the wave
 // factor's growth; no lattice images and no r_z == 0 branch in the kernel).
-__device__ __forceinline__ int pair_rule_fs(double kabs, const Elem& et, const Elem& es, int full)
+__device__ __forceinline__ int pair_rule_fs(double kabs, const Elem& et, const Elem& es, int full,
+                                            double ltol = BQ_RULE_LTOL)
 {
   if (full) return 12;
   double ct[3], cs[3], rt, rs;
@@ -201,7 +202,7 @@ __device__ __forceinline__ int pair_rule_fs(double kabs, const Elem& et, const E
   const double dx = ct[0] - cs[0], dy = ct[1] - cs[1], dz = ct[2] - cs[2];
   const double gap = sqrt(dx * dx + dy * dy + dz * dz) - rt - rs;
   if (!(gap > 0.0) || !(rs > 0.0)) return 12;
-  return rule_from_gap(gap, rs, kabs);
+  return rule_from_gap(gap, rs, kabs, ltol);
 }
 
 // ------------------------------------------------------------------------------------
@@ -532,7 +533,7 @@ __device__ __forceinline__ void fs_store(const double (&acc)[48], bool neg, long
 
 __global__ void __launch_bounds__(256) k_fs_gl(double kr, double ki, const double* __restrict__ E,
                                                const double* __restrict__ ren, const int* __restrict__ sid, int dNe,
-                                               const unsigned char* __restrict__ flag, int full_rule,
+                                               const unsigned char* __restrict__ flag, int full_rule, double ltol,
                                                double2* __restrict__ G, double2* __restrict__ H,
                                                double2* __restrict__ dG, double2* __restrict__ dH)
 {
@@ -559,7 +560,7 @@ __global__ void __launch_bounds__(256) k_fs_gl(double kr, double ki, const doubl
   {
     Elem etg;
     load_elem(E, abs(sid[t]), etg);
-    ng = __builtin_amdgcn_readfirstlane(pair_rule_fs(sqrt(kr * kr + ki * ki), etg, e, full_rule));
+    ng = __builtin_amdgcn_readfirstlane(pair_rule_fs(sqrt(kr * kr + ki * ki), etg, e, full_rule, ltol));
   }
   if (active) {
     const int np = (e.tri ? 3 : 1) * ng * ng;
@@ -1003,7 +1004,7 @@ int bq_assemble_domain_dev(const bq_adom* a, void* work, void* stream)
     double2* dH = (double2*)a->dH;
     BQ_HIP(hipMemsetAsync(dG, 0, sizeof(double2) * 2 * N * N, s));
     BQ_HIP(hipMemsetAsync(dH, 0, sizeof(double2) * 2 * N * N, s));
-    BQ_KTIME("k_fs_gl", s, k_fs_gl<<<gl_blocks, 256, 0, s>>>(a->kr, a->ki, a->E, a->ren, a->sid, dNe, flag, a->full_rule, G, H, dG, dH));
+    BQ_KTIME("k_fs_gl", s, k_fs_gl<<<gl_blocks, 256, 0, s>>>(a->kr, a->ki, a->E, a->ren, a->sid, dNe, flag, a->full_rule, a->plan.rule_ltol, G, H, dG, dH));
     BQ_HIP(hipGetLastError());
     if (ngh) BQ_KTIME("k_fs_gh", s, k_fs_gh<<<ngh, 256, 0, s>>>(a->kr, a->ki, a->E, a->ren, a->sid, dNe, list, G, H, dG, dH, F3));
     BQ_HIP(hipGetLastError());

@@ -10,7 +10,7 @@
 //
 // GPU form: instead of the reference's adaptive stop tests (which make every lane run a
 // different, data-dependent loop), each sum runs over the lattice points inside a disk
-// whose radius follows from the terms' Gaussian decay, exp(-BQ_EXP_CUT) = 4e-18 of the
+// whose radius follows from the terms' Gaussian decay, exp(-P.cut) of the
 // leading term, row by row with the exact row bounds of the disk.  Evanescent terms use the
 // overflow-free erfcx form exp(-b^2 v^2 - z^2/4v^2) erfcx(.) instead of exp(bz) erfc(.).
 #pragma once
@@ -88,7 +88,7 @@ __device__ __forceinline__ int row_centre(double ue, double e2, int lo, int hi)
 template <bool HESS, class Q>
 __device__ void qpgf_fourier_t(const bq_qplan& P, double px, double py, double pz, Q& o)
 {
-  double az = fabs(pz), bcut = BQ_EXP_CUT / az;
+  double az = fabs(pz), bcut = P.cut / az;
   double A2 = P.k2 + bcut * bcut;
   double A = sqrt(A2) * 1.0000001;
   A2 = A * A;
@@ -340,7 +340,7 @@ __device__ void qpgf_ewald_t(const bq_qplan& P, double px, double py, double pz,
 
   // ---- spatial sum: |r + R_l| <= 2 ve sqrt(ke^2 + cut)
   {
-    double Rmax = 2.0 * ve * sqrt(ke2 + BQ_EXP_CUT) * 1.0000001, R2 = Rmax * Rmax;
+    double Rmax = 2.0 * ve * sqrt(ke2 + P.cut) * 1.0000001, R2 = Rmax * Rmax;
     double b1n = sqrt(P.b1x * P.b1x + P.b1y * P.b1y);
     double c1 = -(P.b1x * px + P.b1y * py), w1 = Rmax * b1n;
     int l1lo = (int)ceil(c1 - w1), l1hi = (int)floor(c1 + w1);
@@ -402,7 +402,7 @@ __device__ void qpgf_ewald_t(const bq_qplan& P, double px, double py, double pz,
   }
   // |beta| ve <= sqrt(cut)
   {
-    double A2 = P.k2 + BQ_EXP_CUT * ive * ive;
+    double A2 = P.k2 + P.cut * ive * ive;
     double A = sqrt(A2) * 1.0000001;
     A2 = A * A;
     double d1n = sqrt(P.d1x * P.d1x + P.d1y * P.d1y);
@@ -473,10 +473,10 @@ __device__ void qpgf_v1(const bq_qplan& P, double rx, double ry, double rz, QG& 
   // points inside each truncation disk (area / cell area).
   bool fd = false;
   if (P.fd_allowed && pz != 0.0) {
-    const double bz = BQ_EXP_CUT / fabs(pz);
+    const double bz = P.cut / fabs(pz);
     const double n_fd = kPI * (P.k2 + bz * bz) / P.rcell;
-    const double n_sn = kPI * (P.k2 + BQ_EXP_CUT / (ve * ve)) / P.rcell;
-    const double rs2 = 4.0 * ve * ve * (P.k2 * ve * ve + BQ_EXP_CUT) - pz * pz;
+    const double n_sn = kPI * (P.k2 + P.cut / (ve * ve)) / P.rcell;
+    const double rs2 = 4.0 * ve * ve * (P.k2 * ve * ve + P.cut) - pz * pz;
     const double n_sp = rs2 > 0.0 ? kPI * rs2 / P.cell : 0.0;
     fd = BQ_COST_FD * n_fd < BQ_COST_SPEC * n_sn + BQ_COST_SPAT * n_sp;
   }
@@ -500,10 +500,10 @@ __device__ void qpgf_v2(const bq_qplan& P, double rx, double ry, double rz, QG2&
   const double ve = ewald_param(P, px, py, pz);
   bool fd = false;
   if (P.fd_allowed && pz != 0.0) {
-    const double bz = BQ_EXP_CUT / fabs(pz);
+    const double bz = P.cut / fabs(pz);
     const double n_fd = kPI * (P.k2 + bz * bz) / P.rcell;
-    const double n_sn = kPI * (P.k2 + BQ_EXP_CUT / (ve * ve)) / P.rcell;
-    const double rs2 = 4.0 * ve * ve * (P.k2 * ve * ve + BQ_EXP_CUT) - pz * pz;
+    const double n_sn = kPI * (P.k2 + P.cut / (ve * ve)) / P.rcell;
+    const double rs2 = 4.0 * ve * ve * (P.k2 * ve * ve + P.cut) - pz * pz;
     const double n_sp = rs2 > 0.0 ? kPI * rs2 / P.cell : 0.0;
     fd = BQ_COST_FD * n_fd < BQ_COST_SPEC * n_sn + BQ_COST_SPAT * n_sp;
   }

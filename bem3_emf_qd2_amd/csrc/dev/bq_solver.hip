@@ -28,10 +28,14 @@
 #include "../bq_assemble.h"
 #include "../bq_linalg.h"
 #include "../bq_plan.h"
+#include <cmath>
 #include <cstdlib>
 #include <cfloat>
 #include <climits>
 #include <algorithm>
+
+extern "C" int bq_qpgf_v1_dev(const bq_qplan* plan, int variant, int n, const double* d_r, double* d_out,
+                              hipStream_t s);
 
 namespace {
 
@@ -250,14 +254,110 @@ __global__ void k_live_rows(long N, int n, int ncol, const int* __restrict__ liv
 
 inline unsigned nblk(long n, int b) { return (unsigned)((n + b - 1) / b); }
 
+double asm_tolerance();
+
 // what the matrices depend on besides the mesh: QPD2 (k, Bloch vector, lattice), every
-// medium's wave number and the periodicity switch
+// medium's wave number, the periodicity switch and the assembly tolerance
 std::vector<double> wave_key(const bq_model* m)
 {
   std::vector<double> k(m->qd, m->qd + 8);
   k.insert(k.end(), m->kn, m->kn + 2 * (m->MN + 1));
   k.push_back((double)m->ps);
+  k.push_back(asm_tolerance());
   return k;
+}
+
+// Assembly tolerance: 0 = the reference's accuracy (lattice sums to exp(-40) of their leading
+// term, Gauss orders bounded at 1e-15 of the integrand); tol > 0 = matrices accurate to about
+// tol of their largest entries, for solves whose own accuracy target is far above the
+// reference's rounding (the mixed-precision path): the Gauss-order bound becomes tol and the
+// lattice-sum cut is tuned per incidence (tune_plan).  BQ_ASM_TOL sets the initial value.
+double g_asm_tol = -1.0;
+double asm_tolerance()
+{
+  if (g_asm_tol < 0.0) {
+    const char* e = getenv("BQ_ASM_TOL");
+    const double v = e ? atof(e) : 0.0;
+    g_asm_tol = (v > 0.0 && v < 1e-3) ? v : 0.0;
+  }
+  return g_asm_tol;
+}
+
+// Per-incidence choice of the lattice-sum cut for tolerance tol: the QPGF (value and gradient)
+// at kSamples offsets between random collocation nodes and random points of random source
+// elements of the open region, with the default cut and with smaller ones; the smallest cut
+// whose largest difference, relative to the largest value and gradient over the sample, stays
+// below tol / 8 is taken (the differences shrink monotonically as the cut grows).
+constexpr int kSamples = 4096;
+int tune_plan(bq_sess* S, double tol)
+{
+  S->plan.cut = BQ_EXP_CUT;
+  S->plan.rule_ltol = BQ_RULE_LTOL;
+  S->st.asm_tol = tol;
+  S->st.asm_cut = BQ_EXP_CUT;
+  if (!(tol > 0.0)) return 0;
+  S->plan.rule_ltol = std::max(log(tol), BQ_RULE_LTOL);
+  if (S->ps != 1 || S->dNe.empty() || S->dNe[0] == 0) return 0;
+  const int dNe = S->dNe[0];
+  std::vector<double> R(3 * (size_t)kSamples);
+  unsigned long long x = 0x9e3779b97f4a7c15ULL;  // fixed seed: the same samples for the same mesh
+  auto rnd = [&]() {
+    x ^= x << 13;
+    x ^= x >> 7;
+    x ^= x << 17;
+    return (double)(x >> 11) * (1.0 / 9007199254740992.0);
+  };
+  for (int i = 0; i < kSamples; ++i) {
+    const int t = 1 + (int)(rnd() * dNe) % dNe, sc = 1 + (int)(rnd() * dNe) % dNe;
+    const int at = abs(S->sid_h[0][t]), as = abs(S->sid_h[0][sc]);
+    const int tn = (int)(rnd() * 3.0);  // 0..2: a node of a triangle too
+    const double z = 2.0 * rnd() - 1.0, e = 2.0 * rnd() - 1.0;
+    const double* c = S->m.E + 24 * (size_t)as;  // cr[3][4]
+    for (int d = 0; d < 3; ++d) {
+      const double rs = c[4 * d] + c[4 * d + 1] * z + c[4 * d + 2] * e + c[4 * d + 3] * z * e;
+      R[3 * i + d] = S->m.ren[12 * (size_t)at + 3 * tn + d] - rs;
+    }
+  }
+  static const double cuts[] = {BQ_EXP_CUT, 36.0, 32.0, 29.0, 26.0, 23.0, 20.0};
+  const int nc = (int)(sizeof cuts / sizeof cuts[0]);
+  hipStream_t s = S->s;
+  double *dR = nullptr, *dO = nullptr;
+  BQ_HIP(hipMallocAsync((void**)&dR, sizeof(double) * 3 * kSamples, s));
+  BQ_HIP(hipMallocAsync((void**)&dO, sizeof(double) * 8 * kSamples * nc, s));
+  BQ_HIP(hipMemcpyAsync(dR, R.data(), sizeof(double) * 3 * kSamples, hipMemcpyHostToDevice, s));
+  for (int j = 0; j < nc; ++j) {
+    bq_qplan P = S->plan;
+    P.cut = cuts[j];
+    P.err = nullptr;
+    if (bq_qpgf_v1_dev(&P, 1, kSamples, dR, dO + (size_t)8 * kSamples * j, s)) return -1;
+  }
+  std::vector<double> O((size_t)8 * kSamples * nc);
+  BQ_HIP(hipMemcpyAsync(O.data(), dO, O.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  BQ_HIP(hipFreeAsync(dR, s));
+  BQ_HIP(hipFreeAsync(dO, s));
+  BQ_HIP(hipStreamSynchronize(s));
+  double sg = 0.0, sd = 0.0;
+  for (int i = 0; i < kSamples; ++i) {
+    const double* o = O.data() + 8 * (size_t)i;
+    sg = std::max(sg, std::hypot(o[0], o[1]));
+    for (int c = 0; c < 3; ++c) sd = std::max(sd, std::hypot(o[2 + 2 * c], o[3 + 2 * c]));
+  }
+  if (!(sg > 0.0) || !(sd > 0.0)) return 0;
+  double cut = BQ_EXP_CUT;
+  for (int j = 1; j < nc; ++j) {
+    double err = 0.0;
+    for (int i = 0; i < kSamples; ++i) {
+      const double* a = O.data() + 8 * (size_t)i;
+      const double* b = O.data() + (size_t)8 * kSamples * j + 8 * (size_t)i;
+      err = std::max(err, std::hypot(a[0] - b[0], a[1] - b[1]) / sg);
+      for (int c = 0; c < 3; ++c) err = std::max(err, std::hypot(a[2 + 2 * c] - b[2 + 2 * c], a[3 + 2 * c] - b[3 + 2 * c]) / sd);
+    }
+    if (!(err <= tol / 8.0)) break;
+    cut = cuts[j];
+  }
+  S->plan.cut = cut;
+  S->st.asm_cut = cut;
+  return 0;
 }
 
 }  // namespace
@@ -416,6 +516,7 @@ int bq_sess_assemble(bq_sess* S)
     S->work_bytes = wmax;
     S->bytes += wmax;
   }
+  if (tune_plan(S, asm_tolerance())) return -1;
   if (!S->qerr) BQ_HIP(hipMalloc((void**)&S->qerr, sizeof(int)));
   BQ_HIP(hipMemsetAsync(S->qerr, 0, sizeof(int), s));
   S->plan.err = S->qerr;
@@ -500,6 +601,13 @@ int bq_sess_assemble(bq_sess* S)
 // round trips cost more than the FP32 factors save).  BQ_MIXED=1 or 2 sets the initial mode.
 static int g_precision = -1;
 static const int kMixedMinOrder = 16384;
+
+extern "C" double bq_assembly_tolerance(double tol)
+{
+  asm_tolerance();
+  if (tol == 0.0 || (tol > 0.0 && tol < 1e-3)) g_asm_tol = tol;
+  return g_asm_tol;
+}
 
 extern "C" int bq_solve_precision(int mode)
 {

@@ -574,6 +574,16 @@ int bq_set_incidence(void *qdv, double lambda0, double theta, double phi, double
   return 0;
 }
 
+int bq_model_dims(void *qdv, int *Ne, int *NN, int *dNe)
+{
+  DQD2 *qd = (DQD2 *)qdv;
+  if (Ne) *Ne = qd->bd.Ne;
+  if (NN) *NN = qd->bd.NN;
+  if (dNe && qd->bd.sb)
+    for (int d = 0; d <= qd->MN; d++) dNe[d] = qd->bd.sb[d].Ne;
+  return qd->MN;
+}
+
 void initialize_dqd2(DQD2 *qd)
 {
   bq_release_device_state(qd);

@@ -400,6 +400,19 @@ long long bq_solver_bytes(void *qdv)
   return b;
 }
 
+/* Optional gate around the two heavy phases of a solve (0: assembly, FP64 vector work; 1:
+ * factorisation, solve and boundary fields, matrix work), set by the request loop so that
+ * concurrent solves take the phases in turn and one's assembly runs beside another's
+ * factorisation instead of beside another assembly. */
+static void (*g_gate)(int phase, int enter);
+
+void bq_set_phase_gate(void (*fn)(int phase, int enter)) { g_gate = fn; }
+
+static void gate(int phase, int enter)
+{
+  if (g_gate) g_gate(phase, enter);
+}
+
 /* The reference's solve_bieq pipeline; progress lines when verbose.  Returns 0, or a negative
  * code (the reference's QPGF codes -12 / -23, -1 device failure, -3 singular system) with the
  * reason in bq_last_error(). */
@@ -420,7 +433,12 @@ static int solve_core(DQD2 *qd, int verbose, const char **stage)
   if (!S) return -1;
   int reused = bq_sess_assembled(S), rc = 0;
   *stage = "coefficient matrix assembly";
-  if (!reused && (rc = bq_sess_assemble(S)) != 0) return rc < 0 ? rc : -1;
+  if (!reused) {
+    gate(0, 1);
+    rc = bq_sess_assemble(S);
+    gate(0, 0);
+    if (rc) return rc < 0 ? rc : -1;
+  }
   if (verbose) {
     printf("finished. Elapsed time : %5g (sec)\n", now_s() - ts);
     printf("  solve mEMP boundary value   ");
@@ -430,8 +448,10 @@ static int solve_core(DQD2 *qd, int verbose, const char **stage)
   size_t na = 8 * (size_t)qd->bd.NN;
   double complex *X = calloc(na, sizeof(double complex));
   *stage = "global system solve";
+  gate(1, 1);
   rc = bq_sess_solve_global(S, (double *)X);
   if (rc) {
+    gate(1, 0);
     free(X);
     return rc < 0 ? rc : -1;
   }
@@ -445,7 +465,9 @@ static int solve_core(DQD2 *qd, int verbose, const char **stage)
   }
   ts = now_s();
   *stage = "boundary field";
-  if (boundary_EH(qd, S) || boundary_dEH(qd, S)) return -1;
+  rc = (boundary_EH(qd, S) || boundary_dEH(qd, S)) ? -1 : 0;
+  gate(1, 0);
+  if (rc) return rc;
   double tf = now_s() - ts;
   if (verbose) printf("finished. Elapsed time : %5g (sec)\n", tf);
   bq_stats st = *bq_sess_stats(S);

@@ -23,7 +23,8 @@ class Stats(ctypes.Structure):
                 ("n_qpgf_gl", ctypes.c_longlong), ("n_qpgf_gh", ctypes.c_longlong), ("n_qpgf_sing", ctypes.c_longlong),
                 ("mixed", ctypes.c_int), ("ir_iterations", ctypes.c_int), ("ir_correction", ctypes.c_double),
                 ("path", ctypes.c_int), ("reused", ctypes.c_int), ("backward_error", ctypes.c_double),
-                ("t_factor_ms", ctypes.c_double), ("gmres_iterations", ctypes.c_int), ("qpgf_error", ctypes.c_int)]
+                ("t_factor_ms", ctypes.c_double), ("gmres_iterations", ctypes.c_int), ("qpgf_error", ctypes.c_int),
+                ("asm_tol", ctypes.c_double), ("asm_cut", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -73,10 +73,17 @@ WORKLOADS = {
     # source-target offsets, not on the element count
     # the 6144-quad sweep: one resident model, 16 seeded incidence points, FP64 and mixed
     "c4": dict(kind="sweep", n=32, radius=0.45, n_sphere=1.5 + 0.001j, n0=1.0, d1=(1.0, 0.0), d2=(0.0, 1.0),
-               points=16, seed=4, probes=100, steps=3, warmup=1,
+               points=16, seed=4, probes=100, steps=6, warmup=1,
                label="C4: 6x32x32 = 6144-quad cube-sphere r=0.45um n=1.5+0.001i in n0=1.0, square lattice "
                      "1.0um; sweep of 16 points from default_rng(4): lambda0 U[0.4,0.8]um, theta U[0,40]deg, "
                      "phi U[0,90]deg, TE/TM; 100 probes (EH_t type 1)"),
+    # serving: Poisson request stream over four cube-sphere sizes (the C4 object and lattice)
+    "c5": dict(kind="serve", sizes=(8, 16, 24, 32), radius=0.45, n_sphere=1.5 + 0.001j, n0=1.0, d1=(1.0, 0.0),
+               d2=(0.0, 1.0), stream=2000, rate=0.5, seed=5, probes=100, steps=40, warmup=4,
+               label="C5: serving 2000 Poisson requests (seed 5, 0.5 requests/s) over cube-spheres of 384/1536/3456/"
+                     "6144 quads (p 0.4/0.3/0.2/0.1; r=0.45um n=1.5+0.001i, square lattice 1.0um), lambda0 "
+                     "U[0.4,0.8]um, theta U[0,40]deg, phi U[0,90]deg, TE/TM, 30% polarisation repeats; mixed "
+                     "precision, 100 probes (EH_t type 1)"),
     # the same sweep at 6x8x8 quads (per-evaluation counters, quick checks)
     "c4s": dict(kind="sweep", n=8, radius=0.45, n_sphere=1.5 + 0.001j, n0=1.0, d1=(1.0, 0.0), d2=(0.0, 1.0),
                 points=16, seed=4, probes=100, steps=6, warmup=1,
@@ -262,6 +269,7 @@ def run_sweep(args, w, bq, L):
     mixed result is compared with the FP64 result of its point."""
     from bem3_emf_qd2_amd.solver import Model
     sys.path.insert(0, str(ROOT / "tests"))
+    os.environ.setdefault("BQ_SERVE_LOG", "1")  # the request loop's per-request progress lines
     pts = sweep_points(w)
     if args.warmup + args.steps > len(pts):
         raise SystemExit(f"bench.py: {w['points']} sweep points; --warmup + --steps must not exceed them")
@@ -280,7 +288,14 @@ def run_sweep(args, w, bq, L):
             return st, np.r_[E.ravel(), H.ravel()]
 
         kt = {}
-        modes = ["fp64", "mixed"] if args.precision == "mixed" else [args.precision]
+        if args.passes:
+            modes = [m_ for m_ in args.passes.split(",") if m_]
+        elif args.precision == "mixed":
+            # the plain FP64 path, then the batched mixed-precision path (below); the serial
+            # mixed pass (surface-solution errors) runs when asked for with --passes fp64,mixed
+            modes = ["fp64"] if args.slots > 0 else ["fp64", "mixed"]
+        else:
+            modes = [args.precision]
         with _QuietStdout():
             for mode in modes:
                 # each pass starts from a fresh session, so that no timed point reuses the
@@ -288,6 +303,7 @@ def run_sweep(args, w, bq, L):
                 # the untimed warm-up)
                 m.release_device()
                 bq.solve_precision(mode)
+                bq.assembly_tolerance(args.asm_tol if mode == "mixed" else 0.0)
                 for i, p in enumerate(warm):
                     step(p)
                     print(f"bench.py: {mode} warm-up point {i + 1}/{len(warm)} done", file=sys.stderr, flush=True)
@@ -307,12 +323,42 @@ def run_sweep(args, w, bq, L):
                     kt = _kernel_times(L)
                     L.bq_kernel_timing(0)
             bq.solve_precision("fp64")
+            bq.assembly_tolerance(0.0)
+        batched = None
+        if args.slots > 0 and args.precision == "mixed" and (not args.passes or "batched" in args.passes):
+            # the batched path: the request loop with args.slots worker slots, every timed point
+            # queued at once; one warm-up run first so that each slot holds its session
+            from bem3_emf_qd2_amd.serve import Engine
+            m.release_device()
+            bq.assembly_tolerance(args.asm_tol)
+            eng = Engine([Path(td) / "setup.dat"], nslots=args.slots, mem_limit_gb=args.mem_gb, precision="mixed")
+            as_req = lambda ps: [dict(mesh=0, lambda0=p[0], theta=p[1], phi=p[2], pol=p[3], arrival_s=0.0) for p in ps]
+            wpts = (warm + pts[args.warmup + args.steps:])[:args.slots] or warm
+            _, _, _, rcw = eng.run(as_req(wpts), probes)
+            print(f"bench.py: batched warm-up ({len(wpts)} points) done", file=sys.stderr, flush=True)
+            L.bq_kernel_timing(1)
+            t = time.perf_counter()
+            Eb, Hb, outs, rcb = eng.run(as_req(timed), probes)
+            wall = time.perf_counter() - t
+            kt = _kernel_times(L)
+            L.bq_kernel_timing(0)
+            eng.close()
+            bq.assembly_tolerance(0.0)
+            if rcw or rcb or any(o["status"] for o in outs):
+                raise SystemExit(f"bench.py: batched run failed ({rcw}, {rcb}, {[o['status'] for o in outs]})")
+            makespan = max(o["t_done_s"] for o in outs)
+            batched = dict(wall=wall, makespan=makespan, outs=outs,
+                           probes=[np.r_[Eb[i].ravel(), Hb[i].ravel()] for i in range(len(timed))])
+            for i, o in enumerate(outs):
+                print(f"bench.py: batched point {i + 1}/{len(timed)} slot {o['slot']}: start {o['t_start_s']:.2f} s, "
+                      f"done {o['t_done_s']:.2f} s", file=sys.stderr, flush=True)
         sys.path.insert(0, str(ROOT / "tests"))
         from datfile import read_dat
         dd = read_dat(Path(td) / "setup.dat")
         tris = int(np.sum(dd["ed"][1:, 3] == 0))
         quads, NN = int(dd["Ne"]) - tris, int(dd["NN"])
-        main_mode = modes[-1]
+        serial = modes[-1]                     # the last serial pass
+        head = "batched" if batched is not None else serial
         rows = []
         for i, p in enumerate(timed):
             r = {"lambda0": p[0], "theta_deg": float(np.degrees(p[1])), "phi_deg": float(np.degrees(p[2])),
@@ -323,8 +369,10 @@ def run_sweep(args, w, bq, L):
                 r[f"assemble_ms_{mode}"] = q["st"]["t_assemble_ms"]
                 r[f"solve_ms_{mode}"] = q["st"]["t_solve_ms"]
                 r[f"backward_error_{mode}"] = q["st"]["backward_error"]
-            if "fp64" in res and main_mode != "fp64":
-                a, b = res[main_mode][i], res["fp64"][i]
+                r[f"n_qpgf_gl_{mode}"] = q["st"]["n_qpgf_gl"]
+                r[f"asm_cut_{mode}"] = q["st"]["asm_cut"]
+            if res["fp64"] and res["mixed"]:
+                a, b = res["mixed"][i], res["fp64"][i]
                 n3 = len(b["probes"]) // 2
                 r["surface_rel_err_vs_fp64"] = _surface_err(a["surf"], b["surf"])
                 r["probe_E_rel_err_vs_fp64"] = float(np.abs(a["probes"][:n3] - b["probes"][:n3]).max() /
@@ -332,9 +380,23 @@ def run_sweep(args, w, bq, L):
                 r["probe_H_rel_err_vs_fp64"] = float(np.abs(a["probes"][n3:] - b["probes"][n3:]).max() /
                                                      np.abs(b["probes"][n3:]).max())
                 r["gmres_iterations"] = a["st"]["gmres_iterations"]
+            if batched is not None:
+                o = batched["outs"][i]
+                r["batched_slot"] = o["slot"]
+                r["batched_start_ms"] = o["t_start_s"] * 1e3
+                r["batched_done_ms"] = o["t_done_s"] * 1e3
+                r["batched_gmres_iterations"] = o["gmres_iterations"]
+                r["batched_backward_error"] = o["backward_error"]
+                if res["fp64"]:
+                    pb, pf = batched["probes"][i], res["fp64"][i]["probes"]
+                    n3 = len(pf) // 2
+                    r["batched_probe_E_rel_err_vs_fp64"] = float(np.abs(pb[:n3] - pf[:n3]).max() / np.abs(pf[:n3]).max())
+                    r["batched_probe_H_rel_err_vs_fp64"] = float(np.abs(pb[n3:] - pf[n3:]).max() / np.abs(pf[n3:]).max())
             rows.append(r)
-        st_last = res[main_mode][-1]["st"]
-        roof = _roofline(kt, st_last, args.steps, args.workload)
+        # QPGF evaluations of the timed points' assemblies (identical in every pass)
+        st_ref = res[serial][-1]["st"]
+        evals = {k: float(np.mean([q["st"][k] for q in res[serial]])) for k in ("n_qpgf_gl", "n_qpgf_gh", "n_qpgf_sing")}
+        roof = _roofline(kt, evals, args.steps, args.workload) if kt else None
         cpu = None
         if not args.no_cpu_baseline:
             _set_point(m, timed[0])
@@ -342,8 +404,15 @@ def run_sweep(args, w, bq, L):
             cpu = _cpu_baseline_pairs(Path(td) / "point.dat")
         nbytes = m.device_bytes()
         m.close()
-    lat = np.array([q["lat"] for q in res[main_mode]]) * 1e3
-    total = float(lat.sum() / 1e3)
+    if batched is not None:
+        total = batched["makespan"]
+        lat = np.array([o["t_done_s"] for o in batched["outs"]]) * 1e3
+        service = np.array([o["t_done_s"] - o["t_start_s"] for o in batched["outs"]]) * 1e3
+    else:
+        lat = np.array([q["lat"] for q in res[serial]]) * 1e3
+        total = float(lat.sum() / 1e3)
+    errkeys = [k for k in ("surface_rel_err_vs_fp64", "probe_E_rel_err_vs_fp64", "probe_H_rel_err_vs_fp64",
+                           "batched_probe_E_rel_err_vs_fp64", "batched_probe_H_rel_err_vs_fp64") if k in rows[0]]
     out = {
         "metric": "solved incidences/sec + p50/p99 latency at <=1e-8 rel err vs FP64 plain path",
         "value": args.steps / total,
@@ -356,22 +425,31 @@ def run_sweep(args, w, bq, L):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "c64 factors + f64 (assembly, GMRES residual)" if main_mode == "mixed" else "f64",
+        "dtype": "c64 factors + f64 (assembly, GMRES residual)" if head != "fp64" else "f64",
         "data": "synthetic (generated cube-sphere mesh and seeded plane-wave sweep)",
         "config": {"workload": w["label"], "quads": quads, "triangles": tris, "element_nodes": NN,
                    "na": 8 * NN, "probes": w["probes"], "sweep_points_timed": args.steps,
-                   "parallelism": "single GPU", "solve_precision": main_mode},
-        "max_rel_err_vs_fp64": {k: max(r.get(k, 0.0) for r in rows) for k in
-                                ("surface_rel_err_vs_fp64", "probe_E_rel_err_vs_fp64",
-                                 "probe_H_rel_err_vs_fp64")} if main_mode != "fp64" else None,
+                   "parallelism": "single GPU",
+                   "solve_precision": "mixed" if head != "fp64" else "fp64",
+                   "assembly_tolerance": args.asm_tol if head != "fp64" else 0.0,
+                   "path": (f"batched: request loop, {args.slots} slots, phase-gated (one assembly and one "
+                            f"factorisation at a time), all points queued at t=0; latency = time to completion"
+                            if batched is not None else f"serial {serial}")},
+        "max_rel_err_vs_fp64": {k: max(r[k] for r in rows) for k in errkeys} or None,
         "points": rows,
-        "phase_ms_last_point": st_last,
+        "phase_ms_last_fp64_point": st_ref,
         "device_bytes": nbytes,
         "kernels_ms_per_step": {k: v[0] / args.steps for k, v in sorted(kt.items(), key=lambda x: -x[1][0])},
         "roofline": roof,
         "cpu_baseline": cpu,
     }
-    if "fp64" in res and main_mode != "fp64":
+    if batched is not None:
+        out["batched_service_ms"] = {"p50": float(np.percentile(service, 50)), "p99": float(np.percentile(service, 99))}
+        out["batched_wall_s"] = batched["wall"]
+    if res["mixed"]:
+        ml = np.array([q["lat"] for q in res["mixed"]])
+        out["serial_mixed_value"] = args.steps / float(ml.sum())
+    if res["fp64"] and head != "fp64":
         fl = np.array([q["lat"] for q in res["fp64"]])
         out["fp64_value"] = args.steps / float(fl.sum())
         out["fp64_latency_ms"] = {"p50": float(np.percentile(fl * 1e3, 50)),
@@ -440,6 +518,90 @@ def _cpu_baseline_pairs(model_dat, budget_s=8.0):
                       "; pairs scaled by count; probes not included"}
 
 
+def run_serve(args, w, bq, L):
+    """Serving workload: the first args.steps requests of the seeded stream replayed in real
+    time (their Poisson arrival times) through the request loop; throughput over the span from
+    the first arrival to the last completion, latency = completion - arrival.  A sample (the
+    first request of each mesh) is solved again alone on the plain FP64 path for the error."""
+    from bem3_emf_qd2_amd import meshgen
+    from bem3_emf_qd2_amd.serve import Engine, serving_requests
+    from bem3_emf_qd2_amd.solver import Model
+    os.environ.setdefault("BQ_SERVE_LOG", "1")
+    reqs = serving_requests(w["stream"], w["rate"], seed=w["seed"])[:args.steps]
+    probes = _probes(w, w["probes"], w["seed"])
+    with tempfile.TemporaryDirectory() as td:
+        files = []
+        for n in w["sizes"]:
+            d = Path(td) / f"n{n}"
+            meshgen.sphere_scene(d, n, w["radius"], w["n_sphere"], 0.55, n0=w["n0"], theta=0.3, phi=0.4,
+                                 d1=w["d1"], d2=w["d2"])
+            m = Model.from_dir(d)
+            m.write(d / "setup.dat")
+            m.close()
+            files.append(d / "setup.dat")
+        bq.assembly_tolerance(args.asm_tol)
+        eng = Engine(files, nslots=args.slots or 2, mem_limit_gb=args.mem_gb, precision="mixed")
+        warm = [dict(mesh=i, lambda0=0.5, theta=0.2, phi=0.3, pol="TM", arrival_s=0.0) for i in range(len(files))]
+        _, _, _, rcw = eng.run(warm[:max(args.warmup, 0)], probes)
+        print(f"bench.py: serving warm-up done ({rcw})", file=sys.stderr, flush=True)
+        t = time.perf_counter()
+        E, H, outs, rc = eng.run(reqs, probes)
+        wall = time.perf_counter() - t
+        eng.close()
+        bq.assembly_tolerance(0.0)
+        bad = [i for i, o in enumerate(outs) if o["status"]]
+        if rc or bad:
+            raise SystemExit(f"bench.py: serving run failed ({rc}); requests {bad[:10]}")
+        # accuracy sample: the first request of each mesh alone on the plain FP64 path
+        errs = []
+        ref = Engine(files, nslots=1, mem_limit_gb=args.mem_gb, precision="fp64")
+        for mi in range(len(files)):
+            idx = next((i for i, q in enumerate(reqs) if q["mesh"] == mi), None)
+            if idx is None:
+                continue
+            E1, H1, o1, rc1 = ref.run([dict(reqs[idx], arrival_s=0.0)], probes)
+            a, b = np.r_[E[idx].ravel(), H[idx].ravel()], np.r_[E1[0].ravel(), H1[0].ravel()]
+            errs.append(dict(request=idx, mesh=mi, probe_rel_err_vs_fp64=float(np.abs(a - b).max() / np.abs(b).max())))
+        ref.close()
+    t_first = min(q["arrival_s"] for q in reqs)
+    span = max(o["t_done_s"] for o in outs) - t_first
+    lat = np.array([o["t_done_s"] - o["t_arrival_s"] for o in outs]) * 1e3
+    service = np.array([o["t_done_s"] - o["t_start_s"] for o in outs]) * 1e3
+    quads = [6 * n * n for n in w["sizes"]]
+    per_mesh = {}
+    for q, o in zip(reqs, outs):
+        per_mesh.setdefault(quads[q["mesh"]], []).append((o["t_done_s"] - o["t_start_s"]) * 1e3)
+    out = {
+        "metric": "solved incidences/sec + p50/p99 latency at <=1e-8 rel err vs FP64 plain path",
+        "value": len(reqs) / span,
+        "unit": "incidences/s",
+        "n_gpus": 1,
+        "steps": len(reqs),
+        "warmup": args.warmup,
+        "ms_per_step": span / len(reqs) * 1e3,
+        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+        "service_ms": {"p50": float(np.percentile(service, 50)), "p99": float(np.percentile(service, 99))},
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64 factors + f64 (assembly, GMRES residual)",
+        "data": "synthetic (generated cube-sphere meshes and seeded request stream)",
+        "config": {"workload": w["label"], "requests_served": len(reqs),
+                   "requests_note": f"the first {len(reqs)} of the {w['stream']}-request stream, arrivals in real time",
+                   "slots": args.slots or 2, "mem_limit_gb": args.mem_gb, "parallelism": "single GPU",
+                   "solve_precision": "mixed", "assembly_tolerance": args.asm_tol},
+        "reused_factors": int(sum(o["reused"] for o in outs)),
+        "repeats_in_sample": int(sum("repeat_of" in q for q in reqs)),
+        "service_ms_by_quads": {str(k): float(np.mean(v)) for k, v in sorted(per_mesh.items())},
+        "max_backward_error": float(max(o["backward_error"] for o in outs)),
+        "accuracy_sample": errs,
+        "wall_s": wall,
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -447,6 +609,18 @@ def main():
     ap.add_argument("--warmup", type=int, default=None, help="untimed steps (default per workload: c2 1, c1 2)")
     ap.add_argument("--workload", default="c4", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--slots", type=int, default=2,
+                    help="sweep workloads: worker slots of the batched mixed-precision pass (request loop); "
+                         "0 skips it and reports the serial mixed pass")
+    ap.add_argument("--mem-gb", type=float, default=270.0, help="device memory limit of the request loop (GiB)")
+    ap.add_argument("--asm-tol", type=float, default=1e-12,
+                    help="assembly tolerance of the mixed-precision passes (bq_assembly_tolerance: Gauss-order "
+                         "bound and per-incidence lattice-sum cut); 0 keeps the reference's accuracy.  The FP64 "
+                         "pass always runs at the reference's accuracy")
+    ap.add_argument("--passes", default=None,
+                    help="sweep workloads: comma-separated solve precisions to run in order (default "
+                         "'fp64,mixed': the FP64 pass gives fp64_value and the reference results); "
+                         "'mixed' alone for profiling runs")
     ap.add_argument("--precision", default=None, choices=["fp64", "mixed", "auto"],
                     help="global solve: the plain complex FP64 LU ('fp64', default: the C1 and C2 "
                          "configurations are the plain FP64 path), complex FP32 LU + FP64 iterative "
@@ -483,6 +657,9 @@ def main():
         args.precision = "mixed" if w["kind"] == "sweep" else "fp64"
     if w["kind"] == "sweep":
         run_sweep(args, w, bq, L)
+        return
+    if w["kind"] == "serve":
+        run_serve(args, w, bq, L)
         return
     probes = _probes(w, w["probes"], w["seed"])
     with tempfile.TemporaryDirectory() as td:

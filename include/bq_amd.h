@@ -50,6 +50,9 @@ int bq_qpgf_batch_noalloc(const double *qd8, int variant, int n, const double *d
 
 /* sizeof(DQD2) as compiled into the library (for callers that allocate it opaquely). */
 int bq_sizeof_dqd2(void);
+/* Sizes of a set-up model for callers that hold DQD2 opaquely: elements, element nodes and,
+ * if dNe is not NULL, each domain's element count (dNe[MN + 1]); returns MN. */
+int bq_model_dims(void *qd, int *Ne, int *NN, int *dNe);
 
 /* Total fields at n points of a solved model in one GPU launch (batched EH_t of
  * d3qd2_src/d3qd2_field.c:271).  qd: a DQD2* after solve_bieq or dat_read; r[n][3];
@@ -120,6 +123,40 @@ long long bq_solver_bytes(void *qd);
  * has no session. */
 int bq_solve_stats(void *qd, void *out);
 
+/* ---- request loop (no counterpart in the reference; its d3qd2_bv_solver solves one wave per
+ * run, d3qd2_src/d3qd2_bv_solver.c:3-21) ------------------------------------------------
+ * An engine keeps models resident on the GPU and serves incidence requests with continuous
+ * batching: nslots worker slots (one host thread each, each model session on its own stream)
+ * take ready requests as they arrive, so that work of different requests overlaps on the
+ * device.  A slot prefers a request whose mesh and wave are resident in it (matrices and
+ * factors reused: a polarisation change); a request is admitted when the device memory of its
+ * session fits mem_limit_gb next to the sessions other slots hold (idle ones are released,
+ * least recently used first).  Meshes are given as set-up files (dat_write of a model after
+ * initialize_dqd2).  precision: the bq_solve_precision mode of every solve. */
+typedef struct bq_serve_req {
+  int mesh;                      /* index of the set-up file */
+  double lambda0, theta, phi;    /* wave (um, rad) */
+  int pol;                       /* 1: TM (e0x = 1), 0: TE (e0y = 1); amplitude 1 */
+  double arrival_s;              /* seconds after the start of bq_serve_run */
+} bq_serve_req;
+
+typedef struct bq_serve_out {
+  int status;                    /* 0, or the request's negative code */
+  int slot;                      /* worker slot that served it */
+  int reused;                    /* matrices and factors reused from the slot's previous request */
+  int gmres_iterations;
+  double t_arrival_s, t_start_s, t_done_s; /* seconds after the start of the run */
+  double backward_error;
+} bq_serve_out;
+
+void *bq_serve_create(int nmesh, const char *const *setup_dat, int nslots, double mem_limit_gb, int precision);
+/* Serves nreq requests (arrivals released at req[i].arrival_s); E, H [nreq][nprobe][3] complex
+ * receive EH_t (type 1) at the probes.  Returns 0, -5 if a request failed (its out.status), -4
+ * if a mesh cannot fit the memory limit alone, -2 for bad arguments. */
+int bq_serve_run(void *engine, int nreq, const bq_serve_req *req, int nprobe, const double *probes, double *E,
+                 double *H, bq_serve_out *out);
+void bq_serve_destroy(void *engine);
+
 /* Precision of the global (open-region) linear solve (no counterpart in the reference,
  * whose solver is PARDISO / LAPACKE_zgesv in FP64, d3qd2_src/d3qd2_mkl_solver.c:18-148):
  * mode 0 = complex FP64 LU (default); mode 1 = complex FP32 LU of the same blocks used as
@@ -131,6 +168,18 @@ int bq_solve_stats(void *qd, void *out);
  * the mode in force.  The environment variable BQ_MIXED=1 or 2 sets the initial mode.
  * Process-wide; set it between solves. */
 int bq_solve_precision(int mode);
+
+/* Accuracy of the coefficient-matrix assembly (no counterpart in the reference, whose
+ * lattice sums and Gauss rules aim at full double precision): tol = 0 (default) keeps the
+ * reference's accuracy (lattice sums to exp(-40) of their leading term, distance-chosen
+ * Gauss orders bounded at 1e-15 of the integrand); 0 < tol < 1e-3 assembles each incidence's
+ * matrices to about tol of their largest entries: the Gauss-order bound becomes tol and the
+ * lattice-sum cut is tuned per incidence by evaluating the QPGF at 4096 sampled element
+ * offsets with shorter sums against the full one (the smallest cut whose error stays below
+ * tol / 8).  For the mixed-precision path, whose result is held to 1e-8 of the FP64 path;
+ * tol < 0 only queries.  BQ_ASM_TOL sets the initial value.  Process-wide; set between
+ * solves.  Returns the value in force. */
+double bq_assembly_tolerance(double tol);
 
 /* Per-kernel timing (no counterpart in the reference; for benchmarks and profiling):
  * bq_kernel_timing(1) turns on HIP-event timing of every kernel launch the library makes

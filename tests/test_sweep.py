@@ -172,3 +172,28 @@ def test_gpu_set_incidence_rejects_invalid(tmp_path, gpu_lib):
     m.set_incidence(0.6, 0.2, 0.1)
     m.solve()
     m.close()
+
+
+@pytest.mark.gpu
+def test_gpu_assembly_tolerance_keeps_the_metric_bound(tmp_path, gpu_lib):
+    """The tuned assembly (bq_assembly_tolerance(1e-11): Gauss orders bounded at 1e-11, the
+    lattice-sum cut chosen per incidence from sampled element offsets) with the mixed-precision
+    solve, against the plain FP64 path at the reference's accuracy: surface solution and probe
+    fields within 1e-8 (the metric's bound), the cut below the default 40, and tolerance 0
+    restoring the default cut."""
+    d = _scene(tmp_path / "s", n=4)
+    st64, ref, f64 = _solve(d, tmp_path, "fp64", gpu_lib=gpu_lib)
+    gpu_lib.assembly_tolerance(1e-11)
+    try:
+        stt, tun, ft = _solve(d, tmp_path, "tuned", precision="mixed", gpu_lib=gpu_lib)
+    finally:
+        gpu_lib.assembly_tolerance(0.0)
+    e = _rel(tun, ref)
+    ef = float(np.abs(ft - f64).max() / np.abs(f64).max())
+    print(f"tuned cut {stt.asm_cut} (tol {stt.asm_tol}), QPGF evaluations {stt.n_qpgf} vs {st64.n_qpgf}; "
+          f"surface {e:.2e}, probes {ef:.2e}")
+    assert st64.asm_cut == 40.0 and st64.asm_tol == 0.0
+    assert stt.asm_tol == 1e-11 and stt.asm_cut < 40.0
+    assert stt.n_qpgf < st64.n_qpgf
+    assert e < 1e-8 and ef < 1e-8
+    assert gpu_lib.assembly_tolerance() == 0.0
