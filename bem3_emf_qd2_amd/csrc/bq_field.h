@@ -33,6 +33,9 @@ typedef struct bq_fdata {
   const double *const *Eb, *const *dEb, *const *Hb, *const *dHb, *const *Ub, *const *dUb;
   const double *xli, *wli, *xhi, *whi, *z49, *e49, *w49;
   const double *w34, *z37, *e37, *w37;   /* triangle node-rule weights and 7-point rule */
+  int Nn;                                /* mesh nodes, for the domain test */
+  const double *rn;                      /* [(Nn+1)][3] node coordinates */
+  const int *ed, *md, *sd;               /* [(Ne+1)][4] element nodes (4th 0 for a triangle), main and sub-domain */
 } bq_fdata;
 
 int bq_field_ctx_create(const bq_fdata *f, void **handle);
@@ -44,6 +47,9 @@ int bq_field_ctx_create(const bq_fdata *f, void **handle);
 int bq_field_eval(void *handle, int kind, int n, const double *pts, const int *did, const int *bd_t,
                   const double *bd_ze, int type, double *out);
 void bq_field_ctx_destroy(void *handle);
+/* domain id of n points (the reference's domain_id on folded points, d3qd2_src/d3qd2_setup.c:
+ * 157-245: signed solid angles of the elements of each domain; -1 outside every domain) */
+int bq_field_domain_ids(void *handle, int n, const double *pts, int *did);
 
 int bq_field_eval_dev(const bq_fctx *C, int kind, int npts, const double *d_pts, const int *d_did, const int *d_bd_t,
                       const double *d_bd_ze, int type, double *d_out, void *stream);

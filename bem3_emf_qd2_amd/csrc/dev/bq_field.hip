@@ -387,7 +387,146 @@ namespace {
 struct FieldCtx {
   bq_fctx C;
   std::vector<void*> owned;
+  int Ne = 0, MN = 0;
+  const double* rn = nullptr;
+  const int *ed = nullptr, *md = nullptr, *sd = nullptr;
 };
+
+// ---- domain of a point: sum of signed solid angles (the reference's domain_id) -----------
+// Solid angle of the triangle (r0, r1, r2) seen from the origin, by the spherical excess of
+// the three dihedral angles between the planes through the origin and the edges (the
+// reference's fid_calc_solid_angle, d3qd2_src/d3qd2_setup.c:1351-1422), with its sign from
+// the side the origin is on; flg = -1 when the origin is on the triangle's plane.
+__device__ double tri_omega_dev(const double r0[3], const double r1[3], const double r2[3], int& flg, double n20o[3])
+{
+#pragma clang fp contract(off)
+  auto crs = [](double* o, const double* a, const double* b) {
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+  };
+  auto dt = [](const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; };
+  auto ab = [&](const double* a) { return sqrt(dt(a, a)); };
+  double n01[3], n12[3], n20[3], nt[3];
+  crs(n01, r0, r1);
+  crs(n12, r1, r2);
+  crs(n20, r2, r0);
+  crs(nt, n01, n20);
+  const double a0 = atan2(ab(nt), -dt(n01, n20));
+  crs(nt, n12, n01);
+  const double a1 = atan2(ab(nt), -dt(n12, n01));
+  crs(nt, n20, n12);
+  const double a2 = atan2(ab(nt), -dt(n20, n12));
+  const double D = dt(r0, n12);
+  for (int i = 0; i < 3; ++i) n20o[i] = n20[i];
+  if (D > 0.0) return a0 + a1 + a2 - kPI;
+  if (D < 0.0) return -(a0 + a1 + a2 - kPI);
+  flg = -1;
+  return 0.0;
+}
+
+// element solid angle: a triangle, or a quad as the triangles (0, 1, 2) and (0, 2, 3) with the
+// shared edge's plane reused, as the reference splits it
+__device__ double elem_omega_dev(bool quad, double r[4][3], int& flg)
+{
+#pragma clang fp contract(off)
+  flg = 0;
+  for (int i = 0; i < (quad ? 4 : 3); ++i) {
+    const double a = sqrt(r[i][0] * r[i][0] + r[i][1] * r[i][1] + r[i][2] * r[i][2]);
+    if (a == 0.0) {
+      flg = -1;
+      return 0.0;
+    }
+    r[i][0] /= a;
+    r[i][1] /= a;
+    r[i][2] /= a;
+  }
+  double n20[3];
+  double om = tri_omega_dev(r[0], r[1], r[2], flg, n20);
+  if (flg < 0 || !quad) return om;
+  auto crs = [](double* o, const double* a, const double* b) {
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+  };
+  auto dt = [](const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; };
+  auto ab = [&](const double* a) { return sqrt(dt(a, a)); };
+  double n01[3] = {-n20[0], -n20[1], -n20[2]}, n12[3], nb[3], nt[3];
+  crs(n12, r[2], r[3]);
+  crs(nb, r[3], r[0]);
+  crs(nt, n01, nb);
+  const double a0 = atan2(ab(nt), -dt(n01, nb));
+  crs(nt, n12, n01);
+  const double a1 = atan2(ab(nt), -dt(n12, n01));
+  crs(nt, nb, n12);
+  const double a2 = atan2(ab(nt), -dt(nb, n12));
+  const double D = dt(r[0], n12);
+  if (D > 0.0) om += a0 + a1 + a2 - kPI;
+  else if (D < 0.0) om -= a0 + a1 + a2 - kPI;
+  else flg = -1;
+  return flg < 0 ? 0.0 : om;
+}
+
+// One workgroup per point.  The reference walks the domains in order, adding each element's
+// solid angle to its main domain and subtracting it from its sub-domain, and decides a domain
+// when its sum is complete: the open region when |sum| < 2 pi, another when sum > 2 pi, and a
+// domain at once when the point is on the plane of one of its elements.  So domain d's sum holds
+// its own elements and those of lower-numbered domains that have d as sub-domain.
+__global__ void __launch_bounds__(256) k_domain_id(int n, const double* __restrict__ pts, int Ne, int MN,
+                                                   const double* __restrict__ rn, const int* __restrict__ ed,
+                                                   const int* __restrict__ md, const int* __restrict__ sd,
+                                                   int* __restrict__ did)
+{
+  __shared__ double red[4][BQ_MAXD];
+  __shared__ int rfl[4];
+  const int p = blockIdx.x;
+  if (p >= n) return;
+  const double rt[3] = {pts[3 * p], pts[3 * p + 1], pts[3 * p + 2]};
+  double acc[BQ_MAXD];
+  for (int d = 0; d < BQ_MAXD; ++d) acc[d] = 0.0;
+  int fl = BQ_MAXD;  // lowest domain with an element whose plane holds the point
+  for (int s = 1 + threadIdx.x; s <= Ne; s += 256) {
+    const int m = md[s], q = sd[s];
+    const bool quad = ed[4 * s + 3] != 0;
+    double rv[4][3];
+    for (int j = 0; j < 4; ++j)
+      for (int k = 0; k < 3; ++k) rv[j][k] = rn[3 * (size_t)ed[4 * s + j] + k] - rt[k];
+    int flg = 0;
+    const double om = elem_omega_dev(quad, rv, flg);
+    if (flg < 0) {
+      fl = min(fl, m);
+      continue;
+    }
+    for (int d = 0; d <= MN; ++d) {
+      if (d == m) acc[d] += om;
+      if (d == q && m < q) acc[d] -= om;
+    }
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int d = 0; d <= MN; ++d) {
+    double v = acc[d];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if (lane == 0) red[wv][d] = v;
+  }
+  for (int o = 32; o > 0; o >>= 1) fl = min(fl, __shfl_xor(fl, o));
+  if (lane == 0) rfl[wv] = fl;
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  const int f = min(min(rfl[0], rfl[1]), min(rfl[2], rfl[3]));
+  int res = -1;
+  for (int d = 0; d <= MN; ++d) {
+    if (f == d) {
+      res = d;
+      break;
+    }
+    const double og = red[0][d] + red[1][d] + red[2][d] + red[3][d];
+    if (d == 0 ? fabs(og) < 2.0 * kPI : og > 2.0 * kPI) {
+      res = d;
+      break;
+    }
+  }
+  did[p] = res;
+}
 
 template <class T>
 int up(FieldCtx* F, const T* h, size_t n, const T** out, hipStream_t s)
@@ -434,6 +573,13 @@ extern "C" int bq_field_ctx_create(const bq_fdata* f, void** handle)
   size_t n1 = (size_t)f->Ne + 1;
   for (size_t i = 1; i < n1; ++i)
     if (f->E[24 * i + 21] != 0.0) C.has_tri = 1;
+  F->Ne = f->Ne;
+  F->MN = f->MN;
+  if (f->rn && (up(F, f->rn, (size_t)(f->Nn + 1) * 3, &F->rn, s) || up(F, f->ed, n1 * 4, &F->ed, s) ||
+                up(F, f->md, n1, &F->md, s) || up(F, f->sd, n1, &F->sd, s))) {
+    bq_field_ctx_destroy(F);
+    return -1;
+  }
   if (up(F, f->E, n1 * 24, &C.E, s) || up(F, f->ren, n1 * 12, &C.ren, s) || up(F, f->wen, n1 * 12, &C.wen, s) ||
       up(F, f->kn, (size_t)(f->MN + 1) * 2, &C.kn, s)) {
     bq_field_ctx_destroy(F);
@@ -501,6 +647,29 @@ extern "C" int bq_field_eval(void* handle, int kind, int n, const double* pts, c
     bq::set_error("field: quasi-periodic Green's function, a reciprocal order has beta_n == 0 (code %d)", err);
     return err;
   }
+  return 0;
+}
+
+extern "C" int bq_field_domain_ids(void* handle, int n, const double* pts, int* did)
+{
+  if (n <= 0) return 0;
+  FieldCtx* F = (FieldCtx*)handle;
+  if (!F->rn) {
+    bq::set_error("domain ids: the field context holds no node data");
+    return -2;
+  }
+  hipStream_t s = bq::stream();
+  double* dp = nullptr;
+  int* dd = nullptr;
+  BQ_HIP(hipMallocAsync((void**)&dp, sizeof(double) * 3 * (size_t)n, s));
+  BQ_HIP(hipMallocAsync((void**)&dd, sizeof(int) * (size_t)n, s));
+  BQ_HIP(hipMemcpyAsync(dp, pts, sizeof(double) * 3 * (size_t)n, hipMemcpyHostToDevice, s));
+  BQ_KTIME("k_domain_id", s, k_domain_id<<<n, 256, 0, s>>>(n, dp, F->Ne, F->MN, F->rn, F->ed, F->md, F->sd, dd));
+  BQ_HIP(hipGetLastError());
+  BQ_HIP(hipMemcpyAsync(did, dd, sizeof(int) * (size_t)n, hipMemcpyDeviceToHost, s));
+  BQ_HIP(hipFreeAsync(dp, s));
+  BQ_HIP(hipFreeAsync(dd, s));
+  BQ_HIP(hipStreamSynchronize(s));
   return 0;
 }
 

@@ -117,8 +117,22 @@ static void *context_for(DQD2 *qd)
   f.xli = bd->xli; f.wli = bd->wli; f.xhi = bd->xhi; f.whi = bd->whi;
   f.z49 = bd->zt_49; f.e49 = bd->et_49; f.w49 = bd->wt_49;
   f.w34 = bd->wt_34; f.z37 = bd->zt_37; f.e37 = bd->et_37; f.w37 = bd->wt_37;
+  /* node coordinates and element connectivity for the device domain test */
+  double *rnf = calloc((size_t)(bd->Nn + 1) * 3, sizeof(double));
+  int *edf = calloc((size_t)n1 * 4, sizeof(int));
+  for (int i = 0; i <= bd->Nn; i++)
+    for (int k = 0; k < 3; k++) rnf[3 * i + k] = bd->rn[i][k];
+  for (int i = 0; i < n1; i++)
+    for (int j = 0; j < 4; j++) edf[4 * i + j] = bd->ed[i][j];
+  f.Nn = bd->Nn;
+  f.rn = rnf;
+  f.ed = edf;
+  f.md = bd->md;
+  f.sd = bd->sd;
   void *ctx = NULL;
   int rc = bq_field_ctx_create(&f, &ctx);
+  free(rnf);
+  free(edf);
   for (int d = 0; d <= MN; d++) {
     free(Eb[d]); free(dEb[d]); free(Hb[d]); free(dHb[d]); free(Ub[d]); free(dUb[d]);
   }
@@ -160,6 +174,34 @@ static double *launch(DQD2 *qd, int kind, int m, const double *r0, const int *dd
   return out;
 }
 
+static void fold(DQD2 *qd, const double *rt, double *r0, int *l1, int *l2);
+
+/* domain ids of n folded points in one device launch (the reference's domain_id per point) */
+static void domain_ids(DQD2 *qd, int n, const double *r0, int *did)
+{
+  if (n <= 0) return;
+  pthread_mutex_lock(&g_lock);
+  void *ctx = context_for(qd);
+  int rc = bq_field_domain_ids(ctx, n, r0, did);
+  pthread_mutex_unlock(&g_lock);
+  if (rc) field_fail("domain identification");
+}
+
+int bq_domain_id_batch(void *qdv, int n, const double *r, int *did)
+{
+  DQD2 *qd = (DQD2 *)qdv;
+  if (n < 0 || (n > 0 && (!r || !did))) return -2;
+  if (n == 0) return 0;
+  double *r0 = malloc(sizeof(double) * 3 * (size_t)n);
+  for (int p = 0; p < n; p++) {
+    int l1, l2;
+    fold(qd, r + 3 * p, r0 + 3 * p, &l1, &l2);
+  }
+  domain_ids(qd, n, r0, did);
+  free(r0);
+  return 0;
+}
+
 static void fold(DQD2 *qd, const double *rt, double *r0, int *l1, int *l2)
 {
   if (qd->bd.ps == 1) {
@@ -193,11 +235,13 @@ static void scattered(DQD2 *qd, int kind, int n, const double *r, int type, doub
   double *r0 = malloc(sizeof(double) * 3 * nn);
   int *l1 = malloc(sizeof(int) * nn), *l2 = malloc(sizeof(int) * nn), *dd = malloc(sizeof(int) * nn);
   int m = 0;
-  for (int p = 0; p < n; p++) {
-    fold(qd, r + 3 * p, r0 + 3 * m, &l1[p], &l2[p]);
-    did[p] = domain_id(r0 + 3 * m, qd);
-    if (did[p] >= 0) dd[m++] = did[p];
-  }
+  for (int p = 0; p < n; p++) fold(qd, r + 3 * p, r0 + 3 * p, &l1[p], &l2[p]);
+  domain_ids(qd, n, r0, did);
+  for (int p = 0; p < n; p++)
+    if (did[p] >= 0) {
+      for (int k = 0; k < 3; k++) r0[3 * m + k] = r0[3 * p + k];
+      dd[m++] = did[p];
+    }
   double *out = launch(qd, kind, m, r0, dd, NULL, NULL, type);
   for (int p = 0, q = 0; p < n; p++) {
     for (int i = 0; i < nx; i++) {
@@ -599,11 +643,13 @@ static void eh_memp(DQD2 *qd, int n, const double *r, int type, int total, doubl
   double *r0 = malloc(sizeof(double) * 3 * nn);
   int *l1 = malloc(sizeof(int) * nn), *l2 = malloc(sizeof(int) * nn), *dd = malloc(sizeof(int) * nn);
   int m = 0;
-  for (int p = 0; p < n; p++) {
-    fold(qd, r + 3 * p, r0 + 3 * m, &l1[p], &l2[p]);
-    ret[p] = domain_id(r0 + 3 * m, qd);
-    if (ret[p] >= 0) dd[m++] = ret[p];
-  }
+  for (int p = 0; p < n; p++) fold(qd, r + 3 * p, r0 + 3 * p, &l1[p], &l2[p]);
+  domain_ids(qd, n, r0, ret);
+  for (int p = 0; p < n; p++)
+    if (ret[p] >= 0) {
+      for (int k = 0; k < 3; k++) r0[3 * m + k] = r0[3 * p + k];
+      dd[m++] = ret[p];
+    }
   double *out = launch(qd, 2, m, r0, dd, NULL, NULL, type);
   const double complex ch = qd->pw.lambda_0 / (2.0 * M_PI * I);
   for (int p = 0, q = 0; p < n; p++) {

@@ -3,16 +3,19 @@
 // wave read from ipw.txt per run (d3qd2_src/d3qd2_bv_solver.c:3-21); each request here is that
 // run (wave set-up, solve_bieq, EH_t at probe points) on a model kept resident.
 //
-// Worker slots: one host thread per slot; each owns one DQD2 per mesh (read from the mesh's
-// set-up file), and the solver session of a DQD2 runs on a stream of its own, so requests in
-// different slots overlap on the device (one slot's assembly, FP64 vector work, beside
-// another's factorisation, MFMA work).  Requests enter the ready queue at their arrival times.
-// A free slot takes, in arrival order, the first ready request it can admit: preferably one
-// whose mesh and wave (wavelength, direction) are resident in that slot, whose matrices and
-// factors are then reused (a polarisation or amplitude change only changes the right-hand
-// side, d3qd2_src/d3qd2_setup.c:1309-1349); else the first whose device memory fits the limit
-// next to what the other slots hold, releasing this slot's own idle session first and, if
-// still short, the least recently used idle session of another slot.
+// Resident instances: a pool of models (one DQD2 each, read from its mesh's set-up file), each
+// holding the device session of the last wave it solved (matrices and factors).  Worker slots
+// (one host thread each) take ready requests as they arrive; the solver session of an instance
+// runs on a stream of its own, so requests in different slots overlap on the device, and a
+// phase gate lets at most one solve assemble and one factorise at a time (one request's FP64
+// vector work beside another's matrix work).  A free slot serves, in arrival order:
+//   1. a request whose mesh and wave (wavelength, direction) an idle instance holds: its
+//      matrices and factors are reused (a polarisation or amplitude change only changes the
+//      right-hand side, d3qd2_src/d3qd2_setup.c:1309-1349);
+//   2. else the first request whose wave no busy instance holds (those wait for it): on a new
+//      instance when its session fits the memory limit next to all others (idle instances keep
+//      their waves for later repeats), else on the least recently used idle instance of its
+//      mesh, else on a new instance after releasing idle instances, least recently used first.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -48,18 +51,20 @@ struct Mesh {
   bool measured = false;
 };
 
-struct Slot {
-  std::vector<DQD2*> qd;        // per mesh, read on first use
-  int mesh = -1;                // mesh of the resident session
+struct Inst {
+  int mesh = -1;
+  DQD2* qd = nullptr;
+  bool valid = false;            // a solved wave is resident
   double lam = 0, th = 0, ph = 0;
   bool busy = false;
-  double bytes = 0.0;           // device bytes of the resident session
+  double bytes = 0.0;            // device bytes of its session (reserved estimate while running)
   double last_use = 0.0;
 };
 
 struct Engine {
   std::vector<Mesh> meshes;
-  std::vector<Slot> slots;
+  std::vector<Inst*> pool;
+  int nslots = 1;
   double limit = 0.0;
   int precision = 1;
 };
@@ -104,22 +109,29 @@ double estimate_bytes(const Mesh& m, int precision)
   return b;
 }
 
-DQD2* slot_model(Engine& E, Slot& s, int m)
+Inst* new_inst(Engine& E, int m)
 {
-  if ((int)s.qd.size() < (int)E.meshes.size()) s.qd.resize(E.meshes.size(), nullptr);
-  if (!s.qd[m]) {
-    DQD2* q = calloc(1, (size_t)bq_sizeof_dqd2());
-    dat_read((char*)E.meshes[m].path.c_str(), q);
-    s.qd[m] = q;
-  }
-  return s.qd[m];
+  Inst* I = new Inst();
+  I->mesh = m;
+  I->qd = calloc(1, (size_t)bq_sizeof_dqd2());
+  dat_read((char*)E.meshes[m].path.c_str(), I->qd);
+  E.pool.push_back(I);
+  return I;
 }
 
-void release_slot(Engine& E, Slot& s)
+void drop_inst(Engine& E, Inst* I)
 {
-  if (s.mesh >= 0 && s.qd[s.mesh]) bq_release_solver(s.qd[s.mesh]);
-  s.mesh = -1;
-  s.bytes = 0.0;
+  finalize_dqd2(I->qd);  // releases its solver session
+  free(I->qd);
+  E.pool.erase(std::find(E.pool.begin(), E.pool.end(), I));
+  delete I;
+}
+
+double held(const Engine& E)
+{
+  double b = 0.0;
+  for (const Inst* I : E.pool) b += I->bytes;
+  return b;
 }
 
 }  // namespace
@@ -145,7 +157,7 @@ void* bq_serve_create(int nmesh, const char* const* setup_dat, int nslots, doubl
     m.est_bytes = estimate_bytes(m, precision);
     E->meshes.push_back(m);
   }
-  E->slots.resize(nslots);
+  E->nslots = nslots;
   return E;
 }
 
@@ -153,12 +165,7 @@ void bq_serve_destroy(void* h)
 {
   Engine* E = (Engine*)h;
   if (!E) return;
-  for (Slot& s : E->slots)
-    for (DQD2* q : s.qd)
-      if (q) {
-        finalize_dqd2(q);
-        free(q);
-      }
+  while (!E->pool.empty()) drop_inst(*E, E->pool.back());
   delete E;
 }
 
@@ -184,89 +191,85 @@ int bq_serve_run(void* h, int nreq, const bq_serve_req* req, int nprobe, const d
   int released = 0, finished = 0, failed = 0;
   const clk::time_point t0 = clk::now();
 
-  // bytes held by slots other than w
-  auto held_by_others = [&](int w) {
-    double b = 0.0;
-    for (int i = 0; i < (int)E->slots.size(); ++i)
-      if (i != w) b += E->slots[i].bytes;
-    return b;
+  auto same_wave = [](const Inst* I, const bq_serve_req& r) {
+    return I->valid && I->mesh == r.mesh && I->lam == r.lambda0 && I->th == r.theta && I->ph == r.phi;
   };
-  // pick a ready request for slot w (mu held); -1 if none fits now
-  auto pick = [&](int w) -> int {
-    Slot& s = E->slots[w];
-    for (size_t q = 0; q < ready.size(); ++q) {
-      const bq_serve_req& r = req[ready[q]];
-      if (r.mesh == s.mesh && r.lambda0 == s.lam && r.theta == s.th && r.phi == s.ph) {
-        const int id = ready[q];
-        ready.erase(ready.begin() + q);
-        return id;
-      }
-    }
-    for (size_t q = 0; q < ready.size(); ++q) {
-      const bq_serve_req& r = req[ready[q]];
-      const double need = E->meshes[r.mesh].est_bytes;
-      const double own = (r.mesh == s.mesh) ? s.bytes : 0.0;
-      double others = held_by_others(w);
-      if (others + std::max(need, own) > E->limit) {
-        // release idle sessions of other slots, least recently used first
-        std::vector<int> idle;
-        for (int i = 0; i < (int)E->slots.size(); ++i)
-          if (i != w && !E->slots[i].busy && E->slots[i].mesh >= 0) idle.push_back(i);
-        std::sort(idle.begin(), idle.end(),
-                  [&](int a, int b) { return E->slots[a].last_use < E->slots[b].last_use; });
-        for (int i : idle) {
-          if (others + std::max(need, own) <= E->limit) break;
-          release_slot(*E, E->slots[i]);
-          others = held_by_others(w);
+  // a request and an instance for a free slot (mu held); false if nothing can start now
+  auto pick = [&](int& id, Inst*& inst) -> bool {
+    for (size_t q = 0; q < ready.size(); ++q)
+      for (Inst* I : E->pool)
+        if (!I->busy && same_wave(I, req[ready[q]])) {
+          id = ready[q];
+          inst = I;
+          ready.erase(ready.begin() + q);
+          return true;
         }
+    for (size_t q = 0; q < ready.size(); ++q) {
+      const bq_serve_req& r = req[ready[q]];
+      bool deferred = false;
+      for (const Inst* I : E->pool) deferred = deferred || (I->busy && same_wave(I, r));
+      if (deferred) continue;  // its factors are in use: it reuses them when that request ends
+      // a new instance while memory allows (the idle ones keep their waves for repeats), else
+      // the least recently used idle instance of the mesh, else a new one after releasing idle
+      // instances of any mesh, least recently used first
+      const double need = E->meshes[r.mesh].est_bytes;
+      Inst* best = nullptr;
+      if (held(*E) + need <= E->limit) {
+        best = new_inst(*E, r.mesh);
+        best->bytes = need;
       }
-      if (others + std::max(need, own) <= E->limit) {
-        const int id = ready[q];
-        ready.erase(ready.begin() + q);
-        return id;
+      if (!best)
+        for (Inst* I : E->pool)
+          if (!I->busy && I->mesh == r.mesh && (!best || I->last_use < best->last_use)) best = I;
+      if (!best) {
+        std::vector<Inst*> idle;
+        for (Inst* I : E->pool)
+          if (!I->busy) idle.push_back(I);
+        std::sort(idle.begin(), idle.end(), [](const Inst* a, const Inst* b) { return a->last_use < b->last_use; });
+        size_t k = 0;
+        while (held(*E) + need > E->limit && k < idle.size()) drop_inst(*E, idle[k++]);
+        if (held(*E) + need > E->limit) continue;  // wait for busy instances to end
+        best = new_inst(*E, r.mesh);
+        best->bytes = need;
       }
+      id = ready[q];
+      inst = best;
+      ready.erase(ready.begin() + q);
+      return true;
     }
-    return -1;
+    return false;
   };
 
   auto worker = [&](int w) {
     std::unique_lock<std::mutex> lk(mu);
-    Slot& s = E->slots[w];
     for (;;) {
       int id = -1;
+      Inst* I = nullptr;
       cv.wait(lk, [&] {
         if (released == nreq && ready.empty()) return true;
-        id = pick(w);
-        return id >= 0;
+        return pick(id, I);
       });
       if (id < 0) return;  // nothing left
       const bq_serve_req& r = req[id];
-      s.busy = true;
-      const bool same_mesh = (s.mesh == r.mesh);
-      if (!same_mesh) {
-        release_slot(*E, s);
-        s.mesh = r.mesh;
-      }
-      // reserve the estimate (or the measured size) while the request runs
-      s.bytes = std::max(s.bytes, E->meshes[r.mesh].est_bytes);
+      I->busy = true;
+      I->bytes = std::max(I->bytes, E->meshes[r.mesh].est_bytes);
       lk.unlock();
       bq_serve_out o;
       memset(&o, 0, sizeof o);
       o.slot = w;
       o.t_arrival_s = r.arrival_s;
       o.t_start_s = now_s(t0);
-      DQD2* qd = slot_model(*E, s, r.mesh);
       const double ex = r.pol == 1 ? 1.0 : 0.0, ey = r.pol == 1 ? 0.0 : 1.0;
-      int rc = bq_set_incidence(qd, r.lambda0, r.theta, r.phi, ex, 0.0, ey, 0.0, 0.0);
-      if (rc == 0) rc = bq_solve_bieq_rc(qd);
+      int rc = bq_set_incidence(I->qd, r.lambda0, r.theta, r.phi, ex, 0.0, ey, 0.0, 0.0);
+      if (rc == 0) rc = bq_solve_bieq_rc(I->qd);
       if (rc == 0 && nprobe > 0) {
         std::vector<int> did(nprobe);
-        rc = bq_EH_t_batch(qd, nprobe, probes, 1, E_out + (size_t)id * nprobe * 6, H_out + (size_t)id * nprobe * 6,
-                           did.data());
+        rc = bq_EH_t_batch(I->qd, nprobe, probes, 1, E_out + (size_t)id * nprobe * 6,
+                           H_out + (size_t)id * nprobe * 6, did.data());
       }
       if (rc == 0) {
         bq_stats st;
-        if (bq_solve_stats(qd, &st) == 0) {
+        if (bq_solve_stats(I->qd, &st) == 0) {
           o.reused = st.reused;
           o.gmres_iterations = st.gmres_iterations;
           o.backward_error = st.backward_error;
@@ -274,35 +277,37 @@ int bq_serve_run(void* h, int nreq, const bq_serve_req* req, int nprobe, const d
       }
       o.status = rc;
       o.t_done_s = now_s(t0);
-      const double used = (double)bq_solver_bytes(qd);
+      const double used = (double)bq_solver_bytes(I->qd);
       lk.lock();
       out[id] = o;
       Mesh& m = E->meshes[r.mesh];
       if (rc == 0 && used > 0.0) {
-        s.bytes = used;
+        I->bytes = used;
         if (!m.measured || used > m.est_bytes) m.est_bytes = used;
         m.measured = true;
-        s.lam = r.lambda0;
-        s.th = r.theta;
-        s.ph = r.phi;
+        I->valid = true;
+        I->lam = r.lambda0;
+        I->th = r.theta;
+        I->ph = r.phi;
       } else {
-        release_slot(*E, s);
-        s.lam = s.th = s.ph = 0.0;
+        bq_release_solver(I->qd);
+        I->bytes = 0.0;
+        I->valid = false;
       }
       if (rc != 0) ++failed;
       if (log)
         fprintf(stderr, "bq_serve: request %d (mesh %d) on slot %d: %s, start %.2f s, done %.2f s%s\n", id, r.mesh,
                 w, rc ? "failed" : "ok", o.t_start_s, o.t_done_s, o.reused ? ", factors reused" : "");
-      s.busy = false;
-      s.last_use = o.t_done_s;
+      I->busy = false;
+      I->last_use = o.t_done_s;
       ++finished;
       cv.notify_all();
     }
   };
 
-  if (E->slots.size() > 1) bq_set_phase_gate(gate_fn);
+  if (E->nslots > 1) bq_set_phase_gate(gate_fn);
   std::vector<std::thread> th;
-  for (int w = 0; w < (int)E->slots.size(); ++w) th.emplace_back(worker, w);
+  for (int w = 0; w < E->nslots; ++w) th.emplace_back(worker, w);
   for (int k = 0; k < nreq; ++k) {
     const int id = order[k];
     const double wait = req[id].arrival_s - now_s(t0);

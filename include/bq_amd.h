@@ -59,6 +59,11 @@ int bq_model_dims(void *qd, int *Ne, int *NN, int *dNe);
  * E, H: [n][3] complex out; did[n] out (domain id, -1 if none).  type 0..3 as EH_t. */
 int bq_EH_t_batch(void *qd, int n, const double *r, int type, double *E, double *H, int *did);
 
+/* Domain ids of n points in one GPU launch (batched domain_id / q0_domain_id,
+ * d3qd2_src/d3qd2_setup.c:157-245: on a lattice the point is folded into the unit cell first);
+ * did[n] receives the id, -1 outside every domain.  The field entry points use it. */
+int bq_domain_id_batch(void *qd, int n, const double *r, int *did);
+
 /* Total fields from the modified potential at n points in one GPU launch (batched
  * EH_mEMP_t of d3qd2_src/d3qd2_field.c:175-190).  E, H: [n][3] complex out; ret[n] = domain id,
  * -1 outside every domain, or -did (-99 for the open region) with zero fields for a point too

@@ -175,3 +175,39 @@ def test_gpu_example3_field_images(tmp_path, gpu_lib):
     bar = _read_png(out / "color_bar.png")
     assert bar.shape == (2 * mx, max(2 * mx // 16, 1), 3)
     m.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["sample3/ex3.dat", "sphere_m2/ex.dat", "cone_m1/ex2.dat"])
+def test_gpu_domain_ids_match_host(gpu_lib, case):
+    """bq_domain_id_batch (one device launch, signed solid angles summed per domain by a
+    workgroup) against the host's q0_domain_id (the reference's fid_calc_solid_angle walk,
+    d3qd2_src/d3qd2_setup.c:157-245,1351-1422) for random points in the cell and points just
+    off the surface on both sides of element nodes: identical ids."""
+    import ctypes
+    from datfile import read_dat
+    from bem3_emf_qd2_amd.solver import Model, L as SL
+    G = Path(__file__).resolve().parent / "golden"
+    d = read_dat(G / case)
+    m = Model.from_dat(G / case)
+    lb = SL()
+    D = ctypes.POINTER(ctypes.c_double)
+    lb.q0_domain_id.argtypes = [D, ctypes.c_void_p]
+    lb.bq_domain_id_batch.argtypes = [ctypes.c_void_p, ctypes.c_int, D, ctypes.POINTER(ctypes.c_int)]
+    rng = np.random.default_rng(3)
+    qd = d["qd"]
+    d1, d2 = qd[4:6], qd[6:8]
+    a = rng.uniform(-0.5, 0.5, (3000, 2))
+    pts = np.c_[a[:, :1] * d1 + a[:, 1:] * d2, rng.uniform(-0.6, 0.6, 3000)]
+    ren, wen = d["ren"][1:].reshape(-1, 3), d["wen"][1:].reshape(-1, 3)
+    live = np.linalg.norm(wen, axis=1) > 0
+    nrm = wen[live] / np.linalg.norm(wen[live], axis=1, keepdims=True)
+    sel = rng.choice(len(nrm), 500, replace=False)
+    near = np.r_[ren[live][sel] + 2e-3 * nrm[sel], ren[live][sel] - 2e-3 * nrm[sel]]
+    pts = np.ascontiguousarray(np.r_[pts, near])
+    got = np.zeros(len(pts), np.int32)
+    assert lb.bq_domain_id_batch(m.ptr, len(pts), pts.ctypes.data_as(D), got.ctypes.data_as(ctypes.POINTER(ctypes.c_int))) == 0
+    host = np.array([lb.q0_domain_id(np.ascontiguousarray(p).ctypes.data_as(D), m.ptr) for p in pts])
+    m.close()
+    print(case, "ids", np.bincount(host + 1), "mismatches", int(np.sum(got != host)))
+    assert np.array_equal(got, host)
