@@ -273,6 +273,8 @@ int bq_serve_run(void* h, int nreq, const bq_serve_req* req, int nprobe, const d
           o.reused = st.reused;
           o.gmres_iterations = st.gmres_iterations;
           o.backward_error = st.backward_error;
+          o.n_qpgf_gl = st.reused ? 0 : st.n_qpgf_gl;
+          o.asm_cut = st.asm_cut;
         }
       }
       o.status = rc;

@@ -21,7 +21,8 @@ class Req(ctypes.Structure):
 class Out(ctypes.Structure):
     _fields_ = [("status", ctypes.c_int), ("slot", ctypes.c_int), ("reused", ctypes.c_int),
                 ("gmres_iterations", ctypes.c_int), ("t_arrival_s", ctypes.c_double),
-                ("t_start_s", ctypes.c_double), ("t_done_s", ctypes.c_double), ("backward_error", ctypes.c_double)]
+                ("t_start_s", ctypes.c_double), ("t_done_s", ctypes.c_double), ("backward_error", ctypes.c_double),
+                ("n_qpgf_gl", ctypes.c_longlong), ("asm_cut", ctypes.c_double)]
 
 
 def _L():

@@ -49,6 +49,12 @@ FP64_PEAK_TFS = 78.6
 FLOPS_PER_EVAL = {"c1": {"k_q0_gl": 55999, "k_q0_gh": 51625, "k_q0_sing": 50274},
                   "c2": {"k_q0_gl": 51253, "k_q0_gh": 50007, "k_q0_sing": 50175}}
 FLOPS_PER_EVAL["c2s"] = FLOPS_PER_EVAL["c2"]
+# the 6144-quad sweep, read on its 384-quad variant (c4s) at the same six timed points
+# (profiles/k_q0_gl_c4s_pmc.md): the reference's accuracy (cut 40) and the mixed path's tuned
+# assembly (tolerance 1e-12: cut 26 at every point)
+FLOPS_PER_EVAL["c4"] = {"k_q0_gl": 62737}
+FLOPS_PER_EVAL["c4_tuned"] = {"k_q0_gl": 48457}
+FLOPS_PER_EVAL["c4s"], FLOPS_PER_EVAL["c4s_tuned"] = FLOPS_PER_EVAL["c4"], FLOPS_PER_EVAL["c4_tuned"]
 # HBM bytes per evaluation of k_q0_gl: (FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00) KiB x 1024 of one
 # launch over its evaluations (factors from a 1 GiB copy read with the same counters)
 TRAFFIC_PER_EVAL = {"c1": {"k_q0_gl": (1163230.7 * 2.0 + 9545073.7) * 1024 / 46912896},
@@ -191,7 +197,10 @@ def _roofline(ktimes, stats_per_step, steps, workload):
     name = max(ktimes, key=lambda k: ktimes[k][0])
     tot_ms, cnt = ktimes[name]
     avg_s = tot_ms / cnt / 1e3
-    r = {"kernel": name, "bound": "mfma", "pipe": "fp64-valu", "unit": "TFLOP/s", "peak": FP64_PEAK_TFS,
+    r = {"kernel": name, "bound": "mfma", "pipe": "fp64-valu",
+         "bound_note": "compute bound on the FP64 vector pipe (no matrix instructions); MI355X's FP64 vector "
+                       "and FP64 matrix peaks are both 78.6 TFLOP/s",
+         "unit": "TFLOP/s", "peak": FP64_PEAK_TFS,
          "avg_launch_ms": avg_s * 1e3, "launches_per_step": cnt / steps, "traffic": None, "achieved": None,
          "frac": None}
     if name == "zgetrf_update":
@@ -393,10 +402,14 @@ def run_sweep(args, w, bq, L):
                     r["batched_probe_E_rel_err_vs_fp64"] = float(np.abs(pb[:n3] - pf[:n3]).max() / np.abs(pf[:n3]).max())
                     r["batched_probe_H_rel_err_vs_fp64"] = float(np.abs(pb[n3:] - pf[n3:]).max() / np.abs(pf[n3:]).max())
             rows.append(r)
-        # QPGF evaluations of the timed points' assemblies (identical in every pass)
+        # QPGF evaluations per launch of the pass the kernel times come from
         st_ref = res[serial][-1]["st"]
         evals = {k: float(np.mean([q["st"][k] for q in res[serial]])) for k in ("n_qpgf_gl", "n_qpgf_gh", "n_qpgf_sing")}
-        roof = _roofline(kt, evals, args.steps, args.workload) if kt else None
+        fkey = args.workload
+        if batched is not None:
+            evals["n_qpgf_gl"] = float(np.mean([o["n_qpgf_gl"] for o in batched["outs"]]))
+            fkey = args.workload + "_tuned" if args.asm_tol > 0 else args.workload
+        roof = _roofline(kt, evals, args.steps, fkey) if kt else None
         cpu = None
         if not args.no_cpu_baseline:
             _set_point(m, timed[0])

@@ -152,6 +152,8 @@ typedef struct bq_serve_out {
   int gmres_iterations;
   double t_arrival_s, t_start_s, t_done_s; /* seconds after the start of the run */
   double backward_error;
+  long long n_qpgf_gl;           /* QPGF evaluations of the regular periodic pairs (0 if reused) */
+  double asm_cut;                /* lattice-sum cut of its assembly */
 } bq_serve_out;
 
 void *bq_serve_create(int nmesh, const char *const *setup_dat, int nslots, double mem_limit_gb, int precision);
