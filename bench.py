@@ -193,8 +193,10 @@ def _kernel_times(L):
 
 
 def _roofline(ktimes, stats_per_step, steps, workload):
-    """Roofline object for the kernel with the largest total device time."""
-    name = max(ktimes, key=lambda k: ktimes[k][0])
+    """Roofline object for the hand-written kernel (k_*) with the largest total device time; the
+    timed ranges of library calls (rocBLAS updates inside getrf/getrs) are not candidates."""
+    own = {k: v for k, v in ktimes.items() if k.startswith("k_")} or ktimes
+    name = max(own, key=lambda k: own[k][0])
     tot_ms, cnt = ktimes[name]
     avg_s = tot_ms / cnt / 1e3
     r = {"kernel": name, "bound": "mfma", "pipe": "fp64-valu",
@@ -296,7 +298,7 @@ def run_sweep(args, w, bq, L):
             E, H, did = m.fields(probes, 1)
             return st, np.r_[E.ravel(), H.ravel()]
 
-        kt = {}
+        kt = kt_serial = {}
         if args.passes:
             modes = [m_ for m_ in args.passes.split(",") if m_]
         elif args.precision == "mixed":
@@ -329,7 +331,7 @@ def run_sweep(args, w, bq, L):
                           f"(assembly {st.t_assemble_ms:.0f} ms, solve {st.t_solve_ms:.0f} ms, "
                           f"GMRES {st.gmres_iterations}, eta {st.backward_error:.1e})", file=sys.stderr, flush=True)
                 if mode == modes[-1]:
-                    kt = _kernel_times(L)
+                    kt = kt_serial = _kernel_times(L)
                     L.bq_kernel_timing(0)
             bq.solve_precision("fp64")
             bq.assembly_tolerance(0.0)
@@ -405,11 +407,12 @@ def run_sweep(args, w, bq, L):
         # QPGF evaluations per launch of the pass the kernel times come from
         st_ref = res[serial][-1]["st"]
         evals = {k: float(np.mean([q["st"][k] for q in res[serial]])) for k in ("n_qpgf_gl", "n_qpgf_gh", "n_qpgf_sing")}
-        fkey = args.workload
-        if batched is not None:
-            evals["n_qpgf_gl"] = float(np.mean([o["n_qpgf_gl"] for o in batched["outs"]]))
-            fkey = args.workload + "_tuned" if args.asm_tol > 0 else args.workload
-        roof = _roofline(kt, evals, args.steps, fkey) if kt else None
+        # roofline from the last serial pass: its launches do not share the GPU with another
+        # session's, so a launch's HIP-event duration is the kernel's own
+        fkey = args.workload + "_tuned" if serial == "mixed" and args.asm_tol > 0 else args.workload
+        roof = _roofline(kt_serial, evals, args.steps, fkey) if kt_serial else None
+        if roof is not None:
+            roof["pass"] = f"serial {serial}"
         cpu = None
         if not args.no_cpu_baseline:
             _set_point(m, timed[0])
@@ -456,6 +459,9 @@ def run_sweep(args, w, bq, L):
         "roofline": roof,
         "cpu_baseline": cpu,
     }
+    if batched is not None and kt_serial:
+        out["kernels_ms_per_step_serial"] = {k: v[0] / args.steps
+                                             for k, v in sorted(kt_serial.items(), key=lambda x: -x[1][0])}
     if batched is not None:
         out["batched_service_ms"] = {"p50": float(np.percentile(service, 50)), "p99": float(np.percentile(service, 99))}
         out["batched_wall_s"] = batched["wall"]
