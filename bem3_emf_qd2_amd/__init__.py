@@ -141,3 +141,11 @@ def assembly_tolerance(tol=None) -> float:
     L.bq_assembly_tolerance.argtypes = [ctypes.c_double]
     L.bq_assembly_tolerance.restype = ctypes.c_double
     return float(L.bq_assembly_tolerance(-1.0 if tol is None else float(tol)))
+
+
+def lu_update_mode(mode=None) -> str:
+    """Query (None) or set the update arithmetic of the complex FP32 factorisation:
+    "bf16x3" (split bf16 products on the matrix cores, the default) or "fp32" (rocBLAS
+    cgemm).  Returns the mode in force before the call."""
+    m = {None: -1, "fp32": 0, "bf16x3": 1}[mode]
+    return ["fp32", "bf16x3"][lib().bq_lu_update_mode(m)]

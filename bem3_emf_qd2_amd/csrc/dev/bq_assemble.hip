@@ -211,7 +211,10 @@ __device__ __forceinline__ int pair_rule_fs(double kabs, const Elem& et, const E
 // row does not exist), each lane walks the source's Gauss points with stride 16 or 21.
 // The per-lane sums live in LDS rather than registers: the lattice sums need nearly 256
 // registers, and keeping the kernel within 256 lets two wavefronts share each SIMD.
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
+#ifndef BQ_Q0GL_WAVES
+#define BQ_Q0GL_WAVES 2
+#endif
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BQ_Q0GL_WAVES, BQ_Q0GL_WAVES)))
 k_q0_gl(bq_qplan P, const double* __restrict__ E, const double* __restrict__ ren, const int* __restrict__ sid, int dNe,
         const unsigned char* __restrict__ flag, int full_rule, double2* __restrict__ G, double2* __restrict__ H)
 {

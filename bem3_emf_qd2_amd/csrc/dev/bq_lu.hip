@@ -24,7 +24,10 @@
 // Replaces the reference's MKL PARDISO and LAPACKE_zgesv calls
 // (d3qd2_src/d3qd2_mkl_solver.c:18-148, d3qd2_src/d3qd2_solve_bieq.c:753).
 #include <rocblas/rocblas.h>
+#include <atomic>
+#include <cstring>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 #include "bq_runtime.hpp"
 #include "bq_complex.hpp"
@@ -614,10 +617,31 @@ int gemm_op(hipStream_t s, int ta, int tb, int m, int n, int k, typename P<R>::V
   return 0;
 }
 
+// Updates C -= A B of the FP32 factorization and solves of order >= kSplitMin in every
+// dimension go to the bf16x3 matrix-core kernel (bq_mfma.hip) unless the update mode is 0.
+constexpr int kSplitMin = 128;
+std::atomic<int> g_update_mode{-1};
+
+int update_mode()
+{
+  int m = g_update_mode.load();
+  if (m < 0) {
+    const char* e = getenv("BQ_LU_UPDATE");
+    m = (e && strcmp(e, "fp32") == 0) ? 0 : 1;
+    g_update_mode.store(m);
+  }
+  return m;
+}
+
 template <class R>
 int gemm(hipStream_t s, int m, int n, int k, typename P<R>::V alpha, const typename P<R>::V* A, long lda,
          const typename P<R>::V* B, long ldb, typename P<R>::V beta, typename P<R>::V* C, long ldc)
 {
+  if constexpr (std::is_same<R, float>::value) {
+    if (alpha.x == -1.f && alpha.y == 0.f && beta.x == 1.f && beta.y == 0.f && m >= kSplitMin && n >= kSplitMin &&
+        k >= kSplitMin && update_mode() == 1)
+      return bq_cgemm_split_dev(s, m, n, k, (const float*)A, lda, (const float*)B, ldb, (float*)C, ldc);
+  }
   return gemm_op<R>(s, 0, 0, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
 }
 
@@ -802,6 +826,15 @@ int getrs(hipStream_t s, int n, int nrhs, const typename P<R>::V* A, long lda, c
 }  // namespace
 
 extern "C" {
+
+// Update mode of the FP32 factorization: 1 bf16x3 matrix-core kernel (default), 0 rocBLAS
+// cgemm (FP32 matrix cores); mode < 0 only queries.  Returns the mode in force before.
+int bq_lu_update_mode(int mode)
+{
+  const int old = update_mode();
+  if (mode == 0 || mode == 1) g_update_mode.store(mode);
+  return old;
+}
 
 // C = alpha op(A) op(B) + beta C, op = transpose when the flag is set (rocBLAS, column-major)
 int bq_zgemm_op_dev(void* stream, int ta, int tb, int m, int n, int k, const double* alpha, const double* A, long lda,

@@ -300,7 +300,9 @@ def run_sweep(args, w, bq, L):
 
         kt = kt_serial = {}
         if args.passes:
-            modes = [m_ for m_ in args.passes.split(",") if m_]
+            modes = [m_ for m_ in args.passes.split(",") if m_ in ("fp64", "mixed")]
+            if not modes:
+                raise SystemExit("bench.py: --passes needs a serial pass (fp64 or mixed) besides batched")
         elif args.precision == "mixed":
             # the plain FP64 path, then the batched mixed-precision path (below); the serial
             # mixed pass (surface-solution errors) runs when asked for with --passes fp64,mixed

@@ -188,6 +188,19 @@ int bq_solve_precision(int mode);
  * solves.  Returns the value in force. */
 double bq_assembly_tolerance(double tol);
 
+/* Update arithmetic of the complex FP32 factorisation (the preconditioner of the
+ * mixed-precision solve; the reference's PARDISO call, d3qd2_src/d3qd2_mkl_solver.c:18-148,
+ * factors in FP64 only): 1 = bf16x3 products on the matrix cores (FP32 operands split into
+ * bf16 hi/lo pairs, three products per real product, FP32 accumulation; relative error ~2^-16
+ * per term), 0 = FP32 products (rocBLAS cgemm).  mode < 0 only queries.  BQ_LU_UPDATE=fp32
+ * sets 0 at start-up.  Returns the mode in force before the call. */
+int bq_lu_update_mode(int mode);
+/* C -= A B, complex FP32, column-major device matrices (A m x k, B k x n, C m x n), with the
+ * bf16x3 products above; the building block of the FP32 factorisation's rank-512 updates and
+ * many-right-hand-side solves.  stream: a hipStream_t or NULL.  Returns 0 or -1. */
+int bq_cgemm_split_dev(void *stream, int m, int n, int k, const float *A, long lda, const float *B, long ldb,
+                       float *C, long ldc);
+
 /* Per-kernel timing (no counterpart in the reference; for benchmarks and profiling):
  * bq_kernel_timing(1) turns on HIP-event timing of every kernel launch the library makes
  * (on the stream it launches on) and clears the totals; bq_kernel_timing(0) turns it off.

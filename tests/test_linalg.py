@@ -54,23 +54,99 @@ def test_gpu_zgesv_pivots_across_outer_panels(gpu_lib):
     assert np.abs(X - ref).max() / np.abs(ref).max() < 1e-8
 
 
+# unit roundoff of the FP32 factorisation's updates: FP32 products, or bf16x3 products
+# (x = hi + lo + r with |r| <= 2^-17 |x|; the dropped terms lo lo and r y leave ~2^-15)
+UPDATE_U = {"fp32": 6e-8, "bf16x3": 2.0 ** -15}
+
+
 @pytest.mark.gpu
-@pytest.mark.parametrize("n,nrhs", [(1, 1), (31, 3), (100, 7), (333, 130), (2053, 64)])
-def test_gpu_cgesv_random(gpu_lib, n, nrhs):
-    """The complex FP32 LU (the correction solver of the mixed-precision global solve):
-    backward error at the level of single precision, forward error vs numpy's complex128
-    solve of the same (FP32-rounded) system within cond(A) x 2^-24 x (n + 4)."""
+@pytest.mark.parametrize("mode", ["fp32", "bf16x3"])
+@pytest.mark.parametrize("n,nrhs", [(1, 1), (31, 3), (100, 7), (333, 130), (1100, 300), (2053, 64)])
+def test_gpu_cgesv_random(gpu_lib, n, nrhs, mode):
+    """The complex FP32 LU (the preconditioner of the mixed-precision global solve) with
+    either update arithmetic (the bf16x3 kernel takes the updates of order >= 128: the
+    rank-512 trailing updates for n > 512 and the solve's block updates for >= 128
+    right-hand sides): backward error within 16 u, forward error vs numpy's complex128 solve
+    of the same (FP32-rounded) system within cond(A) x u x (n + 4), u the update's unit
+    roundoff."""
+    u = UPDATE_U[mode]
     rng = np.random.default_rng(7 * n + nrhs)
     A = (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))).astype(np.complex64).astype(complex)
     B = (rng.standard_normal((n, nrhs)) + 1j * rng.standard_normal((n, nrhs))).astype(np.complex64).astype(complex)
-    X = gpu_lib.cgesv(A, B).astype(complex)
+    old = gpu_lib.lu_update_mode(mode)
+    try:
+        X = gpu_lib.cgesv(A, B).astype(complex)
+    finally:
+        gpu_lib.lu_update_mode(old)
     ref = np.linalg.solve(A, B)
     bw = np.abs(A @ X - B).max() / (np.abs(A).max() * np.abs(X).max() * n)
     fw = np.abs(X - ref).max() / np.abs(ref).max()
     cond = np.linalg.cond(A)
-    print(n, nrhs, "backward", bw, "forward", fw, "cond", cond)
-    assert bw < 1e-6
-    assert fw < cond * (n + 4) * 6e-8
+    print(mode, n, nrhs, "backward", bw, "forward", fw, "cond", cond)
+    assert bw < 16 * u
+    assert fw < cond * (n + 4) * u
+
+
+def _bf16_split(x):
+    """hi, lo (float64) of float32 x: hi = bf16(x), lo = bf16(x - hi), round to nearest even."""
+    def rne(v):
+        b = np.asarray(v, np.float32).view(np.uint32).astype(np.uint64)
+        b = ((b + 0x7FFF + ((b >> 16) & 1)) >> 16) << 16
+        return (b & 0xFFFFFFFF).astype(np.uint32).view(np.float32)
+    x = np.asarray(x, np.float32)
+    hi = rne(x)
+    lo = rne((x - hi).astype(np.float32))
+    return hi.astype(np.float64), lo.astype(np.float64)
+
+
+def _bf16x3_product(A, B):
+    """A B with every real product taken as hi hi + hi lo + lo hi, summed exactly (float64)."""
+    arh, arl = _bf16_split(A.real)
+    aih, ail = _bf16_split(A.imag)
+    brh, brl = _bf16_split(B.real)
+    bih, bil = _bf16_split(B.imag)
+    p = lambda xh, xl, yh, yl: xh @ yh + xh @ yl + xl @ yh
+    return (p(arh, arl, brh, brl) - p(aih, ail, bih, bil)) + 1j * (p(arh, arl, bih, bil) + p(aih, ail, brh, brl))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("m,n,k", [(1, 1, 1), (33, 47, 16), (130, 257, 129), (1000, 1300, 512), (1290, 1030, 300),
+                                   (1500, 200, 77)])
+def test_gpu_cgemm_split_matches_bf16x3(gpu_lib, m, n, k):
+    """bq_cgemm_split_dev (C -= A B on the bf16 matrix cores) against the same bf16x3 products
+    summed exactly on the host: the difference is the FP32 accumulation alone (<= 2e-6 of the
+    largest |A B| entry; one dropped hi-lo term would leave ~2^-9).  The shapes cover single
+    elements, k not a multiple of 16, partial 128 x 128 tiles, tile counts with and without
+    the per-XCD remap (multiples of 8) and a partial last group of tile rows; the matrices are
+    strided (leading dimensions m + 5, k + 3, m + 7) and the padding rows must stay untouched.
+    Entries span 2^-6 .. 2^6 in magnitude."""
+    import ctypes
+    from hipmem import DeviceArray
+    L = gpu_lib.lib()
+    V = ctypes.c_void_p
+    L.bq_cgemm_split_dev.argtypes = [V, ctypes.c_int, ctypes.c_int, ctypes.c_int, V, ctypes.c_long, V, ctypes.c_long,
+                                     V, ctypes.c_long]
+    rng = np.random.default_rng(m + 3 * n + 7 * k)
+
+    def rnd(r, c):
+        s = 2.0 ** rng.uniform(-6, 6, (r, c))
+        return ((rng.standard_normal((r, c)) + 1j * rng.standard_normal((r, c))) * s).astype(np.complex64)
+
+    lda, ldb, ldc = m + 5, k + 3, m + 7
+    Acm, Bcm, Ccm = rnd(k, lda), rnd(n, ldb), rnd(n, ldc)   # column-major storage: one row per column
+    A, B, C0 = Acm[:, :m].T, Bcm[:, :k].T, Ccm[:, :m].T
+    dA, dB, dC = DeviceArray(Acm), DeviceArray(Bcm), DeviceArray(Ccm)
+    assert L.bq_cgemm_split_dev(None, m, n, k, dA.ptr, lda, dB.ptr, ldb, dC.ptr, ldc) == 0
+    out = dC.get()
+    got = out[:, :m].T.astype(complex)
+    D = _bf16x3_product(A.astype(complex), B.astype(complex))
+    exact = A.astype(complex) @ B.astype(complex)
+    err = np.abs(got - (C0.astype(complex) - D)).max() / np.abs(D).max()
+    split_err = np.abs(D - exact).max() / np.abs(exact).max()
+    print(m, n, k, "vs bf16x3 sum", err, "bf16x3 vs exact", split_err)
+    assert np.array_equal(out[:, m:], Ccm[:, m:])
+    assert err < 2e-6
+    assert split_err < 2.0 ** -13
 
 
 @pytest.mark.gpu
