@@ -21,7 +21,9 @@
 //                        registers, double-buffered with one barrier per step; each wave
 //                        owns 64 x 64 of C as 2 x 2 blocks of v_mfma_f32_32x32x16_bf16 with
 //                        separate real and imaginary accumulators (128 floats per lane), and
-//                        subtracts them from C at the end.
+//                        subtracts them from C at the end (B's tile goes into the MFMA's
+//                        first operand, so that each block comes out with C's rows along
+//                        the lanes and the read-modify-write of C is coalesced).
 // Workgroups are dispatched round-robin over the 8 XCDs; the tile index is remapped so that
 // each XCD takes a contiguous run of tiles, walked in groups of 8 tile rows, and the A and B
 // tiles in flight on one XCD are shared through its L2.
@@ -189,19 +191,21 @@ __global__ __launch_bounds__(256, 2) void k_cgemm_split(int m, int n, int tiles_
       for (int j = 0; j < 2; ++j) {
         const bf16x8 brh = as_bf(b_op[j][0]), brl = as_bf(b_op[j][1]), bih = as_bf(b_op[j][2]),
                      bil = as_bf(b_op[j][3]);
+        // B's tile in the MFMA's first operand: the block comes out transposed, with C's
+        // rows along the lanes (see the write-back)
         f32x16 r = cr[i][j], q = ci[i][j];
-        r = mma(arh, brh, r);   // re: ar br - ai bi
-        q = mma(arh, bih, q);   // im: ar bi + ai br
-        r = mma(arh, brl, r);
-        q = mma(arh, bil, q);
-        r = mma(arl, brh, r);
-        q = mma(arl, bih, q);
-        r = mma(naih, bih, r);
-        q = mma(aih, brh, q);
-        r = mma(naih, bil, r);
-        q = mma(aih, brl, q);
-        r = mma(nail, bih, r);
-        q = mma(ail, brh, q);
+        r = mma(brh, arh, r);   // re: ar br - ai bi
+        q = mma(bih, arh, q);   // im: ar bi + ai br
+        r = mma(brl, arh, r);
+        q = mma(bil, arh, q);
+        r = mma(brh, arl, r);
+        q = mma(bih, arl, q);
+        r = mma(bih, naih, r);
+        q = mma(brh, aih, q);
+        r = mma(bil, naih, r);
+        q = mma(brl, aih, q);
+        r = mma(bih, nail, r);
+        q = mma(brh, ail, q);
         cr[i][j] = r;
         ci[i][j] = q;
       }
@@ -214,25 +218,28 @@ __global__ __launch_bounds__(256, 2) void k_cgemm_split(int m, int n, int tiles_
 #undef BQ_STAGE_LOAD
 #undef BQ_STAGE_STORE
 
-  // 32 x 32 accumulator layout: register e of lane l is row 8*(e/4) + 4*(l/32) + e%4,
-  // column l%32 of the block
+  // 32 x 32 accumulator layout: register e of lane l is MFMA row 8*(e/4) + 4*(l/32) + e%4
+  // and column l%32, i.e. column 8*(e/4) + 4*(l/32) + e%4 and row l%32 of the C block: per
+  // register the 64 lanes cover 32 consecutive rows of two columns (two 256-byte segments)
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
-      const int col = tc * kTile + (2 * wc + j) * kBlk + (lane & 31);
-      if (col >= n) continue;
-      const int r0 = tr * kTile + (2 * wr + i) * kBlk + 4 * (lane >> 5);
-      float2* cc = C + (long)col * ldc;
+      const int row = tr * kTile + (2 * wr + i) * kBlk + (lane & 31);
+      if (row >= m) continue;
+      const int c0 = tc * kTile + (2 * wc + j) * kBlk + 4 * (lane >> 5);
+      float2* cr0 = C + row;
+      // all 16 loads of the block in flight before the first store; columns past n read
+      // column n - 1 (in bounds) and are not written
+      float2 v[16];
+#pragma unroll
+      for (int e = 0; e < 16; ++e) v[e] = cr0[(long)min(c0 + 8 * (e >> 2) + (e & 3), n - 1) * ldc];
 #pragma unroll
       for (int e = 0; e < 16; ++e) {
-        const int row = r0 + 8 * (e >> 2) + (e & 3);
-        if (row < m) {
-          float2 v = cc[row];
-          v.x -= cr[i][j][e];
-          v.y -= ci[i][j][e];
-          cc[row] = v;
-        }
+        const int col = c0 + 8 * (e >> 2) + (e & 3);
+        v[e].x -= cr[i][j][e];
+        v[e].y -= ci[i][j][e];
+        if (col < n) cr0[(long)col * ldc] = v[e];
       }
     }
 }
