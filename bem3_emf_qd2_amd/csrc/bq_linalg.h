@@ -38,6 +38,10 @@ int bq_cgemm_split_dev(void *stream, int m, int n, int k, const float *A, long l
 /* 1: FP32 factorization updates on the bf16x3 kernel, 0: rocBLAS cgemm; < 0 queries.
  * Returns the previous mode. */
 int bq_lu_update_mode(int mode);
+/* Free the per-stream state of the solves (rocBLAS handle, split-GEMM workspace) before the
+ * stream is destroyed. */
+void bq_lu_release_stream(void *stream);
+void bq_cgemm_split_release(void *stream);
 
 /* The row permutation the factorization stored in dinv (perm[i] = original row placed at
  * row i; device int[n]); single != 0 for the FP32 buffer. */

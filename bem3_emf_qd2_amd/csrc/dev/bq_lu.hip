@@ -827,6 +827,24 @@ int getrs(hipStream_t s, int n, int nrhs, const typename P<R>::V* A, long lda, c
 
 extern "C" {
 
+// Release what the library keeps per stream on the current device (the rocBLAS handle bound
+// to it and the split-GEMM workspace); call before destroying a stream the solves used.
+void bq_lu_release_stream(void* stream)
+{
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  {
+    std::lock_guard<std::mutex> lk(g_hmu);
+    for (size_t i = 0; i < g_handles.size(); ++i)
+      if (g_handles[i].dev == dev && g_handles[i].s == (hipStream_t)stream) {
+        (void)rocblas_destroy_handle(g_handles[i].h);
+        g_handles.erase(g_handles.begin() + i);
+        break;
+      }
+  }
+  bq_cgemm_split_release(stream);
+}
+
 // Update mode of the FP32 factorization: 1 bf16x3 matrix-core kernel (default), 0 rocBLAS
 // cgemm (FP32 matrix cores); mode < 0 only queries.  Returns the mode in force before.
 int bq_lu_update_mode(int mode)

@@ -286,6 +286,24 @@ void* workspace(hipStream_t s, size_t bytes)
 
 extern "C" {
 
+// Free the packed-operand workspace kept for this stream on the current device (call before
+// the stream is destroyed).
+void bq_cgemm_split_release(void* stream)
+{
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  std::lock_guard<std::mutex> lk(g_wmu);
+  for (size_t i = 0; i < g_ws.size(); ++i)
+    if (g_ws[i].dev == dev && g_ws[i].s == (hipStream_t)stream) {
+      if (g_ws[i].p) {
+        (void)hipStreamSynchronize((hipStream_t)stream);
+        (void)hipFree(g_ws[i].p);
+      }
+      g_ws.erase(g_ws.begin() + i);
+      return;
+    }
+}
+
 // C -= A B (complex FP32, column-major): A m x k, B k x n, C m x n.  bf16x3 products on the
 // matrix cores with FP32 accumulation (relative error ~2^-16 per product term).
 int bq_cgemm_split_dev(void* stream, int m, int n, int k, const float* A, long lda, const float* B, long ldb,

@@ -35,10 +35,14 @@
 #include <cfloat>
 #include <cstring>
 #include <algorithm>
+#include <string>
+#include <thread>
 #include "bq_runtime.hpp"
 #include "bq_complex.hpp"
 #include "bq_sess.hpp"
 #include "../bq_linalg.h"
+
+extern "C" const char* bq_last_error(void);
 
 namespace {
 
@@ -704,18 +708,69 @@ int red_factor_t(bq_sess* S)
     k_to_global<V><<<tg, 256, 0, s>>>(n, N, R.ginv[d], (const double2*)S->H[d], (V*)(d ? R.P : R.Mx));
     BQ_HIP(hipGetLastError());
   }
+  // The two domains' chains, LU(G0) -> M = G0^-1 H0 and LU(G1) -> P = G1^-1 H1, are
+  // independent: the object's runs on the session's second stream from a helper thread
+  // (getrf waits on the host for its pivot check), so that the latency-bound panel steps of
+  // one overlap the matrix-core updates of the other.
+  if (!S->s2 && hipStreamCreateWithFlags(&S->s2, hipStreamNonBlocking) != hipSuccess) {
+    S->s2 = nullptr;
+    bq::set_error("second session stream creation failed");
+    return -1;
+  }
+  hipStream_t s2 = S->s2;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  for (auto& e : ev) BQ_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  struct EvFree {
+    hipEvent_t* e;
+    ~EvFree()
+    {
+      for (int i = 0; i < 2; ++i)
+        if (e[i]) (void)hipEventDestroy(e[i]);
+    }
+  } ev_free{ev};
+  BQ_HIP(hipEventRecord(ev[0], s));  // G1, H1 converted
+  BQ_HIP(hipStreamWaitEvent(s2, ev[0], 0));
+  int dev = 0;
+  BQ_HIP(hipGetDevice(&dev));
+  int info1 = 0, rc1 = 0;
+  std::string err1;
+  std::thread helper([&] {
+    if (hipSetDevice(dev) != hipSuccess) {
+      rc1 = -1;
+      err1 = "helper thread: hipSetDevice failed";
+      return;
+    }
+    info1 = LA<F>::getrf(s2, n, R.G1, R.ip1, R.inv1);
+    if (info1 == 0 && LA<F>::getrs(s2, n, n, R.G1, R.ip1, R.inv1, R.P)) rc1 = -1;  // P = G1^-1 H1
+    if (info1 || rc1) err1 = bq_last_error();
+    if (hipEventRecord(ev[1], s2) != hipSuccess && !rc1) {
+      rc1 = -1;
+      err1 = "helper thread: event record failed";
+    }
+  });
   int info = LA<F>::getrf(s, n, R.G0, R.ip0, R.inv0);
+  int rc0 = (info == 0) ? LA<F>::getrs(s, n, n, R.G0, R.ip0, R.inv0, R.Mx) : 0;  // M = G0^-1 H0
+  helper.join();
   if (info) {
     bq::set_error("single-object reduction: G of the open region is singular or not finite (info %d)", info);
+    (void)hipStreamSynchronize(s2);
     return info > 0 ? -3 : info;
   }
-  info = LA<F>::getrf(s, n, R.G1, R.ip1, R.inv1);
-  if (info) {
-    bq::set_error("single-object reduction: G of the object is singular or not finite (info %d)", info);
-    return info > 0 ? -3 : info;
+  if (rc0) {
+    (void)hipStreamSynchronize(s2);
+    return -1;
   }
-  if (LA<F>::getrs(s, n, n, R.G0, R.ip0, R.inv0, R.Mx)) return -1;  // M = G0^-1 H0
-  if (LA<F>::getrs(s, n, n, R.G1, R.ip1, R.inv1, R.P)) return -1;   // P = G1^-1 H1
+  if (info1) {
+    bq::set_error("single-object reduction: G of the object is singular or not finite (info %d)", info1);
+    (void)hipStreamSynchronize(s2);
+    return info1 > 0 ? -3 : info1;
+  }
+  if (rc1) {
+    bq::set_error("%s", err1.c_str());
+    (void)hipStreamSynchronize(s2);
+    return -1;
+  }
+  BQ_HIP(hipStreamWaitEvent(s, ev[1], 0));
   k_add<V><<<nblk(nn, 256), 256, 0, s>>>(nn, (const V*)R.P, (const V*)R.Mx, (V*)R.B);
   BQ_HIP(hipGetLastError());
   info = LA<F>::getrf(s, n, R.B, R.ipB, R.invB);

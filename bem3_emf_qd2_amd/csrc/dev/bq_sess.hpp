@@ -43,6 +43,7 @@ struct bq_sess {
   bq_model m;
   int Ne, NN, MN, ps;
   hipStream_t s;
+  hipStream_t s2 = nullptr;  // second stream: the object's factorization beside the open region's
   bool own_stream = false;
   double *E = nullptr, *ren = nullptr, *nrm = nullptr, *Ui = nullptr, *dUi = nullptr, *kn = nullptr;
   int *eni = nullptr, *md = nullptr, *sd = nullptr;

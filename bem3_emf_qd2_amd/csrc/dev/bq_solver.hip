@@ -496,7 +496,15 @@ void bq_sess_destroy(bq_sess* S)
   for (auto p : S->dG) f(p);
   for (auto p : S->dH) f(p);
   for (auto p : S->tdF) f(p);
-  if (S->own_stream) (void)hipStreamDestroy(S->s);
+  if (S->s2) {
+    (void)hipStreamSynchronize(S->s2);
+    bq_lu_release_stream(S->s2);
+    (void)hipStreamDestroy(S->s2);
+  }
+  if (S->own_stream) {
+    bq_lu_release_stream(S->s);
+    (void)hipStreamDestroy(S->s);
+  }
   delete S;
 }
 
