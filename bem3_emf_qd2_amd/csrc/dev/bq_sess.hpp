@@ -68,6 +68,7 @@ struct bq_sess {
   size_t work_bytes = 0;
   bq_reduced red;
   size_t bytes = 0;          // device bytes of the matrices and work buffers
+  size_t tmp_bytes = 0;      // peak of the temporaries a block-system solve allocated
 };
 
 namespace bq {

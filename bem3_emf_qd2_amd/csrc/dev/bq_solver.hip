@@ -477,7 +477,7 @@ int bq_sess_update(bq_sess* S, const bq_model* m)
 
 int bq_sess_assembled(const bq_sess* S) { return (S->assembled && S->key_now == S->key_asm) ? 1 : 0; }
 
-size_t bq_sess_bytes(const bq_sess* S) { return S->bytes + S->red.bytes; }
+size_t bq_sess_bytes(const bq_sess* S) { return S->bytes + S->red.bytes + S->tmp_bytes; }
 
 void bq_sess_destroy(bq_sess* S)
 {
@@ -890,20 +890,44 @@ int bq_sess_solve_global(bq_sess* S, double* Xh)
   }
   S->st.path = 0;
   S->st.t_factor_ms = 0.0;
-  hipEvent_t e0, e1;
-  BQ_HIP(hipEventCreate(&e0));
-  BQ_HIP(hipEventCreate(&e1));
+  // the block system's buffers live for this call only; every return releases them
+  struct Held {
+    std::vector<void*> p;
+    hipEvent_t e[2] = {nullptr, nullptr};
+    ~Held()
+    {
+      for (void* q : p) (void)hipFree(q);
+      for (hipEvent_t x : e)
+        if (x) (void)hipEventDestroy(x);
+    }
+  } held;
+  auto dm = [&](auto** p, size_t n) {
+    if (dmalloc(p, n)) return true;
+    held.p.push_back((void*)*p);
+    return false;
+  };
+  BQ_HIP(hipEventCreate(&held.e[0]));
+  BQ_HIP(hipEventCreate(&held.e[1]));
+  hipEvent_t e0 = held.e[0], e1 = held.e[1];
   BQ_HIP(hipEventRecord(e0, s));
   const size_t sq = (size_t)n2 * n2 * 2, rect = (size_t)n2 * NN * 2;
   double *M, *M3, *R, *L, *b3, *W, *v, *T, *inv1, *inv2;
   int *ip1, *ip2;
   double* nn3 = nullptr;
-  if (dmalloc(&M, sq) || dmalloc(&M3, sq) || dmalloc(&R, (size_t)n2 * (NN + 3) * 2) || dmalloc(&L, rect) ||
-      dmalloc(&b3, (size_t)n2 * 2) || dmalloc(&W, (size_t)NN * NN * 2) || dmalloc(&v, (size_t)NN * 2) ||
-      dmalloc(&T, (size_t)NN * 6) || dmalloc(&ip1, (size_t)n2) || dmalloc(&ip2, (size_t)n2) ||
-      dmalloc(&nn3, (size_t)NN * 3) || dmalloc(&inv1, (size_t)bq_lu_inv_doubles(n2)) ||
-      dmalloc(&inv2, (size_t)bq_lu_inv_doubles(n2)))
+  const size_t ninv = (size_t)bq_lu_inv_doubles(n2);
+  if (dm(&M, sq) || dm(&M3, sq) || dm(&R, (size_t)n2 * (NN + 3) * 2) || dm(&L, rect) || dm(&b3, (size_t)n2 * 2) ||
+      dm(&W, (size_t)NN * NN * 2) || dm(&v, (size_t)NN * 2) || dm(&T, (size_t)NN * 6) || dm(&ip1, (size_t)n2) ||
+      dm(&ip2, (size_t)n2) || dm(&nn3, (size_t)NN * 3) || dm(&inv1, ninv) || dm(&inv2, ninv))
     return -1;
+  {
+    // peak of this call's temporaries (with the FP32 copies of the mixed solve), reported
+    // with the session's resident bytes so that callers admitting sessions see the peak
+    size_t t = 8 * (2 * sq + (size_t)n2 * (NN + 3) * 2 + rect + (size_t)n2 * 2 + (size_t)NN * NN * 2 +
+                    (size_t)NN * 11 + 2 * ninv) + 8 * (size_t)n2;
+    const int prec = bq_solve_precision(-1);
+    if (prec == 1 || (prec == 2 && n2 >= kMixedMinOrder)) t += 4 * (2 * sq + 2 * rect + (size_t)NN * NN * 2);
+    S->tmp_bytes = std::max(S->tmp_bytes, t);
+  }
   // unit normals per element node e (main side)
   {
     std::vector<double> h((size_t)NN * 3, 0.0);
@@ -973,11 +997,6 @@ int bq_sess_solve_global(bq_sess* S, double* Xh)
       S->st.t_solve_ms = ms;
       S->st.mixed = 1;
       unpack_global(NN, Y, Xh);
-      for (void* p : {(void*)M, (void*)M3, (void*)R, (void*)L, (void*)b3, (void*)W, (void*)v, (void*)T, (void*)ip1,
-                      (void*)ip2, (void*)nn3, (void*)inv1, (void*)inv2})
-        (void)hipFree(p);
-      (void)hipEventDestroy(e0);
-      (void)hipEventDestroy(e1);
       return 0;
     }
     // rc > 0: the refinement did not converge; the FP64 factorization below solves it
@@ -1026,11 +1045,6 @@ int bq_sess_solve_global(bq_sess* S, double* Xh)
   BQ_HIP(hipEventElapsedTime(&ms, e0, e1));
   S->st.t_solve_ms = ms;
   unpack_global(NN, Y, Xh);
-  for (void* p : {(void*)M, (void*)M3, (void*)R, (void*)L, (void*)b3, (void*)W, (void*)v, (void*)T, (void*)ip1,
-                  (void*)ip2, (void*)nn3, (void*)inv1, (void*)inv2})
-    (void)hipFree(p);
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   return 0;
 }
 

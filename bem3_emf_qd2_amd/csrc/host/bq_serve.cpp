@@ -104,8 +104,16 @@ double estimate_bytes(const Mesh& m, int precision)
     work = std::max(work, 29.0 * N * m.dNe[d]);
   }
   const double n = m.NN;
-  // six n x n factors and work matrices, and the FGMRES and preconditioner vectors
-  b += work + 6.0 * (precision == 1 ? 8.0 : 16.0) * n * n + 8000.0 * n;
+  if (m.MN == 1) {
+    // single object: six n x n factors and work matrices, and the FGMRES and preconditioner
+    // vectors
+    b += work + 6.0 * (precision == 1 ? 8.0 : 16.0) * n * n + 8000.0 * n;
+  } else {
+    // block elimination of order 2n: M, M3 (2n x 2n), R, L (2n x n), W (n x n) in FP64, and
+    // their FP32 copies in the mixed solve
+    const double n2 = 2.0 * n, fp64 = 16.0 * (2.0 * n2 * n2 + 2.0 * n2 * n + n * n);
+    b += work + fp64 + (precision != 0 ? 0.5 * fp64 : 0.0) + 8000.0 * n;
+  }
   return b;
 }
 
@@ -279,6 +287,7 @@ int bq_serve_run(void* h, int nreq, const bq_serve_req* req, int nprobe, const d
       }
       o.status = rc;
       o.t_done_s = now_s(t0);
+      std::string why = rc ? bq_last_error() : "";  // this thread's error record
       const double used = (double)bq_solver_bytes(I->qd);
       lk.lock();
       out[id] = o;
@@ -298,8 +307,9 @@ int bq_serve_run(void* h, int nreq, const bq_serve_req* req, int nprobe, const d
       }
       if (rc != 0) ++failed;
       if (log)
-        fprintf(stderr, "bq_serve: request %d (mesh %d) on slot %d: %s, start %.2f s, done %.2f s%s\n", id, r.mesh,
-                w, rc ? "failed" : "ok", o.t_start_s, o.t_done_s, o.reused ? ", factors reused" : "");
+        fprintf(stderr, "bq_serve: request %d (mesh %d) on slot %d: %s, start %.2f s, done %.2f s%s%s%s\n", id,
+                r.mesh, w, rc ? "failed" : "ok", o.t_start_s, o.t_done_s, o.reused ? ", factors reused" : "",
+                rc ? ": " : "", why.c_str());
       I->busy = false;
       I->last_use = o.t_done_s;
       ++finished;

@@ -55,6 +55,9 @@ FLOPS_PER_EVAL["c2s"] = FLOPS_PER_EVAL["c2"]
 FLOPS_PER_EVAL["c4"] = {"k_q0_gl": 62737}
 FLOPS_PER_EVAL["c4_tuned"] = {"k_q0_gl": 48457}
 FLOPS_PER_EVAL["c4s"], FLOPS_PER_EVAL["c4s_tuned"] = FLOPS_PER_EVAL["c4"], FLOPS_PER_EVAL["c4_tuned"]
+# the 1536-quad two-layer sweep has the open region, lattice, wavelength and direction ranges of
+# the 6144-quad sweep, which fix the per-evaluation cost at the reference's cut
+FLOPS_PER_EVAL["c3"] = FLOPS_PER_EVAL["c4"]
 # HBM bytes per evaluation of k_q0_gl: (FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00) KiB x 1024 of one
 # launch over its evaluations (factors from a 1 GiB copy read with the same counters)
 TRAFFIC_PER_EVAL = {"c1": {"k_q0_gl": (1163230.7 * 2.0 + 9545073.7) * 1024 / 46912896},
@@ -77,6 +80,15 @@ WORKLOADS = {
     # cone triangles), same lattice, wavelength, materials and incidence: used to read the
     # QPGF kernels' per-evaluation counters, which depend on the lattice and the spread of
     # source-target offsets, not on the element count
+    # the batched spectral sweep: two-layer 1536-quad sphere, 64 seeded points, FP64 assembly
+    # and solves at the reference's accuracy (a lattice-sum cut tuned at tolerance 1e-14 moves
+    # the probe fields by up to 1.9e-9 on this model, beyond the FP64 path's 1e-10 parity)
+    "c3": dict(kind="sweep", n=16, radius=0.45, n_sphere=1.45, core=(0.3, 1.6 + 0.01j), n0=1.0, d1=(1.0, 0.0),
+               d2=(0.0, 1.0), points=64, seed=3, probes=100, steps=12, warmup=1, precision="fp64", asm_tol=0.0,
+               label="C3: two-layer cube-sphere (shell r=0.45um n=1.45, core r=0.3um n=1.6+0.01i, 6x16x16 = "
+                     "1536 quads per surface) in n0=1.0, square lattice 1.0um; sweep of 64 points from "
+                     "default_rng(3): lambda0 U[0.4,0.8]um, theta U[0,40]deg, phi U[0,90]deg, TE/TM; FP64 "
+                     "assembly and solve; 100 probes (EH_t type 1)"),
     # the 6144-quad sweep: one resident model, 16 seeded incidence points, FP64 and mixed
     "c4": dict(kind="sweep", n=32, radius=0.45, n_sphere=1.5 + 0.001j, n0=1.0, d1=(1.0, 0.0), d2=(0.0, 1.0),
                points=16, seed=4, probes=100, steps=6, warmup=1,
@@ -148,7 +160,7 @@ def _scene(w, d):
     if w["kind"] == "sweep":
         lam, th, ph, pol = sweep_points(w)[0]
         meshgen.sphere_scene(d, w["n"], w["radius"], w["n_sphere"], lam, n0=w["n0"], theta=th, phi=ph, pol=pol,
-                             d1=w["d1"], d2=w["d2"])
+                             d1=w["d1"], d2=w["d2"], core=w.get("core"))
     elif w["kind"] == "sphere_cone":
         meshgen.sphere_cone_scene(d, n=w["n"], cone_h=w.get("cone_h", 0.03), lam=w["lam"], n0=w["n0"],
                                   theta=w["theta"], phi=w["phi"], pol=w["pol"], d1=w["d1"], d2=w["d2"])
@@ -338,13 +350,14 @@ def run_sweep(args, w, bq, L):
             bq.solve_precision("fp64")
             bq.assembly_tolerance(0.0)
         batched = None
-        if args.slots > 0 and args.precision == "mixed" and (not args.passes or "batched" in args.passes):
+        if args.slots > 0 and args.precision in ("fp64", "mixed") and (not args.passes or "batched" in args.passes):
             # the batched path: the request loop with args.slots worker slots, every timed point
             # queued at once; one warm-up run first so that each slot holds its session
             from bem3_emf_qd2_amd.serve import Engine
             m.release_device()
             bq.assembly_tolerance(args.asm_tol)
-            eng = Engine([Path(td) / "setup.dat"], nslots=args.slots, mem_limit_gb=args.mem_gb, precision="mixed")
+            eng = Engine([Path(td) / "setup.dat"], nslots=args.slots, mem_limit_gb=args.mem_gb,
+                         precision=args.precision)
             as_req = lambda ps: [dict(mesh=0, lambda0=p[0], theta=p[1], phi=p[2], pol=p[3], arrival_s=0.0) for p in ps]
             wpts = (warm + pts[args.warmup + args.steps:])[:args.slots] or warm
             _, _, _, rcw = eng.run(as_req(wpts), probes)
@@ -422,6 +435,7 @@ def run_sweep(args, w, bq, L):
             cpu = _cpu_baseline_pairs(Path(td) / "point.dat")
         nbytes = m.device_bytes()
         m.close()
+    prec_head = args.precision if batched is not None else serial
     if batched is not None:
         total = batched["makespan"]
         lat = np.array([o["t_done_s"] for o in batched["outs"]]) * 1e3
@@ -443,12 +457,12 @@ def run_sweep(args, w, bq, L):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "c64 factors + f64 (assembly, GMRES residual)" if head != "fp64" else "f64",
+        "dtype": "c64 factors + f64 (assembly, GMRES residual)" if prec_head == "mixed" else "f64",
         "data": "synthetic (generated cube-sphere mesh and seeded plane-wave sweep)",
         "config": {"workload": w["label"], "quads": quads, "triangles": tris, "element_nodes": NN,
                    "na": 8 * NN, "probes": w["probes"], "sweep_points_timed": args.steps,
                    "parallelism": "single GPU",
-                   "solve_precision": "mixed" if head != "fp64" else "fp64",
+                   "solve_precision": prec_head,
                    "assembly_tolerance": args.asm_tol if head != "fp64" else 0.0,
                    "path": (f"batched: request loop, {args.slots} slots, phase-gated (one assembly and one "
                             f"factorisation at a time), all points queued at t=0; latency = time to completion"
@@ -634,10 +648,10 @@ def main():
                     help="sweep workloads: worker slots of the batched mixed-precision pass (request loop); "
                          "0 skips it and reports the serial mixed pass")
     ap.add_argument("--mem-gb", type=float, default=270.0, help="device memory limit of the request loop (GiB)")
-    ap.add_argument("--asm-tol", type=float, default=1e-12,
-                    help="assembly tolerance of the mixed-precision passes (bq_assembly_tolerance: Gauss-order "
-                         "bound and per-incidence lattice-sum cut); 0 keeps the reference's accuracy.  The FP64 "
-                         "pass always runs at the reference's accuracy")
+    ap.add_argument("--asm-tol", type=float, default=None,
+                    help="assembly tolerance of the mixed-precision and batched passes (bq_assembly_tolerance: "
+                         "Gauss-order bound and per-incidence lattice-sum cut; default 1e-12, c3 1e-14); 0 keeps "
+                         "the reference's accuracy.  The serial FP64 pass always runs at the reference's accuracy")
     ap.add_argument("--passes", default=None,
                     help="sweep workloads: comma-separated solve precisions to run in order (default "
                          "'fp64,mixed': the FP64 pass gives fp64_value and the reference results); "
@@ -675,7 +689,9 @@ def main():
     if args.warmup is None:
         args.warmup = w["warmup"]
     if args.precision is None:
-        args.precision = "mixed" if w["kind"] == "sweep" else "fp64"
+        args.precision = w.get("precision", "mixed" if w["kind"] == "sweep" else "fp64")
+    if args.asm_tol is None:
+        args.asm_tol = w.get("asm_tol", 1e-12)
     if w["kind"] == "sweep":
         run_sweep(args, w, bq, L)
         return

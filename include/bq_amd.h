@@ -122,7 +122,9 @@ int bq_solve_bieq_rc(void *qd);
 /* Device memory of a model's resident solver session is released by finalize_dqd2 or here
  * (no counterpart in the reference; for callers that keep many models). */
 void bq_release_solver(void *qd);
-/* Device bytes held by the resident solver session of qd (all sessions if qd is NULL). */
+/* Device bytes held by the resident solver session of qd (all sessions if qd is NULL), plus
+ * the peak of the temporaries its block-system solves allocate (models with more than one
+ * bounded domain): the memory a solve of the model needs at once. */
 long long bq_solver_bytes(void *qd);
 /* Statistics of qd's last solve_bieq (the bq_solve_last_stats layout) into out; -1 if qd
  * has no session. */

@@ -34,9 +34,9 @@ def _rel(a, b):
     return worst
 
 
-def _scene(d, n=3, lam=0.55, theta=0.3, phi=0.4, pol="TM", radius=0.45, n_sphere=1.5 + 0.001j):
+def _scene(d, n=3, lam=0.55, theta=0.3, phi=0.4, pol="TM", radius=0.45, n_sphere=1.5 + 0.001j, core=None):
     from bem3_emf_qd2_amd import meshgen
-    meshgen.sphere_scene(d, n, radius, n_sphere, lam, theta=theta, phi=phi, pol=pol)
+    meshgen.sphere_scene(d, n, radius, n_sphere, lam, theta=theta, phi=phi, pol=pol, core=core)
     return d
 
 
@@ -111,13 +111,17 @@ def test_gpu_reduced_vs_oracle(tmp_path, gpu_lib):
 
 
 @pytest.mark.gpu
-def test_gpu_set_incidence_equals_fresh_model(tmp_path, gpu_lib):
+@pytest.mark.parametrize("layers", [1, 2])
+def test_gpu_set_incidence_equals_fresh_model(tmp_path, gpu_lib, layers):
     """A sweep on one resident model: after a solve at one wavelength and direction,
     bq_set_incidence moves it to another; the solution and probe fields equal those of a model
-    set up from input files written for the second wave (the same arithmetic: 1e-13)."""
+    set up from input files written for the second wave (the same arithmetic: 1e-13).  The
+    single sphere takes the reduction, the two-layer sphere of the batched spectral sweep (shell
+    n 1.45, core r 0.3 um n 1.6 + 0.01i) the block elimination."""
     from bem3_emf_qd2_amd.solver import Model
-    d1 = _scene(tmp_path / "a", lam=0.55, theta=0.3, phi=0.4)
-    d2 = _scene(tmp_path / "b", lam=0.62, theta=0.5, phi=1.1)
+    kw = dict(n_sphere=1.45, core=(0.3, 1.6 + 0.01j)) if layers == 2 else {}
+    d1 = _scene(tmp_path / "a", lam=0.55, theta=0.3, phi=0.4, **kw)
+    d2 = _scene(tmp_path / "b", lam=0.62, theta=0.5, phi=1.1, **kw)
     m = Model.from_dir(d1)
     m.solve()
     m.set_incidence(0.62, 0.5, 1.1)
