@@ -29,6 +29,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import shutil
 import sys
 import tempfile
 import time
@@ -339,6 +340,10 @@ def run_sweep(args, w, bq, L):
                     st, f = step(p)
                     dt = time.perf_counter() - t
                     m.write(Path(td) / "pt.dat")
+                    if args.save_dat and i == 0:
+                        # the first timed point's solved model, for checks on the host
+                        Path(args.save_dat).mkdir(parents=True, exist_ok=True)
+                        shutil.copy(Path(td) / "pt.dat", Path(args.save_dat) / f"{args.workload}_{mode}_point1.dat")
                     res[mode].append(dict(lat=dt, st=st.as_dict(), probes=f, surf=_surface(Path(td) / "pt.dat")))
                     print(f"bench.py: {mode} point {i + 1}/{len(timed)} lambda0={p[0]:.4f} theta="
                           f"{np.degrees(p[1]):.2f} phi={np.degrees(p[2]):.2f} {p[3]}: {dt:.2f} s "
@@ -644,6 +649,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=None, help="untimed steps (default per workload: c2 1, c1 2)")
     ap.add_argument("--workload", default="c4", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--save-dat", default=None, metavar="DIR",
+                    help="sweep workloads: write the first timed point's solved model (ex.dat layout) of each "
+                         "serial pass to DIR")
     ap.add_argument("--slots", type=int, default=2,
                     help="sweep workloads: worker slots of the batched mixed-precision pass (request loop); "
                          "0 skips it and reports the serial mixed pass")
