@@ -32,9 +32,12 @@ int bq_cgetrs_dev(void *stream, int n, int nrhs, const float *A, long lda, const
 int bq_cgemm_op_dev(void *stream, int ta, int tb, int m, int n, int k, const float *alpha, const float *A,
                     long lda, const float *B, long ldb, const float *beta, float *C, long ldc);
 int bq_cgesv_host(int n, int nrhs, float *A, float *B);
-/* C -= A B in complex FP32 with bf16x3 products on the matrix cores (bq_mfma.hip). */
+/* C -= A B in complex FP32 with bf16x3 products on the matrix cores (bq_mfma.hip); the
+ * _store variant writes C = -A B and allows B == C (in place). */
 int bq_cgemm_split_dev(void *stream, int m, int n, int k, const float *A, long lda, const float *B, long ldb,
                        float *C, long ldc);
+int bq_cgemm_split_store_dev(void *stream, int m, int n, int k, const float *A, long lda, const float *B, long ldb,
+                             float *C, long ldc);
 /* 1: FP32 factorization updates on the bf16x3 kernel, 0: rocBLAS cgemm; < 0 queries.
  * Returns the previous mode. */
 int bq_lu_update_mode(int mode);

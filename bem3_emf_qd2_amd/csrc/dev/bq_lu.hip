@@ -95,6 +95,14 @@ struct P<float> {
                          (const rocblas_float_complex*)B, ldb, (const rocblas_float_complex*)be,
                          (rocblas_float_complex*)Cm, ldc);
   }
+  static rocblas_status gemm_sb(rocblas_handle h, int m, int n, int k, const V* al, const V* A, int lda, long sa,
+                                const V* B, int ldb, long sb, const V* be, V* Cm, int ldc, long sc, int batch)
+  {
+    return rocblas_cgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, m, n, k,
+                                         (const rocblas_float_complex*)al, (const rocblas_float_complex*)A, lda, sa,
+                                         (const rocblas_float_complex*)B, ldb, sb, (const rocblas_float_complex*)be,
+                                         (rocblas_float_complex*)Cm, ldc, sc, batch);
+  }
   static constexpr const char* kt_f = "cgetrf";
   static constexpr const char* kt_p = "cgetrf_panel";
   static constexpr const char* kt_u = "cgetrf_update";
@@ -360,6 +368,47 @@ __global__ void __launch_bounds__(256) k_apply_blk(const typename P<R>::V* __res
     for (int k = 0; k < kNB; ++k) acc = acc + T::c(Ts[k * kNB + r]) * T::c(Bs[c * kNB + k]);
     B[(long)(col0 + c) * ldb + r] = T::v(acc);
   }
+}
+
+// k_apply_blk on the diagonal blocks of several column panels at once: block y applies
+// Tm + y * tstride to B + y * bstride (same jb, ldb and ncol for all)
+template <class R>
+__global__ void __launch_bounds__(256) k_apply_blk_batched(const typename P<R>::V* __restrict__ Tm, long tstride,
+                                                           int jb, typename P<R>::V* __restrict__ B, long bstride,
+                                                           long ldb, int ncol)
+{
+  using V = typename P<R>::V;
+  using C = typename P<R>::C;
+  using T = P<R>;
+  __shared__ V Ts[kNB * kNB];
+  __shared__ V Bs[kNB * 64];
+  const int tid = threadIdx.x, col0 = blockIdx.x * 64;
+  const V* tm = Tm + blockIdx.y * tstride;
+  V* b = B + blockIdx.y * bstride;
+  for (int q = tid; q < kNB * kNB; q += 256) Ts[q] = tm[q];
+  for (int q = tid; q < kNB * 64; q += 256) {
+    const int r = q % kNB, c = q / kNB;
+    Bs[q] = (r < jb && col0 + c < ncol) ? b[(long)(col0 + c) * ldb + r] : T::v(0.0, 0.0);
+  }
+  __syncthreads();
+  for (int q = tid; q < kNB * 64; q += 256) {
+    const int r = q % kNB, c = q / kNB;
+    if (r >= jb || col0 + c >= ncol) continue;
+    C acc = T::mkc(0, 0);
+#pragma unroll 8
+    for (int k = 0; k < kNB; ++k) acc = acc + T::c(Ts[k * kNB + r]) * T::c(Bs[c * kNB + k]);
+    b[(long)(col0 + c) * ldb + r] = T::v(acc);
+  }
+}
+
+// W[b] = -I for nb square blocks of order m (column-major, contiguous)
+template <class R>
+__global__ void k_neg_eye(long total, int m, typename P<R>::V* __restrict__ W)
+{
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const long e = i % ((long)m * m);
+  W[i] = P<R>::v((e % m) == (e / m) ? -1.0 : 0.0, 0.0);
 }
 
 // the interchanges ipiv[0..n) applied in order to 0..n-1 give the row permutation
@@ -645,6 +694,54 @@ int gemm(hipStream_t s, int m, int n, int k, typename P<R>::V alpha, const typen
   return gemm_op<R>(s, 0, 0, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
 }
 
+// -inv(T_bb) of nfull consecutive diagonal blocks of order kNBF of the factored A: the unit
+// lower factor (upper = false) or the upper one, into W (nfull blocks of kNBF x kNBF,
+// column-major, contiguous).  Built for all blocks at once by the in-block substitution of
+// the solves (32-row steps with the inverted 32 x 32 diagonal blocks), starting from -I:
+// 16 batched launches of each kind in all.
+template <class R>
+int block_inverses(hipStream_t s, int nfull, const typename P<R>::V* A, long lda, const typename P<R>::V* inv,
+                   bool upper, typename P<R>::V* W)
+{
+  using V = typename P<R>::V;
+  const V one = P<R>::v(1.0, 0.0), mone = P<R>::v(-1.0, 0.0);
+  const long bs = (long)kNBF * kNBF, tot = bs * nfull;
+  k_neg_eye<R><<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(tot, kNBF, W);
+  BQ_HIP(hipGetLastError());
+  rocblas_handle h = handle_for(s);
+  if (!h) {
+    bq::set_error("rocblas handle creation failed");
+    return -1;
+  }
+  const int steps = kNBF / kNB;
+  const dim3 g(kNBF / 64, nfull);
+  const long tstride = (long)steps * kBlkElems, astride = (long)kNBF * lda + kNBF;
+  for (int q = 0; q < steps; ++q) {
+    const int t = upper ? steps - 1 - q : q, j = t * kNB;
+    k_apply_blk_batched<R><<<g, 256, 0, s>>>(inv + (long)t * kBlkElems + (upper ? kNB * kNB : 0), tstride, kNB, W + j,
+                                             bs, kNBF, kNBF);
+    BQ_HIP(hipGetLastError());
+    rocblas_status st = rocblas_status_success;
+    if (!upper && kNBF - j - kNB > 0)
+      st = P<R>::gemm_sb(h, kNBF - j - kNB, kNBF, kNB, &mone, A + (long)j * lda + (j + kNB), (int)lda, astride, W + j,
+                         kNBF, bs, &one, W + j + kNB, kNBF, bs, nfull);
+    if (upper && j > 0)
+      st = P<R>::gemm_sb(h, j, kNBF, kNB, &mone, A + (long)j * lda, (int)lda, astride, W + j, kNBF, bs, &one, W, kNBF,
+                         bs, nfull);
+    if (st != rocblas_status_success) {
+      bq::set_error("rocblas strided batched gemm failed (%d)", (int)st);
+      return -1;
+    }
+  }
+  return 0;
+}
+
+// X (kNBF x nc, ldx) <- inv(T_bb) X = -Wneg X in place on the matrix cores (Wneg = -inv(T_bb))
+inline int apply_block_inverse(hipStream_t s, int nc, const float2* Wneg, float2* X, long ldx)
+{
+  return bq_cgemm_split_store_dev(s, kNBF, nc, kNBF, (const float*)Wneg, kNBF, (const float*)X, ldx, (float*)X, ldx);
+}
+
 // In-place LU of the n x n column-major matrix A (device); inv receives the inverted
 // diagonal blocks.  Returns 0, a positive k if U(k,k) is exactly zero (1-based), or < 0.
 template <class R>
@@ -657,6 +754,16 @@ int getrf(hipStream_t s, int n, typename P<R>::V* A, long lda, int* ipiv, typena
   V* F = nullptr;
   BQ_HIP(hipMallocAsync((void**)&cand, sizeof(int) * (size_t)2 * nleaf * kNB, s));
   BQ_HIP(hipMallocAsync((void**)&F, sizeof(V) * kNB * kNB, s));
+  // FP32 with the matrix-core updates: U12 of each full panel by one product with the
+  // panel's inverted L11 (Wp = -inv(L11))
+  bool blk_inv = false;
+  V* Wp = nullptr;
+  if constexpr (std::is_same<R, float>::value) {
+    if (update_mode() == 1 && n >= 2 * kNBF) {
+      blk_inv = true;
+      BQ_HIP(hipMallocAsync((void**)&Wp, sizeof(V) * kNBF * kNBF, s));
+    }
+  }
   const int kt_all = bq::ktime_begin(P<R>::kt_f, s);
   for (int j0 = 0; j0 < n; j0 += kNBF) {
     const int jb2 = min(kNBF, n - j0), je = j0 + jb2;
@@ -695,8 +802,17 @@ int getrf(hipStream_t s, int n, typename P<R>::V* A, long lda, int* ipiv, typena
     const int right = n - je;
     if (right > 0) {
       k_swap<R><<<(right + 255) / 256, 256, 0, s>>>(A, lda, je, right, j0, jb2, ipiv);
+      bool done = false;
+      if constexpr (std::is_same<R, float>::value) {
+        if (blk_inv && jb2 == kNBF && right >= kSplitMin) {
+          if (block_inverses<R>(s, 1, A + (long)j0 * lda + j0, lda, inv + (long)(j0 / kNB) * kBlkElems, false, Wp) ||
+              apply_block_inverse(s, right, Wp, A + (long)je * lda + j0, lda))
+            return -1;
+          done = true;
+        }
+      }
       // U12 of the block row: unit-lower solve with the panel's L11, 32 rows at a time
-      for (int j = j0; j < je; j += kNB) {
+      for (int j = j0; j < je && !done; j += kNB) {
         const int jb = min(kNB, je - j);
         k_apply_blk<R><<<(right + 63) / 64, 256, 0, s>>>(inv + (long)(j / kNB) * kBlkElems, jb,
                                                          A + (long)je * lda + j, lda, right);
@@ -721,6 +837,7 @@ int getrf(hipStream_t s, int n, typename P<R>::V* A, long lda, int* ipiv, typena
   bq::ktime_end(kt_all, s);
   BQ_HIP(hipFreeAsync(cand, s));
   BQ_HIP(hipFreeAsync(F, s));
+  if (Wp) BQ_HIP(hipFreeAsync(Wp, s));
   int* dinfo = nullptr;
   BQ_HIP(hipMallocAsync((void**)&dinfo, sizeof(int), s));
   int big = 0x7fffffff;
@@ -782,6 +899,21 @@ int getrs(hipStream_t s, int n, int nrhs, const typename P<R>::V* A, long lda, c
     return 0;
   }
   const unsigned cblk = (unsigned)((nrhs + 63) / 64);
+  // FP32 with the matrix-core updates and many right-hand sides: the full 512-row diagonal
+  // blocks are inverted once (all at a time) and applied as products on the matrix cores
+  const int nfull = n / kNBF;
+  V *WL = nullptr, *WU = nullptr;
+  bool blk_inv = false;
+  if constexpr (std::is_same<R, float>::value) {
+    if (update_mode() == 1 && nrhs >= kSplitMin && nfull > 0) {
+      blk_inv = true;
+      const size_t bs = (size_t)kNBF * kNBF;
+      BQ_HIP(hipMallocAsync((void**)&WL, sizeof(V) * bs * nfull, s));
+      BQ_HIP(hipMallocAsync((void**)&WU, sizeof(V) * bs * nfull, s));
+      if (block_inverses<R>(s, nfull, A, lda, inv, false, WL) || block_inverses<R>(s, nfull, A, lda, inv, true, WU))
+        return -1;
+    }
+  }
   // forward: unit lower L, 512-row blocks of 32-row steps.  With column groups, group g
   // (columns g0[g] .. g0[g] + gn[g]) starts at the block holding row gf[g]: the caller states
   // that its permuted right-hand sides are zero above that row, so are its solutions there.
@@ -794,7 +926,14 @@ int getrs(hipStream_t s, int n, int nrhs, const typename P<R>::V* A, long lda, c
     const unsigned gblk = (unsigned)((nc + 63) / 64);
     for (int j0 = first; j0 < n; j0 += kNBF) {
       const int je = min(n, j0 + kNBF);
-      for (int j = j0; j < je; j += kNB) {
+      bool done = false;
+      if constexpr (std::is_same<R, float>::value) {
+        if (blk_inv && je - j0 == kNBF && nc >= kSplitMin) {
+          if (apply_block_inverse(s, nc, WL + (size_t)(j0 / kNBF) * kNBF * kNBF, Bg + j0, ldb)) return -1;
+          done = true;
+        }
+      }
+      for (int j = j0; j < je && !done; j += kNB) {
         const int jb = min(kNB, je - j);
         k_apply_blk<R><<<gblk, 256, 0, s>>>(inv + (long)(j / kNB) * kBlkElems, jb, Bg + j, ldb, nc);
         BQ_HIP(hipGetLastError());
@@ -810,7 +949,14 @@ int getrs(hipStream_t s, int n, int nrhs, const typename P<R>::V* A, long lda, c
   // backward: U, 512-row blocks from the bottom, 32-row steps upwards inside each
   for (int j0 = ((n - 1) / kNBF) * kNBF; j0 >= 0; j0 -= kNBF) {
     const int je = min(n, j0 + kNBF);
-    for (int j = ((je - 1 - j0) / kNB) * kNB + j0; j >= j0; j -= kNB) {
+    bool done = false;
+    if constexpr (std::is_same<R, float>::value) {
+      if (blk_inv && je - j0 == kNBF) {
+        if (apply_block_inverse(s, nrhs, WU + (size_t)(j0 / kNBF) * kNBF * kNBF, B + j0, ldb)) return -1;
+        done = true;
+      }
+    }
+    for (int j = ((je - 1 - j0) / kNB) * kNB + j0; j >= j0 && !done; j -= kNB) {
       const int jb = min(kNB, je - j);
       k_apply_blk<R><<<cblk, 256, 0, s>>>(inv + (long)(j / kNB) * kBlkElems + kNB * kNB, jb, B + j, ldb, nrhs);
       BQ_HIP(hipGetLastError());
@@ -819,6 +965,8 @@ int getrs(hipStream_t s, int n, int nrhs, const typename P<R>::V* A, long lda, c
     }
     if (j0 > 0 && gemm<R>(s, j0, nrhs, je - j0, mone, A + (long)j0 * lda, lda, B + j0, ldb, one, B, ldb)) return -1;
   }
+  if (WL) BQ_HIP(hipFreeAsync(WL, s));
+  if (WU) BQ_HIP(hipFreeAsync(WU, s));
   bq::ktime_end(kt_all, s);
   return 0;
 }

@@ -124,7 +124,9 @@ __device__ __forceinline__ f32x16 mma(bf16x8 a, bf16x8 b, f32x16 c)
   return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
 }
 
-// C (m x n, ldc) -= A B from the packed operands; grid = tiles_r * tiles_c workgroups of 256.
+// C (m x n, ldc) -= A B from the packed operands (STORE: C = -A B, C not read); grid =
+// tiles_r * tiles_c workgroups of 256.
+template <bool STORE>
 __global__ __launch_bounds__(256, 2) void k_cgemm_split(int m, int n, int tiles_r, int tiles_c, int Kb,
                                                         const uint4* __restrict__ Ap, const uint4* __restrict__ Bp,
                                                         float2* __restrict__ C, long ldc)
@@ -233,7 +235,8 @@ __global__ __launch_bounds__(256, 2) void k_cgemm_split(int m, int n, int tiles_
       // column n - 1 (in bounds) and are not written
       float2 v[16];
 #pragma unroll
-      for (int e = 0; e < 16; ++e) v[e] = cr0[(long)min(c0 + 8 * (e >> 2) + (e & 3), n - 1) * ldc];
+      for (int e = 0; e < 16; ++e)
+        v[e] = STORE ? make_float2(0.f, 0.f) : cr0[(long)min(c0 + 8 * (e >> 2) + (e & 3), n - 1) * ldc];
 #pragma unroll
       for (int e = 0; e < 16; ++e) {
         const int col = c0 + 8 * (e >> 2) + (e & 3);
@@ -284,6 +287,40 @@ void* workspace(hipStream_t s, size_t bytes)
 
 }  // namespace
 
+// C -= A B (store = false) or C = -A B (store = true); the operands are packed before the
+// product is written, so with store = true B may be C itself (C <- -A C in place).
+int split_gemm(hipStream_t s, int m, int n, int k, const float* A, long lda, const float* B, long ldb, float* C,
+               long ldc, bool store)
+{
+  if (m <= 0 || n <= 0) return 0;
+  if (k <= 0 && !store) return 0;
+  const int tiles_r = (m + kTile - 1) / kTile, tiles_c = (n + kTile - 1) / kTile;
+  const int Mb = tiles_r * 4, Nb = tiles_c * 4, Kb = max(1, (k + kKStep - 1) / kKStep);
+  const long na = (long)Kb * Mb * 4 * kTileU4, nb = (long)Kb * Nb * 4 * kTileU4;  // uint4 counts
+  if ((long)tiles_r * tiles_c > 0x7fffffffL) {
+    bq::set_error("bq_cgemm_split_dev: %d x %d tiles exceed one launch", tiles_r, tiles_c);
+    return -1;
+  }
+  uint4* ws = (uint4*)workspace(s, sizeof(uint4) * (size_t)(na + nb));
+  if (!ws) {
+    bq::set_error("bq_cgemm_split_dev: workspace of %.1f MB not allocated", 16.0 * (na + nb) / 1e6);
+    return -1;
+  }
+  uint4 *Ap = ws, *Bp = ws + na;
+  const long ta = (long)Kb * Mb * 64, tb = (long)Kb * Nb * 64;
+  k_pack_a<<<(unsigned)((ta + 255) / 256), 256, 0, s>>>(m, k, (const float2*)A, lda, Mb, Kb, Ap);
+  k_pack_b<<<(unsigned)((tb + 255) / 256), 256, 0, s>>>(k, n, (const float2*)B, ldb, Nb, Kb, Bp);
+  const unsigned grid = (unsigned)(tiles_r * tiles_c);
+  if (store)
+    BQ_KTIME("k_cgemm_split", s,
+             k_cgemm_split<true><<<grid, 256, 0, s>>>(m, n, tiles_r, tiles_c, Kb, Ap, Bp, (float2*)C, ldc));
+  else
+    BQ_KTIME("k_cgemm_split", s,
+             k_cgemm_split<false><<<grid, 256, 0, s>>>(m, n, tiles_r, tiles_c, Kb, Ap, Bp, (float2*)C, ldc));
+  BQ_HIP(hipGetLastError());
+  return 0;
+}
+
 extern "C" {
 
 // Free the packed-operand workspace kept for this stream on the current device (call before
@@ -309,29 +346,15 @@ void bq_cgemm_split_release(void* stream)
 int bq_cgemm_split_dev(void* stream, int m, int n, int k, const float* A, long lda, const float* B, long ldb,
                        float* C, long ldc)
 {
-  if (m <= 0 || n <= 0 || k <= 0) return 0;
-  hipStream_t s = (hipStream_t)stream;
-  const int tiles_r = (m + kTile - 1) / kTile, tiles_c = (n + kTile - 1) / kTile;
-  const int Mb = tiles_r * 4, Nb = tiles_c * 4, Kb = (k + kKStep - 1) / kKStep;
-  const long na = (long)Kb * Mb * 4 * kTileU4, nb = (long)Kb * Nb * 4 * kTileU4;  // uint4 counts
-  if ((long)tiles_r * tiles_c > 0x7fffffffL) {
-    bq::set_error("bq_cgemm_split_dev: %d x %d tiles exceed one launch", tiles_r, tiles_c);
-    return -1;
-  }
-  uint4* ws = (uint4*)workspace(s, sizeof(uint4) * (size_t)(na + nb));
-  if (!ws) {
-    bq::set_error("bq_cgemm_split_dev: workspace of %.1f MB not allocated", 16.0 * (na + nb) / 1e6);
-    return -1;
-  }
-  uint4 *Ap = ws, *Bp = ws + na;
-  const long ta = (long)Kb * Mb * 64, tb = (long)Kb * Nb * 64;
-  k_pack_a<<<(unsigned)((ta + 255) / 256), 256, 0, s>>>(m, k, (const float2*)A, lda, Mb, Kb, Ap);
-  k_pack_b<<<(unsigned)((tb + 255) / 256), 256, 0, s>>>(k, n, (const float2*)B, ldb, Nb, Kb, Bp);
-  BQ_KTIME("k_cgemm_split", s,
-           k_cgemm_split<<<(unsigned)(tiles_r * tiles_c), 256, 0, s>>>(m, n, tiles_r, tiles_c, Kb, Ap, Bp,
-                                                                       (float2*)C, ldc));
-  BQ_HIP(hipGetLastError());
-  return 0;
+  return split_gemm((hipStream_t)stream, m, n, k, A, lda, B, ldb, C, ldc, false);
+}
+
+// C = -A B on the same kernel; B may be C itself (C <- -A C in place: the operands are packed
+// before the product is written).
+int bq_cgemm_split_store_dev(void* stream, int m, int n, int k, const float* A, long lda, const float* B, long ldb,
+                             float* C, long ldc)
+{
+  return split_gemm((hipStream_t)stream, m, n, k, A, lda, B, ldb, C, ldc, true);
 }
 
 }  // extern "C"

@@ -202,6 +202,11 @@ int bq_lu_update_mode(int mode);
  * many-right-hand-side solves.  stream: a hipStream_t or NULL.  Returns 0 or -1. */
 int bq_cgemm_split_dev(void *stream, int m, int n, int k, const float *A, long lda, const float *B, long ldb,
                        float *C, long ldc);
+/* C = -A B on the same kernel, C not read; B may be C itself (in place: C <- -A C), since the
+ * operands are packed before the product is written.  The solves apply inverted 512-row
+ * diagonal blocks of the FP32 factors this way. */
+int bq_cgemm_split_store_dev(void *stream, int m, int n, int k, const float *A, long lda, const float *B, long ldb,
+                             float *C, long ldc);
 
 /* Per-kernel timing (no counterpart in the reference; for benchmarks and profiling):
  * bq_kernel_timing(1) turns on HIP-event timing of every kernel launch the library makes

@@ -233,3 +233,32 @@ def test_gpu_getrs_column_groups(gpu_lib, single):
     print("single" if single else "double", "grouped vs plain", err, "plain vs numpy", fw)
     assert err < (1e-5 if single else 1e-12)
     assert fw < (1e-4 if single else 1e-10)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("m,n", [(512, 3000), (200, 77)])
+def test_gpu_cgemm_split_store_in_place(gpu_lib, m, n):
+    """bq_cgemm_split_store_dev with B == C (C <- -A C in place, as the solves apply their
+    inverted 512-row diagonal blocks): equal to -A C_old summed exactly with bf16x3 products,
+    within the FP32 accumulation (2e-6 of the largest entry); the padding rows of the strided
+    C (leading dimension m + 9) stay untouched."""
+    import ctypes
+    from hipmem import DeviceArray
+    L = gpu_lib.lib()
+    V = ctypes.c_void_p
+    L.bq_cgemm_split_store_dev.argtypes = [V, ctypes.c_int, ctypes.c_int, ctypes.c_int, V, ctypes.c_long, V,
+                                           ctypes.c_long, V, ctypes.c_long]
+    rng = np.random.default_rng(m + n)
+    ldc = m + 9
+    Acm = (rng.standard_normal((m, m)) + 1j * rng.standard_normal((m, m))).astype(np.complex64)
+    Ccm = (rng.standard_normal((n, ldc)) + 1j * rng.standard_normal((n, ldc))).astype(np.complex64)
+    A, C0 = Acm.T, Ccm[:, :m].T
+    dA, dC = DeviceArray(Acm), DeviceArray(Ccm)
+    assert L.bq_cgemm_split_store_dev(None, m, n, m, dA.ptr, m, dC.ptr, ldc, dC.ptr, ldc) == 0
+    out = dC.get()
+    got = out[:, :m].T.astype(complex)
+    D = _bf16x3_product(A.astype(complex), C0.astype(complex))
+    err = np.abs(got + D).max() / np.abs(D).max()
+    print(m, n, "in place vs -bf16x3 sum", err)
+    assert np.array_equal(out[:, m:], Ccm[:, m:])
+    assert err < 2e-6
