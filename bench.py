@@ -64,6 +64,9 @@ FLOPS_PER_EVAL["c3"] = FLOPS_PER_EVAL["c4"]
 TRAFFIC_PER_EVAL = {"c1": {"k_q0_gl": (1163230.7 * 2.0 + 9545073.7) * 1024 / 46912896},
                     "c2": {"k_q0_gl": (12430181.2 * 2.0 + 72190715.0) * 1024 / 385592752}}
 TRAFFIC_PER_EVAL["c2s"] = TRAFFIC_PER_EVAL["c2"]
+# FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00 per evaluation on c4s at the reference's cut
+# (profiles/k_q0_gl_c4s_pmc.md); the 1536-quad two-layer sweep shares the open region's physics
+TRAFFIC_PER_EVAL["c4"] = TRAFFIC_PER_EVAL["c3"] = TRAFFIC_PER_EVAL["c4s"] = {"k_q0_gl": 316.0}
 
 WORKLOADS = {
     "c1": dict(kind="sphere", n=8, radius=0.4, n_sphere=1.5, lam=0.6, n0=1.0, theta=1e-6, phi=0.0, pol="TM",
@@ -235,6 +238,57 @@ def _roofline(ktimes, stats_per_step, steps, workload):
             r["frac"] = ach / FP64_PEAK_TFS
             r["flops_per_eval"] = fpe
     return r
+
+
+def gemm_probe():
+    """k_cgemm_split (C -= A B, complex FP32 operands, bf16x3 products) against rocBLAS cgemm at the
+    update shapes of the 6144-quad factorisation: kernel time from the library's HIP events."""
+    import ctypes
+    sys.path.insert(0, str(ROOT / "tests"))
+    import bem3_emf_qd2_amd as bq
+    from hipmem import DeviceArray, _rt
+    L = bq.lib()
+    if not bq.device_ok():
+        raise SystemExit(f"bench.py: no usable gfx950 device: {L.bq_last_error().decode()}")
+    V = ctypes.c_void_p
+    L.bq_cgemm_split_dev.argtypes = [V, ctypes.c_int, ctypes.c_int, ctypes.c_int, V, ctypes.c_long, V, ctypes.c_long,
+                                     V, ctypes.c_long]
+    L.bq_cgemm_op_dev.argtypes = [V, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, V, V,
+                                  ctypes.c_long, V, ctypes.c_long, V, V, ctypes.c_long]
+    rt = _rt()
+    al = (ctypes.c_float * 2)(-1.0, 0.0)
+    be = (ctypes.c_float * 2)(1.0, 0.0)
+    for (m, n, k) in [(24576, 24576, 512), (16384, 16384, 512), (8192, 8192, 512), (24576, 24576, 256)]:
+        rng = np.random.default_rng(1)
+        A = DeviceArray((rng.standard_normal((k, m)) + 1j * rng.standard_normal((k, m))).astype(np.complex64))
+        B = DeviceArray((rng.standard_normal((n, k)) + 1j * rng.standard_normal((n, k))).astype(np.complex64))
+        C = DeviceArray(np.zeros((n, m), np.complex64))
+        L.bq_cgemm_split_dev(None, m, n, k, A.ptr, m, B.ptr, k, C.ptr, m)
+        rt.hipDeviceSynchronize()
+        reps = 5
+        L.bq_kernel_timing(1)
+        t = time.perf_counter()
+        for _ in range(reps):
+            if L.bq_cgemm_split_dev(None, m, n, k, A.ptr, m, B.ptr, k, C.ptr, m):
+                raise SystemExit(L.bq_last_error().decode())
+        rt.hipDeviceSynchronize()
+        wall = (time.perf_counter() - t) / reps
+        kt = _kernel_times(L)
+        L.bq_kernel_timing(0)
+        kms = kt["k_cgemm_split"][0] / kt["k_cgemm_split"][1]
+        L.bq_cgemm_op_dev(None, 0, 0, m, n, k, al, A.ptr, m, B.ptr, k, be, C.ptr, m)
+        rt.hipDeviceSynchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            L.bq_cgemm_op_dev(None, 0, 0, m, n, k, al, A.ptr, m, B.ptr, k, be, C.ptr, m)
+        rt.hipDeviceSynchronize()
+        rb = (time.perf_counter() - t) / reps
+        fl = 8.0 * m * n * k
+        print(json.dumps(dict(m=m, n=n, k=k, split_kernel_ms=kms, split_wall_ms=wall * 1e3, split_tfs=fl / kms / 1e9,
+                              split_bf16_frac=3 * fl / kms / 1e9 / 2500.0, rocblas_ms=rb * 1e3,
+                              rocblas_tfs=fl / rb / 1e12)), flush=True)
+        for d in (A, B, C):
+            d.free()
 
 
 def _cpu_baseline(model_dat, w, probes, gpu_solve_dims):
@@ -669,6 +723,9 @@ def main():
                          "configurations are the plain FP64 path), complex FP32 LU + FP64 iterative "
                          "refinement ('mixed'; one untimed FP64 step first gives rel_err_vs_fp64), or "
                          "'auto': mixed for block systems of order >= 16384")
+    ap.add_argument("--gemm-probe", action="store_true",
+                    help="only time the FP32 factorisation's update kernel (bf16x3 matrix cores) against rocBLAS "
+                         "cgemm at the C4 update shapes, one JSON line per shape")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run one device-to-device copy of MIB MiB (calibrates FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -680,6 +737,9 @@ def main():
         y.copy_(x)
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes": x.numel() * 4}))
+        return
+    if args.gemm_probe:
+        gemm_probe()
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
