@@ -17,7 +17,11 @@
  * terms below eps*EFS of the sum -- including the reference's early stop of the _v1/_dqG
  * spectral sum when r_z == 0 exactly, whose stop test then compares zero with zero.
  */
+#include <dlfcn.h>
 #include <math.h>
+#include <pthread.h>
+#include <stdio.h>
+#include <stdlib.h>
 #include <complex.h>
 #include "oracle.h"
 
@@ -372,8 +376,37 @@ static int use_fourier(const double *r, double eps, const or_qpd2 *qd)
   return (bN / bNp * exp(-bNp * fabs(r[2])) < eps && bN / bNm * exp(-bNm * fabs(r[2])) < eps);
 }
 
+/* OR_QPGF_REF = path of oracle/_ref/libqpgf_ref.so (the reference's own qgf2_src compiled
+ * here): value + gradient evaluations (mode 0) then call its d3hm_qpgf_d2_v1 instead of the
+ * restatement, so that a complete oracle solve shows where the reference's own code lands on
+ * this machine's arithmetic.  Test instrument only. */
+typedef int (*ref_v1_fn)(double complex *qG, double complex *dqG, double *r, double eps, void *qd);
+static ref_v1_fn g_ref_v1;
+static pthread_once_t g_ref_once = PTHREAD_ONCE_INIT;
+static void ref_load(void)
+{
+  const char *p = getenv("OR_QPGF_REF");
+  if (!p || !*p) return;
+  void *h = dlopen(p, RTLD_NOW);
+  if (!h) {
+    fprintf(stderr, "oracle: OR_QPGF_REF %s: %s\n", p, dlerror());
+    exit(1);
+  }
+  g_ref_v1 = (ref_v1_fn)dlsym(h, "d3hm_qpgf_d2_v1");
+  if (!g_ref_v1) {
+    fprintf(stderr, "oracle: d3hm_qpgf_d2_v1 not in %s\n", p);
+    exit(1);
+  }
+}
+
 int or_qpgf(int mode, const double *r, double eps, const or_qpd2 *qd, double complex *out)
 {
+  pthread_once(&g_ref_once, ref_load);
+  if (mode == 0 && g_ref_v1) {
+    double rr[3] = {r[0], r[1], r[2]};
+    for (int i = 0; i < 10; i++) out[i] = 0.0;
+    return g_ref_v1(&out[0], &out[1], rr, eps, (void *)qd);
+  }
   int v2 = (mode == 1);
   double idd = 1.0 / (qd->vd1[0] * qd->vd2[1] - qd->vd2[0] * qd->vd1[1]);
   int l1 = (int)floor((r[0] * qd->vd2[1] - r[1] * qd->vd2[0]) * idd + 0.5);
