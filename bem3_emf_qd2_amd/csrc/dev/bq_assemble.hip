@@ -206,9 +206,15 @@ __device__ __forceinline__ int pair_rule_fs(double kabs, const Elem& et, const E
 }
 
 // ------------------------------------------------------------------------------------
-// domain 0 (periodic), regular pairs, 12-point rule.  One wavefront per pair (t, s): the
-// target's nodes share the 64 lanes (4 x 16 for a quad, 3 x 21 for a triangle, whose fourth
-// row does not exist), each lane walks the source's Gauss points with stride 16 or 21.
+// domain 0 (periodic), regular pairs.  One wavefront per pair (t, s): the target's nodes
+// share the 64 lanes (4 x 16 for a quad, 3 x 21 for a triangle, whose fourth row does not
+// exist), each lane walks the source's Gauss points with stride 16 or 21.  Every lane
+// evaluates the QPGF in every sweep, with lattice sums that are uniform over the wavefront
+// (qpgf_v1_wave): the lanes' points lie within one element of each other, so their
+// truncation disks nearly coincide, and lanes with the same loop bounds and order data stay
+// converged, where lanes with their own bounds and method diverge at every row end.  A lane
+// past the rule's last point or without a live node evaluates the first live lane's node at
+// the rule's first point and discards the result.
 // The per-lane sums live in LDS rather than registers: the lattice sums need nearly 256
 // registers, and keeping the kernel within 256 lets two wavefronts share each SIMD.
 #ifndef BQ_Q0GL_WAVES
@@ -246,10 +252,12 @@ k_q0_gl(bq_qplan P, const double* __restrict__ E, const double* __restrict__ ren
     load_elem(&selem[0][0], wv, es);
     ng = __builtin_amdgcn_readfirstlane(pair_rule(P, et, es, full_rule));
   }
-  if (active) {
+  const unsigned long long live = __ballot(active);
+  if (live != 0) {
+    const int tsrc = __shfl(tn, __ffsll((long long)live) - 1, 64);
     double rt[3];
     {
-      const double* p = ren + (size_t)abs(sid[t]) * 12 + 3 * tn;
+      const double* p = ren + (size_t)abs(sid[t]) * 12 + 3 * (active ? tn : tsrc);
       rt[0] = p[0];
       rt[1] = p[1];
       rt[2] = p[2];
@@ -258,10 +266,10 @@ k_q0_gl(bq_qplan P, const double* __restrict__ E, const double* __restrict__ ren
     const int nm = (np + gw - 1) / gw;
 #pragma unroll 1
     for (int m = 0; m < nm; ++m) {
-      const int p = sub + gw * m;
-      if (p >= np) break;
+      const int p0 = sub + gw * m;
+      const bool use = active && p0 < np;
       double z, et, w;
-      gauss_point_n(stri, ng, p, z, et, w);
+      gauss_point_n(stri, ng, use ? p0 : 0, z, et, w);
       double R[3], W[3], Nn[4];
       {
         Elem e;
@@ -270,17 +278,19 @@ k_q0_gl(bq_qplan P, const double* __restrict__ E, const double* __restrict__ ren
       }
       R[0] -= rt[0]; R[1] -= rt[1]; R[2] -= rt[2];
       QG q;
-      qpgf_v1(P, R[0], R[1], R[2], q);
-      double aW = sqrt(dot3(W, W));
-      cplx hn = q.d[0] * W[0] + q.d[1] * W[1] + q.d[2] * W[2];
-      shapeN(stri, z, et, Nn);
+      qpgf_v1_wave(P, R[0], R[1], R[2], q);
+      if (use) {
+        double aW = sqrt(dot3(W, W));
+        cplx hn = q.d[0] * W[0] + q.d[1] * W[1] + q.d[2] * W[2];
+        shapeN(stri, z, et, Nn);
 #pragma unroll
-      for (int n = 0; n < 4; ++n) {
-        double gw2 = w * Nn[n] * aW, hw = w * Nn[n];
-        acc[(2 * n) * 256] += gw2 * q.g.re;
-        acc[(2 * n + 1) * 256] += gw2 * q.g.im;
-        acc[(8 + 2 * n) * 256] += hw * hn.re;
-        acc[(8 + 2 * n + 1) * 256] += hw * hn.im;
+        for (int n = 0; n < 4; ++n) {
+          double gw2 = w * Nn[n] * aW, hw = w * Nn[n];
+          acc[(2 * n) * 256] += gw2 * q.g.re;
+          acc[(2 * n + 1) * 256] += gw2 * q.g.im;
+          acc[(8 + 2 * n) * 256] += hw * hn.re;
+          acc[(8 + 2 * n + 1) * 256] += hw * hn.im;
+        }
       }
     }
   }
@@ -299,8 +309,8 @@ k_q0_gl(bq_qplan P, const double* __restrict__ E, const double* __restrict__ ren
   }
 }
 
-// domain 0, flagged pairs: 64x64 rule, F and dF/dt recomputed with it (lattice-sum
-// accumulators in LDS, two wavefronts per SIMD, as in k_q0_gl)
+// domain 0, flagged pairs: 64x64 rule, F and dF/dt recomputed with it (wavefront-uniform
+// lattice sums, accumulators in LDS, two wavefronts per SIMD, as in k_q0_gl)
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
 k_q0_gh(bq_qplan P, const double* __restrict__ E, const double* __restrict__ ren, const int* __restrict__ sid, int dNe,
         const int* __restrict__ list, double2* __restrict__ G, double2* __restrict__ H, double* __restrict__ F3)
@@ -331,7 +341,7 @@ k_q0_gh(bq_qplan P, const double* __restrict__ E, const double* __restrict__ ren
     fs[1] += w * f0;
     fs[2] += w * f1;
     QG q;
-    qpgf_v1(P, R[0], R[1], R[2], q);
+    qpgf_v1_wave(P, R[0], R[1], R[2], q);
     double aW = sqrt(dot3(W, W));
     cplx hn = q.d[0] * W[0] + q.d[1] * W[1] + q.d[2] * W[2];
     shapeN(e.tri, z, et, Nn);
@@ -391,7 +401,8 @@ __device__ double polar_F_part(const Elem& e, double zt, double et)
 }
 
 // domain 0, singular pairs: one block per collocation node (lattice-sum accumulators in
-// LDS, two wavefronts per SIMD, as in k_q0_gl)
+// LDS, two wavefronts per SIMD, as in k_q0_gl; per-lane lattice sums: the polar rule's points
+// run from the node to the element's edge, and their union disks would hold more terms)
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
 k_q0_sing(bq_qplan P, const double* __restrict__ E, const int* __restrict__ sid, int dNe, double2* __restrict__ G,
           double2* __restrict__ H, double* __restrict__ F3)
@@ -897,7 +908,7 @@ __global__ void k_row_diag(int dNe, int open, const double* __restrict__ E, cons
   }
 }
 
-// algorithmic point counts per phase: [0] 12-point rule, [1] 64-point rule, [2] singular
+// algorithmic point counts per phase: [0] regular rules, [1] 64-point rule, [2] singular
 // polar rules (QPGF evaluations for the periodic domain)
 __global__ void __launch_bounds__(256) k_count_points(long n, int dNe, const double* __restrict__ E,
                                                       const int* __restrict__ sid,

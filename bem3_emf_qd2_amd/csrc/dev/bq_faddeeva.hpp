@@ -47,14 +47,17 @@ __device__ __forceinline__ cplx w_cf_upper(double x, double y)
   return r;
 }
 
-// Series of Algorithm 916 for |y| <= 7 and x = |xs| < 10 outside the continued-fraction region.
-__device__ __forceinline__ cplx w_series(double xs, double y)
+// Number of series terms for x = |Re z|: the terms decay like exp(-(a n - x)^2).
+__device__ __forceinline__ int w_series_terms(double x) { return min(60, (int)((x + 6.3) * (1.0 / kFA)) + 2); }
+
+// Series of Algorithm 916 for |y| <= 7 and x = |xs| < 10 outside the continued-fraction region,
+// with nterm >= w_series_terms(|xs|) terms (more terms only add ones below the rounding).
+__device__ __forceinline__ cplx w_series_n(double xs, double y, int nterm)
 {
   const double x = fabs(xs);
   double s1 = 0.0, s2 = 0.0, s3 = 0.0, s4 = 0.0, s5 = 0.0;
   const double ex2 = exp(-x * x);
   const double e2ax = exp(2.0 * kFA * x), em2ax = 1.0 / e2ax;
-  const int nterm = min(60, (int)((x + 6.3) * (1.0 / kFA)) + 2);
   double t = exp(-kFA2), q = exp(-3.0 * kFA2);  // exp(-a^2 n^2), exp(-a^2 (2n+1))
   const double q2 = exp(-2.0 * kFA2);
   double p2 = 1.0, pm2 = 1.0, y2 = y * y;
@@ -84,6 +87,7 @@ __device__ __forceinline__ cplx w_series(double xs, double y)
   double im = coef2 * sinc_of(2.0 * xs * y, s2xy) - coef1 * s2xy + copysign(0.5 * kFC * (s5 - s4), xs);
   return mk(re, im);
 }
+__device__ __forceinline__ cplx w_series(double xs, double y) { return w_series_n(xs, y, w_series_terms(fabs(xs))); }
 
 __device__ __forceinline__ bool w_use_cf(double x, double ya)
 {
@@ -124,6 +128,14 @@ __device__ __forceinline__ cplx faddeeva_w(double xs, double y)
   const double x = fabs(xs), ya = fabs(y);
   if (xs == 0.0 || x >= 10.0 || w_use_cf(x, ya)) return w_cold(xs, y);
   return w_series(xs, y);
+}
+// The same with the series' term count given by the caller (nterm >= w_series_terms(|xs|)), so
+// that lanes of a wavefront whose x differ run one trip count.
+__device__ __forceinline__ cplx faddeeva_w_n(double xs, double y, int nterm)
+{
+  const double x = fabs(xs), ya = fabs(y);
+  if (xs == 0.0 || x >= 10.0 || w_use_cf(x, ya)) return w_cold(xs, y);
+  return w_series_n(xs, y, nterm);
 }
 
 }  // namespace bq

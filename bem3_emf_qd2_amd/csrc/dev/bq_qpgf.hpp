@@ -489,6 +489,214 @@ __device__ void qpgf_v1(const bq_qplan& P, double rx, double ry, double rz, QG& 
   }
 }
 
+// ---- wavefront-uniform form -----------------------------------------------------------------
+// qpgf_v1_wave is qpgf_v1 for a whole wavefront (all 64 lanes must call it together, each with
+// its own point).  The lanes take one method (the Fourier-domain sum only when every lane would
+// take it), and the lattice sums run over the union of the lanes' truncation disks with
+// wave-uniform bounds: the spectral disks are centred on -k_t for every lane, so their union
+// is the largest of them, and every lane adds every order of it (the added evanescent terms are
+// below each lane's own cut); the spatial disks are bounded by one disk around lane 0's point,
+// and a lane adds only the lattice points inside its own disk.  The Faddeeva series take the
+// wavefront's largest term count.  Loop control, order data, the propagating/evanescent choice
+// and the Bloch phases of the spatial sum are then uniform and the lanes stay converged.
+__device__ __forceinline__ double wave_max(double v)
+{
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffff)), hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ double wave_first(double v)
+{
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffff)), hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Ewald spectral term (as spectral_term) with the evanescent erfc branches folded onto one
+// erfcx each: for a = bv +- z/2v <= 0, e^{+-bz} erfc(a) = 2 e^{+-bz} - e12 erfcx(-a), because
+// +-bz - a^2 = -b^2 v^2 - z^2/4v^2; the Faddeeva series take nterm terms.
+__device__ __forceinline__ void spectral_term_w(const bq_qplan& P, double ax, double ay, cplx ce, double pz, double ve,
+                                                double az, int nterm, cplx& F, cplx& Gz)
+{
+  const double b2 = P.k2 - ax * ax - ay * ay;
+  if (b2 > 0.0) {
+    const double b = sqrt(b2), ef = exp(b2 * ve * ve - az * az);
+    const cplx ff = cexpi(b * pz);
+    const cplx w = faddeeva_w_n(b * ve, az, nterm);
+    F = ce * (mk(ff.re, ef * w.im + ff.im) * (2.0 / b));
+    Gz = ce * (mk(ef * w.re - ff.re, -ff.im) * 2.0);
+  } else if (b2 == 0.0) {
+    qpgf_flag(P, -23);
+    F = Gz = mk(0.0, 0.0);
+  } else {
+    const double b = sqrt(-b2), bv = b * ve;
+    const double a1 = bv + az, a2 = bv - az;
+    const double e12 = exp(-bv * bv - az * az);
+    double f1 = e12 * erfcx(fabs(a1)), f2 = e12 * erfcx(fabs(a2));
+    if (a1 <= 0.0) f1 = 2.0 * exp(b * pz) - f1;
+    if (a2 <= 0.0) f2 = 2.0 * exp(-b * pz) - f2;
+    F = mk(ce.im, -ce.re) * ((f1 + f2) / b);
+    Gz = ce * (f1 - f2);
+  }
+}
+
+__device__ void qpgf_fourier_w(const bq_qplan& P, double px, double py, double pz, QG& o)
+{
+  const double az = fabs(pz), bcut = P.cut / az;
+  const double A = wave_max(sqrt(P.k2 + bcut * bcut) * 1.0000001), A2 = A * A;
+  const double d1n = sqrt(P.d1x * P.d1x + P.d1y * P.d1y);
+  const double c1 = (P.vk0 * P.d1x + P.vk1 * P.d1y) / (2.0 * kPI), w1 = A * d1n / (2.0 * kPI);
+  const int n1lo = (int)ceil(c1 - w1), n1hi = (int)floor(c1 + w1);
+  const double g22 = P.g2x * P.g2x + P.g2y * P.g2y;
+  const double gp1 = P.g1x * px + P.g1y * py, gp2 = P.g2x * px + P.g2y * py, ap = -(P.vk0 * px + P.vk1 * py);
+  const cplx step = cexpi(gp2);
+  cplx S = mk(0, 0), Sx = mk(0, 0), Sy = mk(0, 0), Sz = mk(0, 0);
+  for (int n1 = n1lo; n1 <= n1hi; ++n1) {
+    const double ux = -P.vk0 + n1 * P.g1x, uy = -P.vk1 + n1 * P.g1y;
+    int lo, hi;
+    if (!row_range(ux, uy, P.g2x, P.g2y, g22, A2, lo, hi)) continue;
+    const int c = row_centre(ux * P.g2x + uy * P.g2y, g22, lo, hi);
+    row_walk(lo, hi, c, cexpi(ap + n1 * gp1 + c * gp2), step, [&](int n2, cplx E) {
+      const double ax = ux + n2 * P.g2x, ay = uy + n2 * P.g2y;
+      const double b2 = P.k2 - ax * ax - ay * ay;
+      cplx dz, F;
+      if (b2 > 0.0) {
+        const double b = sqrt(b2);
+        dz = E * cexpi(b * az);
+        F = dz * (1.0 / b);
+      } else if (b2 < 0.0) {
+        const double b = sqrt(-b2);
+        dz = E * exp(-b * az);
+        F = mk(dz.im, -dz.re) * (1.0 / b);  // -i dz / b
+      } else {
+        qpgf_flag(P, -12);
+        dz = F = mk(0.0, 0.0);
+      }
+      S += F;
+      Sx += F * ax;
+      Sy += F * ay;
+      Sz += dz;
+    });
+  }
+  const double h = 0.5 * P.aidd, sg = (pz > 0.0) ? 1.0 : -1.0;
+  o.g = cmul_i(S) * h;
+  o.d[0] = Sx * (-h);
+  o.d[1] = Sy * (-h);
+  o.d[2] = Sz * (-h * sg);
+}
+
+__device__ void qpgf_ewald_w(const bq_qplan& P, double px, double py, double pz, double ve, QG& o, bool z0_ref)
+{
+  const double ive = 1.0 / ve, ke = P.k * ve, ke2 = ke * ke;
+  // ---- spatial sum: lane's own disk |r + R_l| <= Rmax, walked over one disk around lane 0
+  {
+    const double Rmax = 2.0 * ve * sqrt(ke2 + P.cut) * 1.0000001, Rmax2 = Rmax * Rmax;
+    const double rho2 = Rmax2 - pz * pz;
+    const double px0 = wave_first(px), py0 = wave_first(py);
+    const double du = sqrt((px - px0) * (px - px0) + (py - py0) * (py - py0));
+    const double Ru = wave_max(rho2 > 0.0 ? du + sqrt(rho2) : 0.0) * 1.0000001, Ru2 = Ru * Ru;
+    const int nterm = w_series_terms(wave_max(ke));
+    const double b1n = sqrt(P.b1x * P.b1x + P.b1y * P.b1y);
+    const double c1 = -(P.b1x * px0 + P.b1y * py0), w1 = Ru * b1n;
+    const int l1lo = (int)ceil(c1 - w1), l1hi = (int)floor(c1 + w1);
+    const double d22 = P.d2x * P.d2x + P.d2y * P.d2y;
+    const double gcon = ive / kSQPI;
+    cplx S = mk(0, 0), Sxy[2] = {mk(0, 0), mk(0, 0)}, Sz = mk(0, 0);
+    const cplx step = mk(P.ph2re, P.ph2im);
+    for (int l1 = l1lo; l1 <= l1hi; ++l1) {
+      const double ux0 = px0 + l1 * P.d1x, uy0 = py0 + l1 * P.d1y;
+      int lo, hi;
+      if (!row_range(ux0, uy0, P.d2x, P.d2y, d22, Ru2, lo, hi)) continue;
+      const int c = row_centre(ux0 * P.d2x + uy0 * P.d2y, d22, lo, hi);
+      const cplx E0 = cexpi(P.vk0 * (l1 * P.d1x + c * P.d2x) + P.vk1 * (l1 * P.d1y + c * P.d2y));
+      const double ux = px + l1 * P.d1x, uy = py + l1 * P.d1y;
+      row_walk(lo, hi, c, E0, step, [&](int l2, cplx Eb) {
+        const double ax = ux + l2 * P.d2x, ay = uy + l2 * P.d2y;
+        const double rl2 = ax * ax + ay * ay + pz * pz;
+        if (rl2 <= Rmax2) {
+          const double rl = sqrt(rl2), irl = 1.0 / rl;
+          const double y = 0.5 * rl * ive;
+          const cplx cef = Eb * exp(ke2 - y * y);
+          const cplx w = faddeeva_w_n(ke, y, nterm);
+          S += cef * (irl * w.re);
+          const cplx cf = cef * (irl * irl * (irl * w.re - P.k * w.im + gcon));
+          Sxy[0] += cf * ax;
+          Sxy[1] += cf * ay;
+          Sz += cf;
+        }
+      });
+    }
+    o.g = S * kI4PI;
+    o.d[0] = Sxy[0] * (-kI4PI);
+    o.d[1] = Sxy[1] * (-kI4PI);
+    o.d[2] = Sz * (-pz * kI4PI);
+  }
+  // ---- spectral sum over the union of the lanes' disks |beta| ve <= sqrt(cut)
+  const double A = wave_max(sqrt(P.k2 + P.cut * ive * ive) * 1.0000001), A2 = A * A;
+  const double vemax = wave_max(ve);
+  if (pz == 0.0 && z0_ref) {
+    spectral_z0_ref(P, px, py, ve, o);
+    return;
+  }
+  const double d1n = sqrt(P.d1x * P.d1x + P.d1y * P.d1y);
+  const double c1 = (P.vk0 * P.d1x + P.vk1 * P.d1y) / (2.0 * kPI), w1 = A * d1n / (2.0 * kPI);
+  const int n1lo = (int)ceil(c1 - w1), n1hi = (int)floor(c1 + w1);
+  const double g22 = P.g2x * P.g2x + P.g2y * P.g2y;
+  const double az = 0.5 * pz / ve;
+  cplx S = mk(0, 0), Sx = mk(0, 0), Sy = mk(0, 0), Sz = mk(0, 0);
+  const double gp1 = P.g1x * px + P.g1y * py, gp2 = P.g2x * px + P.g2y * py, ap = -(P.vk0 * px + P.vk1 * py);
+  const cplx step = cexpi(gp2);
+  for (int n1 = n1lo; n1 <= n1hi; ++n1) {
+    const double ux = -P.vk0 + n1 * P.g1x, uy = -P.vk1 + n1 * P.g1y;
+    int lo, hi;
+    if (!row_range(ux, uy, P.g2x, P.g2y, g22, A2, lo, hi)) continue;
+    const int c = row_centre(ux * P.g2x + uy * P.g2y, g22, lo, hi);
+    row_walk(lo, hi, c, cexpi(ap + n1 * gp1 + c * gp2), step, [&](int n2, cplx E) {
+      const double ax = ux + n2 * P.g2x, ay = uy + n2 * P.g2y;
+      // term count of the propagating orders' Faddeeva series for the largest ve (x = b ve)
+      const double bp2 = P.k2 - ax * ax - ay * ay;
+      const int nt = bp2 > 0.0 ? w_series_terms(sqrt(bp2) * vemax) : 0;
+      cplx F, Gz;
+      spectral_term_w(P, ax, ay, E, pz, ve, az, nt, F, Gz);
+      S += F;
+      Sx += F * ax;
+      Sy += F * ay;
+      Sz += Gz;
+    });
+  }
+  const double cd = 0.25 * P.aidd;
+  o.g += cmul_i(S) * cd;
+  o.d[0] += Sx * (-cd);
+  o.d[1] += Sy * (-cd);
+  o.d[2] += Sz * cd;
+}
+
+__device__ void qpgf_v1_wave(const bq_qplan& P, double rx, double ry, double rz, QG& o, bool z0_ref = true)
+{
+  double l1, l2, px, py;
+  fold_cell(P, rx, ry, l1, l2, px, py);
+  const double pz = rz;
+  const double ve = ewald_param(P, px, py, pz);
+  bool fd = false;
+  if (P.fd_allowed && pz != 0.0) {
+    const double bz = P.cut / fabs(pz);
+    const double n_fd = kPI * (P.k2 + bz * bz) / P.rcell;
+    const double n_sn = kPI * (P.k2 + P.cut / (ve * ve)) / P.rcell;
+    const double rs2 = 4.0 * ve * ve * (P.k2 * ve * ve + P.cut) - pz * pz;
+    const double n_sp = rs2 > 0.0 ? kPI * rs2 / P.cell : 0.0;
+    fd = BQ_COST_FD * n_fd < BQ_COST_SPEC * n_sn + BQ_COST_SPAT * n_sp;
+  }
+  if (__all(fd)) qpgf_fourier_w(P, px, py, pz, o);
+  else qpgf_ewald_w(P, px, py, pz, ve, o, z0_ref);
+  if (l1 != 0.0 || l2 != 0.0) {
+    cplx cf = cexpi(-(P.vk0 * (l1 * P.d1x + l2 * P.d2x) + P.vk1 * (l1 * P.d1y + l2 * P.d2y)));
+    o.g = o.g * cf;
+    for (int i = 0; i < 3; ++i) o.d[i] = o.d[i] * cf;
+  }
+}
+
 // Value, gradient and Hessian at r (the reference's _v2, qgf2_src/d3_qpgf_d2.c:178-230,
 // :887-1043, :2011-2399).  Its stop tests watch the xx and yy second derivatives, which do
 // not vanish at r_z == 0, so every sum here is the converged one.
