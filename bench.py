@@ -53,8 +53,8 @@ FLOPS_PER_EVAL["c2s"] = FLOPS_PER_EVAL["c2"]
 # the 6144-quad sweep, read on its 384-quad variant (c4s) at the same six timed points
 # (profiles/k_q0_gl_c4s_pmc.md): the reference's accuracy (cut 40) and the mixed path's tuned
 # assembly (tolerance 1e-12: cut 26 at every point)
-FLOPS_PER_EVAL["c4"] = {"k_q0_gl": 62737}
-FLOPS_PER_EVAL["c4_tuned"] = {"k_q0_gl": 48457}
+FLOPS_PER_EVAL["c4"] = {"k_q0_gl": 60007}
+FLOPS_PER_EVAL["c4_tuned"] = {"k_q0_gl": 45861}
 FLOPS_PER_EVAL["c4s"], FLOPS_PER_EVAL["c4s_tuned"] = FLOPS_PER_EVAL["c4"], FLOPS_PER_EVAL["c4_tuned"]
 # the 1536-quad two-layer sweep has the open region, lattice, wavelength and direction ranges of
 # the 6144-quad sweep, which fix the per-evaluation cost at the reference's cut
@@ -66,7 +66,15 @@ TRAFFIC_PER_EVAL = {"c1": {"k_q0_gl": (1163230.7 * 2.0 + 9545073.7) * 1024 / 469
 TRAFFIC_PER_EVAL["c2s"] = TRAFFIC_PER_EVAL["c2"]
 # FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00 per evaluation on c4s at the reference's cut
 # (profiles/k_q0_gl_c4s_pmc.md); the 1536-quad two-layer sweep shares the open region's physics
-TRAFFIC_PER_EVAL["c4"] = TRAFFIC_PER_EVAL["c3"] = TRAFFIC_PER_EVAL["c4s"] = {"k_q0_gl": 316.0}
+TRAFFIC_PER_EVAL["c4"] = TRAFFIC_PER_EVAL["c3"] = TRAFFIC_PER_EVAL["c4s"] = {"k_q0_gl": 244.1}
+# k_q0_gl's issue profile on c4s (same counters, six points): active lanes per VALU instruction
+# (SQ_THREAD_CYCLES_VALU / 64 SQ_ACTIVE_INST_VALU; the tail lanes of a pair's last sweep count as
+# active, they evaluate a point whose result is discarded) and the FP64 share of the VALU
+# instructions (ADD + MUL + FMA + TRANS F64 over SQ_INSTS_VALU)
+VALU_PROFILE = {"c4": {"k_q0_gl": {"valu_lane_util": 0.946, "fp64_share_of_valu": 0.719}},
+                "c4_tuned": {"k_q0_gl": {"valu_lane_util": 0.946, "fp64_share_of_valu": 0.712}}}
+VALU_PROFILE["c4s"], VALU_PROFILE["c4s_tuned"] = VALU_PROFILE["c4"], VALU_PROFILE["c4_tuned"]
+VALU_PROFILE["c3"] = VALU_PROFILE["c4"]
 
 WORKLOADS = {
     "c1": dict(kind="sphere", n=8, radius=0.4, n_sphere=1.5, lam=0.6, n0=1.0, theta=1e-6, phi=0.0, pol="TM",
@@ -237,6 +245,9 @@ def _roofline(ktimes, stats_per_step, steps, workload):
             r["achieved"] = ach
             r["frac"] = ach / FP64_PEAK_TFS
             r["flops_per_eval"] = fpe
+        vp = VALU_PROFILE.get(workload, {}).get(name)
+        if vp:
+            r.update(vp)
     return r
 
 
