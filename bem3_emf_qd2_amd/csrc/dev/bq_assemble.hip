@@ -10,13 +10,14 @@
 // 12x12 rule (CBD_SDF), and polar singular rules when t == s.
 //
 // GPU organisation (one launch per phase, all on one stream):
-//   k_classify   one thread per (row, s): F and dF/dt with the 12-point rule (no Green's
+//   k_classify   one thread per (row, s): F and dF/dt with the pair's Gauss order (no Green's
 //                function needed), flags the pair GL / GH / singular / no such row (the
 //                fourth slot of a linear triangle).
-//   k_q0_gl      one wavefront per element pair (t, s): its 4 collocation nodes x 144 Gauss
-//                points = 576 = 9 x 64 QPGF evaluations; one QPGF per Gauss point feeds
-//                all four interpolation functions (the reference evaluates it 4 times).
-//                16-lane butterfly reductions, one row of G and H written per node.
+//   k_q0_gl      one wavefront per element pair (t, s): its 4 collocation nodes x the pair's
+//                n x n Gauss points (n from 3 to 10, or 12), lattice sums uniform over the
+//                wavefront; one QPGF per Gauss point feeds all four interpolation functions
+//                (the reference evaluates it 4 times).  Fixed-order sums over each node's
+//                lanes, one row of G and H written per node.
 //   k_q0_gh      one 256-thread block per flagged pair: 4096 points.
 //   k_q0_sing    one block per collocation node: polar rule (4 sub-triangles x 128 angles
 //                x 12 radii), F by the polar rule and dF/dt by central differences.
@@ -69,50 +70,6 @@ __device__ __forceinline__ void f_terms(const double R[3], const double W[3], co
   f = rdW * iR * iR;
   f0 = iR3 * (3.0 * rdv0 * rdW - dot3(W, c.vz));
   f1 = iR3 * (3.0 * rdv1 * rdW - dot3(W, c.ve));
-}
-
-// ------------------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_classify(const double* __restrict__ E, const double* __restrict__ ren,
-                                                  const int* __restrict__ sid, int dNe, int freespace,
-                                                  double* __restrict__ F3, unsigned char* __restrict__ flag)
-{
-  long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  long N = 4L * dNe;
-  if (idx >= N * dNe) return;
-  int row = (int)(idx / dNe), sc = (int)(idx % dNe);
-  int td = sid[row / 4 + 1], tn = row & 3, sd = sid[sc + 1];
-  double* o = F3 + 3 * idx;
-  if (tn == 3 && elem_is_tri(E, abs(td))) {
-    flag[idx] = 3;
-    o[0] = o[1] = o[2] = 0.0;
-    return;
-  }
-  if (td == sd) {
-    flag[idx] = 2;
-    o[0] = o[1] = o[2] = 0.0;
-    return;
-  }
-  ColPt c;
-  colloc(E, ren, abs(td), tn, c);
-  Elem e;
-  load_elem(E, abs(sd), e);
-  double F = 0, F0 = 0, F1 = 0;
-  const int np = gauss_points(e.tri, false);
-  for (int p = 0; p < np; ++p) {
-    double z, et, w, R[3], W[3], f, f0, f1;
-    gauss_point(e.tri, false, p, z, et, w);
-    rw_at(e, z, et, R, W);
-    R[0] -= c.rt[0]; R[1] -= c.rt[1]; R[2] -= c.rt[2];
-    f_terms(R, W, c, f, f0, f1);
-    F += w * f;
-    F0 += w * f0;
-    F1 += w * f1;
-  }
-  double sg = (freespace && sd < 0) ? -kI4PI : kI4PI;
-  o[0] = F * sg;
-  o[1] = F0 * sg;
-  o[2] = F1 * sg;
-  flag[idx] = (fabs(F0 * kI4PI) > kSDF || fabs(F1 * kI4PI) > kSDF) ? 1 : 0;
 }
 
 // ------------------------------------------------------------------------------------
@@ -204,6 +161,63 @@ __device__ __forceinline__ int pair_rule_fs(double kabs, const Elem& et, const E
   if (!(gap > 0.0) || !(rs > 0.0)) return 12;
   return rule_from_gap(gap, rs, kabs, ltol);
 }
+
+// ------------------------------------------------------------------------------------
+// F (solid angle / 4 pi) and its tangential derivatives per (row, s) take the distance-chosen
+// Gauss order of the pair as the kernels do (no wave factor; the bound tightened by e because
+// the derivative terms grow like 1/R^3 towards the target where the bound allows for 1/R);
+// pairs the bound leaves at 12 points, which include every pair near enough for |dF/dt| to
+// approach the 0.5 switch, keep the reference's 12-point rule.
+__global__ void __launch_bounds__(256) k_classify(const double* __restrict__ E, const double* __restrict__ ren,
+                                                  const int* __restrict__ sid, int dNe, int freespace,
+                                                  int full_rule, double ltol, double* __restrict__ F3,
+                                                  unsigned char* __restrict__ flag)
+{
+  long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long N = 4L * dNe;
+  if (idx >= N * dNe) return;
+  int row = (int)(idx / dNe), sc = (int)(idx % dNe);
+  int td = sid[row / 4 + 1], tn = row & 3, sd = sid[sc + 1];
+  double* o = F3 + 3 * idx;
+  if (tn == 3 && elem_is_tri(E, abs(td))) {
+    flag[idx] = 3;
+    o[0] = o[1] = o[2] = 0.0;
+    return;
+  }
+  if (td == sd) {
+    flag[idx] = 2;
+    o[0] = o[1] = o[2] = 0.0;
+    return;
+  }
+  ColPt c;
+  colloc(E, ren, abs(td), tn, c);
+  Elem e;
+  load_elem(E, abs(sd), e);
+  int ng;
+  {
+    Elem et;
+    load_elem(E, abs(td), et);
+    ng = pair_rule_fs(0.0, et, e, full_rule, ltol - 1.0);
+  }
+  double F = 0, F0 = 0, F1 = 0;
+  const int np = (e.tri ? 3 : 1) * ng * ng;
+  for (int p = 0; p < np; ++p) {
+    double z, et, w, R[3], W[3], f, f0, f1;
+    gauss_point_n(e.tri, ng, p, z, et, w);
+    rw_at(e, z, et, R, W);
+    R[0] -= c.rt[0]; R[1] -= c.rt[1]; R[2] -= c.rt[2];
+    f_terms(R, W, c, f, f0, f1);
+    F += w * f;
+    F0 += w * f0;
+    F1 += w * f1;
+  }
+  double sg = (freespace && sd < 0) ? -kI4PI : kI4PI;
+  o[0] = F * sg;
+  o[1] = F0 * sg;
+  o[2] = F1 * sg;
+  flag[idx] = (fabs(F0 * kI4PI) > kSDF || fabs(F1 * kI4PI) > kSDF) ? 1 : 0;
+}
+
 
 // ------------------------------------------------------------------------------------
 // domain 0 (periodic), regular pairs.  One wavefront per pair (t, s): the target's nodes
@@ -992,7 +1006,7 @@ int bq_assemble_domain_dev(const bq_adom* a, void* work, void* stream)
   BQ_HIP(hipMemsetAsync(count, 0, sizeof(int), s));
   BQ_HIP(hipMemsetAsync(npts, 0, 3 * sizeof(unsigned long long), s));
   const int freespace = a->periodic ? 0 : 1;
-  BQ_KTIME("k_classify", s, k_classify<<<(unsigned)((pairs + 255) / 256), 256, 0, s>>>(a->E, a->ren, a->sid, dNe, freespace, F3, flag));
+  BQ_KTIME("k_classify", s, k_classify<<<(unsigned)((pairs + 255) / 256), 256, 0, s>>>(a->E, a->ren, a->sid, dNe, freespace, a->full_rule, a->plan.rule_ltol, F3, flag));
   BQ_HIP(hipGetLastError());
   k_collect_gh<<<(unsigned)((pairs + 255) / 256), 256, 0, s>>>(pairs, flag, list, count);
   BQ_HIP(hipGetLastError());
