@@ -46,6 +46,17 @@ int bq_lu_update_mode(int mode);
 void bq_lu_release_stream(void *stream);
 void bq_cgemm_split_release(void *stream);
 
+/* Few-column solves (nrhs <= 8) through the inverted 512-row diagonal blocks of both factors:
+ * W (device) holds bq_lu_blkinv_elems(n) complex elements, filled by bq_[zc]lu_blkinv_dev from
+ * the factors and dinv; n must be a multiple of 512 (bq_lu_blkinv_elems returns 0 otherwise). */
+long bq_lu_blkinv_elems(int n);
+int bq_zlu_blkinv_dev(void *stream, int n, const double *A, long lda, const double *dinv, double *W);
+int bq_clu_blkinv_dev(void *stream, int n, const float *A, long lda, const float *dinv, float *W);
+int bq_zgetrs_blkinv_dev(void *stream, int n, int nrhs, const double *A, long lda, const double *dinv,
+                         const double *W, double *B, long ldb);
+int bq_cgetrs_blkinv_dev(void *stream, int n, int nrhs, const float *A, long lda, const float *dinv,
+                         const float *W, float *B, long ldb);
+
 /* The row permutation the factorization stored in dinv (perm[i] = original row placed at
  * row i; device int[n]); single != 0 for the FP32 buffer. */
 const int *bq_lu_perm_dev(const void *dinv, int n, int single);

@@ -74,6 +74,14 @@ struct P<double> {
                          (const rocblas_double_complex*)B, ldb, (const rocblas_double_complex*)be,
                          (rocblas_double_complex*)Cm, ldc);
   }
+  static rocblas_status gemm_sb(rocblas_handle h, int m, int n, int k, const V* al, const V* A, int lda, long sa,
+                                const V* B, int ldb, long sb, const V* be, V* Cm, int ldc, long sc, int batch)
+  {
+    return rocblas_zgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, m, n, k,
+                                         (const rocblas_double_complex*)al, (const rocblas_double_complex*)A, lda, sa,
+                                         (const rocblas_double_complex*)B, ldb, sb, (const rocblas_double_complex*)be,
+                                         (rocblas_double_complex*)Cm, ldc, sc, batch);
+  }
   static constexpr const char* kt_f = "zgetrf";
   static constexpr const char* kt_p = "zgetrf_panel";
   static constexpr const char* kt_u = "zgetrf_update";
@@ -541,9 +549,11 @@ __global__ void __launch_bounds__(256) k_skinny_update(int m, int kk, const type
   __shared__ V xs[kSmallRhs * kNB2];
   __shared__ V part[3 * 64 * kSmallRhs];
   const int tid = threadIdx.x, lane = tid & 63, q = tid >> 6;
-  for (int e = tid; e < kk * nrhs; e += 256) {
+  // columns past nrhs staged as zeros: no per-column branch in the inner loop, so the loads of
+  // A are issued 16 at a time
+  for (int e = tid; e < kk * kSmallRhs; e += 256) {
     const int k = e % kk, c = e / kk;
-    xs[c * kNB2 + k] = X[(long)c * ldx + k];
+    xs[c * kNB2 + k] = c < nrhs ? X[(long)c * ldx + k] : T::v(T::mkc(0, 0));
   }
   __syncthreads();
   const int r = blockIdx.x * 64 + lane;
@@ -552,12 +562,20 @@ __global__ void __launch_bounds__(256) k_skinny_update(int m, int kk, const type
   for (int cc = 0; cc < kSmallRhs; ++cc) acc[cc] = T::mkc(0, 0);
   if (r < m) {
     const int k0 = q * 64, k1 = min(kk, k0 + 64);
-#pragma unroll 16
-    for (int k = k0; k < k1; ++k) {
+    int k = k0;
+    for (; k + 16 <= k1; k += 16) {
+      C a[16];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) a[u] = T::c(A[(long)(k + u) * lda + r]);
+#pragma unroll
+      for (int u = 0; u < 16; ++u)
+#pragma unroll
+        for (int cc = 0; cc < kSmallRhs; ++cc) acc[cc] = acc[cc] + a[u] * T::c(xs[cc * kNB2 + k + u]);
+    }
+    for (; k < k1; ++k) {
       const C a = T::c(A[(long)k * lda + r]);
 #pragma unroll
-      for (int cc = 0; cc < kSmallRhs; ++cc)
-        if (cc < nrhs) acc[cc] = acc[cc] + a * T::c(xs[cc * kNB2 + k]);
+      for (int cc = 0; cc < kSmallRhs; ++cc) acc[cc] = acc[cc] + a * T::c(xs[cc * kNB2 + k]);
     }
   }
   if (q > 0) {
@@ -573,6 +591,67 @@ __global__ void __launch_bounds__(256) k_skinny_update(int m, int kk, const type
       for (int w = 0; w < 3; ++w) t = t + T::c(part[(w * kSmallRhs + cc) * 64 + lane]);
       V* o = Bo + (long)cc * ldb + r;
       *o = T::v(T::c(*o) - t);
+    }
+  }
+}
+
+// Bo[r, c] -= sum_k A[r, k] X[k, c] (STORE: Bo = -sum) for r < m, k < kk <= kNBF, c < nrhs: the
+// products of the solves through the inverted 512-row diagonal blocks.  X (kk x nrhs) is staged
+// in LDS once per workgroup; 64 rows per workgroup, the k range split over its 4 wavefronts
+// (16 loads of A in flight per lane) and reduced in LDS.  X and Bo are disjoint.
+template <class R, bool STORE>
+__global__ void __launch_bounds__(256) k_skinny_blk(int m, int kk, const typename P<R>::V* __restrict__ A, long lda,
+                                                    const typename P<R>::V* __restrict__ X, long ldx, int nrhs,
+                                                    typename P<R>::V* __restrict__ Bo, long ldb)
+{
+  using V = typename P<R>::V;
+  using C = typename P<R>::C;
+  using T = P<R>;
+  __shared__ V xs[kSmallRhs * kNBF];
+  __shared__ V part[3 * 64 * kSmallRhs];
+  const int tid = threadIdx.x, lane = tid & 63, q = tid >> 6;
+  // columns past nrhs staged as zeros, so that the inner loop has no per-column branch and the
+  // loads of A can be issued together
+  for (int e = tid; e < kk * kSmallRhs; e += 256) {
+    const int k = e % kk, c = e / kk;
+    xs[c * kNBF + k] = c < nrhs ? X[(long)c * ldx + k] : T::v(T::mkc(0, 0));
+  }
+  __syncthreads();
+  const int r = blockIdx.x * 64 + lane;
+  C acc[kSmallRhs];
+#pragma unroll
+  for (int cc = 0; cc < kSmallRhs; ++cc) acc[cc] = T::mkc(0, 0);
+  if (r < m) {
+    const int kq = (kk + 3) / 4, k0 = q * kq, k1 = min(kk, k0 + kq);
+    int k = k0;
+    for (; k + 16 <= k1; k += 16) {
+      C a[16];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) a[u] = T::c(A[(long)(k + u) * lda + r]);
+#pragma unroll
+      for (int u = 0; u < 16; ++u)
+#pragma unroll
+        for (int cc = 0; cc < kSmallRhs; ++cc) acc[cc] = acc[cc] + a[u] * T::c(xs[cc * kNBF + k + u]);
+    }
+    for (; k < k1; ++k) {
+      const C a = T::c(A[(long)k * lda + r]);
+#pragma unroll
+      for (int cc = 0; cc < kSmallRhs; ++cc) acc[cc] = acc[cc] + a * T::c(xs[cc * kNBF + k]);
+    }
+  }
+  if (q > 0) {
+#pragma unroll
+    for (int cc = 0; cc < kSmallRhs; ++cc) part[((q - 1) * kSmallRhs + cc) * 64 + lane] = T::v(acc[cc]);
+  }
+  __syncthreads();
+  if (q == 0 && r < m) {
+#pragma unroll
+    for (int cc = 0; cc < kSmallRhs; ++cc) {
+      if (cc >= nrhs) continue;
+      C t = acc[cc];
+      for (int w = 0; w < 3; ++w) t = t + T::c(part[(w * kSmallRhs + cc) * 64 + lane]);
+      V* o = Bo + (long)cc * ldb + r;
+      *o = STORE ? T::v(T::mkc(0, 0) - t) : T::v(T::c(*o) - t);
     }
   }
 }
@@ -740,6 +819,60 @@ int block_inverses(hipStream_t s, int nfull, const typename P<R>::V* A, long lda
 inline int apply_block_inverse(hipStream_t s, int nc, const float2* Wneg, float2* X, long ldx)
 {
   return bq_cgemm_split_store_dev(s, kNBF, nc, kNBF, (const float*)Wneg, kNBF, (const float*)X, ldx, (float*)X, ldx);
+}
+
+// Solve A X = B with at most kSmallRhs right-hand sides through the inverted 512-row diagonal
+// blocks of both factors (W: nfull blocks -inv(L_bb), then nfull blocks -inv(U_bb), from
+// block_inverses; n a multiple of kNBF).  Each block is one product with its inverse and one
+// product of the factor's block column with the rows outside it, so a triangle takes 2 n / 512
+// launches that each read their part of the factor once, where the 256-row substitution of the
+// few-column getrs takes 2 n / 256 launches bound by their 32-row step chains.  The right-hand
+// sides alternate between B and a work copy T: the forward sweep reads T and writes the
+// block solutions into B, the backward sweep reads B and writes into T.
+template <class R>
+int getrs_blkinv(hipStream_t s, int n, int nrhs, const typename P<R>::V* A, long lda, const typename P<R>::V* inv,
+                 const typename P<R>::V* W, typename P<R>::V* B, long ldb)
+{
+  using V = typename P<R>::V;
+  if (n <= 0 || nrhs <= 0) return 0;
+  if (nrhs > kSmallRhs || n % kNBF != 0) {
+    bq::set_error("getrs_blkinv: %d right-hand sides, order %d (at most %d, a multiple of %d)", nrhs, n, kSmallRhs,
+                  kNBF);
+    return -1;
+  }
+  const int kt_all = bq::ktime_begin(P<R>::kt_s, s);
+  const int nfull = n / kNBF;
+  const long bs = (long)kNBF * kNBF;
+  const V* WL = W;
+  const V* WU = W + bs * nfull;
+  const int* perm = perm_of(inv, n);
+  V* T = nullptr;
+  BQ_HIP(hipMallocAsync((void**)&T, sizeof(V) * (size_t)n * nrhs, s));
+  const long tot = (long)n * nrhs;
+  k_gather_rows<R><<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(n, nrhs, perm, B, ldb, T);
+  BQ_HIP(hipGetLastError());
+  const unsigned gb = kNBF / 64;
+  for (int b = 0; b < nfull; ++b) {
+    const int j0 = b * kNBF, je = j0 + kNBF;
+    k_skinny_blk<R, true><<<gb, 256, 0, s>>>(kNBF, kNBF, WL + bs * b, kNBF, T + j0, n, nrhs, B + j0, ldb);
+    if (n - je > 0)
+      k_skinny_blk<R, false><<<(unsigned)((n - je + 63) / 64), 256, 0, s>>>(n - je, kNBF, A + (long)j0 * lda + je,
+                                                                            lda, B + j0, ldb, nrhs, T + je, n);
+    BQ_HIP(hipGetLastError());
+  }
+  for (int b = nfull - 1; b >= 0; --b) {
+    const int j0 = b * kNBF;
+    k_skinny_blk<R, true><<<gb, 256, 0, s>>>(kNBF, kNBF, WU + bs * b, kNBF, B + j0, ldb, nrhs, T + j0, n);
+    if (j0 > 0)
+      k_skinny_blk<R, false><<<(unsigned)((j0 + 63) / 64), 256, 0, s>>>(j0, kNBF, A + (long)j0 * lda, lda, T + j0, n,
+                                                                         nrhs, B, ldb);
+    BQ_HIP(hipGetLastError());
+  }
+  BQ_HIP(hipMemcpy2DAsync(B, ldb * sizeof(V), T, (size_t)n * sizeof(V), (size_t)n * sizeof(V), nrhs,
+                          hipMemcpyDeviceToDevice, s));
+  BQ_HIP(hipFreeAsync(T, s));
+  bq::ktime_end(kt_all, s);
+  return 0;
 }
 
 // In-place LU of the n x n column-major matrix A (device); inv receives the inverted
@@ -971,6 +1104,20 @@ int getrs(hipStream_t s, int n, int nrhs, const typename P<R>::V* A, long lda, c
   return 0;
 }
 
+// -inv of the 512-row diagonal blocks of both factors into W (lower ones, then upper ones)
+template <class R>
+int blkinv_both(hipStream_t s, int n, const typename P<R>::V* A, long lda, const typename P<R>::V* inv,
+                typename P<R>::V* W)
+{
+  if (!(n > 0 && n % kNBF == 0)) {
+    bq::set_error("block inverses: order %d is not a multiple of %d", n, kNBF);
+    return -1;
+  }
+  const int nfull = n / kNBF;
+  if (block_inverses<R>(s, nfull, A, lda, inv, false, W)) return -1;
+  return block_inverses<R>(s, nfull, A, lda, inv, true, W + (long)kNBF * kNBF * nfull);
+}
+
 }  // namespace
 
 extern "C" {
@@ -1100,6 +1247,31 @@ int bq_cgetrs_dev(void* stream, int n, int nrhs, const float* Ad, long lda, cons
 {
   return getrs<float>((hipStream_t)stream, n, nrhs, (const float2*)Ad, lda, ipiv, (const float2*)dinv, (float2*)Bd,
                       ldb);
+}
+
+// The inverted 512-row diagonal blocks of both factors for bq_[zc]getrs_blkinv_dev: complex
+// elements W must hold (0 when n is not a multiple of 512), and their construction.
+long bq_lu_blkinv_elems(int n) { return (n > 0 && n % kNBF == 0) ? 2L * (n / kNBF) * kNBF * kNBF : 0; }
+
+int bq_zlu_blkinv_dev(void* stream, int n, const double* Ad, long lda, const double* dinv, double* W)
+{
+  return blkinv_both<double>((hipStream_t)stream, n, (const double2*)Ad, lda, (const double2*)dinv, (double2*)W);
+}
+int bq_clu_blkinv_dev(void* stream, int n, const float* Ad, long lda, const float* dinv, float* W)
+{
+  return blkinv_both<float>((hipStream_t)stream, n, (const float2*)Ad, lda, (const float2*)dinv, (float2*)W);
+}
+int bq_zgetrs_blkinv_dev(void* stream, int n, int nrhs, const double* Ad, long lda, const double* dinv,
+                         const double* W, double* Bd, long ldb)
+{
+  return getrs_blkinv<double>((hipStream_t)stream, n, nrhs, (const double2*)Ad, lda, (const double2*)dinv,
+                              (const double2*)W, (double2*)Bd, ldb);
+}
+int bq_cgetrs_blkinv_dev(void* stream, int n, int nrhs, const float* Ad, long lda, const float* dinv,
+                         const float* W, float* Bd, long ldb)
+{
+  return getrs_blkinv<float>((hipStream_t)stream, n, nrhs, (const float2*)Ad, lda, (const float2*)dinv,
+                             (const float2*)W, (float2*)Bd, ldb);
 }
 
 // Host-memory convenience: solve A X = B (A n x n, B n x nrhs, both column-major host).

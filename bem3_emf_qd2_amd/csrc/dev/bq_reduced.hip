@@ -435,6 +435,18 @@ struct LA<double> {
   {
     return bq_zgetrs_dev(s, n, nrhs, (const double*)A, n, ip, (const double*)inv, (double*)B, n);
   }
+  static int blkinv(hipStream_t s, int n, const void* A, const void* inv, void* W)
+  {
+    return W ? bq_zlu_blkinv_dev(s, n, (const double*)A, n, (const double*)inv, (double*)W) : 0;
+  }
+  // few columns through the inverted diagonal blocks W when there are any
+  static int getrs_w(hipStream_t s, int n, int nrhs, const void* A, const int* ip, const void* inv, const void* W,
+                     void* B)
+  {
+    if (W && nrhs <= 8)
+      return bq_zgetrs_blkinv_dev(s, n, nrhs, (const double*)A, n, (const double*)inv, (const double*)W, (double*)B, n);
+    return getrs(s, n, nrhs, A, ip, inv, B);
+  }
   static int gemm(hipStream_t s, int m, int nc, int k, cplx al, const void* A, const void* B, cplx be, void* C)
   {
     const double a[2] = {al.re, al.im}, b[2] = {be.re, be.im};
@@ -451,6 +463,17 @@ struct LA<float> {
   static int getrs(hipStream_t s, int n, int nrhs, const void* A, const int* ip, const void* inv, void* B)
   {
     return bq_cgetrs_dev(s, n, nrhs, (const float*)A, n, ip, (const float*)inv, (float*)B, n);
+  }
+  static int blkinv(hipStream_t s, int n, const void* A, const void* inv, void* W)
+  {
+    return W ? bq_clu_blkinv_dev(s, n, (const float*)A, n, (const float*)inv, (float*)W) : 0;
+  }
+  static int getrs_w(hipStream_t s, int n, int nrhs, const void* A, const int* ip, const void* inv, const void* W,
+                     void* B)
+  {
+    if (W && nrhs <= 8)
+      return bq_cgetrs_blkinv_dev(s, n, nrhs, (const float*)A, n, (const float*)inv, (const float*)W, (float*)B, n);
+    return getrs(s, n, nrhs, A, ip, inv, B);
   }
   static int gemm(hipStream_t s, int m, int nc, int k, cplx al, const void* A, const void* B, cplx be, void* C)
   {
@@ -524,7 +547,8 @@ int red_alloc(bq_sess* S, bool cap64)
       dalloc(S, (void**)&R.nloc, (size_t)Nm * 24) ||
       dalloc(S, (void**)&R.part, (size_t)kDotBlocks * (kRestart + 2) * 16) ||
       dalloc(S, &R.vf, (size_t)15 * n * 16) || dalloc(S, (void**)&R.Vb, v8 * (kRestart + 1)) ||
-      dalloc(S, (void**)&R.Zb, v8 * kRestart)) {
+      dalloc(S, (void**)&R.Zb, v8 * kRestart) || dalloc(S, &R.W0, (size_t)bq_lu_blkinv_elems(n) * es) ||
+      dalloc(S, &R.W1, (size_t)bq_lu_blkinv_elems(n) * es) || dalloc(S, &R.WB, (size_t)bq_lu_blkinv_elems(n) * es)) {
     bq::reduced_free(S);
     return -1;
   }
@@ -667,14 +691,14 @@ int red_apply(bq_sess* S, const double2* r, double2* X)
   const int kt = bq::ktime_begin(sizeof(F) == 8 ? "reduced_apply_z" : "reduced_apply_c", s);
   k_cvt<V><<<nblk(2 * n4, 256), 256, 0, s>>>(2 * n4, r, R0);
   BQ_HIP(hipGetLastError());
-  if (LA<F>::getrs(s, n, 4, R.G0, R.ip0, R.inv0, R0)) return -1;
-  if (LA<F>::getrs(s, n, 4, R.G1, R.ip1, R.inv1, R1)) return -1;
+  if (LA<F>::getrs_w(s, n, 4, R.G0, R.ip0, R.inv0, R.W0, R0)) return -1;
+  if (LA<F>::getrs_w(s, n, 4, R.G1, R.ip1, R.inv1, R.W1, R1)) return -1;
   k_gcomb<V><<<nblk(n4, 256), 256, 0, s>>>(n, R0, R1, k.beps);
   BQ_HIP(hipGetLastError());
   if (LA<F>::gemm(s, n, 3, n, cplx{1.0, 0.0}, R.Binv, R1, cplx{0.0, 0.0}, Y4)) return -1;
   k_rhs3<V><<<nblk(n, 256), 256, 0, s>>>(n, R1 + 3L * n, R.nn3, k.sigm, Y4);
   BQ_HIP(hipGetLastError());
-  if (LA<F>::getrs(s, n, 1, R.B, R.ipB, R.invB, Y4 + 3L * n)) return -1;
+  if (LA<F>::getrs_w(s, n, 1, R.B, R.ipB, R.invB, R.WB, Y4 + 3L * n)) return -1;
   k_tnorm<V><<<nblk(n, 256), 256, 0, s>>>(n, Y4 + 3L * n, R.nn3, T);
   BQ_HIP(hipGetLastError());
   const cplx md = {-k.delt.re, -k.delt.im};
@@ -741,7 +765,9 @@ int red_factor_t(bq_sess* S)
       return;
     }
     info1 = LA<F>::getrf(s2, n, R.G1, R.ip1, R.inv1);
-    if (info1 == 0 && LA<F>::getrs(s2, n, n, R.G1, R.ip1, R.inv1, R.P)) rc1 = -1;  // P = G1^-1 H1
+    if (info1 == 0 && (LA<F>::getrs(s2, n, n, R.G1, R.ip1, R.inv1, R.P) ||  // P = G1^-1 H1
+                       LA<F>::blkinv(s2, n, R.G1, R.inv1, R.W1)))
+      rc1 = -1;
     if (info1 || rc1) err1 = bq_last_error();
     if (hipEventRecord(ev[1], s2) != hipSuccess && !rc1) {
       rc1 = -1;
@@ -749,7 +775,9 @@ int red_factor_t(bq_sess* S)
     }
   });
   int info = LA<F>::getrf(s, n, R.G0, R.ip0, R.inv0);
-  int rc0 = (info == 0) ? LA<F>::getrs(s, n, n, R.G0, R.ip0, R.inv0, R.Mx) : 0;  // M = G0^-1 H0
+  int rc0 = (info == 0) ? (LA<F>::getrs(s, n, n, R.G0, R.ip0, R.inv0, R.Mx) ||  // M = G0^-1 H0
+                           LA<F>::blkinv(s, n, R.G0, R.inv0, R.W0))
+                        : 0;
   helper.join();
   if (info) {
     bq::set_error("single-object reduction: G of the open region is singular or not finite (info %d)", info);
@@ -790,7 +818,7 @@ int red_factor_t(bq_sess* S)
     bq::set_error("single-object reduction: Schur complement is singular (info %d)", info);
     return info > 0 ? -3 : info;
   }
-  return 0;
+  return LA<F>::blkinv(s, n, R.B, R.invB, R.WB);
 }
 
 // ||A||_inf of the unreduced operator from exact row sums
@@ -849,7 +877,7 @@ void reduced_free(bq_sess* S)
   for (void* p : {R.G0, R.G1, R.Mx, R.P, R.B, R.Binv, (void*)R.ip0, (void*)R.ip1, (void*)R.ipB, R.inv0, R.inv1,
                   R.invB, (void*)R.ginv[0], (void*)R.ginv[1], (void*)R.gidx[0], (void*)R.gidx[1], (void*)R.nn3,
                   (void*)R.Uig, (void*)R.dUig, (void*)R.rownorm, (void*)R.b, (void*)R.x, (void*)R.r, (void*)R.t,
-                  (void*)R.loc, (void*)R.nloc, (void*)R.part, R.vf, (void*)R.Vb, (void*)R.Zb})
+                  (void*)R.loc, (void*)R.nloc, (void*)R.part, R.vf, (void*)R.Vb, (void*)R.Zb, R.W0, R.W1, R.WB})
     f(p);
   R = bq_reduced();
 }
@@ -1056,10 +1084,11 @@ int reduced_deh(bq_sess* S, int d, const double* EH, double* dEH)
   const void* Gf = d ? R.G1 : R.G0;
   const int* ip = d ? R.ip1 : R.ip0;
   const void* inv = d ? R.inv1 : R.inv0;
+  const void* Wd = d ? R.W1 : R.W0;
   const long n6 = 6L * n;
   if (R.prec == 0) {
     BQ_HIP(hipMemcpyAsync(xg, bg, (size_t)n6 * 16, hipMemcpyDeviceToDevice, s));
-    if (LA<double>::getrs(s, n, 6, Gf, ip, inv, xg)) return -1;
+    if (LA<double>::getrs_w(s, n, 6, Gf, ip, inv, Wd, xg)) return -1;
   } else {
     // FP32 factors + refinement with the FP64 G_d: x += G^-1 (b - G x)
     float2* vf = (float2*)R.vf;
@@ -1069,7 +1098,7 @@ int reduced_deh(bq_sess* S, int d, const double* EH, double* dEH)
     for (int it = 0; it < 10; ++it) {
       k_cvt<float2><<<nblk(n6, 256), 256, 0, s>>>(n6, rg, vf);
       BQ_HIP(hipGetLastError());
-      if (LA<float>::getrs(s, n, 6, Gf, ip, inv, vf)) return -1;
+      if (LA<float>::getrs_w(s, n, 6, Gf, ip, inv, Wd, vf)) return -1;
       k_cvt_back<float2><<<nblk(n6, 256), 256, 0, s>>>(n6, vf, rg);
       k_vaxpy<<<nblk(n6, 256), 256, 0, s>>>(n6, rg, xg);
       BQ_HIP(hipGetLastError());

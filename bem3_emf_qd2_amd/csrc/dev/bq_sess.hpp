@@ -22,6 +22,9 @@ struct bq_reduced {
   void* Binv = nullptr;     // B^-1
   int *ip0 = nullptr, *ip1 = nullptr, *ipB = nullptr;
   void *inv0 = nullptr, *inv1 = nullptr, *invB = nullptr;
+  // inverted 512-row diagonal blocks of the G0, G1 and final (Schur) factors for the few-column
+  // solves of the preconditioner and of dE/dH (null when n is not a multiple of 512)
+  void *W0 = nullptr, *W1 = nullptr, *WB = nullptr;
   int* ginv[2] = {nullptr, nullptr};  // global node -> local row of domain d
   int* gidx[2] = {nullptr, nullptr};  // local row of domain d -> global node (-1: none)
   double* nn3 = nullptr;    // unit normals per global node [n][3]

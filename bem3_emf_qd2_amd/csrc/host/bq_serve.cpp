@@ -105,9 +105,11 @@ double estimate_bytes(const Mesh& m, int precision)
   }
   const double n = m.NN;
   if (m.MN == 1) {
-    // single object: six n x n factors and work matrices, and the FGMRES and preconditioner
-    // vectors
-    b += work + 6.0 * (precision == 1 ? 8.0 : 16.0) * n * n + 8000.0 * n;
+    // single object: six n x n factors and work matrices, the inverted 512-row diagonal blocks
+    // of three factors (3 x 2 x 512 n, when n is a multiple of 512), and the FGMRES and
+    // preconditioner vectors
+    const double blk = (m.NN % 512 == 0) ? 3072.0 * n : 0.0;
+    b += work + (6.0 * n * n + blk) * (precision == 1 ? 8.0 : 16.0) + 8000.0 * n;
   } else {
     // block elimination of order 2n: M, M3 (2n x 2n), R, L (2n x n), W (n x n) in FP64, and
     // their FP32 copies in the mixed solve

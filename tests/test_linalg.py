@@ -236,6 +236,65 @@ def test_gpu_getrs_column_groups(gpu_lib, single):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("single,n", [(0, 1024), (1, 1024), (1, 1536)])
+def test_gpu_getrs_through_block_inverses(gpu_lib, single, n):
+    """Few-column solves through the inverted 512-row diagonal blocks of both factors
+    (bq_[zc]lu_blkinv_dev, bq_[zc]getrs_blkinv_dev; the preconditioner and dE/dH solves of the
+    single-object reduction) against the plain solve with the same factors, for 1 to 8
+    right-hand sides on a system whose anti-diagonal forces interchanges: 1e-12 (FP64) and 1e-5
+    (FP32) of the largest entry, and the plain solve against numpy.  An order that is not a
+    multiple of 512 has no block inverses and is refused."""
+    import ctypes
+    from hipmem import DeviceArray
+    L = gpu_lib.lib()
+    rng = np.random.default_rng(17 + n + single)
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)) + 8.0 * np.sqrt(n) * np.eye(n)[::-1]
+    cdt, rdt = (np.complex64, np.float32) if single else (np.complex128, np.float64)
+    V = ctypes.c_void_p
+    L.bq_lu_inv_doubles.restype = ctypes.c_long
+    L.bq_lu_blkinv_elems.restype = ctypes.c_long
+    L.bq_lu_blkinv_elems.argtypes = [ctypes.c_int]
+    assert L.bq_lu_blkinv_elems(1000) == 0
+    nw = L.bq_lu_blkinv_elems(n)
+    assert nw == 2 * n * 512
+    getrf = L.bq_cgetrf_dev if single else L.bq_zgetrf_dev
+    getrs = L.bq_cgetrs_dev if single else L.bq_zgetrs_dev
+    blkinv = L.bq_clu_blkinv_dev if single else L.bq_zlu_blkinv_dev
+    getrs_w = L.bq_cgetrs_blkinv_dev if single else L.bq_zgetrs_blkinv_dev
+    getrf.argtypes = [V, ctypes.c_int, V, ctypes.c_long, V, V]
+    getrs.argtypes = [V, ctypes.c_int, ctypes.c_int, V, ctypes.c_long, V, V, V, ctypes.c_long]
+    blkinv.argtypes = [V, ctypes.c_int, V, ctypes.c_long, V, V]
+    getrs_w.argtypes = [V, ctypes.c_int, ctypes.c_int, V, ctypes.c_long, V, V, V, ctypes.c_long]
+    dA = DeviceArray(np.asfortranarray(A).T.astype(cdt))
+    dinv = DeviceArray(np.zeros(L.bq_lu_inv_doubles(n), rdt))
+    dip = DeviceArray(np.zeros(n, np.int32))
+    dW = DeviceArray(np.zeros(nw, cdt))
+    assert getrf(None, n, dA.ptr, n, dip.ptr, dinv.ptr) == 0
+    assert blkinv(None, n, dA.ptr, n, dinv.ptr, dW.ptr) == 0
+    worst = 0.0
+    for nrhs in (1, 4, 6, 8):
+        B = rng.standard_normal((n, nrhs)) + 1j * rng.standard_normal((n, nrhs))
+        dB1 = DeviceArray(np.asfortranarray(B).T.astype(cdt))
+        dB2 = DeviceArray(np.asfortranarray(B).T.astype(cdt))
+        assert getrs(None, n, nrhs, dA.ptr, n, dip.ptr, dinv.ptr, dB1.ptr, n) == 0
+        assert getrs_w(None, n, nrhs, dA.ptr, n, dinv.ptr, dW.ptr, dB2.ptr, n) == 0
+        X1, X2 = dB1.get().T, dB2.get().T
+        dB1.free()
+        dB2.free()
+        err = np.abs(X1 - X2).max() / np.abs(X1).max()
+        ref = np.linalg.solve(A, B)
+        fw = np.abs(X1 - ref).max() / np.abs(ref).max()
+        print("single" if single else "double", n, nrhs, "block inverses vs plain", err, "plain vs numpy", fw)
+        assert fw < (1e-4 if single else 1e-10)
+        worst = max(worst, err)
+    dB = DeviceArray(np.zeros((9, n), cdt))
+    assert getrs_w(None, n, 9, dA.ptr, n, dinv.ptr, dW.ptr, dB.ptr, n) != 0  # more than 8 columns
+    for d in (dA, dinv, dip, dW, dB):
+        d.free()
+    assert worst < (1e-5 if single else 1e-12)
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("m,n", [(512, 3000), (200, 77)])
 def test_gpu_cgemm_split_store_in_place(gpu_lib, m, n):
     """bq_cgemm_split_store_dev with B == C (C <- -A C in place, as the solves apply their
