@@ -140,10 +140,14 @@ constexpr int kTsFan = kTsRows / kNB;  // candidate sets merged per workgroup
 // cout[blk][nb] receives the chosen rows.  When gridDim.x == 1 the call is the last level:
 // F (nb x nb, column-major) receives the eliminated pivot rows (unit L below the diagonal,
 // U on and above) and ipiv[j..j+jb) the LAPACK-style interchanges for those rows.
+// Four wavefronts, one row of nb columns per lane: a column step is an argmax within each
+// wavefront, one barrier to combine the four candidates (ties to the lowest row), the pivot
+// row through LDS and a second barrier; every lane then forms its own multiplier, so the step
+// needs no exchange of multipliers.
 template <class R>
-__global__ void __launch_bounds__(1024) k_tslu(int mode, int n, const typename P<R>::V* __restrict__ A, long lda, int j, int jb,
-                                               const int* __restrict__ cin, int nin, int* __restrict__ cout,
-                                               typename P<R>::V* __restrict__ F, int* __restrict__ ipiv)
+__global__ void __launch_bounds__(256) k_tslu(int mode, int n, const typename P<R>::V* __restrict__ A, long lda, int j, int jb,
+                                              const int* __restrict__ cin, int nin, int* __restrict__ cout,
+                                              typename P<R>::V* __restrict__ F, int* __restrict__ ipiv)
 {
   using V = typename P<R>::V;
   using C = typename P<R>::C;
@@ -151,47 +155,39 @@ __global__ void __launch_bounds__(1024) k_tslu(int mode, int n, const typename P
   __shared__ R vbuf[4];
   __shared__ int ibuf[4];
   __shared__ V prow[kNB];
-  __shared__ V lbuf[kTsRows];
   __shared__ int order[kNB];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int g = wave >> 2, q = wave & 3;  // row group, column group
-  const int slot = g * 64 + lane;
   int row_id = -1;
   if (mode == 0) {
-    const int r = j + blockIdx.x * kTsRows + slot;
+    const int r = j + blockIdx.x * kTsRows + tid;
     row_id = r < n ? r : -1;
   } else {
-    const int set = kTsFan * blockIdx.x + slot / kNB, k = slot % kNB;
+    const int set = kTsFan * blockIdx.x + tid / kNB, k = tid % kNB;
     row_id = (set < nin && k < jb) ? cin[set * kNB + k] : -1;
   }
-  C a[8];
+  C a[kNB];
 #pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    const int col = q * 8 + k;
-    a[k] = (row_id >= 0 && col < jb) ? T::c(A[(long)(j + col) * lda + row_id]) : T::mkc(0.0, 0.0);
-  }
+  for (int col = 0; col < kNB; ++col)
+    a[col] = (row_id >= 0 && col < jb) ? T::c(A[(long)(j + col) * lda + row_id]) : T::mkc(0.0, 0.0);
   bool live = row_id >= 0;
   int mystep = -1;
 #pragma unroll
   for (int c = 0; c < kNB; ++c) {
     if (c < jb) {
-      const int qc = c / 8, kc = c % 8;
-      if (q == qc) {  // column owners: argmax over their 64 rows
-        R v = live ? fabs(a[kc].re) + fabs(a[kc].im) : -1.0;
-        int idx = live ? row_id : 0x7fffffff;
+      R v = live ? fabs(a[c].re) + fabs(a[c].im) : -1.0;
+      int idx = live ? row_id : 0x7fffffff;
 #pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) {
-          const R ov = __shfl_xor(v, off);
-          const int oi = __shfl_xor(idx, off);
-          if (ov > v || (ov == v && oi < idx)) {
-            v = ov;
-            idx = oi;
-          }
+      for (int off = 32; off >= 1; off >>= 1) {
+        const R ov = __shfl_xor(v, off);
+        const int oi = __shfl_xor(idx, off);
+        if (ov > v || (ov == v && oi < idx)) {
+          v = ov;
+          idx = oi;
         }
-        if (lane == 0) {
-          vbuf[g] = v;
-          ibuf[g] = idx;
-        }
+      }
+      if (lane == 0) {
+        vbuf[wave] = v;
+        ibuf[wave] = idx;
       }
       __syncthreads();
       R pv = vbuf[0];
@@ -203,28 +199,20 @@ __global__ void __launch_bounds__(1024) k_tslu(int mode, int n, const typename P
           pi = ibuf[h];
         }
       if (tid == 0) order[c] = pv < 0.0 ? -1 : pi;
-      const bool me = pv >= 0.0 && live && row_id == pi;
-      if (me) {
+      if (pv >= 0.0 && live && row_id == pi) {
 #pragma unroll
-        for (int k = 0; k < 8; ++k) prow[q * 8 + k] = T::v(a[k].re, a[k].im);
+        for (int k = 0; k < kNB; ++k) prow[k] = T::v(a[k].re, a[k].im);
         live = false;
         mystep = c;
       }
       __syncthreads();
-      if (pv >= 0.0) {
-        if (q == qc && live) {
-          const C piv = T::c(prow[c]);
-          const C l = (piv.re != 0.0 || piv.im != 0.0) ? T::div(a[kc], piv) : T::mkc(0.0, 0.0);
-          a[kc] = l;
-          lbuf[slot] = T::v(l.re, l.im);
-        }
-        __syncthreads();
-        if (live) {
-          const C l = T::c(lbuf[slot]);
+      if (pv >= 0.0 && live) {
+        const C piv = T::c(prow[c]);
+        const C l = (piv.re != 0.0 || piv.im != 0.0) ? T::div(a[c], piv) : T::mkc(0.0, 0.0);
+        a[c] = l;
 #pragma unroll
-          for (int k = 0; k < 8; ++k)
-            if (q * 8 + k > c) a[k] = a[k] - l * T::c(prow[q * 8 + k]);
-        }
+        for (int k = 0; k < kNB; ++k)
+          if (k > c) a[k] = a[k] - l * T::c(prow[k]);
       }
     }
   }
@@ -233,7 +221,7 @@ __global__ void __launch_bounds__(1024) k_tslu(int mode, int n, const typename P
   if (gridDim.x == 1) {
     if (mystep >= 0) {
 #pragma unroll
-      for (int k = 0; k < 8; ++k) F[(q * 8 + k) * kNB + mystep] = T::v(a[k].re, a[k].im);
+      for (int k = 0; k < kNB; ++k) F[k * kNB + mystep] = T::v(a[k].re, a[k].im);
     }
     if (tid == 0) {
       int pos[kNB];
@@ -908,10 +896,10 @@ int getrf(hipStream_t s, int n, typename P<R>::V* A, long lda, int* ipiv, typena
       int nset = (n - j + kTsRows - 1) / kTsRows;
       int* cur = cand;
       int* nxt = cand + (size_t)nleaf * kNB;
-      k_tslu<R><<<nset, 1024, 0, s>>>(0, n, A, lda, j, jb, nullptr, 0, cur, F, ipiv);
+      k_tslu<R><<<nset, kTsRows, 0, s>>>(0, n, A, lda, j, jb, nullptr, 0, cur, F, ipiv);
       while (nset > 1) {
         const int nout = (nset + kTsFan - 1) / kTsFan;
-        k_tslu<R><<<nout, 1024, 0, s>>>(1, n, A, lda, j, jb, cur, nset, nxt, F, ipiv);
+        k_tslu<R><<<nout, kTsRows, 0, s>>>(1, n, A, lda, j, jb, cur, nset, nxt, F, ipiv);
         int* t = cur;
         cur = nxt;
         nxt = t;
