@@ -124,10 +124,9 @@ constexpr int kBlkElems = 2 * kNB * kNB;  // inv(L11) then inv(U11), complex, pe
 
 
 // ---- panel: tournament pivoting ---------------------------------------------------------
-// The m x nb panel is split into blocks of 256 rows.  One 1024-thread workgroup runs partial
-// pivoting on a block held in registers -- 4 row groups of 64 rows (one row per lane) times
-// 4 column groups of 8 columns (one wavefront each) -- and keeps only the nb chosen rows
-// (leaf level).  Up to 8 candidate sets are then reduced the same way on their original rows
+// The m x nb panel is split into blocks of 256 rows.  One 256-thread workgroup runs partial
+// pivoting on a block held in registers (one row of nb columns per lane) and keeps only the
+// nb chosen rows (leaf level).  Up to 8 candidate sets are then reduced the same way on their original rows
 // (merge levels) until nb rows remain; they become the panel's pivots in the order the last
 // reduction chose them, and that reduction's eliminated rows are the factors L11, U11 of the
 // diagonal block.  This replaces a column-by-column pivot search over the whole panel (a
@@ -136,14 +135,74 @@ constexpr int kBlkElems = 2 * kNB * kNB;  // inv(L11) then inv(U11), complex, pe
 constexpr int kTsRows = 256;  // rows per tournament workgroup
 constexpr int kTsFan = kTsRows / kNB;  // candidate sets merged per workgroup
 
+// Wave-wide maximum (every lane receives it): four DPP steps give each 16-lane row its maximum,
+// then the four rows' values are read as scalars.
+template <int CTRL>
+__device__ __forceinline__ float dpp_mov(float x)
+{
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double x)
+{
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ float rd_lane(float x, int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l)); }
+__device__ __forceinline__ double rd_lane(double x, int l)
+{
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane((int)b, l), hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+template <class R>
+__device__ __forceinline__ R wave_max_all(R x)
+{
+  x = fmax(x, dpp_mov<0xB1>(x));   // quad permutation [1,0,3,2]
+  x = fmax(x, dpp_mov<0x4E>(x));   // quad permutation [2,3,0,1]
+  x = fmax(x, dpp_mov<0x141>(x));  // half-row mirror: the two quads of each 8 lanes
+  x = fmax(x, dpp_mov<0x140>(x));  // row mirror: the two halves of each 16 lanes
+  return fmax(fmax(rd_lane(x, 0), rd_lane(x, 16)), fmax(rd_lane(x, 32), rd_lane(x, 48)));
+}
+
+// 1/p with p scaled by a power of two so that |p|^2 neither overflows nor underflows (zero for
+// p = 0): the multipliers of a column are a * (1/p), as LAPACK's cgetf2/zgetf2 scale by the
+// reciprocal of the pivot
+__device__ __forceinline__ cplxf recip_c(cplxf p)
+{
+  const float m = fmaxf(fabsf(p.re), fabsf(p.im));
+  if (!(m > 0.f)) return cplxf{0.f, 0.f};
+  int e;
+  (void)frexpf(m, &e);
+  const float r = ldexpf(p.re, -e), i = ldexpf(p.im, -e);
+  const float d = __builtin_amdgcn_rcpf(fmaf(r, r, i * i));
+  return cplxf{ldexpf(r * d, -e), ldexpf(-i * d, -e)};
+}
+__device__ __forceinline__ cplx recip_c(cplx p)
+{
+  const double m = fmax(fabs(p.re), fabs(p.im));
+  if (!(m > 0.0)) return mk(0.0, 0.0);
+  int e;
+  (void)frexp(m, &e);
+  const double r = ldexp(p.re, -e), i = ldexp(p.im, -e), q = fma(r, r, i * i);
+  double d = __builtin_amdgcn_rcp(q);  // estimate refined by two Newton steps
+  d = fma(d, fma(-q, d, 1.0), d);
+  d = fma(d, fma(-q, d, 1.0), d);
+  return mk(ldexp(r * d, -e), ldexp(-i * d, -e));
+}
+
 // mode 0 (leaf): rows j + 256 blk + slot.  mode 1 (merge): slot -> set 8 blk + slot / nb.
 // cout[blk][nb] receives the chosen rows.  When gridDim.x == 1 the call is the last level:
 // F (nb x nb, column-major) receives the eliminated pivot rows (unit L below the diagonal,
 // U on and above) and ipiv[j..j+jb) the LAPACK-style interchanges for those rows.
-// Four wavefronts, one row of nb columns per lane: a column step is an argmax within each
-// wavefront, one barrier to combine the four candidates (ties to the lowest row), the pivot
-// row through LDS and a second barrier; every lane then forms its own multiplier, so the step
-// needs no exchange of multipliers.
+// Four wavefronts, one row of nb columns per lane.  A column step: the wavefront's maximum of
+// |re| + |im| by DPP, its first lane holding it writes its value, row number and the row's
+// remaining columns to LDS, one barrier, and every lane takes the largest of the four (ties to
+// the lowest wavefront), forms its own multiplier and updates its row.  The LDS slots alternate
+// between two buffers, so the next column's writes need no second barrier.  Ties go to the
+// lowest thread: the lowest row in a leaf block.
 template <class R>
 __global__ void __launch_bounds__(256) k_tslu(int mode, int n, const typename P<R>::V* __restrict__ A, long lda, int j, int jb,
                                               const int* __restrict__ cin, int nin, int* __restrict__ cout,
@@ -152,10 +211,9 @@ __global__ void __launch_bounds__(256) k_tslu(int mode, int n, const typename P<
   using V = typename P<R>::V;
   using C = typename P<R>::C;
   using T = P<R>;
-  __shared__ R vbuf[4];
-  __shared__ int ibuf[4];
-  __shared__ V prow[kNB];
-  __shared__ int order[kNB];
+  __shared__ R vbuf[2][4];
+  __shared__ int ibuf[2][4];
+  __shared__ V prow[2][4][kNB];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   int row_id = -1;
   if (mode == 0) {
@@ -170,70 +228,64 @@ __global__ void __launch_bounds__(256) k_tslu(int mode, int n, const typename P<
   for (int col = 0; col < kNB; ++col)
     a[col] = (row_id >= 0 && col < jb) ? T::c(A[(long)(j + col) * lda + row_id]) : T::mkc(0.0, 0.0);
   bool live = row_id >= 0;
-  int mystep = -1;
+  int mystep = -1, myord = -1;
 #pragma unroll
   for (int c = 0; c < kNB; ++c) {
     if (c < jb) {
-      R v = live ? fabs(a[c].re) + fabs(a[c].im) : -1.0;
-      int idx = live ? row_id : 0x7fffffff;
+      const int b = c & 1;
+      const R v = live ? fabs(a[c].re) + fabs(a[c].im) : R(-1.0);
+      const R wm = wave_max_all(v);
+      const unsigned long long hit = __ballot(v == wm);
+      const int wl = hit ? __ffsll((long long)hit) - 1 : 0;
+      if (lane == wl) {
+        vbuf[b][wave] = v;
+        ibuf[b][wave] = row_id;
 #pragma unroll
-      for (int off = 32; off >= 1; off >>= 1) {
-        const R ov = __shfl_xor(v, off);
-        const int oi = __shfl_xor(idx, off);
-        if (ov > v || (ov == v && oi < idx)) {
-          v = ov;
-          idx = oi;
-        }
-      }
-      if (lane == 0) {
-        vbuf[wave] = v;
-        ibuf[wave] = idx;
+        for (int k = 0; k < kNB; ++k)
+          if (k >= c) prow[b][wave][k] = T::v(a[k].re, a[k].im);
       }
       __syncthreads();
-      R pv = vbuf[0];
-      int pi = ibuf[0];
+      R pv = vbuf[b][0];
+      int pw = 0;
 #pragma unroll
       for (int h = 1; h < 4; ++h)
-        if (vbuf[h] > pv || (vbuf[h] == pv && ibuf[h] < pi)) {
-          pv = vbuf[h];
-          pi = ibuf[h];
+        if (vbuf[b][h] > pv) {
+          pv = vbuf[b][h];
+          pw = h;
         }
-      if (tid == 0) order[c] = pv < 0.0 ? -1 : pi;
+      const int pi = ibuf[b][pw];
+      if (wave == 0 && lane == c) myord = pv >= 0.0 ? pi : -1;
       if (pv >= 0.0 && live && row_id == pi) {
-#pragma unroll
-        for (int k = 0; k < kNB; ++k) prow[k] = T::v(a[k].re, a[k].im);
         live = false;
         mystep = c;
       }
-      __syncthreads();
       if (pv >= 0.0 && live) {
-        const C piv = T::c(prow[c]);
-        const C l = (piv.re != 0.0 || piv.im != 0.0) ? T::div(a[c], piv) : T::mkc(0.0, 0.0);
+        const C l = a[c] * recip_c(T::c(prow[b][pw][c]));
         a[c] = l;
 #pragma unroll
         for (int k = 0; k < kNB; ++k)
-          if (k > c) a[k] = a[k] - l * T::c(prow[k]);
+          if (k > c) a[k] = a[k] - l * T::c(prow[b][pw][k]);
       }
     }
   }
-  __syncthreads();
-  if (tid < kNB) cout[blockIdx.x * kNB + tid] = tid < jb ? order[tid] : -1;
+  if (wave == 0 && lane < kNB) cout[blockIdx.x * kNB + lane] = myord;
   if (gridDim.x == 1) {
     if (mystep >= 0) {
 #pragma unroll
       for (int k = 0; k < kNB; ++k) F[k * kNB + mystep] = T::v(a[k].re, a[k].im);
     }
-    if (tid == 0) {
-      int pos[kNB];
-      for (int c = 0; c < kNB; ++c) pos[c] = c < jb ? order[c] : -1;
+    if (wave == 0) {
+      // interchanges: step c swaps row j + c with the current place of its pivot row; lane c2
+      // follows where the pivot row of step c2 sits
+      int pos = myord, mine = 0;
       for (int c = 0; c < jb; ++c) {
         const int t = j + c;
-        const int p = pos[c] >= 0 ? pos[c] : t;
-        ipiv[t] = p;
-        if (p != t)
-          for (int c2 = c + 1; c2 < jb; ++c2)
-            if (pos[c2] == t) pos[c2] = p;
+        const int pc = __builtin_amdgcn_readlane(pos, c);
+        const int p = pc >= 0 ? pc : t;
+        if (lane == c) mine = p;
+        if (p != t && lane > c && pos == t) pos = p;
       }
+      if (lane < jb) ipiv[j + lane] = mine;
     }
   }
 }
