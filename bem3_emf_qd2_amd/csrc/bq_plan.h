@@ -54,6 +54,12 @@ typedef struct bq_qplan {
 #ifndef BQ_COST_SPAT
 #define BQ_COST_SPAT 6.1
 #endif
+/* Weight of the Fourier-domain term in the wavefront-uniform form, where a wavefront takes the
+ * Fourier-domain sum only when every lane's own choice is it.  Timed on the 6144-quad sphere's
+ * assembly (k_q0_gl per point): 0.4 3.36 s, 0.55 3.25 s, 0.7 3.22 s, 1.0 3.24 s, 1.4 3.38 s. */
+#ifndef BQ_COST_FD_WAVE
+#define BQ_COST_FD_WAVE 0.7
+#endif
 
 /* Fill a plan from the reference's QPD2 fields (k, vk[3], vd1[2], vd2[2]).
  * Returns 0, or -1 if the lattice is degenerate. */

@@ -686,7 +686,7 @@ __device__ void qpgf_v1_wave(const bq_qplan& P, double rx, double ry, double rz,
     const double n_sn = kPI * (P.k2 + P.cut / (ve * ve)) / P.rcell;
     const double rs2 = 4.0 * ve * ve * (P.k2 * ve * ve + P.cut) - pz * pz;
     const double n_sp = rs2 > 0.0 ? kPI * rs2 / P.cell : 0.0;
-    fd = BQ_COST_FD * n_fd < BQ_COST_SPEC * n_sn + BQ_COST_SPAT * n_sp;
+    fd = BQ_COST_FD_WAVE * n_fd < BQ_COST_SPEC * n_sn + BQ_COST_SPAT * n_sp;
   }
   if (__all(fd)) qpgf_fourier_w(P, px, py, pz, o);
   else qpgf_ewald_w(P, px, py, pz, ve, o, z0_ref);
