@@ -173,23 +173,22 @@ __device__ __forceinline__ R wave_max_all(R x)
 __device__ __forceinline__ cplxf recip_c(cplxf p)
 {
   const float m = fmaxf(fabsf(p.re), fabsf(p.im));
-  if (!(m > 0.f)) return cplxf{0.f, 0.f};
   int e;
   (void)frexpf(m, &e);
   const float r = ldexpf(p.re, -e), i = ldexpf(p.im, -e);
-  const float d = __builtin_amdgcn_rcpf(fmaf(r, r, i * i));
+  const float d = m > 0.f ? __builtin_amdgcn_rcpf(fmaf(r, r, i * i)) : 0.f;
   return cplxf{ldexpf(r * d, -e), ldexpf(-i * d, -e)};
 }
 __device__ __forceinline__ cplx recip_c(cplx p)
 {
   const double m = fmax(fabs(p.re), fabs(p.im));
-  if (!(m > 0.0)) return mk(0.0, 0.0);
   int e;
   (void)frexp(m, &e);
-  const double r = ldexp(p.re, -e), i = ldexp(p.im, -e), q = fma(r, r, i * i);
+  const double r = ldexp(p.re, -e), i = ldexp(p.im, -e), q = m > 0.0 ? fma(r, r, i * i) : 1.0;
   double d = __builtin_amdgcn_rcp(q);  // estimate refined by two Newton steps
   d = fma(d, fma(-q, d, 1.0), d);
   d = fma(d, fma(-q, d, 1.0), d);
+  if (!(m > 0.0)) d = 0.0;
   return mk(ldexp(r * d, -e), ldexp(-i * d, -e));
 }
 
@@ -245,26 +244,41 @@ __global__ void __launch_bounds__(256) k_tslu(int mode, int n, const typename P<
           if (k >= c) prow[b][wave][k] = T::v(a[k].re, a[k].im);
       }
       __syncthreads();
-      R pv = vbuf[b][0];
-      int pw = 0;
+      R vv[4];
+      int iv[4];
+#pragma unroll
+      for (int h = 0; h < 4; ++h) {
+        vv[h] = vbuf[b][h];
+        iv[h] = ibuf[b][h];
+      }
+      R pv = vv[0];
+      int pw = 0, pi = iv[0];
 #pragma unroll
       for (int h = 1; h < 4; ++h)
-        if (vbuf[b][h] > pv) {
-          pv = vbuf[b][h];
+        if (vv[h] > pv) {
+          pv = vv[h];
           pw = h;
+          pi = iv[h];
         }
-      const int pi = ibuf[b][pw];
+      // the pivot row (the same LDS address for every lane)
+      V pr[kNB];
+#pragma unroll
+      for (int k = 0; k < kNB; ++k)
+        if (k >= c) pr[k] = prow[b][pw][k];
       if (wave == 0 && lane == c) myord = pv >= 0.0 ? pi : -1;
       if (pv >= 0.0 && live && row_id == pi) {
         live = false;
         mystep = c;
       }
       if (pv >= 0.0 && live) {
-        const C l = a[c] * recip_c(T::c(prow[b][pw][c]));
+        const C l = a[c] * recip_c(T::c(pr[c]));
         a[c] = l;
 #pragma unroll
         for (int k = 0; k < kNB; ++k)
-          if (k > c) a[k] = a[k] - l * T::c(prow[b][pw][k]);
+          if (k > c) {  // a[k] -= l pr[k]
+            a[k].re = fma(-l.re, pr[k].x, fma(l.im, pr[k].y, a[k].re));
+            a[k].im = fma(-l.re, pr[k].y, fma(-l.im, pr[k].x, a[k].im));
+          }
       }
     }
   }
@@ -486,6 +500,15 @@ __global__ void __launch_bounds__(256) k_gather_rows(int n, int nrhs, const int*
   if (q >= (long)n * nrhs) return;
   const int r = (int)(q % n), c = (int)(q / n);
   Bt[(long)c * n + r] = B[(long)c * ldb + perm[r]];
+}
+
+template <class R>
+__global__ void k_identity(int n, typename P<R>::V* __restrict__ X, long ldx)
+{
+  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (long)n * n) return;
+  const long c = q / n, r = q - c * n;
+  X[c * ldx + r] = P<R>::v(r == c ? 1.0 : 0.0, 0.0);
 }
 
 template <class R>
@@ -1024,20 +1047,28 @@ int getrf(hipStream_t s, int n, typename P<R>::V* A, long lda, int* ipiv, typena
 }
 
 // Solve A X = B with the factors from getrf; B is n x nrhs column-major (device).
+// With ident, B holds the n x n identity, unpermuted: the solve yields U^-1 L^-1 (the inverse
+// with its columns permuted, A^-1[:, perm[i]] = X[:, i]), and the forward sweep of block row
+// j0 .. je-1 touches only columns 0 .. je-1 (column c of L^-1 is zero above row c): n^3 / 6
+// complex multiply-adds instead of n^3 / 2.
 template <class R>
 int getrs(hipStream_t s, int n, int nrhs, const typename P<R>::V* A, long lda, const int* ipiv,
           const typename P<R>::V* inv, typename P<R>::V* B, long ldb, int ng = 0, const int* g0 = nullptr,
-          const int* gn = nullptr, const int* gf = nullptr)
+          const int* gn = nullptr, const int* gf = nullptr, bool ident = false)
 {
   using V = typename P<R>::V;
   const V one = P<R>::v(1.0, 0.0), mone = P<R>::v(-1.0, 0.0);
   if (n <= 0 || nrhs <= 0) return 0;
+  if (ident && (nrhs != n || ng > 0)) {
+    bq::set_error("getrs: the identity form takes n right-hand sides and no column groups");
+    return -1;
+  }
   const int kt_all = bq::ktime_begin(P<R>::kt_s, s);
   const int* perm = perm_of(inv, n);  // stored by getrf (ipiv is kept for the interface)
   (void)ipiv;
   // rows permuted in place, kPermChunk columns at a time through one n x kPermChunk buffer
   // (not a copy of all of B: with n right-hand sides that would double the solve's memory)
-  {
+  if (!ident) {
     const int chunk = min(nrhs, kPermChunk);
     V* Bt = nullptr;
     BQ_HIP(hipMallocAsync((void**)&Bt, sizeof(V) * (size_t)n * chunk, s));
@@ -1096,26 +1127,27 @@ int getrs(hipStream_t s, int n, int nrhs, const typename P<R>::V* A, long lda, c
     const int first = ng > 0 ? (max(0, gf[g]) / kNBF) * kNBF : 0;
     if (nc <= 0) continue;
     V* Bg = B + (long)c0 * ldb;
-    const unsigned gblk = (unsigned)((nc + 63) / 64);
     for (int j0 = first; j0 < n; j0 += kNBF) {
       const int je = min(n, j0 + kNBF);
+      const int ncb = ident ? min(nc, je) : nc;  // columns with nonzeros in this block row
+      const unsigned gblk = (unsigned)((ncb + 63) / 64);
       bool done = false;
       if constexpr (std::is_same<R, float>::value) {
-        if (blk_inv && je - j0 == kNBF && nc >= kSplitMin) {
-          if (apply_block_inverse(s, nc, WL + (size_t)(j0 / kNBF) * kNBF * kNBF, Bg + j0, ldb)) return -1;
+        if (blk_inv && je - j0 == kNBF && ncb >= kSplitMin) {
+          if (apply_block_inverse(s, ncb, WL + (size_t)(j0 / kNBF) * kNBF * kNBF, Bg + j0, ldb)) return -1;
           done = true;
         }
       }
       for (int j = j0; j < je && !done; j += kNB) {
         const int jb = min(kNB, je - j);
-        k_apply_blk<R><<<gblk, 256, 0, s>>>(inv + (long)(j / kNB) * kBlkElems, jb, Bg + j, ldb, nc);
+        k_apply_blk<R><<<gblk, 256, 0, s>>>(inv + (long)(j / kNB) * kBlkElems, jb, Bg + j, ldb, ncb);
         BQ_HIP(hipGetLastError());
-        if (je - j - jb > 0 && gemm<R>(s, je - j - jb, nc, jb, mone, A + (long)j * lda + (j + jb), lda, Bg + j, ldb,
+        if (je - j - jb > 0 && gemm<R>(s, je - j - jb, ncb, jb, mone, A + (long)j * lda + (j + jb), lda, Bg + j, ldb,
                                        one, Bg + j + jb, ldb))
           return -1;
       }
       if (n - je > 0 &&
-          gemm<R>(s, n - je, nc, je - j0, mone, A + (long)j0 * lda + je, lda, Bg + j0, ldb, one, Bg + je, ldb))
+          gemm<R>(s, n - je, ncb, je - j0, mone, A + (long)j0 * lda + je, lda, Bg + j0, ldb, one, Bg + je, ldb))
         return -1;
     }
   }
@@ -1235,6 +1267,29 @@ int bq_zgemm_skinny_dev(void* stream, int m, int nrhs, long k, const double* A, 
 const int* bq_lu_perm_dev(const void* dinv, int n, int single)
 {
   return single ? perm_of((const float2*)dinv, n) : perm_of((const double2*)dinv, n);
+}
+
+// X (n x n, ldx) = U^-1 L^-1 from getrf's factors: the inverse of A with its columns permuted,
+// A^-1[:, perm[i]] = X[:, i] for perm from bq_lu_perm_dev.
+int bq_zlu_inverse_dev(void* stream, int n, const double* Ad, long lda, const double* dinv, double* Xd, long ldx)
+{
+  if (n <= 0) return 0;
+  hipStream_t s = (hipStream_t)stream;
+  const long tot = (long)n * n;
+  k_identity<double><<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(n, (double2*)Xd, ldx);
+  BQ_HIP(hipGetLastError());
+  return getrs<double>(s, n, n, (const double2*)Ad, lda, nullptr, (const double2*)dinv, (double2*)Xd, ldx, 0, nullptr,
+                       nullptr, nullptr, true);
+}
+int bq_clu_inverse_dev(void* stream, int n, const float* Ad, long lda, const float* dinv, float* Xd, long ldx)
+{
+  if (n <= 0) return 0;
+  hipStream_t s = (hipStream_t)stream;
+  const long tot = (long)n * n;
+  k_identity<float><<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(n, (float2*)Xd, ldx);
+  BQ_HIP(hipGetLastError());
+  return getrs<float>(s, n, n, (const float2*)Ad, lda, nullptr, (const float2*)dinv, (float2*)Xd, ldx, 0, nullptr,
+                      nullptr, nullptr, true);
 }
 
 // bq_zgetrs_dev / bq_cgetrs_dev with column groups: group g (columns g0[g] .. g0[g]+gn[g]-1,

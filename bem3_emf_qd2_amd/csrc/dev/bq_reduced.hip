@@ -97,26 +97,23 @@ __global__ void k_add(long len, const F2* __restrict__ a, const F2* __restrict__
   if (i < len) c[i] = mkv<F2>((double)a[i].x + (double)b[i].x, (double)a[i].y + (double)b[i].y);
 }
 
+// S = P + b M - sD (Binv o n n^T), written over S's buffer (which may hold the LU of B), from
+// X = U^-1 L^-1 of B's factors: Binv[:, perm[c]] = X[:, c].  Binv in natural order replaces P
+// (each element of P is read, then overwritten, by the same thread).
 template <class F2>
-__global__ void k_eye(int n, F2* __restrict__ A)
+__global__ void k_schur(int n, F2* __restrict__ P, const F2* __restrict__ M, const F2* __restrict__ X,
+                        const int* __restrict__ perm, const double* __restrict__ nn3, cplx beps, cplx sd,
+                        F2* __restrict__ S)
 {
   const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (long)n * n) return;
-  const long j = idx / n, i = idx - j * n;
-  A[idx] = mkv<F2>(i == j ? 1.0 : 0.0, 0.0);
-}
-
-// S = P + b M - sD (Binv o n n^T), written over S's buffer (which may hold the LU of B)
-template <class F2>
-__global__ void k_schur(int n, const F2* __restrict__ P, const F2* __restrict__ M, const F2* __restrict__ Binv,
-                        const double* __restrict__ nn3, cplx beps, cplx sd, F2* __restrict__ S)
-{
-  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (long)n * n) return;
-  const long j = idx / n, i = idx - j * n;
+  const long c = idx / n, i = idx - c * n, j = perm[c];
+  const long ij = j * n + i;
   const double g = nn3[3 * i] * nn3[3 * j] + nn3[3 * i + 1] * nn3[3 * j + 1] + nn3[3 * i + 2] * nn3[3 * j + 2];
-  const cplx v = ldv(P, idx) + beps * ldv(M, idx) - (sd * ldv(Binv, idx)) * g;
-  stv(S, idx, v);
+  const cplx x = ldv(X, idx);
+  const cplx v = ldv(P, ij) + beps * ldv(M, ij) - (sd * x) * g;
+  stv(S, ij, v);
+  stv(P, ij, x);
 }
 
 template <class F2>
@@ -439,6 +436,11 @@ struct LA<double> {
   {
     return W ? bq_zlu_blkinv_dev(s, n, (const double*)A, n, (const double*)inv, (double*)W) : 0;
   }
+  static int inverse(hipStream_t s, int n, const void* A, const void* inv, void* X)
+  {
+    return bq_zlu_inverse_dev(s, n, (const double*)A, n, (const double*)inv, (double*)X, n);
+  }
+  static const int* perm(const void* inv, int n) { return bq_lu_perm_dev(inv, n, 0); }
   // few columns through the inverted diagonal blocks W when there are any
   static int getrs_w(hipStream_t s, int n, int nrhs, const void* A, const int* ip, const void* inv, const void* W,
                      void* B)
@@ -468,6 +470,11 @@ struct LA<float> {
   {
     return W ? bq_clu_blkinv_dev(s, n, (const float*)A, n, (const float*)inv, (float*)W) : 0;
   }
+  static int inverse(hipStream_t s, int n, const void* A, const void* inv, void* X)
+  {
+    return bq_clu_inverse_dev(s, n, (const float*)A, n, (const float*)inv, (float*)X, n);
+  }
+  static const int* perm(const void* inv, int n) { return bq_lu_perm_dev(inv, n, 1); }
   static int getrs_w(hipStream_t s, int n, int nrhs, const void* A, const int* ip, const void* inv, const void* W,
                      void* B)
   {
@@ -806,13 +813,14 @@ int red_factor_t(bq_sess* S)
     bq::set_error("single-object reduction: P + M is singular (info %d)", info);
     return info > 0 ? -3 : info;
   }
-  k_eye<V><<<nblk(nn, 256), 256, 0, s>>>(n, (V*)R.Binv);
-  BQ_HIP(hipGetLastError());
-  if (LA<F>::getrs(s, n, n, R.B, R.ipB, R.invB, R.Binv)) return -1;
+  // B^-1 with permuted columns, then the Schur complement, which also writes B^-1 in natural
+  // order over P (no longer needed); the two buffers then trade places
+  if (LA<F>::inverse(s, n, R.B, R.invB, R.Binv)) return -1;
   const cplx sd = hmul(k.sigm, k.delt);
-  k_schur<V><<<nblk(nn, 256), 256, 0, s>>>(n, (const V*)R.P, (const V*)R.Mx, (const V*)R.Binv, R.nn3, k.beps, sd,
-                                           (V*)R.B);
+  k_schur<V><<<nblk(nn, 256), 256, 0, s>>>(n, (V*)R.P, (const V*)R.Mx, (const V*)R.Binv, LA<F>::perm(R.invB, n),
+                                           R.nn3, k.beps, sd, (V*)R.B);
   BQ_HIP(hipGetLastError());
+  std::swap(R.P, R.Binv);
   info = LA<F>::getrf(s, n, R.B, R.ipB, R.invB);
   if (info) {
     bq::set_error("single-object reduction: Schur complement is singular (info %d)", info);

@@ -295,6 +295,57 @@ def test_gpu_getrs_through_block_inverses(gpu_lib, single, n):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("single,n", [(0, 700), (0, 1536), (1, 1024), (1, 1300)])
+def test_gpu_lu_inverse(gpu_lib, single, n):
+    """bq_[zc]lu_inverse_dev (B^-1 of the single-object reduction): X = U^-1 L^-1 from the
+    factors, whose forward sweep skips the zeros of L^-1, read through the factorization's
+    permutation (A^-1[:, perm[i]] = X[:, i]) against the plain solve with the identity as
+    right-hand sides (1e-12 FP64, 1e-5 FP32 of the largest entry) and against numpy's inverse;
+    orders below, at and between multiples of 512 (full and partial blocks, both FP32 update
+    paths), on systems whose anti-diagonal forces interchanges."""
+    import ctypes
+    from hipmem import DeviceArray
+    L = gpu_lib.lib()
+    rng = np.random.default_rng(31 + n + single)
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)) + 8.0 * np.sqrt(n) * np.eye(n)[::-1]
+    cdt, rdt = (np.complex64, np.float32) if single else (np.complex128, np.float64)
+    V = ctypes.c_void_p
+    L.bq_lu_inv_doubles.restype = ctypes.c_long
+    getrf = L.bq_cgetrf_dev if single else L.bq_zgetrf_dev
+    getrs = L.bq_cgetrs_dev if single else L.bq_zgetrs_dev
+    inverse = L.bq_clu_inverse_dev if single else L.bq_zlu_inverse_dev
+    getrf.argtypes = [V, ctypes.c_int, V, ctypes.c_long, V, V]
+    getrs.argtypes = [V, ctypes.c_int, ctypes.c_int, V, ctypes.c_long, V, V, V, ctypes.c_long]
+    inverse.argtypes = [V, ctypes.c_int, V, ctypes.c_long, V, V, ctypes.c_long]
+    L.bq_lu_perm_dev.argtypes = [V, ctypes.c_int, ctypes.c_int]
+    L.bq_lu_perm_dev.restype = V
+    dA = DeviceArray(np.asfortranarray(A).T.astype(cdt))
+    dinv = DeviceArray(np.zeros(L.bq_lu_inv_doubles(n), rdt))
+    dip = DeviceArray(np.zeros(n, np.int32))
+    assert getrf(None, n, dA.ptr, n, dip.ptr, dinv.ptr) == 0
+    off = L.bq_lu_perm_dev(dinv.ptr, n, single) - dinv.ptr.value
+    perm = dinv.get().view(np.uint8)[off:off + 4 * n].view(np.int32).copy()
+    assert sorted(perm.tolist()) == list(range(n))
+    assert not np.array_equal(perm, np.arange(n))
+    dX = DeviceArray(np.full((n, n), np.nan, cdt))  # every entry must be written
+    assert inverse(None, n, dA.ptr, n, dinv.ptr, dX.ptr, n) == 0
+    dI = DeviceArray(np.eye(n, dtype=cdt))
+    assert getrs(None, n, n, dA.ptr, n, dip.ptr, dinv.ptr, dI.ptr, n) == 0
+    X, Ainv = dX.get().T, dI.get().T
+    for d in (dA, dinv, dip, dX, dI):
+        d.free()
+    Xn = np.empty_like(X)
+    Xn[:, perm] = X  # A^-1[:, perm[i]] = X[:, i]
+    assert np.isfinite(Xn).all()
+    err = np.abs(Xn - Ainv).max() / np.abs(Ainv).max()
+    ref = np.linalg.inv(A)
+    fw = np.abs(Xn - ref).max() / np.abs(ref).max()
+    print("single" if single else "double", n, "inverse vs plain solve", err, "vs numpy", fw)
+    assert err < (1e-5 if single else 1e-12)
+    assert fw < (1e-4 if single else 1e-10)
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("m,n", [(512, 3000), (200, 77)])
 def test_gpu_cgemm_split_store_in_place(gpu_lib, m, n):
     """bq_cgemm_split_store_dev with B == C (C <- -A C in place, as the solves apply their
