@@ -499,13 +499,29 @@ __device__ void qpgf_v1(const bq_qplan& P, double rx, double ry, double rz, QG& 
 // and a lane adds only the lattice points inside its own disk.  The Faddeeva series take the
 // wavefront's largest term count.  Loop control, order data, the propagating/evanescent choice
 // and the Bloch phases of the spatial sum are then uniform and the lanes stay converged.
+template <int CTRL>
+__device__ __forceinline__ double wave_dpp(double x)
+{
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ double wave_lane(double x, int l)
+{
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane((int)b, l), hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+// maximum over the 64 lanes: four DPP steps give each 16-lane row its maximum (quad
+// permutations [1,0,3,2] and [2,3,0,1], half-row mirror, row mirror), then the rows' values
 __device__ __forceinline__ double wave_max(double v)
 {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
-  const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffff)), hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
-  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+  v = fmax(v, wave_dpp<0xB1>(v));
+  v = fmax(v, wave_dpp<0x4E>(v));
+  v = fmax(v, wave_dpp<0x141>(v));
+  v = fmax(v, wave_dpp<0x140>(v));
+  return fmax(fmax(wave_lane(v, 0), wave_lane(v, 16)), fmax(wave_lane(v, 32), wave_lane(v, 48)));
 }
 __device__ __forceinline__ double wave_first(double v)
 {
