@@ -240,6 +240,9 @@ k_q0_gl(bq_qplan P, const double* __restrict__ E, const double* __restrict__ ren
 {
   __shared__ double sacc[16][256];
   __shared__ double selem[4][kElemStride];
+  __shared__ double setab[kErfcxTab];
+  erfcx_stage(setab);
+  __syncthreads();
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   // one wavefront per (t, s): make the pair index wave-uniform so that the element data of
   // s are fetched with scalar loads and held in scalar registers
@@ -292,7 +295,7 @@ k_q0_gl(bq_qplan P, const double* __restrict__ E, const double* __restrict__ ren
       }
       R[0] -= rt[0]; R[1] -= rt[1]; R[2] -= rt[2];
       QG q;
-      qpgf_v1_wave(P, R[0], R[1], R[2], q);
+      qpgf_v1_wave(P, R[0], R[1], R[2], q, setab);
       if (use) {
         double aW = sqrt(dot3(W, W));
         cplx hn = q.d[0] * W[0] + q.d[1] * W[1] + q.d[2] * W[2];
@@ -331,6 +334,9 @@ k_q0_gh(bq_qplan P, const double* __restrict__ E, const double* __restrict__ ren
 {
   __shared__ double lds[19 * 4];
   __shared__ double sacc[16][256];
+  __shared__ double setab[kErfcxTab];
+  erfcx_stage(setab);
+  __syncthreads();
   const long code = list[blockIdx.x];
   const long N = 4L * dNe, row = code / dNe;
   const int sc = (int)(code % dNe), t = (int)(row / 4) + 1, tn = (int)(row & 3);
@@ -355,7 +361,7 @@ k_q0_gh(bq_qplan P, const double* __restrict__ E, const double* __restrict__ ren
     fs[1] += w * f0;
     fs[2] += w * f1;
     QG q;
-    qpgf_v1_wave(P, R[0], R[1], R[2], q);
+    qpgf_v1_wave(P, R[0], R[1], R[2], q, setab);
     double aW = sqrt(dot3(W, W));
     cplx hn = q.d[0] * W[0] + q.d[1] * W[1] + q.d[2] * W[2];
     shapeN(e.tri, z, et, Nn);

@@ -31,6 +31,43 @@ __device__ __forceinline__ double rcp_nr(double d)
   return fma(r, e, r);
 }
 
+// erfcx(x) = exp(x^2) erfc(x) for x >= 0 from a table in LDS (copied there by erfcx_stage):
+// piecewise degree-7 polynomials in t = 4/(4+x) on 64 equal intervals of [4/54, 1] (x <= 50;
+// coefficients from gen_erfcx_table.py), the asymptotic series 1/(x sqrt(pi)) (1 - 1/(2x^2) +
+// 3/(4x^4) - ...) beyond.  Relative error about 1e-15.  The lanes of a wavefront with nearby
+// arguments read the same coefficients (LDS broadcast): about 40 vector instructions, against
+// about 100 for the device library's erfcx.
+constexpr int kErfcxTab = 64 * 8;
+__device__ const double kErfcxCoef[kErfcxTab] = {
+#include "bq_erfcx_table.inc"
+};
+__device__ __forceinline__ void erfcx_stage(double* lds)
+{
+  for (int i = threadIdx.x; i < kErfcxTab; i += blockDim.x) lds[i] = kErfcxCoef[i];
+}
+__device__ __forceinline__ double erfcx_tab(double x, const double* tab)
+{
+  if (x > 50.0) {
+    const double r = rcp_nr(x), r2 = 0.5 * r * r;
+    return kISQPI * r * (1.0 - r2 * (1.0 - 3.0 * r2 * (1.0 - 5.0 * r2 * (1.0 - 7.0 * r2 * (1.0 - 9.0 * r2)))));
+  }
+  constexpr double t0 = 4.0 / 54.0, sc = 64.0 / (1.0 - t0);
+  const double s = (4.0 * rcp_nr(4.0 + x) - t0) * sc;
+  const int k = max(0, min(63, (int)s));
+  const double u = 2.0 * (s - k) - 1.0;
+  const double* c = tab + 8 * k;
+  double h = c[7];
+#pragma unroll
+  for (int p = 6; p >= 0; --p) h = fma(h, u, c[p]);
+  return h;
+}
+// erfcx on the real line: the table, or the device library's when tab is null
+__device__ __forceinline__ double erfcx_any(double x, const double* tab)
+{
+  if (!tab) return erfcx(x);
+  return x >= 0.0 ? erfcx_tab(x, tab) : 2.0 * exp(x * x) - erfcx_tab(-x, tab);
+}
+
 __device__ __forceinline__ double sinc_of(double x, double sx)
 {
   return (fabs(x) < 1e-4) ? 1.0 - x * x * (1.0 / 6.0) : sx / x;
@@ -51,8 +88,9 @@ __device__ __forceinline__ cplx w_cf_upper(double x, double y)
 __device__ __forceinline__ int w_series_terms(double x) { return min(60, (int)((x + 6.3) * (1.0 / kFA)) + 2); }
 
 // Series of Algorithm 916 for |y| <= 7 and x = |xs| < 10 outside the continued-fraction region,
-// with nterm >= w_series_terms(|xs|) terms (more terms only add ones below the rounding).
-__device__ __forceinline__ cplx w_series_n(double xs, double y, int nterm)
+// with nterm >= w_series_terms(|xs|) terms (more terms only add ones below the rounding);
+// erfcx from the LDS table et (erfcx_tab), or the device library's when et is null.
+__device__ __forceinline__ cplx w_series_n(double xs, double y, int nterm, const double* et = nullptr)
 {
   const double x = fabs(xs);
   double s1 = 0.0, s2 = 0.0, s3 = 0.0, s4 = 0.0, s5 = 0.0;
@@ -76,7 +114,7 @@ __device__ __forceinline__ cplx w_series_n(double xs, double y, int nterm)
     t *= q;
     q *= q2;
   }
-  const double ex2erfcx = (y > -6.0) ? ex2 * erfcx(y) : 2.0 * exp(y2 - x * x);
+  const double ex2erfcx = (y > -6.0) ? ex2 * erfcx_any(y, et) : 2.0 * exp(y2 - x * x);
   double sxy, cxy, s2xy, c2xy;
   sincos_fast(xs * y, &sxy, &cxy);
   s2xy = 2.0 * sxy * cxy;
@@ -131,11 +169,11 @@ __device__ __forceinline__ cplx faddeeva_w(double xs, double y)
 }
 // The same with the series' term count given by the caller (nterm >= w_series_terms(|xs|)), so
 // that lanes of a wavefront whose x differ run one trip count.
-__device__ __forceinline__ cplx faddeeva_w_n(double xs, double y, int nterm)
+__device__ __forceinline__ cplx faddeeva_w_n(double xs, double y, int nterm, const double* et = nullptr)
 {
   const double x = fabs(xs), ya = fabs(y);
   if (xs == 0.0 || x >= 10.0 || w_use_cf(x, ya)) return w_cold(xs, y);
-  return w_series_n(xs, y, nterm);
+  return w_series_n(xs, y, nterm, et);
 }
 
 }  // namespace bq

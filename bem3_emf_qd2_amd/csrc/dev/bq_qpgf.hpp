@@ -534,13 +534,13 @@ __device__ __forceinline__ double wave_first(double v)
 // erfcx each: for a = bv +- z/2v <= 0, e^{+-bz} erfc(a) = 2 e^{+-bz} - e12 erfcx(-a), because
 // +-bz - a^2 = -b^2 v^2 - z^2/4v^2; the Faddeeva series take nterm terms.
 __device__ __forceinline__ void spectral_term_w(const bq_qplan& P, double ax, double ay, cplx ce, double pz, double ve,
-                                                double az, int nterm, cplx& F, cplx& Gz)
+                                                double az, int nterm, const double* et, cplx& F, cplx& Gz)
 {
   const double b2 = P.k2 - ax * ax - ay * ay;
   if (b2 > 0.0) {
     const double b = sqrt(b2), ef = exp(b2 * ve * ve - az * az);
     const cplx ff = cexpi(b * pz);
-    const cplx w = faddeeva_w_n(b * ve, az, nterm);
+    const cplx w = faddeeva_w_n(b * ve, az, nterm, et);
     F = ce * (mk(ff.re, ef * w.im + ff.im) * (2.0 / b));
     Gz = ce * (mk(ef * w.re - ff.re, -ff.im) * 2.0);
   } else if (b2 == 0.0) {
@@ -550,7 +550,7 @@ __device__ __forceinline__ void spectral_term_w(const bq_qplan& P, double ax, do
     const double b = sqrt(-b2), bv = b * ve;
     const double a1 = bv + az, a2 = bv - az;
     const double e12 = exp(-bv * bv - az * az);
-    double f1 = e12 * erfcx(fabs(a1)), f2 = e12 * erfcx(fabs(a2));
+    double f1 = e12 * erfcx_tab(fabs(a1), et), f2 = e12 * erfcx_tab(fabs(a2), et);
     if (a1 <= 0.0) f1 = 2.0 * exp(b * pz) - f1;
     if (a2 <= 0.0) f2 = 2.0 * exp(-b * pz) - f2;
     F = mk(ce.im, -ce.re) * ((f1 + f2) / b);
@@ -603,7 +603,8 @@ __device__ void qpgf_fourier_w(const bq_qplan& P, double px, double py, double p
   o.d[2] = Sz * (-h * sg);
 }
 
-__device__ void qpgf_ewald_w(const bq_qplan& P, double px, double py, double pz, double ve, QG& o, bool z0_ref)
+__device__ void qpgf_ewald_w(const bq_qplan& P, double px, double py, double pz, double ve, QG& o, bool z0_ref,
+                             const double* et)
 {
   const double ive = 1.0 / ve, ke = P.k * ve, ke2 = ke * ke;
   // ---- spatial sum: lane's own disk |r + R_l| <= Rmax, walked over one disk around lane 0
@@ -635,7 +636,7 @@ __device__ void qpgf_ewald_w(const bq_qplan& P, double px, double py, double pz,
           const double rl = sqrt(rl2), irl = 1.0 / rl;
           const double y = 0.5 * rl * ive;
           const cplx cef = Eb * exp(ke2 - y * y);
-          const cplx w = faddeeva_w_n(ke, y, nterm);
+          const cplx w = faddeeva_w_n(ke, y, nterm, et);
           S += cef * (irl * w.re);
           const cplx cf = cef * (irl * irl * (irl * w.re - P.k * w.im + gcon));
           Sxy[0] += cf * ax;
@@ -675,7 +676,7 @@ __device__ void qpgf_ewald_w(const bq_qplan& P, double px, double py, double pz,
       const double bp2 = P.k2 - ax * ax - ay * ay;
       const int nt = bp2 > 0.0 ? w_series_terms(sqrt(bp2) * vemax) : 0;
       cplx F, Gz;
-      spectral_term_w(P, ax, ay, E, pz, ve, az, nt, F, Gz);
+      spectral_term_w(P, ax, ay, E, pz, ve, az, nt, et, F, Gz);
       S += F;
       Sx += F * ax;
       Sy += F * ay;
@@ -689,7 +690,8 @@ __device__ void qpgf_ewald_w(const bq_qplan& P, double px, double py, double pz,
   o.d[2] += Sz * cd;
 }
 
-__device__ void qpgf_v1_wave(const bq_qplan& P, double rx, double ry, double rz, QG& o, bool z0_ref = true)
+__device__ void qpgf_v1_wave(const bq_qplan& P, double rx, double ry, double rz, QG& o, const double* et,
+                             bool z0_ref = true)
 {
   double l1, l2, px, py;
   fold_cell(P, rx, ry, l1, l2, px, py);
@@ -705,7 +707,7 @@ __device__ void qpgf_v1_wave(const bq_qplan& P, double rx, double ry, double rz,
     fd = BQ_COST_FD_WAVE * n_fd < BQ_COST_SPEC * n_sn + BQ_COST_SPAT * n_sp;
   }
   if (__all(fd)) qpgf_fourier_w(P, px, py, pz, o);
-  else qpgf_ewald_w(P, px, py, pz, ve, o, z0_ref);
+  else qpgf_ewald_w(P, px, py, pz, ve, o, z0_ref, et);
   if (l1 != 0.0 || l2 != 0.0) {
     cplx cf = cexpi(-(P.vk0 * (l1 * P.d1x + l2 * P.d2x) + P.vk1 * (l1 * P.d1y + l2 * P.d2y)));
     o.g = o.g * cf;

@@ -173,6 +173,15 @@ __global__ void __launch_bounds__(256) k_faddeeva(int n, const double* __restric
   w[2 * i + 1] = v.im;
 }
 
+__global__ void __launch_bounds__(256) k_erfcx(int n, const double* __restrict__ x, double* __restrict__ out)
+{
+  __shared__ double tab[bq::kErfcxTab];
+  bq::erfcx_stage(tab);
+  __syncthreads();
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = bq::erfcx_any(x[i], tab);
+}
+
 __global__ void __launch_bounds__(256) k_qpgf_v1(bq_qplan P, int z0_ref, int n, const double* __restrict__ r,
                                                  double* __restrict__ out)
 {
@@ -234,6 +243,26 @@ int bq_faddeeva_w_batch(int n, const double* z, double* w)
   BQ_HIP(hipMemcpyAsync(w, dw, bytes, hipMemcpyDeviceToHost, s));
   BQ_HIP(hipFreeAsync(dz, s));
   BQ_HIP(hipFreeAsync(dw, s));
+  BQ_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+int bq_erfcx_batch(int n, const double* x, double* out)
+{
+  if (n < 0 || (n > 0 && (!x || !out))) { bq::set_error("bq_erfcx_batch: bad arguments"); return -2; }
+  if (n == 0) return 0;
+  if (bq::ensure_device()) return -1;
+  hipStream_t s = bq::stream();
+  double *dx = nullptr, *dy = nullptr;
+  const size_t bytes = sizeof(double) * (size_t)n;
+  BQ_HIP(hipMallocAsync((void**)&dx, bytes, s));
+  BQ_HIP(hipMallocAsync((void**)&dy, bytes, s));
+  BQ_HIP(hipMemcpyAsync(dx, x, bytes, hipMemcpyHostToDevice, s));
+  k_erfcx<<<(n + 255) / 256, 256, 0, s>>>(n, dx, dy);
+  BQ_HIP(hipGetLastError());
+  BQ_HIP(hipMemcpyAsync(out, dy, bytes, hipMemcpyDeviceToHost, s));
+  BQ_HIP(hipFreeAsync(dx, s));
+  BQ_HIP(hipFreeAsync(dy, s));
   BQ_HIP(hipStreamSynchronize(s));
   return 0;
 }

@@ -22,6 +22,13 @@ int bq_device_ok(void);
  * Faddeeva_w (qgf2_src/Faddeeva.h:39, Faddeeva.cc:679), relative accuracy ~1e-15. */
 int bq_faddeeva_w_batch(int n, const double *z, double *w);
 
+/* erfcx(x) = exp(x^2) erfc(x) for n real points, out[n], through the table form that the
+ * assembly's wavefront-uniform lattice sums use in place of the device library's erfcx
+ * (piecewise polynomials in 4/(4+x), generated by csrc/dev/gen_erfcx_table.py; relative
+ * accuracy ~1e-15).  The reference has no separate entry point: its Ewald sums call
+ * Faddeeva::erfcx (qgf2_src/Faddeeva.cc), whose values this reproduces. */
+int bq_erfcx_batch(int n, const double *x, double *out);
+
 /* Quasi-periodic Green's function value and gradient for n relative positions r[n][3].
  * qd8 = the reference's QPD2 {k, vk[3], vd1[2], vd2[2]} as 8 doubles.
  * out[n][8] = qG, dqG/dx, dqG/dy, dqG/dz (complex).  Batched form of d3hm_qpgf_d2_v1
