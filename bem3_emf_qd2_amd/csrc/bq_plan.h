@@ -34,12 +34,24 @@ typedef struct bq_qplan {
   double cell, rcell;      /* |det(d1,d2)| and |det(g1,g2)| (unit-cell areas) */
   double cut;              /* lattice-sum truncation: terms below exp(-cut) are dropped */
   double rule_ltol;        /* log of the error bound of the distance-chosen Gauss rule */
+  /* Faddeeva-series weights of the wavefront-uniform spatial sums (device, NULL: none), for
+   * the Ewald parameters v_j = veps0 exp(-j wtab_lrho), j < wtab_nv: rows of BQ_WTAB_ROW
+   * doubles, [v_j, x_j = k v_j, exp(-x_j^2), term count, 0, 0], then for n = 1 .. BQ_WTAB_N:
+   * (a n)^2, exp(-(a n)^2 - x^2), exp(-(a n + x)^2), a n exp(-(a n + x)^2), exp(-(a n - x)^2),
+   * a n exp(-(a n - x)^2) with a the series parameter of bq_faddeeva.hpp */
+  const double *wtab;
+  int wtab_nv;
+  double wtab_lrho;
   int *err;                /* device word (may be NULL): a kernel stores the reference's
                               negative QPGF code there (atomic minimum) when a reciprocal
                               order has beta_n^2 = k^2 - |k_t + G|^2 == 0 exactly (a Wood
                               anomaly): -12 in the Fourier-domain sum, -23 in the Ewald
                               spectral sum (qgf2_src/d3_qpgf_d2.h:113-123) */
 } bq_qplan;
+
+#define BQ_WTAB_N 60
+#define BQ_WTAB_ROW (6 * (BQ_WTAB_N + 1)) /* header, then BQ_WTAB_N terms */
+#define BQ_WTAB_MAXV 128
 
 /* Relative cost of one lattice term of each kind, for choosing the Fourier-domain or the
  * Ewald sum per point (both converge to the same value): Fourier-domain term 1, Ewald

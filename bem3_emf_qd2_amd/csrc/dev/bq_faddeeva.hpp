@@ -127,6 +127,40 @@ __device__ __forceinline__ cplx w_series_n(double xs, double y, int nterm, const
 }
 __device__ __forceinline__ cplx w_series(double xs, double y) { return w_series_n(xs, y, w_series_terms(fabs(xs))); }
 
+// Read-only memory read through scalar loads when the address is wave-uniform.
+typedef const __attribute__((address_space(4))) double* sconst_dp;
+
+// w_series_n for x = xs > 0 with the x-dependent weights of the series precomputed (one row of
+// the plan's weight table, T[6 n + 0..5] = (a n)^2, exp(-(a n)^2 - x^2), exp(-(a n + x)^2),
+// a n exp(-(a n + x)^2), exp(-(a n - x)^2), a n exp(-(a n - x)^2), T[2] = exp(-x^2)); the row
+// address is wave-uniform, so the weights come through scalar loads and a term takes one
+// reciprocal and five fused multiply-adds.
+__device__ __forceinline__ cplx w_series_w(double x, double y, int nterm, sconst_dp T, const double* et)
+{
+  double s1 = 0.0, s2 = 0.0, s3 = 0.0, s4 = 0.0, s5 = 0.0;
+  const double ex2 = T[2], y2 = y * y;
+#pragma unroll 2
+  for (int n = 1; n <= nterm; ++n) {
+    const sconst_dp c = T + 6 * n;
+    const double r = rcp_nr(c[0] + y2);
+    s1 = fma(c[1], r, s1);
+    s2 = fma(c[2], r, s2);
+    s4 = fma(c[3], r, s4);
+    s3 = fma(c[4], r, s3);
+    s5 = fma(c[5], r, s5);
+  }
+  const double ex2erfcx = (y > -6.0) ? ex2 * erfcx_any(y, et) : 2.0 * exp(y2 - x * x);
+  double sxy, cxy, s2xy, c2xy;
+  sincos_fast(x * y, &sxy, &cxy);
+  s2xy = 2.0 * sxy * cxy;
+  c2xy = fma(-2.0 * sxy, sxy, 1.0);
+  const double coef1 = ex2erfcx - kFC * y * s1;
+  const double coef2 = kFC * x * ex2;
+  double re = coef1 * c2xy + coef2 * sxy * sinc_of(x * y, sxy) + 0.5 * kFC * y * (s2 + s3);
+  double im = coef2 * sinc_of(2.0 * x * y, s2xy) - coef1 * s2xy + 0.5 * kFC * (s5 - s4);
+  return mk(re, im);
+}
+
 __device__ __forceinline__ bool w_use_cf(double x, double ya)
 {
   return ya > 7.0 || (x > 6.0 && (ya > 0.1 || (x > 8.0 && ya > 1e-10) || x > 28.0));
@@ -174,6 +208,13 @@ __device__ __forceinline__ cplx faddeeva_w_n(double xs, double y, int nterm, con
   const double x = fabs(xs), ya = fabs(y);
   if (xs == 0.0 || x >= 10.0 || w_use_cf(x, ya)) return w_cold(xs, y);
   return w_series_n(xs, y, nterm, et);
+}
+// The same for xs > 0 with the series weights from a row of the plan's weight table.
+__device__ __forceinline__ cplx faddeeva_w_w(double xs, double y, int nterm, sconst_dp T, const double* et)
+{
+  const double ya = fabs(y);
+  if (xs >= 10.0 || w_use_cf(xs, ya)) return w_cold(xs, y);
+  return w_series_w(xs, y, nterm, T, et);
 }
 
 }  // namespace bq

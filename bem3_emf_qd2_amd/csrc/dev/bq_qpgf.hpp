@@ -523,6 +523,7 @@ __device__ __forceinline__ double wave_max(double v)
   v = fmax(v, wave_dpp<0x140>(v));
   return fmax(fmax(wave_lane(v, 0), wave_lane(v, 16)), fmax(wave_lane(v, 32), wave_lane(v, 48)));
 }
+__device__ __forceinline__ double wave_min(double v) { return -wave_max(-v); }
 __device__ __forceinline__ double wave_first(double v)
 {
   const long long b = __double_as_longlong(v);
@@ -604,7 +605,7 @@ __device__ void qpgf_fourier_w(const bq_qplan& P, double px, double py, double p
 }
 
 __device__ void qpgf_ewald_w(const bq_qplan& P, double px, double py, double pz, double ve, QG& o, bool z0_ref,
-                             const double* et)
+                             const double* et, sconst_dp wrow)
 {
   const double ive = 1.0 / ve, ke = P.k * ve, ke2 = ke * ke;
   // ---- spatial sum: lane's own disk |r + R_l| <= Rmax, walked over one disk around lane 0
@@ -614,7 +615,7 @@ __device__ void qpgf_ewald_w(const bq_qplan& P, double px, double py, double pz,
     const double px0 = wave_first(px), py0 = wave_first(py);
     const double du = sqrt((px - px0) * (px - px0) + (py - py0) * (py - py0));
     const double Ru = wave_max(rho2 > 0.0 ? du + sqrt(rho2) : 0.0) * 1.0000001, Ru2 = Ru * Ru;
-    const int nterm = w_series_terms(wave_max(ke));
+    const int nterm = wrow ? (int)wrow[3] : w_series_terms(wave_max(ke));
     const double b1n = sqrt(P.b1x * P.b1x + P.b1y * P.b1y);
     const double c1 = -(P.b1x * px0 + P.b1y * py0), w1 = Ru * b1n;
     const int l1lo = (int)ceil(c1 - w1), l1hi = (int)floor(c1 + w1);
@@ -636,7 +637,7 @@ __device__ void qpgf_ewald_w(const bq_qplan& P, double px, double py, double pz,
           const double rl = sqrt(rl2), irl = 1.0 / rl;
           const double y = 0.5 * rl * ive;
           const cplx cef = Eb * exp(ke2 - y * y);
-          const cplx w = faddeeva_w_n(ke, y, nterm, et);
+          const cplx w = wrow ? faddeeva_w_w(ke, y, nterm, wrow, et) : faddeeva_w_n(ke, y, nterm, et);
           S += cef * (irl * w.re);
           const cplx cf = cef * (irl * irl * (irl * w.re - P.k * w.im + gcon));
           Sxy[0] += cf * ax;
@@ -696,7 +697,21 @@ __device__ void qpgf_v1_wave(const bq_qplan& P, double rx, double ry, double rz,
   double l1, l2, px, py;
   fold_cell(P, rx, ry, l1, l2, px, py);
   const double pz = rz;
-  const double ve = ewald_param(P, px, py, pz);
+  double ve = ewald_param(P, px, py, pz);
+  // With the plan's weight table the wavefront takes one Ewald parameter, the largest grid
+  // value v_j not above any lane's own (so no lane's exponential amplification grows), and
+  // the spatial sums' Faddeeva series read their weights from row j.  A wavefront with a lane
+  // at r_z == 0 keeps the lanes' own parameters (the reference's stop rule there depends on it).
+  sconst_dp wrow = nullptr;
+  if (P.wtab && __all(pz != 0.0 || !z0_ref)) {
+    const double vmin = wave_min(ve);
+    int j = __builtin_amdgcn_readfirstlane(max(0, (int)ceil(log(P.veps0 / vmin) / P.wtab_lrho)));
+    if (j < P.wtab_nv && ((sconst_dp)P.wtab)[(long)j * BQ_WTAB_ROW] > vmin) ++j;
+    if (j < P.wtab_nv) {
+      wrow = (sconst_dp)P.wtab + (long)j * BQ_WTAB_ROW;
+      ve = wrow[0];
+    }
+  }
   bool fd = false;
   if (P.fd_allowed && pz != 0.0) {
     const double bz = P.cut / fabs(pz);
@@ -707,7 +722,7 @@ __device__ void qpgf_v1_wave(const bq_qplan& P, double rx, double ry, double rz,
     fd = BQ_COST_FD_WAVE * n_fd < BQ_COST_SPEC * n_sn + BQ_COST_SPAT * n_sp;
   }
   if (__all(fd)) qpgf_fourier_w(P, px, py, pz, o);
-  else qpgf_ewald_w(P, px, py, pz, ve, o, z0_ref, et);
+  else qpgf_ewald_w(P, px, py, pz, ve, o, z0_ref, et, wrow);
   if (l1 != 0.0 || l2 != 0.0) {
     cplx cf = cexpi(-(P.vk0 * (l1 * P.d1x + l2 * P.d2x) + P.vk1 * (l1 * P.d1y + l2 * P.d2y)));
     o.g = o.g * cf;

@@ -67,6 +67,8 @@ struct bq_sess {
   std::vector<double> key_asm;
   std::vector<double> key_now;
   int* qerr = nullptr;       // QPGF error word of the assembly launches (device)
+  double* wtab = nullptr;    // Faddeeva-series weights of the plan (device, BQ_WTAB_MAXV rows)
+  std::vector<double> wtab_h;  // the same on the host, kept while its copy may be in flight
   void* work = nullptr;      // assembly work buffer
   size_t work_bytes = 0;
   bq_reduced red;

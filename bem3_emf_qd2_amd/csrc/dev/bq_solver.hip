@@ -28,6 +28,7 @@
 #include "../bq_assemble.h"
 #include "../bq_linalg.h"
 #include "../bq_plan.h"
+#include "bq_faddeeva.hpp"
 #include <cmath>
 #include <cstdlib>
 #include <cfloat>
@@ -283,6 +284,49 @@ double asm_tolerance()
   return g_asm_tol;
 }
 
+// The plan's Faddeeva-series weights (bq_qplan.wtab) for the incidence's wave number: rows for
+// the Ewald parameters v_j = veps0 1.02^-j down to sqrt(BQ_EW_EXPM) / k, the smallest that the
+// parameter rule gives (at r = 0).
+int build_wtab(bq_sess* S)
+{
+  bq_qplan& P = S->plan;
+  P.wtab = nullptr;
+  P.wtab_nv = 0;
+  if (S->ps != 1) return 0;
+  const double lrho = log(1.02), vlow = sqrt(BQ_EW_EXPM) / P.k;
+  int nv = 1;
+  if (P.veps0 > vlow) nv = (int)ceil(log(P.veps0 / vlow) / lrho) + 2;
+  nv = std::min(nv, BQ_WTAB_MAXV);
+  if (!S->wtab) {
+    BQ_HIP(hipMalloc((void**)&S->wtab, sizeof(double) * BQ_WTAB_ROW * BQ_WTAB_MAXV));
+    S->bytes += sizeof(double) * BQ_WTAB_ROW * BQ_WTAB_MAXV;
+  }
+  S->wtab_h.assign((size_t)BQ_WTAB_ROW * nv, 0.0);
+  for (int j = 0; j < nv; ++j) {
+    double* r = S->wtab_h.data() + (size_t)j * BQ_WTAB_ROW;
+    const double v = P.veps0 * exp(-j * lrho), x = P.k * v;
+    r[0] = v;
+    r[1] = x;
+    r[2] = exp(-x * x);
+    r[3] = (double)std::min(BQ_WTAB_N, (int)((x + 6.3) * (1.0 / bq::kFA)) + 2);  // w_series_terms(x)
+    for (int n = 1; n <= BQ_WTAB_N; ++n) {
+      const double an = bq::kFA * n, em = exp(-(an + x) * (an + x)), ep = exp(-(an - x) * (an - x));
+      double* c = r + 6 * n;
+      c[0] = an * an;
+      c[1] = exp(-an * an - x * x);
+      c[2] = em;
+      c[3] = an * em;
+      c[4] = ep;
+      c[5] = an * ep;
+    }
+  }
+  BQ_HIP(hipMemcpyAsync(S->wtab, S->wtab_h.data(), sizeof(double) * S->wtab_h.size(), hipMemcpyHostToDevice, S->s));
+  P.wtab = S->wtab;
+  P.wtab_nv = nv;
+  P.wtab_lrho = lrho;
+  return 0;
+}
+
 // Per-incidence choice of the lattice-sum cut for tolerance tol: the QPGF (value and gradient)
 // at kSamples offsets between random collocation nodes and random points of random source
 // elements of the open region, with the default cut and with smaller ones; the smallest cut
@@ -486,6 +530,7 @@ void bq_sess_destroy(bq_sess* S)
   bq::reduced_free(S);
   if (S->work) (void)hipFree(S->work);
   if (S->qerr) (void)hipFree(S->qerr);
+  if (S->wtab) (void)hipFree(S->wtab);
   auto f = [](void* p) { if (p) (void)hipFree(p); };
   f(S->E); f(S->ren); f(S->nrm); f(S->Ui); f(S->dUi); f(S->kn); f(S->eni); f(S->md); f(S->sd);
   for (auto p : S->sid) f(p);
@@ -524,7 +569,7 @@ int bq_sess_assemble(bq_sess* S)
     S->work_bytes = wmax;
     S->bytes += wmax;
   }
-  if (tune_plan(S, asm_tolerance())) return -1;
+  if (tune_plan(S, asm_tolerance()) || build_wtab(S)) return -1;
   if (!S->qerr) BQ_HIP(hipMalloc((void**)&S->qerr, sizeof(int)));
   BQ_HIP(hipMemsetAsync(S->qerr, 0, sizeof(int), s));
   S->plan.err = S->qerr;

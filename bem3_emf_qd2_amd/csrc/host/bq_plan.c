@@ -14,6 +14,9 @@ int bq_qplan_init(bq_qplan *p, double k, const double *vk, const double *vd1, co
   p->k = k;
   p->k2 = k * k;
   p->err = 0;
+  p->wtab = 0;
+  p->wtab_nv = 0;
+  p->wtab_lrho = 0.0;
   p->cut = BQ_EXP_CUT;
   p->rule_ltol = BQ_RULE_LTOL;
   p->vk0 = vk[0];
