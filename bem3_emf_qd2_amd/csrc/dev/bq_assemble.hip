@@ -421,14 +421,17 @@ __device__ double polar_F_part(const Elem& e, double zt, double et)
 }
 
 // domain 0, singular pairs: one block per collocation node (lattice-sum accumulators in
-// LDS, two wavefronts per SIMD, as in k_q0_gl; per-lane lattice sums: the polar rule's points
-// run from the node to the element's edge, and their union disks would hold more terms)
+// LDS, two wavefronts per SIMD, wavefront-uniform lattice sums as in k_q0_gl; 128 angles per
+// sector keep every wavefront wholly active or wholly idle in the angle loop)
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
 k_q0_sing(bq_qplan P, const double* __restrict__ E, const int* __restrict__ sid, int dNe, double2* __restrict__ G,
           double2* __restrict__ H, double* __restrict__ F3)
 {
   __shared__ double lds[21 * 4];
   __shared__ double sacc[16][256];
+  __shared__ double setab[kErfcxTab];
+  erfcx_stage(setab);
+  __syncthreads();
   const long N = 4L * dNe, row = blockIdx.x;
   const int t = (int)(row / 4) + 1, tn = (int)(row & 3), as = abs(sid[t]);
   Elem e;
@@ -457,13 +460,13 @@ k_q0_sing(bq_qplan P, const double* __restrict__ E, const int* __restrict__ sid,
         W[d] = e.cw[d][0] + e.cw[d][1] * z + e.cw[d][2] * ee;
       }
       QG qq;
-      qpgf_v1(P, R[0], R[1], R[2], qq);
+      qpgf_v1_wave(P, R[0], R[1], R[2], qq, setab);
       // the reference takes G here from d3hm_qpgf_d2_qG and the gradient from _dqG
       // (d3qd2_src/q0_bil_coef.c:366, :399, :444, :489); they differ only at R_z == 0
-      if (R[2] == 0.0) {
+      if (__any(R[2] == 0.0)) {
         QG qa;
-        qpgf_v1(P, R[0], R[1], R[2], qa, false);
-        qq.g = qa.g;
+        qpgf_v1_wave(P, R[0], R[1], R[2], qa, setab, false);
+        if (R[2] == 0.0) qq.g = qa.g;
       }
       double aW = sqrt(dot3(W, W));
       cplx hn = qq.d[0] * W[0] + qq.d[1] * W[1] + qq.d[2] * W[2];
