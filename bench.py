@@ -47,32 +47,32 @@ FP64_PEAK_TFS = 78.6
 # (rocprofv3 PMC) over its device-counted evaluations; C2's figures are read on the same scene
 # at reduced resolution (workload c2s: same lattice, wavelength, materials and incidence).
 # profiles/k_q0_gl_c1_pmc.md and profiles/k_q0_gl_c2s_pmc.md hold the counter values.
-FLOPS_PER_EVAL = {"c1": {"k_q0_gl": 55999, "k_q0_gh": 51625, "k_q0_sing": 50274},
-                  "c2": {"k_q0_gl": 51253, "k_q0_gh": 50007, "k_q0_sing": 50175}}
+FLOPS_PER_EVAL = {"c1": {"k_q0_gl": 42868, "k_q0_gh": 36660, "k_q0_sing": 36531},
+                  "c2": {"k_q0_gl": 35489, "k_q0_gh": 36690, "k_q0_sing": 36984}}
 FLOPS_PER_EVAL["c2s"] = FLOPS_PER_EVAL["c2"]
 # the 6144-quad sweep, read on its 384-quad variant (c4s) at the same six timed points
 # (profiles/k_q0_gl_c4s_pmc.md): the reference's accuracy (cut 40) and the mixed path's tuned
 # assembly (tolerance 1e-12: cut 26 at every point)
-FLOPS_PER_EVAL["c4"] = {"k_q0_gl": 60007}
-FLOPS_PER_EVAL["c4_tuned"] = {"k_q0_gl": 45861}
+FLOPS_PER_EVAL["c4"] = {"k_q0_gl": 43112}
+FLOPS_PER_EVAL["c4_tuned"] = {"k_q0_gl": 34246}
 FLOPS_PER_EVAL["c4s"], FLOPS_PER_EVAL["c4s_tuned"] = FLOPS_PER_EVAL["c4"], FLOPS_PER_EVAL["c4_tuned"]
 # the 1536-quad two-layer sweep has the open region, lattice, wavelength and direction ranges of
 # the 6144-quad sweep, which fix the per-evaluation cost at the reference's cut
 FLOPS_PER_EVAL["c3"] = FLOPS_PER_EVAL["c4"]
 # HBM bytes per evaluation of k_q0_gl: (FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00) KiB x 1024 of one
 # launch over its evaluations (factors from a 1 GiB copy read with the same counters)
-TRAFFIC_PER_EVAL = {"c1": {"k_q0_gl": (1163230.7 * 2.0 + 9545073.7) * 1024 / 46912896},
-                    "c2": {"k_q0_gl": (12430181.2 * 2.0 + 72190715.0) * 1024 / 385592752}}
+TRAFFIC_PER_EVAL = {"c1": {"k_q0_gl": (6234276.5 * 2.0 + 7736990.2) * 1024 / 41990176},
+                    "c2": {"k_q0_gl": (47751495.6 * 2.0 + 50115568.2) * 1024 / 350386455}}
 TRAFFIC_PER_EVAL["c2s"] = TRAFFIC_PER_EVAL["c2"]
 # FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00 per evaluation on c4s at the reference's cut
 # (profiles/k_q0_gl_c4s_pmc.md); the 1536-quad two-layer sweep shares the open region's physics
-TRAFFIC_PER_EVAL["c4"] = TRAFFIC_PER_EVAL["c3"] = TRAFFIC_PER_EVAL["c4s"] = {"k_q0_gl": 244.1}
+TRAFFIC_PER_EVAL["c4"] = TRAFFIC_PER_EVAL["c3"] = TRAFFIC_PER_EVAL["c4s"] = {"k_q0_gl": 470.3}
 # k_q0_gl's issue profile on c4s (same counters, six points): active lanes per VALU instruction
 # (SQ_THREAD_CYCLES_VALU / 64 SQ_ACTIVE_INST_VALU; the tail lanes of a pair's last sweep count as
 # active, they evaluate a point whose result is discarded) and the FP64 share of the VALU
 # instructions (ADD + MUL + FMA + TRANS F64 over SQ_INSTS_VALU)
-VALU_PROFILE = {"c4": {"k_q0_gl": {"valu_lane_util": 0.946, "fp64_share_of_valu": 0.719}},
-                "c4_tuned": {"k_q0_gl": {"valu_lane_util": 0.946, "fp64_share_of_valu": 0.712}}}
+VALU_PROFILE = {"c4": {"k_q0_gl": {"valu_lane_util": 0.942, "fp64_share_of_valu": 0.687}},
+                "c4_tuned": {"k_q0_gl": {"valu_lane_util": 0.941, "fp64_share_of_valu": 0.679}}}
 VALU_PROFILE["c4s"], VALU_PROFILE["c4s_tuned"] = VALU_PROFILE["c4"], VALU_PROFILE["c4_tuned"]
 VALU_PROFILE["c3"] = VALU_PROFILE["c4"]
 
