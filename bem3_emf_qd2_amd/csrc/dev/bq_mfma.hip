@@ -43,6 +43,9 @@ constexpr int kBlk = 32;             // MFMA block edge
 constexpr int kKStep = 16;           // complex k per stage (the MFMA's k)
 constexpr int kTileU4 = 64;          // one 32 x 16 plane = 64 lanes x 16 bytes
 constexpr int kStageU4 = 4 * 4 * kTileU4;  // 4 blocks x 4 planes = 16 KB
+#ifndef BQ_GEMM_DIRECT
+#define BQ_GEMM_DIRECT 1
+#endif
 constexpr int kGroupRows = 8;        // tile rows per swizzle group
 
 __device__ __forceinline__ unsigned bf16_rne(float x)
@@ -247,6 +250,122 @@ __global__ __launch_bounds__(256, 2) void k_cgemm_split(int m, int n, int tiles_
     }
 }
 
+// ---- the same product with the stages loaded straight into LDS ----------------------------
+// global_load_lds_dwordx4 puts each lane's 16 bytes at the wavefront's LDS base + 16 x lane, the
+// layout the MFMA reads, so a stage needs no staging registers and no LDS stores; the two stage
+// buffers are separate LDS objects and the loop is unrolled by two, so that the loads into one
+// buffer do not make the reads of the other wait for them.
+__device__ __forceinline__ void stage_direct(uint4* la, uint4* lb, const uint4* pa, const uint4* pb, int tid, int w)
+{
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    __builtin_amdgcn_global_load_lds((const void*)(pa + tid + 256 * q),
+                                     (__attribute__((address_space(3))) void*)(la + 64 * w + 256 * q), 16, 0, 0);
+    __builtin_amdgcn_global_load_lds((const void*)(pb + tid + 256 * q),
+                                     (__attribute__((address_space(3))) void*)(lb + 64 * w + 256 * q), 16, 0, 0);
+  }
+}
+
+// the 48 MFMAs of one 16-wide k step on a wave's 64 x 64 part of the tile
+__device__ __forceinline__ void stage_mma(const uint4* la, const uint4* lb, int lane, int wr, int wc,
+                                          f32x16 (&cr)[2][2], f32x16 (&ci)[2][2])
+{
+  uint4 b_op[2][4];
+#pragma unroll
+  for (int j = 0; j < 2; ++j)
+#pragma unroll
+    for (int p = 0; p < 4; ++p) b_op[j][p] = lb[((2 * wc + j) * 4 + p) * kTileU4 + lane];
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    uint4 a_op[4];
+#pragma unroll
+    for (int p = 0; p < 4; ++p) a_op[p] = la[((2 * wr + i) * 4 + p) * kTileU4 + lane];
+    const bf16x8 arh = as_bf(a_op[0]), arl = as_bf(a_op[1]), aih = as_bf(a_op[2]), ail = as_bf(a_op[3]);
+    const bf16x8 naih = neg_bf(a_op[2]), nail = neg_bf(a_op[3]);
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const bf16x8 brh = as_bf(b_op[j][0]), brl = as_bf(b_op[j][1]), bih = as_bf(b_op[j][2]), bil = as_bf(b_op[j][3]);
+      f32x16 r = cr[i][j], q = ci[i][j];
+      r = mma(brh, arh, r);
+      q = mma(bih, arh, q);
+      r = mma(brl, arh, r);
+      q = mma(bil, arh, q);
+      r = mma(brh, arl, r);
+      q = mma(bih, arl, q);
+      r = mma(bih, naih, r);
+      q = mma(brh, aih, q);
+      r = mma(bil, naih, r);
+      q = mma(brl, aih, q);
+      r = mma(bih, nail, r);
+      q = mma(brh, ail, q);
+      cr[i][j] = r;
+      ci[i][j] = q;
+    }
+  }
+}
+
+template <bool STORE>
+__global__ __launch_bounds__(256, 2) void k_cgemm_split_d(int m, int n, int tiles_r, int tiles_c, int Kb,
+                                                          const uint4* __restrict__ Ap, const uint4* __restrict__ Bp,
+                                                          float2* __restrict__ C, long ldc)
+{
+  __shared__ uint4 l0a[kStageU4], l0b[kStageU4], l1a[kStageU4], l1b[kStageU4];
+  const int ntiles = tiles_r * tiles_c;
+  int t = blockIdx.x;
+  if ((ntiles & 7) == 0) t = (t & 7) * (ntiles >> 3) + (t >> 3);  // contiguous run per XCD
+  const int span = kGroupRows * tiles_c, g = t / span, first = g * kGroupRows;
+  const int gsz = min(kGroupRows, tiles_r - first), in = t - g * span;
+  const int tr = first + in % gsz, tc = in / gsz;
+  const int tid = threadIdx.x, lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6), wr = w >> 1, wc = w & 1;
+  const int Mb = tiles_r * 4, Nb = tiles_c * 4;
+  const uint4* a_src = Ap + (long)(tr * 4) * 4 * kTileU4;
+  const uint4* b_src = Bp + (long)(tc * 4) * 4 * kTileU4;
+  const long a_step = (long)Mb * 4 * kTileU4, b_step = (long)Nb * 4 * kTileU4;
+  f32x16 cr[2][2], ci[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      cr[i][j] = f32x16{};
+      ci[i][j] = f32x16{};
+    }
+  stage_direct(l0a, l0b, a_src, b_src, tid, w);
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  for (int kb = 0; kb < Kb; kb += 2) {
+    if (kb + 1 < Kb) stage_direct(l1a, l1b, a_src + (long)(kb + 1) * a_step, b_src + (long)(kb + 1) * b_step, tid, w);
+    stage_mma(l0a, l0b, lane, wr, wc, cr, ci);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (kb + 1 < Kb) {
+      if (kb + 2 < Kb) stage_direct(l0a, l0b, a_src + (long)(kb + 2) * a_step, b_src + (long)(kb + 2) * b_step, tid, w);
+      stage_mma(l1a, l1b, lane, wr, wc, cr, ci);
+      __builtin_amdgcn_s_waitcnt(0);
+      __syncthreads();
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int row = tr * kTile + (2 * wr + i) * kBlk + (lane & 31);
+      if (row >= m) continue;
+      const int c0 = tc * kTile + (2 * wc + j) * kBlk + 4 * (lane >> 5);
+      float2* cr0 = C + row;
+      float2 v[16];
+#pragma unroll
+      for (int e = 0; e < 16; ++e)
+        v[e] = STORE ? make_float2(0.f, 0.f) : cr0[(long)min(c0 + 8 * (e >> 2) + (e & 3), n - 1) * ldc];
+#pragma unroll
+      for (int e = 0; e < 16; ++e) {
+        const int col = c0 + 8 * (e >> 2) + (e & 3);
+        v[e].x -= cr[i][j][e];
+        v[e].y -= ci[i][j][e];
+        if (col < n) cr0[(long)col * ldc] = v[e];
+      }
+    }
+}
+
 // packed-operand workspace per (device, stream), grown on demand and kept for the process
 struct Ws {
   int dev;
@@ -311,7 +430,13 @@ int split_gemm(hipStream_t s, int m, int n, int k, const float* A, long lda, con
   k_pack_a<<<(unsigned)((ta + 255) / 256), 256, 0, s>>>(m, k, (const float2*)A, lda, Mb, Kb, Ap);
   k_pack_b<<<(unsigned)((tb + 255) / 256), 256, 0, s>>>(k, n, (const float2*)B, ldb, Nb, Kb, Bp);
   const unsigned grid = (unsigned)(tiles_r * tiles_c);
-  if (store)
+  if (BQ_GEMM_DIRECT && store)
+    BQ_KTIME("k_cgemm_split", s,
+             k_cgemm_split_d<true><<<grid, 256, 0, s>>>(m, n, tiles_r, tiles_c, Kb, Ap, Bp, (float2*)C, ldc));
+  else if (BQ_GEMM_DIRECT)
+    BQ_KTIME("k_cgemm_split", s,
+             k_cgemm_split_d<false><<<grid, 256, 0, s>>>(m, n, tiles_r, tiles_c, Kb, Ap, Bp, (float2*)C, ldc));
+  else if (store)
     BQ_KTIME("k_cgemm_split", s,
              k_cgemm_split<true><<<grid, 256, 0, s>>>(m, n, tiles_r, tiles_c, Kb, Ap, Bp, (float2*)C, ldc));
   else
