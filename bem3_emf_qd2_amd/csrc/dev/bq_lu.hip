@@ -538,6 +538,7 @@ int* perm_of(V* inv, int n)
 // applies the block's columns of L (or U) to the rows outside it, reading them once.
 constexpr int kSmallRhs = 8;
 constexpr int kPermChunk = 512;  // columns per pass of the in-place row permutation
+constexpr int kSolveGrp = 4;     // 512-row blocks per rank-2048 update of the many-column solves
 
 // rows [j0, je) of B (nrhs columns): LOWER: X = inv(L_blk) X (unit lower, steps downwards);
 // else X = inv(U_blk) X (steps upwards).  inv: the 32x32 inverses getrf stored.
@@ -1127,48 +1128,66 @@ int getrs(hipStream_t s, int n, int nrhs, const typename P<R>::V* A, long lda, c
     const int first = ng > 0 ? (max(0, gf[g]) / kNBF) * kNBF : 0;
     if (nc <= 0) continue;
     V* Bg = B + (long)c0 * ldb;
-    for (int j0 = first; j0 < n; j0 += kNBF) {
-      const int je = min(n, j0 + kNBF);
-      const int ncb = ident ? min(nc, je) : nc;  // columns with nonzeros in this block row
-      const unsigned gblk = (unsigned)((ncb + 63) / 64);
-      bool done = false;
-      if constexpr (std::is_same<R, float>::value) {
-        if (blk_inv && je - j0 == kNBF && ncb >= kSplitMin) {
-          if (apply_block_inverse(s, ncb, WL + (size_t)(j0 / kNBF) * kNBF * kNBF, Bg + j0, ldb)) return -1;
-          done = true;
+    // kSolveGrp blocks at a time: in-group updates of the group's later rows, then one update of
+    // all rows below the group with rank kSolveGrp x 512 (each row below is read and written
+    // once per group rather than once per block)
+    for (int ga = first; ga < n; ga += kSolveGrp * kNBF) {
+      const int gz = min(n, ga + kSolveGrp * kNBF);
+      for (int j0 = ga; j0 < gz; j0 += kNBF) {
+        const int je = min(n, j0 + kNBF);
+        const int ncb = ident ? min(nc, je) : nc;  // columns with nonzeros in this block row
+        const unsigned gblk = (unsigned)((ncb + 63) / 64);
+        bool done = false;
+        if constexpr (std::is_same<R, float>::value) {
+          if (blk_inv && je - j0 == kNBF && ncb >= kSplitMin) {
+            if (apply_block_inverse(s, ncb, WL + (size_t)(j0 / kNBF) * kNBF * kNBF, Bg + j0, ldb)) return -1;
+            done = true;
+          }
         }
-      }
-      for (int j = j0; j < je && !done; j += kNB) {
-        const int jb = min(kNB, je - j);
-        k_apply_blk<R><<<gblk, 256, 0, s>>>(inv + (long)(j / kNB) * kBlkElems, jb, Bg + j, ldb, ncb);
-        BQ_HIP(hipGetLastError());
-        if (je - j - jb > 0 && gemm<R>(s, je - j - jb, ncb, jb, mone, A + (long)j * lda + (j + jb), lda, Bg + j, ldb,
-                                       one, Bg + j + jb, ldb))
+        for (int j = j0; j < je && !done; j += kNB) {
+          const int jb = min(kNB, je - j);
+          k_apply_blk<R><<<gblk, 256, 0, s>>>(inv + (long)(j / kNB) * kBlkElems, jb, Bg + j, ldb, ncb);
+          BQ_HIP(hipGetLastError());
+          if (je - j - jb > 0 && gemm<R>(s, je - j - jb, ncb, jb, mone, A + (long)j * lda + (j + jb), lda, Bg + j,
+                                         ldb, one, Bg + j + jb, ldb))
+            return -1;
+        }
+        if (gz - je > 0 &&
+            gemm<R>(s, gz - je, ncb, je - j0, mone, A + (long)j0 * lda + je, lda, Bg + j0, ldb, one, Bg + je, ldb))
           return -1;
       }
-      if (n - je > 0 &&
-          gemm<R>(s, n - je, ncb, je - j0, mone, A + (long)j0 * lda + je, lda, Bg + j0, ldb, one, Bg + je, ldb))
+      const int ncg = ident ? min(nc, gz) : nc;
+      if (n - gz > 0 &&
+          gemm<R>(s, n - gz, ncg, gz - ga, mone, A + (long)ga * lda + gz, lda, Bg + ga, ldb, one, Bg + gz, ldb))
         return -1;
     }
   }
-  // backward: U, 512-row blocks from the bottom, 32-row steps upwards inside each
-  for (int j0 = ((n - 1) / kNBF) * kNBF; j0 >= 0; j0 -= kNBF) {
-    const int je = min(n, j0 + kNBF);
-    bool done = false;
-    if constexpr (std::is_same<R, float>::value) {
-      if (blk_inv && je - j0 == kNBF) {
-        if (apply_block_inverse(s, nrhs, WU + (size_t)(j0 / kNBF) * kNBF * kNBF, B + j0, ldb)) return -1;
-        done = true;
+  // backward: U, 512-row blocks from the bottom, 32-row steps upwards inside each, grouped as
+  // the forward sweep
+  const int nblk = (n + kNBF - 1) / kNBF;
+  for (int bh = nblk - 1; bh >= 0; bh -= kSolveGrp) {
+    const int bl = max(0, bh - kSolveGrp + 1), ga = bl * kNBF, gz = min(n, (bh + 1) * kNBF);
+    for (int b = bh; b >= bl; --b) {
+      const int j0 = b * kNBF, je = min(n, j0 + kNBF);
+      bool done = false;
+      if constexpr (std::is_same<R, float>::value) {
+        if (blk_inv && je - j0 == kNBF) {
+          if (apply_block_inverse(s, nrhs, WU + (size_t)(j0 / kNBF) * kNBF * kNBF, B + j0, ldb)) return -1;
+          done = true;
+        }
       }
-    }
-    for (int j = ((je - 1 - j0) / kNB) * kNB + j0; j >= j0 && !done; j -= kNB) {
-      const int jb = min(kNB, je - j);
-      k_apply_blk<R><<<cblk, 256, 0, s>>>(inv + (long)(j / kNB) * kBlkElems + kNB * kNB, jb, B + j, ldb, nrhs);
-      BQ_HIP(hipGetLastError());
-      if (j > j0 && gemm<R>(s, j - j0, nrhs, jb, mone, A + (long)j * lda + j0, lda, B + j, ldb, one, B + j0, ldb))
+      for (int j = ((je - 1 - j0) / kNB) * kNB + j0; j >= j0 && !done; j -= kNB) {
+        const int jb = min(kNB, je - j);
+        k_apply_blk<R><<<cblk, 256, 0, s>>>(inv + (long)(j / kNB) * kBlkElems + kNB * kNB, jb, B + j, ldb, nrhs);
+        BQ_HIP(hipGetLastError());
+        if (j > j0 && gemm<R>(s, j - j0, nrhs, jb, mone, A + (long)j * lda + j0, lda, B + j, ldb, one, B + j0, ldb))
+          return -1;
+      }
+      if (j0 > ga &&
+          gemm<R>(s, j0 - ga, nrhs, je - j0, mone, A + (long)j0 * lda + ga, lda, B + j0, ldb, one, B + ga, ldb))
         return -1;
     }
-    if (j0 > 0 && gemm<R>(s, j0, nrhs, je - j0, mone, A + (long)j0 * lda, lda, B + j0, ldb, one, B, ldb)) return -1;
+    if (ga > 0 && gemm<R>(s, ga, nrhs, gz - ga, mone, A + (long)ga * lda, lda, B + ga, ldb, one, B, ldb)) return -1;
   }
   if (WL) BQ_HIP(hipFreeAsync(WL, s));
   if (WU) BQ_HIP(hipFreeAsync(WU, s));
