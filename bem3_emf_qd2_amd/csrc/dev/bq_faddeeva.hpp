@@ -68,6 +68,15 @@ __device__ __forceinline__ double erfcx_any(double x, const double* tab)
   return x >= 0.0 ? erfcx_tab(x, tab) : 2.0 * exp(x * x) - erfcx_tab(-x, tab);
 }
 
+__device__ __forceinline__ double rsqrt_nr(double x)
+{
+  // hardware reciprocal-square-root estimate refined by two Newton steps (full double precision)
+  double y = __builtin_amdgcn_rsq(x);
+  const double h = 0.5 * x;
+  y = y * fma(-h * y, y, 1.5);
+  return y * fma(-h * y, y, 1.5);
+}
+
 __device__ __forceinline__ double sinc_of(double x, double sx)
 {
   return (fabs(x) < 1e-4) ? 1.0 - x * x * (1.0 / 6.0) : sx / x;

@@ -81,6 +81,12 @@ __device__ __forceinline__ int row_centre(double ue, double e2, int lo, int hi)
   const int c = (int)rint(-ue / e2);
   return c < lo ? lo : (c > hi ? hi : c);
 }
+// the same from 1 / e2 (any start within lo .. hi gives the same row sum)
+__device__ __forceinline__ int row_centre_r(double ue, double ie2, int lo, int hi)
+{
+  const int c = (int)rint(-ue * ie2);
+  return c < lo ? lo : (c > hi ? hi : c);
+}
 
 // Fourier-domain sum for the folded point (px, py, pz), pz != 0.  With HESS (Q = QG2) the
 // second derivatives are summed too: each order is exp(i a.p + i b |z|) / b, so d2/dxdy brings
@@ -539,22 +545,22 @@ __device__ __forceinline__ void spectral_term_w(const bq_qplan& P, double ax, do
 {
   const double b2 = P.k2 - ax * ax - ay * ay;
   if (b2 > 0.0) {
-    const double b = sqrt(b2), ef = exp(b2 * ve * ve - az * az);
+    const double ib = rsqrt_nr(b2), b = b2 * ib, ef = exp(b2 * ve * ve - az * az);
     const cplx ff = cexpi(b * pz);
     const cplx w = faddeeva_w_n(b * ve, az, nterm, et);
-    F = ce * (mk(ff.re, ef * w.im + ff.im) * (2.0 / b));
+    F = ce * (mk(ff.re, ef * w.im + ff.im) * (2.0 * ib));
     Gz = ce * (mk(ef * w.re - ff.re, -ff.im) * 2.0);
   } else if (b2 == 0.0) {
     qpgf_flag(P, -23);
     F = Gz = mk(0.0, 0.0);
   } else {
-    const double b = sqrt(-b2), bv = b * ve;
+    const double ib = rsqrt_nr(-b2), b = -b2 * ib, bv = b * ve;
     const double a1 = bv + az, a2 = bv - az;
     const double e12 = exp(-bv * bv - az * az);
     double f1 = e12 * erfcx_tab(fabs(a1), et), f2 = e12 * erfcx_tab(fabs(a2), et);
     if (a1 <= 0.0) f1 = 2.0 * exp(b * pz) - f1;
     if (a2 <= 0.0) f2 = 2.0 * exp(-b * pz) - f2;
-    F = mk(ce.im, -ce.re) * ((f1 + f2) / b);
+    F = mk(ce.im, -ce.re) * ((f1 + f2) * ib);
     Gz = ce * (f1 - f2);
   }
 }
@@ -574,19 +580,19 @@ __device__ void qpgf_fourier_w(const bq_qplan& P, double px, double py, double p
     const double ux = -P.vk0 + n1 * P.g1x, uy = -P.vk1 + n1 * P.g1y;
     int lo, hi;
     if (!row_range(ux, uy, P.g2x, P.g2y, g22, A2, lo, hi)) continue;
-    const int c = row_centre(ux * P.g2x + uy * P.g2y, g22, lo, hi);
+    const int c = row_centre_r(ux * P.g2x + uy * P.g2y, 1.0 / g22, lo, hi);
     row_walk(lo, hi, c, cexpi(ap + n1 * gp1 + c * gp2), step, [&](int n2, cplx E) {
       const double ax = ux + n2 * P.g2x, ay = uy + n2 * P.g2y;
       const double b2 = P.k2 - ax * ax - ay * ay;
       cplx dz, F;
       if (b2 > 0.0) {
-        const double b = sqrt(b2);
+        const double ib = rsqrt_nr(b2), b = b2 * ib;
         dz = E * cexpi(b * az);
-        F = dz * (1.0 / b);
+        F = dz * ib;
       } else if (b2 < 0.0) {
-        const double b = sqrt(-b2);
+        const double ib = rsqrt_nr(-b2), b = -b2 * ib;
         dz = E * exp(-b * az);
-        F = mk(dz.im, -dz.re) * (1.0 / b);  // -i dz / b
+        F = mk(dz.im, -dz.re) * ib;  // -i dz / b
       } else {
         qpgf_flag(P, -12);
         dz = F = mk(0.0, 0.0);
@@ -627,14 +633,14 @@ __device__ void qpgf_ewald_w(const bq_qplan& P, double px, double py, double pz,
       const double ux0 = px0 + l1 * P.d1x, uy0 = py0 + l1 * P.d1y;
       int lo, hi;
       if (!row_range(ux0, uy0, P.d2x, P.d2y, d22, Ru2, lo, hi)) continue;
-      const int c = row_centre(ux0 * P.d2x + uy0 * P.d2y, d22, lo, hi);
+      const int c = row_centre_r(ux0 * P.d2x + uy0 * P.d2y, 1.0 / d22, lo, hi);
       const cplx E0 = cexpi(P.vk0 * (l1 * P.d1x + c * P.d2x) + P.vk1 * (l1 * P.d1y + c * P.d2y));
       const double ux = px + l1 * P.d1x, uy = py + l1 * P.d1y;
       row_walk(lo, hi, c, E0, step, [&](int l2, cplx Eb) {
         const double ax = ux + l2 * P.d2x, ay = uy + l2 * P.d2y;
         const double rl2 = ax * ax + ay * ay + pz * pz;
         if (rl2 <= Rmax2) {
-          const double rl = sqrt(rl2), irl = 1.0 / rl;
+          const double irl = rsqrt_nr(rl2), rl = rl2 * irl;
           const double y = 0.5 * rl * ive;
           const cplx cef = Eb * exp(ke2 - y * y);
           const cplx w = wrow ? faddeeva_w_w(ke, y, nterm, wrow, et) : faddeeva_w_n(ke, y, nterm, et);
@@ -670,7 +676,7 @@ __device__ void qpgf_ewald_w(const bq_qplan& P, double px, double py, double pz,
     const double ux = -P.vk0 + n1 * P.g1x, uy = -P.vk1 + n1 * P.g1y;
     int lo, hi;
     if (!row_range(ux, uy, P.g2x, P.g2y, g22, A2, lo, hi)) continue;
-    const int c = row_centre(ux * P.g2x + uy * P.g2y, g22, lo, hi);
+    const int c = row_centre_r(ux * P.g2x + uy * P.g2y, 1.0 / g22, lo, hi);
     row_walk(lo, hi, c, cexpi(ap + n1 * gp1 + c * gp2), step, [&](int n2, cplx E) {
       const double ax = ux + n2 * P.g2x, ay = uy + n2 * P.g2y;
       // term count of the propagating orders' Faddeeva series for the largest ve (x = b ve)
