@@ -36,9 +36,10 @@ typedef struct bq_qplan {
   double rule_ltol;        /* log of the error bound of the distance-chosen Gauss rule */
   /* Faddeeva-series weights of the wavefront-uniform spatial sums (device, NULL: none), for
    * the Ewald parameters v_j = veps0 exp(-j wtab_lrho), j < wtab_nv: rows of BQ_WTAB_ROW
-   * doubles, [v_j, x_j = k v_j, exp(-x_j^2), term count, 0, 0], then for n = 1 .. BQ_WTAB_N:
+   * doubles, [v_j, x_j = k v_j, exp(-x_j^2), term count, 2a/pi, 1 if a single Newton step
+   * suffices for the terms' reciprocals else 0], then for n = 1 .. BQ_WTAB_N:
    * (a n)^2, exp(-(a n)^2 - x^2), exp(-(a n + x)^2), a n exp(-(a n + x)^2), exp(-(a n - x)^2),
-   * a n exp(-(a n - x)^2) with a the series parameter of bq_faddeeva.hpp */
+   * a n exp(-(a n - x)^2), a = pi / sqrt(-ln(relerr / 2)) the series parameter of Algorithm 916 */
   const double *wtab;
   int wtab_nv;
   double wtab_lrho;

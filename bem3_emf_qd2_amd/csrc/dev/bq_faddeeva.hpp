@@ -79,7 +79,7 @@ __device__ __forceinline__ double rsqrt_nr(double x)
 
 __device__ __forceinline__ double sinc_of(double x, double sx)
 {
-  return (fabs(x) < 1e-4) ? 1.0 - x * x * (1.0 / 6.0) : sx / x;
+  return (fabs(x) < 1e-4) ? 1.0 - x * x * (1.0 / 6.0) : sx * rcp_nr(x);
 }
 
 // continued fraction for Im z >= 0
@@ -141,22 +141,38 @@ typedef const __attribute__((address_space(4))) double* sconst_dp;
 
 // w_series_n for x = xs > 0 with the x-dependent weights of the series precomputed (one row of
 // the plan's weight table, T[6 n + 0..5] = (a n)^2, exp(-(a n)^2 - x^2), exp(-(a n + x)^2),
-// a n exp(-(a n + x)^2), exp(-(a n - x)^2), a n exp(-(a n - x)^2), T[2] = exp(-x^2)); the row
-// address is wave-uniform, so the weights come through scalar loads and a term takes one
-// reciprocal and five fused multiply-adds.
+// a n exp(-(a n + x)^2), exp(-(a n - x)^2), a n exp(-(a n - x)^2), T[2] = exp(-x^2), T[4] = 2a/pi
+// for the row's series parameter a, T[5] = 1 when the row's accuracy needs only one Newton step
+// on a reciprocal); the row address is wave-uniform, so the weights come through scalar loads
+// and a term takes one reciprocal and five fused multiply-adds.
 __device__ __forceinline__ cplx w_series_w(double x, double y, int nterm, sconst_dp T, const double* et)
 {
   double s1 = 0.0, s2 = 0.0, s3 = 0.0, s4 = 0.0, s5 = 0.0;
-  const double ex2 = T[2], y2 = y * y;
+  const double ex2 = T[2], y2 = y * y, kFC = T[4];
+  if (T[5] != 0.0) {
 #pragma unroll 2
-  for (int n = 1; n <= nterm; ++n) {
-    const sconst_dp c = T + 6 * n;
-    const double r = rcp_nr(c[0] + y2);
-    s1 = fma(c[1], r, s1);
-    s2 = fma(c[2], r, s2);
-    s4 = fma(c[3], r, s4);
-    s3 = fma(c[4], r, s3);
-    s5 = fma(c[5], r, s5);
+    for (int n = 1; n <= nterm; ++n) {
+      const sconst_dp c = T + 6 * n;
+      const double d = c[0] + y2;
+      double r = __builtin_amdgcn_rcp(d);
+      r = fma(r, fma(-d, r, 1.0), r);
+      s1 = fma(c[1], r, s1);
+      s2 = fma(c[2], r, s2);
+      s4 = fma(c[3], r, s4);
+      s3 = fma(c[4], r, s3);
+      s5 = fma(c[5], r, s5);
+    }
+  } else {
+#pragma unroll 2
+    for (int n = 1; n <= nterm; ++n) {
+      const sconst_dp c = T + 6 * n;
+      const double r = rcp_nr(c[0] + y2);
+      s1 = fma(c[1], r, s1);
+      s2 = fma(c[2], r, s2);
+      s4 = fma(c[3], r, s4);
+      s3 = fma(c[4], r, s3);
+      s5 = fma(c[5], r, s5);
+    }
   }
   const double ex2erfcx = (y > -6.0) ? ex2 * erfcx_any(y, et) : 2.0 * exp(y2 - x * x);
   double sxy, cxy, s2xy, c2xy;

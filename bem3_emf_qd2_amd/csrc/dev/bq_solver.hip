@@ -286,7 +286,10 @@ double asm_tolerance()
 
 // The plan's Faddeeva-series weights (bq_qplan.wtab) for the incidence's wave number: rows for
 // the Ewald parameters v_j = veps0 1.02^-j down to sqrt(BQ_EW_EXPM) / k, the smallest that the
-// parameter rule gives (at r = 0).
+// parameter rule gives (at r = 0).  The series' relative error is Algorithm 916's parameter,
+// a = pi / sqrt(-ln(relerr / 2)): 1e-16 at the reference's accuracy, and a tenth of the
+// assembly tolerance (at most 1e-13) for a tuned assembly, which then also takes a single
+// Newton step for the terms' reciprocals.
 int build_wtab(bq_sess* S)
 {
   bq_qplan& P = S->plan;
@@ -301,6 +304,11 @@ int build_wtab(bq_sess* S)
     BQ_HIP(hipMalloc((void**)&S->wtab, sizeof(double) * BQ_WTAB_ROW * BQ_WTAB_MAXV));
     S->bytes += sizeof(double) * BQ_WTAB_ROW * BQ_WTAB_MAXV;
   }
+  const double tol = S->st.asm_tol;
+  const bool coarse = tol >= 1e-12;
+  const double L = coarse ? -log(0.5 * std::min(0.1 * tol, 1e-13)) : 0.0;
+  const double a = coarse ? M_PI / sqrt(L) : bq::kFA;
+  const double span = coarse ? sqrt(L) + 0.18 : 6.3;  // a n - x beyond which the terms are negligible
   S->wtab_h.assign((size_t)BQ_WTAB_ROW * nv, 0.0);
   for (int j = 0; j < nv; ++j) {
     double* r = S->wtab_h.data() + (size_t)j * BQ_WTAB_ROW;
@@ -308,9 +316,11 @@ int build_wtab(bq_sess* S)
     r[0] = v;
     r[1] = x;
     r[2] = exp(-x * x);
-    r[3] = (double)std::min(BQ_WTAB_N, (int)((x + 6.3) * (1.0 / bq::kFA)) + 2);  // w_series_terms(x)
+    r[3] = (double)std::min(BQ_WTAB_N, (int)((x + span) / a) + 2);
+    r[4] = 2.0 * a / M_PI;
+    r[5] = coarse ? 1.0 : 0.0;
     for (int n = 1; n <= BQ_WTAB_N; ++n) {
-      const double an = bq::kFA * n, em = exp(-(an + x) * (an + x)), ep = exp(-(an - x) * (an - x));
+      const double an = a * n, em = exp(-(an + x) * (an + x)), ep = exp(-(an - x) * (an - x));
       double* c = r + 6 * n;
       c[0] = an * an;
       c[1] = exp(-an * an - x * x);
