@@ -568,6 +568,35 @@ __device__ __forceinline__ void fs_store(const double (&acc)[48], bool neg, long
   }
 }
 
+// fs_store for one lane's 12 of the 48 sums (q-th twelve, q = 0..3): sum 12 q + 2 i + (0, 1) is
+// quantity (12 q + 2 i) / 8 (G, H, dG/dtz, dG/dte, dH/dtz, dH/dte) of shape function
+// ((12 q + 2 i) % 8) / 2
+__device__ __forceinline__ void fs_store12(const double (&v)[12], int q, bool neg, long row, long N, long col,
+                                           double2* G, double2* H, double2* dG, double2* dH)
+{
+  const double sh = neg ? -kI4PI : kI4PI;
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    const int idx = 12 * q + 2 * i, qty = idx >> 3, n = (idx & 7) >> 1;
+    double2* dst;
+    double sc;
+    if (qty == 0) {
+      dst = G + row * N;
+      sc = kI4PI;
+    } else if (qty == 1) {
+      dst = H + row * N;
+      sc = sh;
+    } else if (qty < 4) {
+      dst = dG + (2 * row + (qty - 2)) * N;
+      sc = -kI4PI;
+    } else {
+      dst = dH + (2 * row + (qty - 4)) * N;
+      sc = sh;
+    }
+    dst[col + n] = make_double2(v[2 * i] * sc, v[2 * i + 1] * sc);
+  }
+}
+
 __global__ void __launch_bounds__(256) k_fs_gl(double kr, double ki, const double* __restrict__ E,
                                                const double* __restrict__ ren, const int* __restrict__ sid, int dNe,
                                                const unsigned char* __restrict__ flag, int full_rule, double ltol,
@@ -614,8 +643,27 @@ __global__ void __launch_bounds__(256) k_fs_gl(double kr, double ki, const doubl
       fs_point(R, W, Nn, c, k, w, acc);
     }
   }
-  group_sum<16, 48>(acc);
-  if (active && sub == 0) fs_store(acc, sd < 0, row, N, 4L * (s - 1), G, H, dG, dH);
+  // reduce-scatter over the 16 lanes of the row: halve the 48 sums at each of the first two
+  // levels (each lane keeps the half its lane bit selects and adds the partner's copy of it),
+  // then sum the remaining 12 over the last two; lane sub = 4q holds the 12 totals from
+  // 12 q (q = 0..3)
+  double h24[24], h12[12];
+  const bool up8 = (sub & 8) != 0, up4 = (sub & 4) != 0;
+#pragma unroll
+  for (int i = 0; i < 24; ++i) {
+    const double keep = up8 ? acc[24 + i] : acc[i], send = up8 ? acc[i] : acc[24 + i];
+    h24[i] = keep + __shfl_xor(send, 8, 16);
+  }
+#pragma unroll
+  for (int i = 0; i < 12; ++i) {
+    const double keep = up4 ? h24[12 + i] : h24[i], send = up4 ? h24[i] : h24[12 + i];
+    h12[i] = keep + __shfl_xor(send, 4, 16);
+  }
+#pragma unroll
+  for (int i = 0; i < 12; ++i) h12[i] += __shfl_xor(h12[i], 2, 16);
+#pragma unroll
+  for (int i = 0; i < 12; ++i) h12[i] += __shfl_xor(h12[i], 1, 16);
+  if (active && (sub & 3) == 0) fs_store12(h12, sub >> 2, sd < 0, row, N, 4L * (s - 1), G, H, dG, dH);
 }
 
 __global__ void __launch_bounds__(256) k_fs_gh(double kr, double ki, const double* __restrict__ E,
