@@ -540,12 +540,13 @@ __device__ __forceinline__ double wave_first(double v)
 // Ewald spectral term (as spectral_term) with the evanescent erfc branches folded onto one
 // erfcx each: for a = bv +- z/2v <= 0, e^{+-bz} erfc(a) = 2 e^{+-bz} - e12 erfcx(-a), because
 // +-bz - a^2 = -b^2 v^2 - z^2/4v^2; the Faddeeva series take nterm terms.
+// ez = exp(b2 v^2 - az^2) of the order (exp(-bv^2 - az^2) when evanescent), carried by the caller
 __device__ __forceinline__ void spectral_term_w(const bq_qplan& P, double ax, double ay, cplx ce, double pz, double ve,
-                                                double az, int nterm, const double* et, cplx& F, cplx& Gz)
+                                                double az, int nterm, const double* et, double ez, cplx& F, cplx& Gz)
 {
   const double b2 = P.k2 - ax * ax - ay * ay;
   if (b2 > 0.0) {
-    const double ib = rsqrt_nr(b2), b = b2 * ib, ef = exp(b2 * ve * ve - az * az);
+    const double ib = rsqrt_nr(b2), b = b2 * ib, ef = ez;
     const cplx ff = cexpi(b * pz);
     const cplx w = faddeeva_w_n(b * ve, az, nterm, et);
     F = ce * (mk(ff.re, ef * w.im + ff.im) * (2.0 * ib));
@@ -556,7 +557,7 @@ __device__ __forceinline__ void spectral_term_w(const bq_qplan& P, double ax, do
   } else {
     const double ib = rsqrt_nr(-b2), b = -b2 * ib, bv = b * ve;
     const double a1 = bv + az, a2 = bv - az;
-    const double e12 = exp(-bv * bv - az * az);
+    const double e12 = ez;
     double f1 = e12 * erfcx_tab(fabs(a1), et), f2 = e12 * erfcx_tab(fabs(a2), et);
     if (a1 <= 0.0) f1 = 2.0 * exp(b * pz) - f1;
     if (a2 <= 0.0) f2 = 2.0 * exp(-b * pz) - f2;
@@ -671,24 +672,54 @@ __device__ void qpgf_ewald_w(const bq_qplan& P, double px, double py, double pz,
   const double az = 0.5 * pz / ve;
   cplx S = mk(0, 0), Sx = mk(0, 0), Sy = mk(0, 0), Sz = mk(0, 0);
   const double gp1 = P.g1x * px + P.g1y * py, gp2 = P.g2x * px + P.g2y * py, ap = -(P.vk0 * px + P.vk1 * py);
-  const cplx step = cexpi(gp2);
+  const cplx step = cexpi(gp2), back = conj(step);
+  // Along a row the orders' factor exp(-v^2 (|a|^2 - k^2) - az^2) is a Gaussian in n2: it is
+  // carried from the row's centre by the ratio R(n2) = exp(-v^2 (2 u.g2 + (2 n2 + 1) g22)) of
+  // consecutive orders, itself multiplied by Qg = exp(-2 v^2 g22) per step (three exponentials
+  // per row instead of one per order)
+  const double v2 = ve * ve, eaz = exp(-az * az), Qg = exp(-2.0 * v2 * g22);
+  double ux_cur = 0.0, uy_cur = 0.0;
+  auto term = [&](int n2, cplx E, double ez) {
+    const double ax = ux_cur + n2 * P.g2x, ay = uy_cur + n2 * P.g2y;
+    // term count of the propagating orders' Faddeeva series for the largest ve (x = b ve)
+    const double bp2 = P.k2 - ax * ax - ay * ay;
+    const int nt = bp2 > 0.0 ? w_series_terms(sqrt(bp2) * vemax) : 0;
+    cplx F, Gz;
+    spectral_term_w(P, ax, ay, E, pz, ve, az, nt, et, ez, F, Gz);
+    S += F;
+    Sx += F * ax;
+    Sy += F * ay;
+    Sz += Gz;
+  };
   for (int n1 = n1lo; n1 <= n1hi; ++n1) {
     const double ux = -P.vk0 + n1 * P.g1x, uy = -P.vk1 + n1 * P.g1y;
     int lo, hi;
     if (!row_range(ux, uy, P.g2x, P.g2y, g22, A2, lo, hi)) continue;
-    const int c = row_centre_r(ux * P.g2x + uy * P.g2y, 1.0 / g22, lo, hi);
-    row_walk(lo, hi, c, cexpi(ap + n1 * gp1 + c * gp2), step, [&](int n2, cplx E) {
-      const double ax = ux + n2 * P.g2x, ay = uy + n2 * P.g2y;
-      // term count of the propagating orders' Faddeeva series for the largest ve (x = b ve)
-      const double bp2 = P.k2 - ax * ax - ay * ay;
-      const int nt = bp2 > 0.0 ? w_series_terms(sqrt(bp2) * vemax) : 0;
-      cplx F, Gz;
-      spectral_term_w(P, ax, ay, E, pz, ve, az, nt, et, F, Gz);
-      S += F;
-      Sx += F * ax;
-      Sy += F * ay;
-      Sz += Gz;
-    });
+    const double ue = ux * P.g2x + uy * P.g2y;
+    const int c = row_centre_r(ue, 1.0 / g22, lo, hi);
+    ux_cur = ux;
+    uy_cur = uy;
+    const double axc = ux + c * P.g2x, ayc = uy + c * P.g2y;
+    const double Gc = exp(-v2 * (axc * axc + ayc * ayc - P.k2)) * eaz;
+    const double Rb = exp(v2 * (2.0 * ue + (2.0 * c - 1.0) * g22));  // G(c - 1) / G(c)
+    const cplx E0 = cexpi(ap + n1 * gp1 + c * gp2);
+    cplx E = E0;
+    double Gv = Gc, R = exp(-v2 * (2.0 * ue + (2.0 * c + 1.0) * g22));  // G(c + 1) / G(c)
+    for (int n2 = c; n2 <= hi; ++n2) {
+      term(n2, E, Gv);
+      E = E * step;
+      Gv *= R;
+      R *= Qg;
+    }
+    E = E0 * back;
+    Gv = Gc * Rb;
+    R = Rb * Qg;
+    for (int n2 = c - 1; n2 >= lo; --n2) {
+      term(n2, E, Gv);
+      E = E * back;
+      Gv *= R;
+      R *= Qg;
+    }
   }
   const double cd = 0.25 * P.aidd;
   o.g += cmul_i(S) * cd;
