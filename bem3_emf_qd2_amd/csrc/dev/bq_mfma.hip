@@ -16,9 +16,9 @@
 //                        block, four planes (re hi, re lo, im hi, im lo) of 64 lanes x 8 bf16,
 //                        in exactly the order the MFMA reads them (one 16-byte load per lane);
 //                        the block counts are padded to multiples of 4 with zero tiles.
-//   k_cgemm_split        one workgroup (4 waves) per 128 x 128 tile of C; per 16-wide k step
-//                        the workgroup's A and B tiles (16 KB each) are staged in LDS through
-//                        registers, double-buffered with one barrier per step; each wave
+//   k_cgemm_split_d      one workgroup (4 waves) per 128 x 128 tile of C; per 16-wide k step
+//                        the workgroup's A and B tiles (16 KB each) are loaded straight into LDS
+//                        (global_load_lds_dwordx4), double-buffered with one barrier per step; each wave
 //                        owns 64 x 64 of C as 2 x 2 blocks of v_mfma_f32_32x32x16_bf16 with
 //                        separate real and imaginary accumulators (128 floats per lane), and
 //                        subtracts them from C at the end (B's tile goes into the MFMA's
@@ -43,9 +43,6 @@ constexpr int kBlk = 32;             // MFMA block edge
 constexpr int kKStep = 16;           // complex k per stage (the MFMA's k)
 constexpr int kTileU4 = 64;          // one 32 x 16 plane = 64 lanes x 16 bytes
 constexpr int kStageU4 = 4 * 4 * kTileU4;  // 4 blocks x 4 planes = 16 KB
-#ifndef BQ_GEMM_DIRECT
-#define BQ_GEMM_DIRECT 1
-#endif
 constexpr int kGroupRows = 8;        // tile rows per swizzle group
 
 __device__ __forceinline__ unsigned bf16_rne(float x)
@@ -127,134 +124,12 @@ __device__ __forceinline__ f32x16 mma(bf16x8 a, bf16x8 b, f32x16 c)
   return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
 }
 
-// C (m x n, ldc) -= A B from the packed operands (STORE: C = -A B, C not read); grid =
-// tiles_r * tiles_c workgroups of 256.
-template <bool STORE>
-__global__ __launch_bounds__(256, 2) void k_cgemm_split(int m, int n, int tiles_r, int tiles_c, int Kb,
-                                                        const uint4* __restrict__ Ap, const uint4* __restrict__ Bp,
-                                                        float2* __restrict__ C, long ldc)
-{
-  __shared__ uint4 lds[2][2][kStageU4];  // [buffer][A, B]
-  const int ntiles = tiles_r * tiles_c;
-  int t = blockIdx.x;
-  if ((ntiles & 7) == 0) t = (t & 7) * (ntiles >> 3) + (t >> 3);  // contiguous run per XCD
-  const int span = kGroupRows * tiles_c, g = t / span, first = g * kGroupRows;
-  const int gsz = min(kGroupRows, tiles_r - first), in = t - g * span;
-  const int tr = first + in % gsz, tc = in / gsz;
-
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, wr = w >> 1, wc = w & 1;
-  const int Mb = tiles_r * 4, Nb = tiles_c * 4;
-  const uint4* a_src = Ap + (long)(tr * 4) * 4 * kTileU4;
-  const uint4* b_src = Bp + (long)(tc * 4) * 4 * kTileU4;
-  const long a_step = (long)Mb * 4 * kTileU4, b_step = (long)Nb * 4 * kTileU4;
-
-  f32x16 cr[2][2], ci[2][2];
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      cr[i][j] = f32x16{};
-      ci[i][j] = f32x16{};
-    }
-
-  // the staging registers are named one by one: an array here is not kept in registers
-  uint4 ra0, ra1, ra2, ra3, rb0, rb1, rb2, rb3;
-#define BQ_STAGE_LOAD(pa, pb)                                                                   \
-  ra0 = (pa)[tid];  ra1 = (pa)[tid + 256];  ra2 = (pa)[tid + 512];  ra3 = (pa)[tid + 768];      \
-  rb0 = (pb)[tid];  rb1 = (pb)[tid + 256];  rb2 = (pb)[tid + 512];  rb3 = (pb)[tid + 768];
-#define BQ_STAGE_STORE(b)                                                                       \
-  lds[b][0][tid] = ra0;  lds[b][0][tid + 256] = ra1;  lds[b][0][tid + 512] = ra2;               \
-  lds[b][0][tid + 768] = ra3;  lds[b][1][tid] = rb0;  lds[b][1][tid + 256] = rb1;               \
-  lds[b][1][tid + 512] = rb2;  lds[b][1][tid + 768] = rb3;
-  BQ_STAGE_LOAD(a_src, b_src)
-  BQ_STAGE_STORE(0)
-  __syncthreads();
-
-  for (int kb = 0; kb < Kb; ++kb) {
-    const int buf = kb & 1;
-    const bool more = kb + 1 < Kb;
-    if (more) {
-      const uint4* an = a_src + (long)(kb + 1) * a_step;
-      const uint4* bn = b_src + (long)(kb + 1) * b_step;
-      BQ_STAGE_LOAD(an, bn)
-    }
-    const uint4* la = lds[buf][0];
-    const uint4* lb = lds[buf][1];
-    uint4 b_op[2][4];
-#pragma unroll
-    for (int j = 0; j < 2; ++j)
-#pragma unroll
-      for (int p = 0; p < 4; ++p) b_op[j][p] = lb[((2 * wc + j) * 4 + p) * kTileU4 + lane];
-#pragma unroll
-    for (int i = 0; i < 2; ++i) {
-      uint4 a_op[4];
-#pragma unroll
-      for (int p = 0; p < 4; ++p) a_op[p] = la[((2 * wr + i) * 4 + p) * kTileU4 + lane];
-      const bf16x8 arh = as_bf(a_op[0]), arl = as_bf(a_op[1]), aih = as_bf(a_op[2]), ail = as_bf(a_op[3]);
-      const bf16x8 naih = neg_bf(a_op[2]), nail = neg_bf(a_op[3]);
-#pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        const bf16x8 brh = as_bf(b_op[j][0]), brl = as_bf(b_op[j][1]), bih = as_bf(b_op[j][2]),
-                     bil = as_bf(b_op[j][3]);
-        // B's tile in the MFMA's first operand: the block comes out transposed, with C's
-        // rows along the lanes (see the write-back)
-        f32x16 r = cr[i][j], q = ci[i][j];
-        r = mma(brh, arh, r);   // re: ar br - ai bi
-        q = mma(bih, arh, q);   // im: ar bi + ai br
-        r = mma(brl, arh, r);
-        q = mma(bil, arh, q);
-        r = mma(brh, arl, r);
-        q = mma(bih, arl, q);
-        r = mma(bih, naih, r);
-        q = mma(brh, aih, q);
-        r = mma(bil, naih, r);
-        q = mma(brl, aih, q);
-        r = mma(bih, nail, r);
-        q = mma(brh, ail, q);
-        cr[i][j] = r;
-        ci[i][j] = q;
-      }
-    }
-    if (more) {
-      BQ_STAGE_STORE(buf ^ 1)
-    }
-    __syncthreads();
-  }
-#undef BQ_STAGE_LOAD
-#undef BQ_STAGE_STORE
-
-  // 32 x 32 accumulator layout: register e of lane l is MFMA row 8*(e/4) + 4*(l/32) + e%4
-  // and column l%32, i.e. column 8*(e/4) + 4*(l/32) + e%4 and row l%32 of the C block: per
-  // register the 64 lanes cover 32 consecutive rows of two columns (two 256-byte segments)
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const int row = tr * kTile + (2 * wr + i) * kBlk + (lane & 31);
-      if (row >= m) continue;
-      const int c0 = tc * kTile + (2 * wc + j) * kBlk + 4 * (lane >> 5);
-      float2* cr0 = C + row;
-      // all 16 loads of the block in flight before the first store; columns past n read
-      // column n - 1 (in bounds) and are not written
-      float2 v[16];
-#pragma unroll
-      for (int e = 0; e < 16; ++e)
-        v[e] = STORE ? make_float2(0.f, 0.f) : cr0[(long)min(c0 + 8 * (e >> 2) + (e & 3), n - 1) * ldc];
-#pragma unroll
-      for (int e = 0; e < 16; ++e) {
-        const int col = c0 + 8 * (e >> 2) + (e & 3);
-        v[e].x -= cr[i][j][e];
-        v[e].y -= ci[i][j][e];
-        if (col < n) cr0[(long)col * ldc] = v[e];
-      }
-    }
-}
-
-// ---- the same product with the stages loaded straight into LDS ----------------------------
-// global_load_lds_dwordx4 puts each lane's 16 bytes at the wavefront's LDS base + 16 x lane, the
-// layout the MFMA reads, so a stage needs no staging registers and no LDS stores; the two stage
-// buffers are separate LDS objects and the loop is unrolled by two, so that the loads into one
-// buffer do not make the reads of the other wait for them.
+// ---- C (m x n, ldc) -= A B from the packed operands (STORE: C = -A B, C not read) ---------
+// grid = tiles_r * tiles_c workgroups of 256.  global_load_lds_dwordx4 puts each lane's 16 bytes
+// at the wavefront's LDS base + 16 x lane, the layout the MFMA reads, so a stage needs no
+// staging registers and no LDS stores; the two stage buffers are separate LDS objects and the
+// loop is unrolled by two, so that the loads into one buffer do not make the reads of the other
+// wait for them.
 __device__ __forceinline__ void stage_direct(uint4* la, uint4* lb, const uint4* pa, const uint4* pb, int tid, int w)
 {
 #pragma unroll
@@ -430,18 +305,12 @@ int split_gemm(hipStream_t s, int m, int n, int k, const float* A, long lda, con
   k_pack_a<<<(unsigned)((ta + 255) / 256), 256, 0, s>>>(m, k, (const float2*)A, lda, Mb, Kb, Ap);
   k_pack_b<<<(unsigned)((tb + 255) / 256), 256, 0, s>>>(k, n, (const float2*)B, ldb, Nb, Kb, Bp);
   const unsigned grid = (unsigned)(tiles_r * tiles_c);
-  if (BQ_GEMM_DIRECT && store)
+  if (store)
     BQ_KTIME("k_cgemm_split", s,
              k_cgemm_split_d<true><<<grid, 256, 0, s>>>(m, n, tiles_r, tiles_c, Kb, Ap, Bp, (float2*)C, ldc));
-  else if (BQ_GEMM_DIRECT)
-    BQ_KTIME("k_cgemm_split", s,
-             k_cgemm_split_d<false><<<grid, 256, 0, s>>>(m, n, tiles_r, tiles_c, Kb, Ap, Bp, (float2*)C, ldc));
-  else if (store)
-    BQ_KTIME("k_cgemm_split", s,
-             k_cgemm_split<true><<<grid, 256, 0, s>>>(m, n, tiles_r, tiles_c, Kb, Ap, Bp, (float2*)C, ldc));
   else
     BQ_KTIME("k_cgemm_split", s,
-             k_cgemm_split<false><<<grid, 256, 0, s>>>(m, n, tiles_r, tiles_c, Kb, Ap, Bp, (float2*)C, ldc));
+             k_cgemm_split_d<false><<<grid, 256, 0, s>>>(m, n, tiles_r, tiles_c, Kb, Ap, Bp, (float2*)C, ldc));
   BQ_HIP(hipGetLastError());
   return 0;
 }
