@@ -149,3 +149,9 @@ def lu_update_mode(mode=None) -> str:
     cgemm).  Returns the mode in force before the call."""
     m = {None: -1, "fp32": 0, "bf16x3": 1}[mode]
     return ["fp32", "bf16x3"][lib().bq_lu_update_mode(m)]
+
+
+def lu_lookahead(on=None) -> bool:
+    """Query (None) or switch (True / False) the look-ahead of the dense factorisations
+    (bq_lu_lookahead).  Returns the setting in force before the call."""
+    return bool(lib().bq_lu_lookahead(-1 if on is None else int(bool(on))))

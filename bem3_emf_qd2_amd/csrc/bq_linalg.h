@@ -41,8 +41,11 @@ int bq_cgemm_split_store_dev(void *stream, int m, int n, int k, const float *A, 
 /* 1: FP32 factorization updates on the bf16x3 kernel, 0: rocBLAS cgemm; < 0 queries.
  * Returns the previous mode. */
 int bq_lu_update_mode(int mode);
-/* Free the per-stream state of the solves (rocBLAS handle, split-GEMM workspace) before the
- * stream is destroyed. */
+/* 1: look-ahead of the factorizations (a second stream per stream), 0: off; < 0 queries.
+ * Returns the previous mode. */
+int bq_lu_lookahead(int mode);
+/* Free the per-stream state of the solves (rocBLAS handle, split-GEMM workspace, look-ahead
+ * stream) before the stream is destroyed. */
 void bq_lu_release_stream(void *stream);
 void bq_cgemm_split_release(void *stream);
 

@@ -14,7 +14,9 @@
 //   5. rocBLAS zgemm updates the rest of the outer panel.
 // Then the panel's interchanges go to all other columns, U12 of the outer block row is
 // solved with the panel's unit L11 (32-column steps), and one rank-512 zgemm updates the
-// trailing matrix (MFMA on gfx950: FP64, or FP32 for the single-precision factors).  A rank-32 update per 32 columns would stream the
+// trailing matrix (MFMA on gfx950: FP64, or FP32 for the single-precision factors); with the
+// look-ahead the next panel's columns are updated first and the rest on a second stream,
+// concurrently with the next panel's steps.  A rank-32 update per 32 columns would stream the
 // trailing matrix through HBM 8x as often; at order 36 648 that traffic alone is ~10 s.
 // Solves: the permutation getrf stored applies the pivots in one gather, then, per 512-row
 // block, 32-row diagonal steps X_j = inv(T_jj) X_j (k_apply_blk) with in-block zgemm
@@ -789,6 +791,55 @@ rocblas_handle handle_for(hipStream_t s)
   return h;
 }
 
+// The look-ahead stream of getrf per (device, stream): a second stream at the default priority
+// with two events (panel factored on the main stream; right-hand update done on this one),
+// created on first use and kept for the process.
+struct SideSlot {
+  int dev;
+  hipStream_t s;
+  hipStream_t side;
+  hipEvent_t ev[2];
+};
+std::vector<SideSlot> g_side;
+
+bool side_for(hipStream_t s, SideSlot* out)
+{
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  std::lock_guard<std::mutex> lk(g_hmu);
+  for (const SideSlot& q : g_side)
+    if (q.dev == dev && q.s == s) {
+      *out = q;
+      return true;
+    }
+  SideSlot q{dev, s, nullptr, {nullptr, nullptr}};
+  if (hipStreamCreateWithFlags(&q.side, hipStreamNonBlocking) != hipSuccess) return false;
+  for (hipEvent_t& e : q.ev)
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+      for (hipEvent_t& f : q.ev)
+        if (f) (void)hipEventDestroy(f);
+      (void)hipStreamDestroy(q.side);
+      return false;
+    }
+  g_side.push_back(q);
+  *out = q;
+  return true;
+}
+
+// Look-ahead of the factorization (1, default; BQ_LU_LOOKAHEAD=0 sets 0 at start-up)
+std::atomic<int> g_lookahead{-1};
+
+int lookahead_mode()
+{
+  int m = g_lookahead.load();
+  if (m < 0) {
+    const char* e = getenv("BQ_LU_LOOKAHEAD");
+    m = (e && strcmp(e, "0") == 0) ? 0 : 1;
+    g_lookahead.store(m);
+  }
+  return m;
+}
+
 template <class R>
 int gemm_op(hipStream_t s, int ta, int tb, int m, int n, int k, typename P<R>::V alpha, const typename P<R>::V* A,
             long lda, const typename P<R>::V* B, long ldb, typename P<R>::V beta, typename P<R>::V* C, long ldc)
@@ -939,30 +990,85 @@ int getrs_blkinv(hipStream_t s, int n, int nrhs, const typename P<R>::V* A, long
   return 0;
 }
 
+// The factored outer panel j0 .. j0+jb2-1 applied to the columns c0 .. c0+w-1 right of it,
+// on stream st: the panel's interchanges, U12 = inv(L11) A12 (one product with
+// Wneg = -inv(L11) when given and w >= kSplitMin, else 32-row steps) and the trailing
+// update A22 -= L21 U12.
+template <class R>
+int panel_right(hipStream_t st, int n, typename P<R>::V* A, long lda, const int* ipiv, const typename P<R>::V* inv,
+                int j0, int jb2, int c0, int w, const typename P<R>::V* Wneg)
+{
+  using V = typename P<R>::V;
+  const V one = P<R>::v(1.0, 0.0), mone = P<R>::v(-1.0, 0.0);
+  const int je = j0 + jb2;
+  V* Ar = A + (long)c0 * lda;
+  k_swap<R><<<(w + 255) / 256, 256, 0, st>>>(A, lda, c0, w, j0, jb2, ipiv);
+  BQ_HIP(hipGetLastError());
+  bool done = false;
+  if constexpr (std::is_same<R, float>::value) {
+    if (Wneg && w >= kSplitMin) {
+      if (apply_block_inverse(st, w, Wneg, Ar + j0, lda)) return -1;
+      done = true;
+    }
+  }
+  for (int j = j0; j < je && !done; j += kNB) {
+    const int jb = min(kNB, je - j);
+    k_apply_blk<R><<<(w + 63) / 64, 256, 0, st>>>(inv + (long)(j / kNB) * kBlkElems, jb, Ar + j, lda, w);
+    BQ_HIP(hipGetLastError());
+    if (je - j - jb > 0 &&
+        gemm<R>(st, je - j - jb, w, jb, mone, A + (long)j * lda + (j + jb), lda, Ar + j, lda, one, Ar + j + jb, lda))
+      return -1;
+  }
+  int rc = 0;
+  if (n - je > 0)
+    BQ_KTIME(P<R>::kt_u, st,
+             rc = gemm<R>(st, n - je, w, jb2, mone, A + (long)j0 * lda + je, lda, Ar + j0, lda, one, Ar + je, lda));
+  return rc ? -1 : 0;
+}
+
 // In-place LU of the n x n column-major matrix A (device); inv receives the inverted
 // diagonal blocks.  Returns 0, a positive k if U(k,k) is exactly zero (1-based), or < 0.
+// With the look-ahead (n > 1024), the columns right of each outer panel are split: the next
+// panel's 512 columns are brought up to date on s, ahead of the rest, which the look-ahead
+// stream updates while s already factors that next panel; the latency-bound 32-column steps
+// of one panel thus overlap the matrix-core update of the previous one.  Before the next
+// panel's interchanges reach the columns left of it (whose L21 the update reads) and its
+// own right-hand work starts, s waits for that update.
 template <class R>
 int getrf(hipStream_t s, int n, typename P<R>::V* A, long lda, int* ipiv, typename P<R>::V* inv)
 {
   using V = typename P<R>::V;
   const V one = P<R>::v(1.0, 0.0), mone = P<R>::v(-1.0, 0.0);
   const int nleaf = (n + kTsRows - 1) / kTsRows;
+  SideSlot side{};
+  const bool la = lookahead_mode() == 1 && n > 2 * kNBF;
+  if (la && !side_for(s, &side)) {
+    bq::set_error("getrf: look-ahead stream creation failed");
+    return -1;
+  }
   int* cand = nullptr;
   V* F = nullptr;
   BQ_HIP(hipMallocAsync((void**)&cand, sizeof(int) * (size_t)2 * nleaf * kNB, s));
   BQ_HIP(hipMallocAsync((void**)&F, sizeof(V) * kNB * kNB, s));
   // FP32 with the matrix-core updates: U12 of each full panel by one product with the
-  // panel's inverted L11 (Wp = -inv(L11))
+  // panel's inverted L11 (-inv(L11) in Wp; two alternating buffers with the look-ahead,
+  // whose stream may still read the previous panel's)
   bool blk_inv = false;
   V* Wp = nullptr;
+  const long wsz = (long)kNBF * kNBF;
   if constexpr (std::is_same<R, float>::value) {
     if (update_mode() == 1 && n >= 2 * kNBF) {
       blk_inv = true;
-      BQ_HIP(hipMallocAsync((void**)&Wp, sizeof(V) * kNBF * kNBF, s));
+      BQ_HIP(hipMallocAsync((void**)&Wp, sizeof(V) * wsz * (la ? 2 : 1), s));
     }
   }
   const int kt_all = bq::ktime_begin(P<R>::kt_f, s);
-  for (int j0 = 0; j0 < n; j0 += kNBF) {
+  if (la) {  // the look-ahead stream starts from A as s has it
+    BQ_HIP(hipEventRecord(side.ev[0], s));
+    BQ_HIP(hipStreamWaitEvent(side.side, side.ev[0], 0));
+  }
+  bool pend = false;  // a right-hand update outstanding on the look-ahead stream
+  for (int j0 = 0, pi = 0; j0 < n; j0 += kNBF, ++pi) {
     const int jb2 = min(kNBF, n - j0), je = j0 + jb2;
     const int kt_p = bq::ktime_begin(P<R>::kt_p, s);
     for (int j = j0; j < je; j += kNB) {
@@ -994,41 +1100,32 @@ int getrf(hipStream_t s, int n, typename P<R>::V* A, long lda, int* ipiv, typena
           return -1;
       }
     }
-    // the panel's interchanges on the columns left and right of it
+    if (pend) {
+      BQ_HIP(hipStreamWaitEvent(s, side.ev[1], 0));
+      pend = false;
+    }
+    // the panel's interchanges on the columns left of it
     if (j0 > 0) k_swap<R><<<(j0 + 255) / 256, 256, 0, s>>>(A, lda, 0, j0, j0, jb2, ipiv);
+    BQ_HIP(hipGetLastError());
     const int right = n - je;
-    if (right > 0) {
-      k_swap<R><<<(right + 255) / 256, 256, 0, s>>>(A, lda, je, right, j0, jb2, ipiv);
-      bool done = false;
-      if constexpr (std::is_same<R, float>::value) {
-        if (blk_inv && jb2 == kNBF && right >= kSplitMin) {
-          if (block_inverses<R>(s, 1, A + (long)j0 * lda + j0, lda, inv + (long)(j0 / kNB) * kBlkElems, false, Wp) ||
-              apply_block_inverse(s, right, Wp, A + (long)je * lda + j0, lda))
-            return -1;
-          done = true;
-        }
-      }
-      // U12 of the block row: unit-lower solve with the panel's L11, 32 rows at a time
-      for (int j = j0; j < je && !done; j += kNB) {
-        const int jb = min(kNB, je - j);
-        k_apply_blk<R><<<(right + 63) / 64, 256, 0, s>>>(inv + (long)(j / kNB) * kBlkElems, jb,
-                                                         A + (long)je * lda + j, lda, right);
-        BQ_HIP(hipGetLastError());
-        if (je - j - jb > 0 && gemm<R>(s, je - j - jb, right, jb, mone, A + (long)j * lda + (j + jb), lda,
-                                       A + (long)je * lda + j, lda, one, A + (long)je * lda + (j + jb), lda))
-          return -1;
-      }
+    V* Wcur = nullptr;
+    if (right > 0 && blk_inv && jb2 == kNBF && right >= kSplitMin) {
+      Wcur = Wp + (la ? (pi & 1) * wsz : 0);
+      if (block_inverses<R>(s, 1, A + (long)j0 * lda + j0, lda, inv + (long)(j0 / kNB) * kBlkElems, false, Wcur))
+        return -1;
     }
     bq::ktime_end(kt_p, s);
-    BQ_HIP(hipGetLastError());
-    if (right > 0) {
-      int rc = 0;
-      BQ_KTIME(P<R>::kt_u, s,
-               rc = gemm<R>(s, right, right, jb2, mone, A + (long)j0 * lda + je, lda, A + (long)je * lda + j0, lda, one,
-                            A + (long)je * lda + je, lda));
-      if (rc) return -1;
+    const int r1 = la ? min(right, kNBF) : right, r2 = right - r1;
+    if (r2 > 0) {
+      BQ_HIP(hipEventRecord(side.ev[0], s));
+      BQ_HIP(hipStreamWaitEvent(side.side, side.ev[0], 0));
+      if (panel_right<R>(side.side, n, A, lda, ipiv, inv, j0, jb2, je + r1, r2, Wcur)) return -1;
+      BQ_HIP(hipEventRecord(side.ev[1], side.side));
+      pend = true;
     }
+    if (r1 > 0 && panel_right<R>(s, n, A, lda, ipiv, inv, j0, jb2, je, r1, Wcur)) return -1;
   }
+  if (pend) BQ_HIP(hipStreamWaitEvent(s, side.ev[1], 0));
   k_ipiv_perm<<<1, 1, 0, s>>>(n, ipiv, perm_of(inv, n));
   BQ_HIP(hipGetLastError());
   bq::ktime_end(kt_all, s);
@@ -1229,6 +1326,34 @@ void bq_lu_release_stream(void* stream)
       }
   }
   bq_cgemm_split_release(stream);
+  // the look-ahead stream of this stream, with its handle and workspace
+  SideSlot q{};
+  bool found = false;
+  {
+    std::lock_guard<std::mutex> lk(g_hmu);
+    for (size_t i = 0; i < g_side.size(); ++i)
+      if (g_side[i].dev == dev && g_side[i].s == (hipStream_t)stream) {
+        q = g_side[i];
+        g_side.erase(g_side.begin() + i);
+        found = true;
+        break;
+      }
+  }
+  if (found) {
+    (void)hipStreamSynchronize(q.side);
+    bq_lu_release_stream(q.side);
+    for (hipEvent_t e : q.ev) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(q.side);
+  }
+}
+
+// Look-ahead of the factorization: 1 on (default), 0 off; mode < 0 only queries.  Returns the
+// mode in force before.
+int bq_lu_lookahead(int mode)
+{
+  const int old = lookahead_mode();
+  if (mode == 0 || mode == 1) g_lookahead.store(mode);
+  return old;
 }
 
 // Update mode of the FP32 factorization: 1 bf16x3 matrix-core kernel (default), 0 rocBLAS

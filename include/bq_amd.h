@@ -204,6 +204,15 @@ double bq_assembly_tolerance(double tol);
  * per term), 0 = FP32 products (rocBLAS cgemm).  mode < 0 only queries.  BQ_LU_UPDATE=fp32
  * sets 0 at start-up.  Returns the mode in force before the call. */
 int bq_lu_update_mode(int mode);
+/* Look-ahead of the dense factorisations (FP64 and FP32; no counterpart in the reference, whose
+ * PARDISO call, d3qd2_src/d3qd2_mkl_solver.c:18-148, schedules its own factorisation):
+ * 1 (default) = for order > 1024 the columns right of each 512-column panel are updated in
+ * two parts, the next panel's first and the rest on a second stream per calling stream,
+ * concurrently with the next panel's pivot steps; 0 = one stream.  The same algorithm either
+ * way; the products may be tiled differently, so results agree to rounding.  mode < 0 only
+ * queries.  BQ_LU_LOOKAHEAD=0 sets 0 at start-up.
+ * Returns the mode in force before the call. */
+int bq_lu_lookahead(int mode);
 /* C -= A B, complex FP32, column-major device matrices (A m x k, B k x n, C m x n), with the
  * bf16x3 products above; the building block of the FP32 factorisation's rank-512 updates and
  * many-right-hand-side solves.  stream: a hipStream_t or NULL.  Returns 0 or -1. */

@@ -87,6 +87,45 @@ def test_gpu_cgesv_random(gpu_lib, n, nrhs, mode):
     assert fw < cond * (n + 4) * u
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["z", "c"])
+def test_gpu_lu_lookahead_on_off(gpu_lib, prec):
+    """The factorisation's look-ahead (for order > 1024 the columns right of each 512-column
+    panel are updated in two parts, the next panel's on the calling stream and the rest on a
+    second stream while the next panel is factored) against the one-stream order, at n = 2600:
+    five full panels and a partial one, so the second part runs 1576 .. 40 columns wide (the
+    last below the 128 at which the FP32 path solves U12 by one product with inv(L11)).  A
+    matrix with a small leading block forces interchanges across panels.  Both solves meet
+    the backward-error bound of their precision and agree within cond(A) (n + 4) u."""
+    n, nrhs = 2600, 5
+    rng = np.random.default_rng(29)
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    A[:700, :700] *= 1e-3
+    B = rng.standard_normal((n, nrhs)) + 1j * rng.standard_normal((n, nrhs))
+    if prec == "c":
+        A = A.astype(np.complex64).astype(complex)
+        B = B.astype(np.complex64).astype(complex)
+        u, bw_max = UPDATE_U["bf16x3"], 16 * UPDATE_U["bf16x3"]
+        solve = lambda: gpu_lib.cgesv(A, B).astype(complex)
+    else:
+        u, bw_max = 1.1e-16, 1e-13
+        solve = lambda: gpu_lib.zgesv(A, B)
+    old = gpu_lib.lu_lookahead(True)
+    try:
+        X1 = solve()
+        gpu_lib.lu_lookahead(False)
+        X0 = solve()
+    finally:
+        gpu_lib.lu_lookahead(old)
+    cond = np.linalg.cond(A)
+    for X in (X1, X0):
+        bw = np.abs(A @ X - B).max() / (np.abs(A).max() * np.abs(X).max() * n)
+        assert bw < bw_max, bw
+    d = np.abs(X1 - X0).max() / np.abs(X0).max()
+    print(prec, "on/off", d, "cond", cond)
+    assert d < cond * (n + 4) * u
+
+
 def _bf16_split(x):
     """hi, lo (float64) of float32 x: hi = bf16(x), lo = bf16(x - hi), round to nearest even."""
     def rne(v):
