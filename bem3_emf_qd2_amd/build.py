@@ -35,6 +35,12 @@ HIP_FLAGS = [
 C_FLAGS = ["-O2", "-fPIC", "-std=gnu11", "-Wall", "-Wno-unused-variable",
            "-Wno-unused-but-set-variable", f"-I{ROOT / 'include'}", f"-I{CSRC}"]
 LINK_LIBS = ["-lrocblas", "-lm", "-lpthread"]
+# Per-source additions.  The assembly kernels' polynomial and constant-heavy code: with
+# machine loop-invariant code motion the compiler hoists every double constant of the lattice
+# sums into registers for the whole kernel (spilling scalars to vector lanes and vectors to
+# scratch) and copies them into the destinations of v_fmac_f64; without it the constants are
+# built next to their use in scalar registers.
+HIP_FILE_FLAGS = {"bq_assemble.hip": ["-mllvm", "--disable-machine-licm"]}
 
 
 def _run(cmd: list[str]) -> tuple[list[str], int, str]:
@@ -59,7 +65,8 @@ def build(verbose: bool = False) -> Path:
     hip, c = sources()
     jobs = []
     for s in hip:
-        jobs.append([HIPCC, *HIP_FLAGS, "-c", str(s), "-o", str(BUILD / (s.stem + ".hip.o"))])
+        jobs.append([HIPCC, *HIP_FLAGS, *HIP_FILE_FLAGS.get(s.name, []), "-c", str(s), "-o",
+                     str(BUILD / (s.stem + ".hip.o"))])
     for s in c:
         if s.suffix == ".cpp":
             jobs.append(["g++", *CXX_FLAGS, "-c", str(s), "-o", str(BUILD / (s.stem + ".cpp.o"))])

@@ -65,7 +65,7 @@ __device__ __forceinline__ double erfcx_tab(double x, const double* tab)
 __device__ __forceinline__ double erfcx_any(double x, const double* tab)
 {
   if (!tab) return erfcx(x);
-  return x >= 0.0 ? erfcx_tab(x, tab) : 2.0 * exp(x * x) - erfcx_tab(-x, tab);
+  return x >= 0.0 ? erfcx_tab(x, tab) : 2.0 * exp_fast(x * x) - erfcx_tab(-x, tab);
 }
 
 __device__ __forceinline__ double rsqrt_nr(double x)
@@ -73,8 +73,8 @@ __device__ __forceinline__ double rsqrt_nr(double x)
   // hardware reciprocal-square-root estimate refined by two Newton steps (full double precision)
   double y = __builtin_amdgcn_rsq(x);
   const double h = 0.5 * x;
-  y = y * fma(-h * y, y, 1.5);
-  return y * fma(-h * y, y, 1.5);
+  y = y * fma_s(-h * y, y, 1.5);
+  return y * fma_s(-h * y, y, 1.5);
 }
 
 __device__ __forceinline__ double sinc_of(double x, double sx)
@@ -103,8 +103,8 @@ __device__ __forceinline__ cplx w_series_n(double xs, double y, int nterm, const
 {
   const double x = fabs(xs);
   double s1 = 0.0, s2 = 0.0, s3 = 0.0, s4 = 0.0, s5 = 0.0;
-  const double ex2 = exp(-x * x);
-  const double e2ax = exp(2.0 * kFA * x), em2ax = 1.0 / e2ax;
+  const double ex2 = exp_fast(-x * x);
+  const double e2ax = exp_fast(2.0 * kFA * x), em2ax = 1.0 / e2ax;
   double t = exp(-kFA2), q = exp(-3.0 * kFA2);  // exp(-a^2 n^2), exp(-a^2 (2n+1))
   const double q2 = exp(-2.0 * kFA2);
   double p2 = 1.0, pm2 = 1.0, y2 = y * y;
@@ -123,7 +123,7 @@ __device__ __forceinline__ cplx w_series_n(double xs, double y, int nterm, const
     t *= q;
     q *= q2;
   }
-  const double ex2erfcx = (y > -6.0) ? ex2 * erfcx_any(y, et) : 2.0 * exp(y2 - x * x);
+  const double ex2erfcx = (y > -6.0) ? ex2 * erfcx_any(y, et) : 2.0 * exp_fast(y2 - x * x);
   double sxy, cxy, s2xy, c2xy;
   sincos_fast(xs * y, &sxy, &cxy);
   s2xy = 2.0 * sxy * cxy;
@@ -174,7 +174,7 @@ __device__ __forceinline__ cplx w_series_w(double x, double y, int nterm, sconst
       s5 = fma(c[5], r, s5);
     }
   }
-  const double ex2erfcx = (y > -6.0) ? ex2 * erfcx_any(y, et) : 2.0 * exp(y2 - x * x);
+  const double ex2erfcx = (y > -6.0) ? ex2 * erfcx_any(y, et) : 2.0 * exp_fast(y2 - x * x);
   double sxy, cxy, s2xy, c2xy;
   sincos_fast(x * y, &sxy, &cxy);
   s2xy = 2.0 * sxy * cxy;

@@ -559,8 +559,8 @@ __device__ __forceinline__ void spectral_term_w(const bq_qplan& P, double ax, do
     const double a1 = bv + az, a2 = bv - az;
     const double e12 = ez;
     double f1 = e12 * erfcx_tab(fabs(a1), et), f2 = e12 * erfcx_tab(fabs(a2), et);
-    if (a1 <= 0.0) f1 = 2.0 * exp(b * pz) - f1;
-    if (a2 <= 0.0) f2 = 2.0 * exp(-b * pz) - f2;
+    if (a1 <= 0.0) f1 = 2.0 * exp_fast(b * pz) - f1;
+    if (a2 <= 0.0) f2 = 2.0 * exp_fast(-b * pz) - f2;
     F = mk(ce.im, -ce.re) * ((f1 + f2) * ib);
     Gz = ce * (f1 - f2);
   }
@@ -592,7 +592,7 @@ __device__ void qpgf_fourier_w(const bq_qplan& P, double px, double py, double p
         F = dz * ib;
       } else if (b2 < 0.0) {
         const double ib = rsqrt_nr(-b2), b = -b2 * ib;
-        dz = E * exp(-b * az);
+        dz = E * exp_fast(-b * az);
         F = mk(dz.im, -dz.re) * ib;  // -i dz / b
       } else {
         qpgf_flag(P, -12);
@@ -643,7 +643,7 @@ __device__ void qpgf_ewald_w(const bq_qplan& P, double px, double py, double pz,
         if (rl2 <= Rmax2) {
           const double irl = rsqrt_nr(rl2), rl = rl2 * irl;
           const double y = 0.5 * rl * ive;
-          const cplx cef = Eb * exp(ke2 - y * y);
+          const cplx cef = Eb * exp_fast(ke2 - y * y);
           const cplx w = wrow ? faddeeva_w_w(ke, y, nterm, wrow, et) : faddeeva_w_n(ke, y, nterm, et);
           S += cef * (irl * w.re);
           const cplx cf = cef * (irl * irl * (irl * w.re - P.k * w.im + gcon));
@@ -677,7 +677,7 @@ __device__ void qpgf_ewald_w(const bq_qplan& P, double px, double py, double pz,
   // carried from the row's centre by the ratio R(n2) = exp(-v^2 (2 u.g2 + (2 n2 + 1) g22)) of
   // consecutive orders, itself multiplied by Qg = exp(-2 v^2 g22) per step (three exponentials
   // per row instead of one per order)
-  const double v2 = ve * ve, eaz = exp(-az * az), Qg = exp(-2.0 * v2 * g22);
+  const double v2 = ve * ve, eaz = exp_fast(-az * az), Qg = exp_fast(-2.0 * v2 * g22);
   double ux_cur = 0.0, uy_cur = 0.0;
   auto term = [&](int n2, cplx E, double ez) {
     const double ax = ux_cur + n2 * P.g2x, ay = uy_cur + n2 * P.g2y;
@@ -700,11 +700,11 @@ __device__ void qpgf_ewald_w(const bq_qplan& P, double px, double py, double pz,
     ux_cur = ux;
     uy_cur = uy;
     const double axc = ux + c * P.g2x, ayc = uy + c * P.g2y;
-    const double Gc = exp(-v2 * (axc * axc + ayc * ayc - P.k2)) * eaz;
-    const double Rb = exp(v2 * (2.0 * ue + (2.0 * c - 1.0) * g22));  // G(c - 1) / G(c)
+    const double Gc = exp_fast(-v2 * (axc * axc + ayc * ayc - P.k2)) * eaz;
+    const double Rb = exp_fast(v2 * (2.0 * ue + (2.0 * c - 1.0) * g22));  // G(c - 1) / G(c)
     const cplx E0 = cexpi(ap + n1 * gp1 + c * gp2);
     cplx E = E0;
-    double Gv = Gc, R = exp(-v2 * (2.0 * ue + (2.0 * c + 1.0) * g22));  // G(c + 1) / G(c)
+    double Gv = Gc, R = exp_fast(-v2 * (2.0 * ue + (2.0 * c + 1.0) * g22));  // G(c + 1) / G(c)
     for (int n2 = c; n2 <= hi; ++n2) {
       term(n2, E, Gv);
       E = E * step;

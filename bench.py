@@ -295,11 +295,42 @@ def gemm_probe():
         rt.hipDeviceSynchronize()
         rb = (time.perf_counter() - t) / reps
         fl = 8.0 * m * n * k
-        print(json.dumps(dict(m=m, n=n, k=k, split_kernel_ms=kms, split_wall_ms=wall * 1e3, split_tfs=fl / kms / 1e9,
-                              split_bf16_frac=3 * fl / kms / 1e9 / 2500.0, rocblas_ms=rb * 1e3,
-                              rocblas_tfs=fl / rb / 1e12)), flush=True)
         for d in (A, B, C):
             d.free()
+        lmm, lup = _library_bf16_ms(m, n, k)
+        print(json.dumps(dict(m=m, n=n, k=k, split_kernel_ms=kms, split_wall_ms=wall * 1e3, split_tfs=fl / kms / 1e9,
+                              split_bf16_frac=3 * fl / kms / 1e9 / 2500.0, rocblas_ms=rb * 1e3,
+                              rocblas_tfs=fl / rb / 1e12, library_bf16_product_ms=lmm,
+                              library_bf16_product_frac=3 * fl / lmm / 1e9 / 2500.0, library_bf16_update_ms=lup,
+                              library_bf16_update_frac=3 * fl / lup / 1e9 / 2500.0)), flush=True)
+
+
+def _library_bf16_ms(m, n, k):
+    """The same bf16 matrix work as one real GEMM in the vendor library (torch -> hipBLASLt):
+    C' (2m x n, FP32) -= A' B' with A' = 2m x 6k, B' = 6k x n in bf16 (the complex product as a
+    2 x 2 real block, each operand's hi, lo, hi split stacked along k), i.e. the library's rate for
+    the matrix-core work the split kernel does.  Returns the product alone (C' = A' B', C not
+    read) and the update (addmm, beta = 1, alpha = -1), each the mean of five calls after one
+    (torch's events)."""
+    import torch
+    a = torch.randn(2 * m, 6 * k, device="cuda", dtype=torch.bfloat16)
+    b = torch.randn(6 * k, n, device="cuda", dtype=torch.bfloat16)
+    c = torch.zeros(2 * m, n, device="cuda", dtype=torch.float32)
+    out = []
+    for f in (lambda: torch.mm(a, b, out_dtype=torch.float32),
+              lambda: torch.addmm(c, a, b, out_dtype=torch.float32, beta=1.0, alpha=-1.0)):
+        f()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(5):
+            f()
+        e1.record()
+        torch.cuda.synchronize()
+        out.append(e0.elapsed_time(e1) / 5)
+    del a, b, c
+    torch.cuda.empty_cache()
+    return out
 
 
 def _cpu_baseline(model_dat, w, probes, gpu_solve_dims):
