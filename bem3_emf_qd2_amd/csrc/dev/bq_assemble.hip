@@ -229,10 +229,14 @@ __global__ void __launch_bounds__(256) k_classify(const double* __restrict__ E, 
 // converged, where lanes with their own bounds and method diverge at every row end.  A lane
 // past the rule's last point or without a live node evaluates the first live lane's node at
 // the rule's first point and discards the result.
-// The per-lane sums live in LDS rather than registers: the lattice sums need nearly 256
-// registers, and keeping the kernel within 256 lets two wavefronts share each SIMD.
+// The per-lane sums live in LDS rather than registers, and the kernel is held to 128
+// registers so that four wavefronts share each SIMD (the lattice sums' long dependent FP64
+// chains need the other wavefronts to hide their latency more than they need registers:
+// the spills this costs go to scratch that stays in cache; at C4 k_q0_gl took 2.44 s per
+// point at two wavefronts, 2.06 s at three and 2.01 s at four; at five the LDS limits the
+// CU to four workgroups and the 102 registers spill heavily, 4.07 s).
 #ifndef BQ_Q0GL_WAVES
-#define BQ_Q0GL_WAVES 2
+#define BQ_Q0GL_WAVES 4
 #endif
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BQ_Q0GL_WAVES, BQ_Q0GL_WAVES)))
 k_q0_gl(bq_qplan P, const double* __restrict__ E, const double* __restrict__ ren, const int* __restrict__ sid, int dNe,
@@ -327,8 +331,8 @@ k_q0_gl(bq_qplan P, const double* __restrict__ E, const double* __restrict__ ren
 }
 
 // domain 0, flagged pairs: 64x64 rule, F and dF/dt recomputed with it (wavefront-uniform
-// lattice sums, accumulators in LDS, two wavefronts per SIMD, as in k_q0_gl)
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
+// lattice sums, accumulators in LDS, three wavefronts per SIMD)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3)))
 k_q0_gh(bq_qplan P, const double* __restrict__ E, const double* __restrict__ ren, const int* __restrict__ sid, int dNe,
         const int* __restrict__ list, double2* __restrict__ G, double2* __restrict__ H, double* __restrict__ F3)
 {
@@ -421,9 +425,10 @@ __device__ double polar_F_part(const Elem& e, double zt, double et)
 }
 
 // domain 0, singular pairs: one block per collocation node (lattice-sum accumulators in
-// LDS, two wavefronts per SIMD, wavefront-uniform lattice sums as in k_q0_gl; 128 angles per
-// sector keep every wavefront wholly active or wholly idle in the angle loop)
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
+// LDS, three wavefronts per SIMD: 108 ms per C4 point against 125 ms at two and 111 ms at
+// four; wavefront-uniform lattice sums as in k_q0_gl; 128 angles per sector keep every
+// wavefront wholly active or wholly idle in the angle loop)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3)))
 k_q0_sing(bq_qplan P, const double* __restrict__ E, const int* __restrict__ sid, int dNe, double2* __restrict__ G,
           double2* __restrict__ H, double* __restrict__ F3)
 {
@@ -597,11 +602,12 @@ __device__ __forceinline__ void fs_store12(const double (&v)[12], int q, bool ne
   }
 }
 
-__global__ void __launch_bounds__(256) k_fs_gl(double kr, double ki, const double* __restrict__ E,
-                                               const double* __restrict__ ren, const int* __restrict__ sid, int dNe,
-                                               const unsigned char* __restrict__ flag, int full_rule, double ltol,
-                                               double2* __restrict__ G, double2* __restrict__ H,
-                                               double2* __restrict__ dG, double2* __restrict__ dH)
+// three wavefronts per SIMD (168 registers, no spills): 167 ms per C4 point against 197 ms at
+// two (171 registers) and 217 ms at four
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3)))
+k_fs_gl(double kr, double ki, const double* __restrict__ E, const double* __restrict__ ren,
+        const int* __restrict__ sid, int dNe, const unsigned char* __restrict__ flag, int full_rule, double ltol,
+        double2* __restrict__ G, double2* __restrict__ H, double2* __restrict__ dG, double2* __restrict__ dH)
 {
   const int lane = threadIdx.x & 63;
   // one wavefront per (t, s): make the pair index wave-uniform so that the element data of
