@@ -53,8 +53,8 @@ FLOPS_PER_EVAL["c2s"] = FLOPS_PER_EVAL["c2"]
 # the 6144-quad sweep, read on its 384-quad variant (c4s) at the same six timed points
 # (profiles/k_q0_gl_c4s_pmc.md): the reference's accuracy (cut 40) and the mixed path's tuned
 # assembly (tolerance 1e-12: cut 26 at every point)
-FLOPS_PER_EVAL["c4"] = {"k_q0_gl": 43112}
-FLOPS_PER_EVAL["c4_tuned"] = {"k_q0_gl": 34246}
+FLOPS_PER_EVAL["c4"] = {"k_q0_gl": 38705}
+FLOPS_PER_EVAL["c4_tuned"] = {"k_q0_gl": 29427}
 FLOPS_PER_EVAL["c4s"], FLOPS_PER_EVAL["c4s_tuned"] = FLOPS_PER_EVAL["c4"], FLOPS_PER_EVAL["c4_tuned"]
 # the 1536-quad two-layer sweep has the open region, lattice, wavelength and direction ranges of
 # the 6144-quad sweep, which fix the per-evaluation cost at the reference's cut
@@ -64,15 +64,17 @@ FLOPS_PER_EVAL["c3"] = FLOPS_PER_EVAL["c4"]
 TRAFFIC_PER_EVAL = {"c1": {"k_q0_gl": (6234276.5 * 2.0 + 7736990.2) * 1024 / 41990176},
                     "c2": {"k_q0_gl": (47751495.6 * 2.0 + 50115568.2) * 1024 / 350386455}}
 TRAFFIC_PER_EVAL["c2s"] = TRAFFIC_PER_EVAL["c2"]
-# FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00 per evaluation on c4s at the reference's cut
-# (profiles/k_q0_gl_c4s_pmc.md); the 1536-quad two-layer sweep shares the open region's physics
-TRAFFIC_PER_EVAL["c4"] = TRAFFIC_PER_EVAL["c3"] = TRAFFIC_PER_EVAL["c4s"] = {"k_q0_gl": 470.3}
+# FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00 per evaluation on c4s at the reference's cut and at the
+# tuned cut (profiles/k_q0_gl_c4s_pmc.md): nearly all of it the register spills of the kernel's
+# 128-register budget; the 1536-quad two-layer sweep shares the open region's physics
+TRAFFIC_PER_EVAL["c4"] = TRAFFIC_PER_EVAL["c3"] = TRAFFIC_PER_EVAL["c4s"] = {"k_q0_gl": 1241.2}
+TRAFFIC_PER_EVAL["c4_tuned"] = TRAFFIC_PER_EVAL["c4s_tuned"] = {"k_q0_gl": 1277.3}
 # k_q0_gl's issue profile on c4s (same counters, six points): active lanes per VALU instruction
 # (SQ_THREAD_CYCLES_VALU / 64 SQ_ACTIVE_INST_VALU; the tail lanes of a pair's last sweep count as
 # active, they evaluate a point whose result is discarded) and the FP64 share of the VALU
 # instructions (ADD + MUL + FMA + TRANS F64 over SQ_INSTS_VALU)
-VALU_PROFILE = {"c4": {"k_q0_gl": {"valu_lane_util": 0.942, "fp64_share_of_valu": 0.687}},
-                "c4_tuned": {"k_q0_gl": {"valu_lane_util": 0.941, "fp64_share_of_valu": 0.679}}}
+VALU_PROFILE = {"c4": {"k_q0_gl": {"valu_lane_util": 0.951, "fp64_share_of_valu": 0.779}},
+                "c4_tuned": {"k_q0_gl": {"valu_lane_util": 0.952, "fp64_share_of_valu": 0.765}}}
 VALU_PROFILE["c4s"], VALU_PROFILE["c4s_tuned"] = VALU_PROFILE["c4"], VALU_PROFILE["c4_tuned"]
 VALU_PROFILE["c3"] = VALU_PROFILE["c4"]
 
@@ -297,7 +299,10 @@ def gemm_probe():
         fl = 8.0 * m * n * k
         for d in (A, B, C):
             d.free()
-        lmm, lup = _library_bf16_ms(m, n, k)
+        import subprocess
+        lib_run = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--library-bf16", str(m), str(n), str(k)],
+                                 capture_output=True, text=True, check=True)
+        lmm, lup = json.loads(lib_run.stdout.strip().splitlines()[-1])
         print(json.dumps(dict(m=m, n=n, k=k, split_kernel_ms=kms, split_wall_ms=wall * 1e3, split_tfs=fl / kms / 1e9,
                               split_bf16_frac=3 * fl / kms / 1e9 / 2500.0, rocblas_ms=rb * 1e3,
                               rocblas_tfs=fl / rb / 1e12, library_bf16_product_ms=lmm,
@@ -768,6 +773,10 @@ def main():
     ap.add_argument("--gemm-probe", action="store_true",
                     help="only time the FP32 factorisation's update kernel (bf16x3 matrix cores) against rocBLAS "
                          "cgemm at the C4 update shapes, one JSON line per shape")
+    ap.add_argument("--library-bf16", type=int, nargs=3, default=None, metavar=("M", "N", "K"),
+                    help="only time the vendor library's bf16 GEMM for the split kernel's matrix work at complex "
+                         "shape M x N x K (a separate process of --gemm-probe: torch and the library each bring "
+                         "their own HIP runtime)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run one device-to-device copy of MIB MiB (calibrates FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -779,6 +788,9 @@ def main():
         y.copy_(x)
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes": x.numel() * 4}))
+        return
+    if args.library_bf16:
+        print(json.dumps(_library_bf16_ms(*args.library_bf16)))
         return
     if args.gemm_probe:
         gemm_probe()
