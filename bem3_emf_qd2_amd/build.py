@@ -38,8 +38,8 @@ LINK_LIBS = ["-lrocblas", "-lm", "-lpthread"]
 # Per-source additions.  The assembly kernels' polynomial and constant-heavy code: with
 # machine loop-invariant code motion the compiler hoists every double constant of the lattice
 # sums into registers for the whole kernel (spilling scalars to vector lanes and vectors to
-# scratch) and copies them into the destinations of v_fmac_f64; without it the constants are
-# built next to their use in scalar registers.
+# scratch) and copies them into the destinations of v_fmac_f64; without it each constant is
+# built next to its use and holds no register across the sums.
 HIP_FILE_FLAGS = {"bq_assemble.hip": ["-mllvm", "--disable-machine-licm"]}
 
 

@@ -44,26 +44,6 @@ __device__ __forceinline__ cplx cdiv(cplx a, cplx b)
   double r = b.re / b.im, d = b.im + b.re * r;
   return cplx{(a.re * r + a.im) / d, (a.im * r - a.re) / d};
 }
-// d = a b + c as one v_fma_f64 with c read from a scalar register pair.  Horner steps written
-// with it keep a polynomial's coefficients out of the vector registers: left to itself the
-// compiler holds each coefficient in a VGPR pair for the whole kernel and copies it into the
-// destination before a v_fmac_f64 (two instructions per step and two registers per
-// coefficient).
-__device__ __forceinline__ double fma_s(double a, double b, double c)
-{
-  double d;
-  asm("v_fma_f64 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "s"(c));
-  return d;
-}
-// c1 r + c0 with both coefficients in scalar registers: a Horner chain's first step, whose
-// leading coefficient would otherwise be copied into a vector register pair
-__device__ __forceinline__ double lin_s(double r, double c1, double c0)
-{
-  double d;
-  asm("v_mul_f64 %0, %1, %2\n\tv_add_f64 %0, %0, %3" : "=&v"(d) : "v"(r), "s"(c1), "s"(c0));
-  return d;
-}
-
 // sin and cos: x = k pi/2 + r with pi/2 split into three doubles (P1 + P2 + P3, rounded
 // from the exact value).  x - k P1 is exact under fma for 1 <= |x| < 2^52 because both
 // operands are multiples of 2^-52 and the difference is below 2, so r keeps full relative
@@ -76,23 +56,25 @@ __device__ __forceinline__ void sincos_fast(double x, double* s, double* c)
   r = fma(-k, 0x1.1a62633145c07p-54, r);
   r = fma(-k, -0x1.f1976b7ed8fbcp-110, r);
   const double r2 = r * r;
-  double ps = lin_s(r2, -0x1.2f49b46814157p-57, 0x1.952c77030ad4ap-49);  // -1/19!, 1/17!
-  ps = fma_s(ps, r2, -0x1.ae7f3e733b81fp-41);        // -1/15!
-  ps = fma_s(ps, r2, 0x1.6124613a86d09p-33);         //  1/13!
-  ps = fma_s(ps, r2, -0x1.ae64567f544e4p-26);        // -1/11!
-  ps = fma_s(ps, r2, 0x1.71de3a556c734p-19);         //  1/9!
-  ps = fma_s(ps, r2, -0x1.a01a01a01a01ap-13);        // -1/7!
-  ps = fma_s(ps, r2, 0x1.1111111111111p-7);          //  1/5!
-  ps = fma_s(ps, r2, -0x1.5555555555555p-3);         // -1/3!
+  double ps = -0x1.2f49b46814157p-57;             // -1/19!
+  ps = fma(ps, r2, 0x1.952c77030ad4ap-49);         //  1/17!
+  ps = fma(ps, r2, -0x1.ae7f3e733b81fp-41);        // -1/15!
+  ps = fma(ps, r2, 0x1.6124613a86d09p-33);         //  1/13!
+  ps = fma(ps, r2, -0x1.ae64567f544e4p-26);        // -1/11!
+  ps = fma(ps, r2, 0x1.71de3a556c734p-19);         //  1/9!
+  ps = fma(ps, r2, -0x1.a01a01a01a01ap-13);        // -1/7!
+  ps = fma(ps, r2, 0x1.1111111111111p-7);          //  1/5!
+  ps = fma(ps, r2, -0x1.5555555555555p-3);         // -1/3!
   const double sr = fma(ps * r2, r, r);
-  double pc = lin_s(r2, -0x1.6827863b97d97p-53, 0x1.ae7f3e733b81fp-45);  // -1/18!, 1/16!
-  pc = fma_s(pc, r2, -0x1.93974a8c07c9dp-37);        // -1/14!
-  pc = fma_s(pc, r2, 0x1.1eed8eff8d898p-29);         //  1/12!
-  pc = fma_s(pc, r2, -0x1.27e4fb7789f5cp-22);        // -1/10!
-  pc = fma_s(pc, r2, 0x1.a01a01a01a01ap-16);         //  1/8!
-  pc = fma_s(pc, r2, -0x1.6c16c16c16c17p-10);        // -1/6!
-  pc = fma_s(pc, r2, 0x1.5555555555555p-5);          //  1/4!
-  pc = fma(pc, r2, -0.5);                            // -1/2!
+  double pc = -0x1.6827863b97d97p-53;              // -1/18!
+  pc = fma(pc, r2, 0x1.ae7f3e733b81fp-45);         //  1/16!
+  pc = fma(pc, r2, -0x1.93974a8c07c9dp-37);        // -1/14!
+  pc = fma(pc, r2, 0x1.1eed8eff8d898p-29);         //  1/12!
+  pc = fma(pc, r2, -0x1.27e4fb7789f5cp-22);        // -1/10!
+  pc = fma(pc, r2, 0x1.a01a01a01a01ap-16);         //  1/8!
+  pc = fma(pc, r2, -0x1.6c16c16c16c17p-10);        // -1/6!
+  pc = fma(pc, r2, 0x1.5555555555555p-5);          //  1/4!
+  pc = fma(pc, r2, -0.5);                          // -1/2!
   const double cr = fma(pc, r2, 1.0);
   const int q = (int)(k - 4.0 * floor(k * 0.25));
   const double sv = (q & 1) ? cr : sr, cv = (q & 1) ? sr : cr;
@@ -110,16 +92,17 @@ __device__ __forceinline__ double exp_fast(double t)
   const double k = rint(t * 0x1.71547652b82fep+0);
   double r = fma(-k, 0x1.62e42fee00000p-1, t);
   r = fma(-k, 0x1.a39ef35793c76p-33, r);
-  double p = lin_s(r, 0x1.6124613a86d09p-33, 0x1.1eed8eff8d898p-29);  // 1/13!, 1/12!
-  p = fma_s(p, r, 0x1.ae64567f544e4p-26);           // 1/11!
-  p = fma_s(p, r, 0x1.27e4fb7789f5cp-22);           // 1/10!
-  p = fma_s(p, r, 0x1.71de3a556c734p-19);           // 1/9!
-  p = fma_s(p, r, 0x1.a01a01a01a01ap-16);           // 1/8!
-  p = fma_s(p, r, 0x1.a01a01a01a01ap-13);           // 1/7!
-  p = fma_s(p, r, 0x1.6c16c16c16c17p-10);           // 1/6!
-  p = fma_s(p, r, 0x1.1111111111111p-7);            // 1/5!
-  p = fma_s(p, r, 0x1.5555555555555p-5);            // 1/4!
-  p = fma_s(p, r, 0x1.5555555555555p-3);            // 1/3!
+  double p = 0x1.6124613a86d09p-33;               // 1/13!
+  p = fma(p, r, 0x1.1eed8eff8d898p-29);           // 1/12!
+  p = fma(p, r, 0x1.ae64567f544e4p-26);           // 1/11!
+  p = fma(p, r, 0x1.27e4fb7789f5cp-22);           // 1/10!
+  p = fma(p, r, 0x1.71de3a556c734p-19);           // 1/9!
+  p = fma(p, r, 0x1.a01a01a01a01ap-16);           // 1/8!
+  p = fma(p, r, 0x1.a01a01a01a01ap-13);           // 1/7!
+  p = fma(p, r, 0x1.6c16c16c16c17p-10);           // 1/6!
+  p = fma(p, r, 0x1.1111111111111p-7);            // 1/5!
+  p = fma(p, r, 0x1.5555555555555p-5);            // 1/4!
+  p = fma(p, r, 0x1.5555555555555p-3);            // 1/3!
   p = fma(p, r, 0.5);
   p = fma(p, r, 1.0);
   p = fma(p, r, 1.0);

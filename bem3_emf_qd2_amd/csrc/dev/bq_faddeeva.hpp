@@ -73,8 +73,8 @@ __device__ __forceinline__ double rsqrt_nr(double x)
   // hardware reciprocal-square-root estimate refined by two Newton steps (full double precision)
   double y = __builtin_amdgcn_rsq(x);
   const double h = 0.5 * x;
-  y = y * fma_s(-h * y, y, 1.5);
-  return y * fma_s(-h * y, y, 1.5);
+  y = y * fma(-h * y, y, 1.5);
+  return y * fma(-h * y, y, 1.5);
 }
 
 __device__ __forceinline__ double sinc_of(double x, double sx)
