@@ -602,8 +602,8 @@ __device__ __forceinline__ void fs_store12(const double (&v)[12], int q, bool ne
   }
 }
 
-// three wavefronts per SIMD (168 registers, no spills): 167 ms per C4 point against 197 ms at
-// two (171 registers) and 217 ms at four
+// three wavefronts per SIMD (168 registers, no spills): 167-176 ms per C4 point against
+// 197 ms at two (171 registers) and 217-226 ms at four
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3)))
 k_fs_gl(double kr, double ki, const double* __restrict__ E, const double* __restrict__ ren,
         const int* __restrict__ sid, int dNe, const unsigned char* __restrict__ flag, int full_rule, double ltol,
